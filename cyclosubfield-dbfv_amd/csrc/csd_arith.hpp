@@ -1,0 +1,76 @@
+// 64-bit modular arithmetic shared by the host plan builders and the gfx950 kernels.
+//
+// Montgomery reduction with R = 2^64 (hal/mont.rs:38-76) and Barrett multiplication by a
+// constant (hal/ntt.rs:126-158). The device forms avoid forming full 128-bit sums where the
+// low word is known: for x = lo + 2^64 hi and m = lo * (-N^-1) mod 2^64, the low word of
+// m * N is -lo mod 2^64, so (x + m N) >> 64 = hi + mulhi(m, N) + [lo != 0].
+#pragma once
+#include <stdint.h>
+
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+
+#if defined(__HIPCC__)
+#define CSD_HD __host__ __device__ __forceinline__
+#else
+#define CSD_HD inline
+#endif
+
+struct CsdMod {
+    int64_t N;
+    int64_t ninv;  // -N^-1 mod 2^64
+    int64_t r2;    // 2^128 mod N (0 mapped to N)
+    int64_t r3;    // 2^192 mod N (0 mapped to N)
+};
+
+// a multiplication by a fixed constant z: u = round(-z 2^64 / N)
+struct CsdBarrett {
+    int64_t z;
+    int64_t u;
+};
+
+CSD_HD uint64_t csd_umulhi(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * (u128)b) >> 64); }
+
+// high word of the signed 128-bit product
+CSD_HD int64_t csd_smulhi(int64_t a, int64_t b) { return (int64_t)(((i128)a * (i128)b) >> 64); }
+
+// high word of a signed * non-negative product
+CSD_HD int64_t csd_smulhi_pos(int64_t a, int64_t b_nonneg) {
+    uint64_t h = csd_umulhi((uint64_t)a, (uint64_t)b_nonneg);
+    return (int64_t)(h - (a < 0 ? (uint64_t)b_nonneg : 0));
+}
+
+// red128: |x| <= 2^63 N + N  ->  [-N, N], congruent to 2^-64 x (mont.rs:38-45)
+CSD_HD int64_t csd_red_words(const CsdMod& m, uint64_t lo, int64_t hi) {
+    int64_t mm = (int64_t)(lo * (uint64_t)m.ninv);
+    return (int64_t)((uint64_t)hi + (uint64_t)csd_smulhi_pos(mm, m.N) + (lo != 0 ? 1u : 0u));
+}
+CSD_HD int64_t csd_red128(const CsdMod& m, i128 x) { return csd_red_words(m, (uint64_t)x, (int64_t)(x >> 64)); }
+
+// red128(a * b): the Montgomery product of two i64 values
+CSD_HD int64_t csd_mont_mul(const CsdMod& m, int64_t a, int64_t b) {
+    uint64_t lo = (uint64_t)a * (uint64_t)b;
+    int64_t hi = csd_smulhi(a, b);
+    return csd_red_words(m, lo, hi);
+}
+
+// double_red128: any i128 -> [-N, N], congruent to 2^-128 x (mont.rs:57-66)
+CSD_HD int64_t csd_double_red128(const CsdMod& m, i128 x) {
+    uint64_t lo = (uint64_t)x;
+    int64_t hi = (int64_t)(x >> 64);
+    int64_t mm = (int64_t)(lo * (uint64_t)m.ninv);
+    int64_t t_hi = csd_smulhi_pos(mm, m.N);
+    // r1 = hi + t_hi + carry(lo + lo(t)), lo(t) = -lo mod 2^64, as an i128
+    i128 r1 = (i128)hi + (i128)t_hi + (lo != 0 ? 1 : 0);
+    return csd_red128(m, r1);
+}
+
+CSD_HD int64_t csd_to_mont64(const CsdMod& m, int64_t x) { return csd_mont_mul(m, x, m.r2); }
+CSD_HD int64_t csd_to_mont128(const CsdMod& m, i128 x) { return csd_mont_mul(m, csd_red128(m, x), m.r3); }
+
+// BarrettMul::mul (ntt.rs:149-152): |x| <= 2^63 -> |t| <= 3N/4, t = z x mod N
+CSD_HD int64_t csd_barrett(CsdBarrett b, int64_t x, int64_t n) {
+    uint64_t lo = (uint64_t)b.u * (uint64_t)x;
+    int64_t q = (int64_t)((uint64_t)csd_smulhi(b.u, x) + (lo >> 63));
+    return (int64_t)((uint64_t)b.z * (uint64_t)x + (uint64_t)q * (uint64_t)n);
+}

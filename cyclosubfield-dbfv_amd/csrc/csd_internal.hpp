@@ -1,0 +1,74 @@
+// Internal declarations shared by the translation units of libcsdbfv_hal.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/csdbfv_hal.h"
+#include "csd_arith.hpp"
+
+struct CsdError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void csd_set_error(const std::string& s);
+
+#define CSD_TRY try {
+#define CSD_CATCH                          \
+    }                                      \
+    catch (const std::exception& e) {      \
+        csd_set_error(e.what());           \
+        return 1;                          \
+    }                                      \
+    return 0;
+
+#define HIPCHECK(x)                                                                                            \
+    do {                                                                                                       \
+        hipError_t e_ = (x);                                                                                   \
+        if (e_ != hipSuccess) throw CsdError(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+
+// A context: one device, one in-order stream, and a size-bucketed cache of device blocks so
+// that the temporaries of composed ring operations do not hit hipMalloc/hipFree (which
+// synchronise the device) on every call.
+struct csd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::map<size_t, std::vector<void*>> pool;
+    std::map<void*, size_t> live;
+
+    void* alloc_bytes(size_t bytes);
+    void free_bytes(void* p);
+    size_t alloc_size(void* p);
+    ~csd_ctx();
+};
+
+struct Shape4 {
+    int64_t d[4];
+};
+struct Str4 {
+    int64_t s[4];
+};
+
+struct ModTable {
+    CsdMod m[CSD_MAX_MODULI];
+};
+
+Shape4 shape_of(const csd_view* v);
+int64_t numel(const csd_view* v);
+void check_view(const csd_view* v, const char* what);
+Str4 bcast_strides(const csd_view* src, const csd_view* dst, const char* what);
+Str4 own_strides(const csd_view* v);
+ModTable make_mod_table(const csd_modulus* mods, size_t nmods, int64_t d1);
+void csd_zero_fill(csd_ctx* ctx, void* p, size_t bytes);
+
+// host-side modular helpers (plan and constant construction)
+int64_t csd_host_mulmod(int64_t a, int64_t b, int64_t q);
+int64_t csd_host_powmod(int64_t b, uint64_t e, int64_t q);
+CsdBarrett csd_host_barrett(int64_t value, int64_t n);
+int64_t csd_host_root_of_unity(int64_t q, size_t n);

@@ -1,0 +1,368 @@
+"""Python front end of libcsdbfv_hal.so: the MI355X backend for the ``hal::Backend`` trait
+(cyclosubfield-dbfv/src/hal/mod.rs:108-350).
+
+``HipBackend`` exposes the trait methods under their reference names (``hal_allocate_i64``,
+``hal_ntt_mul_mont64_dim``, ...). Tensors are ``DeviceTensor`` views over HBM with the reference's
+view semantics (hal/cpu.rs:187-296): slicing with step-1 ranges gives a strided view, ``split``
+gives two disjoint views, the last stride is always 1. Errors of the C ABI (contract violations,
+HIP failures) surface as ``HalError`` — the reference panics in the same places.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcsdbfv_hal.so")
+
+I128 = np.dtype([("lo", "<u8"), ("hi", "<i8")])
+
+
+class HalError(RuntimeError):
+    pass
+
+
+class CsdView(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("shape", ctypes.c_int64 * 4), ("strides", ctypes.c_int64 * 4)]
+
+
+class CsdModulus(ctypes.Structure):
+    """MontgomeryModulus (hal/mont.rs:4-10)."""
+
+    _fields_ = [("N", ctypes.c_int64), ("neg_inv_N", ctypes.c_int64), ("r2", ctypes.c_int64), ("r3", ctypes.c_int64)]
+
+    def modulus(self):
+        return self.N
+
+
+_lib = None
+
+
+def lib():
+    """Loads the native library; there is no fallback: without it nothing in this package runs."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HalError(f"{LIB_PATH} is not built (run cyclosubfield-dbfv_amd/build_lib.py)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.csd_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise HalError(lib().csd_last_error().decode())
+
+
+def _norm_slice(s, n):
+    if isinstance(s, int):
+        if s < 0:
+            s += n
+        if not 0 <= s < n:
+            raise IndexError("index out of range")
+        return s, s + 1
+    assert isinstance(s, slice) and s.step in (None, 1), "only step-1 slices are views"
+    start, stop, _ = s.indices(n)
+    if stop < start:
+        stop = start
+    return start, stop
+
+
+class DeviceTensor:
+    """A 4-D strided view over device memory (i64 or i128 entries)."""
+
+    def __init__(self, backend, ptr, shape, strides, elem, owner=None):
+        self.backend = backend
+        self.ptr = ptr
+        self.shape = tuple(int(x) for x in shape)
+        self.strides = tuple(int(x) for x in strides)
+        self.elem = elem
+        self._owner = owner  # keeps the allocation alive while views exist
+
+    @property
+    def ndim(self):
+        return 4
+
+    def __getitem__(self, idx):
+        if not isinstance(idx, tuple):
+            idx = (idx,)
+        idx = tuple(i for i in idx if i is not Ellipsis)
+        idx = idx + (slice(None),) * (4 - len(idx))
+        off = 0
+        shape = []
+        for d in range(4):
+            a, b = _norm_slice(idx[d], self.shape[d])
+            off += a * self.strides[d]
+            shape.append(b - a)
+        return DeviceTensor(self.backend, self.ptr + off * self.elem, shape, self.strides, self.elem, self._owner or self)
+
+    def slice(self, r0, r1, r2, r3):
+        return self[r0, r1, r2, r3]
+
+    def split(self, at, dim):
+        """TensorMut::split (hal/cpu.rs:267-284)."""
+        assert 0 <= at <= self.shape[dim]
+        s1 = list(self.shape)
+        s1[dim] = at
+        s2 = list(self.shape)
+        s2[dim] = self.shape[dim] - at
+        first = DeviceTensor(self.backend, self.ptr, s1, self.strides, self.elem, self._owner or self)
+        second = DeviceTensor(self.backend, self.ptr + at * self.strides[dim] * self.elem, s2, self.strides, self.elem,
+                              self._owner or self)
+        return first, second
+
+    def view(self):
+        v = CsdView()
+        v.ptr = self.ptr
+        for d in range(4):
+            v.shape[d] = self.shape[d]
+            v.strides[d] = self.strides[d]
+        return v
+
+    def numel(self):
+        return int(np.prod(self.shape))
+
+    def __repr__(self):
+        return f"DeviceTensor(shape={self.shape}, strides={self.strides}, {'i64' if self.elem == 8 else 'i128'})"
+
+
+class _Allocation(DeviceTensor):
+    def __del__(self):
+        be = self.backend
+        if be is not None and be.ctx is not None and _lib is not None:
+            _lib.csd_hal_free(be.ctx, ctypes.c_void_p(self.ptr))
+
+
+class HipNTTPlan:
+    def __init__(self, backend, handle, length):
+        self.backend, self.handle, self.len = backend, handle, length
+
+    def __del__(self):
+        if self.handle is not None and _lib is not None and self.backend.ctx is not None:
+            _lib.csd_ntt_plan_free(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+
+class HipPermutationPlan:
+    def __init__(self, backend, handle, length):
+        self.backend, self.handle, self.len = backend, handle, length
+
+    def __del__(self):
+        if self.handle is not None and _lib is not None and self.backend.ctx is not None:
+            _lib.csd_perm_plan_free(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+
+def _row_major_strides(shape):
+    st = [1, 1, 1, 1]
+    for d in (2, 1, 0):
+        st[d] = st[d + 1] * shape[d + 1]
+    return st
+
+
+_DIM = {"Dim2": 2, "Dim3": 3, 2: 2, 3: 3}
+
+
+def _mods(mods):
+    return (CsdModulus * len(mods))(*mods), ctypes.c_size_t(len(mods))
+
+
+def _plans(plans):
+    return (ctypes.c_void_p * len(plans))(*[p.handle for p in plans]), ctypes.c_size_t(len(plans))
+
+
+class HipBackend:
+    """The hal::Backend trait on one MI355X, see the module docstring."""
+
+    def __init__(self, device=0):
+        self.ctx = None
+        ctx = ctypes.c_void_p()
+        check(lib().csd_ctx_create(ctypes.c_int(device), ctypes.byref(ctx)))
+        self.ctx = ctx
+        self.device = device
+
+    def close(self):
+        if self.ctx is not None:
+            lib().csd_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def sync(self):
+        check(lib().csd_ctx_sync(self.ctx))
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        check(lib().csd_ctx_stream(self.ctx, ctypes.byref(s)))
+        return s.value
+
+    # --- allocation and transfer (hal/mod.rs:115-125) --------------------------------------------
+    def _alloc(self, shape, elem):
+        shape = [int(x) for x in shape]
+        ptr = ctypes.c_void_p()
+        check(lib().csd_hal_allocate(self.ctx, ctypes.c_int(elem), (ctypes.c_int64 * 4)(*shape), ctypes.byref(ptr)))
+        return _Allocation(self, ptr.value, shape, _row_major_strides(shape), elem)
+
+    def hal_allocate_i64(self, shape):
+        return self._alloc(shape, 8)
+
+    def hal_allocate_i128(self, shape):
+        return self._alloc(shape, 16)
+
+    def hal_import(self, dst, src):
+        assert dst.elem == 8
+        a = np.ascontiguousarray(np.asarray(src, dtype=np.int64).reshape(-1))
+        check(lib().csd_hal_import(self.ctx, ctypes.byref(dst.view()), a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.size)))
+
+    def hal_export(self, src, dst=None):
+        assert src.elem == 8
+        out = np.zeros(src.numel(), dtype=np.int64)
+        check(lib().csd_hal_export(self.ctx, ctypes.byref(src.view()), out.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(out.size)))
+        if dst is not None:
+            dst[:] = out
+        return out
+
+    def import_i128(self, dst, words):
+        """words: I128 numpy array (or int64 pairs) in row-major order of the view."""
+        assert dst.elem == 16
+        a = np.ascontiguousarray(words).view(np.int64).reshape(-1)
+        check(lib().csd_import_i128(self.ctx, ctypes.byref(dst.view()), a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.size // 2)))
+
+    def export_i128(self, src):
+        assert src.elem == 16
+        out = np.zeros(src.numel(), dtype=I128)
+        check(lib().csd_export_i128(self.ctx, ctypes.byref(src.view()), out.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(out.size)))
+        return out
+
+    # --- copies and wrapping arithmetic (hal/mod.rs:127-173) --------------------------------------
+    def _bin(self, fn, dst, src):
+        check(getattr(lib(), fn)(self.ctx, ctypes.byref(dst.view()), ctypes.byref(src.view())))
+
+    def hal_assign(self, dst, src):
+        self._bin("csd_hal_assign", dst, src)
+
+    def hal_assign_narrow(self, dst, src):
+        self._bin("csd_hal_assign_narrow", dst, src)
+
+    def hal_assign_i128(self, dst, src):
+        self._bin("csd_hal_assign_i128", dst, src)
+
+    def hal_add_i64(self, dst, src):
+        self._bin("csd_hal_add_i64", dst, src)
+
+    def hal_sub_i64(self, dst, src):
+        self._bin("csd_hal_sub_i64", dst, src)
+
+    def hal_add_i128(self, dst, src):
+        self._bin("csd_hal_add_i128", dst, src)
+
+    def hal_fma_i64(self, dst, lhs, rhs):
+        check(lib().csd_hal_fma_i64(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()), ctypes.byref(rhs.view())))
+
+    def hal_mul_i64(self, dst, lhs, rhs):
+        check(lib().csd_hal_mul_i64(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()), ctypes.byref(rhs.view())))
+
+    def hal_inner_product_i64(self, dst, lhs, rhs):
+        assert len(lhs) == len(rhs)
+        lv = (CsdView * len(lhs))(*[x.view() for x in lhs])
+        rv = (CsdView * len(rhs))(*[x.view() for x in rhs])
+        check(lib().csd_hal_inner_product_i64(self.ctx, ctypes.byref(dst.view()), lv, rv, ctypes.c_size_t(len(lhs))))
+
+    def hal_matmul_i64(self, dst, lhs, rhs):
+        check(lib().csd_hal_matmul_i64(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()), ctypes.byref(rhs.view())))
+
+    # --- Montgomery arithmetic (hal/mod.rs:175-207) -----------------------------------------------
+    def hal_create_modulus(self, prime):
+        m = CsdModulus()
+        check(lib().csd_hal_create_modulus(ctypes.c_int64(prime), ctypes.byref(m)))
+        return m
+
+    def _modop(self, fn, dst, src, mods):
+        arr, n = _mods(mods)
+        check(getattr(lib(), fn)(self.ctx, ctypes.byref(dst.view()), ctypes.byref(src.view()), arr, n))
+
+    def hal_mul_mont64(self, dst, src, mods):
+        self._modop("csd_hal_mul_mont64", dst, src, mods)
+
+    def hal_add_red_mont64_double(self, dst, src, mods):
+        self._modop("csd_hal_add_red_mont64_double", dst, src, mods)
+
+    def hal_red_mont64_double(self, dst, src, mods):
+        self._modop("csd_hal_red_mont64_double", dst, src, mods)
+
+    def hal_add_red_mont64(self, dst, src, mods):
+        self._modop("csd_hal_add_red_mont64", dst, src, mods)
+
+    def hal_red_mont64(self, dst, src, mods):
+        self._modop("csd_hal_red_mont64", dst, src, mods)
+
+    def hal_i128_to_mont64(self, dst, src, mods):
+        self._modop("csd_hal_i128_to_mont64", dst, src, mods)
+
+    def hal_i64_to_mont64(self, dst, mods):
+        arr, n = _mods(mods)
+        check(lib().csd_hal_i64_to_mont64(self.ctx, ctypes.byref(dst.view()), arr, n))
+
+    # --- NTTs (hal/mod.rs:209-250) -----------------------------------------------------------------
+    def hal_create_ntt_plan(self, modulus, inner_shape):
+        inner_shape = list(inner_shape)
+        h = ctypes.c_void_p()
+        shp = (ctypes.c_size_t * max(1, len(inner_shape)))(*inner_shape)
+        check(lib().csd_hal_create_ntt_plan(self.ctx, ctypes.byref(modulus), shp, ctypes.c_size_t(len(inner_shape)), ctypes.byref(h)))
+        return HipNTTPlan(self, h.value, int(np.prod(inner_shape)) if inner_shape else 1)
+
+    def hal_ntt_mont64_dim(self, dst, plans, dim):
+        arr, n = _plans(plans)
+        check(lib().csd_hal_ntt_mont64_dim(self.ctx, ctypes.byref(dst.view()), arr, n, ctypes.c_int(_DIM[dim])))
+
+    def hal_intt_mont64_dim(self, dst, plans, dim):
+        arr, n = _plans(plans)
+        check(lib().csd_hal_intt_mont64_dim(self.ctx, ctypes.byref(dst.view()), arr, n, ctypes.c_int(_DIM[dim])))
+
+    def hal_ntt_mul_mont64_dim(self, dst, plans, dim, mul, mods):
+        arr, n = _plans(plans)
+        marr, mn = _mods(mods)
+        check(lib().csd_hal_ntt_mul_mont64_dim(self.ctx, ctypes.byref(dst.view()), arr, n, ctypes.c_int(_DIM[dim]),
+                                               ctypes.byref(mul.view()), marr, mn))
+
+    def hal_ntt_mont64(self, dst, dim2, dim3):
+        self.hal_ntt_mont64_dim(dst, dim2, 2)
+        self.hal_ntt_mont64_dim(dst, dim3, 3)
+
+    def hal_intt_mont64(self, dst, dim2, dim3):
+        self.hal_intt_mont64_dim(dst, dim2, 2)
+        self.hal_intt_mont64_dim(dst, dim3, 3)
+
+    # --- base-conversion corrections (hal/mod.rs:252-337) ------------------------------------------
+    def hal_bconv_correct_mont64(self, dst, lhs, rhs, mods, shift):
+        arr, n = _mods(mods)
+        check(lib().csd_hal_bconv_correct_mont64(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()),
+                                                 ctypes.byref(rhs.view()), arr, n, ctypes.c_size_t(shift)))
+
+    def hal_bconv_correct_pow2(self, dst, src, factor0, factor1, shift, log2_modulus):
+        check(lib().csd_hal_bconv_correct_pow2(self.ctx, ctypes.byref(dst.view()), ctypes.byref(src.view()),
+                                               ctypes.c_int64(factor0), ctypes.c_int64(factor1),
+                                               ctypes.c_size_t(shift), ctypes.c_size_t(log2_modulus)))
+
+    def hal_bconv_correct_pow2_wide(self, dst, src, factor0, factor1, shift, log2_modulus):
+        def split(x):
+            x &= (1 << 128) - 1
+            lo, hi = x & ((1 << 64) - 1), x >> 64
+            s = lambda w: ctypes.c_int64(w - (1 << 64) if w >= 1 << 63 else w)
+            return s(lo), s(hi)
+
+        f0l, f0h = split(factor0)
+        f1l, f1h = split(factor1)
+        check(lib().csd_hal_bconv_correct_pow2_wide(self.ctx, ctypes.byref(dst.view()), ctypes.byref(src.view()),
+                                                    f0l, f0h, f1l, f1h, ctypes.c_size_t(shift), ctypes.c_size_t(log2_modulus)))
+
+    # --- permutations (hal/mod.rs:339-349) ---------------------------------------------------------
+    def hal_create_permutation(self, inner_shape, shifts):
+        n = len(inner_shape)
+        h = ctypes.c_void_p()
+        shp = (ctypes.c_size_t * max(1, n))(*inner_shape)
+        sh = (ctypes.c_size_t * max(1, n))(*shifts)
+        check(lib().csd_hal_create_permutation(self.ctx, shp, sh, ctypes.c_size_t(n), ctypes.byref(h)))
+        return HipPermutationPlan(self, h.value, int(np.prod(inner_shape)) if n else 1)
+
+    def hal_permute(self, dst, src, plan, dim):
+        check(lib().csd_hal_permute(self.ctx, ctypes.byref(dst.view()), ctypes.byref(src.view()), ctypes.c_void_p(plan.handle),
+                                    ctypes.c_int(_DIM[dim])))

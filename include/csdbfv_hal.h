@@ -1,0 +1,128 @@
+/*
+ * csdbfv_hal.h — C ABI of the MI355X (gfx950) backend for the `hal::Backend` trait of
+ * cyclosubfield-dbfv (cyclosubfield-dbfv/src/hal/mod.rs:108-350), plus the ring-level fused
+ * entry points that the reference's `ring`/`bfv` layers compose from several trait calls.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a non-zero status on failure; `csd_last_error()`
+ *    then describes the failure (the reference panics on the same contract violations, e.g.
+ *    hal/cpu.rs:537, 568, 959-964, 993-996).
+ *  - Tensors live in device memory (HBM). A view is {ptr, shape[4], strides[4]} with strides in
+ *    elements of the tensor's type (8 bytes for i64 tensors, 16 bytes for i128 tensors), and the
+ *    last stride 1 (hal/cpu.rs:3-6). A source operand may have length 1 in any dimension where the
+ *    destination does not; it is then broadcast (hal/mod.rs:86-90).
+ *  - Moduli are passed per call as an array indexed by dimension 1 (hal/mod.rs:92-96); at most
+ *    CSD_MAX_MODULI entries.
+ *  - All work is enqueued in order on the context's HIP stream; `csd_hal_export` and
+ *    `csd_ctx_sync` wait for it (the export is the only device->host read, as in the trait).
+ */
+#ifndef CSDBFV_HAL_H
+#define CSDBFV_HAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CSD_MAX_MODULI 64
+
+typedef struct csd_ctx csd_ctx;
+typedef struct csd_ntt_plan csd_ntt_plan;
+typedef struct csd_perm_plan csd_perm_plan;
+
+typedef struct {
+    void* ptr;
+    int64_t shape[4];
+    int64_t strides[4];
+} csd_view;
+
+/* MontgomeryModulus (hal/mont.rs:4-10): N, -N^-1 mod 2^64, 2^128 mod N, 2^192 mod N. */
+typedef struct {
+    int64_t N;
+    int64_t neg_inv_N;
+    int64_t r2;
+    int64_t r3;
+} csd_modulus;
+
+enum { CSD_DIM2 = 2, CSD_DIM3 = 3 };
+
+const char* csd_last_error(void);
+
+/* ---- context (no reference counterpart: the reference backend is a stateless CPU struct,
+ *      hal/cpu.rs:489-491) ------------------------------------------------------------------- */
+int csd_ctx_create(int device, csd_ctx** out);
+int csd_ctx_destroy(csd_ctx* ctx);
+int csd_ctx_sync(csd_ctx* ctx);
+/* the hipStream_t all work of the context is enqueued on */
+int csd_ctx_stream(csd_ctx* ctx, void** stream);
+
+/* ---- allocation and transfer --------------------------------------------------------------- */
+/* hal_allocate_i64 / hal_allocate_i128 (hal/mod.rs:119-122): zero-initialised, row-major;
+ * elem_bytes is 8 (i64) or 16 (i128). */
+int csd_hal_allocate(csd_ctx* ctx, int elem_bytes, const int64_t shape[4], void** out);
+int csd_hal_free(csd_ctx* ctx, void* ptr);
+/* hal_import / hal_export (hal/mod.rs:124-125): host data in row-major order of the view. */
+int csd_hal_import(csd_ctx* ctx, const csd_view* dst, const int64_t* src, size_t len);
+int csd_hal_export(csd_ctx* ctx, const csd_view* src, int64_t* dst, size_t len);
+/* i128 counterparts (no trait method; the reference tests read i128 tensors directly,
+ * hal/cpu.rs:1264-1454): two int64 words (low, high) per entry. */
+int csd_import_i128(csd_ctx* ctx, const csd_view* dst, const int64_t* src_words, size_t len);
+int csd_export_i128(csd_ctx* ctx, const csd_view* src, int64_t* dst_words, size_t len);
+
+/* ---- copies and wrapping integer arithmetic ------------------------------------------------ */
+int csd_hal_assign(csd_ctx* ctx, const csd_view* dst, const csd_view* src);        /* mod.rs:129 */
+int csd_hal_assign_narrow(csd_ctx* ctx, const csd_view* dst, const csd_view* src); /* mod.rs:134 */
+int csd_hal_assign_i128(csd_ctx* ctx, const csd_view* dst, const csd_view* src);   /* mod.rs:137 */
+int csd_hal_add_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* src);       /* mod.rs:139 */
+int csd_hal_sub_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* src);       /* mod.rs:140 */
+int csd_hal_add_i128(csd_ctx* ctx, const csd_view* dst, const csd_view* src);      /* mod.rs:141 */
+int csd_hal_fma_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs); /* mod.rs:144 */
+int csd_hal_mul_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs); /* mod.rs:149 */
+/* hal_inner_product_i64 (mod.rs:151-163): dst = sum_t lhs[t] * rhs[t], overwriting. */
+int csd_hal_inner_product_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs, size_t terms);
+/* hal_matmul_i64 (mod.rs:165-173): dst[i,j,k,l] = sum_u lhs[i,j,u,0] * rhs[i,u,k,l]. */
+int csd_hal_matmul_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs);
+
+/* ---- Montgomery arithmetic ------------------------------------------------------------------ */
+int csd_hal_create_modulus(int64_t prime, csd_modulus* out); /* mod.rs:175, mont.rs:13-33 */
+int csd_hal_mul_mont64(csd_ctx* ctx, const csd_view* dst, const csd_view* src, const csd_modulus* mods, size_t nmods); /* mod.rs:179 */
+int csd_hal_add_red_mont64_double(csd_ctx* ctx, const csd_view* dst, const csd_view* src, const csd_modulus* mods, size_t nmods); /* mod.rs:185 */
+int csd_hal_red_mont64_double(csd_ctx* ctx, const csd_view* dst, const csd_view* src, const csd_modulus* mods, size_t nmods); /* mod.rs:189 */
+int csd_hal_add_red_mont64(csd_ctx* ctx, const csd_view* dst, const csd_view* src, const csd_modulus* mods, size_t nmods); /* mod.rs:195 */
+int csd_hal_red_mont64(csd_ctx* ctx, const csd_view* dst, const csd_view* src, const csd_modulus* mods, size_t nmods); /* mod.rs:199 */
+int csd_hal_i128_to_mont64(csd_ctx* ctx, const csd_view* dst, const csd_view* src, const csd_modulus* mods, size_t nmods); /* mod.rs:203 */
+int csd_hal_i64_to_mont64(csd_ctx* ctx, const csd_view* dst, const csd_modulus* mods, size_t nmods); /* mod.rs:207 */
+
+/* ---- NTTs ----------------------------------------------------------------------------------- */
+/* hal_create_ntt_plan (mod.rs:209, ntt.rs:561-566): group NTT over C_n0 x ... x C_n(r-1). */
+int csd_hal_create_ntt_plan(csd_ctx* ctx, const csd_modulus* modulus, const size_t* inner_shape, size_t n, csd_ntt_plan** out);
+int csd_ntt_plan_free(csd_ntt_plan* plan);
+/* hal_ntt_mont64_dim / hal_intt_mont64_dim / hal_ntt_mul_mont64_dim (mod.rs:216-238);
+ * plans[j] is the plan for dimension-1 index j; dim is CSD_DIM2 or CSD_DIM3. */
+int csd_hal_ntt_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim);
+int csd_hal_intt_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim);
+int csd_hal_ntt_mul_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim,
+                               const csd_view* mul, const csd_modulus* mods, size_t nmods);
+
+/* ---- base-conversion corrections -------------------------------------------------------------- */
+int csd_hal_bconv_correct_mont64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs,
+                                 const csd_modulus* mods, size_t nmods, size_t shift); /* mod.rs:263-270 */
+int csd_hal_bconv_correct_pow2(csd_ctx* ctx, const csd_view* dst, const csd_view* src, int64_t factor0, int64_t factor1,
+                               size_t shift, size_t log2_modulus); /* mod.rs:289-297 */
+/* factor0/factor1 are i128 given as (low, high) words (mod.rs:329-337) */
+int csd_hal_bconv_correct_pow2_wide(csd_ctx* ctx, const csd_view* dst, const csd_view* src, int64_t factor0_lo,
+                                    int64_t factor0_hi, int64_t factor1_lo, int64_t factor1_hi, size_t shift,
+                                    size_t log2_modulus);
+
+/* ---- permutations (Galois automorphisms) -------------------------------------------------------- */
+int csd_hal_create_permutation(csd_ctx* ctx, const size_t* inner_shape, const size_t* shifts, size_t n, csd_perm_plan** out); /* mod.rs:341 */
+int csd_perm_plan_free(csd_perm_plan* plan);
+int csd_hal_permute(csd_ctx* ctx, const csd_view* dst, const csd_view* src, const csd_perm_plan* plan, int dim); /* mod.rs:343-349 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CSDBFV_HAL_H */

@@ -1,0 +1,78 @@
+/*
+ * csdbfv_ring.h — ring-level entry points of the MI355X backend: the RNS ring O_L/q of
+ * cyclosubfield-dbfv's `ring::RingContext` (cyclosubfield-dbfv/src/ring.rs:594-1852) and the
+ * scheme-level HAL sequences of `bfv` that the hot path serves, computed on the device with the
+ * kernels of csdbfv_hal.h. Tensors follow the reference layout [count, r, deg_L', deg_K]
+ * (ring.rs:5-12) in Montgomery form. RNS bases are arrays of indices into the context's primes.
+ * Conventions (status codes, views, streams) as in csdbfv_hal.h.
+ */
+#ifndef CSDBFV_RING_H
+#define CSDBFV_RING_H
+
+#include "csdbfv_hal.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct csd_ring csd_ring;
+
+/* ---- field descriptors (field.rs:128-520, data field.rs:529-769); pure host functions ------- */
+/* m, deg_L, deg_K, deg_L', |Gal(M|L)|, rou_order and the cyclic orders (L' factors first). */
+int csd_field_info(const char* field_name, uint64_t* m, uint64_t* deg_l, uint64_t* deg_k, uint64_t* deg_lprime,
+                   uint64_t* gal_ml_len, uint64_t* rou_order, size_t* orders, size_t orders_cap, size_t* n_orders,
+                   size_t* lprime_factor_count);
+/* FieldDescriptor::gal_to_shift (field.rs:327-336) */
+int csd_field_gal_to_shift(const char* field_name, uint64_t g, int64_t* shifts, size_t cap, size_t* n);
+/* the prime sampler of RingContext::new (ring.rs:136-183, 620-650): initial base and the
+ * extended pool, no device work. */
+int csd_sample_primes(const char* field_name, size_t log2_q_lo, size_t log2_q_hi, size_t t_bits, int64_t* out,
+                      size_t cap, size_t* n, size_t* initial_len);
+
+/* ---- contexts ------------------------------------------------------------------------------- */
+/* RingContext::new (ring.rs:620-650) */
+int csd_ring_create(csd_ctx* ctx, const char* field_name, size_t log2_q_lo, size_t log2_q_hi, size_t t_bits,
+                    csd_ring** out);
+/* RingContext::new_raw (ring.rs:657-677): arbitrary primes, base conversions only */
+int csd_ring_create_raw(csd_ctx* ctx, const char* field_name, const int64_t* primes, size_t n, csd_ring** out);
+int csd_ring_destroy(csd_ring* ring);
+int csd_ring_primes(csd_ring* ring, int64_t* out, size_t cap, size_t* n, size_t* initial_len);
+/* precompute_rns_factor_transforms (ring.rs:1310-1316) */
+int csd_ring_precompute_transforms(csd_ring* ring);
+/* the NTT-domain CRT kernels of one prime, as RnsFactorKernels (ring.rs:559-568); lengths
+ * deg_L' and deg_K each */
+int csd_ring_crt_kernels(csd_ring* ring, size_t prime_index, int64_t* lprime, int64_t* k, int64_t* inv_lprime,
+                         int64_t* inv_k);
+/* extension_rns_base_for (ring.rs:716-731) */
+int csd_ring_extension_base(csd_ring* ring, const size_t* base, size_t r, size_t* out, size_t cap, size_t* n);
+
+/* ---- transforms and conversions --------------------------------------------------------------- */
+/* pb_to_crt_tensor / crt_to_pb_tensor (ring.rs:1117-1161), in place, plain form (four HAL calls). */
+int csd_ring_pb_to_crt(csd_ring* ring, const csd_view* data, const size_t* base, size_t r);
+int csd_ring_crt_to_pb(csd_ring* ring, const csd_view* data, const size_t* base, size_t r);
+/* bconv_tensors (ring.rs:1526-1535, BConvCache::bconv 1663-1701); destroys src */
+int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t* dst_base, size_t r_dst, const csd_view* src,
+                   const size_t* src_base, size_t r_src);
+/* bconv_gadget_tensors (ring.rs:1548-1557, GadgetBConvCache::bconv 1809-1851); digits are
+ * half-open position ranges [digit_bounds[2i], digit_bounds[2i+1]); destroys src */
+int csd_ring_bconv_gadget(csd_ring* ring, const csd_view* dst, const csd_view* src, const size_t* base, size_t r,
+                          const size_t* digit_bounds, size_t ndigits);
+/* mod_switch_down (bfv.rs:628-662): dst [count, new_len, ., .] from src [count, r, ., .] (period
+ * basis); src is left unchanged */
+int csd_ring_mod_switch_down(csd_ring* ring, const csd_view* dst, const csd_view* src, const size_t* base, size_t r,
+                             size_t new_len);
+/* apply_galois (ring.rs:1183-1222): dst = sigma_g(src), any basis */
+int csd_ring_apply_galois(csd_ring* ring, const csd_view* dst, const csd_view* src, uint64_t g);
+/* reduce_wide_tensor (ring.rs:846-865) for rows of equal reduction kind: full = 1 takes
+ * red_mont64_double + i64_to_mont64, full = 0 a single red_mont64 */
+int csd_ring_reduce_wide(csd_ring* ring, const csd_view* dst, const csd_view* wide, const size_t* base, size_t r, int full);
+/* the [1, r, 1, 1] constant tensors of ring.rs:876-920 and 1165-1177, written into dst:
+ * kind 0: mont(scalar mod p); kind 1: mont((scalar mod p)^-1) with the scalar given as the
+ * product of the primes at `scalar_base` */
+int csd_ring_scalar_tensor(csd_ring* ring, const csd_view* dst, const size_t* base, size_t r, int64_t scalar);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CSDBFV_RING_H */
