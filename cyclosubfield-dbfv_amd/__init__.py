@@ -4,3 +4,4 @@ transforms, base conversion, key-switch inner products), behind the C ABI of
 include/csdbfv_hal.h. The package directory name is not a Python identifier; load it with
 `importlib` (see `load_package` in __graft_entry__.py)."""
 from .hal import I128, DeviceTensor, HalError, HipBackend, lib  # noqa: F401
+from . import ring  # noqa: E402,F401

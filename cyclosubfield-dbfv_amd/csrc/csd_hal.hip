@@ -20,6 +20,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/csdbfv_hal.h"
@@ -77,6 +78,54 @@ size_t csd_ctx::alloc_size(void* p) {
     std::lock_guard<std::mutex> g(mu);
     auto it = live.find(p);
     return it == live.end() ? 0 : it->second;
+}
+
+TimedScope::TimedScope(csd_ctx* c, const char* name, double bytes) : ctx(c), idx(SIZE_MAX) {
+    if (!ctx->timing) return;
+    csd_ctx::TimedLaunch t;
+    t.name = name;
+    t.bytes = bytes;
+    HIPCHECK(hipEventCreate(&t.start));
+    HIPCHECK(hipEventCreate(&t.stop));
+    HIPCHECK(hipEventRecord(t.start, ctx->stream));
+    idx = ctx->timed.size();
+    ctx->timed.push_back(t);
+}
+
+TimedScope::~TimedScope() {
+    if (idx != SIZE_MAX) (void)hipEventRecord(ctx->timed[idx].stop, ctx->stream);
+}
+
+extern "C" int csd_ctx_set_timing(csd_ctx* ctx, int on) {
+    CSD_TRY ctx->timing = on != 0;
+    CSD_CATCH
+}
+
+// one line per kernel name: "<name> <launches> <total ms> <total algorithmic bytes>"
+extern "C" int csd_ctx_timing_report(csd_ctx* ctx, char* buf, size_t cap) {
+    CSD_TRY HIPCHECK(hipStreamSynchronize(ctx->stream));
+    std::map<std::string, std::tuple<long, double, double>> agg;
+    for (auto& t : ctx->timed) {
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, t.start, t.stop));
+        auto& a = agg[t.name];
+        std::get<0>(a) += 1;
+        std::get<1>(a) += ms;
+        std::get<2>(a) += t.bytes;
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    ctx->timed.clear();
+    std::string out;
+    char line[256];
+    for (auto& kv : agg) {
+        snprintf(line, sizeof line, "%s %ld %.6f %.1f\n", kv.first.c_str(), std::get<0>(kv.second), std::get<1>(kv.second),
+                 std::get<2>(kv.second));
+        out += line;
+    }
+    if (out.size() + 1 > cap) throw CsdError("timing report buffer too small");
+    memcpy(buf, out.c_str(), out.size() + 1);
+    CSD_CATCH
 }
 
 __global__ void k_fill_zero(uint64_t* p, size_t n) {

@@ -41,11 +41,28 @@ struct csd_ctx {
     std::mutex mu;
     std::map<size_t, std::vector<void*>> pool;
     std::map<void*, size_t> live;
+    // opt-in per-kernel timing: HIP events recorded on `stream` around each timed launch
+    bool timing = false;
+    struct TimedLaunch {
+        std::string name;
+        hipEvent_t start, stop;
+        double bytes;
+    };
+    std::vector<TimedLaunch> timed;
 
     void* alloc_bytes(size_t bytes);
     void free_bytes(void* p);
     size_t alloc_size(void* p);
     ~csd_ctx();
+};
+
+// Records start/stop events around the launches in its scope when the context's timing is on.
+// `bytes` is the launch's algorithmic HBM traffic (read + write of each operand once).
+struct TimedScope {
+    csd_ctx* ctx;
+    size_t idx;
+    TimedScope(csd_ctx* c, const char* name, double bytes);
+    ~TimedScope();
 };
 
 struct Shape4 {

@@ -546,6 +546,8 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         HIPCHECK(hipFuncSetAttribute((const void*)k_ntt, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
+    static const char* names[2][3] = {{"ntt_dim2", "ntt_mul_dim2", "intt_dim2"}, {"ntt_dim3", "ntt_mul_dim3", "intt_dim3"}};
+    TimedScope ts(ctx, names[dim - 2][dir], 16.0 * (double)numel(dst));
     hipLaunchKernelGGL(k_ntt, dim3((unsigned)blocks), dim3(NTT_THREADS), lds, ctx->stream, a, t);
     HIPCHECK(hipGetLastError());
 }

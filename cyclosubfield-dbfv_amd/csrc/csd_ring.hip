@@ -1,0 +1,1019 @@
+// Ring layer of the MI355X backend: the RNS ring O_L/q of cyclosubfield-dbfv's
+// `ring::RingContext` (cyclosubfield-dbfv/src/ring.rs:594-1852) over the HAL of csd_hal.hip.
+//
+// Host responsibilities (one-time setup, as in the reference): field descriptors, RNS prime
+// sampling, CRT-kernel traces, base-conversion constants. Device work: every transform,
+// conversion and reduction goes through the HAL entry points, so the results are exactly the
+// reference's HAL call sequences (ring.rs:1117-1161, 1663-1701, 1809-1851; bfv.rs:628-662).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../include/csdbfv_ring.h"
+#include "csd_internal.hpp"
+
+// ---------------------------------------------------------------------------------------------
+// field descriptors (field.rs)
+
+namespace {
+
+uint64_t mulmod_u(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)(((u128)a * b) % m); }
+
+uint64_t inv_mod_u(uint64_t a, uint64_t n) {
+    i128 old_r = (i128)(a % n), r = (i128)n, old_s = 1, s = 0;
+    while (r != 0) {
+        i128 q = old_r / r, t = old_r - q * r;
+        old_r = r;
+        r = t;
+        t = old_s - q * s;
+        old_s = s;
+        s = t;
+    }
+    if (!(old_r == 1 || old_r == -1)) throw CsdError("elements are not coprime");
+    i128 v = (old_s * old_r) % (i128)n;
+    if (v < 0) v += n;
+    return (uint64_t)v;
+}
+
+uint64_t crt2(uint64_t a1, uint64_t m1, uint64_t a2, uint64_t m2) {
+    if (m1 == 1) return a2 % m2;
+    if (m2 == 1) return a1 % m1;
+    uint64_t k = mulmod_u((a2 + m2 - a1 % m2) % m2, inv_mod_u(m1 % m2, m2), m2);
+    return (uint64_t)(((u128)a1 + (u128)m1 * k) % ((u128)m1 * m2));
+}
+
+std::vector<uint64_t> subgroup(const std::vector<uint64_t>& gens, uint64_t m) {
+    std::vector<uint64_t> out{1};
+    std::vector<uint64_t> frontier{1};
+    std::unordered_map<uint64_t, bool> seen{{1, true}};
+    while (!frontier.empty()) {
+        uint64_t x = frontier.back();
+        frontier.pop_back();
+        for (uint64_t g : gens) {
+            uint64_t y = mulmod_u(x, g, m);
+            if (!seen.count(y)) {
+                seen[y] = true;
+                out.push_back(y);
+                frontier.push_back(y);
+            }
+        }
+    }
+    std::sort(out.begin(), out.end());
+    return out;
+}
+
+struct FieldData {
+    const char* name;
+    uint64_t m, m_prime;
+    std::vector<uint64_t> mp_gens, mk_gens;
+    std::vector<std::pair<uint64_t, size_t>> lq_cyc, kq_cyc;
+};
+
+// field.rs:529-769
+const std::vector<FieldData>& field_table() {
+    static const std::vector<FieldData> t = {
+        {"field_36864", 8536255, 13, {}, {3283177}, {{3939811, 4}, {7879621, 3}},
+         {{7316791, 2}, {1169351, 3}, {4560466, 8}, {8071246, 64}}},
+        {"field_32768", 4259905, 13, {1310741}, {3604537}, {{655371, 4}}, {{2949168, 8192}}},
+        {"field_32768_ex", 3342387, 17, {2162722}, {2752556}, {{1179667, 8}}, {{3276853, 4096}}},
+        {"field_24576", 851981, 13, {}, {196613}, {{327686, 4}, {458760, 3}}, {{393225, 2048}}},
+        {"field_65536", 5570645, 5, {}, {1114131, 2293811}, {{4456517, 4}}, {{5113926, 2}, {5505111, 8192}}},
+        {"field_49152", 35297665, 13, {29867256}, {2715207}, {{32582461, 4}},
+         {{1291681, 3}, {20965738, 64}, {30783429, 64}}},
+        {"field_49152_ex", 10158235, 5, {}, {4063296, 9830566}, {{6094942, 4}},
+         {{7798901, 3}, {9683156, 2}, {4127301, 2048}}},
+        {"field_192", 90627, 3, {}, {30211}, {{60419, 2}}, {{45427, 2}, {4183, 3}, {47533, 16}}},
+        {"field_32", 255, 17, {}, {137}, {{241, 16}}, {{52, 2}}},
+        {"field_18", 1729, 19, {274}, {457}, {{1548, 3}}, {{1483, 2}, {932, 3}}},
+        {"field_16", 257, 1, {}, {2}, {}, {{155, 16}}},
+        {"field_2", 7, 1, {}, {2}, {}, {{6, 2}}},
+    };
+    return t;
+}
+
+struct Field {
+    std::string name;
+    uint64_t m, m_prime;
+    std::vector<uint64_t> m_factors;
+    std::vector<std::pair<uint64_t, size_t>> lq_cyc;
+    size_t lp_count, deg_l, deg_k, deg_lp;
+    std::vector<uint64_t> gal_ml, gal_mp_lp, gal_mk_k;
+    std::unordered_map<uint64_t, size_t> tindex;
+
+    uint64_t m_k() const { return m / m_prime; }
+
+    std::vector<uint64_t> transversal_of(size_t lo, size_t hi) const {  // field.rs:275-295
+        std::vector<uint64_t> res{1};
+        for (size_t f = lo; f < hi; f++) {
+            uint64_t g = lq_cyc[f].first;
+            size_t n = lq_cyc[f].second;
+            std::vector<uint64_t> powers;
+            uint64_t p = 1;
+            for (size_t e = 0; e < n; e++) {
+                powers.push_back(p);
+                p = mulmod_u(p, g, m);
+            }
+            std::vector<uint64_t> next;
+            for (uint64_t x : res)
+                for (uint64_t q : powers) next.push_back(mulmod_u(x, q, m));
+            res = next;
+        }
+        return res;
+    }
+    std::vector<uint64_t> lq_transversal() const { return transversal_of(0, lq_cyc.size()); }
+    std::vector<uint64_t> lp_transversal() const { return transversal_of(0, lp_count); }
+    std::vector<uint64_t> kq_transversal() const { return transversal_of(lp_count, lq_cyc.size()); }
+    std::vector<size_t> orders() const {
+        std::vector<size_t> o;
+        for (auto& c : lq_cyc) o.push_back(c.second);
+        return o;
+    }
+    uint64_t rou_order() const {  // field.rs:440
+        uint64_t l = 1;
+        for (auto& c : lq_cyc) {
+            uint64_t a = l, b = c.second;
+            while (b) {
+                uint64_t t = a % b;
+                a = b;
+                b = t;
+            }
+            l = l / a * c.second;
+        }
+        return m * l;
+    }
+    std::vector<int64_t> gal_to_shift(uint64_t g) const {  // field.rs:327-343
+        size_t idx = SIZE_MAX;
+        for (uint64_t h : gal_ml) {
+            auto it = tindex.find(mulmod_u(g % m, h, m));
+            if (it != tindex.end()) {
+                idx = it->second;
+                break;
+            }
+        }
+        if (idx == SIZE_MAX) throw CsdError("not an element of (Z/mZ)* or inconsistent subgroup data");
+        std::vector<int64_t> res(lq_cyc.size());
+        for (size_t i = lq_cyc.size(); i-- > 0;) {
+            res[i] = (int64_t)(idx % lq_cyc[i].second);
+            idx /= lq_cyc[i].second;
+        }
+        return res;
+    }
+};
+
+std::shared_ptr<Field> make_field(const std::string& name) {  // FieldDescriptor::new (field.rs:159-228)
+    static std::map<std::string, std::shared_ptr<Field>> cache;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(name);
+    if (it != cache.end()) return it->second;
+    const FieldData* d = nullptr;
+    for (auto& x : field_table())
+        if (name == x.name) d = &x;
+    if (!d) throw CsdError("unknown field " + name);
+    auto f = std::make_shared<Field>();
+    f->name = name;
+    f->m = d->m;
+    f->m_prime = d->m_prime;
+    if (d->m_prime != 1) f->m_factors.push_back(d->m_prime);
+    uint64_t rest = d->m / d->m_prime, p = 3, phi = 1;
+    while (rest > 1) {
+        if (rest % p == 0) {
+            rest /= p;
+            f->m_factors.push_back(p);
+        }
+        p += 2;
+    }
+    for (uint64_t q : f->m_factors) phi *= q - 1;
+    f->gal_mp_lp = subgroup(d->mp_gens, f->m);
+    f->gal_mk_k = subgroup(d->mk_gens, f->m);
+    std::vector<uint64_t> all = d->mp_gens;
+    all.insert(all.end(), d->mk_gens.begin(), d->mk_gens.end());
+    f->gal_ml = subgroup(all, f->m);
+    f->lq_cyc = d->lq_cyc;
+    f->lq_cyc.insert(f->lq_cyc.end(), d->kq_cyc.begin(), d->kq_cyc.end());
+    f->lp_count = d->lq_cyc.size();
+    f->deg_l = 1;
+    f->deg_lp = 1;
+    for (auto& c : f->lq_cyc) f->deg_l *= c.second;
+    for (auto& c : d->lq_cyc) f->deg_lp *= c.second;
+    f->deg_k = f->deg_l / f->deg_lp;
+    if (phi != f->deg_l * f->gal_ml.size()) throw CsdError("transversal size must match [L : Q]");
+    auto tr = f->lq_transversal();
+    for (size_t i = 0; i < tr.size(); i++) {
+        if (f->tindex.count(tr[i])) throw CsdError("generators do not induce an isomorphism");
+        f->tindex[tr[i]] = i;
+    }
+    cache[name] = f;
+    return f;
+}
+
+// ---------------------------------------------------------------------------------------------
+// RNS primes (ring.rs:136-183)
+
+uint64_t powmod_u(uint64_t b, uint64_t e, uint64_t m) {
+    uint64_t r = 1 % m;
+    b %= m;
+    while (e) {
+        if (e & 1) r = mulmod_u(r, b, m);
+        b = mulmod_u(b, b, m);
+        e >>= 1;
+    }
+    return r;
+}
+
+bool is_prime_u64(uint64_t n) {  // deterministic Miller-Rabin for 64-bit integers
+    if (n < 2) return false;
+    static const uint64_t bases[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    for (uint64_t q : bases)
+        if (n % q == 0) return n == q;
+    uint64_t d = n - 1;
+    int s = 0;
+    while ((d & 1) == 0) {
+        d >>= 1;
+        s++;
+    }
+    for (uint64_t a : bases) {
+        uint64_t x = powmod_u(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool comp = true;
+        for (int i = 1; i < s; i++) {
+            x = mulmod_u(x, x, n);
+            if (x == n - 1) {
+                comp = false;
+                break;
+            }
+        }
+        if (comp) return false;
+    }
+    return true;
+}
+
+int64_t largest_prime_leq_congruent_to_one(int64_t leq_than, int64_t mod) {
+    if (!(leq_than > mod)) throw CsdError("prime sampling: bound below the congruence modulus");
+    int64_t cur = leq_than - (leq_than - 1) % mod;
+    while (!is_prime_u64((uint64_t)cur)) {
+        cur -= mod;
+        if (cur <= 0) throw CsdError("prime sampling: no prime found");
+    }
+    return cur;
+}
+
+std::vector<int64_t> extend_sampled_primes(const std::vector<int64_t>& begin_with, size_t min_bits, size_t max_bits,
+                                           size_t max_each, int64_t mod) {
+    std::vector<int64_t> res = begin_with;
+    double cur_bits = 0.0;
+    for (int64_t p : res) cur_bits += std::log2((double)p);
+    int64_t upper = (int64_t)1 << max_each;
+    while (cur_bits < (double)min_bits) {
+        double remaining = (double)min_bits - cur_bits;
+        double allowed;
+        if (remaining < (double)max_each) {
+            allowed = std::min((double)max_bits - cur_bits, (double)max_each);
+        } else {
+            double req = std::ceil(remaining / (double)max_each);
+            allowed = std::min(((double)max_bits - cur_bits) / req, (double)max_each);
+        }
+        upper = std::min(upper, (int64_t)1 << (int)std::floor(allowed));
+        int64_t prime = largest_prime_leq_congruent_to_one(upper, mod);
+        upper = prime - 1;
+        while (std::find(begin_with.begin(), begin_with.end(), prime) != begin_with.end()) {
+            prime = largest_prime_leq_congruent_to_one(upper, mod);
+            upper = prime - 1;
+        }
+        cur_bits += std::log2((double)prime);
+        res.push_back(prime);
+    }
+    return res;
+}
+
+// the smallest representative in absolute value of x mod p (odd p)
+int64_t smallest_lift(uint64_t x_mod_p, int64_t p) {
+    return x_mod_p > (uint64_t)p / 2 ? (int64_t)x_mod_p - p : (int64_t)x_mod_p;
+}
+
+// (x << 64) % p with a truncating remainder (ring.rs:109)
+int64_t mont_repr(int64_t x, int64_t p) { return (int64_t)(((i128)x << 64) % (i128)p); }
+
+uint64_t mod_of(int64_t x, int64_t p) {
+    int64_t r = x % p;
+    return (uint64_t)(r < 0 ? r + p : r);
+}
+
+// bit length of a product of positive 64-bit integers
+size_t product_bit_length(const std::vector<int64_t>& xs) {
+    std::vector<uint32_t> a{1};
+    for (int64_t x : xs) {
+        uint64_t lo = (uint64_t)x & 0xffffffffu, hi = (uint64_t)x >> 32;
+        std::vector<uint32_t> r(a.size() + 3, 0);
+        for (int half = 0; half < 2; half++) {
+            uint64_t m = half ? hi : lo;
+            uint64_t carry = 0;
+            for (size_t i = 0; i < a.size(); i++) {
+                uint64_t t = (uint64_t)a[i] * m + r[i + half] + carry;
+                r[i + half] = (uint32_t)t;
+                carry = t >> 32;
+            }
+            size_t k = a.size() + half;
+            while (carry) {
+                uint64_t t = (uint64_t)r[k] + carry;
+                r[k++] = (uint32_t)t;
+                carry = t >> 32;
+            }
+        }
+        while (r.size() > 1 && r.back() == 0) r.pop_back();
+        a = r;
+    }
+    size_t top = a.back() ? 32 - __builtin_clz(a.back()) : 0;
+    return (a.size() - 1) * 32 + top;
+}
+
+struct DevBuf {
+    csd_ctx* ctx = nullptr;
+    void* p = nullptr;
+    DevBuf() {}
+    DevBuf(csd_ctx* c, size_t bytes) : ctx(c), p(c->alloc_bytes(bytes)) {}
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) ctx->free_bytes(p);
+    }
+};
+
+csd_view make_view(void* p, int64_t s0, int64_t s1, int64_t s2, int64_t s3) {
+    csd_view v;
+    v.ptr = p;
+    v.shape[0] = s0;
+    v.shape[1] = s1;
+    v.shape[2] = s2;
+    v.shape[3] = s3;
+    v.strides[3] = 1;
+    v.strides[2] = s3;
+    v.strides[1] = s2 * s3;
+    v.strides[0] = s1 * s2 * s3;
+    return v;
+}
+
+// a sub-view [lo, hi) along one dimension; `elem` is the element size in bytes
+csd_view sub(const csd_view& v, int dim, int64_t lo, int64_t hi, int elem) {
+    csd_view r = v;
+    r.ptr = (char*)v.ptr + lo * v.strides[dim] * elem;
+    r.shape[dim] = hi - lo;
+    return r;
+}
+
+void ok(int rc) {
+    if (rc != 0) throw CsdError(csd_last_error());
+}
+
+std::unique_ptr<DevBuf> upload(csd_ctx* ctx, const std::vector<int64_t>& v) {
+    auto b = std::make_unique<DevBuf>(ctx, std::max<size_t>(8, v.size() * 8));
+    if (!v.empty()) {
+        HIPCHECK(hipMemcpyAsync(b->p, v.data(), v.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return b;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// the ring context
+
+struct Kernels {  // RnsFactorKernels (ring.rs:559-568)
+    std::vector<int64_t> lp, k, inv_lp, inv_k;
+};
+
+struct TransformCache {  // ring.rs:578-592
+    std::vector<csd_modulus> mods;
+    std::vector<csd_ntt_plan*> plans_lp, plans_k;
+    std::unique_ptr<DevBuf> k_lp, k_k, ik_lp, ik_k;
+    ~TransformCache() {
+        for (auto p : plans_lp) csd_ntt_plan_free(p);
+        for (auto p : plans_k) csd_ntt_plan_free(p);
+    }
+};
+
+struct BConvCache {  // ring.rs:1596-1604
+    size_t log2_qtilde, acc;
+    std::unique_ptr<DevBuf> q_over_p_inv, matrix, q_mod_p;
+};
+
+struct GadgetCache {  // ring.rs:1714-1727
+    size_t log2_qtilde, acc;
+    std::unique_ptr<DevBuf> d_over_p_inv, matrix, d_mod_p;
+};
+
+struct csd_ring {
+    csd_ctx* ctx;
+    std::shared_ptr<Field> f;
+    std::vector<int64_t> primes;
+    std::vector<csd_modulus> mods;
+    std::vector<bool> supports_transform;
+    size_t initial_len, t_bits;
+    std::map<int64_t, Kernels> kernels;
+    std::map<std::vector<size_t>, std::unique_ptr<TransformCache>> tcache;
+    std::map<std::pair<std::vector<size_t>, std::vector<size_t>>, std::unique_ptr<BConvCache>> bcache;
+    std::map<std::pair<std::vector<size_t>, std::vector<size_t>>, std::unique_ptr<GadgetCache>> gcache;
+    std::map<std::vector<int64_t>, std::pair<csd_perm_plan*, csd_perm_plan*>> pcache;
+    std::map<std::pair<std::vector<size_t>, std::vector<size_t>>, std::unique_ptr<DevBuf>> inv_scalar_cache;
+
+    ~csd_ring() {
+        for (auto& kv : pcache) {
+            csd_perm_plan_free(kv.second.first);
+            csd_perm_plan_free(kv.second.second);
+        }
+    }
+
+    std::vector<csd_modulus> mods_for(const std::vector<size_t>& base) const {
+        std::vector<csd_modulus> r;
+        for (size_t i : base) r.push_back(mods.at(i));
+        return r;
+    }
+
+    // products of the primes at `base`, reduced mod p
+    uint64_t prod_mod(const std::vector<size_t>& base, int64_t p, size_t skip = SIZE_MAX) const {
+        uint64_t r = 1 % (uint64_t)p;
+        for (size_t t = 0; t < base.size(); t++)
+            if (t != skip) r = mulmod_u(r, (uint64_t)primes[base[t]] % (uint64_t)p, (uint64_t)p);
+        return r;
+    }
+
+    void check_base(const std::vector<size_t>& base) const {
+        for (size_t i : base)
+            if (i >= primes.size()) throw CsdError("RNS base index out of range");
+    }
+
+    // compute_rns_factor_kernels (ring.rs:1337-1415); the forward NTTs run on the device
+    const Kernels& factor_kernels(size_t idx, csd_ntt_plan* plan_lp, csd_ntt_plan* plan_k) {
+        int64_t p = primes[idx];
+        auto it = kernels.find(p);
+        if (it != kernels.end()) return it->second;
+        const Field& F = *f;
+        uint64_t m = F.m;
+        uint64_t zeta = 0;
+        for (uint64_t c = 2;; c++) {
+            zeta = powmod_u(c, ((uint64_t)p - 1) / m, (uint64_t)p);
+            bool good = true;
+            for (uint64_t fac : F.m_factors)
+                if (powmod_u(zeta, m / fac, (uint64_t)p) == 1) good = false;
+            if (good) break;
+        }
+        uint64_t z_mp = powmod_u(zeta, crt2(1, F.m_prime, 0, F.m_k()), (uint64_t)p);
+        uint64_t z_mk = powmod_u(zeta, crt2(0, F.m_prime, 1, F.m_k()), (uint64_t)p);
+        auto traces = [&](uint64_t zp, uint64_t order, const std::vector<uint64_t>& sub_, const std::vector<uint64_t>& tr) {
+            std::vector<uint64_t> powers(order);
+            uint64_t cur = 1;
+            for (uint64_t e = 0; e < order; e++) {
+                powers[e] = cur;
+                cur = mulmod_u(cur, zp, (uint64_t)p);
+            }
+            std::vector<int64_t> out;
+            for (uint64_t g : tr) {
+                uint64_t gi = inv_mod_u(g, m), s = 0;
+                for (uint64_t x : sub_) s = (s + powers[mulmod_u(x, gi, m) % order]) % (uint64_t)p;
+                out.push_back((int64_t)s);
+            }
+            return out;
+        };
+        auto k_lp = traces(z_mp, F.m_prime, F.gal_mp_lp, F.lp_transversal());
+        auto k_k = traces(z_mk, F.m_k(), F.gal_mk_k, F.kq_transversal());
+        auto to_ntt = [&](const std::vector<int64_t>& kern, csd_ntt_plan* plan, int dim) {
+            std::vector<int64_t> mk;
+            for (int64_t x : kern) mk.push_back(mont_repr(x, p));
+            DevBuf b(ctx, mk.size() * 8);
+            csd_view v = dim == 2 ? make_view(b.p, 1, 1, (int64_t)mk.size(), 1) : make_view(b.p, 1, 1, 1, (int64_t)mk.size());
+            ok(csd_hal_import(ctx, &v, mk.data(), mk.size()));
+            ok(csd_hal_ntt_mont64_dim(ctx, &v, &plan, 1, dim));
+            std::vector<int64_t> out(mk.size());
+            ok(csd_hal_export(ctx, &v, out.data(), out.size()));
+            for (auto& x : out) x = (int64_t)mod_of(x, p);
+            return out;
+        };
+        Kernels K;
+        K.lp = to_ntt(k_lp, plan_lp, 2);
+        K.k = to_ntt(k_k, plan_k, 3);
+        // from_mont_repr then invert then mont_repr (ring.rs:1395-1407)
+        uint64_t r_inv = powmod_u(mod_of(mont_repr(1, p), p), (uint64_t)p - 2, (uint64_t)p);
+        auto invert = [&](const std::vector<int64_t>& kern) {
+            std::vector<int64_t> out;
+            for (int64_t x : kern) {
+                uint64_t plain = mulmod_u(mod_of(x, p), r_inv, (uint64_t)p);
+                if (plain == 0) throw CsdError("CRT kernel is not invertible");
+                out.push_back(mont_repr((int64_t)powmod_u(plain, (uint64_t)p - 2, (uint64_t)p), p));
+            }
+            return out;
+        };
+        K.inv_lp = invert(K.lp);
+        K.inv_k = invert(K.k);
+        return kernels[p] = K;
+    }
+
+    TransformCache& transform_cache(const std::vector<size_t>& base) {  // ring.rs:1252-1303
+        auto it = tcache.find(base);
+        if (it != tcache.end()) return *it->second;
+        check_base(base);
+        auto c = std::make_unique<TransformCache>();
+        const Field& F = *f;
+        auto orders = F.orders();
+        std::vector<size_t> o_lp(orders.begin(), orders.begin() + F.lp_count), o_k(orders.begin() + F.lp_count, orders.end());
+        std::vector<int64_t> klp, kk, iklp, ikk;
+        for (size_t idx : base) {
+            if (!supports_transform[idx]) throw CsdError("RNS factor doesn't support the basis transforms");
+            csd_ntt_plan *plp = nullptr, *pk = nullptr;
+            ok(csd_hal_create_ntt_plan(ctx, &mods[idx], o_lp.data(), o_lp.size(), &plp));
+            c->plans_lp.push_back(plp);
+            ok(csd_hal_create_ntt_plan(ctx, &mods[idx], o_k.data(), o_k.size(), &pk));
+            c->plans_k.push_back(pk);
+            const Kernels& K = factor_kernels(idx, plp, pk);
+            klp.insert(klp.end(), K.lp.begin(), K.lp.end());
+            kk.insert(kk.end(), K.k.begin(), K.k.end());
+            iklp.insert(iklp.end(), K.inv_lp.begin(), K.inv_lp.end());
+            ikk.insert(ikk.end(), K.inv_k.begin(), K.inv_k.end());
+        }
+        c->mods = mods_for(base);
+        c->k_lp = upload(ctx, klp);
+        c->k_k = upload(ctx, kk);
+        c->ik_lp = upload(ctx, iklp);
+        c->ik_k = upload(ctx, ikk);
+        auto& ref = *c;
+        tcache[base] = std::move(c);
+        return ref;
+    }
+
+    // pb_to_crt_tensor / crt_to_pb_tensor (ring.rs:1117-1161)
+    void transform(const csd_view& data, const std::vector<size_t>& base, bool inverse_kernels) {
+        TransformCache& c = transform_cache(base);
+        int64_t r = (int64_t)base.size();
+        if (data.shape[1] != r || data.shape[2] != (int64_t)f->deg_lp || data.shape[3] != (int64_t)f->deg_k)
+            throw CsdError("transform: tensor shape doesn't match [count, r, deg_L', deg_K]");
+        csd_view klp = make_view(inverse_kernels ? c.ik_lp->p : c.k_lp->p, 1, r, (int64_t)f->deg_lp, 1);
+        csd_view kk = make_view(inverse_kernels ? c.ik_k->p : c.k_k->p, 1, r, 1, (int64_t)f->deg_k);
+        ok(csd_hal_ntt_mul_mont64_dim(ctx, &data, c.plans_lp.data(), r, CSD_DIM2, &klp, c.mods.data(), r));
+        ok(csd_hal_ntt_mul_mont64_dim(ctx, &data, c.plans_k.data(), r, CSD_DIM3, &kk, c.mods.data(), r));
+        ok(csd_hal_intt_mont64_dim(ctx, &data, c.plans_lp.data(), r, CSD_DIM2));
+        ok(csd_hal_intt_mont64_dim(ctx, &data, c.plans_k.data(), r, CSD_DIM3));
+    }
+
+    BConvCache& bconv_cache(const std::vector<size_t>& src, const std::vector<size_t>& dst) {  // ring.rs:1606-1661
+        auto key = std::make_pair(src, dst);
+        auto it = bcache.find(key);
+        if (it != bcache.end()) return *it->second;
+        check_base(src);
+        check_base(dst);
+        size_t r = src.size(), rd = dst.size();
+        if (r == 0 || rd == 0) throw CsdError("bconv: empty base");
+        int64_t max_p = 0, min_p = INT64_MAX;
+        for (size_t i : src) {
+            max_p = std::max(max_p, primes[i]);
+            min_p = std::min(min_p, primes[i]);
+        }
+        for (size_t i : dst) max_p = std::max(max_p, primes[i]);
+        u128 x = (u128)4 * r * (u128)max_p;  // abs_log2_ceil(4 r max_p + 1) = bit_length(4 r max_p)
+        size_t bits = 0;
+        while (x) {
+            bits++;
+            x >>= 1;
+        }
+        auto c = std::make_unique<BConvCache>();
+        c->log2_qtilde = std::max<size_t>(64, bits);
+        if (c->log2_qtilde >= 126) throw CsdError("bconv: correction precision out of range");
+        u128 qtilde = (u128)1 << c->log2_qtilde;
+        u128 corr_max = (qtilde - 1) / (u128)min_p + 1;
+        if (corr_max > (u128)INT64_MAX) throw CsdError("coefficients of correction vector don't fit into an i64");
+        c->acc = (size_t)((((u128)1 << 127) - 1) / ((u128)max_p * (u128)max_p));
+        if (c->acc < 16) throw CsdError("modular reductions would be needed too often");
+        // q_over_p_inv_tensor (ring.rs:1029-1043): lift((q/p)^-1 mod p)
+        std::vector<int64_t> qopi;
+        for (size_t t = 0; t < r; t++) {
+            int64_t p = primes[src[t]];
+            qopi.push_back(smallest_lift(inv_mod_u(prod_mod(src, p, t), (uint64_t)p), p));
+        }
+        // rows: lift((q/p_j) * 2^128 mod p'_i); correction row round(q~/p_j)
+        std::vector<int64_t> mat;
+        for (size_t i = 0; i < rd; i++) {
+            int64_t pp = primes[dst[i]];
+            uint64_t r_sqr = powmod_u(((u128)1 << 64) % (u128)pp, 2, (uint64_t)pp);
+            for (size_t j = 0; j < r; j++) mat.push_back(smallest_lift(mulmod_u(prod_mod(src, pp, j), r_sqr, (uint64_t)pp), pp));
+        }
+        for (size_t j = 0; j < r; j++) {
+            u128 p = (u128)primes[src[j]];
+            mat.push_back((int64_t)((2 * qtilde + p) / (2 * p)));
+        }
+        // q_mod_p' (ring.rs:1651-1658): lift(q * (-2^64) mod p')
+        std::vector<int64_t> qm;
+        for (size_t i = 0; i < rd; i++) {
+            int64_t pp = primes[dst[i]];
+            uint64_t neg_r = (uint64_t)pp - (uint64_t)(((u128)1 << 64) % (u128)pp);
+            qm.push_back(smallest_lift(mulmod_u(prod_mod(src, pp), neg_r % (uint64_t)pp, (uint64_t)pp), pp));
+        }
+        c->q_over_p_inv = upload(ctx, qopi);
+        c->matrix = upload(ctx, mat);
+        c->q_mod_p = upload(ctx, qm);
+        auto& ref = *c;
+        bcache[key] = std::move(c);
+        return ref;
+    }
+
+    // BConvCache::bconv (ring.rs:1663-1701); destroys src
+    void bconv(const csd_view& dst, const std::vector<size_t>& dbase, const csd_view& src, const std::vector<size_t>& sbase) {
+        BConvCache& c = bconv_cache(sbase, dbase);
+        int64_t r = (int64_t)sbase.size(), rd = (int64_t)dbase.size();
+        int64_t cnt = src.shape[0], n2 = src.shape[2], n3 = src.shape[3];
+        if (src.shape[1] != r) throw CsdError("bconv: source doesn't match its base");
+        if (dst.shape[0] != cnt || dst.shape[1] != rd || dst.shape[2] != n2 || dst.shape[3] != n3)
+            throw CsdError("bconv: destination shape");
+        auto smods = mods_for(sbase), dmods = mods_for(dbase);
+        csd_view qopi = make_view(c.q_over_p_inv->p, 1, r, 1, 1);
+        ok(csd_hal_mul_mont64(ctx, &src, &qopi, smods.data(), r));
+        DevBuf corr(ctx, (size_t)cnt * n2 * n3 * 16), tmp(ctx, (size_t)cnt * (rd + 1) * n2 * n3 * 16);
+        csd_view vcorr = make_view(corr.p, cnt, 1, n2, n3);
+        csd_view vtmp = make_view(tmp.p, cnt, rd + 1, n2, n3);
+        csd_view mat = make_view(c.matrix->p, 1, rd + 1, r, 1);
+        int chunk = 0;
+        for (int64_t j = 0; j < r; j += (int64_t)c.acc, chunk++) {
+            int64_t e = std::min<int64_t>(j + (int64_t)c.acc, r);
+            csd_view ms = sub(mat, 2, j, e, 8), ss = sub(src, 1, j, e, 8);
+            ok(csd_hal_matmul_i64(ctx, &vtmp, &ms, &ss));
+            csd_view tv = sub(vtmp, 1, 0, rd, 16), tc = sub(vtmp, 1, rd, rd + 1, 16);
+            if (chunk == 0) {
+                ok(csd_hal_red_mont64_double(ctx, &dst, &tv, dmods.data(), rd));
+                ok(csd_hal_assign_i128(ctx, &vcorr, &tc));
+            } else {
+                ok(csd_hal_add_red_mont64_double(ctx, &dst, &tv, dmods.data(), rd));
+                ok(csd_hal_add_i128(ctx, &vcorr, &tc));
+            }
+        }
+        csd_view qm = make_view(c.q_mod_p->p, 1, rd, 1, 1);
+        ok(csd_hal_bconv_correct_mont64(ctx, &dst, &vcorr, &qm, dmods.data(), rd, c.log2_qtilde));
+        ok(csd_hal_i64_to_mont64(ctx, &dst, dmods.data(), rd));
+    }
+
+    GadgetCache& gadget_cache(const std::vector<size_t>& base, const std::vector<size_t>& bounds) {  // ring.rs:1730-1807
+        auto key = std::make_pair(base, bounds);
+        auto it = gcache.find(key);
+        if (it != gcache.end()) return *it->second;
+        check_base(base);
+        size_t r = base.size(), d = bounds.size() / 2;
+        if (d == 0) throw CsdError("gadget bconv: no digits");
+        for (size_t i = 0; i < d; i++) {
+            if (!(bounds[2 * i] < bounds[2 * i + 1] && bounds[2 * i + 1] <= r)) throw CsdError("gadget bconv: invalid digit");
+            if (i > 0 && bounds[2 * i - 1] > bounds[2 * i]) throw CsdError("digits must be sorted and disjoint");
+        }
+        int64_t max_p = 0, min_p = INT64_MAX;
+        size_t max_len = 0;
+        for (size_t i : base) {
+            max_p = std::max(max_p, primes[i]);
+            min_p = std::min(min_p, primes[i]);
+        }
+        for (size_t i = 0; i < d; i++) max_len = std::max(max_len, bounds[2 * i + 1] - bounds[2 * i]);
+        u128 x = (u128)4 * max_len * (u128)max_p;
+        size_t bits = 0;
+        while (x) {
+            bits++;
+            x >>= 1;
+        }
+        auto c = std::make_unique<GadgetCache>();
+        c->log2_qtilde = std::max<size_t>(64, bits);
+        u128 qtilde = (u128)1 << c->log2_qtilde;
+        c->acc = (size_t)((((u128)1 << 127) - 1) / ((u128)max_p * (u128)max_p));
+        std::vector<int64_t> dopi(r, 0), mat(d * (r + 1) * r, 0), dmp;
+        for (size_t i = 0; i < d; i++) {
+            std::vector<size_t> dig(base.begin() + bounds[2 * i], base.begin() + bounds[2 * i + 1]);
+            for (size_t u = bounds[2 * i]; u < bounds[2 * i + 1]; u++) {
+                int64_t pu = primes[base[u]];
+                dopi[u] = smallest_lift(inv_mod_u(prod_mod(dig, pu, u - bounds[2 * i]), (uint64_t)pu), pu);
+            }
+            for (size_t j = 0; j < r; j++) {
+                int64_t pj = primes[base[j]];
+                uint64_t r_sqr = powmod_u(((u128)1 << 64) % (u128)pj, 2, (uint64_t)pj);
+                for (size_t u = bounds[2 * i]; u < bounds[2 * i + 1]; u++)
+                    mat[(i * (r + 1) + j) * r + u] =
+                        smallest_lift(mulmod_u(prod_mod(dig, pj, u - bounds[2 * i]), r_sqr, (uint64_t)pj), pj);
+                uint64_t neg_r = (uint64_t)pj - (uint64_t)(((u128)1 << 64) % (u128)pj);
+                dmp.push_back(smallest_lift(mulmod_u(prod_mod(dig, pj), neg_r % (uint64_t)pj, (uint64_t)pj), pj));
+            }
+            for (size_t u = bounds[2 * i]; u < bounds[2 * i + 1]; u++) {
+                u128 pu = (u128)primes[base[u]];
+                mat[(i * (r + 1) + r) * r + u] = (int64_t)((2 * qtilde + pu) / (2 * pu));
+            }
+        }
+        c->d_over_p_inv = upload(ctx, dopi);
+        c->matrix = upload(ctx, mat);
+        c->d_mod_p = upload(ctx, dmp);
+        auto& ref = *c;
+        gcache[key] = std::move(c);
+        return ref;
+    }
+
+    // GadgetBConvCache::bconv (ring.rs:1809-1851); destroys src
+    void bconv_gadget(const csd_view& dst, const csd_view& src, const std::vector<size_t>& base, const std::vector<size_t>& bounds) {
+        GadgetCache& c = gadget_cache(base, bounds);
+        int64_t r = (int64_t)base.size(), d = (int64_t)bounds.size() / 2;
+        int64_t n2 = src.shape[2], n3 = src.shape[3];
+        if (src.shape[0] != 1 || src.shape[1] != r) throw CsdError("gadget bconv: source shape");
+        if (dst.shape[0] != d || dst.shape[1] != r || dst.shape[2] != n2 || dst.shape[3] != n3)
+            throw CsdError("gadget bconv: destination shape");
+        auto m = mods_for(base);
+        csd_view dopi = make_view(c.d_over_p_inv->p, 1, r, 1, 1);
+        ok(csd_hal_mul_mont64(ctx, &src, &dopi, m.data(), r));
+        DevBuf corr(ctx, (size_t)d * n2 * n3 * 16), tmp(ctx, (size_t)d * (r + 1) * n2 * n3 * 16);
+        csd_view vcorr = make_view(corr.p, d, 1, n2, n3), vtmp = make_view(tmp.p, d, r + 1, n2, n3);
+        csd_view mat = make_view(c.matrix->p, d, r + 1, r, 1);
+        int chunk = 0;
+        for (int64_t u = 0; u < r; u += (int64_t)c.acc, chunk++) {
+            int64_t e = std::min<int64_t>(u + (int64_t)c.acc, r);
+            csd_view ms = sub(mat, 2, u, e, 8), ss = sub(src, 1, u, e, 8);
+            ok(csd_hal_matmul_i64(ctx, &vtmp, &ms, &ss));
+            csd_view tv = sub(vtmp, 1, 0, r, 16), tc = sub(vtmp, 1, r, r + 1, 16);
+            if (chunk == 0) {
+                ok(csd_hal_red_mont64_double(ctx, &dst, &tv, m.data(), r));
+                ok(csd_hal_assign_i128(ctx, &vcorr, &tc));
+            } else {
+                ok(csd_hal_add_red_mont64_double(ctx, &dst, &tv, m.data(), r));
+                ok(csd_hal_add_i128(ctx, &vcorr, &tc));
+            }
+        }
+        csd_view dm = make_view(c.d_mod_p->p, d, r, 1, 1);
+        ok(csd_hal_bconv_correct_mont64(ctx, &dst, &vcorr, &dm, m.data(), r, c.log2_qtilde));
+        ok(csd_hal_i64_to_mont64(ctx, &dst, m.data(), r));
+    }
+
+    // inverse_scalar_tensor (ring.rs:1165-1177) for the scalar prod(primes at `of`)
+    DevBuf& inverse_product_tensor(const std::vector<size_t>& base, const std::vector<size_t>& of) {
+        auto key = std::make_pair(base, of);
+        auto it = inv_scalar_cache.find(key);
+        if (it != inv_scalar_cache.end()) return *it->second;
+        std::vector<int64_t> v;
+        for (size_t i : base) {
+            int64_t p = primes[i];
+            uint64_t s = prod_mod(of, p);
+            if (s == 0) throw CsdError("scalar is not invertible");
+            v.push_back(mont_repr(smallest_lift(inv_mod_u(s, (uint64_t)p), p), p));
+        }
+        auto b = upload(ctx, v);
+        auto& ref = *b;
+        inv_scalar_cache[key] = std::move(b);
+        return ref;
+    }
+
+    // mod_switch_down (bfv.rs:628-662)
+    void mod_switch_down(const csd_view& dst, const csd_view& src, const std::vector<size_t>& base, size_t new_len) {
+        int64_t cnt = src.shape[0], n2 = src.shape[2], n3 = src.shape[3];
+        int64_t r = (int64_t)base.size(), nl = (int64_t)new_len;
+        if (src.shape[1] != r || nl > r || nl < 1) throw CsdError("mod_switch_down: invalid base length");
+        if (dst.shape[0] != cnt || dst.shape[1] != nl || dst.shape[2] != n2 || dst.shape[3] != n3)
+            throw CsdError("mod_switch_down: destination shape");
+        std::vector<size_t> kept(base.begin(), base.begin() + nl), dropped(base.begin() + nl, base.end());
+        csd_view src_kept = sub(src, 1, 0, nl, 8);
+        ok(csd_hal_assign(ctx, &dst, &src_kept));
+        if (dropped.empty()) return;
+        int64_t rdr = (int64_t)dropped.size();
+        DevBuf dpart(ctx, (size_t)cnt * rdr * n2 * n3 * 8), tmp(ctx, (size_t)cnt * nl * n2 * n3 * 8);
+        csd_view vd = make_view(dpart.p, cnt, rdr, n2, n3), vt = make_view(tmp.p, cnt, nl, n2, n3);
+        csd_view src_dropped = sub(src, 1, nl, r, 8);
+        ok(csd_hal_assign(ctx, &vd, &src_dropped));
+        bconv(vt, kept, vd, dropped);
+        ok(csd_hal_sub_i64(ctx, &dst, &vt));
+        DevBuf& inv = inverse_product_tensor(kept, dropped);
+        csd_view vinv = make_view(inv.p, 1, nl, 1, 1);
+        auto m = mods_for(kept);
+        ok(csd_hal_mul_mont64(ctx, &dst, &vinv, m.data(), nl));
+    }
+
+    // apply_galois (ring.rs:1183-1222)
+    void apply_galois(const csd_view& dst, const csd_view& src, uint64_t g) {
+        auto shift = f->gal_to_shift(g);
+        auto it = pcache.find(shift);
+        if (it == pcache.end()) {
+            auto o = f->orders();
+            size_t nl = f->lp_count;
+            std::vector<size_t> s_lp, s_k;
+            for (size_t i = 0; i < shift.size(); i++) (i < nl ? s_lp : s_k).push_back((size_t)shift[i]);
+            csd_perm_plan *plp = nullptr, *pk = nullptr;
+            ok(csd_hal_create_permutation(ctx, o.data(), s_lp.data(), nl, &plp));
+            ok(csd_hal_create_permutation(ctx, o.data() + nl, s_k.data(), o.size() - nl, &pk));
+            it = pcache.emplace(shift, std::make_pair(plp, pk)).first;
+        }
+        DevBuf tmp(ctx, (size_t)numel(&src) * 8);
+        csd_view vt = make_view(tmp.p, src.shape[0], src.shape[1], src.shape[2], src.shape[3]);
+        ok(csd_hal_permute(ctx, &vt, &src, it->second.first, CSD_DIM2));
+        ok(csd_hal_permute(ctx, &dst, &vt, it->second.second, CSD_DIM3));
+    }
+};
+
+static std::vector<size_t> to_vec(const size_t* p, size_t n) { return std::vector<size_t>(p, p + n); }
+
+extern "C" int csd_field_info(const char* name, uint64_t* m, uint64_t* deg_l, uint64_t* deg_k, uint64_t* deg_lprime,
+                              uint64_t* gal_ml_len, uint64_t* rou_order, size_t* orders, size_t cap, size_t* n_orders,
+                              size_t* lp_count) {
+    CSD_TRY auto f = make_field(name);
+    *m = f->m;
+    *deg_l = f->deg_l;
+    *deg_k = f->deg_k;
+    *deg_lprime = f->deg_lp;
+    *gal_ml_len = f->gal_ml.size();
+    *rou_order = f->rou_order();
+    auto o = f->orders();
+    if (o.size() > cap) throw CsdError("orders buffer too small");
+    for (size_t i = 0; i < o.size(); i++) orders[i] = o[i];
+    *n_orders = o.size();
+    *lp_count = f->lp_count;
+    CSD_CATCH
+}
+
+extern "C" int csd_field_gal_to_shift(const char* name, uint64_t g, int64_t* shifts, size_t cap, size_t* n) {
+    CSD_TRY auto f = make_field(name);
+    auto s = f->gal_to_shift(g);
+    if (s.size() > cap) throw CsdError("shift buffer too small");
+    for (size_t i = 0; i < s.size(); i++) shifts[i] = s[i];
+    *n = s.size();
+    CSD_CATCH
+}
+
+static void sample(const Field& f, size_t lo, size_t hi, size_t t_bits, std::vector<int64_t>& all, size_t& init_len) {
+    int64_t rou = (int64_t)f.rou_order();
+    auto base = extend_sampled_primes({}, lo, hi, 57, rou);
+    all = extend_sampled_primes(base, 2 * hi + 60 + t_bits, 2 * hi + 60 + t_bits + 30, 57, rou);  // ring.rs:620-632
+    init_len = base.size();
+}
+
+extern "C" int csd_sample_primes(const char* name, size_t lo, size_t hi, size_t t_bits, int64_t* out, size_t cap,
+                                 size_t* n, size_t* initial_len) {
+    CSD_TRY if (hi <= lo) throw CsdError("log2_q range is empty");
+    auto f = make_field(name);
+    std::vector<int64_t> all;
+    size_t il = 0;
+    sample(*f, lo, hi, t_bits, all, il);
+    if (all.size() > cap) throw CsdError("prime buffer too small");
+    std::copy(all.begin(), all.end(), out);
+    *n = all.size();
+    *initial_len = il;
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_create(csd_ctx* ctx, const char* name, size_t lo, size_t hi, size_t t_bits, csd_ring** out) {
+    CSD_TRY if (hi <= lo) throw CsdError("log2_q range is empty");
+    auto r = std::make_unique<csd_ring>();
+    r->ctx = ctx;
+    r->f = make_field(name);
+    size_t il = 0;
+    sample(*r->f, lo, hi, t_bits, r->primes, il);
+    if (r->primes.size() > CSD_MAX_MODULI) throw CsdError("too many RNS factors");
+    r->initial_len = il;
+    r->t_bits = t_bits;
+    for (int64_t p : r->primes) {
+        if (p < ((int64_t)1 << 40) || p > ((int64_t)1 << 57) - 1) throw CsdError("RNS factor out of range");
+        csd_modulus m;
+        ok(csd_hal_create_modulus(p, &m));
+        r->mods.push_back(m);
+        r->supports_transform.push_back(true);
+    }
+    *out = r.release();
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_create_raw(csd_ctx* ctx, const char* name, const int64_t* primes, size_t n, csd_ring** out) {
+    CSD_TRY auto r = std::make_unique<csd_ring>();
+    r->ctx = ctx;
+    r->f = make_field(name);
+    if (n > CSD_MAX_MODULI) throw CsdError("too many RNS factors");
+    for (size_t i = 0; i < n; i++) {
+        int64_t p = primes[i];
+        if (p < ((int64_t)1 << 40) || p > ((int64_t)1 << 57) - 1) throw CsdError("RNS factor out of range");
+        for (size_t k = 0; k < i; k++)
+            if (primes[k] == p) throw CsdError("duplicate RNS factor");
+        csd_modulus m;
+        ok(csd_hal_create_modulus(p, &m));
+        r->primes.push_back(p);
+        r->mods.push_back(m);
+        r->supports_transform.push_back(false);
+    }
+    r->initial_len = n;
+    r->t_bits = 0;
+    *out = r.release();
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_destroy(csd_ring* ring) {
+    CSD_TRY delete ring;
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_primes(csd_ring* ring, int64_t* out, size_t cap, size_t* n, size_t* initial_len) {
+    CSD_TRY if (ring->primes.size() > cap) throw CsdError("prime buffer too small");
+    std::copy(ring->primes.begin(), ring->primes.end(), out);
+    *n = ring->primes.size();
+    *initial_len = ring->initial_len;
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_precompute_transforms(csd_ring* ring) {
+    CSD_TRY for (size_t i = 0; i < ring->primes.size(); i++) if (ring->supports_transform[i]) ring->transform_cache({i});
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_crt_kernels(csd_ring* ring, size_t idx, int64_t* lp, int64_t* k, int64_t* ilp, int64_t* ik) {
+    CSD_TRY if (idx >= ring->primes.size()) throw CsdError("prime index out of range");
+    ring->transform_cache({idx});
+    const Kernels& K = ring->kernels.at(ring->primes[idx]);
+    std::copy(K.lp.begin(), K.lp.end(), lp);
+    std::copy(K.k.begin(), K.k.end(), k);
+    std::copy(K.inv_lp.begin(), K.inv_lp.end(), ilp);
+    std::copy(K.inv_k.begin(), K.inv_k.end(), ik);
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_extension_base(csd_ring* ring, const size_t* base, size_t r, size_t* out, size_t cap, size_t* n) {
+    CSD_TRY auto b = to_vec(base, r);
+    for (size_t i : b)
+        if (i >= ring->initial_len) throw CsdError("extension base: base must lie in the initial base");
+    std::vector<int64_t> ps;
+    for (size_t i : b) ps.push_back(ring->primes[i]);
+    double required = (double)(product_bit_length(ps) + ring->t_bits + 60);  // ring.rs:716-731
+    double bits = 0.0;
+    std::vector<size_t> res;
+    for (size_t idx = ring->initial_len; idx < ring->primes.size(); idx++) {
+        if (bits >= required) break;
+        res.push_back(idx);
+        bits += std::log2((double)ring->primes[idx]);
+    }
+    if (bits < required) throw CsdError("extension pool too small for lifted multiplication over this base");
+    if (res.size() > cap) throw CsdError("buffer too small");
+    std::copy(res.begin(), res.end(), out);
+    *n = res.size();
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_pb_to_crt(csd_ring* ring, const csd_view* data, const size_t* base, size_t r) {
+    CSD_TRY check_view(data, "pb_to_crt");
+    ring->transform(*data, to_vec(base, r), false);
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_crt_to_pb(csd_ring* ring, const csd_view* data, const size_t* base, size_t r) {
+    CSD_TRY check_view(data, "crt_to_pb");
+    ring->transform(*data, to_vec(base, r), true);
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t* dst_base, size_t rd, const csd_view* src,
+                              const size_t* src_base, size_t rs) {
+    CSD_TRY check_view(dst, "bconv dst");
+    check_view(src, "bconv src");
+    ring->bconv(*dst, to_vec(dst_base, rd), *src, to_vec(src_base, rs));
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_bconv_gadget(csd_ring* ring, const csd_view* dst, const csd_view* src, const size_t* base, size_t r,
+                                     const size_t* bounds, size_t nd) {
+    CSD_TRY check_view(dst, "gadget bconv dst");
+    check_view(src, "gadget bconv src");
+    ring->bconv_gadget(*dst, *src, to_vec(base, r), to_vec(bounds, 2 * nd));
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_mod_switch_down(csd_ring* ring, const csd_view* dst, const csd_view* src, const size_t* base, size_t r,
+                                        size_t new_len) {
+    CSD_TRY check_view(dst, "mod_switch_down dst");
+    check_view(src, "mod_switch_down src");
+    ring->mod_switch_down(*dst, *src, to_vec(base, r), new_len);
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_apply_galois(csd_ring* ring, const csd_view* dst, const csd_view* src, uint64_t g) {
+    CSD_TRY check_view(dst, "apply_galois dst");
+    check_view(src, "apply_galois src");
+    ring->apply_galois(*dst, *src, g);
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_reduce_wide(csd_ring* ring, const csd_view* dst, const csd_view* wide, const size_t* base, size_t r,
+                                    int full) {
+    CSD_TRY auto m = ring->mods_for(to_vec(base, r));
+    if (full) {
+        ok(csd_hal_red_mont64_double(ring->ctx, dst, wide, m.data(), r));
+        ok(csd_hal_i64_to_mont64(ring->ctx, dst, m.data(), r));
+    } else {
+        ok(csd_hal_red_mont64(ring->ctx, dst, wide, m.data(), r));
+    }
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_scalar_tensor(csd_ring* ring, const csd_view* dst, const size_t* base, size_t r, int64_t scalar) {
+    CSD_TRY if (dst->shape[0] != 1 || dst->shape[1] != (int64_t)r || dst->shape[2] != 1 || dst->shape[3] != 1)
+        throw CsdError("scalar tensor must have shape [1, r, 1, 1]");
+    std::vector<int64_t> v;
+    for (size_t t = 0; t < r; t++) {
+        int64_t p = ring->primes.at(base[t]);
+        v.push_back(mont_repr(scalar % p, p));  // ring.rs:876-888
+    }
+    ok(csd_hal_import(ring->ctx, dst, v.data(), v.size()));
+    CSD_CATCH
+}

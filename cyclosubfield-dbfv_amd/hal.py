@@ -194,6 +194,20 @@ class HipBackend:
         check(lib().csd_ctx_stream(self.ctx, ctypes.byref(s)))
         return s.value
 
+    def set_timing(self, on):
+        """Records HIP events around the library's timed launches on the context stream."""
+        check(lib().csd_ctx_set_timing(self.ctx, ctypes.c_int(1 if on else 0)))
+
+    def timing_report(self):
+        """{kernel name: (launches, total ms, total algorithmic bytes)}; resets the record."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(lib().csd_ctx_timing_report(self.ctx, buf, ctypes.c_size_t(len(buf))))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms, by = line.split()
+            out[name] = (int(n), float(ms), float(by))
+        return out
+
     # --- allocation and transfer (hal/mod.rs:115-125) --------------------------------------------
     def _alloc(self, shape, elem):
         shape = [int(x) for x in shape]

@@ -1,0 +1,126 @@
+"""Python front end of the ring-level C ABI (include/csdbfv_ring.h): the RNS ring of
+``ring::RingContext`` (cyclosubfield-dbfv/src/ring.rs:594-1852) and the bfv HAL sequences of
+the hot path, computed on the MI355X. Method names follow the reference (``pb_to_crt_tensor``,
+``bconv_tensors``, ``mod_switch_down`` ...); tensors are ``DeviceTensor`` views."""
+import ctypes
+
+import numpy as np
+
+from .hal import check, lib
+
+
+def _sizes(xs):
+    xs = [int(x) for x in xs]
+    return (ctypes.c_size_t * max(1, len(xs)))(*xs), ctypes.c_size_t(len(xs))
+
+
+def field_info(name):
+    """m, deg_L, deg_K, deg_L', |Gal(M|L)|, rou_order, cyclic orders, number of L'-factors."""
+    m, dl, dk, dlp, gml, rou = (ctypes.c_uint64() for _ in range(6))
+    orders = (ctypes.c_size_t * 16)()
+    n, lp = ctypes.c_size_t(), ctypes.c_size_t()
+    check(lib().csd_field_info(name.encode(), ctypes.byref(m), ctypes.byref(dl), ctypes.byref(dk), ctypes.byref(dlp),
+                               ctypes.byref(gml), ctypes.byref(rou), orders, ctypes.c_size_t(16), ctypes.byref(n), ctypes.byref(lp)))
+    return dict(m=m.value, deg_l=dl.value, deg_k=dk.value, deg_lprime=dlp.value, gal_ml=gml.value, rou_order=rou.value,
+                orders=[orders[i] for i in range(n.value)], lprime_factor_count=lp.value)
+
+
+def gal_to_shift(name, g):
+    out = (ctypes.c_int64 * 16)()
+    n = ctypes.c_size_t()
+    check(lib().csd_field_gal_to_shift(name.encode(), ctypes.c_uint64(g), out, ctypes.c_size_t(16), ctypes.byref(n)))
+    return [out[i] for i in range(n.value)]
+
+
+def sample_primes(name, log2_q, t_bits):
+    out = (ctypes.c_int64 * 128)()
+    n, il = ctypes.c_size_t(), ctypes.c_size_t()
+    check(lib().csd_sample_primes(name.encode(), ctypes.c_size_t(log2_q[0]), ctypes.c_size_t(log2_q[1]), ctypes.c_size_t(t_bits),
+                                  out, ctypes.c_size_t(128), ctypes.byref(n), ctypes.byref(il)))
+    return [out[i] for i in range(n.value)], il.value
+
+
+class RingContext:
+    def __init__(self, backend, field, log2_q=None, t_bits=0, raw_primes=None):
+        self.backend = backend
+        self.field_name = field
+        self.info = field_info(field)
+        h = ctypes.c_void_p()
+        if raw_primes is not None:
+            arr = (ctypes.c_int64 * len(raw_primes))(*raw_primes)
+            check(lib().csd_ring_create_raw(backend.ctx, field.encode(), arr, ctypes.c_size_t(len(raw_primes)), ctypes.byref(h)))
+        else:
+            check(lib().csd_ring_create(backend.ctx, field.encode(), ctypes.c_size_t(log2_q[0]), ctypes.c_size_t(log2_q[1]),
+                                        ctypes.c_size_t(t_bits), ctypes.byref(h)))
+        self.handle = h
+        out = (ctypes.c_int64 * 128)()
+        n, il = ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().csd_ring_primes(self.handle, out, ctypes.c_size_t(128), ctypes.byref(n), ctypes.byref(il)))
+        self.primes = [out[i] for i in range(n.value)]
+        self.initial_rns_base_len = il.value
+
+    def close(self):
+        if self.handle is not None:
+            lib().csd_ring_destroy(self.handle)
+            self.handle = None
+
+    def initial_rns_base(self):
+        return list(range(self.initial_rns_base_len))
+
+    def allocate(self, count, base):
+        return self.backend.hal_allocate_i64([count, len(base), self.info["deg_lprime"], self.info["deg_k"]])
+
+    def precompute_rns_factor_transforms(self):
+        check(lib().csd_ring_precompute_transforms(self.handle))
+
+    def crt_kernels(self, idx):
+        dlp, dk = self.info["deg_lprime"], self.info["deg_k"]
+        a, b, c, d = (np.zeros(n, dtype=np.int64) for n in (dlp, dk, dlp, dk))
+        check(lib().csd_ring_crt_kernels(self.handle, ctypes.c_size_t(idx), *[x.ctypes.data_as(ctypes.c_void_p) for x in (a, b, c, d)]))
+        return a, b, c, d
+
+    def extension_rns_base_for(self, base):
+        b, n = _sizes(base)
+        out = (ctypes.c_size_t * 128)()
+        cnt = ctypes.c_size_t()
+        check(lib().csd_ring_extension_base(self.handle, b, n, out, ctypes.c_size_t(128), ctypes.byref(cnt)))
+        return [out[i] for i in range(cnt.value)]
+
+    def pb_to_crt_tensor(self, data, base):
+        b, n = _sizes(base)
+        check(lib().csd_ring_pb_to_crt(self.handle, ctypes.byref(data.view()), b, n))
+
+    def crt_to_pb_tensor(self, data, base):
+        b, n = _sizes(base)
+        check(lib().csd_ring_crt_to_pb(self.handle, ctypes.byref(data.view()), b, n))
+
+    def bconv_tensors(self, dst, dst_base, src, src_base):
+        db, dn = _sizes(dst_base)
+        sb, sn = _sizes(src_base)
+        check(lib().csd_ring_bconv(self.handle, ctypes.byref(dst.view()), db, dn, ctypes.byref(src.view()), sb, sn))
+
+    def bconv_gadget_tensors(self, dst, src, base, digits):
+        b, n = _sizes(base)
+        bounds = [x for d in digits for x in d]
+        bb, _ = _sizes(bounds)
+        check(lib().csd_ring_bconv_gadget(self.handle, ctypes.byref(dst.view()), ctypes.byref(src.view()), b, n, bb,
+                                          ctypes.c_size_t(len(digits))))
+
+    def mod_switch_down(self, dst, src, base, new_len):
+        b, n = _sizes(base)
+        check(lib().csd_ring_mod_switch_down(self.handle, ctypes.byref(dst.view()), ctypes.byref(src.view()), b, n,
+                                             ctypes.c_size_t(new_len)))
+
+    def apply_galois(self, dst, src, g):
+        check(lib().csd_ring_apply_galois(self.handle, ctypes.byref(dst.view()), ctypes.byref(src.view()), ctypes.c_uint64(g)))
+
+    def reduce_wide_tensor(self, dst, wide, base, full):
+        b, n = _sizes(base)
+        check(lib().csd_ring_reduce_wide(self.handle, ctypes.byref(dst.view()), ctypes.byref(wide.view()), b, n,
+                                         ctypes.c_int(1 if full else 0)))
+
+    def scalar_tensor(self, base, scalar):
+        t = self.backend.hal_allocate_i64([1, len(base), 1, 1])
+        b, n = _sizes(base)
+        check(lib().csd_ring_scalar_tensor(self.handle, ctypes.byref(t.view()), b, n, ctypes.c_int64(scalar)))
+        return t

@@ -57,6 +57,11 @@ int csd_ctx_destroy(csd_ctx* ctx);
 int csd_ctx_sync(csd_ctx* ctx);
 /* the hipStream_t all work of the context is enqueued on */
 int csd_ctx_stream(csd_ctx* ctx, void** stream);
+/* opt-in kernel timing with HIP events on the context's stream (no reference counterpart; the
+ * reference's tracing spans, hal/cpu.rs:560-1086, play this role on the CPU). The report has one
+ * line per kernel: "<name> <launches> <total ms> <total algorithmic bytes>", and resets. */
+int csd_ctx_set_timing(csd_ctx* ctx, int on);
+int csd_ctx_timing_report(csd_ctx* ctx, char* buf, size_t cap);
 
 /* ---- allocation and transfer --------------------------------------------------------------- */
 /* hal_allocate_i64 / hal_allocate_i128 (hal/mod.rs:119-122): zero-initialised, row-major;

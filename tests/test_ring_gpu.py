@@ -1,0 +1,189 @@
+"""Parity of the ring layer on the MI355X (csd_ring_* through the C ABI) with the oracle ring
+layer, bit for bit: period<->CRT transforms (C1 field_2, field_18, field_36864 at full depth),
+CRT kernels, BEHZ base conversion (the reference KAT and random), gadget conversion, mod-switch
+and Galois automorphisms; plus the reference's own ring KATs run on the device."""
+import numpy as np
+import pytest
+
+from _pkg import load
+from oracle import ring as R
+from test_hal_gpu import from_dev, rand_i64, same, to_dev
+
+pytestmark = pytest.mark.gpu
+
+BCONV_PRIMES = [
+    144115188075855859, 144115188075855847, 144115188075855823, 144115188075855811, 144115188075855803,
+    144115188075855761, 144115188075855677, 144115188075855599, 144115188075855509, 144115188075855449,
+    144115188075855439, 144115188075855421, 144115188075855413,
+]
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    be = load().HipBackend(0)
+    yield be
+    be.close()
+
+
+@pytest.fixture(scope="module")
+def rings(gpu, cpu):
+    pkg = load()
+    made = {}
+
+    def get(field, log2_q, t_bits):
+        key = (field, log2_q, t_bits)
+        if key not in made:
+            made[key] = (pkg.ring.RingContext(gpu, field, log2_q, t_bits),
+                         R.RingContext(cpu, R.FieldDescriptor(field), log2_q, t_bits))
+        return made[key]
+
+    yield get
+    for g, _ in made.values():
+        g.close()
+
+
+def uniform_residues(rng, primes, shape):
+    """residues uniform in [0, p) per limb (dimension 1)"""
+    out = np.zeros(shape, dtype=np.int64)
+    for j, p in enumerate(primes):
+        out[:, j] = rng.integers(0, p, size=out[:, j].shape, dtype=np.int64)
+    return out
+
+
+@pytest.mark.parametrize("field,log2_q,t_bits,count", [("field_2", (200, 230), 0, 16), ("field_18", (100, 120), 4, 3),
+                                                       ("field_192", (100, 120), 8, 2), ("field_36864", (960, 980), 76, 2)])
+def test_transform_parity(gpu, cpu, rings, field, log2_q, t_bits, count):
+    """Config C1 (field_2, 4 primes, 16 elements) and larger fields: pb->CRT, a pointwise
+    Montgomery product, CRT->pb; every residue bit-exact with the oracle."""
+    g, c = rings(field, log2_q, t_bits)
+    assert g.primes == c.primes
+    base = g.initial_rns_base()
+    primes = [g.primes[i] for i in base]
+    rng = np.random.default_rng(2024)
+    shape = (count, len(base), g.info["deg_lprime"], g.info["deg_k"])
+    data = uniform_residues(rng, primes, shape)
+    other = uniform_residues(rng, primes, shape)
+    dc, oc = data.copy(), other.copy()
+    c.pb_to_crt_tensor(dc, base)
+    cpu.hal_mul_mont64(dc, oc, c.mods_for(base))
+    c.crt_to_pb_tensor(dc, base)
+    dg, og = to_dev(gpu, data), to_dev(gpu, other)
+    g.pb_to_crt_tensor(dg, base)
+    gpu.hal_mul_mont64(dg, og, [gpu.hal_create_modulus(p) for p in primes])
+    g.crt_to_pb_tensor(dg, base)
+    assert same(from_dev(gpu, dg), dc)
+
+
+def test_crt_kernels_match_oracle(gpu, cpu, rings):
+    g, c = rings("field_36864", (960, 980), 76)
+    for idx in [0, 16, 36]:
+        k = g.crt_kernels(idx)
+        tc = c.transform_cache([idx])
+        exp = [c._kernels[c.primes[idx]][t] for t in range(4)]
+        for a, b in zip(k, exp):
+            assert list(a) == list(b)
+        del tc
+
+
+def test_field18_square_kat_on_device(gpu, rings):
+    """ring.rs:1974-2012 (sage KAT), through the device transforms and wide accumulation."""
+    g, _ = rings("field_18", (100, 120), 4)
+    base = [0]
+    p = g.primes[0]
+    els = g.allocate(1, base)
+    data = [0] * 18
+    for blk in range(3):
+        data[blk * 6] = R.mont_repr(1, p)
+    gpu.hal_import(els, data)
+    g.pb_to_crt_tensor(els, base)
+    wide = gpu.hal_allocate_i128([1, 1, 3, 6])
+    gpu.hal_mul_i64(wide, els, els)
+    sqr = g.allocate(1, base)
+    g.reduce_wide_tensor(sqr, wide, base, full=False)
+    g.crt_to_pb_tensor(sqr, base)
+    out = gpu.hal_export(sqr)
+    exp = [3, 4, 2, -2, -4, -2]
+    for blk in range(3):
+        for j in range(6):
+            assert int(out[blk * 6 + j]) % p == R.montgomery_repr(exp[j], p)
+
+
+def test_bconv_kat_on_device(gpu):
+    """ring.rs:1859-1917, on the device."""
+    ring = load().ring.RingContext(gpu, "field_2", raw_primes=BCONV_PRIMES)
+    frm, to = BCONV_PRIMES[:7], BCONV_PRIMES[7:]
+    d1 = [0, 1, 2, 3, 4, 5, 6, 7]
+    d2 = [1 << 64, 3 << 100, -7 << 100, 9 << 120, -1 << 120, 5 << 120, 0, 7 << 120]
+    vec = [R.montgomery_repr(R._trunc_mod(x, p), p) for data in (d1, d2) for p in frm for x in data]
+    src = gpu.hal_allocate_i64([2, 7, 8, 1])
+    gpu.hal_import(src, vec)
+    dst = gpu.hal_allocate_i64([2, 6, 8, 1])
+    ring.bconv_tensors(dst, list(range(7, 13)), src, list(range(7)))
+    out = gpu.hal_export(dst)
+    for b, data in enumerate((d1, d2)):
+        for i, p in enumerate(to):
+            for j, x in enumerate(data):
+                assert int(out[b * 48 + i * 8 + j]) % p == R.montgomery_repr(x, p)
+    ring.close()
+
+
+def test_bconv_parity(gpu, cpu, rings):
+    """The lift q -> extension of field_36864 (17 -> 20 primes) and its reverse, bit-exact."""
+    g, c = rings("field_36864", (960, 980), 76)
+    base = g.initial_rns_base()
+    ext = g.extension_rns_base_for(base)
+    assert ext == c.extension_rns_base_for(base)
+    rng = np.random.default_rng(7)
+    for sb, db in [(base, ext), (ext, base), (base[:5], base[5:9])]:
+        primes = [g.primes[i] for i in sb]
+        src = uniform_residues(rng, primes, (2, len(sb), 12, 3072))
+        src[0, :, 0, :8] = 0  # zero and small values too
+        dc = np.zeros((2, len(db), 12, 3072), dtype=np.int64)
+        c.bconv_tensors(dc, db, src.copy(), sb)
+        dg = gpu.hal_allocate_i64(dc.shape)
+        g.bconv_tensors(dg, db, to_dev(gpu, src), sb)
+        assert same(from_dev(gpu, dg), dc)
+
+
+def test_gadget_bconv_parity(gpu, cpu, rings):
+    g, c = rings("field_36864", (960, 980), 76)
+    base = g.initial_rns_base()
+    digits = R.select_digits(9, len(base))
+    rng = np.random.default_rng(8)
+    src = uniform_residues(rng, [g.primes[i] for i in base], (1, len(base), 12, 3072))
+    dc = np.zeros((len(digits), len(base), 12, 3072), dtype=np.int64)
+    c.bconv_gadget_tensors(dc, src.copy(), base, digits)
+    dg = gpu.hal_allocate_i64(dc.shape)
+    g.bconv_gadget_tensors(dg, to_dev(gpu, src), base, digits)
+    assert same(from_dev(gpu, dg), dc)
+
+
+@pytest.mark.parametrize("r,new_len", [(17, 16), (17, 3), (9, 8), (3, 1)])
+def test_mod_switch_parity(gpu, cpu, rings, r, new_len):
+    """Dynamic scaling (bfv.rs:628-662) at uneven levels, bit-exact."""
+    g, c = rings("field_36864", (960, 980), 76)
+    base = list(range(r))
+    rng = np.random.default_rng(100 + r)
+    src = uniform_residues(rng, [g.primes[i] for i in base], (2, r, 12, 3072))
+    dc = R.mod_switch_down(c, src.copy(), base, new_len)
+    dg = gpu.hal_allocate_i64((2, new_len, 12, 3072))
+    sg = to_dev(gpu, src)
+    g.mod_switch_down(dg, sg, base, new_len)
+    assert same(from_dev(gpu, dg), dc)
+    assert same(from_dev(gpu, sg), src)  # the source is left unchanged
+
+
+def test_galois_parity(gpu, cpu, rings):
+    g, c = rings("field_36864", (960, 980), 76)
+    base = [0, 1, 2]
+    rng = np.random.default_rng(9)
+    src = rand_i64(rng, (2, 3, 12, 3072))
+    f = c.field
+    for gel in [f.lq_cyclic[0][0], f.lq_cyclic[3][0], f.lq_cyclic[5][0] * f.lq_cyclic[1][0] % f.m, 2]:
+        if np.gcd(gel, f.m) != 1:
+            continue
+        dc = np.zeros_like(src)
+        c.apply_galois(dc, src, gel)
+        dg = gpu.hal_allocate_i64(src.shape)
+        g.apply_galois(dg, to_dev(gpu, src), gel)
+        assert same(from_dev(gpu, dg), dc)
