@@ -149,8 +149,27 @@ def main():
     avg_ms = total_ms / launches
     bytes_per_launch = total_bytes / launches
     achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
-    per_kernel = {k: {"launches": n, "avg_ms": round(t / n, 4), "GB/s": round(b / n / (t / n / 1000.0) / 1e9, 1)}
-                  for k, (n, t, b) in kernels.items()}
+
+    # the integer-VALU side: modular multiplications per launch (from the plans' schedules)
+    # against the measured Barrett-multiplication ceiling of the chip
+    m0 = be.hal_create_modulus(primes[0])
+    plan_lp = be.hal_create_ntt_plan(m0, info["orders"][:info["lprime_factor_count"]])
+    plan_k = be.hal_create_ntt_plan(m0, info["orders"][info["lprime_factor_count"]:])
+    rows = args.count * len(primes) * info["deg_lprime"]
+    cols = args.count * len(primes) * info["deg_k"]
+    mults = {"ntt_mul_dim3": rows * (plan_k.mults("fwd_unreduced") + info["deg_k"]),
+             "intt_dim3": rows * plan_k.mults("inv"),
+             "ntt_mul_dim2": cols * (plan_lp.mults("fwd_unreduced") + info["deg_lprime"]),
+             "intt_dim2": cols * plan_lp.mults("inv")}
+    copy_gbs = be.calibrate("copy", 1 << 30)
+    barrett_gps = be.calibrate("barrett", 1 << 24)
+    per_kernel = {}
+    for k, (n, t, b) in kernels.items():
+        e = {"launches": n, "avg_ms": round(t / n, 4), "GB/s": round(b / n / (t / n / 1000.0) / 1e9, 1)}
+        if k in mults:
+            e["Gmul/s"] = round(mults[k] / (t / n / 1000.0) / 1e9, 1)
+            e["valu_frac"] = round(e["Gmul/s"] / barrett_gps, 3)
+        per_kernel[k] = e
     result = {
         "metric": "period<->CRT transforms/s @field_36864",
         "value": round(value, 1),
@@ -169,7 +188,10 @@ def main():
                    "field": FIELD, "rns_primes": len(primes), "elements": args.count, "form": "plain"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 4)},
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 4),
+                     "valu": {"achieved": per_kernel[name].get("Gmul/s"), "ceiling": round(barrett_gps, 1),
+                              "unit": "G modmul/s", "frac": per_kernel[name].get("valu_frac")}},
+        "ceilings": {"hbm_copy_GB/s": round(copy_gbs, 1), "barrett_Gmul/s": round(barrett_gps, 1)},
         "kernels": per_kernel,
         "wall_s": round(wall, 3),
     }

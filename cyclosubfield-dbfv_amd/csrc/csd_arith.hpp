@@ -29,10 +29,25 @@ struct CsdBarrett {
     int64_t u;
 };
 
+// The signed 64 x 64 -> 128 products below are formed as one unsigned product (four 32-bit
+// multiply-adds on gfx950) plus sign corrections of the high word, instead of the compiler's
+// sign-extended 128 x 128 multiply.
 CSD_HD uint64_t csd_umulhi(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * (u128)b) >> 64); }
 
+// (low, high) words of the signed product a * b
+CSD_HD void csd_smul_wide(int64_t a, int64_t b, uint64_t& lo, int64_t& hi) {
+    u128 p = (u128)(uint64_t)a * (uint64_t)b;
+    lo = (uint64_t)p;
+    hi = (int64_t)((uint64_t)(p >> 64) - (a < 0 ? (uint64_t)b : 0) - (b < 0 ? (uint64_t)a : 0));
+}
+
 // high word of the signed 128-bit product
-CSD_HD int64_t csd_smulhi(int64_t a, int64_t b) { return (int64_t)(((i128)a * (i128)b) >> 64); }
+CSD_HD int64_t csd_smulhi(int64_t a, int64_t b) {
+    uint64_t lo;
+    int64_t hi;
+    csd_smul_wide(a, b, lo, hi);
+    return hi;
+}
 
 // high word of a signed * non-negative product
 CSD_HD int64_t csd_smulhi_pos(int64_t a, int64_t b_nonneg) {
@@ -49,8 +64,9 @@ CSD_HD int64_t csd_red128(const CsdMod& m, i128 x) { return csd_red_words(m, (ui
 
 // red128(a * b): the Montgomery product of two i64 values
 CSD_HD int64_t csd_mont_mul(const CsdMod& m, int64_t a, int64_t b) {
-    uint64_t lo = (uint64_t)a * (uint64_t)b;
-    int64_t hi = csd_smulhi(a, b);
+    uint64_t lo;
+    int64_t hi;
+    csd_smul_wide(a, b, lo, hi);
     return csd_red_words(m, lo, hi);
 }
 
@@ -70,7 +86,10 @@ CSD_HD int64_t csd_to_mont128(const CsdMod& m, i128 x) { return csd_mont_mul(m, 
 
 // BarrettMul::mul (ntt.rs:149-152): |x| <= 2^63 -> |t| <= 3N/4, t = z x mod N
 CSD_HD int64_t csd_barrett(CsdBarrett b, int64_t x, int64_t n) {
-    uint64_t lo = (uint64_t)b.u * (uint64_t)x;
-    int64_t q = (int64_t)((uint64_t)csd_smulhi(b.u, x) + (lo >> 63));
-    return (int64_t)((uint64_t)b.z * (uint64_t)x + (uint64_t)q * (uint64_t)n);
+    uint64_t lo;
+    int64_t hi;
+    csd_smul_wide(b.u, x, lo, hi);
+    // q = (u x + 2^63) >> 64 = hi + [lo >= 2^63]
+    uint64_t q = (uint64_t)hi + (lo >> 63);
+    return (int64_t)((uint64_t)b.z * (uint64_t)x + q * (uint64_t)n);
 }

@@ -991,3 +991,75 @@ extern "C" int csd_hal_permute(csd_ctx* ctx, const csd_view* dst, const csd_view
     HIPCHECK(hipGetLastError());
     CSD_CATCH
 }
+
+// ---------------------------------------------------------------------------------------------
+// calibration kernels: the two ceilings the hot-path kernels are measured against
+
+__global__ __launch_bounds__(256) void k_calib_copy(const int4* __restrict__ src, int4* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// `iters` dependent Barrett multiplications per element, 4 independent chains per thread
+__global__ __launch_bounds__(256) void k_calib_barrett(int64_t* data, CsdBarrett b, int64_t n, int iters) {
+    size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    int64_t x0 = data[i], x1 = data[i + 1], x2 = data[i + 2], x3 = data[i + 3];
+    for (int k = 0; k < iters; k++) {
+        x0 = csd_barrett(b, x0, n);
+        x1 = csd_barrett(b, x1, n);
+        x2 = csd_barrett(b, x2, n);
+        x3 = csd_barrett(b, x3, n);
+    }
+    data[i] = x0;
+    data[i + 1] = x1;
+    data[i + 2] = x2;
+    data[i + 3] = x3;
+}
+
+// kind 0: device-to-device copy of `n` bytes, result in GB/s (read + write bytes / time);
+// kind 1: Barrett multiplications, `n` elements x 64 iterations, result in G multiplications/s
+extern "C" int csd_calibrate(csd_ctx* ctx, int kind, size_t n, double* out) {
+    CSD_TRY hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    float ms = 0;
+    if (kind == 0) {
+        size_t words = n / 16;
+        void* a = ctx->alloc_bytes(words * 16);
+        void* b = ctx->alloc_bytes(words * 16);
+        csd_zero_fill(ctx, a, words * 16);
+        unsigned blocks = 256 * 64;
+        hipLaunchKernelGGL(k_calib_copy, dim3(blocks), dim3(256), 0, ctx->stream, (const int4*)a, (int4*)b, words);
+        HIPCHECK(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < 5; r++)
+            hipLaunchKernelGGL(k_calib_copy, dim3(blocks), dim3(256), 0, ctx->stream, (const int4*)a, (int4*)b, words);
+        HIPCHECK(hipEventRecord(e1, ctx->stream));
+        HIPCHECK(hipEventSynchronize(e1));
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        *out = 5.0 * 2.0 * (double)(words * 16) / (ms / 1000.0) / 1e9;
+        ctx->free_bytes(a);
+        ctx->free_bytes(b);
+    } else if (kind == 1) {
+        size_t elems = (n + 1023) / 1024 * 1024;
+        void* a = ctx->alloc_bytes(elems * 8);
+        csd_zero_fill(ctx, a, elems * 8);
+        const int64_t N = 144115176243904321;
+        CsdBarrett bm = csd_host_barrett(123456789123, N);
+        unsigned blocks = (unsigned)(elems / 4 / 256);
+        hipLaunchKernelGGL(k_calib_barrett, dim3(blocks), dim3(256), 0, ctx->stream, (int64_t*)a, bm, N, 8);
+        HIPCHECK(hipEventRecord(e0, ctx->stream));
+        hipLaunchKernelGGL(k_calib_barrett, dim3(blocks), dim3(256), 0, ctx->stream, (int64_t*)a, bm, N, 64);
+        HIPCHECK(hipEventRecord(e1, ctx->stream));
+        HIPCHECK(hipEventSynchronize(e1));
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        *out = (double)elems * 64.0 / (ms / 1000.0) / 1e9;
+        ctx->free_bytes(a);
+    } else {
+        throw CsdError("unknown calibration kind");
+    }
+    HIPCHECK(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    CSD_CATCH
+}

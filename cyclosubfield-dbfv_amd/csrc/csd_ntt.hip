@@ -97,12 +97,23 @@ struct DevStep {
 };
 
 struct DevFactor {
-    int32_t len, radix, nstages;
+    int32_t len, radix, nstages, index;
     uint32_t S, magic_S;  // suffix length (product of the later factor lengths)
     uint32_t stage_len[NTT_MAX_STAGES], part[NTT_MAX_STAGES], nblocks[NTT_MAX_STAGES];
     uint32_t magic_part[NTT_MAX_STAGES], magic_nb[NTT_MAX_STAGES];
     int32_t tw_off[NTT_MAX_STAGES], itw_off[NTT_MAX_STAGES];
     CsdBarrett zeta, zeta2;
+};
+
+// Register-blocked passes: a pass applies stages [s0, s0 + g) of one factor to groups of R^g
+// elements held in registers (in decreasing stage order for the inverse).
+#define NTT_MAX_PASSES 24
+#define INFO_REDUCE 1  // a full reduction pass precedes this stage (in processing order)
+#define INFO_SKIP 2    // forward: twiddle-1 blocks may skip their multiplication
+#define INFO_FUSE 4    // forward: fused output reduction; inverse: fused len^-1 scaling
+
+struct DevPass {
+    uint8_t factor, s0, g, radix;
 };
 
 struct DevPlan {
@@ -112,6 +123,10 @@ struct DevPlan {
     DevFactor f[NTT_MAX_FACTORS];
     int32_t nsteps[3];
     DevStep steps[3][NTT_MAX_STEPS];
+    // per (direction, factor, stage): INFO_* bits, derived from `steps`
+    uint8_t info[3][NTT_MAX_FACTORS][NTT_MAX_STAGES];
+    int32_t npasses[3];
+    DevPass passes[3][NTT_MAX_PASSES];
     // followed in memory by the Barrett twiddle table
 };
 
@@ -176,6 +191,41 @@ static void build_schedule(DevPlan& P, int dir) {
     if (st.size() > NTT_MAX_STEPS) throw CsdError("NTT schedule too long");
     P.nsteps[dir] = (int32_t)st.size();
     for (size_t i = 0; i < st.size(); i++) P.steps[dir][i] = st[i];
+    // the same schedule as per-stage flags, for the register-blocked kernels
+    bool pending_reduce = false;
+    for (const DevStep& s : st) {
+        if (s.kind == STEP_REDUCE) {
+            pending_reduce = true;
+        } else if (s.kind == STEP_FWD || s.kind == STEP_INV) {
+            uint8_t v = pending_reduce ? INFO_REDUCE : 0;
+            if (s.kind == STEP_FWD) v |= ((s.flags & 1) ? INFO_SKIP : 0) | ((s.flags & 2) ? INFO_FUSE : 0);
+            if (s.kind == STEP_INV) v |= (s.flags & 1) ? INFO_FUSE : 0;
+            P.info[dir][s.factor][s.stage] = v;
+            pending_reduce = false;
+        }
+    }
+    // passes: at most 3 radix-2 or 2 radix-3 stages per register block
+    std::vector<DevPass> passes;
+    for (int fi = 0; fi < P.nfactors; fi++) {
+        const DevFactor& F = P.f[fi];
+        const int G = F.radix == 2 ? 3 : 2;
+        if (!inverse) {
+            for (int s0 = 0; s0 < F.nstages;) {
+                int g = std::min(G, F.nstages - s0);
+                passes.push_back(DevPass{(uint8_t)fi, (uint8_t)s0, (uint8_t)g, (uint8_t)F.radix});
+                s0 += g;
+            }
+        } else {
+            for (int hi = F.nstages; hi > 0;) {
+                int g = std::min(G, hi);
+                passes.push_back(DevPass{(uint8_t)fi, (uint8_t)(hi - g), (uint8_t)g, (uint8_t)F.radix});
+                hi -= g;
+            }
+        }
+    }
+    if (passes.size() > NTT_MAX_PASSES) throw CsdError("NTT plan: too many passes");
+    P.npasses[dir] = (int32_t)passes.size();
+    for (size_t i = 0; i < passes.size(); i++) P.passes[dir][i] = passes[i];
 }
 
 static bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
@@ -209,6 +259,7 @@ extern "C" int csd_hal_create_ntt_plan(csd_ctx* ctx, const csd_modulus* modulus,
         size_t L = lens[fi];
         suffix /= L;
         F.len = (int32_t)L;
+        F.index = (int32_t)fi;
         F.S = (uint32_t)suffix;
         F.magic_S = magic_for(F.S);
         if (is_pow2(L)) {
@@ -292,6 +343,49 @@ extern "C" int csd_hal_create_ntt_plan(csd_ctx* ctx, const csd_modulus* modulus,
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     plan->dev_tw = (const CsdBarrett*)((const char*)plan->dev + sizeof(DevPlan));
     *out = plan;
+    CSD_CATCH
+}
+
+// Modular multiplications (Barrett or Montgomery) one transform of the plan performs in
+// direction dir (0 forward reduced, 1 forward unreduced, 2 inverse), following the schedule and
+// the twiddle-1 skips exactly; the algorithmic work unit of the integer-VALU roofline.
+extern "C" int csd_ntt_plan_mults(const csd_ntt_plan* plan, int dir, uint64_t* out) {
+    CSD_TRY if (dir < 0 || dir > 2) throw CsdError("dir must be 0, 1 or 2");
+    const DevPlan& P = plan->host;
+    const uint64_t L = (uint64_t)P.len;
+    uint64_t cnt = 0;
+    for (int i = 0; i < P.nsteps[dir]; i++) {
+        const DevStep& s = P.steps[dir][i];
+        if (s.kind == STEP_REDUCE || s.kind == STEP_SCALE_ONE || s.kind == STEP_SCALE_INV) {
+            cnt += L;
+            continue;
+        }
+        const DevFactor& F = P.f[s.factor];
+        const uint64_t prefixes = L / ((uint64_t)F.len * F.S);
+        const uint64_t per_block = (uint64_t)F.part[s.stage] * F.S * prefixes;  // butterflies per block index
+        for (uint32_t b = 0; b < F.nblocks[s.stage]; b++) {
+            if (s.kind == STEP_FWD) {
+                const bool fuse = s.flags & 2;
+                if (F.radix == 2) {
+                    bool skip = (s.flags & 1) && plan->tw[F.tw_off[s.stage] + b].z == 1;
+                    cnt += per_block * ((skip ? 0 : 1) + (fuse ? 2 : 0));
+                } else {
+                    bool skip = (s.flags & 1) && plan->tw[F.tw_off[s.stage] + 6 * b].z == 1;
+                    cnt += per_block * ((skip ? 4 : 6) + (fuse ? 3 : 0));
+                }
+            } else {
+                const bool fuse = s.flags & 1;
+                if (F.radix == 2) {
+                    bool skip = !fuse && plan->tw[F.itw_off[s.stage] + b].z == 1;
+                    cnt += per_block * (fuse ? 2 : (skip ? 0 : 1));
+                } else {
+                    bool skip = !fuse && plan->tw[F.itw_off[s.stage] + 2 * b].z == 1;
+                    cnt += per_block * (4 + (fuse ? 3 : (skip ? 0 : 2)));
+                }
+            }
+        }
+    }
+    *out = cnt;
     CSD_CATCH
 }
 
@@ -498,6 +592,380 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt(NttArgs a, ModTable mt) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// register-blocked butterflies (the reference's butterflies, hal/ntt.rs:335-350, 378-407,
+// 450-476, 494-531, with the per-stage decisions read from DevPlan::info)
+
+__device__ __forceinline__ void bf2_fwd(int64_t& x, int64_t& y, CsdBarrett t, uint8_t info, CsdBarrett one, int64_t n) {
+    const bool skip = (info & INFO_SKIP) && t.z == 1;
+    const int64_t ty = skip ? y : csd_barrett(t, y, n);
+    int64_t o0 = (int64_t)((uint64_t)x + (uint64_t)ty);
+    int64_t o1 = (int64_t)((uint64_t)x - (uint64_t)ty);
+    if (info & INFO_FUSE) {
+        o0 = csd_barrett(one, o0, n);
+        o1 = csd_barrett(one, o1, n);
+    }
+    x = o0;
+    y = o1;
+}
+
+__device__ __forceinline__ void bf2_inv(int64_t& x, int64_t& y, CsdBarrett t, uint8_t info, CsdBarrett inv_len, int64_t n) {
+    const bool fuse = info & INFO_FUSE;
+    const CsdBarrett ym = fuse ? inv_len : t;
+    const bool skip = !fuse && ym.z == 1;
+    const int64_t sum = (int64_t)((uint64_t)x + (uint64_t)y);
+    const int64_t dif = (int64_t)((uint64_t)x - (uint64_t)y);
+    x = fuse ? csd_barrett(ym, sum, n) : sum;
+    y = skip ? dif : csd_barrett(ym, dif, n);
+}
+
+__device__ __forceinline__ void bf3_fwd(int64_t& x, int64_t& y, int64_t& z, const CsdBarrett* w6, uint8_t info, CsdBarrett one,
+                                        int64_t n) {
+    const bool skip = (info & INFO_SKIP) && w6[0].z == 1;
+    const int64_t tb = skip ? y : csd_barrett(w6[0], y, n);
+    const int64_t tbz = csd_barrett(w6[1], y, n);
+    const int64_t tbzz = csd_barrett(w6[2], y, n);
+    const int64_t tc = skip ? z : csd_barrett(w6[3], z, n);
+    const int64_t tcz = csd_barrett(w6[4], z, n);
+    const int64_t tczz = csd_barrett(w6[5], z, n);
+    int64_t o0 = (int64_t)((uint64_t)x + (uint64_t)tb + (uint64_t)tc);
+    int64_t o1 = (int64_t)((uint64_t)x + (uint64_t)tbz + (uint64_t)tczz);
+    int64_t o2 = (int64_t)((uint64_t)x + (uint64_t)tbzz + (uint64_t)tcz);
+    if (info & INFO_FUSE) {
+        o0 = csd_barrett(one, o0, n);
+        o1 = csd_barrett(one, o1, n);
+        o2 = csd_barrett(one, o2, n);
+    }
+    x = o0;
+    y = o1;
+    z = o2;
+}
+
+__device__ __forceinline__ void bf3_inv(int64_t& x, int64_t& y, int64_t& z, const CsdBarrett* w2, uint8_t info,
+                                        CsdBarrett inv_len, CsdBarrett zeta, CsdBarrett zeta2, int64_t n) {
+    const bool fuse = info & INFO_FUSE;
+    const CsdBarrett ym = fuse ? inv_len : w2[0];
+    const CsdBarrett zm = fuse ? inv_len : w2[1];
+    const bool skip = !fuse && ym.z == 1;
+    const int64_t y1z = csd_barrett(zeta, y, n);
+    const int64_t y1zz = csd_barrett(zeta2, y, n);
+    const int64_t y2z = csd_barrett(zeta, z, n);
+    const int64_t y2zz = csd_barrett(zeta2, z, n);
+    const int64_t sum = (int64_t)((uint64_t)x + (uint64_t)y + (uint64_t)z);
+    const int64_t bb = (int64_t)((uint64_t)x + (uint64_t)y1zz + (uint64_t)y2z);
+    const int64_t cc = (int64_t)((uint64_t)x + (uint64_t)y1z + (uint64_t)y2zz);
+    x = fuse ? csd_barrett(ym, sum, n) : sum;
+    y = skip ? bb : csd_barrett(ym, bb, n);
+    z = skip ? cc : csd_barrett(zm, cc, n);
+}
+
+template <int R>
+struct Pow {
+    static constexpr int of(int e) { return e == 0 ? 1 : R * of(e - 1); }
+};
+
+// One level (stage s = s0 + i) of a register block of M = R^G elements: members of a butterfly are
+// D = R^(G-1-i) apart, and sub-block `sb` of the block maps to stage-s block (blk0 R^i + sb) mod R^s.
+template <int R, int G, int I, bool INV>
+__device__ __forceinline__ void block_level(int64_t (&v)[Pow<R>::of(G)], const DevPlan* __restrict__ P,
+                                            const CsdBarrett* __restrict__ TW, const DevFactor& F, int dir, int s0,
+                                            uint32_t blk0) {
+    constexpr int M = Pow<R>::of(G);
+    constexpr int D = Pow<R>::of(G - 1 - I);
+    constexpr int NSB = Pow<R>::of(I);
+    const int s = s0 + I;
+    const uint8_t info = P->info[dir][F.index][s];
+    const int64_t n = P->n;
+    if (info & INFO_REDUCE) {
+#pragma unroll
+        for (int m = 0; m < M; m++) v[m] = csd_barrett(P->one, v[m], n);
+    }
+    const uint32_t nbk = F.nblocks[s];
+#pragma unroll
+    for (int sb = 0; sb < NSB; sb++) {
+        uint32_t g = blk0 * (uint32_t)NSB + (uint32_t)sb;
+        uint32_t blk = R == 2 ? (g & (nbk - 1)) : (g - fdiv(g, F.magic_nb[s]) * nbk);
+#pragma unroll
+        for (int jj = 0; jj < D; jj++) {
+            const int m0 = sb * D * R + jj;
+            if constexpr (R == 2) {
+                if constexpr (!INV)
+                    bf2_fwd(v[m0], v[m0 + D], TW[F.tw_off[s] + blk], info, P->one, n);
+                else
+                    bf2_inv(v[m0], v[m0 + D], TW[F.itw_off[s] + blk], info, P->inv_len, n);
+            } else {
+                if constexpr (!INV)
+                    bf3_fwd(v[m0], v[m0 + D], v[m0 + 2 * D], TW + F.tw_off[s] + 6 * blk, info, P->one, n);
+                else
+                    bf3_inv(v[m0], v[m0 + D], v[m0 + 2 * D], TW + F.itw_off[s] + 2 * blk, info, P->inv_len, F.zeta, F.zeta2, n);
+            }
+        }
+    }
+}
+
+template <int R, int G, bool INV>
+__device__ __forceinline__ void block_stages(int64_t (&v)[Pow<R>::of(G)], const DevPlan* P, const CsdBarrett* TW,
+                                             const DevFactor& F, int dir, int s0, uint32_t blk0) {
+    if constexpr (!INV) {
+        block_level<R, G, 0, false>(v, P, TW, F, dir, s0, blk0);
+        if constexpr (G > 1) block_level<R, G, 1, false>(v, P, TW, F, dir, s0, blk0);
+        if constexpr (G > 2) block_level<R, G, 2, false>(v, P, TW, F, dir, s0, blk0);
+    } else {
+        if constexpr (G > 2) block_level<R, G, 2, true>(v, P, TW, F, dir, s0, blk0);
+        if constexpr (G > 1) block_level<R, G, 1, true>(v, P, TW, F, dir, s0, blk0);
+        block_level<R, G, 0, true>(v, P, TW, F, dir, s0, blk0);
+    }
+}
+
+// Runs one pass over all groups of `rows` LDS-resident lines of length L.
+template <int R, int G, bool INV>
+__device__ __forceinline__ void run_pass(int64_t* __restrict__ lds, int L, int rows, uint32_t magic_groups,
+                                         const DevPlan* __restrict__ P, const CsdBarrett* __restrict__ TW, const DevPass ps,
+                                         int dir) {
+    constexpr int M = Pow<R>::of(G);
+    const DevFactor& F = P->f[ps.factor];
+    const uint32_t S = F.S, mS = F.magic_S;
+    const uint32_t SL = F.stage_len[ps.s0];
+    const uint32_t sub = SL / M;
+    const uint32_t msub = sub <= 1 ? 0u : (uint32_t)((((uint64_t)1) << 32) / sub + 1);
+    const uint32_t ngroups = (uint32_t)L / M;
+    const uint32_t total = ngroups * (uint32_t)rows;
+    for (uint32_t w = threadIdx.x; w < total; w += blockDim.x) {
+        const uint32_t row = rows == 1 ? 0u : fdiv(w, magic_groups);
+        const uint32_t gam = w - row * ngroups;
+        const uint32_t t = fdiv(gam, mS);
+        const uint32_t suffix = gam - t * S;
+        const uint32_t blk0 = fdiv(t, msub);
+        const uint32_t jj = t - blk0 * sub;
+        const uint32_t base = (blk0 * SL + jj) * S + suffix;
+        const uint32_t stride = sub * S;
+        int64_t* line = lds + (size_t)row * L;
+        int64_t v[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) v[m] = line[base + m * stride];
+        block_stages<R, G, INV>(v, P, TW, F, dir, ps.s0, blk0);
+#pragma unroll
+        for (int m = 0; m < M; m++) line[base + m * stride] = v[m];
+    }
+}
+
+struct RowArgs {
+    int64_t d0, d1, d2, d3;
+    int64_t st[3];
+    int64_t* data;
+    const DevPlan* plans[CSD_MAX_MODULI];
+    int dir;
+    int rows;           // lines per workgroup
+    uint32_t magic_L;   // for g / L with g < rows * L
+    uint32_t magic_groups[5];  // for w / (L / M), M = 2, 4, 8, 3, 9 (pass kinds (2,1) (2,2) (2,3) (3,1) (3,2))
+    int64_t chunks;     // workgroups per (i, j)
+    const int64_t* mul;
+    Str4 smul;
+};
+
+// dim-3 transforms (contiguous lines): lines staged in LDS, stages run in register blocks
+template <bool INV>
+__global__ __launch_bounds__(256) void k_ntt_rows(RowArgs a, ModTable mt) {
+    extern __shared__ int64_t lds[];
+    const int64_t chunk = blockIdx.x % a.chunks;
+    const int64_t ij = blockIdx.x / a.chunks;
+    const int64_t j = ij % a.d1;
+    const int64_t i = ij / a.d1;
+    const DevPlan* __restrict__ P = a.plans[j];
+    const CsdBarrett* __restrict__ TW = (const CsdBarrett*)(P + 1);
+    const int L = P->len;
+    const int64_t k0 = chunk * a.rows;
+    const int rows = (int)min((int64_t)a.rows, a.d2 - k0);
+    int64_t* base = a.data + i * a.st[0] + j * a.st[1] + k0 * a.st[2];
+    const int total = rows * L;
+    for (int g = threadIdx.x; g < total; g += blockDim.x) {
+        const uint32_t r = a.rows == 1 ? 0u : fdiv((uint32_t)g, a.magic_L);
+        const int p = g - (int)r * L;
+        lds[g] = base[r * a.st[2] + p];
+    }
+    __syncthreads();
+    const int dir = a.dir;
+    if (P->nfactors == 0) {
+        if (dir != 1) {
+            const CsdBarrett m = dir == 2 ? P->inv_len : P->one;
+            for (int g = threadIdx.x; g < total; g += blockDim.x) lds[g] = csd_barrett(m, lds[g], P->n);
+        }
+    } else {
+        const int np = P->npasses[dir];
+        for (int q = 0; q < np; q++) {
+            const DevPass ps = P->passes[dir][q];
+            if (ps.radix == 2) {
+                if (ps.g == 3)
+                    run_pass<2, 3, INV>(lds, L, rows, a.magic_groups[2], P, TW, ps, dir);
+                else if (ps.g == 2)
+                    run_pass<2, 2, INV>(lds, L, rows, a.magic_groups[1], P, TW, ps, dir);
+                else
+                    run_pass<2, 1, INV>(lds, L, rows, a.magic_groups[0], P, TW, ps, dir);
+            } else {
+                if (ps.g == 2)
+                    run_pass<3, 2, INV>(lds, L, rows, a.magic_groups[4], P, TW, ps, dir);
+                else
+                    run_pass<3, 1, INV>(lds, L, rows, a.magic_groups[3], P, TW, ps, dir);
+            }
+            __syncthreads();
+        }
+    }
+    const CsdMod m = mt.m[j];
+    for (int g = threadIdx.x; g < total; g += blockDim.x) {
+        const uint32_t r = a.rows == 1 ? 0u : fdiv((uint32_t)g, a.magic_L);
+        const int p = g - (int)r * L;
+        int64_t x = lds[g];
+        if (a.mul)
+            x = csd_mont_mul(m, x, a.mul[i * a.smul.s[0] + j * a.smul.s[1] + (k0 + r) * a.smul.s[2] + p * a.smul.s[3]]);
+        base[r * a.st[2] + p] = x;
+    }
+}
+
+// dim-2 transforms of short length: one thread owns a whole column (F0 * F1 values) in registers.
+// F0, F1 are the factor lengths (F1 = 1 for single-factor plans, F0 = F1 = 1 for the identity).
+template <int F>
+struct ColFactorShape {
+    static constexpr int R = (F & (F - 1)) == 0 ? 2 : 3;
+    static constexpr int NST = R == 2 ? (F == 2 ? 1 : F == 4 ? 2 : F == 8 ? 3 : F == 16 ? 4 : 0) : (F == 3 ? 1 : F == 9 ? 2 : 0);
+};
+
+// stage s of factor f (length F, suffix S, PRE prefixes) on a register-resident column
+template <int F, int S, int PRE, bool INV, int s>
+__device__ __forceinline__ void col_stage(int64_t (&v)[F * S * PRE], const DevPlan* __restrict__ P,
+                                          const CsdBarrett* __restrict__ TW, const DevFactor& Fd, int f, int dir) {
+    constexpr int R = ColFactorShape<F>::R;
+    constexpr int SL = F / Pow<R>::of(s);
+    constexpr int part = SL / R;
+    constexpr int d = part * S;
+    const int64_t n = P->n;
+    const uint8_t info = P->info[dir][f][s];
+    if (info & INFO_REDUCE) {
+#pragma unroll
+        for (int e = 0; e < F * S * PRE; e++) v[e] = csd_barrett(P->one, v[e], n);
+    }
+#pragma unroll
+    for (int b = 0; b < F / SL; b++) {
+        CsdBarrett t2;
+        const CsdBarrett* t3;
+        if constexpr (R == 2)
+            t2 = TW[(INV ? Fd.itw_off[s] : Fd.tw_off[s]) + b];
+        else
+            t3 = TW + (INV ? Fd.itw_off[s] + 2 * b : Fd.tw_off[s] + 6 * b);
+#pragma unroll
+        for (int pre = 0; pre < PRE; pre++)
+#pragma unroll
+            for (int jj = 0; jj < part; jj++)
+#pragma unroll
+                for (int x = 0; x < S; x++) {
+                    constexpr int stride_pre = F * S;
+                    const int e0 = pre * stride_pre + (b * SL + jj) * S + x;
+                    if constexpr (R == 2) {
+                        if constexpr (!INV)
+                            bf2_fwd(v[e0], v[e0 + d], t2, info, P->one, n);
+                        else
+                            bf2_inv(v[e0], v[e0 + d], t2, info, P->inv_len, n);
+                    } else {
+                        if constexpr (!INV)
+                            bf3_fwd(v[e0], v[e0 + d], v[e0 + 2 * d], t3, info, P->one, n);
+                        else
+                            bf3_inv(v[e0], v[e0 + d], v[e0 + 2 * d], t3, info, P->inv_len, Fd.zeta, Fd.zeta2, n);
+                    }
+                }
+    }
+}
+
+template <int F, int S, int PRE, bool INV, int SI>
+__device__ __forceinline__ void col_stages(int64_t (&v)[F * S * PRE], const DevPlan* P, const CsdBarrett* TW, const DevFactor& Fd,
+                                           int f, int dir) {
+    constexpr int NST = ColFactorShape<F>::NST;
+    if constexpr (SI < NST) {
+        col_stage<F, S, PRE, INV, INV ? NST - 1 - SI : SI>(v, P, TW, Fd, f, dir);
+        col_stages<F, S, PRE, INV, SI + 1>(v, P, TW, Fd, f, dir);
+    }
+}
+
+template <int F, int S, int PRE, bool INV>
+__device__ __forceinline__ void col_factor(int64_t (&v)[F * S * PRE], const DevPlan* __restrict__ P,
+                                           const CsdBarrett* __restrict__ TW, int f, int dir) {
+    static_assert(ColFactorShape<F>::NST > 0, "unsupported column factor");
+    col_stages<F, S, PRE, INV, 0>(v, P, TW, P->f[f], f, dir);
+}
+
+struct ColArgs {
+    int64_t d0, d1, d3;
+    int64_t st[3];
+    int64_t* data;
+    const DevPlan* plans[CSD_MAX_MODULI];
+    int dir;
+    int64_t chunks;
+    const int64_t* mul;
+    Str4 smul;
+};
+
+template <int F0, int F1, bool INV>
+__global__ __launch_bounds__(256) void k_ntt_cols(ColArgs a, ModTable mt) {
+    constexpr int L = F0 * F1;
+    const int64_t chunk = blockIdx.x % a.chunks;
+    const int64_t ij = blockIdx.x / a.chunks;
+    const int64_t j = ij % a.d1;
+    const int64_t i = ij / a.d1;
+    const int64_t l = chunk * 256 + threadIdx.x;
+    if (l >= a.d3) return;
+    const DevPlan* __restrict__ P = a.plans[j];
+    const CsdBarrett* __restrict__ TW = (const CsdBarrett*)(P + 1);
+    int64_t* base = a.data + i * a.st[0] + j * a.st[1] + l;
+    int64_t v[L];
+#pragma unroll
+    for (int k = 0; k < L; k++) v[k] = base[k * a.st[2]];
+    const int dir = a.dir;
+    if constexpr (L == 1) {
+        if (dir != 1) v[0] = csd_barrett(dir == 2 ? P->inv_len : P->one, v[0], P->n);
+    } else {
+        // factor 0 acts on the most significant digit (suffix F1), then factor 1 (suffix 1)
+        col_factor<F0, F1, 1, INV>(v, P, TW, 0, dir);
+        if constexpr (F1 > 1) col_factor<F1, 1, F0, INV>(v, P, TW, 1, dir);
+    }
+    const CsdMod m = mt.m[j];
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+        int64_t x = v[k];
+        if (a.mul) x = csd_mont_mul(m, x, a.mul[i * a.smul.s[0] + j * a.smul.s[1] + k * a.smul.s[2] + l * a.smul.s[3]]);
+        base[k * a.st[2]] = x;
+    }
+}
+
+typedef void (*ColKernel)(ColArgs, ModTable);
+
+template <int F0, int F1>
+static ColKernel col_kernel_for(bool inv) {
+    return inv ? (ColKernel)k_ntt_cols<F0, F1, true> : (ColKernel)k_ntt_cols<F0, F1, false>;
+}
+
+// the column kernel for a plan's factor list, or null if it has no register-resident variant
+static ColKernel pick_col_kernel(const DevPlan& P, bool inv) {
+    int f0 = P.nfactors > 0 ? P.f[0].len : 1, f1 = P.nfactors > 1 ? P.f[1].len : 1;
+    if (P.nfactors > 2) return nullptr;
+    if (f0 == 4 && f1 == 3) return col_kernel_for<4, 3>(inv);
+    if (f0 == 1 && f1 == 1) return col_kernel_for<1, 1>(inv);
+    if (f1 != 1) return nullptr;
+    switch (f0) {
+        case 2: return col_kernel_for<2, 1>(inv);
+        case 3: return col_kernel_for<3, 1>(inv);
+        case 4: return col_kernel_for<4, 1>(inv);
+        case 8: return col_kernel_for<8, 1>(inv);
+        case 9: return col_kernel_for<9, 1>(inv);
+        case 16: return col_kernel_for<16, 1>(inv);
+        default: return nullptr;
+    }
+}
+
+static bool g_use_generic_ntt = false;  // forces the one-stage-per-LDS-pass kernel (tests)
+
+extern "C" int csd_ntt_force_generic(int on) {
+    g_use_generic_ntt = on != 0;
+    return 0;
+}
+
 static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim, int dir,
                        const csd_view* mul, const csd_modulus* mods, size_t nmods) {
     check_view(dst, "ntt");
@@ -529,7 +997,77 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         t = make_mod_table(mods, nmods, a.d1);
     }
     if (numel(dst) == 0) return;
-    // tile: as many transforms as fit 32 KiB of LDS (at least one)
+    static const char* names[2][3] = {{"ntt_dim2", "ntt_mul_dim2", "intt_dim2"}, {"ntt_dim3", "ntt_mul_dim3", "intt_dim3"}};
+    const bool inv = dir == 2;
+    if (!g_use_generic_ntt && dim == 2) {
+        // register-resident columns, if every plan has the same supported factor list
+        ColKernel k = pick_col_kernel(plans[0]->host, inv);
+        for (size_t jj = 1; jj < nplans && k; jj++) {
+            const DevPlan& p0 = plans[0]->host;
+            const DevPlan& pj = plans[jj]->host;
+            if (pj.nfactors != p0.nfactors) k = nullptr;
+            for (int f = 0; k && f < p0.nfactors; f++)
+                if (pj.f[f].len != p0.f[f].len) k = nullptr;
+        }
+        if (k) {
+            ColArgs c;
+            memset(&c, 0, sizeof(c));
+            c.d0 = a.d0;
+            c.d1 = a.d1;
+            c.d3 = a.d3;
+            for (int d = 0; d < 3; d++) c.st[d] = a.st[d];
+            c.data = a.data;
+            for (size_t jj = 0; jj < nplans; jj++) c.plans[jj] = a.plans[jj];
+            c.dir = dir;
+            c.chunks = (a.d3 + 255) / 256;
+            c.mul = a.mul;
+            c.smul = a.smul;
+            int64_t blocks = a.d0 * a.d1 * c.chunks;
+            if (blocks > (int64_t)INT32_MAX) throw CsdError("ntt: too many tiles");
+            TimedScope ts(ctx, names[0][dir], 16.0 * (double)numel(dst));
+            hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, c, t);
+            HIPCHECK(hipGetLastError());
+            return;
+        }
+    }
+    if (!g_use_generic_ntt && dim == 3 && L <= 16384) {
+        RowArgs r;
+        memset(&r, 0, sizeof(r));
+        r.d0 = a.d0;
+        r.d1 = a.d1;
+        r.d2 = a.d2;
+        r.d3 = a.d3;
+        for (int d = 0; d < 3; d++) r.st[d] = a.st[d];
+        r.data = a.data;
+        for (size_t jj = 0; jj < nplans; jj++) r.plans[jj] = a.plans[jj];
+        r.dir = dir;
+        int rows = 1;
+        while (rows * 2 * L <= 6144 && rows * 2 <= a.d2 && rows < 256) rows *= 2;
+        r.rows = rows;
+        r.magic_L = magic_for((uint32_t)L);
+        const int Ms[5] = {2, 4, 8, 3, 9};
+        for (int q = 0; q < 5; q++) r.magic_groups[q] = L % Ms[q] == 0 ? magic_for((uint32_t)(L / Ms[q])) : 0u;
+        r.chunks = (a.d2 + rows - 1) / rows;
+        r.mul = a.mul;
+        r.smul = a.smul;
+        int64_t blocks = a.d0 * a.d1 * r.chunks;
+        if (blocks > (int64_t)INT32_MAX) throw CsdError("ntt: too many tiles");
+        size_t lds = (size_t)rows * L * 8;
+        static bool row_attr = false;
+        if (!row_attr) {
+            HIPCHECK(hipFuncSetAttribute((const void*)k_ntt_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCHECK(hipFuncSetAttribute((const void*)k_ntt_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            row_attr = true;
+        }
+        TimedScope ts(ctx, names[1][dir], 16.0 * (double)numel(dst));
+        if (inv)
+            hipLaunchKernelGGL(k_ntt_rows<true>, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, r, t);
+        else
+            hipLaunchKernelGGL(k_ntt_rows<false>, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, r, t);
+        HIPCHECK(hipGetLastError());
+        return;
+    }
+    // generic form: one stage per LDS pass; a tile holds as many transforms as fit 32 KiB
     int V = 1;
     const int64_t other = dim == 2 ? a.d3 : a.d2;
     while (V * 2 * L <= 4096 && V * 2 <= 256 && V < other) V *= 2;
@@ -546,8 +1084,9 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         HIPCHECK(hipFuncSetAttribute((const void*)k_ntt, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
-    static const char* names[2][3] = {{"ntt_dim2", "ntt_mul_dim2", "intt_dim2"}, {"ntt_dim3", "ntt_mul_dim3", "intt_dim3"}};
-    TimedScope ts(ctx, names[dim - 2][dir], 16.0 * (double)numel(dst));
+    static const char* gnames[2][3] = {{"generic_ntt_dim2", "generic_ntt_mul_dim2", "generic_intt_dim2"},
+                                       {"generic_ntt_dim3", "generic_ntt_mul_dim3", "generic_intt_dim3"}};
+    TimedScope ts(ctx, gnames[dim - 2][dir], 16.0 * (double)numel(dst));
     hipLaunchKernelGGL(k_ntt, dim3((unsigned)blocks), dim3(NTT_THREADS), lds, ctx->stream, a, t);
     HIPCHECK(hipGetLastError());
 }

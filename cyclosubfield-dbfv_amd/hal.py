@@ -137,6 +137,13 @@ class HipNTTPlan:
     def __init__(self, backend, handle, length):
         self.backend, self.handle, self.len = backend, handle, length
 
+    def mults(self, direction):
+        """Modular multiplications of one transform: direction 'fwd', 'fwd_unreduced' or 'inv'."""
+        out = ctypes.c_uint64()
+        d = {"fwd": 0, "fwd_unreduced": 1, "inv": 2}[direction]
+        check(lib().csd_ntt_plan_mults(ctypes.c_void_p(self.handle), ctypes.c_int(d), ctypes.byref(out)))
+        return out.value
+
     def __del__(self):
         if self.handle is not None and _lib is not None and self.backend.ctx is not None:
             _lib.csd_ntt_plan_free(ctypes.c_void_p(self.handle))
@@ -197,6 +204,13 @@ class HipBackend:
     def set_timing(self, on):
         """Records HIP events around the library's timed launches on the context stream."""
         check(lib().csd_ctx_set_timing(self.ctx, ctypes.c_int(1 if on else 0)))
+
+    def calibrate(self, kind, n):
+        """kind 'copy': device copy of n bytes -> GB/s; kind 'barrett': n elements x 64 dependent
+        Barrett multiplications -> G multiplications/s."""
+        out = ctypes.c_double()
+        check(lib().csd_calibrate(self.ctx, ctypes.c_int({"copy": 0, "barrett": 1}[kind]), ctypes.c_size_t(n), ctypes.byref(out)))
+        return out.value
 
     def timing_report(self):
         """{kernel name: (launches, total ms, total algorithmic bytes)}; resets the record."""

@@ -62,6 +62,10 @@ int csd_ctx_stream(csd_ctx* ctx, void** stream);
  * line per kernel: "<name> <launches> <total ms> <total algorithmic bytes>", and resets. */
 int csd_ctx_set_timing(csd_ctx* ctx, int on);
 int csd_ctx_timing_report(csd_ctx* ctx, char* buf, size_t cap);
+/* Calibration of the two ceilings the kernels are measured against (no reference counterpart):
+ * kind 0: device-to-device copy of n bytes -> achieved GB/s (read + write);
+ * kind 1: n independent chains of 64 Barrett multiplications -> G multiplications/s. */
+int csd_calibrate(csd_ctx* ctx, int kind, size_t n, double* out);
 
 /* ---- allocation and transfer --------------------------------------------------------------- */
 /* hal_allocate_i64 / hal_allocate_i128 (hal/mod.rs:119-122): zero-initialised, row-major;
@@ -104,12 +108,19 @@ int csd_hal_i64_to_mont64(csd_ctx* ctx, const csd_view* dst, const csd_modulus* 
 /* hal_create_ntt_plan (mod.rs:209, ntt.rs:561-566): group NTT over C_n0 x ... x C_n(r-1). */
 int csd_hal_create_ntt_plan(csd_ctx* ctx, const csd_modulus* modulus, const size_t* inner_shape, size_t n, csd_ntt_plan** out);
 int csd_ntt_plan_free(csd_ntt_plan* plan);
+/* modular multiplications one transform of the plan performs (dir 0: forward reduced, 1: forward
+ * unreduced, 2: inverse), following its reduction schedule and twiddle-1 skips */
+int csd_ntt_plan_mults(const csd_ntt_plan* plan, int dir, uint64_t* out);
 /* hal_ntt_mont64_dim / hal_intt_mont64_dim / hal_ntt_mul_mont64_dim (mod.rs:216-238);
  * plans[j] is the plan for dimension-1 index j; dim is CSD_DIM2 or CSD_DIM3. */
 int csd_hal_ntt_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim);
 int csd_hal_intt_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim);
 int csd_hal_ntt_mul_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim,
                                const csd_view* mul, const csd_modulus* mods, size_t nmods);
+/* Kernel selection for the NTT entry points (no reference counterpart): 0 (default) uses the
+ * register-blocked kernels where a plan has one, 1 forces the one-stage-per-LDS-pass kernel for
+ * every plan. Both produce identical results; the switch exists to test and time each form. */
+int csd_ntt_force_generic(int on);
 
 /* ---- base-conversion corrections -------------------------------------------------------------- */
 int csd_hal_bconv_correct_mont64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs,

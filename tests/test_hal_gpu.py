@@ -349,3 +349,28 @@ def test_errors_are_reported(gpu):
     plan = gpu.hal_create_ntt_plan(gpu.hal_create_modulus(Q57), [4])
     with pytest.raises(pkg.HalError):
         gpu.hal_ntt_mont64_dim(t, [plan, plan], 2)
+
+
+@pytest.mark.parametrize("q,inner", [(P36864[0], [2, 3, 8, 64]), (P36864[1], [4, 3]), (Q62_RADIX2, [16, 2]), (Q61_RADIX3, [81])])
+def test_generic_ntt_kernel_parity(gpu, cpu, q, inner):
+    """The one-stage-per-pass LDS kernel (the fallback for plans without a register-blocked
+    variant) stays bit-exact when forced for every plan."""
+    pkg = load()
+    n = int(np.prod(inner))
+    mc, mg = cpu.hal_create_modulus(q), gpu.hal_create_modulus(q)
+    pc, pg = cpu.hal_create_ntt_plan(mc, inner), gpu.hal_create_ntt_plan(mg, inner)
+    rng = np.random.default_rng(31)
+    pkg.lib().csd_ntt_force_generic(1)
+    try:
+        for dim in (2, 3):
+            shape = (2, 1, n, 9) if dim == 2 else (2, 1, 3, n)
+            data = rand_i64(rng, shape, -2 * q, 2 * q)
+            kern = rand_i64(rng, (1, 1, n, 1) if dim == 2 else (1, 1, 1, n), -q, q)
+            dc, dg = data.copy(), to_dev(gpu, data)
+            cpu.hal_ntt_mul_mont64_dim(dc, [pc], dim, kern, [mc])
+            gpu.hal_ntt_mul_mont64_dim(dg, [pg], dim, to_dev(gpu, kern), [mg])
+            cpu.hal_intt_mont64_dim(dc, [pc], dim)
+            gpu.hal_intt_mont64_dim(dg, [pg], dim)
+            assert same(from_dev(gpu, dg), dc)
+    finally:
+        pkg.lib().csd_ntt_force_generic(0)
