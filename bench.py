@@ -88,6 +88,72 @@ def cpu_baseline(primes_base, host_data, sample):
             "sample": f"{sample} ring elements of the same batch, pb->CRT + CRT->pb ({2 * sample} transforms, {dt:.1f} s)"}
 
 
+def bench_galois(pkg, be, steps, batch=32, seed=SEED, cpu=True):
+    """Bootstrapping's Galois key switching at field_49152 (m = 35297665, n = 49152, 23-prime base,
+    23 gadget digits): the full homcrt rotation set (40 rotations, keys resident in HBM, 16.6 GB) on
+    a batch of `batch` ciphertexts with hoisted digits; one step = every rotation of every ciphertext."""
+    field, log2_q, t_bits = "field_49152", (1280, 1300), 76
+    ring = pkg.ring.RingContext(be, field, log2_q, t_bits)
+    base = ring.initial_rns_base()
+    r = len(base)
+    d = r
+    gs, _ = pkg.ring.homcrt_galois_elements(field)
+    n2, n3 = ring.info["deg_lprime"], ring.info["deg_k"]
+    parts = be.hal_allocate_i64((batch * d, r, n2, n3))
+    c0 = be.hal_allocate_i64((batch, r, n2, n3))
+    keys = [be.hal_allocate_i64((2 * d, r, n2, n3)) for _ in gs]
+    out = be.hal_allocate_i64((len(gs) * batch * 2, r, n2, n3))
+    ring.fill_uniform(parts, base, seed + 1)
+    ring.fill_uniform(c0, base, seed + 2)
+    for t, k in enumerate(keys):
+        ring.fill_uniform(k, base, seed + 100 + t)
+    stream = be.stream()
+    ev = HipEvents()
+    ring.galois_keyswitch(base, batch, d, parts, c0, gs, keys, out)  # warm-up
+    be.sync()
+    be.set_timing(True)
+    ev.start(stream)
+    for _ in range(steps):
+        ring.galois_keyswitch(base, batch, d, parts, c0, gs, keys, out)
+    ms = ev.stop(stream)
+    be.set_timing(False)
+    rep = be.timing_report()["galois_keyswitch"]
+    avg_ms = rep[1] / rep[0]
+    bytes_per_launch = rep[2] / rep[0]
+    res = {
+        "workload": f"field_49152 Galois key switching: {len(gs)} homcrt rotations x {batch} ciphertexts, r = {r} primes, "
+                    f"{d} gadget digits, keys {len(gs) * 2 * d * r * n2 * n3 * 8 / 2**30:.1f} GiB resident in HBM",
+        "value": round(len(gs) * batch * steps / (ms / 1000.0), 1), "unit": "key switches/s",
+        "ms_per_step": round(ms / steps, 3),
+        "kernel": {"name": "galois_keyswitch", "avg_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch,
+                   "GB/s": round(bytes_per_launch / (avg_ms / 1000.0) / 1e9, 1),
+                   "frac": round(bytes_per_launch / (avg_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    if cpu:
+        # oracle: the unfused reference sequence for one (ciphertext, rotation) on the same data,
+        # which doubles as a full-size spot check of the device result
+        from oracle import cpu_backend
+        from oracle import ring as oring
+
+        cb = cpu_backend.CpuBackend()
+        oc = oring.RingContext(cb, oring.FieldDescriptor(field), log2_q, t_bits)
+        hp = be.hal_export(parts[0:d]).reshape(d, r, n2, n3)
+        hc = np.zeros((2, r, n2, n3), dtype=np.int64)
+        hc[0] = be.hal_export(c0[0:1]).reshape(r, n2, n3)
+        hk = be.hal_export(keys[0]).reshape(2 * d, r, n2, n3)
+        t0 = time.perf_counter()
+        exp = oring.hom_gal_one(oc, hc, base, gs[0], hp, [hk[2 * i:2 * i + 2] for i in range(d)], [(i, i + 1) for i in range(d)])
+        dt = time.perf_counter() - t0
+        got = be.hal_export(out[0:2]).reshape(2, r, n2, n3)
+        if not (got == exp).all():
+            raise SystemExit("bench: galois key switch differs from the oracle")
+        res["cpu_baseline"] = {"value": round(1.0 / dt, 3), "unit": "key switches/s", "cores": 1, "kind": "port",
+                               "sample": f"1 ciphertext x 1 rotation, same data, in RAM ({dt:.2f} s); also checked bit-exact"}
+    del parts, c0, keys, out
+    ring.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +162,8 @@ def main():
     ap.add_argument("--count", type=int, default=128, help="ring elements in the batch (2 per ciphertext)")
     ap.add_argument("--cpu-sample", type=int, default=4, help="ring elements in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workloads", default="galois",
+                    help="comma-separated extra workloads beside the headline transforms: galois")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -198,6 +266,12 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(primes, host, min(args.cpu_sample, args.count))
     ring.close()
+    del data
+    workloads = {}
+    if "galois" in args.workloads:
+        workloads["galois_keyswitch"] = bench_galois(pkg, be, max(1, args.steps // 5), cpu=not args.no_cpu)
+    if workloads:
+        result["workloads"] = workloads
     be.close()
     print(json.dumps(result))
 

@@ -1,0 +1,169 @@
+// Fused and batched forms of the ring-level hot path (beyond one-kernel-per-HAL-call).
+//
+//  * Galois key switching for a batch of ciphertexts and a set of rotations
+//    (bfv_hom_gal, cyclosubfield-dbfv/src/bfv.rs:783-841): one kernel per rotation computes,
+//    for every ciphertext of the batch, sigma_g of the hoisted gadget digits (a gather through
+//    the automorphism's permutation, ring.rs:1183-1222), the inner product with the rotation's
+//    key digits (hal/cpu.rs:706-788), the added sigma_g(c0) * mont(1) (ring.rs:496-500) and the
+//    wide reduction (ring.rs:846-865). Keys stay resident in HBM and each key digit is read once
+//    per group of ciphertexts instead of once per ciphertext. The i128 sums wrap exactly like the
+//    reference's, so outputs are bit-identical to the unfused HAL sequence.
+//  * Uniform random residues for synthetic inputs and keys (splitmix64 of the element index).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/csdbfv_ring.h"
+#include "csd_internal.hpp"
+
+// ---------------------------------------------------------------------------------------------
+// synthetic residues
+
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct FillArgs {
+    int64_t d0, d1, d2, d3;
+    int64_t st[4];
+    int64_t* data;
+    uint64_t seed;
+};
+
+// data[i, j, k, l] = uniform in [0, p_j): the high word of a 64-bit random times p_j
+__global__ void k_fill_uniform(FillArgs a, ModTable mt) {
+    int64_t total = a.d0 * a.d1 * a.d2 * a.d3;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t l = e % a.d3, t = e / a.d3;
+        int64_t k = t % a.d2;
+        t /= a.d2;
+        int64_t j = t % a.d1, i = t / a.d1;
+        uint64_t r = splitmix64_at(a.seed, (uint64_t)e);
+        a.data[i * a.st[0] + j * a.st[1] + k * a.st[2] + l * a.st[3]] = (int64_t)csd_umulhi(r, (uint64_t)mt.m[j].N);
+    }
+}
+
+extern "C" int csd_fill_uniform(csd_ctx* ctx, const csd_view* dst, const csd_modulus* mods, size_t nmods, uint64_t seed) {
+    CSD_TRY check_view(dst, "fill_uniform");
+    ModTable t = make_mod_table(mods, nmods, dst->shape[1]);
+    FillArgs a;
+    a.d0 = dst->shape[0];
+    a.d1 = dst->shape[1];
+    a.d2 = dst->shape[2];
+    a.d3 = dst->shape[3];
+    for (int d = 0; d < 4; d++) a.st[d] = dst->strides[d];
+    a.data = (int64_t*)dst->ptr;
+    a.seed = seed;
+    int64_t total = numel(dst);
+    if (total == 0) return 0;
+    unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_fill_uniform, dim3(blocks), dim3(256), 0, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+    CSD_CATCH
+}
+
+// ---------------------------------------------------------------------------------------------
+// batched Galois key switching
+
+#define GKS_CB 8  // ciphertexts per thread: each key digit load serves GKS_CB ciphertexts
+
+struct GksArgs {
+    int64_t r, n2, n3;
+    int d, batch;
+    const int64_t* parts;  // [batch, d, r, n2, n3]
+    const int64_t* c0;     // [batch, r, n2, n3]
+    const int64_t* key;    // [d, 2, r, n2, n3]
+    const uint32_t* perm2;
+    const uint32_t* perm3;
+    int64_t* out;  // [batch, 2, r, n2, n3]
+    int full0, full1;
+    int64_t chunks;  // position chunks of 256 along dim 3
+    int64_t one[CSD_MAX_MODULI];  // mont(1) per limb
+};
+
+__device__ __forceinline__ int64_t reduce_wide_entry(const CsdMod& m, i128 x, int full) {
+    // ring.rs:846-865: a single Montgomery reduction, or the double reduction then to_mont64
+    return full ? csd_to_mont64(m, csd_double_red128(m, x)) : csd_red128(m, x);
+}
+
+__global__ __launch_bounds__(256) void k_galois_keyswitch(GksArgs a, ModTable mt) {
+    // block -> (ciphertext group, j, k, l chunk)
+    int64_t b = blockIdx.x;
+    const int64_t c = b % a.chunks;
+    b /= a.chunks;
+    const int64_t k = b % a.n2;
+    b /= a.n2;
+    const int64_t j = b % a.r;
+    const int64_t cg = b / a.r;
+    const int64_t l = c * 256 + threadIdx.x;
+    if (l >= a.n3) return;
+    const int64_t plane = a.n2 * a.n3;
+    const int64_t limb = a.r * plane;
+    const int64_t src_pos = j * plane + (int64_t)a.perm2[k] * a.n3 + a.perm3[l];
+    const int64_t dst_pos = j * plane + k * a.n3 + l;
+    const int c_first = (int)(cg * GKS_CB);
+    const int nc = min(GKS_CB, a.batch - c_first);
+    i128 acc0[GKS_CB], acc1[GKS_CB];
+#pragma unroll
+    for (int q = 0; q < GKS_CB; q++) acc0[q] = acc1[q] = 0;
+    for (int i = 0; i < a.d; i++) {
+        const int64_t k0 = a.key[(int64_t)(2 * i) * limb + dst_pos];
+        const int64_t k1 = a.key[(int64_t)(2 * i + 1) * limb + dst_pos];
+#pragma unroll
+        for (int q = 0; q < GKS_CB; q++) {
+            if (q < nc) {
+                const int64_t p = a.parts[((int64_t)(c_first + q) * a.d + i) * limb + src_pos];
+                acc0[q] = (i128)((u128)acc0[q] + (u128)((i128)p * (i128)k0));
+                acc1[q] = (i128)((u128)acc1[q] + (u128)((i128)p * (i128)k1));
+            }
+        }
+    }
+    const CsdMod m = mt.m[j];
+    const int64_t one = a.one[j];
+#pragma unroll
+    for (int q = 0; q < GKS_CB; q++) {
+        if (q < nc) {
+            const int64_t x0 = a.c0[(int64_t)(c_first + q) * limb + src_pos];
+            const i128 s0 = (i128)((u128)acc0[q] + (u128)((i128)x0 * (i128)one));
+            int64_t* o = a.out + (int64_t)(c_first + q) * 2 * limb + dst_pos;
+            o[0] = reduce_wide_entry(m, s0, a.full0);
+            o[limb] = reduce_wide_entry(m, acc1[q], a.full1);
+        }
+    }
+}
+
+// exported through the ring (which owns the permutations); see csd_ring.hip
+void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t r, int64_t n2, int64_t n3, const int64_t* parts,
+                                 const int64_t* c0, const int64_t* key, const uint32_t* perm2, const uint32_t* perm3,
+                                 int64_t* out, const csd_modulus* mods, int full0, int full1) {
+    GksArgs a;
+    memset(&a, 0, sizeof(a));
+    a.r = r;
+    a.n2 = n2;
+    a.n3 = n3;
+    a.d = (int)d;
+    a.batch = (int)batch;
+    a.parts = parts;
+    a.c0 = c0;
+    a.key = key;
+    a.perm2 = perm2;
+    a.perm3 = perm3;
+    a.out = out;
+    a.full0 = full0;
+    a.full1 = full1;
+    a.chunks = (n3 + 255) / 256;
+    for (int64_t jj = 0; jj < r; jj++) a.one[jj] = (int64_t)(((u128)1 << 64) % (u128)mods[jj].N);  // mont(1)
+    ModTable t = make_mod_table(mods, (size_t)r, r);
+    int64_t groups = (batch + GKS_CB - 1) / GKS_CB;
+    int64_t blocks = groups * r * n2 * a.chunks;
+    if (blocks > (int64_t)INT32_MAX) throw CsdError("galois key switch: grid too large");
+    double bytes = 8.0 * (double)(batch * d * r * n2 * n3 + batch * r * n2 * n3 + 2 * batch * r * n2 * n3 + 2 * d * r * n2 * n3);
+    TimedScope ts(ctx, "galois_keyswitch", bytes);
+    hipLaunchKernelGGL(k_galois_keyswitch, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+}

@@ -894,12 +894,6 @@ extern "C" int csd_hal_bconv_correct_pow2_wide(csd_ctx* ctx, const csd_view* dst
 // ---------------------------------------------------------------------------------------------
 // permutations (hal/cpu.rs:1059-1122)
 
-struct csd_perm_plan {
-    csd_ctx* ctx;
-    uint32_t* dperm;
-    size_t len;
-    std::vector<uint32_t> host;
-};
 
 extern "C" int csd_hal_create_permutation(csd_ctx* ctx, const size_t* inner_shape, const size_t* shifts, size_t n,
                                           csd_perm_plan** out) {

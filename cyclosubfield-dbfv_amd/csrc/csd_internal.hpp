@@ -65,6 +65,18 @@ struct TimedScope {
     ~TimedScope();
 };
 
+// dst[e] = src[perm[e]] along one dimension (hal/cpu.rs:481-485)
+struct csd_perm_plan {
+    csd_ctx* ctx;
+    uint32_t* dperm;
+    size_t len;
+    std::vector<uint32_t> host;
+};
+
+void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t r, int64_t n2, int64_t n3, const int64_t* parts,
+                                 const int64_t* c0, const int64_t* key, const uint32_t* perm2, const uint32_t* perm3,
+                                 int64_t* out, const csd_modulus* mods, int full0, int full1);
+
 struct Shape4 {
     int64_t d[4];
 };

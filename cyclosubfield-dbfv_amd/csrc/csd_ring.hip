@@ -787,8 +787,8 @@ struct csd_ring {
         ok(csd_hal_mul_mont64(ctx, &dst, &vinv, m.data(), nl));
     }
 
-    // apply_galois (ring.rs:1183-1222)
-    void apply_galois(const csd_view& dst, const csd_view& src, uint64_t g) {
+    // the (dim 2, dim 3) permutations of the automorphism g (ring.rs:1187-1210)
+    std::pair<csd_perm_plan*, csd_perm_plan*> galois_perms(uint64_t g) {
         auto shift = f->gal_to_shift(g);
         auto it = pcache.find(shift);
         if (it == pcache.end()) {
@@ -801,10 +801,51 @@ struct csd_ring {
             ok(csd_hal_create_permutation(ctx, o.data() + nl, s_k.data(), o.size() - nl, &pk));
             it = pcache.emplace(shift, std::make_pair(plp, pk)).first;
         }
+        return it->second;
+    }
+
+    // apply_galois (ring.rs:1183-1222)
+    void apply_galois(const csd_view& dst, const csd_view& src, uint64_t g) {
+        auto perms = galois_perms(g);
         DevBuf tmp(ctx, (size_t)numel(&src) * 8);
         csd_view vt = make_view(tmp.p, src.shape[0], src.shape[1], src.shape[2], src.shape[3]);
-        ok(csd_hal_permute(ctx, &vt, &src, it->second.first, CSD_DIM2));
-        ok(csd_hal_permute(ctx, &dst, &vt, it->second.second, CSD_DIM3));
+        ok(csd_hal_permute(ctx, &vt, &src, perms.first, CSD_DIM2));
+        ok(csd_hal_permute(ctx, &dst, &vt, perms.second, CSD_DIM3));
+    }
+
+    // The key-switch part of bfv_hom_gal (bfv.rs:816-839) for a batch of ciphertexts whose
+    // gadget digits were hoisted, over a set of rotations; see csd_fused.hip.
+    void galois_keyswitch(const std::vector<size_t>& base, int64_t batch, int64_t d, const csd_view& parts, const csd_view& c0,
+                          const std::vector<uint64_t>& gs, const csd_view* keys, const csd_view& out) {
+        int64_t r = (int64_t)base.size(), n2 = (int64_t)f->deg_lp, n3 = (int64_t)f->deg_k;
+        auto contiguous = [](const csd_view& v, int64_t s0, int64_t s1, int64_t s2, int64_t s3) {
+            csd_view e = make_view(v.ptr, s0, s1, s2, s3);
+            for (int t = 0; t < 4; t++)
+                if (v.shape[t] != e.shape[t] || (e.shape[t] > 1 && v.strides[t] != e.strides[t])) return false;
+            return true;
+        };
+        if (!contiguous(parts, batch * d, r, n2, n3)) throw CsdError("galois key switch: parts must be a contiguous [batch*d, r, n2, n3] tensor");
+        if (!contiguous(c0, batch, r, n2, n3)) throw CsdError("galois key switch: c0 must be a contiguous [batch, r, n2, n3] tensor");
+        if (!contiguous(out, (int64_t)gs.size() * batch * 2, r, n2, n3))
+            throw CsdError("galois key switch: out must be a contiguous [ngal*batch*2, r, n2, n3] tensor");
+        auto m = mods_for(base);
+        // WideRingEls bounds (ring.rs:354-365, 441-455): d terms of bound-1 operands, plus the
+        // bound-1 c0 on row 0; a row needs the full reduction above 2^63 min_p (ring.rs:366)
+        u128 max_p = 0, min_p = ~(u128)0;
+        for (size_t i : base) {
+            max_p = std::max<u128>(max_p, (u128)primes[i]);
+            min_p = std::min<u128>(min_p, (u128)primes[i]);
+        }
+        u128 b1 = (u128)d * max_p * max_p, b0 = b1 + max_p * max_p;
+        int full0 = b0 > (min_p << 63), full1 = b1 > (min_p << 63);
+        int64_t per_g = batch * 2 * r * n2 * n3;
+        for (size_t gi = 0; gi < gs.size(); gi++) {
+            if (!contiguous(keys[gi], 2 * d, r, n2, n3)) throw CsdError("galois key switch: keys must be contiguous [2*d, r, n2, n3]");
+            auto perms = galois_perms(gs[gi]);
+            csd_galois_keyswitch_launch(ctx, batch, d, r, n2, n3, (const int64_t*)parts.ptr, (const int64_t*)c0.ptr,
+                                        (const int64_t*)keys[gi].ptr, perms.first->dperm, perms.second->dperm,
+                                        (int64_t*)out.ptr + (int64_t)gi * per_g, m.data(), full0, full1);
+        }
     }
 };
 
@@ -834,6 +875,68 @@ extern "C" int csd_field_gal_to_shift(const char* name, uint64_t g, int64_t* shi
     if (s.size() > cap) throw CsdError("shift buffer too small");
     for (size_t i = 0; i < s.size(); i++) shifts[i] = s[i];
     *n = s.size();
+    CSD_CATCH
+}
+
+// required_galois_elements (bootstrap/homcrt.rs:336-349): per prime p | m_K, the generator of the
+// inertia group I_p (field.rs:200-213) and the baby-step/giant-step split of the cyclic
+// transform of length |J_p| = (p - 1) / |I_p ∩ Gal(M|L)| (homcrt.rs:320-334)
+static std::vector<uint64_t> homcrt_galois_elements(const Field& f, std::vector<uint64_t>* orders = nullptr) {
+    std::unordered_map<uint64_t, bool> ml;
+    for (uint64_t h : f.gal_ml) ml[h] = true;
+    std::vector<uint64_t> res;
+    for (uint64_t p : f.m_factors) {
+        if (p == f.m_prime && f.m_prime != 1) continue;
+        std::vector<uint64_t> pf;
+        uint64_t rest = p - 1;
+        for (uint64_t q = 2; rest > 1; q++)
+            if (rest % q == 0) {
+                pf.push_back(q);
+                while (rest % q == 0) rest /= q;
+            }
+        uint64_t gen = 2;
+        for (;; gen++) {
+            bool ok_ = true;
+            for (uint64_t q : pf)
+                if (powmod_u(gen, (p - 1) / q, p) == 1) ok_ = false;
+            if (ok_) break;
+        }
+        uint64_t ipg = crt2(gen, p, 1, f.m / p);
+        uint64_t hp = 0, x = 1;
+        for (uint64_t e = 0; e < p - 1; e++) {
+            if (ml.count(x)) hp++;
+            x = mulmod_u(x, ipg, f.m);
+        }
+        uint64_t len = (p - 1) / hp;
+        if (orders) orders->push_back(len);
+        if (len <= 1) continue;
+        uint64_t best = 1, best_cost = UINT64_MAX;
+        for (uint64_t c = 1; c <= len; c++) {
+            uint64_t cost = c + 3 * ((len - 1) / c + 1);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best = c;
+            }
+        }
+        uint64_t giants = (len - 1) / best + 1;
+        for (uint64_t j = 1; j < best; j++) res.push_back(powmod_u(ipg, j, f.m));
+        if (giants > 1) res.push_back(powmod_u(ipg, best, f.m));
+    }
+    std::sort(res.begin(), res.end());
+    res.erase(std::unique(res.begin(), res.end()), res.end());
+    return res;
+}
+
+extern "C" int csd_field_homcrt_galois_elements(const char* name, uint64_t* out, size_t cap, size_t* n, uint64_t* jp_orders,
+                                                size_t jp_cap, size_t* n_jp) {
+    CSD_TRY auto f = make_field(name);
+    std::vector<uint64_t> orders;
+    auto g = homcrt_galois_elements(*f, &orders);
+    if (g.size() > cap || orders.size() > jp_cap) throw CsdError("buffer too small");
+    std::copy(g.begin(), g.end(), out);
+    std::copy(orders.begin(), orders.end(), jp_orders);
+    *n = g.size();
+    *n_jp = orders.size();
     CSD_CATCH
 }
 
@@ -991,6 +1094,15 @@ extern "C" int csd_ring_apply_galois(csd_ring* ring, const csd_view* dst, const 
     CSD_TRY check_view(dst, "apply_galois dst");
     check_view(src, "apply_galois src");
     ring->apply_galois(*dst, *src, g);
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t digits,
+                                         const csd_view* parts, const csd_view* c0, const uint64_t* gs, size_t ngal,
+                                         const csd_view* keys, const csd_view* out) {
+    CSD_TRY if (batch == 0 || digits == 0 || ngal == 0) throw CsdError("galois key switch: empty batch, digits or rotations");
+    ring->galois_keyswitch(to_vec(base, r), (int64_t)batch, (int64_t)digits, *parts, *c0, std::vector<uint64_t>(gs, gs + ngal),
+                           keys, *out);
     CSD_CATCH
 }
 

@@ -32,6 +32,16 @@ def gal_to_shift(name, g):
     return [out[i] for i in range(n.value)]
 
 
+def homcrt_galois_elements(name):
+    """required_galois_elements (bootstrap/homcrt.rs:336-349) and the |J_p| of each p | m_K."""
+    out = (ctypes.c_uint64 * 4096)()
+    jp = (ctypes.c_uint64 * 16)()
+    n, njp = ctypes.c_size_t(), ctypes.c_size_t()
+    check(lib().csd_field_homcrt_galois_elements(name.encode(), out, ctypes.c_size_t(4096), ctypes.byref(n), jp,
+                                                 ctypes.c_size_t(16), ctypes.byref(njp)))
+    return [out[i] for i in range(n.value)], [jp[i] for i in range(njp.value)]
+
+
 def sample_primes(name, log2_q, t_bits):
     out = (ctypes.c_int64 * 128)()
     n, il = ctypes.c_size_t(), ctypes.c_size_t()
@@ -113,6 +123,22 @@ class RingContext:
 
     def apply_galois(self, dst, src, g):
         check(lib().csd_ring_apply_galois(self.handle, ctypes.byref(dst.view()), ctypes.byref(src.view()), ctypes.c_uint64(g)))
+
+    def galois_keyswitch(self, base, batch, digits, parts, c0, gs, keys, out):
+        """Batched key switching of bfv_hom_gal (bfv.rs:816-839): see csd_ring_galois_keyswitch."""
+        b, n = _sizes(base)
+        garr = (ctypes.c_uint64 * len(gs))(*gs)
+        karr = (type(parts.view()) * len(keys))(*[k.view() for k in keys])
+        check(lib().csd_ring_galois_keyswitch(self.handle, b, n, ctypes.c_size_t(batch), ctypes.c_size_t(digits),
+                                              ctypes.byref(parts.view()), ctypes.byref(c0.view()), garr,
+                                              ctypes.c_size_t(len(gs)), karr, ctypes.byref(out.view())))
+
+    def fill_uniform(self, dst, base, seed):
+        """Synthetic residues uniform in [0, p) per limb (splitmix64 of the flat index)."""
+        mods = [self.backend.hal_create_modulus(self.primes[i]) for i in base]
+        arr = (type(mods[0]) * len(mods))(*mods)
+        check(lib().csd_fill_uniform(self.backend.ctx, ctypes.byref(dst.view()), arr, ctypes.c_size_t(len(mods)),
+                                     ctypes.c_uint64(seed)))
 
     def reduce_wide_tensor(self, dst, wide, base, full):
         b, n = _sizes(base)

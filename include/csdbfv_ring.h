@@ -24,6 +24,11 @@ int csd_field_info(const char* field_name, uint64_t* m, uint64_t* deg_l, uint64_
                    size_t* lprime_factor_count);
 /* FieldDescriptor::gal_to_shift (field.rs:327-336) */
 int csd_field_gal_to_shift(const char* field_name, uint64_t g, int64_t* shifts, size_t cap, size_t* n);
+/* required_galois_elements (bootstrap/homcrt.rs:336-349): the Galois elements whose keys the
+ * homomorphic CRT transform needs (baby steps and one giant step per prime p | m_K), sorted,
+ * and the cyclic transform lengths |J_p| */
+int csd_field_homcrt_galois_elements(const char* field_name, uint64_t* out, size_t cap, size_t* n, uint64_t* jp_orders,
+                                     size_t jp_cap, size_t* n_jp);
 /* the prime sampler of RingContext::new (ring.rs:136-183, 620-650): initial base and the
  * extended pool, no device work. */
 int csd_sample_primes(const char* field_name, size_t log2_q_lo, size_t log2_q_hi, size_t t_bits, int64_t* out,
@@ -63,6 +68,19 @@ int csd_ring_mod_switch_down(csd_ring* ring, const csd_view* dst, const csd_view
                              size_t new_len);
 /* apply_galois (ring.rs:1183-1222): dst = sigma_g(src), any basis */
 int csd_ring_apply_galois(csd_ring* ring, const csd_view* dst, const csd_view* src, uint64_t g);
+/* The key-switching inner products of bfv_hom_gal (bfv.rs:816-839), batched: for ciphertexts
+ * c < batch with hoisted gadget digits parts[c*digits + i] (CRT basis, [batch*digits, r, ., .])
+ * and first components c0[c] ([batch, r, ., .]), and rotations g = gs[t] with keys[t] the
+ * [2*digits, r, ., .] tensor of key digit i, row s at index 2i+s (KeySwitchKey.keys, resident
+ * in HBM), writes out[(t*batch + c)*2 + s] = row s of
+ *   reduce_wide( sum_i sigma_g(parts[c,i]) * key_i + (s == 0 ? sigma_g(c0[c]) * mont(1) : 0) ).
+ * All operands are bound-1 (almost reduced); all tensors contiguous. One fused kernel per
+ * rotation; bit-identical to apply_galois + inner_product + add + reduce_wide. */
+int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t digits,
+                              const csd_view* parts, const csd_view* c0, const uint64_t* gs, size_t ngal,
+                              const csd_view* keys, const csd_view* out);
+/* synthetic inputs: dst[i, j, k, l] uniform in [0, p_j) from splitmix64 of the flat index */
+int csd_fill_uniform(csd_ctx* ctx, const csd_view* dst, const csd_modulus* mods, size_t nmods, uint64_t seed);
 /* reduce_wide_tensor (ring.rs:846-865) for rows of equal reduction kind: full = 1 takes
  * red_mont64_double + i64_to_mont64, full = 0 a single red_mont64 */
 int csd_ring_reduce_wide(csd_ring* ring, const csd_view* dst, const csd_view* wide, const size_t* base, size_t r, int full);

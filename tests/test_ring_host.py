@@ -76,6 +76,29 @@ def test_benchmark_bases():
     assert init == 23
 
 
+@pytest.mark.parametrize("name,count,jp", [("field_49152", 40, [4, 64, 192]), ("field_36864", 27, [4, 6, 24, 64]),
+                                           ("field_18", 5, [6, 3])])
+def test_homcrt_rotation_sets(name, count, jp):
+    """required_galois_elements (homcrt.rs:336-349): |J_p| = (p-1)/|I_p ∩ Gal(M|L)| per p | m_K and
+    the BSGS split (homcrt.rs:320-334); every element is a unit mod m and maps to a nonzero shift."""
+    ring = load().ring
+    gs, orders = ring.homcrt_galois_elements(name)
+    assert orders == jp
+    assert len(gs) == count == len(set(gs))
+    f = R.FieldDescriptor(name)
+    import math
+    for g in gs:
+        assert math.gcd(g, f.m) == 1
+        assert any(s != 0 for s in f.gal_to_shift(g))
+    # baby steps of one factor are consecutive powers of one generator: for |J_p| = 64 the split is
+    # 13 baby steps x 5 giant steps (12 powers plus the giant step power 13)
+    if name == "field_49152":
+        def split(n):
+            best = min(range(1, n + 1), key=lambda c: c + 3 * ((n - 1) // c + 1))
+            return best, (n - 1) // best + 1
+        assert [split(n) for n in jp] == [(4, 1), (13, 5), (24, 8)]
+
+
 def test_host_errors():
     pkg = load()
     with pytest.raises(pkg.HalError):
