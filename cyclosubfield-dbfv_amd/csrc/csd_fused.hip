@@ -137,6 +137,85 @@ __global__ __launch_bounds__(256) void k_galois_keyswitch(GksArgs a, ModTable mt
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// mod-switch by one limb for a work-list of uneven ciphertext elements (bfv.rs:628-662 with the
+// single-prime BConvCache of ring.rs:1606-1701): per element e at level r (limbs 0..r-1, period
+// basis) and position pos, with x the dropped limb's value:
+//   x1   = mont_mul(x, qopi)                                  qopi = lift((q/p_d)^-1) = 1 here
+//   d_i  = double_red(M_i x1) + red(((C x1 + 2^(lq-1)) >> lq) qm_i),   then to_mont64
+//   out_i = mont_mul(in_i - d_i, inv_i)                        (i < r - 1)
+// M_i = lift(2^128 mod p_i), C = round(2^lq / p_d), qm_i = lift(-p_d 2^64 mod p_i) and
+// inv_i = mont(lift(p_d^-1 mod p_i)) are the per-level constants of `MsLevel`.
+
+struct MsLevel {
+    int64_t M[CSD_MAX_MODULI], qm[CSD_MAX_MODULI], inv[CSD_MAX_MODULI];
+    int64_t C, qopi;
+    int32_t lq, r;
+};
+
+struct MsArgs {
+    int64_t* const* in;     // per element: [r, npos] (period basis)
+    int64_t* const* out;    // per element: [r - 1, npos]
+    const int32_t* level;   // per element: index into `levels`
+    const MsLevel* levels;
+    int64_t npos, chunks;
+};
+
+__global__ __launch_bounds__(256) void k_mod_switch_one(MsArgs a, ModTable mt) {
+    const int64_t e = blockIdx.x / a.chunks;
+    const int64_t pos = (blockIdx.x % a.chunks) * 256 + threadIdx.x;
+    if (pos >= a.npos) return;
+    const MsLevel& lv = a.levels[a.level[e]];
+    const int r = lv.r;
+    const int64_t* in = a.in[e];
+    int64_t* out = a.out[e];
+    const CsdMod md = mt.m[r - 1];
+    const int64_t x1 = csd_mont_mul(md, in[(int64_t)(r - 1) * a.npos + pos], lv.qopi);
+    const i128 corr = (i128)lv.C * (i128)x1;
+    const i128 q = (i128)((u128)corr + ((u128)1 << (lv.lq - 1))) >> lv.lq;
+    for (int i = 0; i < r - 1; i++) {
+        const CsdMod m = mt.m[i];
+        int64_t d = csd_double_red128(m, (i128)lv.M[i] * (i128)x1);
+        d = (int64_t)((uint64_t)d + (uint64_t)csd_red128(m, (i128)((u128)q * (u128)(i128)lv.qm[i])));
+        d = csd_to_mont64(m, d);
+        const int64_t diff = (int64_t)((uint64_t)in[(int64_t)i * a.npos + pos] - (uint64_t)d);
+        out[(int64_t)i * a.npos + pos] = csd_mont_mul(m, diff, lv.inv[i]);
+    }
+}
+
+void csd_mod_switch_one_launch(csd_ctx* ctx, int64_t* const* in, int64_t* const* out, const int32_t* level, const void* levels,
+                               int64_t nelems, int64_t npos, const csd_modulus* mods, size_t nmods, double bytes) {
+    MsArgs a;
+    a.in = in;
+    a.out = out;
+    a.level = level;
+    a.levels = (const MsLevel*)levels;
+    a.npos = npos;
+    a.chunks = (npos + 255) / 256;
+    ModTable t = make_mod_table(mods, nmods, (int64_t)nmods);
+    if (nelems == 0) return;
+    TimedScope ts(ctx, "mod_switch_one", bytes);
+    hipLaunchKernelGGL(k_mod_switch_one, dim3((unsigned)(nelems * a.chunks)), dim3(256), 0, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+}
+
+size_t csd_mod_switch_level_bytes() { return sizeof(MsLevel); }
+
+void csd_mod_switch_fill_level(void* dst, int r, const int64_t* M, const int64_t* qm, const int64_t* inv, int64_t C, int lq,
+                               int64_t qopi) {
+    MsLevel* lv = (MsLevel*)dst;
+    memset(lv, 0, sizeof(MsLevel));
+    for (int i = 0; i < r - 1; i++) {
+        lv->M[i] = M[i];
+        lv->qm[i] = qm[i];
+        lv->inv[i] = inv[i];
+    }
+    lv->C = C;
+    lv->qopi = qopi;
+    lv->lq = lq;
+    lv->r = r;
+}
+
 // exported through the ring (which owns the permutations); see csd_ring.hip
 void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t r, int64_t n2, int64_t n3, const int64_t* parts,
                                  const int64_t* c0, const int64_t* key, const uint32_t* perm2, const uint32_t* perm3,

@@ -77,6 +77,26 @@ void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t
                                  const int64_t* c0, const int64_t* key, const uint32_t* perm2, const uint32_t* perm3,
                                  int64_t* out, const csd_modulus* mods, int full0, int full1);
 
+// launch shape of the work-list NTT kernels (0 = the built-in choice)
+struct CsdSegTune {
+    int rows = 0;          // dim-3: lines per workgroup
+    int threads = 0;       // dim-3: workgroup size
+    int col_threads = 256; // dim-2: columns per workgroup
+};
+
+// NTT along dim 2 or 3 of `nslices` [d2, d3] matrices (row stride st2) at seg_base[s], each
+// transformed with plans[seg_limb[s]]; `mul` (optional, [1, nplans, d2|1, d3|1]) is multiplied in
+// Montgomery form after the transform, in any direction.
+void csd_ntt_segments(csd_ctx* ctx, int64_t* const* seg_base, const int32_t* seg_limb, int64_t nslices, int64_t d2, int64_t d3,
+                      int64_t st2, csd_ntt_plan* const* plans, size_t nplans, int dim, int dir, const csd_view* mul,
+                      const csd_modulus* mods, const char* timer, const CsdSegTune& tune);
+
+void csd_mod_switch_one_launch(csd_ctx* ctx, int64_t* const* in, int64_t* const* out, const int32_t* level, const void* levels,
+                               int64_t nelems, int64_t npos, const csd_modulus* mods, size_t nmods, double bytes);
+size_t csd_mod_switch_level_bytes();
+void csd_mod_switch_fill_level(void* dst, int r, const int64_t* M, const int64_t* qm, const int64_t* inv, int64_t C, int lq,
+                               int64_t qopi);
+
 struct Shape4 {
     int64_t d[4];
 };

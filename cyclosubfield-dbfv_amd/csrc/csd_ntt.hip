@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -143,6 +144,8 @@ static uint32_t magic_for(uint32_t d) { return d <= 1 ? 0u : (uint32_t)((((uint6
 
 static const u128 I64MAX_U = (u128)INT64_MAX;
 
+static void build_passes(DevPlan& P, int dir, int g2, int g3);
+
 // Replays CpuNTTPlan::apply's bound recursion (hal/ntt.rs:581-657) and records what each
 // factor's LazyFactorNTT::forward/backward would execute.
 static void build_schedule(DevPlan& P, int dir) {
@@ -204,11 +207,18 @@ static void build_schedule(DevPlan& P, int dir) {
             pending_reduce = false;
         }
     }
-    // passes: at most 3 radix-2 or 2 radix-3 stages per register block
+    build_passes(P, dir, 3, 2);
+}
+
+// Groups each factor's stages into register-blocked LDS passes of at most g2 radix-2 or g3
+// radix-3 stages. The grouping changes only the LDS round trips, never the arithmetic: every
+// stage still applies its own schedule bits, so all groupings give identical results.
+static void build_passes(DevPlan& P, int dir, int g2, int g3) {
+    const bool inverse = dir == 2;
     std::vector<DevPass> passes;
     for (int fi = 0; fi < P.nfactors; fi++) {
         const DevFactor& F = P.f[fi];
-        const int G = F.radix == 2 ? 3 : 2;
+        const int G = F.radix == 2 ? g2 : g3;
         if (!inverse) {
             for (int s0 = 0; s0 < F.nstages;) {
                 int g = std::min(G, F.nstages - s0);
@@ -386,6 +396,16 @@ extern "C" int csd_ntt_plan_mults(const csd_ntt_plan* plan, int dir, uint64_t* o
         }
     }
     *out = cnt;
+    CSD_CATCH
+}
+
+// Regroups the plan's LDS passes (an autotuning knob, see build_passes) and refreshes the
+// device copy of the plan header; results are unchanged.
+extern "C" int csd_ntt_plan_set_grouping(csd_ntt_plan* plan, int g2, int g3) {
+    CSD_TRY if (g2 < 1 || g2 > 3 || g3 < 1 || g3 > 2) throw CsdError("grouping: 1 <= g2 <= 3 and 1 <= g3 <= 2");
+    for (int dir = 0; dir < 3; dir++) build_passes(plan->host, dir, g2, g3);
+    HIPCHECK(hipMemcpyAsync(plan->dev, &plan->host, sizeof(DevPlan), hipMemcpyHostToDevice, plan->ctx->stream));
+    HIPCHECK(hipStreamSynchronize(plan->ctx->stream));
     CSD_CATCH
 }
 
@@ -717,6 +737,11 @@ __device__ __forceinline__ void block_stages(int64_t (&v)[Pow<R>::of(G)], const 
     }
 }
 
+// LDS slot of line element e: one padding slot after every 8 elements, so that the strided
+// groups of the late stages (8 consecutive elements per lane) spread over all banks
+__device__ __forceinline__ uint32_t lds_slot(uint32_t e) { return e + (e >> 3); }
+__host__ __device__ __forceinline__ uint32_t lds_line(uint32_t L) { return L + (L >> 3) + 1; }
+
 // Runs one pass over all groups of `rows` LDS-resident lines of length L.
 template <int R, int G, bool INV>
 __device__ __forceinline__ void run_pass(int64_t* __restrict__ lds, int L, int rows, uint32_t magic_groups,
@@ -730,6 +755,7 @@ __device__ __forceinline__ void run_pass(int64_t* __restrict__ lds, int L, int r
     const uint32_t msub = sub <= 1 ? 0u : (uint32_t)((((uint64_t)1) << 32) / sub + 1);
     const uint32_t ngroups = (uint32_t)L / M;
     const uint32_t total = ngroups * (uint32_t)rows;
+    const uint32_t lline = lds_line((uint32_t)L);
     for (uint32_t w = threadIdx.x; w < total; w += blockDim.x) {
         const uint32_t row = rows == 1 ? 0u : fdiv(w, magic_groups);
         const uint32_t gam = w - row * ngroups;
@@ -739,13 +765,13 @@ __device__ __forceinline__ void run_pass(int64_t* __restrict__ lds, int L, int r
         const uint32_t jj = t - blk0 * sub;
         const uint32_t base = (blk0 * SL + jj) * S + suffix;
         const uint32_t stride = sub * S;
-        int64_t* line = lds + (size_t)row * L;
+        int64_t* line = lds + (size_t)row * lline;
         int64_t v[M];
 #pragma unroll
-        for (int m = 0; m < M; m++) v[m] = line[base + m * stride];
+        for (int m = 0; m < M; m++) v[m] = line[lds_slot(base + m * stride)];
         block_stages<R, G, INV>(v, P, TW, F, dir, ps.s0, blk0);
 #pragma unroll
-        for (int m = 0; m < M; m++) line[base + m * stride] = v[m];
+        for (int m = 0; m < M; m++) line[lds_slot(base + m * stride)] = v[m];
     }
 }
 
@@ -761,34 +787,51 @@ struct RowArgs {
     int64_t chunks;     // workgroups per (i, j)
     const int64_t* mul;
     Str4 smul;
+    // work-list mode (uneven batches): slice s is the [d2, d3] matrix at seg_base[s] with limb
+    // index seg_limb[s]; blocks enumerate (slice, chunk) instead of (i, j, chunk)
+    int64_t* const* seg_base;
+    const int32_t* seg_limb;
 };
 
 // dim-3 transforms (contiguous lines): lines staged in LDS, stages run in register blocks
 template <bool INV>
-__global__ __launch_bounds__(256) void k_ntt_rows(RowArgs a, ModTable mt) {
+__global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
     extern __shared__ int64_t lds[];
     const int64_t chunk = blockIdx.x % a.chunks;
     const int64_t ij = blockIdx.x / a.chunks;
-    const int64_t j = ij % a.d1;
-    const int64_t i = ij / a.d1;
+    int64_t i, j;
+    int64_t* base;
+    const int64_t k0 = chunk * a.rows;
+    if (a.seg_base) {
+        i = 0;
+        j = a.seg_limb[ij];
+        base = a.seg_base[ij] + k0 * a.st[2];
+    } else {
+        j = ij % a.d1;
+        i = ij / a.d1;
+        base = a.data + i * a.st[0] + j * a.st[1] + k0 * a.st[2];
+    }
     const DevPlan* __restrict__ P = a.plans[j];
     const CsdBarrett* __restrict__ TW = (const CsdBarrett*)(P + 1);
     const int L = P->len;
-    const int64_t k0 = chunk * a.rows;
+    const uint32_t lline = lds_line((uint32_t)L);
     const int rows = (int)min((int64_t)a.rows, a.d2 - k0);
-    int64_t* base = a.data + i * a.st[0] + j * a.st[1] + k0 * a.st[2];
     const int total = rows * L;
     for (int g = threadIdx.x; g < total; g += blockDim.x) {
         const uint32_t r = a.rows == 1 ? 0u : fdiv((uint32_t)g, a.magic_L);
         const int p = g - (int)r * L;
-        lds[g] = base[r * a.st[2] + p];
+        lds[r * lline + lds_slot(p)] = base[r * a.st[2] + p];
     }
     __syncthreads();
     const int dir = a.dir;
     if (P->nfactors == 0) {
         if (dir != 1) {
             const CsdBarrett m = dir == 2 ? P->inv_len : P->one;
-            for (int g = threadIdx.x; g < total; g += blockDim.x) lds[g] = csd_barrett(m, lds[g], P->n);
+            for (int g = threadIdx.x; g < total; g += blockDim.x) {
+                const uint32_t r = a.rows == 1 ? 0u : fdiv((uint32_t)g, a.magic_L);
+                const uint32_t s = r * lline + lds_slot(g - r * L);
+                lds[s] = csd_barrett(m, lds[s], P->n);
+            }
         }
     } else {
         const int np = P->npasses[dir];
@@ -814,7 +857,7 @@ __global__ __launch_bounds__(256) void k_ntt_rows(RowArgs a, ModTable mt) {
     for (int g = threadIdx.x; g < total; g += blockDim.x) {
         const uint32_t r = a.rows == 1 ? 0u : fdiv((uint32_t)g, a.magic_L);
         const int p = g - (int)r * L;
-        int64_t x = lds[g];
+        int64_t x = lds[r * lline + lds_slot(p)];
         if (a.mul)
             x = csd_mont_mul(m, x, a.mul[i * a.smul.s[0] + j * a.smul.s[1] + (k0 + r) * a.smul.s[2] + p * a.smul.s[3]]);
         base[r * a.st[2] + p] = x;
@@ -900,6 +943,8 @@ struct ColArgs {
     int64_t chunks;
     const int64_t* mul;
     Str4 smul;
+    int64_t* const* seg_base;  // work-list mode, as in RowArgs
+    const int32_t* seg_limb;
 };
 
 template <int F0, int F1, bool INV>
@@ -907,13 +952,21 @@ __global__ __launch_bounds__(256) void k_ntt_cols(ColArgs a, ModTable mt) {
     constexpr int L = F0 * F1;
     const int64_t chunk = blockIdx.x % a.chunks;
     const int64_t ij = blockIdx.x / a.chunks;
-    const int64_t j = ij % a.d1;
-    const int64_t i = ij / a.d1;
-    const int64_t l = chunk * 256 + threadIdx.x;
+    const int64_t l = chunk * blockDim.x + threadIdx.x;
     if (l >= a.d3) return;
+    int64_t i, j;
+    int64_t* base;
+    if (a.seg_base) {
+        i = 0;
+        j = a.seg_limb[ij];
+        base = a.seg_base[ij] + l;
+    } else {
+        j = ij % a.d1;
+        i = ij / a.d1;
+        base = a.data + i * a.st[0] + j * a.st[1] + l;
+    }
     const DevPlan* __restrict__ P = a.plans[j];
     const CsdBarrett* __restrict__ TW = (const CsdBarrett*)(P + 1);
-    int64_t* base = a.data + i * a.st[0] + j * a.st[1] + l;
     int64_t v[L];
 #pragma unroll
     for (int k = 0; k < L; k++) v[k] = base[k * a.st[2]];
@@ -1044,6 +1097,33 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         int rows = 1;
         while (rows * 2 * L <= 6144 && rows * 2 <= a.d2 && rows < 256) rows *= 2;
         r.rows = rows;
+        // block size: the candidate with the best (resident threads per CU) x (fraction of
+        // non-idle iterations over this plan's passes); measured on lines of 3072, two lines per
+        // 256-thread block beat one line per 128 or 192 threads
+        int threads = 256;
+        {
+            const DevPlan& P0 = plans[0]->host;
+            const size_t lds_block = (size_t)rows * lds_line((uint32_t)L) * 8;
+            double best = -1;
+            for (int T : {256, 512}) {
+                double busy = 0, slots = 0;
+                for (int q = 0; q < P0.npasses[dir]; q++) {
+                    const DevPass& ps = P0.passes[dir][q];
+                    int M = ps.radix == 2 ? (1 << ps.g) : (ps.g == 1 ? 3 : 9);
+                    double groups = (double)rows * (double)(L / M);
+                    busy += groups;
+                    slots += std::ceil(groups / T) * T;
+                }
+                int by_lds = (int)std::max<size_t>(1, (160 * 1024) / lds_block);
+                int by_threads = 2048 / T;
+                double resident = (double)std::min(by_lds, by_threads) * T;
+                double score = resident * (slots > 0 ? busy / slots : 1.0);
+                if (score > best + 1e-9) {
+                    best = score;
+                    threads = T;
+                }
+            }
+        }
         r.magic_L = magic_for((uint32_t)L);
         const int Ms[5] = {2, 4, 8, 3, 9};
         for (int q = 0; q < 5; q++) r.magic_groups[q] = L % Ms[q] == 0 ? magic_for((uint32_t)(L / Ms[q])) : 0u;
@@ -1052,7 +1132,8 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         r.smul = a.smul;
         int64_t blocks = a.d0 * a.d1 * r.chunks;
         if (blocks > (int64_t)INT32_MAX) throw CsdError("ntt: too many tiles");
-        size_t lds = (size_t)rows * L * 8;
+        size_t lds = (size_t)rows * lds_line((uint32_t)L) * 8;
+        if (lds > 160 * 1024) throw CsdError("ntt: line too long for LDS staging");
         static bool row_attr = false;
         if (!row_attr) {
             HIPCHECK(hipFuncSetAttribute((const void*)k_ntt_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
@@ -1061,9 +1142,9 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         }
         TimedScope ts(ctx, names[1][dir], 16.0 * (double)numel(dst));
         if (inv)
-            hipLaunchKernelGGL(k_ntt_rows<true>, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, r, t);
+            hipLaunchKernelGGL(k_ntt_rows<true>, dim3((unsigned)blocks), dim3(threads), lds, ctx->stream, r, t);
         else
-            hipLaunchKernelGGL(k_ntt_rows<false>, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, r, t);
+            hipLaunchKernelGGL(k_ntt_rows<false>, dim3((unsigned)blocks), dim3(threads), lds, ctx->stream, r, t);
         HIPCHECK(hipGetLastError());
         return;
     }
@@ -1088,6 +1169,99 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
                                        {"generic_ntt_dim3", "generic_ntt_mul_dim3", "generic_intt_dim3"}};
     TimedScope ts(ctx, gnames[dim - 2][dir], 16.0 * (double)numel(dst));
     hipLaunchKernelGGL(k_ntt, dim3((unsigned)blocks), dim3(NTT_THREADS), lds, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+}
+
+// Work-list transforms for uneven batches: `nslices` matrices [d2, d3] (row stride st2) at the
+// device addresses seg_base[s], each transformed with plans[seg_limb[s]] along `dim` in direction
+// `dir` (0 forward reduced, 1 forward unreduced, 2 inverse), optionally followed by the
+// Montgomery multiplication with mul[0, limb, ., .] (broadcast over its length-1 dimensions).
+// One launch for the whole work-list; the per-slice results equal the per-tensor HAL calls.
+void csd_ntt_segments(csd_ctx* ctx, int64_t* const* seg_base, const int32_t* seg_limb, int64_t nslices, int64_t d2, int64_t d3,
+                      int64_t st2, csd_ntt_plan* const* plans, size_t nplans, int dim, int dir, const csd_view* mul,
+                      const csd_modulus* mods, const char* timer, const CsdSegTune& tune) {
+    if (nplans > CSD_MAX_MODULI) throw CsdError("too many plans");
+    const int64_t L = dim == 2 ? d2 : d3;
+    for (size_t jj = 0; jj < nplans; jj++)
+        if (plans[jj]->host.len != L) throw CsdError("work-list ntt: plan length mismatch");
+    ModTable t;
+    memset(&t, 0, sizeof(t));
+    Str4 smul{{0, 0, 0, 0}};
+    if (mul) {
+        if (mul->shape[0] != 1 || mul->shape[1] != (int64_t)nplans) throw CsdError("work-list ntt: multiplier shape");
+        for (int d = 1; d < 4; d++) smul.s[d] = mul->shape[d] == 1 ? 0 : mul->strides[d];
+        if ((mul->shape[2] != 1 && mul->shape[2] != d2) || (mul->shape[3] != 1 && mul->shape[3] != d3))
+            throw CsdError("work-list ntt: multiplier shape");
+        t = make_mod_table(mods, nplans, (int64_t)nplans);
+    }
+    if (nslices == 0) return;
+    const double bytes = 16.0 * (double)nslices * d2 * d3;
+    const bool inv = dir == 2;
+    if (dim == 2) {
+        ColKernel k = pick_col_kernel(plans[0]->host, inv);
+        if (!k) throw CsdError("work-list ntt: no column kernel for this plan");
+        for (size_t jj = 1; jj < nplans; jj++) {
+            const DevPlan &p0 = plans[0]->host, &pj = plans[jj]->host;
+            bool same = pj.nfactors == p0.nfactors;
+            for (int f = 0; same && f < p0.nfactors; f++) same = pj.f[f].len == p0.f[f].len;
+            if (!same) throw CsdError("work-list ntt: plans must share one factor list");
+        }
+        const int ct = tune.col_threads;
+        if (ct < 64 || ct > 256 || ct % 64) throw CsdError("work-list ntt: column block size must be 64, 128, 192 or 256");
+        ColArgs c;
+        memset(&c, 0, sizeof(c));
+        c.d0 = 1;
+        c.d1 = (int64_t)nplans;
+        c.d3 = d3;
+        c.st[2] = st2;
+        for (size_t jj = 0; jj < nplans; jj++) c.plans[jj] = (const DevPlan*)plans[jj]->dev;
+        c.dir = dir;
+        c.chunks = (d3 + ct - 1) / ct;
+        c.mul = mul ? (const int64_t*)mul->ptr : nullptr;
+        c.smul = smul;
+        c.seg_base = seg_base;
+        c.seg_limb = seg_limb;
+        if (nslices * c.chunks > (int64_t)INT32_MAX) throw CsdError("work-list ntt: too many tiles");
+        TimedScope ts(ctx, timer, bytes);
+        hipLaunchKernelGGL(k, dim3((unsigned)(nslices * c.chunks)), dim3(ct), 0, ctx->stream, c, t);
+        HIPCHECK(hipGetLastError());
+        return;
+    }
+    RowArgs r;
+    memset(&r, 0, sizeof(r));
+    r.d0 = 1;
+    r.d1 = (int64_t)nplans;
+    r.d2 = d2;
+    r.d3 = d3;
+    r.st[2] = st2;
+    for (size_t jj = 0; jj < nplans; jj++) r.plans[jj] = (const DevPlan*)plans[jj]->dev;
+    r.dir = dir;
+    int rows = 1;
+    if (tune.rows > 0)
+        rows = (int)std::min<int64_t>(tune.rows, d2);
+    else
+        while (rows * 2 * L <= 6144 && rows * 2 <= d2 && rows < 256) rows *= 2;
+    r.rows = rows;
+    r.magic_L = magic_for((uint32_t)L);
+    const int Ms[5] = {2, 4, 8, 3, 9};
+    for (int q = 0; q < 5; q++) r.magic_groups[q] = L % Ms[q] == 0 ? magic_for((uint32_t)(L / Ms[q])) : 0u;
+    r.chunks = (d2 + rows - 1) / rows;
+    r.mul = mul ? (const int64_t*)mul->ptr : nullptr;
+    r.smul = smul;
+    r.seg_base = seg_base;
+    r.seg_limb = seg_limb;
+    size_t lds = (size_t)rows * lds_line((uint32_t)L) * 8;
+    if (lds > 160 * 1024) throw CsdError("work-list ntt: line too long for LDS staging");
+    HIPCHECK(hipFuncSetAttribute((const void*)k_ntt_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    HIPCHECK(hipFuncSetAttribute((const void*)k_ntt_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    const int threads = tune.threads > 0 ? tune.threads : (L * rows >= 8192 ? 512 : 256);
+    if (threads % 64 || threads > 512) throw CsdError("work-list ntt: row block size must be a multiple of 64, at most 512");
+    if (nslices * r.chunks > (int64_t)INT32_MAX) throw CsdError("work-list ntt: too many tiles");
+    TimedScope ts(ctx, timer, bytes);
+    if (inv)
+        hipLaunchKernelGGL(k_ntt_rows<true>, dim3((unsigned)(nslices * r.chunks)), dim3(threads), lds, ctx->stream, r, t);
+    else
+        hipLaunchKernelGGL(k_ntt_rows<false>, dim3((unsigned)(nslices * r.chunks)), dim3(threads), lds, ctx->stream, r, t);
     HIPCHECK(hipGetLastError());
 }
 

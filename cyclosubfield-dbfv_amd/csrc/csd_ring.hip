@@ -403,6 +403,7 @@ struct TransformCache {  // ring.rs:578-592
 struct BConvCache {  // ring.rs:1596-1604
     size_t log2_qtilde, acc;
     std::unique_ptr<DevBuf> q_over_p_inv, matrix, q_mod_p;
+    std::vector<int64_t> host_qopi, host_matrix, host_qm;  // the same constants, kept for fused kernels
 };
 
 struct GadgetCache {  // ring.rs:1714-1727
@@ -616,6 +617,9 @@ struct csd_ring {
         c->q_over_p_inv = upload(ctx, qopi);
         c->matrix = upload(ctx, mat);
         c->q_mod_p = upload(ctx, qm);
+        c->host_qopi = qopi;
+        c->host_matrix = mat;
+        c->host_qm = qm;
         auto& ref = *c;
         bcache[key] = std::move(c);
         return ref;
@@ -746,10 +750,7 @@ struct csd_ring {
     }
 
     // inverse_scalar_tensor (ring.rs:1165-1177) for the scalar prod(primes at `of`)
-    DevBuf& inverse_product_tensor(const std::vector<size_t>& base, const std::vector<size_t>& of) {
-        auto key = std::make_pair(base, of);
-        auto it = inv_scalar_cache.find(key);
-        if (it != inv_scalar_cache.end()) return *it->second;
+    std::vector<int64_t> inverse_product_values(const std::vector<size_t>& base, const std::vector<size_t>& of) const {
         std::vector<int64_t> v;
         for (size_t i : base) {
             int64_t p = primes[i];
@@ -757,7 +758,14 @@ struct csd_ring {
             if (s == 0) throw CsdError("scalar is not invertible");
             v.push_back(mont_repr(smallest_lift(inv_mod_u(s, (uint64_t)p), p), p));
         }
-        auto b = upload(ctx, v);
+        return v;
+    }
+
+    DevBuf& inverse_product_tensor(const std::vector<size_t>& base, const std::vector<size_t>& of) {
+        auto key = std::make_pair(base, of);
+        auto it = inv_scalar_cache.find(key);
+        if (it != inv_scalar_cache.end()) return *it->second;
+        auto b = upload(ctx, inverse_product_values(base, of));
         auto& ref = *b;
         inv_scalar_cache[key] = std::move(b);
         return ref;
@@ -1127,5 +1135,362 @@ extern "C" int csd_ring_scalar_tensor(csd_ring* ring, const csd_view* dst, const
         v.push_back(mont_repr(scalar % p, p));  // ring.rs:876-888
     }
     ok(csd_hal_import(ring->ctx, dst, v.data(), v.size()));
+    CSD_CATCH
+}
+
+// ---------------------------------------------------------------------------------------------
+// batched uneven chain: pb_to_crt -> mul_mont64 by a CRT-basis plaintext -> crt_to_pb ->
+// mod_switch_down by one limb, for ciphertexts at different levels. Every (element, limb) slice
+// of every ciphertext is one entry of a work list; each of the eight transform phases is one
+// launch over the whole list, the plaintext product rides on the store of the fourth, and the
+// mod-switch is one launch over all elements. The phases can be replayed as one hipGraph.
+
+struct csd_batch_plan {
+    csd_ring* ring = nullptr;
+    std::vector<size_t> base;
+    std::vector<size_t> limbs;
+    TransformCache* tc = nullptr;
+    csd_view pt;
+    int64_t n2 = 0, n3 = 0, nslices = 0, nelems = 0;
+    std::unique_ptr<DevBuf> tables;
+    int64_t** seg_base = nullptr;
+    int32_t* seg_limb = nullptr;
+    int64_t** ms_in = nullptr;
+    int64_t** ms_out = nullptr;
+    int32_t* ms_level = nullptr;
+    void* levels = nullptr;
+    csd_batch_tuning tune{0, 0, 256, 3, 2};
+    hipGraphExec_t graph = nullptr;
+
+    ~csd_batch_plan() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
+
+    double chain_bytes() const {
+        // eight transform passes over every slice (read + write each value once), the plaintext
+        // read of the fourth, and the mod-switch (read r limbs, write r - 1) per element
+        const double slice = 8.0 * n2 * n3;
+        double b = 8 * 2 * slice * nslices + slice * nslices;
+        for (size_t c = 0; c < limbs.size(); c++) b += 2.0 * slice * (2 * limbs[c] - 1);
+        return b;
+    }
+
+    CsdSegTune seg_tune() const {
+        CsdSegTune t;
+        t.rows = tune.rows;
+        t.threads = tune.threads;
+        t.col_threads = tune.col_threads;
+        return t;
+    }
+
+    // the chain on the stream; ev (optional) receives an event before each phase and one after
+    void enqueue(hipEvent_t* ev = nullptr) {
+        csd_ctx* ctx = ring->ctx;
+        const int64_t R = (int64_t)base.size();
+        const CsdSegTune t = seg_tune();
+        csd_view klp = make_view(tc->k_lp->p, 1, R, n2, 1), kk = make_view(tc->k_k->p, 1, R, 1, n3);
+        csd_view iklp = make_view(tc->ik_lp->p, 1, R, n2, 1), ikk = make_view(tc->ik_k->p, 1, R, 1, n3);
+        csd_view ptv = pt;
+        ptv.shape[1] = R;
+        const csd_modulus* m = tc->mods.data();
+        auto phase = [&](int q) {
+            if (ev) HIPCHECK(hipEventRecord(ev[q], ctx->stream));
+        };
+        // pb_to_crt (ring.rs:1118-1141)
+        phase(0);
+        csd_ntt_segments(ctx, seg_base, seg_limb, nslices, n2, n3, n3, tc->plans_lp.data(), R, 2, 1, &klp, m, "batch_ntt_mul_dim2", t);
+        phase(1);
+        csd_ntt_segments(ctx, seg_base, seg_limb, nslices, n2, n3, n3, tc->plans_k.data(), R, 3, 1, &kk, m, "batch_ntt_mul_dim3", t);
+        phase(2);
+        csd_ntt_segments(ctx, seg_base, seg_limb, nslices, n2, n3, n3, tc->plans_lp.data(), R, 2, 2, nullptr, m, "batch_intt_dim2", t);
+        phase(3);
+        // intt along dim 3, then the plaintext product (hal mul_mont64) on the way out
+        csd_ntt_segments(ctx, seg_base, seg_limb, nslices, n2, n3, n3, tc->plans_k.data(), R, 3, 2, &ptv, m, "batch_intt_mul_dim3", t);
+        // crt_to_pb (ring.rs:1144-1161)
+        phase(4);
+        csd_ntt_segments(ctx, seg_base, seg_limb, nslices, n2, n3, n3, tc->plans_lp.data(), R, 2, 1, &iklp, m, "batch_ntt_mul_dim2", t);
+        phase(5);
+        csd_ntt_segments(ctx, seg_base, seg_limb, nslices, n2, n3, n3, tc->plans_k.data(), R, 3, 1, &ikk, m, "batch_ntt_mul_dim3", t);
+        phase(6);
+        csd_ntt_segments(ctx, seg_base, seg_limb, nslices, n2, n3, n3, tc->plans_lp.data(), R, 2, 2, nullptr, m, "batch_intt_dim2", t);
+        phase(7);
+        csd_ntt_segments(ctx, seg_base, seg_limb, nslices, n2, n3, n3, tc->plans_k.data(), R, 3, 2, nullptr, m, "batch_intt_dim3", t);
+        // mod_switch_down by one limb (bfv.rs:628-662)
+        phase(8);
+        double msb = 0;
+        for (size_t c = 0; c < limbs.size(); c++) msb += 2.0 * 8.0 * n2 * n3 * (2 * limbs[c] - 1);
+        csd_mod_switch_one_launch(ctx, ms_in, ms_out, ms_level, levels, nelems, n2 * n3, m, (size_t)R, msb);
+        phase(9);
+    }
+
+    void drop_graph() {
+        if (graph) {
+            HIPCHECK(hipStreamSynchronize(ring->ctx->stream));
+            HIPCHECK(hipGraphExecDestroy(graph));
+            graph = nullptr;
+        }
+    }
+
+    void capture() {
+        csd_ctx* ctx = ring->ctx;
+        const bool timing = ctx->timing;
+        ctx->timing = false;  // no event records inside the captured region
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        try {
+            enqueue();
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            ctx->timing = timing;
+            throw;
+        }
+        ctx->timing = timing;
+        HIPCHECK(hipStreamEndCapture(ctx->stream, &g));
+        hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(e);
+    }
+
+    void run(bool use_graph) {
+        if (nslices == 0) return;
+        if (!use_graph) {
+            enqueue();
+            return;
+        }
+        if (!graph) capture();
+        TimedScope ts(ring->ctx, "batch_chain_graph", chain_bytes());
+        HIPCHECK(hipGraphLaunch(graph, ring->ctx->stream));
+    }
+
+    void set_tuning(const csd_batch_tuning& t) {
+        if (t.g2 < 1 || t.g2 > 3 || t.g3 < 1 || t.g3 > 2) throw CsdError("tuning: 1 <= g2 <= 3 and 1 <= g3 <= 2");
+        if (t.col_threads < 64 || t.col_threads > 256 || t.col_threads % 64) throw CsdError("tuning: col_threads");
+        if (t.threads < 0 || t.threads > 512 || t.threads % 64) throw CsdError("tuning: threads");
+        if (t.rows < 0 || t.rows > 16) throw CsdError("tuning: rows");
+        if (t.rows > 0 && (size_t)t.rows * (n3 + n3 / 8 + 1) * 8 > 160 * 1024) throw CsdError("tuning: rows exceed LDS");
+        drop_graph();
+        if (t.g2 != tune.g2 || t.g3 != tune.g3) {
+            for (auto p : tc->plans_lp) ok(csd_ntt_plan_set_grouping(p, t.g2, t.g3));
+            for (auto p : tc->plans_k) ok(csd_ntt_plan_set_grouping(p, t.g2, t.g3));
+        }
+        tune = t;
+    }
+
+    // chain time of the current tuning (ms, best of reps) and the per-phase split
+    double time_chain(int reps, double* phase_ms) {
+        csd_ctx* ctx = ring->ctx;
+        hipEvent_t ev[10];
+        for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+        double best = 1e300;
+        std::vector<double> best_phase(9, 0.0);
+        try {
+            enqueue();  // warm-up
+            for (int q = 0; q < reps; q++) {
+                enqueue(ev);
+                HIPCHECK(hipStreamSynchronize(ctx->stream));
+                float tot = 0;
+                HIPCHECK(hipEventElapsedTime(&tot, ev[0], ev[9]));
+                if (tot < best) {
+                    best = tot;
+                    for (int k = 0; k < 9; k++) {
+                        float f = 0;
+                        HIPCHECK(hipEventElapsedTime(&f, ev[k], ev[k + 1]));
+                        best_phase[k] = f;
+                    }
+                }
+            }
+        } catch (...) {
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        if (phase_ms)
+            for (int k = 0; k < 9; k++) phase_ms[k] = best_phase[k];
+        return best;
+    }
+
+    // coordinate search: dim-3 launch shape, then pass grouping, then the dim-2 block size.
+    // Every candidate computes the same values; only the time differs.
+    double autotune(int reps) {
+        const bool timing = ring->ctx->timing;
+        ring->ctx->timing = false;
+        auto dim3_ms = [](const double* ph) { return ph[1] + ph[3] + ph[5] + ph[7]; };
+        auto dim2_ms = [](const double* ph) { return ph[0] + ph[2] + ph[4] + ph[6]; };
+        double ph[9];
+        csd_batch_tuning best = tune;
+        try {
+            set_tuning(best);
+            time_chain(reps, ph);
+            double best3 = dim3_ms(ph);
+            const size_t line = (size_t)(n3 + n3 / 8 + 1) * 8;
+            for (int rows : {1, 2, 3, 4, 6, 8})
+                for (int threads : {128, 256, 384, 512}) {
+                    if ((size_t)rows * line > 160 * 1024 || rows > n2) continue;
+                    if (threads > rows * n3) continue;
+                    csd_batch_tuning c = best;
+                    c.rows = rows;
+                    c.threads = threads;
+                    set_tuning(c);
+                    time_chain(reps, ph);
+                    if (dim3_ms(ph) < best3) {
+                        best3 = dim3_ms(ph);
+                        best = c;
+                    }
+                }
+            for (int g2 = 1; g2 <= 3; g2++)
+                for (int g3 = 1; g3 <= 2; g3++) {
+                    csd_batch_tuning c = best;
+                    c.g2 = g2;
+                    c.g3 = g3;
+                    set_tuning(c);
+                    time_chain(reps, ph);
+                    if (dim3_ms(ph) < best3) {
+                        best3 = dim3_ms(ph);
+                        best = c;
+                    }
+                }
+            set_tuning(best);
+            time_chain(reps, ph);
+            double best2 = dim2_ms(ph);
+            for (int ct : {64, 128, 192, 256}) {
+                csd_batch_tuning c = best;
+                c.col_threads = ct;
+                set_tuning(c);
+                time_chain(reps, ph);
+                if (dim2_ms(ph) < best2) {
+                    best2 = dim2_ms(ph);
+                    best = c;
+                }
+            }
+            set_tuning(best);
+        } catch (...) {
+            ring->ctx->timing = timing;
+            throw;
+        }
+        ring->ctx->timing = timing;
+        return time_chain(reps, nullptr);
+    }
+};
+
+static csd_batch_plan* batch_create(csd_ring* ring, const std::vector<size_t>& base, size_t count, int64_t* const* cts,
+                                    int64_t* const* outs, const size_t* limbs, const csd_view& pt) {
+    const Field& F = *ring->f;
+    const int64_t R = (int64_t)base.size();
+    if (R < 2) throw CsdError("batch: the base needs at least two primes");
+    if (R > CSD_MAX_MODULI) throw CsdError("batch: base too long");
+    auto plan = std::make_unique<csd_batch_plan>();
+    plan->ring = ring;
+    plan->base = base;
+    plan->n2 = (int64_t)F.deg_lp;
+    plan->n3 = (int64_t)F.deg_k;
+    check_view(&pt, "batch plaintext");
+    if (pt.shape[0] != 1 || pt.shape[1] < R || pt.shape[2] != plan->n2 || pt.shape[3] != plan->n3)
+        throw CsdError("batch: plaintext must be [1, >= r, deg_L', deg_K]");
+    if (pt.strides[3] != 1 || pt.strides[2] != plan->n3 || pt.strides[1] != plan->n2 * plan->n3)
+        throw CsdError("batch: plaintext must be contiguous per limb");
+    plan->pt = pt;
+    plan->tc = &ring->transform_cache(base);
+    const int64_t npos = plan->n2 * plan->n3;
+    std::vector<int64_t*> sb, msi, mso;
+    std::vector<int32_t> sl, lvl;
+    for (size_t c = 0; c < count; c++) {
+        const size_t r = limbs[c];
+        if (r < 2 || (int64_t)r > R) throw CsdError("batch: limb counts must lie in [2, r]");
+        if (!cts[c] || !outs[c]) throw CsdError("batch: null ciphertext pointer");
+        plan->limbs.push_back(r);
+        for (int64_t e = 0; e < 2; e++) {
+            for (size_t j = 0; j < r; j++) {
+                sb.push_back(cts[c] + (e * (int64_t)r + (int64_t)j) * npos);
+                sl.push_back((int32_t)j);
+            }
+            msi.push_back(cts[c] + e * (int64_t)r * npos);
+            mso.push_back(outs[c] + e * (int64_t)(r - 1) * npos);
+            lvl.push_back((int32_t)r);
+        }
+    }
+    plan->nslices = (int64_t)sb.size();
+    plan->nelems = (int64_t)msi.size();
+    // per-level constants of the one-limb BConvCache (ring.rs:1606-1661) and of
+    // inverse_scalar_tensor (ring.rs:1165-1177), taken from the ring's own caches
+    const size_t lvb = csd_mod_switch_level_bytes();
+    std::vector<char> lv(lvb * (size_t)(R + 1), 0);
+    for (int64_t r = 2; r <= R; r++) {
+        std::vector<size_t> kept(base.begin(), base.begin() + r - 1), dropped{base[(size_t)r - 1]};
+        BConvCache& bc = ring->bconv_cache(dropped, kept);
+        if (bc.acc < 1) throw CsdError("batch: bconv accumulation bound");
+        std::vector<int64_t> M(bc.host_matrix.begin(), bc.host_matrix.begin() + (r - 1));
+        const int64_t C = bc.host_matrix[(size_t)r - 1];
+        std::vector<int64_t> inv = ring->inverse_product_values(kept, dropped);
+        csd_mod_switch_fill_level(lv.data() + lvb * (size_t)r, (int)r, M.data(), bc.host_qm.data(), inv.data(), C,
+                                  (int)bc.log2_qtilde, bc.host_qopi[0]);
+    }
+    // one device table block: slice pointers and limbs, element pointers and levels, constants
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_sb = 0, o_sl = al(o_sb + sb.size() * 8), o_mi = al(o_sl + sl.size() * 4), o_mo = al(o_mi + msi.size() * 8),
+                 o_lv = al(o_mo + mso.size() * 8), o_c = al(o_lv + lvl.size() * 4), total = al(o_c + lv.size());
+    std::vector<char> blob(total, 0);
+    if (!sb.empty()) memcpy(blob.data() + o_sb, sb.data(), sb.size() * 8);
+    if (!sl.empty()) memcpy(blob.data() + o_sl, sl.data(), sl.size() * 4);
+    if (!msi.empty()) memcpy(blob.data() + o_mi, msi.data(), msi.size() * 8);
+    if (!mso.empty()) memcpy(blob.data() + o_mo, mso.data(), mso.size() * 8);
+    if (!lvl.empty()) memcpy(blob.data() + o_lv, lvl.data(), lvl.size() * 4);
+    memcpy(blob.data() + o_c, lv.data(), lv.size());
+    csd_ctx* ctx = ring->ctx;
+    plan->tables = std::make_unique<DevBuf>(ctx, total);
+    char* d = (char*)plan->tables->p;
+    HIPCHECK(hipMemcpyAsync(d, blob.data(), total, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    plan->seg_base = (int64_t**)(d + o_sb);
+    plan->seg_limb = (int32_t*)(d + o_sl);
+    plan->ms_in = (int64_t**)(d + o_mi);
+    plan->ms_out = (int64_t**)(d + o_mo);
+    plan->ms_level = (int32_t*)(d + o_lv);
+    plan->levels = d + o_c;
+    // the plans of the transform cache may carry a grouping from an earlier batch plan
+    for (auto p : plan->tc->plans_lp) ok(csd_ntt_plan_set_grouping(p, plan->tune.g2, plan->tune.g3));
+    for (auto p : plan->tc->plans_k) ok(csd_ntt_plan_set_grouping(p, plan->tune.g2, plan->tune.g3));
+    return plan.release();
+}
+
+extern "C" int csd_ring_batch_create(csd_ring* ring, const size_t* base, size_t r, size_t count, int64_t* const* cts,
+                                     int64_t* const* outs, const size_t* limbs, const csd_view* pt, csd_batch_plan** out) {
+    CSD_TRY if (!out || !pt || (count && (!cts || !outs || !limbs))) throw CsdError("batch: null argument");
+    *out = batch_create(ring, to_vec(base, r), count, cts, outs, limbs, *pt);
+    CSD_CATCH
+}
+
+extern "C" int csd_batch_run(csd_batch_plan* plan, int use_graph) {
+    CSD_TRY plan->run(use_graph != 0);
+    CSD_CATCH
+}
+
+extern "C" int csd_batch_autotune(csd_batch_plan* plan, int reps, csd_batch_tuning* chosen, double* ms) {
+    CSD_TRY if (reps < 1) throw CsdError("autotune: reps must be positive");
+    double t = plan->autotune(reps);
+    if (chosen) *chosen = plan->tune;
+    if (ms) *ms = t;
+    CSD_CATCH
+}
+
+extern "C" int csd_batch_set_tuning(csd_batch_plan* plan, const csd_batch_tuning* t) {
+    CSD_TRY plan->set_tuning(*t);
+    CSD_CATCH
+}
+
+extern "C" int csd_batch_get_tuning(const csd_batch_plan* plan, csd_batch_tuning* t) {
+    CSD_TRY *t = plan->tune;
+    CSD_CATCH
+}
+
+extern "C" int csd_batch_info(const csd_batch_plan* plan, int64_t* nslices, int64_t* nelems, double* bytes) {
+    CSD_TRY if (nslices) *nslices = plan->nslices;
+    if (nelems) *nelems = plan->nelems;
+    if (bytes) *bytes = plan->chain_bytes();
+    CSD_CATCH
+}
+
+extern "C" int csd_batch_free(csd_batch_plan* plan) {
+    CSD_TRY delete plan;
     CSD_CATCH
 }

@@ -145,8 +145,91 @@ class RingContext:
         check(lib().csd_ring_reduce_wide(self.handle, ctypes.byref(dst.view()), ctypes.byref(wide.view()), b, n,
                                          ctypes.c_int(1 if full else 0)))
 
+    def batch_plan(self, base, cts, outs, pt):
+        """A BatchPlan for ciphertexts at the levels len(cts[c][0]) over prefixes of `base`."""
+        return BatchPlan(self, base, cts, outs, pt)
+
     def scalar_tensor(self, base, scalar):
         t = self.backend.hal_allocate_i64([1, len(base), 1, 1])
         b, n = _sizes(base)
         check(lib().csd_ring_scalar_tensor(self.handle, ctypes.byref(t.view()), b, n, ctypes.c_int64(scalar)))
         return t
+
+
+class BatchTuning(ctypes.Structure):
+    """Launch shapes of the batched chain (csd_batch_tuning)."""
+    _fields_ = [("rows", ctypes.c_int), ("threads", ctypes.c_int), ("col_threads", ctypes.c_int), ("g2", ctypes.c_int),
+                ("g3", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchPlan:
+    """Ciphertexts at uneven levels through pb_to_crt_tensor (ring.rs:1118-1141), a CRT-basis
+    plaintext product (hal mul_mont64), crt_to_pb_tensor (ring.rs:1144-1161) and
+    mod_switch_down by one limb (bfv.rs:628-662), each phase one work-list launch over all
+    (element, limb) slices; see csd_ring_batch_create. ``cts[c]`` is a contiguous
+    [2, limbs[c], deg_L', deg_K] tensor over base[:limbs[c]] (overwritten with the crt_to_pb
+    result), ``outs[c]`` receives the [2, limbs[c] - 1, ., .] mod-switched result."""
+
+    def __init__(self, ring, base, cts, outs, pt):
+        self.ring = ring
+        self.handle = None
+        self._keep = (list(cts), list(outs), pt)
+        limbs = [c.shape[1] for c in cts]
+        n2, n3 = ring.info["deg_lprime"], ring.info["deg_k"]
+        if len(outs) != len(cts):
+            raise ValueError("batch: one output per ciphertext")
+        for c, o, r in zip(cts, outs, limbs):
+            if c.shape != (2, r, n2, n3) or o.shape != (2, r - 1, n2, n3):
+                raise ValueError("batch: ciphertext or output shape")
+            if c.strides != (r * n2 * n3, n2 * n3, n3, 1) or o.strides != ((r - 1) * n2 * n3, n2 * n3, n3, 1):
+                raise ValueError("batch: ciphertexts and outputs must be contiguous")
+        n = len(cts)
+        P = ctypes.POINTER(ctypes.c_int64)
+        carr = (P * max(n, 1))(*[ctypes.cast(ctypes.c_void_p(c.ptr), P) for c in cts])
+        oarr = (P * max(n, 1))(*[ctypes.cast(ctypes.c_void_p(o.ptr), P) for o in outs])
+        larr = (ctypes.c_size_t * max(n, 1))(*limbs)
+        b, nb = _sizes(base)
+        h = ctypes.c_void_p()
+        check(lib().csd_ring_batch_create(ring.handle, b, nb, ctypes.c_size_t(n), carr, oarr, larr,
+                                          ctypes.byref(pt.view()), ctypes.byref(h)))
+        self.handle = h
+
+    def run(self, graph=True):
+        check(lib().csd_batch_run(self.handle, ctypes.c_int(1 if graph else 0)))
+
+    def autotune(self, reps=3):
+        t = BatchTuning()
+        ms = ctypes.c_double()
+        check(lib().csd_batch_autotune(self.handle, ctypes.c_int(reps), ctypes.byref(t), ctypes.byref(ms)))
+        return t.as_dict(), ms.value
+
+    def set_tuning(self, **kw):
+        t = BatchTuning()
+        check(lib().csd_batch_get_tuning(self.handle, ctypes.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(lib().csd_batch_set_tuning(self.handle, ctypes.byref(t)))
+
+    def tuning(self):
+        t = BatchTuning()
+        check(lib().csd_batch_get_tuning(self.handle, ctypes.byref(t)))
+        return t.as_dict()
+
+    def info(self):
+        ns, ne, b = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        check(lib().csd_batch_info(self.handle, ctypes.byref(ns), ctypes.byref(ne), ctypes.byref(b)))
+        return {"slices": ns.value, "elements": ne.value, "bytes": b.value}
+
+    def close(self):
+        if self.handle:
+            check(lib().csd_batch_free(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -111,6 +111,10 @@ int csd_ntt_plan_free(csd_ntt_plan* plan);
 /* modular multiplications one transform of the plan performs (dir 0: forward reduced, 1: forward
  * unreduced, 2: inverse), following its reduction schedule and twiddle-1 skips */
 int csd_ntt_plan_mults(const csd_ntt_plan* plan, int dir, uint64_t* out);
+/* LDS pass grouping of the register-blocked kernels (no reference counterpart; a tuning knob):
+ * at most g2 radix-2 (1..3) or g3 radix-3 (1..2) stages per pass. The arithmetic, and so every
+ * result, is the same for all groupings; default 3 and 2. */
+int csd_ntt_plan_set_grouping(csd_ntt_plan* plan, int g2, int g3);
 /* hal_ntt_mont64_dim / hal_intt_mont64_dim / hal_ntt_mul_mont64_dim (mod.rs:216-238);
  * plans[j] is the plan for dimension-1 index j; dim is CSD_DIM2 or CSD_DIM3. */
 int csd_hal_ntt_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim);

@@ -89,6 +89,40 @@ int csd_ring_reduce_wide(csd_ring* ring, const csd_view* dst, const csd_view* wi
  * product of the primes at `scalar_base` */
 int csd_ring_scalar_tensor(csd_ring* ring, const csd_view* dst, const size_t* base, size_t r, int64_t scalar);
 
+/* ---- batched uneven chain ------------------------------------------------------------------ */
+/* The reference runs every ring operation on one RingEls with one RNS base (ring.rs:198-208), so
+ * ciphertexts at different levels after dynamic scaling take one call sequence each. A batch plan
+ * takes `count` ciphertexts at once: ciphertext c is the contiguous [2, limbs[c], deg_L', deg_K]
+ * tensor at cts[c] over the base prefix base[0 .. limbs[c]), 2 <= limbs[c] <= r, and the plan runs
+ * per ciphertext exactly
+ *   pb_to_crt_tensor (ring.rs:1118-1141), hal mul_mont64 by pt[:, 0 .. limbs[c]) (cpu.rs mul_mont64),
+ *   crt_to_pb_tensor (ring.rs:1144-1161), mod_switch_down by one limb (bfv.rs:628-662)
+ * writing the [2, limbs[c] - 1, ., .] result to outs[c]; cts[c] is overwritten with the
+ * crt_to_pb output. pt is a [1, >= max limbs, deg_L', deg_K] CRT-basis tensor. All slices of all
+ * ciphertexts go through each phase in one work-list launch; the results are bit-identical to
+ * the per-ciphertext calls. The plan keeps the pointers: cts, outs and pt must stay allocated
+ * while it is used. */
+typedef struct csd_batch_plan csd_batch_plan;
+/* launch shapes: dim-3 lines per workgroup and workgroup size (0 = built-in choice), dim-2
+ * columns per workgroup (64..256), NTT pass grouping (radix-2 stages g2 in 1..3, radix-3 stages
+ * g3 in 1..2 per LDS pass; applies to the plans of the whole base) */
+typedef struct csd_batch_tuning {
+    int rows, threads, col_threads, g2, g3;
+} csd_batch_tuning;
+int csd_ring_batch_create(csd_ring* ring, const size_t* base, size_t r, size_t count, int64_t* const* cts,
+                          int64_t* const* outs, const size_t* limbs, const csd_view* pt, csd_batch_plan** out);
+/* runs the chain on the context stream; use_graph = 1 replays a hipGraph of the phases, captured
+ * on the first such call (and again after a tuning change) */
+int csd_batch_run(csd_batch_plan* plan, int use_graph);
+/* times launch-shape candidates on the plan's own data (which it overwrites), keeps the fastest
+ * and reports it with its chain time in ms */
+int csd_batch_autotune(csd_batch_plan* plan, int reps, csd_batch_tuning* chosen, double* ms);
+int csd_batch_set_tuning(csd_batch_plan* plan, const csd_batch_tuning* t);
+int csd_batch_get_tuning(const csd_batch_plan* plan, csd_batch_tuning* t);
+/* slices (element, limb) and elements in the work list; algorithmic HBM bytes of one chain */
+int csd_batch_info(const csd_batch_plan* plan, int64_t* nslices, int64_t* nelems, double* bytes);
+int csd_batch_free(csd_batch_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif
