@@ -285,6 +285,110 @@ def bench_batch(pkg, be, steps, count=256, seed=SEED, cpu=True):
     return res
 
 
+MFMA_INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
+
+
+def bench_zp(pkg, be, steps, nvec=512, seed=SEED, cpu=True):
+    """The dense K-part matrix of the period->CRT transform at field_36864 (the 3072 x 3072
+    group-circulant of the CRT kernel, one per prime of the 17-prime base) applied to `nvec`
+    vectors of 3072 residues per prime, exactly: the int8-digit MFMA contraction, the plain VALU
+    contraction, and the NTT path (ntt_mul + intt along dimension 3) for the same map."""
+    ring = pkg.ring.RingContext(be, FIELD, LOG2_Q, T_BITS)
+    base = ring.initial_rns_base()
+    R = len(base)
+    n = ring.info["deg_k"]
+    primes = [ring.primes[i] for i in base]
+    mods = [be.hal_create_modulus(p) for p in primes]
+    t0 = time.perf_counter()
+    a = ring.transform_matrix(base, 3)
+    mat = be.zp_matrix(a, mods)
+    be.sync()
+    setup_s = time.perf_counter() - t0
+    del a
+    x = be.hal_allocate_i64((1, R, nvec, n))
+    ring.fill_uniform(x, base, seed + 7)
+    y_mfma = be.hal_allocate_i64(x.shape)
+    y_plain = be.hal_allocate_i64(x.shape)
+    stream = be.stream()
+    ev = HipEvents()
+    be.zp_apply_dim(mat, y_mfma, x, 3, form="mfma")  # warm-up
+    be.sync()
+    be.set_timing(True)
+    ev.start(stream)
+    for _ in range(steps):
+        be.zp_apply_dim(mat, y_mfma, x, 3, form="mfma")
+    mfma_ms = ev.stop(stream) / steps
+    be.set_timing(False)
+    kern = be.timing_report()
+    plain_steps = max(1, steps // 4)
+    ev.start(stream)
+    for _ in range(plain_steps):
+        be.zp_apply_dim(mat, y_plain, x, 3, form="plain")
+    plain_ms = ev.stop(stream) / plain_steps
+    ym = be.hal_export(y_mfma)
+    if not (ym == be.hal_export(y_plain)).all():
+        raise SystemExit("bench: MFMA contraction differs from the plain kernel")
+    # the NTT path for the same map: ntt_mul along dim 3 by the NTT-domain CRT kernel, then intt
+    orders = ring.info["orders"][ring.info["lprime_factor_count"]:]
+    plans = [be.hal_create_ntt_plan(m, orders) for m in mods]
+    kk = np.stack([ring.crt_kernels(i)[1] for i in base]).reshape(1, R, 1, n)
+    kdev = be.hal_allocate_i64(kk.shape)
+    be.hal_import(kdev, kk)
+    xn = be.hal_allocate_i64(x.shape)
+    be.hal_assign(xn, x)
+    be.hal_ntt_mul_mont64_dim(xn, plans, 3, kdev, mods)
+    be.hal_intt_mont64_dim(xn, plans, 3)
+    pv = np.array(primes, dtype=np.int64).reshape(-1, 1)
+    if not (np.mod(be.hal_export(xn).reshape(R, -1), pv) == ym.reshape(R, -1)).all():
+        raise SystemExit("bench: MFMA contraction differs from the NTT transform mod p")
+    ev.start(stream)
+    for _ in range(steps):
+        be.hal_ntt_mul_mont64_dim(xn, plans, 3, kdev, mods)
+        be.hal_intt_mont64_dim(xn, plans, 3)
+    ntt_ms = ev.stop(stream) / steps
+    zp_macs = R * n * n * nvec
+    int8_ops = 2 * 64 * zp_macs
+    mk = kern["zp_mfma"]
+    mfma_kernel_ms = mk[1] / mk[0]
+    achieved = int8_ops / (mfma_kernel_ms / 1000.0) / 1e12
+    res = {
+        "workload": f"field_36864 dense K-part transform matrix ({n} x {n} group-circulant per prime, {R} primes) "
+                    f"applied to {nvec} vectors per prime, exact over Z_p",
+        "value": round(R * nvec / (mfma_ms / 1000.0), 1), "unit": "vector-limb maps/s", "ms_per_step": round(mfma_ms, 3),
+        "plain_valu": {"ms_per_step": round(plain_ms, 3), "value": round(R * nvec / (plain_ms / 1000.0), 1)},
+        "ntt_path": {"ms_per_step": round(ntt_ms, 3), "value": round(R * nvec / (ntt_ms / 1000.0), 1),
+                     "note": "the O(n log n) NTT form of the same map (equal mod p)"},
+        "speedup_vs_plain_valu": round(plain_ms / mfma_ms, 2),
+        "zp_macs_per_s": round(zp_macs / (mfma_ms / 1000.0) / 1e12, 3), "zp_macs_unit": "T Z_p-MAC/s",
+        "roofline": {"bound": "mfma", "kernel": "zp_mfma", "achieved": round(achieved, 1), "peak": MFMA_INT8_PEAK_TOPS,
+                     "unit": "TOPS int8", "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 4),
+                     "ops_per_launch": int8_ops, "avg_launch_ms": round(mfma_kernel_ms, 4)},
+        "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in kern.items()},
+        "setup_s": round(setup_s, 2),
+    }
+    if cpu:
+        from oracle import cpu_backend
+        from oracle import ring as oring
+
+        cb = cpu_backend.CpuBackend()
+        orc = oring.RingContext(cb, oring.FieldDescriptor(FIELD), LOG2_Q, T_BITS)
+        cache = orc.transform_cache(base)
+        sample = 12
+        hx = be.hal_export(x[:, :, 0:sample]).reshape(1, R, sample, n)
+        t0 = time.perf_counter()
+        cb.hal_ntt_mul_mont64_dim(hx, cache["ntts_k"], 3, cache["crt_kernel_k"], cache["mods"])
+        cb.hal_intt_mont64_dim(hx, cache["ntts_k"], 3)
+        dt = time.perf_counter() - t0
+        if not (np.mod(hx.reshape(R, -1), pv) == ym.reshape(R, nvec, n)[:, :sample].reshape(R, -1)).all():
+            raise SystemExit("bench: MFMA contraction differs from the oracle")
+        res["cpu_baseline"] = {"value": round(R * sample / dt, 1), "unit": "vector-limb maps/s", "cores": 1, "kind": "port",
+                               "sample": f"{sample} vectors x {R} primes through the oracle's NTT form ({dt:.2f} s), "
+                                         "also checked equal mod p"}
+    del mat, x, y_mfma, y_plain, xn
+    ring.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,8 +397,8 @@ def main():
     ap.add_argument("--count", type=int, default=128, help="ring elements in the batch (2 per ciphertext)")
     ap.add_argument("--cpu-sample", type=int, default=4, help="ring elements in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="batch,galois",
-                    help="comma-separated extra workloads beside the headline transforms: batch, galois, or none")
+    ap.add_argument("--workloads", default="batch,zp,galois",
+                    help="comma-separated extra workloads beside the headline transforms: batch, zp, galois, or none")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -401,6 +505,8 @@ def main():
     workloads = {}
     if "batch" in args.workloads:
         workloads["batch_uneven_chain"] = bench_batch(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
+    if "zp" in args.workloads:
+        workloads["dense_zp_mfma"] = bench_zp(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
     if "galois" in args.workloads:
         workloads["galois_keyswitch"] = bench_galois(pkg, be, max(1, args.steps // 5), cpu=not args.no_cpu)
     if workloads:

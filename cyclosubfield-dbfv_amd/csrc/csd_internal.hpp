@@ -97,6 +97,9 @@ size_t csd_mod_switch_level_bytes();
 void csd_mod_switch_fill_level(void* dst, int r, const int64_t* M, const int64_t* qm, const int64_t* inv, int64_t C, int lq,
                                int64_t qopi);
 
+// out[l][a][b] = kern[l][a - b] over the group C_o0 x ... (row-major mixed-radix indices)
+void csd_circulant_launch(csd_ctx* ctx, const int64_t* kern, int64_t* out, int64_t r, const std::vector<size_t>& orders);
+
 struct Shape4 {
     int64_t d[4];
 };

@@ -1494,3 +1494,41 @@ extern "C" int csd_batch_free(csd_batch_plan* plan) {
     CSD_TRY delete plan;
     CSD_CATCH
 }
+
+// The dense matrix of the period->CRT transform (inverse = 0) or of CRT->period (inverse = 1)
+// along one dimension, per limb of `base`: pb_to_crt_tensor (ring.rs:1118-1141) is the
+// convolution over the cyclic group of each dimension with the CRT kernel, so the matrix is the
+// group-circulant C[a][b] = k[a - b]. k is read off the device transform itself, as the image of
+// the unit vector e_0 under that dimension's ntt_mul + intt pair.
+extern "C" int csd_ring_transform_matrix(csd_ring* ring, const size_t* base, size_t r, int dim, int inverse, const csd_view* out) {
+    CSD_TRY check_view(out, "transform matrix");
+    if (dim != 2 && dim != 3) throw CsdError("transform matrix: dim must be 2 or 3");
+    std::vector<size_t> b = to_vec(base, r);
+    TransformCache& c = ring->transform_cache(b);
+    const Field& F = *ring->f;
+    auto orders = F.orders();
+    std::vector<size_t> o(dim == 2 ? orders.begin() : orders.begin() + F.lp_count,
+                          dim == 2 ? orders.begin() + F.lp_count : orders.end());
+    const int64_t n = dim == 2 ? (int64_t)F.deg_lp : (int64_t)F.deg_k, R = (int64_t)r;
+    if (out->shape[0] != 1 || out->shape[1] != R || out->shape[2] != n || out->shape[3] != n)
+        throw CsdError("transform matrix: output must be [1, r, n, n]");
+    if (out->strides[2] != n || out->strides[1] != n * n) throw CsdError("transform matrix: output must be contiguous");
+    csd_ctx* ctx = ring->ctx;
+    DevBuf kv(ctx, (size_t)R * n * 8);
+    std::vector<int64_t> delta((size_t)(R * n), 0);
+    for (int64_t l = 0; l < R; l++) delta[(size_t)(l * n)] = 1;
+    csd_view v = dim == 2 ? make_view(kv.p, 1, R, n, 1) : make_view(kv.p, 1, R, 1, n);
+    ok(csd_hal_import(ctx, &v, delta.data(), delta.size()));
+    if (dim == 2) {
+        csd_view kl = make_view(inverse ? c.ik_lp->p : c.k_lp->p, 1, R, n, 1);
+        ok(csd_hal_ntt_mul_mont64_dim(ctx, &v, c.plans_lp.data(), R, CSD_DIM2, &kl, c.mods.data(), R));
+        ok(csd_hal_intt_mont64_dim(ctx, &v, c.plans_lp.data(), R, CSD_DIM2));
+    } else {
+        csd_view kk = make_view(inverse ? c.ik_k->p : c.k_k->p, 1, R, 1, n);
+        ok(csd_hal_ntt_mul_mont64_dim(ctx, &v, c.plans_k.data(), R, CSD_DIM3, &kk, c.mods.data(), R));
+        ok(csd_hal_intt_mont64_dim(ctx, &v, c.plans_k.data(), R, CSD_DIM3));
+    }
+    csd_circulant_launch(ctx, (const int64_t*)kv.p, (int64_t*)out->ptr, R, o);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    CSD_CATCH
+}

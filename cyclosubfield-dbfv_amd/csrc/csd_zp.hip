@@ -1,0 +1,443 @@
+// Exact dense linear maps over Z_p on the matrix cores.
+//
+// Y[v][i] = sum_k A[i][k] X[v][k] mod p for r moduli at once (one matrix per modulus), where
+// the vectors X[v] run along dimension 2 or 3 of a [count, r, d2, d3] tensor. The dense forms of
+// the period<->CRT transform (the group-circulant matrices of the convolutions in
+// ring.rs:1118-1161) and of the cyclic transforms of bootstrap/homcrt.rs:191-214 are such maps.
+//
+// The int8 form: every entry is centred (|x| <= p/2 < 2^56) and split into eight balanced
+// base-256 digits x = sum_d x_d 256^d, x_d in [-128, 127]. The 64 digit-pair products are
+// grouped by shift s = a + b into 15 int32 accumulators C_s = sum_{a+b=s} A_a X_b, each a chain
+// of v_mfma_i32_16x16x64_i8. |C_s| <= K * 8 * 2^14 < 2^31 for K-chunks of at most 16320 terms,
+// and sum_s 2^(8s) C_s = sum_k A[i][k] X[v][k] exactly, |.| < K 2^112 < 2^127, so the i128
+// recombination is exact; double_red128 and a multiplication by 2^192 mod p (mont.rs:57-66)
+// bring it to [0, p). No floating point anywhere: the result is exact.
+//
+// The plain form computes the same sums with 64 x 64 -> 128-bit products on the VALU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "csd_internal.hpp"
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+static constexpr int ZP_DIGITS = 8;
+static constexpr int64_t ZP_MAX_CHUNK = 16320;  // 64 * floor((2^31 - 1) / (8 * 2^14 * 64))
+static constexpr int64_t ZP_PLAIN_CHUNK = 8192; // centred products < 2^112: 2^13 terms < 2^125
+
+struct csd_zp_matrix {
+    csd_ctx* ctx;
+    int64_t r, m, k, mpad, kpad;
+    int8_t* planes;   // [r][8][mpad][kpad] digit planes
+    int64_t* values;  // [r][m][k] the entries as given
+    std::vector<csd_modulus> mods;
+};
+
+// the vectors of a view along `dim`: vector v = (hi, lo) = (v / nlo, v % nlo), element k
+struct VecLayout {
+    int64_t s_limb, s_hi, s_lo, s_elem, nlo, nvec, nelem;
+};
+
+static VecLayout vec_layout(const csd_view* x, int dim) {
+    VecLayout L;
+    L.s_limb = x->strides[1];
+    L.s_hi = x->strides[0];
+    if (dim == 3) {
+        L.s_lo = x->strides[2];
+        L.s_elem = x->strides[3];
+        L.nlo = x->shape[2];
+        L.nelem = x->shape[3];
+    } else if (dim == 2) {
+        L.s_lo = x->strides[3];
+        L.s_elem = x->strides[2];
+        L.nlo = x->shape[3];
+        L.nelem = x->shape[2];
+    } else {
+        throw CsdError("zp: dim must be 2 or 3");
+    }
+    L.nvec = x->shape[0] * L.nlo;
+    return L;
+}
+
+__device__ __forceinline__ int64_t zp_centre(int64_t x, int64_t p) {
+    // [-2p, 2p] -> [-p/2, p/2]
+    const int64_t h = p >> 1;
+    if (x > h) x -= p;
+    if (x > h) x -= p;
+    if (x < -h) x += p;
+    if (x < -h) x += p;
+    return x;
+}
+
+// v = sum_k a_k b_k (exact, any i128) -> v mod p in [0, p)
+__device__ __forceinline__ int64_t zp_reduce(const CsdMod& md, i128 v) {
+    int64_t t = csd_mont_mul(md, csd_double_red128(md, v), md.r3);  // 2^-128 v 2^192 2^-64 = v
+    if (t < 0) t += md.N;
+    if (t >= md.N) t -= md.N;
+    return t;
+}
+
+// ---------------------------------------------------------------------------------------------
+// digit planes: thread (l, v, q) splits elements 16q .. 16q + 15 of vector v of limb l
+
+struct DigitArgs {
+    const int64_t* src;
+    VecLayout L;
+    int64_t vpad, kpad;
+    int8_t* dst;  // [r][8][vpad][kpad]
+};
+
+__global__ __launch_bounds__(256) void k_zp_digits(DigitArgs a, ModTable mt, int64_t total) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int64_t nq = a.kpad / 16;
+    const int64_t q = t % nq;
+    const int64_t v = (t / nq) % a.vpad;
+    const int64_t l = t / (nq * a.vpad);
+    const int64_t p = mt.m[l].N;
+    uint32_t w[ZP_DIGITS][4];
+#pragma unroll
+    for (int d = 0; d < ZP_DIGITS; d++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) w[d][u] = 0;
+    if (v < a.L.nvec) {
+        const int64_t* base = a.src + l * a.L.s_limb + (v / a.L.nlo) * a.L.s_hi + (v % a.L.nlo) * a.L.s_lo;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int64_t k = q * 16 + e;
+            int64_t c = k < a.L.nelem ? zp_centre(base[k * a.L.s_elem], p) : 0;
+#pragma unroll
+            for (int d = 0; d < ZP_DIGITS; d++) {
+                int32_t lo = (int32_t)(c & 255);
+                lo = lo >= 128 ? lo - 256 : lo;
+                c = (c - lo) >> 8;
+                w[d][e >> 2] |= ((uint32_t)lo & 255u) << (8 * (e & 3));
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < ZP_DIGITS; d++) {
+        uint4 val = make_uint4(w[d][0], w[d][1], w[d][2], w[d][3]);
+        *(uint4*)(a.dst + ((l * ZP_DIGITS + d) * a.vpad + v) * a.kpad + q * 16) = val;
+    }
+}
+
+static void launch_digits(csd_ctx* ctx, const int64_t* src, const VecLayout& L, int64_t r, int64_t vpad, int64_t kpad,
+                          int8_t* dst, const ModTable& mt) {
+    DigitArgs a;
+    a.src = src;
+    a.L = L;
+    a.vpad = vpad;
+    a.kpad = kpad;
+    a.dst = dst;
+    const int64_t total = r * vpad * (kpad / 16);
+    if (total == 0) return;
+    if ((total + 255) / 256 > (int64_t)INT32_MAX) throw CsdError("zp: too many digit tiles");
+    TimedScope ts(ctx, "zp_digits", 8.0 * r * L.nvec * L.nelem + (double)r * ZP_DIGITS * vpad * kpad);
+    hipLaunchKernelGGL(k_zp_digits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, a, mt, total);
+    HIPCHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+// the int8 contraction. Workgroup = 4 waves = 32 rows x 64 vectors of one limb; wave w owns rows
+// i0 + 16 (w & 1) .. +16 and vectors j0 + 32 (w >> 1) .. +32 as two 16 x 16 output blocks.
+// MFMA operand layout: lane (x + 16 g) supplies row x of A (resp. vector x of X) and 16
+// consecutive k of k-group g; the output of a block sits in lane (v + 16 g) as rows 4 g .. 4 g + 3
+// of vector v. A and X use the same k assignment, so every product pairs equal k.
+
+struct MfmaArgs {
+    const int8_t* a;  // [r][8][mpad][kpad]
+    const int8_t* x;  // [r][8][npad][kpad]
+    int64_t* y;
+    VecLayout Ly;     // output vectors (element i = row of A)
+    int64_t m, mpad, npad, kpad, chunk;
+    int64_t mtiles, ntiles, nblocks;
+};
+
+__global__ __launch_bounds__(256) void k_zp_mfma(MfmaArgs a, ModTable mt) {
+    // XCD-aware order: the hardware deals consecutive workgroups to the 8 XCDs in turn; logical
+    // tile t = (b mod 8) * (nblocks / 8) + b / 8 keeps the tiles sharing A rows on one XCD's L2
+    int64_t b = blockIdx.x;
+    if (a.nblocks % 8 == 0) b = (b % 8) * (a.nblocks / 8) + b / 8;
+    const int64_t nt = b % a.ntiles;
+    const int64_t mtile = (b / a.ntiles) % a.mtiles;
+    const int64_t l = b / (a.ntiles * a.mtiles);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x16 = lane & 15, g = lane >> 4;
+    const int64_t i0 = mtile * 32 + (wave & 1) * 16;
+    const int64_t j0 = nt * 64 + (wave >> 1) * 32;
+    const int64_t pa = a.mpad * a.kpad, px = a.npad * a.kpad;
+    const int8_t* A = a.a + (l * ZP_DIGITS * a.mpad + i0 + x16) * a.kpad + g * 16;
+    const int8_t* X = a.x + (l * ZP_DIGITS * a.npad + j0 + x16) * a.kpad + g * 16;
+    const CsdMod md = mt.m[l];
+    int64_t total[2][4];
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) total[u][r] = 0;
+    for (int64_t kc = 0; kc < a.kpad; kc += a.chunk) {
+        const int64_t ke = min(a.kpad, kc + a.chunk);
+        v4i acc[2][2 * ZP_DIGITS - 1];
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int s = 0; s < 2 * ZP_DIGITS - 1; s++) acc[u][s] = (v4i){0, 0, 0, 0};
+        for (int64_t k0 = kc; k0 < ke; k0 += 64) {
+            v4i av[ZP_DIGITS];
+#pragma unroll
+            for (int d = 0; d < ZP_DIGITS; d++) av[d] = *(const v4i*)(A + d * pa + k0);
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                v4i xv[ZP_DIGITS];
+#pragma unroll
+                for (int d = 0; d < ZP_DIGITS; d++) xv[d] = *(const v4i*)(X + d * px + u * 16 * a.kpad + k0);
+#pragma unroll
+                for (int da = 0; da < ZP_DIGITS; da++)
+#pragma unroll
+                    for (int dx = 0; dx < ZP_DIGITS; dx++)
+                        acc[u][da + dx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[da], xv[dx], acc[u][da + dx], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                u128 v = 0;
+#pragma unroll
+                for (int s = 0; s < 2 * ZP_DIGITS - 1; s++) v += (u128)(i128)acc[u][s][r] << (8 * s);
+                int64_t t = total[u][r] + zp_reduce(md, (i128)v);
+                total[u][r] = t >= md.N ? t - md.N : t;
+            }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int64_t v = j0 + u * 16 + x16;
+        if (v >= a.Ly.nvec) continue;
+        int64_t* out = a.y + l * a.Ly.s_limb + (v / a.Ly.nlo) * a.Ly.s_hi + (v % a.Ly.nlo) * a.Ly.s_lo;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t i = i0 + 4 * g + r;
+            if (i < a.m) out[i * a.Ly.s_elem] = total[u][r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// the plain form: 16 x 16 outputs per workgroup, 64-term slices of A and X staged in LDS
+
+struct PlainArgs {
+    const int64_t* a;  // [r][m][k]
+    const int64_t* x;
+    int64_t* y;
+    VecLayout Lx, Ly;
+    int64_t m, k, mtiles, ntiles;
+};
+
+__global__ __launch_bounds__(256) void k_zp_plain(PlainArgs a, ModTable mt) {
+    __shared__ int64_t ta[16][65], tx[16][65];
+    const int64_t nt = blockIdx.x % a.ntiles;
+    const int64_t mtile = (blockIdx.x / a.ntiles) % a.mtiles;
+    const int64_t l = blockIdx.x / (a.ntiles * a.mtiles);
+    const int ti = threadIdx.x & 15, tv = threadIdx.x >> 4;
+    const int64_t i = mtile * 16 + ti, v = nt * 16 + tv;
+    const CsdMod md = mt.m[l];
+    const int64_t p = md.N;
+    const int64_t* A = a.a + l * a.m * a.k;
+    const int64_t* Xl = a.x + l * a.Lx.s_limb;
+    int64_t total = 0;
+    for (int64_t kc = 0; kc < a.k; kc += ZP_PLAIN_CHUNK) {
+        const int64_t ke = min(a.k, kc + ZP_PLAIN_CHUNK);
+        u128 acc = 0;
+        for (int64_t k0 = kc; k0 < ke; k0 += 64) {
+            for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+                const int rr = e >> 6, kk = e & 63;
+                const int64_t k = k0 + kk;
+                const int64_t ia = mtile * 16 + rr, iv = nt * 16 + rr;
+                ta[rr][kk] = (ia < a.m && k < ke) ? zp_centre(A[ia * a.k + k], p) : 0;
+                tx[rr][kk] = (iv < a.Lx.nvec && k < ke)
+                                 ? zp_centre(Xl[(iv / a.Lx.nlo) * a.Lx.s_hi + (iv % a.Lx.nlo) * a.Lx.s_lo + k * a.Lx.s_elem], p)
+                                 : 0;
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int kk = 0; kk < 64; kk++) {
+                uint64_t lo;
+                int64_t hi;
+                csd_smul_wide(ta[ti][kk], tx[tv][kk], lo, hi);
+                acc += ((u128)(uint64_t)hi << 64) | lo;
+            }
+            __syncthreads();
+        }
+        int64_t t = total + zp_reduce(md, (i128)acc);
+        total = t >= p ? t - p : t;
+    }
+    if (i < a.m && v < a.Ly.nvec)
+        a.y[l * a.Ly.s_limb + (v / a.Ly.nlo) * a.Ly.s_hi + (v % a.Ly.nlo) * a.Ly.s_lo + i * a.Ly.s_elem] = total;
+}
+
+// ---------------------------------------------------------------------------------------------
+// group-circulant matrices: out[l][a][b] = kern[l][a - b], the difference taken in the group
+// C_o0 x ... x C_o(f-1) with mixed-radix (row-major) indices
+
+struct CircArgs {
+    const int64_t* kern;  // [r][n] contiguous
+    int64_t* out;         // [r][n][n] contiguous
+    int64_t n, r;
+    int nf;
+    int64_t orders[8], strides[8];
+};
+
+__global__ __launch_bounds__(256) void k_circulant(CircArgs c) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.r * c.n * c.n) return;
+    const int64_t bcol = t % c.n, arow = (t / c.n) % c.n, l = t / (c.n * c.n);
+    int64_t idx = 0;
+    for (int f = 0; f < c.nf; f++) {
+        const int64_t da = (arow / c.strides[f]) % c.orders[f], db = (bcol / c.strides[f]) % c.orders[f];
+        idx += ((da - db + c.orders[f]) % c.orders[f]) * c.strides[f];
+    }
+    c.out[t] = c.kern[l * c.n + idx];
+}
+
+void csd_circulant_launch(csd_ctx* ctx, const int64_t* kern, int64_t* out, int64_t r, const std::vector<size_t>& orders) {
+    CircArgs c;
+    memset(&c, 0, sizeof(c));
+    if (orders.size() > 8) throw CsdError("circulant: too many cyclic factors");
+    c.kern = kern;
+    c.out = out;
+    c.r = r;
+    c.nf = (int)orders.size();
+    int64_t n = 1;
+    for (size_t f = orders.size(); f-- > 0;) {
+        c.orders[f] = (int64_t)orders[f];
+        c.strides[f] = n;
+        n *= (int64_t)orders[f];
+    }
+    c.n = n;
+    const int64_t total = r * n * n;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_circulant, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, c);
+    HIPCHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+
+static int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+extern "C" int csd_zp_matrix_create(csd_ctx* ctx, const csd_view* a, const csd_modulus* mods, size_t nmods, csd_zp_matrix** out) {
+    CSD_TRY check_view(a, "zp matrix");
+    if (a->shape[0] != 1) throw CsdError("zp matrix: dimension 0 must be 1");
+    const int64_t r = a->shape[1], m = a->shape[2], k = a->shape[3];
+    if (r < 1 || m < 1 || k < 1) throw CsdError("zp matrix: empty matrix");
+    ModTable mt = make_mod_table(mods, nmods, r);
+    for (size_t j = 0; j < nmods; j++)
+        if (mods[j].N >= ((int64_t)1 << 57)) throw CsdError("zp matrix: moduli must be below 2^57");
+    auto* z = new csd_zp_matrix();
+    z->ctx = ctx;
+    z->r = r;
+    z->m = m;
+    z->k = k;
+    z->mpad = round_up(m, 32);
+    z->kpad = round_up(k, 64);
+    z->mods.assign(mods, mods + nmods);
+    try {
+        z->planes = (int8_t*)ctx->alloc_bytes((size_t)r * ZP_DIGITS * z->mpad * z->kpad);
+        z->values = (int64_t*)ctx->alloc_bytes((size_t)r * m * k * 8);
+        // a contiguous copy of the entries (the plain form reads it)
+        csd_view dst;
+        dst.ptr = z->values;
+        for (int d = 0; d < 4; d++) dst.shape[d] = a->shape[d];
+        dst.strides[3] = 1;
+        dst.strides[2] = k;
+        dst.strides[1] = m * k;
+        dst.strides[0] = r * m * k;
+        if (csd_hal_assign(ctx, &dst, a) != 0) throw CsdError(csd_last_error());
+        VecLayout L = vec_layout(a, 3);
+        launch_digits(ctx, (const int64_t*)a->ptr, L, r, z->mpad, z->kpad, z->planes, mt);
+    } catch (...) {
+        if (z->planes) ctx->free_bytes(z->planes);
+        if (z->values) ctx->free_bytes(z->values);
+        delete z;
+        throw;
+    }
+    *out = z;
+    CSD_CATCH
+}
+
+extern "C" int csd_zp_matrix_free(csd_zp_matrix* z) {
+    CSD_TRY if (!z) return 0;
+    HIPCHECK(hipStreamSynchronize(z->ctx->stream));
+    z->ctx->free_bytes(z->planes);
+    z->ctx->free_bytes(z->values);
+    delete z;
+    CSD_CATCH
+}
+
+extern "C" int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* z, const csd_view* dst, const csd_view* src, int dim, int form,
+                                size_t max_chunk) {
+    CSD_TRY check_view(dst, "zp apply dst");
+    check_view(src, "zp apply src");
+    if (form != 0 && form != 1) throw CsdError("zp apply: form must be 0 (int8 MFMA) or 1 (plain)");
+    const VecLayout Lx = vec_layout(src, dim), Ly = vec_layout(dst, dim);
+    if (src->shape[1] != z->r || dst->shape[1] != z->r) throw CsdError("zp apply: dimension 1 must equal the number of moduli");
+    if (Lx.nelem != z->k) throw CsdError("zp apply: source vectors must have the matrix's column count");
+    if (Ly.nelem != z->m) throw CsdError("zp apply: destination vectors must have the matrix's row count");
+    if (Lx.nvec != Ly.nvec || src->shape[0] != dst->shape[0] || Lx.nlo != Ly.nlo)
+        throw CsdError("zp apply: source and destination must hold the same vectors");
+    if (dst->ptr == src->ptr) throw CsdError("zp apply: destination must not alias the source");
+    ModTable mt = make_mod_table(z->mods.data(), z->mods.size(), z->r);
+    const int64_t nvec = Lx.nvec;
+    if (nvec == 0) return 0;
+    if (form == 1) {
+        PlainArgs a;
+        a.a = z->values;
+        a.x = (const int64_t*)src->ptr;
+        a.y = (int64_t*)dst->ptr;
+        a.Lx = Lx;
+        a.Ly = Ly;
+        a.m = z->m;
+        a.k = z->k;
+        a.mtiles = (z->m + 15) / 16;
+        a.ntiles = (nvec + 15) / 16;
+        const int64_t blocks = z->r * a.mtiles * a.ntiles;
+        if (blocks > (int64_t)INT32_MAX) throw CsdError("zp apply: too many tiles");
+        TimedScope ts(ctx, "zp_plain", 8.0 * z->r * (z->m * z->k + nvec * (z->k + z->m)));
+        hipLaunchKernelGGL(k_zp_plain, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, mt);
+        HIPCHECK(hipGetLastError());
+        return 0;
+    }
+    int64_t chunk = ZP_MAX_CHUNK;
+    if (max_chunk) chunk = std::min<int64_t>(chunk, (int64_t)max_chunk / 64 * 64);
+    if (chunk < 64) throw CsdError("zp apply: chunk must hold at least 64 terms");
+    const int64_t npad = round_up(nvec, 64);
+    int8_t* xd = (int8_t*)ctx->alloc_bytes((size_t)z->r * ZP_DIGITS * npad * z->kpad);
+    try {
+        launch_digits(ctx, (const int64_t*)src->ptr, Lx, z->r, npad, z->kpad, xd, mt);
+        MfmaArgs a;
+        a.a = z->planes;
+        a.x = xd;
+        a.y = (int64_t*)dst->ptr;
+        a.Ly = Ly;
+        a.m = z->m;
+        a.mpad = z->mpad;
+        a.npad = npad;
+        a.kpad = z->kpad;
+        a.chunk = chunk;
+        a.mtiles = z->mpad / 32;
+        a.ntiles = npad / 64;
+        a.nblocks = z->r * a.mtiles * a.ntiles;
+        if (a.nblocks > (int64_t)INT32_MAX) throw CsdError("zp apply: too many tiles");
+        TimedScope ts(ctx, "zp_mfma",
+                      (double)z->r * ZP_DIGITS * (z->mpad * z->kpad + npad * z->kpad) + 8.0 * z->r * nvec * z->m);
+        hipLaunchKernelGGL(k_zp_mfma, dim3((unsigned)a.nblocks), dim3(256), 0, ctx->stream, a, mt);
+        HIPCHECK(hipGetLastError());
+    } catch (...) {
+        ctx->free_bytes(xd);
+        throw;
+    }
+    ctx->free_bytes(xd);
+    CSD_CATCH
+}

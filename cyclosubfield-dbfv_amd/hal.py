@@ -160,6 +160,19 @@ class HipPermutationPlan:
             self.handle = None
 
 
+class ZpMatrix:
+    """r dense matrices over Z_p (one per modulus) with their int8 digit planes split once
+    (csd_zp_matrix_create)."""
+
+    def __init__(self, backend, handle, shape):
+        self.backend, self.handle, self.shape = backend, handle, shape
+
+    def __del__(self):
+        if self.handle is not None and _lib is not None and self.backend.ctx is not None:
+            _lib.csd_zp_matrix_free(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+
 def _row_major_strides(shape):
     st = [1, 1, 1, 1]
     for d in (2, 1, 0):
@@ -383,6 +396,20 @@ class HipBackend:
                                                     f0l, f0h, f1l, f1h, ctypes.c_size_t(shift), ctypes.c_size_t(log2_modulus)))
 
     # --- permutations (hal/mod.rs:339-349) ---------------------------------------------------------
+    def zp_matrix(self, a, mods):
+        """Dense maps over Z_p: a is a [1, r, m, k] tensor with entries in [-2p, 2p]."""
+        h = ctypes.c_void_p()
+        arr, n = _mods(mods)
+        check(lib().csd_zp_matrix_create(self.ctx, ctypes.byref(a.view()), arr, n, ctypes.byref(h)))
+        return ZpMatrix(self, h.value, a.shape)
+
+    def zp_apply_dim(self, mat, dst, src, dim, form="mfma", max_chunk=0):
+        """dst = mat . src along dim (2 or 3), exact, result in [0, p): form "mfma" (int8 digits on
+        the matrix cores) or "plain" (64-bit VALU products)."""
+        check(lib().csd_zp_apply_dim(self.ctx, ctypes.c_void_p(mat.handle), ctypes.byref(dst.view()),
+                                     ctypes.byref(src.view()), ctypes.c_int(_DIM[dim]),
+                                     ctypes.c_int({"mfma": 0, "plain": 1}[form]), ctypes.c_size_t(max_chunk)))
+
     def hal_create_permutation(self, inner_shape, shifts):
         n = len(inner_shape)
         h = ctypes.c_void_p()

@@ -145,6 +145,15 @@ class RingContext:
         check(lib().csd_ring_reduce_wide(self.handle, ctypes.byref(dst.view()), ctypes.byref(wide.view()), b, n,
                                          ctypes.c_int(1 if full else 0)))
 
+    def transform_matrix(self, base, dim, inverse=False):
+        """The dense [1, r, n, n] matrix of pb_to_crt (or crt_to_pb) along dim 2 or 3."""
+        n = self.info["deg_lprime"] if dim == 2 else self.info["deg_k"]
+        out = self.backend.hal_allocate_i64((1, len(base), n, n))
+        b, nb = _sizes(base)
+        check(lib().csd_ring_transform_matrix(self.handle, b, nb, ctypes.c_int(dim), ctypes.c_int(1 if inverse else 0),
+                                              ctypes.byref(out.view())))
+        return out
+
     def batch_plan(self, base, cts, outs, pt):
         """A BatchPlan for ciphertexts at the levels len(cts[c][0]) over prefixes of `base`."""
         return BatchPlan(self, base, cts, outs, pt)

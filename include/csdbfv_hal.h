@@ -141,6 +141,24 @@ int csd_hal_create_permutation(csd_ctx* ctx, const size_t* inner_shape, const si
 int csd_perm_plan_free(csd_perm_plan* plan);
 int csd_hal_permute(csd_ctx* ctx, const csd_view* dst, const csd_view* src, const csd_perm_plan* plan, int dim); /* mod.rs:343-349 */
 
+/* ---- exact dense maps over Z_p (no reference counterpart) ----------------------------------- */
+/* The reference applies its Z_p-linear maps (the period<->CRT transforms, ring.rs:1118-1161, and
+ * homcrt's cyclic transforms, bootstrap/homcrt.rs:191-214) through NTTs and Galois automorphisms.
+ * These entry points apply them as dense matrices instead, for any square or rectangular map:
+ *   dst[c, l, ..., i along dim, ...] = sum_k A_l[i][k] src[c, l, ..., k along dim, ...] mod p_l
+ * with the result in [0, p_l), equal mod p_l to every other exact evaluation (the NTT path
+ * included). Entries of A and src must lie in [-2p, 2p]; moduli below 2^57.
+ * form 0: balanced 8-bit digits on int8 MFMA (v_mfma_i32_16x16x64_i8), int32 accumulation over
+ *         K-chunks of at most 16320 terms (max_chunk = 0) or max_chunk (a multiple of 64),
+ *         exact i128 recombination, Montgomery reduction;
+ * form 1: the plain VALU kernel (64 x 64 -> 128-bit products). */
+typedef struct csd_zp_matrix csd_zp_matrix;
+/* a: [1, r, m, k], one m x k matrix per modulus; digits are split once here */
+int csd_zp_matrix_create(csd_ctx* ctx, const csd_view* a, const csd_modulus* mods, size_t nmods, csd_zp_matrix** out);
+int csd_zp_matrix_free(csd_zp_matrix* m);
+int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* a, const csd_view* dst, const csd_view* src, int dim, int form,
+                     size_t max_chunk);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,6 +89,12 @@ int csd_ring_reduce_wide(csd_ring* ring, const csd_view* dst, const csd_view* wi
  * product of the primes at `scalar_base` */
 int csd_ring_scalar_tensor(csd_ring* ring, const csd_view* dst, const size_t* base, size_t r, int64_t scalar);
 
+/* The dense n x n matrix, per limb of `base`, of pb_to_crt_tensor (ring.rs:1118-1141; inverse =
+ * 0) or crt_to_pb_tensor (ring.rs:1144-1161; inverse = 1) restricted to one dimension (dim 2: the
+ * L' part, n = deg_L'; dim 3: the K part, n = deg_K): the group-circulant C[a][b] = k[a - b] of
+ * the CRT kernel k of that dimension. out: contiguous [1, r, n, n]. */
+int csd_ring_transform_matrix(csd_ring* ring, const size_t* base, size_t r, int dim, int inverse, const csd_view* out);
+
 /* ---- batched uneven chain ------------------------------------------------------------------ */
 /* The reference runs every ring operation on one RingEls with one RNS base (ring.rs:198-208), so
  * ciphertexts at different levels after dynamic scaling take one call sequence each. A batch plan
