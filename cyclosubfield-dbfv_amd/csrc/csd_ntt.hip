@@ -775,6 +775,181 @@ __device__ __forceinline__ void run_pass(int64_t* __restrict__ lds, int L, int r
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Compile-time pass shapes for the dimension-3 plans of the benchmark fields. The generic pass
+// above derives its index decomposition and its twiddle blocks at run time; here every divisor
+// is a constant, and a register block that starts at stage 0 of its factor has block index
+// (blk0 R^i + sb) mod R^i = sb, a constant, so its twiddles are wave-uniform scalar loads. The
+// arithmetic (stages, schedule bits, twiddles) is the generic pass's, so results are identical.
+
+template <int R, int G, int S0, int I, bool INV>
+__device__ __forceinline__ void ct_level(int64_t (&v)[Pow<R>::of(G)], const DevPlan* __restrict__ P,
+                                         const CsdBarrett* __restrict__ TW, const DevFactor& F, int dir, uint32_t blk0) {
+    constexpr int M = Pow<R>::of(G);
+    constexpr int D = Pow<R>::of(G - 1 - I);
+    constexpr int NSB = Pow<R>::of(I);
+    constexpr int s = S0 + I;
+    constexpr uint32_t nbk = (uint32_t)Pow<R>::of(s);
+    const uint8_t info = P->info[dir][F.index][s];
+    const int64_t n = P->n;
+    if (info & INFO_REDUCE) {
+#pragma unroll
+        for (int m = 0; m < M; m++) v[m] = csd_barrett(P->one, v[m], n);
+    }
+#pragma unroll
+    for (int sb = 0; sb < NSB; sb++) {
+        const uint32_t blk = (blk0 * (uint32_t)NSB + (uint32_t)sb) % nbk;
+#pragma unroll
+        for (int jj = 0; jj < D; jj++) {
+            const int m0 = sb * D * R + jj;
+            if constexpr (R == 2) {
+                if constexpr (!INV)
+                    bf2_fwd(v[m0], v[m0 + D], TW[F.tw_off[s] + blk], info, P->one, n);
+                else
+                    bf2_inv(v[m0], v[m0 + D], TW[F.itw_off[s] + blk], info, P->inv_len, n);
+            } else {
+                if constexpr (!INV)
+                    bf3_fwd(v[m0], v[m0 + D], v[m0 + 2 * D], TW + F.tw_off[s] + 6 * blk, info, P->one, n);
+                else
+                    bf3_inv(v[m0], v[m0 + D], v[m0 + 2 * D], TW + F.itw_off[s] + 2 * blk, info, P->inv_len, F.zeta, F.zeta2, n);
+            }
+        }
+    }
+}
+
+template <int R, int G, int S0, bool INV>
+__device__ __forceinline__ void ct_stages(int64_t (&v)[Pow<R>::of(G)], const DevPlan* P, const CsdBarrett* TW,
+                                          const DevFactor& F, int dir, uint32_t blk0) {
+    if constexpr (!INV) {
+        ct_level<R, G, S0, 0, false>(v, P, TW, F, dir, blk0);
+        if constexpr (G > 1) ct_level<R, G, S0, 1, false>(v, P, TW, F, dir, blk0);
+        if constexpr (G > 2) ct_level<R, G, S0, 2, false>(v, P, TW, F, dir, blk0);
+    } else {
+        if constexpr (G > 2) ct_level<R, G, S0, 2, true>(v, P, TW, F, dir, blk0);
+        if constexpr (G > 1) ct_level<R, G, S0, 1, true>(v, P, TW, F, dir, blk0);
+        ct_level<R, G, S0, 0, true>(v, P, TW, F, dir, blk0);
+    }
+}
+
+// stages [S0, S0 + G) of factor FI (radix R, suffix S, stage length SL at S0) on `rows` lines of L
+template <int L, int FI, int R, int G, int S, int SL, int S0, bool INV>
+__device__ __forceinline__ void ct_pass(int64_t* __restrict__ lds, int rows, const DevPlan* __restrict__ P,
+                                        const CsdBarrett* __restrict__ TW, int dir) {
+    constexpr int M = Pow<R>::of(G);
+    constexpr uint32_t sub = SL / M, NG = L / M, stride = sub * S, lline = L + L / 8 + 1;
+    const DevFactor& F = P->f[FI];
+    const uint32_t total = NG * (uint32_t)rows;
+    for (uint32_t w = threadIdx.x; w < total; w += blockDim.x) {
+        const uint32_t row = w / NG, gam = w % NG;
+        const uint32_t t = gam / S, suffix = gam % S;
+        const uint32_t blk0 = t / sub, jj = t % sub;
+        const uint32_t base = (blk0 * SL + jj) * S + suffix;
+        int64_t* line = lds + row * lline;
+        int64_t v[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) v[m] = line[lds_slot(base + m * stride)];
+        ct_stages<R, G, S0, INV>(v, P, TW, F, dir, blk0);
+#pragma unroll
+        for (int m = 0; m < M; m++) line[lds_slot(base + m * stride)] = v[m];
+    }
+    __syncthreads();
+}
+
+// The compiled shapes. SHAPE 1: factors [2, 3, 8, 64] (field_36864, deg_K = 3072); SHAPE 2:
+// [3, 64, 64] (field_49152, deg_K = 12288); both with the default grouping (3 radix-2 / 1 radix-3
+// stages per pass), forward passes in increasing and inverse passes in decreasing stage order.
+template <int SHAPE>
+struct CtShape;
+template <>
+struct CtShape<1> {
+    static constexpr int L = 3072;
+};
+template <>
+struct CtShape<2> {
+    static constexpr int L = 12288;
+};
+
+template <int SHAPE, bool INV>
+__device__ __forceinline__ void ct_passes(int64_t* lds, int rows, const DevPlan* P, const CsdBarrett* TW, int dir) {
+    if constexpr (SHAPE == 1) {
+        constexpr int L = 3072;
+        ct_pass<L, 0, 2, 1, 1536, 2, 0, INV>(lds, rows, P, TW, dir);
+        ct_pass<L, 1, 3, 1, 512, 3, 0, INV>(lds, rows, P, TW, dir);
+        ct_pass<L, 2, 2, 3, 64, 8, 0, INV>(lds, rows, P, TW, dir);
+        if constexpr (!INV) {
+            ct_pass<L, 3, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
+            ct_pass<L, 3, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
+        } else {
+            ct_pass<L, 3, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
+            ct_pass<L, 3, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
+        }
+    } else {
+        constexpr int L = 12288;
+        ct_pass<L, 0, 3, 1, 4096, 3, 0, INV>(lds, rows, P, TW, dir);
+        if constexpr (!INV) {
+            ct_pass<L, 1, 2, 3, 64, 64, 0, INV>(lds, rows, P, TW, dir);
+            ct_pass<L, 1, 2, 3, 64, 8, 3, INV>(lds, rows, P, TW, dir);
+            ct_pass<L, 2, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
+            ct_pass<L, 2, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
+        } else {
+            ct_pass<L, 1, 2, 3, 64, 8, 3, INV>(lds, rows, P, TW, dir);
+            ct_pass<L, 1, 2, 3, 64, 64, 0, INV>(lds, rows, P, TW, dir);
+            ct_pass<L, 2, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
+            ct_pass<L, 2, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
+        }
+    }
+}
+
+// the (factor, s0, g) pass list a compiled shape runs, for checking a plan against it
+static bool ct_shape_matches(const DevPlan& P, int dir, int shape) {
+    struct Pf {
+        int len, S;
+    };
+    std::vector<Pf> fs;
+    std::vector<DevPass> ps;
+    if (shape == 1) {
+        fs = {{2, 1536}, {3, 512}, {8, 64}, {64, 1}};
+        ps = {{0, 0, 1, 2}, {1, 0, 1, 3}, {2, 0, 3, 2}};
+        if (dir != 2) {
+            ps.push_back({3, 0, 3, 2});
+            ps.push_back({3, 3, 3, 2});
+        } else {
+            ps.push_back({3, 3, 3, 2});
+            ps.push_back({3, 0, 3, 2});
+        }
+    } else if (shape == 2) {
+        fs = {{3, 4096}, {64, 64}, {64, 1}};
+        ps = {{0, 0, 1, 3}};
+        for (uint8_t f = 1; f <= 2; f++) {
+            if (dir != 2) {
+                ps.push_back({f, 0, 3, 2});
+                ps.push_back({f, 3, 3, 2});
+            } else {
+                ps.push_back({f, 3, 3, 2});
+                ps.push_back({f, 0, 3, 2});
+            }
+        }
+    } else {
+        return false;
+    }
+    if (P.nfactors != (int)fs.size()) return false;
+    for (int f = 0; f < P.nfactors; f++) {
+        const DevFactor& F = P.f[f];
+        if (F.len != fs[f].len || (int)F.S != fs[f].S || F.index != f) return false;
+        uint32_t nb = 1;
+        for (int s = 0; s < F.nstages; s++) {
+            if (F.nblocks[s] != nb || F.stage_len[s] != (uint32_t)F.len / nb) return false;
+            nb *= (uint32_t)F.radix;
+        }
+    }
+    if (P.npasses[dir] != (int)ps.size()) return false;
+    for (size_t q = 0; q < ps.size(); q++) {
+        const DevPass& a = P.passes[dir][q];
+        if (a.factor != ps[q].factor || a.s0 != ps[q].s0 || a.g != ps[q].g || a.radix != ps[q].radix) return false;
+    }
+    return true;
+}
+
 struct RowArgs {
     int64_t d0, d1, d2, d3;
     int64_t st[3];
@@ -793,8 +968,9 @@ struct RowArgs {
     const int32_t* seg_limb;
 };
 
-// dim-3 transforms (contiguous lines): lines staged in LDS, stages run in register blocks
-template <bool INV>
+// dim-3 transforms (contiguous lines): lines staged in LDS, stages run in register blocks;
+// SHAPE 0 reads the pass list from the plan, SHAPE 1 and 2 are the compiled shapes above
+template <bool INV, int SHAPE = 0>
 __global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
     extern __shared__ int64_t lds[];
     const int64_t chunk = blockIdx.x % a.chunks;
@@ -813,18 +989,24 @@ __global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
     }
     const DevPlan* __restrict__ P = a.plans[j];
     const CsdBarrett* __restrict__ TW = (const CsdBarrett*)(P + 1);
-    const int L = P->len;
+    int L;
+    if constexpr (SHAPE == 0)
+        L = P->len;
+    else
+        L = CtShape<SHAPE>::L;
     const uint32_t lline = lds_line((uint32_t)L);
     const int rows = (int)min((int64_t)a.rows, a.d2 - k0);
     const int total = rows * L;
     for (int g = threadIdx.x; g < total; g += blockDim.x) {
-        const uint32_t r = a.rows == 1 ? 0u : fdiv((uint32_t)g, a.magic_L);
+        const uint32_t r = a.rows == 1 ? 0u : (SHAPE ? (uint32_t)g / (uint32_t)L : fdiv((uint32_t)g, a.magic_L));
         const int p = g - (int)r * L;
         lds[r * lline + lds_slot(p)] = base[r * a.st[2] + p];
     }
     __syncthreads();
     const int dir = a.dir;
-    if (P->nfactors == 0) {
+    if constexpr (SHAPE != 0) {
+        ct_passes<SHAPE, INV>(lds, rows, P, TW, dir);
+    } else if (P->nfactors == 0) {
         if (dir != 1) {
             const CsdBarrett m = dir == 2 ? P->inv_len : P->one;
             for (int g = threadIdx.x; g < total; g += blockDim.x) {
@@ -855,7 +1037,7 @@ __global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
     }
     const CsdMod m = mt.m[j];
     for (int g = threadIdx.x; g < total; g += blockDim.x) {
-        const uint32_t r = a.rows == 1 ? 0u : fdiv((uint32_t)g, a.magic_L);
+        const uint32_t r = a.rows == 1 ? 0u : (SHAPE ? (uint32_t)g / (uint32_t)L : fdiv((uint32_t)g, a.magic_L));
         const int p = g - (int)r * L;
         int64_t x = lds[r * lline + lds_slot(p)];
         if (a.mul)
@@ -1013,10 +1195,44 @@ static ColKernel pick_col_kernel(const DevPlan& P, bool inv) {
 }
 
 static bool g_use_generic_ntt = false;  // forces the one-stage-per-LDS-pass kernel (tests)
+static bool g_use_plan_rows = false;    // keeps dim-3 transforms on the plan-driven row kernel (tests)
 
 extern "C" int csd_ntt_force_generic(int on) {
-    g_use_generic_ntt = on != 0;
+    if (on < 0 || on > 2) {
+        csd_set_error("csd_ntt_force_generic: mode must be 0, 1 or 2");
+        return 1;
+    }
+    g_use_generic_ntt = on == 1;
+    g_use_plan_rows = on == 2;
     return 0;
+}
+
+typedef void (*RowKernel)(RowArgs, ModTable);
+
+// the dim-3 row kernel for these plans: a compiled shape when every plan matches one, else the
+// plan-driven form; the dynamic LDS limit is raised once per kernel
+static RowKernel pick_row_kernel(csd_ntt_plan* const* plans, size_t nplans, int dir) {
+    static bool attr = false;
+    static const RowKernel ks[2][3] = {{k_ntt_rows<false, 0>, k_ntt_rows<false, 1>, k_ntt_rows<false, 2>},
+                                       {k_ntt_rows<true, 0>, k_ntt_rows<true, 1>, k_ntt_rows<true, 2>}};
+    if (!attr) {
+        for (auto& row : ks)
+            for (RowKernel k : row)
+                HIPCHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    int shape = 0;
+    if (!g_use_plan_rows && nplans > 0) {
+        for (int c : {1, 2}) {
+            bool all = true;
+            for (size_t jj = 0; jj < nplans && all; jj++) all = ct_shape_matches(plans[jj]->host, dir, c);
+            if (all) {
+                shape = c;
+                break;
+            }
+        }
+    }
+    return ks[dir == 2 ? 1 : 0][shape];
 }
 
 static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim, int dir,
@@ -1134,17 +1350,9 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         if (blocks > (int64_t)INT32_MAX) throw CsdError("ntt: too many tiles");
         size_t lds = (size_t)rows * lds_line((uint32_t)L) * 8;
         if (lds > 160 * 1024) throw CsdError("ntt: line too long for LDS staging");
-        static bool row_attr = false;
-        if (!row_attr) {
-            HIPCHECK(hipFuncSetAttribute((const void*)k_ntt_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            HIPCHECK(hipFuncSetAttribute((const void*)k_ntt_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            row_attr = true;
-        }
+        RowKernel rk = pick_row_kernel(plans, nplans, dir);
         TimedScope ts(ctx, names[1][dir], 16.0 * (double)numel(dst));
-        if (inv)
-            hipLaunchKernelGGL(k_ntt_rows<true>, dim3((unsigned)blocks), dim3(threads), lds, ctx->stream, r, t);
-        else
-            hipLaunchKernelGGL(k_ntt_rows<false>, dim3((unsigned)blocks), dim3(threads), lds, ctx->stream, r, t);
+        hipLaunchKernelGGL(rk, dim3((unsigned)blocks), dim3(threads), lds, ctx->stream, r, t);
         HIPCHECK(hipGetLastError());
         return;
     }
@@ -1252,16 +1460,12 @@ void csd_ntt_segments(csd_ctx* ctx, int64_t* const* seg_base, const int32_t* seg
     r.seg_limb = seg_limb;
     size_t lds = (size_t)rows * lds_line((uint32_t)L) * 8;
     if (lds > 160 * 1024) throw CsdError("work-list ntt: line too long for LDS staging");
-    HIPCHECK(hipFuncSetAttribute((const void*)k_ntt_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIPCHECK(hipFuncSetAttribute((const void*)k_ntt_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     const int threads = tune.threads > 0 ? tune.threads : (L * rows >= 8192 ? 512 : 256);
     if (threads % 64 || threads > 512) throw CsdError("work-list ntt: row block size must be a multiple of 64, at most 512");
     if (nslices * r.chunks > (int64_t)INT32_MAX) throw CsdError("work-list ntt: too many tiles");
+    RowKernel rk = pick_row_kernel(plans, nplans, dir);
     TimedScope ts(ctx, timer, bytes);
-    if (inv)
-        hipLaunchKernelGGL(k_ntt_rows<true>, dim3((unsigned)(nslices * r.chunks)), dim3(threads), lds, ctx->stream, r, t);
-    else
-        hipLaunchKernelGGL(k_ntt_rows<false>, dim3((unsigned)(nslices * r.chunks)), dim3(threads), lds, ctx->stream, r, t);
+    hipLaunchKernelGGL(rk, dim3((unsigned)(nslices * r.chunks)), dim3(threads), lds, ctx->stream, r, t);
     HIPCHECK(hipGetLastError());
 }
 

@@ -122,8 +122,10 @@ int csd_hal_intt_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* con
 int csd_hal_ntt_mul_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim,
                                const csd_view* mul, const csd_modulus* mods, size_t nmods);
 /* Kernel selection for the NTT entry points (no reference counterpart): 0 (default) uses the
- * register-blocked kernels where a plan has one, 1 forces the one-stage-per-LDS-pass kernel for
- * every plan. Both produce identical results; the switch exists to test and time each form. */
+ * register-blocked kernels where a plan has one, with the compiled pass shapes of the
+ * field_36864 / field_49152 K parts where the plans match them; 1 forces the one-stage-per-LDS-pass
+ * kernel for every plan; 2 keeps the register-blocked kernels but reads every pass list from the
+ * plan. All produce identical results; the switch exists to test and time each form. */
 int csd_ntt_force_generic(int on);
 
 /* ---- base-conversion corrections -------------------------------------------------------------- */

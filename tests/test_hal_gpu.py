@@ -374,3 +374,38 @@ def test_generic_ntt_kernel_parity(gpu, cpu, q, inner):
             assert same(from_dev(gpu, dg), dc)
     finally:
         pkg.lib().csd_ntt_force_generic(0)
+
+
+@pytest.mark.parametrize("field,log2_q,inner", [("field_36864", (960, 980), [2, 3, 8, 64]),
+                                                ("field_49152", (1280, 1300), [3, 64, 64])])
+@pytest.mark.parametrize("mode", [0, 2])
+def test_compiled_row_kernels_parity(gpu, cpu, field, log2_q, inner, mode):
+    """Dimension-3 transforms of the benchmark fields' K parts: the compiled-shape row kernels
+    (mode 0, the default) and the plan-driven row kernel (mode 2) are both bit-exact with the
+    oracle in all three directions, on several lines per workgroup and inputs in [-2q, 2q]."""
+    pkg = load()
+    primes, _ = pkg.ring.sample_primes(field, log2_q, 76)
+    q = primes[3]
+    n = int(np.prod(inner))
+    mc, mg = cpu.hal_create_modulus(q), gpu.hal_create_modulus(q)
+    pc, pg = cpu.hal_create_ntt_plan(mc, inner), gpu.hal_create_ntt_plan(mg, inner)
+    rng = np.random.default_rng(n + mode)
+    pkg.lib().csd_ntt_force_generic(mode)
+    try:
+        data = rand_i64(rng, (2, 1, 3, n), -2 * q, 2 * q)
+        kern = rand_i64(rng, (1, 1, 1, n), -q, q)
+        dc, dg = data.copy(), to_dev(gpu, data)
+        cpu.hal_ntt_mont64_dim(dc, [pc], 3)
+        gpu.hal_ntt_mont64_dim(dg, [pg], 3)
+        assert same(from_dev(gpu, dg), dc)
+        cpu.hal_intt_mont64_dim(dc, [pc], 3)
+        gpu.hal_intt_mont64_dim(dg, [pg], 3)
+        assert same(from_dev(gpu, dg), dc)
+        cpu.hal_ntt_mul_mont64_dim(dc, [pc], 3, kern, [mc])
+        gpu.hal_ntt_mul_mont64_dim(dg, [pg], 3, to_dev(gpu, kern), [mg])
+        assert same(from_dev(gpu, dg), dc)
+        cpu.hal_intt_mont64_dim(dc, [pc], 3)
+        gpu.hal_intt_mont64_dim(dg, [pg], 3)
+        assert same(from_dev(gpu, dg), dc)
+    finally:
+        pkg.lib().csd_ntt_force_generic(0)
