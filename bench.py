@@ -285,6 +285,66 @@ def bench_batch(pkg, be, steps, count=256, seed=SEED, cpu=True):
     return res
 
 
+def bench_hom_mul(pkg, be, steps, pairs=64, ndigits=9, seed=SEED, cpu=True):
+    """The ring multiplication of a BFV ciphertext pair at field_36864 (bfv_hom_mul,
+    bfv.rs:747-775): lift to the 37-prime extended base, tensor product in wide accumulators,
+    rescale by t = 2^76, relinearization with a 9-digit key (gadget base conversion + key-switch
+    inner product), on `pairs` ciphertext pairs, one plain HAL call sequence per pair."""
+    ring = pkg.ring.RingContext(be, FIELD, LOG2_Q, T_BITS)
+    base = ring.initial_rns_base()
+    R = len(base)
+    n2, n3 = ring.info["deg_lprime"], ring.info["deg_k"]
+    t = 1 << T_BITS
+    digits = pkg.bfv.select_digits(ndigits, R)
+    cts = be.hal_allocate_i64((2 * pairs * 2, R, n2, n3))
+    ring.fill_uniform(cts, base, seed + 11)
+    keys = be.hal_allocate_i64((2 * ndigits, R, n2, n3))
+    ring.fill_uniform(keys, base, seed + 12)
+    key_list = [keys[2 * i:2 * i + 2] for i in range(ndigits)]
+    ring.precompute_rns_factor_transforms()
+    stream = be.stream()
+    ev = HipEvents()
+
+    def step():
+        out = None
+        for k in range(pairs):
+            out = pkg.bfv.bfv_hom_mul(ring, t, cts[4 * k:4 * k + 2], cts[4 * k + 2:4 * k + 4], base, key_list, digits)
+        return out
+
+    first = step()  # warm-up (also builds the bconv and transform caches)
+    be.sync()
+    ev.start(stream)
+    for _ in range(steps):
+        step()
+    ms = ev.stop(stream) / steps
+    res = {
+        "workload": f"field_36864 bfv_hom_mul: {pairs} ciphertext pairs, r = {R} primes lifted to "
+                    f"{R + len(ring.extension_rns_base_for(base))}, t = 2^{T_BITS}, relinearization key of {ndigits} digits",
+        "value": round(pairs / (ms / 1000.0), 1), "unit": "ciphertext multiplications/s", "ms_per_step": round(ms, 3),
+    }
+    if cpu:
+        from oracle import cpu_backend
+        from oracle import ring as oring
+
+        cb = cpu_backend.CpuBackend()
+        orc = oring.RingContext(cb, oring.FieldDescriptor(FIELD), LOG2_Q, T_BITS)
+        k = pairs - 1
+        hl = be.hal_export(cts[4 * k:4 * k + 2]).reshape(2, R, n2, n3)
+        hr = be.hal_export(cts[4 * k + 2:4 * k + 4]).reshape(2, R, n2, n3)
+        hk = be.hal_export(keys).reshape(2 * ndigits, R, n2, n3)
+        orc.transform_cache(base)
+        t0 = time.perf_counter()
+        want = oring.bfv_hom_mul(orc, t, hl, hr, base, [hk[2 * i:2 * i + 2] for i in range(ndigits)], digits)
+        dt = time.perf_counter() - t0
+        if not (be.hal_export(first).reshape(want.shape) == want).all():
+            raise SystemExit("bench: bfv_hom_mul differs from the oracle")
+        res["cpu_baseline"] = {"value": round(1.0 / dt, 3), "unit": "ciphertext multiplications/s", "cores": 1, "kind": "port",
+                               "sample": f"1 pair ({dt:.2f} s, caches built beforehand), also checked bit-exact"}
+    del cts, keys
+    ring.close()
+    return res
+
+
 MFMA_INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
 
 
@@ -397,8 +457,9 @@ def main():
     ap.add_argument("--count", type=int, default=128, help="ring elements in the batch (2 per ciphertext)")
     ap.add_argument("--cpu-sample", type=int, default=4, help="ring elements in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="batch,zp,galois",
-                    help="comma-separated extra workloads beside the headline transforms: batch, zp, galois, or none")
+    ap.add_argument("--workloads", default="hom_mul,batch,zp,galois",
+                    help="comma-separated extra workloads beside the headline transforms: hom_mul, batch, zp, galois, "
+                         "or none")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -503,6 +564,8 @@ def main():
     ring.close()
     del data
     workloads = {}
+    if "hom_mul" in args.workloads:
+        workloads["bfv_hom_mul"] = bench_hom_mul(pkg, be, max(1, args.steps // 5), cpu=not args.no_cpu)
     if "batch" in args.workloads:
         workloads["batch_uneven_chain"] = bench_batch(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
     if "zp" in args.workloads:

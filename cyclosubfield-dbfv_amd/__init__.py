@@ -5,3 +5,4 @@ include/csdbfv_hal.h. The package directory name is not a Python identifier; loa
 `importlib` (see `load_package` in __graft_entry__.py)."""
 from .hal import I128, DeviceTensor, HalError, HipBackend, lib  # noqa: F401
 from . import ring  # noqa: E402,F401
+from . import bfv  # noqa: E402,F401

@@ -1495,6 +1495,34 @@ extern "C" int csd_batch_free(csd_batch_plan* plan) {
     CSD_CATCH
 }
 
+// scalar_tensor (ring.rs:876-888) for an i128 scalar given as (low, high) words
+extern "C" int csd_ring_scalar_tensor_wide(csd_ring* ring, const csd_view* dst, const size_t* base, size_t r, int64_t lo,
+                                           int64_t hi) {
+    CSD_TRY if (dst->shape[0] != 1 || dst->shape[1] != (int64_t)r || dst->shape[2] != 1 || dst->shape[3] != 1)
+        throw CsdError("scalar tensor must have shape [1, r, 1, 1]");
+    const i128 scalar = (i128)(((u128)(uint64_t)hi << 64) | (uint64_t)lo);
+    std::vector<int64_t> v;
+    for (size_t t = 0; t < r; t++) {
+        int64_t p = ring->primes.at(base[t]);
+        v.push_back(mont_repr((int64_t)(scalar % (i128)p), p));  // truncating remainder, as Rust's %
+    }
+    ok(csd_hal_import(ring->ctx, dst, v.data(), v.size()));
+    CSD_CATCH
+}
+
+// inverse_scalar_tensor (ring.rs:1165-1177) for the scalar prod(primes at `of`)
+extern "C" int csd_ring_inverse_product_tensor(csd_ring* ring, const csd_view* dst, const size_t* base, size_t r,
+                                               const size_t* of, size_t nof) {
+    CSD_TRY if (dst->shape[0] != 1 || dst->shape[1] != (int64_t)r || dst->shape[2] != 1 || dst->shape[3] != 1)
+        throw CsdError("scalar tensor must have shape [1, r, 1, 1]");
+    std::vector<size_t> b = to_vec(base, r), o = to_vec(of, nof);
+    ring->check_base(b);
+    ring->check_base(o);
+    std::vector<int64_t> v = ring->inverse_product_values(b, o);
+    ok(csd_hal_import(ring->ctx, dst, v.data(), v.size()));
+    CSD_CATCH
+}
+
 // The dense matrix of the period->CRT transform (inverse = 0) or of CRT->period (inverse = 1)
 // along one dimension, per limb of `base`: pb_to_crt_tensor (ring.rs:1118-1141) is the
 // convolution over the cyclic group of each dimension with the CRT kernel, so the matrix is the

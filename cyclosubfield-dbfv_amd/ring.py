@@ -68,8 +68,10 @@ class RingContext:
         check(lib().csd_ring_primes(self.handle, out, ctypes.c_size_t(128), ctypes.byref(n), ctypes.byref(il)))
         self.primes = [out[i] for i in range(n.value)]
         self.initial_rns_base_len = il.value
+        self._consts = {}
 
     def close(self):
+        self._consts = {}
         if self.handle is not None:
             lib().csd_ring_destroy(self.handle)
             self.handle = None
@@ -159,10 +161,139 @@ class RingContext:
         return BatchPlan(self, base, cts, outs, pt)
 
     def scalar_tensor(self, base, scalar):
+        """scalar_tensor (ring.rs:876-888): mont(scalar mod p) per prime; any i128 scalar. The
+        tensors are constants, kept per (base, scalar) so repeated sequences upload nothing."""
+        key = ("scalar", tuple(base), scalar)
+        hit = self._consts.get(key)
+        if hit is not None:
+            return hit
         t = self.backend.hal_allocate_i64([1, len(base), 1, 1])
         b, n = _sizes(base)
-        check(lib().csd_ring_scalar_tensor(self.handle, ctypes.byref(t.view()), b, n, ctypes.c_int64(scalar)))
+        if -(1 << 63) <= scalar < (1 << 63):
+            check(lib().csd_ring_scalar_tensor(self.handle, ctypes.byref(t.view()), b, n, ctypes.c_int64(scalar)))
+        else:
+            if not -(1 << 127) <= scalar < (1 << 127):
+                raise ValueError("scalar does not fit an i128")
+            w = scalar & ((1 << 128) - 1)
+            lo, hi = w & ((1 << 64) - 1), w >> 64
+            check(lib().csd_ring_scalar_tensor_wide(self.handle, ctypes.byref(t.view()), b, n,
+                                                    ctypes.c_int64(lo - (1 << 64) if lo >= 1 << 63 else lo),
+                                                    ctypes.c_int64(hi - (1 << 64) if hi >= 1 << 63 else hi)))
+        self._consts[key] = t
         return t
+
+    def inverse_product_tensor(self, base, of):
+        """inverse_scalar_tensor (ring.rs:1165-1177) of the product of the primes at `of`."""
+        key = ("inverse", tuple(base), tuple(of))
+        hit = self._consts.get(key)
+        if hit is not None:
+            return hit
+        t = self.backend.hal_allocate_i64([1, len(base), 1, 1])
+        b, n = _sizes(base)
+        o, no = _sizes(of)
+        check(lib().csd_ring_inverse_product_tensor(self.handle, ctypes.byref(t.view()), b, n, o, no))
+        self._consts[key] = t
+        return t
+
+    def mods_for(self, base):
+        key = ("mods", tuple(base))
+        hit = self._consts.get(key)
+        if hit is None:
+            hit = self._consts[key] = [self.backend.hal_create_modulus(self.primes[i]) for i in base]
+        return hit
+
+    def allocate_wide(self, count, base):
+        return WideRingEls(self, count, base)
+
+    def reduce_wide(self, wide):
+        """reduce_wide (ring.rs:846-865): one reduction kind per run of rows."""
+        out = self.allocate(wide.count(), wide.rns_base)
+        start = 0
+        while start < wide.count():
+            full = wide.needs_full_reduction(start)
+            end = start + 1
+            while end < wide.count() and wide.needs_full_reduction(end) == full:
+                end += 1
+            self.reduce_wide_tensor(out[start:end], wide.data[start:end], wide.rns_base, full)
+            start = end
+        return out
+
+
+class WideRingEls:
+    """WideRingEls (ring.rs:330-532): i128 accumulators in the doubly-Montgomery domain with the
+    reference's per-row bound tracking; fresh rows are overwritten (ring.rs:373-413)."""
+
+    def __init__(self, ring, count, base):
+        self.ring = ring
+        self.rns_base = list(base)
+        self.data = ring.backend.hal_allocate_i128([count, len(base), ring.info["deg_lprime"], ring.info["deg_k"]])
+        self._one = None
+        self._neg_one = None
+        self.max_p = max(ring.primes[i] for i in base)
+        self.min_p = min(ring.primes[i] for i in base)
+        self.bound = [0] * count
+
+    def count(self):
+        return len(self.bound)
+
+    def one(self):
+        if self._one is None:
+            self._one = self.ring.scalar_tensor(self.rns_base, 1)
+        return self._one
+
+    def neg_one(self):
+        if self._neg_one is None:
+            self._neg_one = self.ring.scalar_tensor(self.rns_base, -1)
+        return self._neg_one
+
+    def record_step(self, rows, step):
+        for i in rows:
+            nb = self.bound[i] + step
+            if nb > (1 << 127) - 1:
+                raise OverflowError("wide ring element overflow: too many accumulation steps")
+            self.bound[i] = nb
+
+    def needs_full_reduction(self, row):
+        return self.bound[row] > (self.min_p << 63)
+
+    def _accumulate(self, rows, lhs, rhs):
+        be = self.ring.backend
+        rows = list(rows)
+        start, end_all = rows[0], rows[-1] + 1
+        while start < end_all:
+            fresh = self.bound[start] == 0
+            end = start + 1
+            while end < end_all and (self.bound[end] == 0) == fresh:
+                end += 1
+
+            def sl(t):
+                return t if t.shape[0] == 1 else t[start - rows[0]:end - rows[0]]
+
+            dst = self.data[start:end]
+            if fresh:
+                be.hal_mul_i64(dst, sl(lhs), sl(rhs))
+            else:
+                be.hal_fma_i64(dst, sl(lhs), sl(rhs))
+            start = end
+
+    def fma(self, rows, lhs, lhs_bound, rhs, rhs_bound):
+        self._accumulate(rows, lhs, rhs)
+        self.record_step(rows, lhs_bound * rhs_bound * self.max_p * self.max_p)
+
+    def inner_product(self, rows, terms):
+        rows = list(rows)
+        if any(self.bound[i] != 0 for i in rows):
+            raise ValueError("inner_product overwrites: the rows must be fresh")
+        self.ring.backend.hal_inner_product_i64(self.data[rows[0]:rows[-1] + 1], [t[0] for t in terms], [t[2] for t in terms])
+        self.record_step(rows, sum(lb * rb * self.max_p * self.max_p for (_, lb, _, rb) in terms))
+
+    def add(self, rows, src, src_bound):
+        self._accumulate(rows, src, self.one())
+        self.record_step(rows, src_bound * self.max_p * self.max_p)
+
+    def sub(self, rows, src, src_bound):
+        self._accumulate(rows, src, self.neg_one())
+        self.record_step(rows, src_bound * self.max_p * self.max_p)
 
 
 class BatchTuning(ctypes.Structure):

@@ -84,10 +84,13 @@ int csd_fill_uniform(csd_ctx* ctx, const csd_view* dst, const csd_modulus* mods,
 /* reduce_wide_tensor (ring.rs:846-865) for rows of equal reduction kind: full = 1 takes
  * red_mont64_double + i64_to_mont64, full = 0 a single red_mont64 */
 int csd_ring_reduce_wide(csd_ring* ring, const csd_view* dst, const csd_view* wide, const size_t* base, size_t r, int full);
-/* the [1, r, 1, 1] constant tensors of ring.rs:876-920 and 1165-1177, written into dst:
- * kind 0: mont(scalar mod p); kind 1: mont((scalar mod p)^-1) with the scalar given as the
- * product of the primes at `scalar_base` */
+/* the [1, r, 1, 1] constant tensor mont(scalar mod p) of ring.rs:876-888, written into dst */
 int csd_ring_scalar_tensor(csd_ring* ring, const csd_view* dst, const size_t* base, size_t r, int64_t scalar);
+/* scalar_tensor (ring.rs:876-888) for an i128 scalar (low, high words), e.g. t = 2^76 */
+int csd_ring_scalar_tensor_wide(csd_ring* ring, const csd_view* dst, const size_t* base, size_t r, int64_t lo, int64_t hi);
+/* inverse_scalar_tensor (ring.rs:1165-1177) of the product of the primes at `of` */
+int csd_ring_inverse_product_tensor(csd_ring* ring, const csd_view* dst, const size_t* base, size_t r, const size_t* of,
+                                    size_t nof);
 
 /* The dense n x n matrix, per limb of `base`, of pb_to_crt_tensor (ring.rs:1118-1141; inverse =
  * 0) or crt_to_pb_tensor (ring.rs:1144-1161; inverse = 1) restricted to one dimension (dim 2: the

@@ -840,3 +840,42 @@ def hom_gal_one(ring, ct, base, g, parts, keys, digits):
     key_switch_inner_product(ring, acc, gal_parts, keys, digits)
     acc.add(range(0, 1), gal_c0, 1)
     return ring.reduce_wide(acc)
+
+
+def clip_digits(digits, base_len):
+    """bfv.rs:120-126"""
+    return [(a, min(b, base_len)) for a, b in digits if a < base_len]
+
+
+def relin_crt(ring, prod, base, keys, digits):
+    """relin (bfv.rs:699-741) with CRT-basis output: `prod` is a [3, r, ., .] period-basis
+    product (destroyed); keys[i] is the [2, r_key, ., .] CRT-basis key of digit i over a base of
+    which `base` is a prefix."""
+    r = len(base)
+    acc = ring.allocate_wide(2, base)
+    digits = clip_digits(digits, r)
+    parts = gadget_decompose(ring, prod[2:3], base, digits)           # key_switch_acc, bfv.rs:498-528
+    key_switch_inner_product(ring, acc, parts, [np.ascontiguousarray(k[:, :r]) for k in keys], digits)
+    ring.pb_to_crt_tensor(prod[0:2], base)
+    acc.add(range(0, 2), prod[0:2], 1)
+    return ring.reduce_wide(acc)
+
+
+def bfv_hom_mul(ring, t, lhs, rhs, base, keys, digits):
+    """bfv_hom_mul (bfv.rs:747-775) with CRT-basis output, for CRT-basis ciphertexts lhs, rhs
+    ([2, r, ., .] each) over `base`."""
+    base_len = len(base)
+
+    def lift(ct):
+        ct_pb = ct.copy()
+        ring.crt_to_pb_tensor(ct_pb, base)
+        return lift_to_extended(ring, ct_pb, base, crt=ct)
+
+    lhs_l, lifted = lift(lhs)
+    rhs_l, _ = lift(rhs)
+    acc = ring.allocate_wide(3, lifted)
+    hom_mul_raw_acc(ring, acc, 0, lhs_l, rhs_l)
+    prod = ring.reduce_wide(acc)
+    ring.crt_to_pb_tensor(prod, lifted)
+    rescaled = rescale_to_base(ring, prod, lifted, base_len, t)
+    return relin_crt(ring, rescaled, base, keys, digits)

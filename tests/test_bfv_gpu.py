@@ -1,0 +1,96 @@
+"""The bfv-layer sequences of the hot path on the MI355X (the ring multiplication of config C2:
+lift to the extended base, tensor product in wide accumulators, rescale, relinearization with
+gadget base conversion and the key-switch inner product), bit-exact against the oracle's
+sequences (bfv.rs:408-426, 498-528, 564-622, 667-775) on the same seeded inputs."""
+import numpy as np
+import pytest
+
+from _pkg import load
+from oracle import ring as R
+from test_hal_gpu import from_dev, same, to_dev
+from test_ring_gpu import uniform_residues
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("field_18", (100, 120), 4, 2), ("field_192", (100, 120), 8, 2), ("field_36864", (960, 980), 76, 9)]
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    be = load().HipBackend(0)
+    yield be
+    be.close()
+
+
+@pytest.fixture(scope="module")
+def rings(gpu, cpu):
+    pkg = load()
+    made = {}
+
+    def get(field, log2_q, t_bits):
+        key = (field, log2_q, t_bits)
+        if key not in made:
+            made[key] = (pkg.ring.RingContext(gpu, field, log2_q, t_bits),
+                         R.RingContext(cpu, R.FieldDescriptor(field), log2_q, t_bits))
+        return made[key]
+
+    yield get
+    for g, _ in made.values():
+        g.close()
+
+
+def setup(rings, field, log2_q, t_bits, ndigits, seed):
+    g, c = rings(field, log2_q, t_bits)
+    base = g.initial_rns_base()
+    primes = [g.primes[i] for i in base]
+    rng = np.random.default_rng(seed)
+    shape = (2, len(base), g.info["deg_lprime"], g.info["deg_k"])
+    lhs, rhs = uniform_residues(rng, primes, shape), uniform_residues(rng, primes, shape)
+    digits = R.select_digits(ndigits, len(base))
+    keys = [uniform_residues(rng, primes, shape) for _ in digits]
+    return g, c, base, lhs, rhs, digits, keys
+
+
+@pytest.mark.parametrize("field,log2_q,t_bits,ndigits", CASES)
+def test_lift_and_rescale_parity(gpu, cpu, rings, field, log2_q, t_bits, ndigits):
+    pkg = load()
+    g, c, base, lhs, rhs, digits, keys = setup(rings, field, log2_q, t_bits, ndigits, 11)
+    assert g.extension_rns_base_for(base) == c.extension_rns_base_for(base)
+    lc, lifted_c = R.lift_to_extended(c, lhs.copy(), base)
+    lg, lifted_g = pkg.bfv.lift_to_extended(g, to_dev(gpu, lhs), base)
+    assert lifted_g == lifted_c
+    assert same(from_dev(gpu, lg), lc)
+    # rescale a period-basis tensor over the lifted base
+    x = uniform_residues(np.random.default_rng(12), [g.primes[i] for i in lifted_c], (2, len(lifted_c)) + lhs.shape[2:])
+    t = 1 << t_bits if t_bits else 2
+    rc = R.rescale_to_base(c, x.copy(), lifted_c, len(base), t)
+    rg = pkg.bfv.rescale_to_base(g, to_dev(gpu, x), lifted_g, len(base), t)
+    assert same(from_dev(gpu, rg), rc)
+
+
+@pytest.mark.parametrize("field,log2_q,t_bits,ndigits", CASES)
+def test_relin_parity(gpu, cpu, rings, field, log2_q, t_bits, ndigits):
+    pkg = load()
+    g, c, base, lhs, rhs, digits, keys = setup(rings, field, log2_q, t_bits, ndigits, 21)
+    prod = np.concatenate([lhs, rhs[0:1]])  # a 3-row period-basis product
+    want = R.relin_crt(c, prod.copy(), base, keys, digits)
+    got = pkg.bfv.relin_crt(g, to_dev(gpu, prod), base, [to_dev(gpu, k) for k in keys], digits)
+    assert same(from_dev(gpu, got), want)
+
+
+@pytest.mark.parametrize("field,log2_q,t_bits,ndigits", CASES)
+def test_hom_mul_parity(gpu, cpu, rings, field, log2_q, t_bits, ndigits):
+    """bfv_hom_mul (bfv.rs:747-775): the whole C2 ring multiplication, bit-exact"""
+    pkg = load()
+    g, c, base, lhs, rhs, digits, keys = setup(rings, field, log2_q, t_bits, ndigits, 31)
+    t = 1 << t_bits if t_bits else 2
+    want = R.bfv_hom_mul(c, t, lhs, rhs, base, keys, digits)
+    got = pkg.bfv.bfv_hom_mul(g, t, to_dev(gpu, lhs), to_dev(gpu, rhs), base, [to_dev(gpu, k) for k in keys], digits)
+    assert same(from_dev(gpu, got), want)
+
+
+def test_select_digits_matches_oracle():
+    pkg = load()
+    for r in (1, 2, 17, 23, 37):
+        for d in range(1, r + 1):
+            assert pkg.bfv.select_digits(d, r) == R.select_digits(d, r)
