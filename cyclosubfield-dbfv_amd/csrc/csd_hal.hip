@@ -162,8 +162,40 @@ extern "C" int csd_ctx_destroy(csd_ctx* ctx) {
     for (auto& kv : ctx->pool)
         for (void* p : kv.second) (void)hipFree(p);
     for (auto& kv : ctx->live) (void)hipFree(kv.first);
+    if (ctx->copy_stream) {
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        (void)hipStreamDestroy(ctx->copy_stream);
+    }
+    for (auto& kv : ctx->pinned) (void)hipHostFree(kv.first);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    CSD_CATCH
+}
+
+// Pinned (page-locked) host memory, allocated as such (never registered after the fact), for
+// tensors that live on the host and are streamed to the device, e.g. Galois keys that do not
+// stay resident. Freed with csd_host_free or at context destruction.
+extern "C" int csd_host_alloc(csd_ctx* ctx, size_t bytes, void** out) {
+    CSD_TRY if (!out) throw CsdError("csd_host_alloc: null output");
+    void* p = nullptr;
+    HIPCHECK(hipHostMalloc(&p, std::max<size_t>(bytes, 8), hipHostMallocDefault));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->pinned[p] = bytes;
+    *out = p;
+    CSD_CATCH
+}
+
+extern "C" int csd_host_free(csd_ctx* ctx, void* p) {
+    CSD_TRY if (!p) return 0;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto it = ctx->pinned.find(p);
+        if (it == ctx->pinned.end()) throw CsdError("csd_host_free: not a block of this context");
+        ctx->pinned.erase(it);
+    }
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->copy_stream) HIPCHECK(hipStreamSynchronize(ctx->copy_stream));
+    HIPCHECK(hipHostFree(p));
     CSD_CATCH
 }
 

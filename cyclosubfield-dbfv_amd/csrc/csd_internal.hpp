@@ -38,6 +38,8 @@ void csd_set_error(const std::string& s);
 struct csd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // host->device key streaming, created on first use
+    std::map<void*, size_t> pinned;     // pinned host blocks handed out by csd_host_alloc
     std::mutex mu;
     std::map<size_t, std::vector<void*>> pool;
     std::map<void*, size_t> live;

@@ -855,6 +855,65 @@ struct csd_ring {
                                         (int64_t*)out.ptr + (int64_t)gi * per_g, m.data(), full0, full1);
         }
     }
+
+    // The same key switch with the keys in pinned host memory (the role of the CPU build's
+    // KeyStore, keystore.rs:50-127): two device staging buffers; the copy of key t + 1 on the
+    // copy stream overlaps the kernel of key t on the compute stream, ordered by events.
+    void galois_keyswitch_streamed(const std::vector<size_t>& base, int64_t batch, int64_t d, const csd_view& parts,
+                                   const csd_view& c0, const std::vector<uint64_t>& gs, const int64_t* const* host_keys,
+                                   const csd_view& out) {
+        const int64_t r = (int64_t)base.size(), n2 = (int64_t)f->deg_lp, n3 = (int64_t)f->deg_k;
+        const size_t key_bytes = (size_t)(2 * d * r * n2 * n3) * 8;
+        for (size_t gi = 0; gi < gs.size(); gi++) {
+            if (!host_keys[gi]) throw CsdError("galois key switch: null host key");
+            std::lock_guard<std::mutex> g(ctx->mu);
+            bool found = false;
+            for (auto& kv : ctx->pinned) {
+                const char* b = (const char*)kv.first;
+                if ((const char*)host_keys[gi] >= b && (const char*)host_keys[gi] + key_bytes <= b + kv.second) found = true;
+            }
+            if (!found) throw CsdError("galois key switch: host keys must lie in blocks from csd_host_alloc");
+        }
+        if (!ctx->copy_stream) HIPCHECK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+        DevBuf stage0(ctx, key_bytes), stage1(ctx, key_bytes);
+        void* stage[2] = {stage0.p, stage1.p};
+        hipEvent_t copied[2], consumed[2];
+        for (int k = 0; k < 2; k++) {
+            HIPCHECK(hipEventCreateWithFlags(&copied[k], hipEventDisableTiming));
+            HIPCHECK(hipEventCreateWithFlags(&consumed[k], hipEventDisableTiming));
+        }
+        auto cleanup = [&]() {
+            (void)hipStreamSynchronize(ctx->copy_stream);
+            (void)hipStreamSynchronize(ctx->stream);
+            for (int k = 0; k < 2; k++) {
+                (void)hipEventDestroy(copied[k]);
+                (void)hipEventDestroy(consumed[k]);
+            }
+        };
+        try {
+            // the staging buffers came from the pool: earlier work on the compute stream may still use them
+            HIPCHECK(hipEventRecord(consumed[0], ctx->stream));
+            HIPCHECK(hipEventRecord(consumed[1], ctx->stream));
+            const int64_t per_g = batch * 2 * r * n2 * n3;
+            std::vector<csd_view> kv(gs.size());
+            for (size_t gi = 0; gi < gs.size(); gi++) {
+                const int k = (int)(gi & 1);
+                HIPCHECK(hipStreamWaitEvent(ctx->copy_stream, consumed[k], 0));
+                HIPCHECK(hipMemcpyAsync(stage[k], host_keys[gi], key_bytes, hipMemcpyHostToDevice, ctx->copy_stream));
+                HIPCHECK(hipEventRecord(copied[k], ctx->copy_stream));
+                HIPCHECK(hipStreamWaitEvent(ctx->stream, copied[k], 0));
+                csd_view key = make_view(stage[k], 2 * d, r, n2, n3);
+                csd_view o = sub(out, 0, (int64_t)gi * batch * 2, (int64_t)(gi + 1) * batch * 2, 8);
+                (void)per_g;
+                galois_keyswitch(base, batch, d, parts, c0, std::vector<uint64_t>{gs[gi]}, &key, o);
+                HIPCHECK(hipEventRecord(consumed[k], ctx->stream));
+            }
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    }
 };
 
 static std::vector<size_t> to_vec(const size_t* p, size_t n) { return std::vector<size_t>(p, p + n); }
@@ -1111,6 +1170,15 @@ extern "C" int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, siz
     CSD_TRY if (batch == 0 || digits == 0 || ngal == 0) throw CsdError("galois key switch: empty batch, digits or rotations");
     ring->galois_keyswitch(to_vec(base, r), (int64_t)batch, (int64_t)digits, *parts, *c0, std::vector<uint64_t>(gs, gs + ngal),
                            keys, *out);
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_galois_keyswitch_streamed(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t digits,
+                                                  const csd_view* parts, const csd_view* c0, const uint64_t* gs, size_t ngal,
+                                                  const int64_t* const* host_keys, const csd_view* out) {
+    CSD_TRY if (batch == 0 || digits == 0 || ngal == 0) throw CsdError("galois key switch: empty batch, digits or rotations");
+    ring->galois_keyswitch_streamed(to_vec(base, r), (int64_t)batch, (int64_t)digits, *parts, *c0,
+                                    std::vector<uint64_t>(gs, gs + ngal), host_keys, *out);
     CSD_CATCH
 }
 

@@ -160,6 +160,36 @@ class HipPermutationPlan:
             self.handle = None
 
 
+class PinnedHostTensor:
+    """A row-major int64 array in pinned host memory (csd_host_alloc), as a numpy view."""
+
+    def __init__(self, backend, shape):
+        self.backend = backend
+        self.shape = tuple(int(x) for x in shape)
+        n = int(np.prod(self.shape))
+        p = ctypes.c_void_p()
+        check(lib().csd_host_alloc(backend.ctx, ctypes.c_size_t(8 * max(n, 1)), ctypes.byref(p)))
+        self.ptr = p.value
+        self.array = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_int64)), shape=(max(n, 1),))[:n].reshape(self.shape)
+
+    def fill_from(self, src):
+        """device -> pinned host copy of a device tensor with the same number of entries"""
+        n = int(np.prod(self.shape))
+        check(lib().csd_hal_export(self.backend.ctx, ctypes.byref(src.view()), ctypes.c_void_p(self.ptr), ctypes.c_size_t(n)))
+
+    def free(self):
+        if self.ptr is not None and _lib is not None and self.backend.ctx is not None:
+            self.array = None
+            _lib.csd_host_free(self.backend.ctx, ctypes.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class ZpMatrix:
     """r dense matrices over Z_p (one per modulus) with their int8 digit planes split once
     (csd_zp_matrix_create)."""
@@ -396,6 +426,10 @@ class HipBackend:
                                                     f0l, f0h, f1l, f1h, ctypes.c_size_t(shift), ctypes.c_size_t(log2_modulus)))
 
     # --- permutations (hal/mod.rs:339-349) ---------------------------------------------------------
+    def host_alloc(self, shape):
+        """pinned host memory for a host-resident int64 tensor (PinnedHostTensor)"""
+        return PinnedHostTensor(self, shape)
+
     def zp_matrix(self, a, mods):
         """Dense maps over Z_p: a is a [1, r, m, k] tensor with entries in [-2p, 2p]."""
         h = ctypes.c_void_p()

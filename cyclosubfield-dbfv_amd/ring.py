@@ -135,6 +135,17 @@ class RingContext:
                                               ctypes.byref(parts.view()), ctypes.byref(c0.view()), garr,
                                               ctypes.c_size_t(len(gs)), karr, ctypes.byref(out.view())))
 
+    def galois_keyswitch_streamed(self, base, batch, digits, parts, c0, gs, host_keys, out):
+        """galois_keyswitch with the keys in pinned host memory (PinnedHostTensor each), streamed
+        to the device rotation by rotation; see csd_ring_galois_keyswitch_streamed."""
+        b, n = _sizes(base)
+        garr = (ctypes.c_uint64 * len(gs))(*gs)
+        P = ctypes.POINTER(ctypes.c_int64)
+        karr = (P * len(host_keys))(*[ctypes.cast(ctypes.c_void_p(k.ptr), P) for k in host_keys])
+        check(lib().csd_ring_galois_keyswitch_streamed(self.handle, b, n, ctypes.c_size_t(batch), ctypes.c_size_t(digits),
+                                                       ctypes.byref(parts.view()), ctypes.byref(c0.view()), garr,
+                                                       ctypes.c_size_t(len(gs)), karr, ctypes.byref(out.view())))
+
     def fill_uniform(self, dst, base, seed):
         """Synthetic residues uniform in [0, p) per limb (splitmix64 of the flat index)."""
         mods = [self.backend.hal_create_modulus(self.primes[i]) for i in base]

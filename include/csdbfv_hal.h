@@ -72,6 +72,11 @@ int csd_calibrate(csd_ctx* ctx, int kind, size_t n, double* out);
  * elem_bytes is 8 (i64) or 16 (i128). */
 int csd_hal_allocate(csd_ctx* ctx, int elem_bytes, const int64_t shape[4], void** out);
 int csd_hal_free(csd_ctx* ctx, void* ptr);
+/* pinned host memory for host-resident tensors streamed to the device (no trait counterpart; the
+ * CPU build's KeyStore::store / CpuTensor::offload, keystore.rs:50-127, cpu.rs:82-185, move key
+ * tensors out of RAM the same way). Allocated page-locked from the start. */
+int csd_host_alloc(csd_ctx* ctx, size_t bytes, void** out);
+int csd_host_free(csd_ctx* ctx, void* ptr);
 /* hal_import / hal_export (hal/mod.rs:124-125): host data in row-major order of the view. */
 int csd_hal_import(csd_ctx* ctx, const csd_view* dst, const int64_t* src, size_t len);
 int csd_hal_export(csd_ctx* ctx, const csd_view* src, int64_t* dst, size_t len);

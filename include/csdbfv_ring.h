@@ -79,6 +79,15 @@ int csd_ring_apply_galois(csd_ring* ring, const csd_view* dst, const csd_view* s
 int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t digits,
                               const csd_view* parts, const csd_view* c0, const uint64_t* gs, size_t ngal,
                               const csd_view* keys, const csd_view* out);
+/* The same key switch with the keys in pinned host memory instead of HBM, the fallback for key
+ * sets larger than the device (the CPU build streams them from its NVMe KeyStore,
+ * keystore.rs:50-127): host_keys[t] points to the contiguous [2*digits, r, ., .] key of rotation t
+ * inside a block from csd_host_alloc. Keys are copied through two device staging buffers on a
+ * copy stream, each copy overlapping the previous rotation's kernel. Bit-identical to
+ * csd_ring_galois_keyswitch. */
+int csd_ring_galois_keyswitch_streamed(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t digits,
+                                       const csd_view* parts, const csd_view* c0, const uint64_t* gs, size_t ngal,
+                                       const int64_t* const* host_keys, const csd_view* out);
 /* synthetic inputs: dst[i, j, k, l] uniform in [0, p_j) from splitmix64 of the flat index */
 int csd_fill_uniform(csd_ctx* ctx, const csd_view* dst, const csd_modulus* mods, size_t nmods, uint64_t seed);
 /* reduce_wide_tensor (ring.rs:846-865) for rows of equal reduction kind: full = 1 takes
