@@ -280,3 +280,100 @@ def test_kernel_values_are_traces(cpu):
         cand += 1
     # transversal [1, 6]: g^-1 = 1 and 6; subgroup {1, 2, 4}
     assert k_k == [(z + z ** 2 + z ** 4) % p, (pow(z, 6, p) + pow(z, 12, p) + pow(z, 24, p)) % p]
+
+
+def _crt_centred(residues, primes):
+    """the integer in (-Q/2, Q/2] with the given residues"""
+    Q = 1
+    for p in primes:
+        Q *= p
+    x = 0
+    for v, p in zip(residues, primes):
+        Qp = Q // p
+        x += int(v) % p * Qp * pow(Qp % p, -1, p)
+    x %= Q
+    return x - Q if x > Q // 2 else x
+
+
+def _encode(ring, xs, base, shape):
+    """integers xs (one per coefficient) as a Montgomery-form period-basis tensor over base"""
+    n = shape[2] * shape[3]
+    out = np.zeros((1, len(base), shape[2], shape[3]), dtype=np.int64)
+    for j, i in enumerate(base):
+        p = ring.primes[i]
+        out[0, j] = np.array([R.mont_repr(x % p, p) for x in xs], dtype=np.int64).reshape(shape[2], shape[3])
+    return out
+
+
+def _decode(ring, t, base):
+    """a Montgomery-form tensor [1, r, ., .] over base -> centred integers per coefficient"""
+    primes = [ring.primes[i] for i in base]
+    flat = t.reshape(len(base), -1)
+    return [_crt_centred([R.from_mont_repr(int(flat[j, k]), primes[j]) for j in range(len(base))], primes)
+            for k in range(flat.shape[1])]
+
+
+def test_lift_to_extended_is_a_lift(cpu):
+    """lift_to_extended (bfv.rs:564-595): over base + extension the element represents the
+    centred lift of x mod q plus at most one multiple of q (the BEHZ conversion's epsilon)."""
+    ring = R.RingContext(cpu, R.FieldDescriptor("field_18"), (100, 120), 4)
+    base = ring.initial_rns_base()
+    q = ring.modulus_of(base)
+    rng = np.random.default_rng(3)
+    shape = (1, len(base), 3, 6)
+    xs = [int(rng.integers(-(1 << 62), 1 << 62)) * int(rng.integers(1, 1 << 40)) % q for _ in range(18)]
+    xs = [x - q if x > q // 2 else x for x in xs]
+    lifted, lb = R.lift_to_extended(ring, _encode(ring, xs, base, shape), base)
+    ring.crt_to_pb_tensor(lifted, lb)
+    got = _decode(ring, lifted, lb)
+    for x, y in zip(xs, got):
+        assert y - x in (-q, 0, q)
+
+
+def test_rescale_to_base_rounds(cpu):
+    """rescale_to_base (bfv.rs:601-622): for X = A B with |A|, |B| <= q/2 (a tensor-product
+    coefficient) over base + extension, the result is round(t X / q) mod q up to +-1."""
+    ring = R.RingContext(cpu, R.FieldDescriptor("field_18"), (100, 120), 4)
+    base = ring.initial_rns_base()
+    ext = ring.extension_rns_base_for(base)
+    lifted = base + ext
+    q = ring.modulus_of(base)
+    t = 16
+    rng = np.random.default_rng(4)
+    xs = []
+    for _ in range(18):
+        a = int(rng.integers(0, 1 << 62)) * int(rng.integers(0, 1 << 62)) % q - q // 2
+        b = int(rng.integers(0, 1 << 62)) * int(rng.integers(0, 1 << 62)) % q - q // 2
+        xs.append(a * b)
+    x = _encode(ring, xs, lifted, (1, len(lifted), 3, 6))
+    out = R.rescale_to_base(ring, x, lifted, len(base), t)
+    got = _decode(ring, out, base)
+    for X, y in zip(xs, got):
+        want = (2 * t * X + q) // (2 * q)
+        d = (y - want) % q
+        assert min(d, q - d) <= 1
+
+
+def test_gadget_decompose_recomposes(cpu):
+    """gadget_decompose (bfv.rs:408-426): every digit is x mod D_i up to a multiple of D_i, small,
+    and sum_i d_i (D/D_i) ((D/D_i)^-1 mod D_i) = x mod D."""
+    ring = R.RingContext(cpu, R.FieldDescriptor("field_18"), (160, 180), 4)
+    base = ring.initial_rns_base()
+    digits = R.select_digits(2, len(base))
+    D = ring.modulus_of(base)
+    rng = np.random.default_rng(5)
+    xs = [int(rng.integers(0, 1 << 62)) * int(rng.integers(0, 1 << 62)) * int(rng.integers(0, 1 << 62)) % D
+          for _ in range(18)]
+    parts = R.gadget_decompose(ring, _encode(ring, xs, base, (1, len(base), 3, 6)), base, digits)
+    ring.crt_to_pb_tensor(parts, base)
+    dec = [_decode(ring, parts[i:i + 1], base) for i in range(len(digits))]
+    for k, x in enumerate(xs):
+        total = 0
+        for i, (a, b) in enumerate(digits):
+            Di = ring.modulus_of(base[a:b])
+            d = dec[i][k]
+            assert (d - x) % Di == 0
+            assert abs(d) <= 2 * Di
+            g = (D // Di) * pow((D // Di) % Di, -1, Di)
+            total += d * g
+        assert (total - x) % D == 0
