@@ -13,6 +13,11 @@ path serves, on top of any backend with the ``hal::Backend`` method names (here:
 * bfv.rs:388-426, 493-683, 747-841 — digits, gadget decomposition, key-switch inner products,
   lift/rescale/mod-switch, hom_mul and hom_gal sequences.
 
+Pins beyond the reference's KATs: lift_to_extended, rescale_to_base and gadget_decompose are
+checked against their closed forms by CRT reconstruction (tests/test_oracle_ring.py). The
+compositions relin_crt and bfv_hom_mul are parity unpinned beyond their pinned steps (the
+reference checks them only through decryption).
+
 Big-integer constants use Python ints, which is what the reference's feanor-math BigInt/Zn
 arithmetic computes exactly. Primality (feanor-math's Miller-Rabin, ring.rs:139) is decided with
 the deterministic Miller-Rabin base set for 64-bit integers.
