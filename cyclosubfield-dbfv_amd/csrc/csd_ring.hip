@@ -411,8 +411,11 @@ struct GadgetCache {  // ring.rs:1714-1727
     std::unique_ptr<DevBuf> d_over_p_inv, matrix, d_mod_p;
 };
 
+static bool g_overlap_halves = true;  // see csd_ring_set_overlap
+
 struct csd_ring {
     csd_ctx* ctx;
+    hipStream_t aux_stream = nullptr;  // second stream of the half-batch overlap
     std::shared_ptr<Field> f;
     std::vector<int64_t> primes;
     std::vector<csd_modulus> mods;
@@ -426,6 +429,10 @@ struct csd_ring {
     std::map<std::pair<std::vector<size_t>, std::vector<size_t>>, std::unique_ptr<DevBuf>> inv_scalar_cache;
 
     ~csd_ring() {
+        if (aux_stream) {
+            (void)hipStreamSynchronize(aux_stream);
+            (void)hipStreamDestroy(aux_stream);
+        }
         for (auto& kv : pcache) {
             csd_perm_plan_free(kv.second.first);
             csd_perm_plan_free(kv.second.second);
@@ -556,10 +563,48 @@ struct csd_ring {
             throw CsdError("transform: tensor shape doesn't match [count, r, deg_L', deg_K]");
         csd_view klp = make_view(inverse_kernels ? c.ik_lp->p : c.k_lp->p, 1, r, (int64_t)f->deg_lp, 1);
         csd_view kk = make_view(inverse_kernels ? c.ik_k->p : c.k_k->p, 1, r, 1, (int64_t)f->deg_k);
-        ok(csd_hal_ntt_mul_mont64_dim(ctx, &data, c.plans_lp.data(), r, CSD_DIM2, &klp, c.mods.data(), r));
-        ok(csd_hal_ntt_mul_mont64_dim(ctx, &data, c.plans_k.data(), r, CSD_DIM3, &kk, c.mods.data(), r));
-        ok(csd_hal_intt_mont64_dim(ctx, &data, c.plans_lp.data(), r, CSD_DIM2));
-        ok(csd_hal_intt_mont64_dim(ctx, &data, c.plans_k.data(), r, CSD_DIM3));
+        auto step = [&](const csd_view& d, int q) {
+            if (q == 0) ok(csd_hal_ntt_mul_mont64_dim(ctx, &d, c.plans_lp.data(), r, CSD_DIM2, &klp, c.mods.data(), r));
+            if (q == 1) ok(csd_hal_ntt_mul_mont64_dim(ctx, &d, c.plans_k.data(), r, CSD_DIM3, &kk, c.mods.data(), r));
+            if (q == 2) ok(csd_hal_intt_mont64_dim(ctx, &d, c.plans_lp.data(), r, CSD_DIM2));
+            if (q == 3) ok(csd_hal_intt_mont64_dim(ctx, &d, c.plans_k.data(), r, CSD_DIM3));
+        };
+        const int64_t cnt = data.shape[0];
+        const int64_t slice_bytes = r * (int64_t)f->deg_lp * (int64_t)f->deg_k * 8;
+        if (!g_overlap_halves || cnt < 2 || cnt * slice_bytes < ((int64_t)64 << 20)) {
+            for (int q = 0; q < 4; q++) step(data, q);
+            return;
+        }
+        // Two halves of the batch on two streams, the second one step behind the first: the
+        // memory-bound dim-2 passes of one half run beside the VALU-bound dim-3 passes of the
+        // other. Each element still sees the same four passes in the same order.
+        if (!aux_stream) HIPCHECK(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+        const csd_view a = sub(data, 0, 0, cnt / 2, 8), b = sub(data, 0, cnt / 2, cnt, 8);
+        hipStream_t main_stream = ctx->stream;
+        hipEvent_t ev[2];
+        HIPCHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        try {
+            step(a, 0);
+            HIPCHECK(hipEventRecord(ev[0], main_stream));  // b starts once a's first pass is issued
+            HIPCHECK(hipStreamWaitEvent(aux_stream, ev[0], 0));
+            for (int q = 0; q < 4; q++) {
+                ctx->stream = aux_stream;
+                step(b, q);
+                ctx->stream = main_stream;
+                if (q + 1 < 4) step(a, q + 1);
+            }
+            HIPCHECK(hipEventRecord(ev[1], aux_stream));
+            HIPCHECK(hipStreamWaitEvent(main_stream, ev[1], 0));
+        } catch (...) {
+            ctx->stream = main_stream;
+            (void)hipStreamSynchronize(aux_stream);
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
+            throw;
+        }
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
     }
 
     BConvCache& bconv_cache(const std::vector<size_t>& src, const std::vector<size_t>& dst) {  // ring.rs:1606-1661
@@ -1171,6 +1216,11 @@ extern "C" int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, siz
     ring->galois_keyswitch(to_vec(base, r), (int64_t)batch, (int64_t)digits, *parts, *c0, std::vector<uint64_t>(gs, gs + ngal),
                            keys, *out);
     CSD_CATCH
+}
+
+extern "C" int csd_ring_set_overlap(int on) {
+    g_overlap_halves = on != 0;
+    return 0;
 }
 
 extern "C" int csd_ring_galois_keyswitch_streamed(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t digits,

@@ -50,6 +50,12 @@ def sample_primes(name, log2_q, t_bits):
     return [out[i] for i in range(n.value)], il.value
 
 
+def set_overlap(on):
+    """Two-stream half-batch overlap of the transforms (csd_ring_set_overlap); results are
+    identical either way."""
+    check(lib().csd_ring_set_overlap(ctypes.c_int(1 if on else 0)))
+
+
 class RingContext:
     def __init__(self, backend, field, log2_q=None, t_bits=0, raw_primes=None):
         self.backend = backend

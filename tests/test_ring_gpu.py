@@ -187,3 +187,34 @@ def test_galois_parity(gpu, cpu, rings):
         dg = gpu.hal_allocate_i64(src.shape)
         g.apply_galois(dg, to_dev(gpu, src), gel)
         assert same(from_dev(gpu, dg), dc)
+
+
+def test_transform_overlap_halves(gpu, cpu, rings):
+    """A batch large enough for the two-stream half-batch overlap (14 elements, 70 MB): the same
+    residues as with the overlap off, and as the oracle."""
+    pkg = load()
+    g, c = rings("field_36864", (960, 980), 76)
+    base = g.initial_rns_base()
+    primes = [g.primes[i] for i in base]
+    rng = np.random.default_rng(404)
+    shape = (14, len(base), 12, 3072)
+    data = uniform_residues(rng, primes, shape)
+    other = uniform_residues(rng, primes, (1,) + shape[1:])
+    mods = [gpu.hal_create_modulus(p) for p in primes]
+    outs = []
+    for on in (True, False):
+        pkg.ring.set_overlap(on)
+        try:
+            dg = to_dev(gpu, data)
+            g.pb_to_crt_tensor(dg, base)
+            gpu.hal_mul_mont64(dg, to_dev(gpu, other), mods)
+            g.crt_to_pb_tensor(dg, base)
+            outs.append(from_dev(gpu, dg))
+        finally:
+            pkg.ring.set_overlap(True)
+    assert same(outs[0], outs[1])
+    dc = data[[0, 13]].copy()
+    c.pb_to_crt_tensor(dc, base)
+    cpu.hal_mul_mont64(dc, other, c.mods_for(base))
+    c.crt_to_pb_tensor(dc, base)
+    assert same(outs[0][[0, 13]], dc)
