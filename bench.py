@@ -249,6 +249,18 @@ def bench_batch(pkg, be, steps, count=256, seed=SEED, cpu=True):
     for _ in range(steps):
         plan.run(graph=True)
     graph_ms = ev.stop(stream) / steps
+    # the other form of the chain, for comparison: persistent one-launch if the tuner chose the
+    # staged launches, else the staged launches
+    chosen = plan.tuning()
+    other = dict(chosen, persistent=0, grid=0) if chosen["persistent"] else dict(chosen, persistent=1, rows=1, grid=0, g2=3, g3=2)
+    plan.set_tuning(**other)
+    plan.run(graph=True)
+    be.sync()
+    ev.start(stream)
+    for _ in range(steps):
+        plan.run(graph=True)
+    other_ms = ev.stop(stream) / steps
+    plan.set_tuning(**chosen)
     # per-kernel split of one eager chain (the graph replays carry no per-kernel events)
     be.set_timing(True)
     plan.run(graph=False)
@@ -272,6 +284,7 @@ def bench_batch(pkg, be, steps, count=256, seed=SEED, cpu=True):
         "speedup_vs_plain": round(plain_ms / graph_ms, 3),
         "chain_GB/s": round(info["bytes"] / (graph_ms / 1000.0) / 1e9, 1),
         "tuning": tuning, "autotune_s": round(tune_s, 2),
+        ("staged_ms_per_step" if chosen["persistent"] else "persistent_ms_per_step"): round(other_ms, 3),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": round(total_ms / launches, 4)},
         "kernels": per_kernel,

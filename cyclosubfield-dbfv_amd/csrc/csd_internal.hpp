@@ -93,6 +93,14 @@ void csd_ntt_segments(csd_ctx* ctx, int64_t* const* seg_base, const int32_t* seg
                       int64_t st2, csd_ntt_plan* const* plans, size_t nplans, int dim, int dir, const csd_view* mul,
                       const csd_modulus* mods, const char* timer, const CsdSegTune& tune);
 
+// the persistent form of the batch plan's transform chain (see csd_ntt.hip)
+int csd_batch_chain_occupancy(int rows);
+bool csd_batch_chain_supported(csd_ntt_plan* const* plp, csd_ntt_plan* const* pk, size_t R);
+void csd_batch_chain_launch(csd_ctx* ctx, int64_t* const* seg_base, const int32_t* seg_limb, int64_t nslices,
+                            csd_ntt_plan* const* plp, csd_ntt_plan* const* pk, size_t R, const int64_t* klp, const int64_t* kk,
+                            const int64_t* iklp, const int64_t* ikk, const int64_t* pt, const csd_modulus* mods, int rows,
+                            int grid, double bytes);
+
 void csd_mod_switch_one_launch(csd_ctx* ctx, int64_t* const* in, int64_t* const* out, const int32_t* level, const void* levels,
                                int64_t nelems, int64_t npos, const csd_modulus* mods, size_t nmods, double bytes);
 size_t csd_mod_switch_level_bytes();

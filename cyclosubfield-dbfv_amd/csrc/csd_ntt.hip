@@ -1485,3 +1485,160 @@ extern "C" int csd_hal_ntt_mul_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd
     launch_ntt(ctx, dst, plans, nplans, dim, 1, mul, mods, nmods);
     CSD_CATCH
 }
+
+// ---------------------------------------------------------------------------------------------
+// The transform chain of the batch plan as one persistent launch: each workgroup owns whole
+// (element, limb) slices, s = blockIdx.x, blockIdx.x + gridDim.x, ..., and runs all eight passes
+// of pb_to_crt -> plaintext product -> crt_to_pb on a slice before taking the next one, so a
+// slice moves between passes through the L2 / Infinity Cache instead of HBM. The passes are the
+// column and compiled-shape row passes above, in the order and with the schedules of the
+// separate launches, so the results are identical. Workgroup barriers order the passes (all
+// waves of a workgroup share one CU's vector L1, so its global writes are visible after them).
+
+struct ChainArgs {
+    int64_t* const* seg_base;
+    const int32_t* seg_limb;
+    int64_t nslices;
+    const DevPlan* plp[CSD_MAX_MODULI];
+    const DevPlan* pk[CSD_MAX_MODULI];
+    const int64_t *klp, *kk, *iklp, *ikk;  // [R][n2] and [R][n3] CRT-kernel multipliers
+    const int64_t* pt;                     // [R][n2][n3] plaintext, CRT basis
+    int rows;                              // lines per LDS round of the row passes
+};
+
+// one column pass (dim 2) of a slice: thread-owned columns, the whole plan in registers
+template <int F0, int F1, bool INV>
+__device__ __forceinline__ void chain_cols(int64_t* __restrict__ base, int n3, const DevPlan* __restrict__ P, int dir,
+                                           const int64_t* __restrict__ mul, const CsdMod& m) {
+    constexpr int L = F0 * F1;
+    const CsdBarrett* __restrict__ TW = (const CsdBarrett*)(P + 1);
+    for (int l = threadIdx.x; l < n3; l += blockDim.x) {
+        int64_t v[L];
+#pragma unroll
+        for (int k = 0; k < L; k++) v[k] = base[k * n3 + l];
+        col_factor<F0, F1, 1, INV>(v, P, TW, 0, dir);
+        if constexpr (F1 > 1) col_factor<F1, 1, F0, INV>(v, P, TW, 1, dir);
+#pragma unroll
+        for (int k = 0; k < L; k++) {
+            int64_t x = v[k];
+            if (mul) x = csd_mont_mul(m, x, mul[k]);
+            base[k * n3 + l] = x;
+        }
+    }
+}
+
+// one row pass (dim 3) of a slice: `rows` lines at a time through LDS; mul_row (length n3) is
+// shared by all rows, mul_mat ([n2][n3]) is per row; at most one of them is set
+template <int SHAPE, bool INV>
+__device__ __forceinline__ void chain_rows(int64_t* __restrict__ lds, int64_t* __restrict__ base, int n2, int rows_per,
+                                           const DevPlan* __restrict__ P, int dir, const int64_t* __restrict__ mul_row,
+                                           const int64_t* __restrict__ mul_mat, const CsdMod& m) {
+    constexpr int L = CtShape<SHAPE>::L;
+    constexpr uint32_t lline = L + L / 8 + 1;
+    const CsdBarrett* __restrict__ TW = (const CsdBarrett*)(P + 1);
+    for (int k0 = 0; k0 < n2; k0 += rows_per) {
+        const int rows = min(rows_per, n2 - k0);
+        const int total = rows * L;
+        for (int g = threadIdx.x; g < total; g += blockDim.x) {
+            const int r = g / L, p = g - r * L;
+            lds[r * lline + lds_slot(p)] = base[(k0 + r) * L + p];
+        }
+        __syncthreads();
+        ct_passes<SHAPE, INV>(lds, rows, P, TW, dir);
+        for (int g = threadIdx.x; g < total; g += blockDim.x) {
+            const int r = g / L, p = g - r * L;
+            int64_t x = lds[r * lline + lds_slot(p)];
+            if (mul_row) x = csd_mont_mul(m, x, mul_row[p]);
+            if (mul_mat) x = csd_mont_mul(m, x, mul_mat[(k0 + r) * L + p]);
+            base[(k0 + r) * L + p] = x;
+        }
+        __syncthreads();
+    }
+}
+
+template <int F0, int F1, int SHAPE>
+__global__ __launch_bounds__(256) void k_batch_chain(ChainArgs a, ModTable mt) {
+    extern __shared__ int64_t lds[];
+    constexpr int N2 = F0 * F1, N3 = CtShape<SHAPE>::L;
+    for (int64_t s = blockIdx.x; s < a.nslices; s += gridDim.x) {
+        int64_t* base = a.seg_base[s];
+        const int j = a.seg_limb[s];
+        const CsdMod m = mt.m[j];
+        const DevPlan* plp = a.plp[j];
+        const DevPlan* pk = a.pk[j];
+        // pb_to_crt (ring.rs:1118-1141)
+        chain_cols<F0, F1, false>(base, N3, plp, 1, a.klp + (int64_t)j * N2, m);
+        __syncthreads();
+        chain_rows<SHAPE, false>(lds, base, N2, a.rows, pk, 1, a.kk + (int64_t)j * N3, nullptr, m);
+        chain_cols<F0, F1, true>(base, N3, plp, 2, nullptr, m);
+        __syncthreads();
+        // intt along dim 3, then the plaintext product (hal mul_mont64)
+        chain_rows<SHAPE, true>(lds, base, N2, a.rows, pk, 2, nullptr, a.pt + (int64_t)j * N2 * N3, m);
+        // crt_to_pb (ring.rs:1144-1161)
+        chain_cols<F0, F1, false>(base, N3, plp, 1, a.iklp + (int64_t)j * N2, m);
+        __syncthreads();
+        chain_rows<SHAPE, false>(lds, base, N2, a.rows, pk, 1, a.ikk + (int64_t)j * N3, nullptr, m);
+        chain_cols<F0, F1, true>(base, N3, plp, 2, nullptr, m);
+        __syncthreads();
+        chain_rows<SHAPE, true>(lds, base, N2, a.rows, pk, 2, nullptr, nullptr, m);
+    }
+}
+
+// Whether the persistent chain has a compiled form for these plans: dim-2 plans of factors
+// [4, 3] and dim-3 plans of compiled shape 1 (the field_36864 slices), default pass grouping.
+bool csd_batch_chain_supported(csd_ntt_plan* const* plp, csd_ntt_plan* const* pk, size_t R) {
+    for (size_t j = 0; j < R; j++) {
+        const DevPlan& a = plp[j]->host;
+        if (a.nfactors != 2 || a.f[0].len != 4 || a.f[1].len != 3) return false;
+        for (int dir : {1, 2})
+            if (!ct_shape_matches(pk[j]->host, dir, 1)) return false;
+    }
+    return R > 0 && R <= CSD_MAX_MODULI;
+}
+
+// workgroups of the persistent chain that fit one CU at once with `rows` lines of LDS
+int csd_batch_chain_occupancy(int rows) {
+    static bool attr = false;
+    if (!attr) {
+        HIPCHECK(hipFuncSetAttribute((const void*)k_batch_chain<4, 3, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    int n = 0;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_chain<4, 3, 1>, 256, (size_t)rows * lds_line(3072) * 8));
+    return std::max(n, 1);
+}
+
+void csd_batch_chain_launch(csd_ctx* ctx, int64_t* const* seg_base, const int32_t* seg_limb, int64_t nslices,
+                            csd_ntt_plan* const* plp, csd_ntt_plan* const* pk, size_t R, const int64_t* klp, const int64_t* kk,
+                            const int64_t* iklp, const int64_t* ikk, const int64_t* pt, const csd_modulus* mods, int rows,
+                            int grid, double bytes) {
+    if (!csd_batch_chain_supported(plp, pk, R)) throw CsdError("persistent chain: no compiled form for these plans");
+    if (rows < 1 || rows > 4) throw CsdError("persistent chain: 1 to 4 lines per LDS round");
+    if (grid < 1) throw CsdError("persistent chain: empty grid");
+    ChainArgs a;
+    memset(&a, 0, sizeof(a));
+    a.seg_base = seg_base;
+    a.seg_limb = seg_limb;
+    a.nslices = nslices;
+    for (size_t j = 0; j < R; j++) {
+        a.plp[j] = (const DevPlan*)plp[j]->dev;
+        a.pk[j] = (const DevPlan*)pk[j]->dev;
+    }
+    a.klp = klp;
+    a.kk = kk;
+    a.iklp = iklp;
+    a.ikk = ikk;
+    a.pt = pt;
+    a.rows = rows;
+    ModTable t = make_mod_table(mods, R, (int64_t)R);
+    const size_t lds = (size_t)rows * lds_line(3072) * 8;
+    static bool attr = false;
+    if (!attr) {
+        HIPCHECK(hipFuncSetAttribute((const void*)k_batch_chain<4, 3, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    if (nslices == 0) return;
+    TimedScope ts(ctx, "batch_chain_persistent", bytes);
+    hipLaunchKernelGGL((k_batch_chain<4, 3, 1>), dim3((unsigned)std::min<int64_t>(grid, nslices)), dim3(256), lds, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+}

@@ -1277,7 +1277,8 @@ struct csd_batch_plan {
     int64_t** ms_out = nullptr;
     int32_t* ms_level = nullptr;
     void* levels = nullptr;
-    csd_batch_tuning tune{0, 0, 256, 3, 2};
+    csd_batch_tuning tune{0, 0, 256, 3, 2, 0, 0};
+    int num_cus = 0;
     hipGraphExec_t graph = nullptr;
 
     ~csd_batch_plan() {
@@ -1314,6 +1315,22 @@ struct csd_batch_plan {
         auto phase = [&](int q) {
             if (ev) HIPCHECK(hipEventRecord(ev[q], ctx->stream));
         };
+        if (tune.persistent) {
+            // the eight transform passes as one persistent launch (csd_batch_chain_launch)
+            phase(0);
+            const int rows = tune.rows > 0 ? tune.rows : 1;
+            const int grid = tune.grid > 0 ? tune.grid : default_grid(rows);
+            double cb = 8 * 2 * 8.0 * n2 * n3 * nslices + 8.0 * n2 * n3 * nslices;
+            csd_batch_chain_launch(ctx, seg_base, seg_limb, nslices, tc->plans_lp.data(), tc->plans_k.data(), (size_t)R,
+                                   (const int64_t*)tc->k_lp->p, (const int64_t*)tc->k_k->p, (const int64_t*)tc->ik_lp->p,
+                                   (const int64_t*)tc->ik_k->p, (const int64_t*)pt.ptr, m, rows, grid, cb);
+            for (int q = 1; q <= 8; q++) phase(q);
+            double msb = 0;
+            for (size_t c = 0; c < limbs.size(); c++) msb += 2.0 * 8.0 * n2 * n3 * (2 * limbs[c] - 1);
+            csd_mod_switch_one_launch(ctx, ms_in, ms_out, ms_level, levels, nelems, n2 * n3, m, (size_t)R, msb);
+            phase(9);
+            return;
+        }
         // pb_to_crt (ring.rs:1118-1141)
         phase(0);
         csd_ntt_segments(ctx, seg_base, seg_limb, nslices, n2, n3, n3, tc->plans_lp.data(), R, 2, 1, &klp, m, "batch_ntt_mul_dim2", t);
@@ -1339,6 +1356,20 @@ struct csd_batch_plan {
         for (size_t c = 0; c < limbs.size(); c++) msb += 2.0 * 8.0 * n2 * n3 * (2 * limbs[c] - 1);
         csd_mod_switch_one_launch(ctx, ms_in, ms_out, ms_level, levels, nelems, n2 * n3, m, (size_t)R, msb);
         phase(9);
+    }
+
+    // workgroups resident at once: CUs x the persistent chain's occupancy with `rows` lines of LDS
+    int default_grid(int rows) {
+        if (!num_cus) {
+            hipDeviceProp_t prop;
+            HIPCHECK(hipGetDeviceProperties(&prop, ring->ctx->device));
+            num_cus = prop.multiProcessorCount;
+        }
+        return num_cus * csd_batch_chain_occupancy(rows);
+    }
+
+    bool persistent_supported() const {
+        return csd_batch_chain_supported(tc->plans_lp.data(), tc->plans_k.data(), base.size());
     }
 
     void drop_graph() {
@@ -1387,6 +1418,10 @@ struct csd_batch_plan {
         if (t.threads < 0 || t.threads > 512 || t.threads % 64) throw CsdError("tuning: threads");
         if (t.rows < 0 || t.rows > 16) throw CsdError("tuning: rows");
         if (t.rows > 0 && (size_t)t.rows * (n3 + n3 / 8 + 1) * 8 > 160 * 1024) throw CsdError("tuning: rows exceed LDS");
+        if (t.persistent != 0 && t.persistent != 1) throw CsdError("tuning: persistent is 0 or 1");
+        if (t.grid < 0) throw CsdError("tuning: grid");
+        if (t.persistent && (t.rows > 4 || t.g2 != 3 || t.g3 != 2)) throw CsdError("tuning: the persistent chain runs 1-4 lines, default grouping");
+        if (t.persistent && !persistent_supported()) throw CsdError("tuning: no persistent chain for this field");
         drop_graph();
         if (t.g2 != tune.g2 || t.g3 != tune.g3) {
             for (auto p : tc->plans_lp) ok(csd_ntt_plan_set_grouping(p, t.g2, t.g3));
@@ -1437,6 +1472,8 @@ struct csd_batch_plan {
         auto dim2_ms = [](const double* ph) { return ph[0] + ph[2] + ph[4] + ph[6]; };
         double ph[9];
         csd_batch_tuning best = tune;
+        best.persistent = 0;  // the staged search first; the persistent chain is compared after it
+        best.grid = 0;
         try {
             set_tuning(best);
             time_chain(reps, ph);
@@ -1482,12 +1519,38 @@ struct csd_batch_plan {
                 }
             }
             set_tuning(best);
+            // the persistent chain against the best staged form, on the whole chain time
+            if (persistent_supported()) {
+                double best_total = time_chain(reps, nullptr);
+                csd_batch_tuning staged = best;
+                for (int rows : {1, 2})
+                    for (int mult : {1, 2, 0}) {
+                        csd_batch_tuning c = staged;
+                        c.persistent = 1;
+                        c.rows = rows;
+                        c.g2 = 3;
+                        c.g3 = 2;
+                        c.grid = mult ? num_cus_or_query() * mult : 0;
+                        set_tuning(c);
+                        const double tot = time_chain(reps, nullptr);
+                        if (tot < best_total) {
+                            best_total = tot;
+                            best = c;
+                        }
+                    }
+                set_tuning(best);
+            }
         } catch (...) {
             ring->ctx->timing = timing;
             throw;
         }
         ring->ctx->timing = timing;
         return time_chain(reps, nullptr);
+    }
+
+    int num_cus_or_query() {
+        if (!num_cus) (void)default_grid(1);
+        return num_cus;
     }
 };
 

@@ -316,7 +316,7 @@ class WideRingEls:
 class BatchTuning(ctypes.Structure):
     """Launch shapes of the batched chain (csd_batch_tuning)."""
     _fields_ = [("rows", ctypes.c_int), ("threads", ctypes.c_int), ("col_threads", ctypes.c_int), ("g2", ctypes.c_int),
-                ("g3", ctypes.c_int)]
+                ("g3", ctypes.c_int), ("persistent", ctypes.c_int), ("grid", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

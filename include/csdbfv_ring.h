@@ -128,9 +128,12 @@ int csd_ring_transform_matrix(csd_ring* ring, const size_t* base, size_t r, int 
 typedef struct csd_batch_plan csd_batch_plan;
 /* launch shapes: dim-3 lines per workgroup and workgroup size (0 = built-in choice), dim-2
  * columns per workgroup (64..256), NTT pass grouping (radix-2 stages g2 in 1..3, radix-3 stages
- * g3 in 1..2 per LDS pass; applies to the plans of the whole base) */
+ * g3 in 1..2 per LDS pass; applies to the plans of the whole base); persistent = 1 runs the eight
+ * transform passes as one persistent launch in which each workgroup owns whole slices (lines per
+ * LDS round = rows, 1..4; grid workgroups, 0 = all that fit at once; field_36864 slices, default
+ * grouping), 0 runs one launch per pass. */
 typedef struct csd_batch_tuning {
-    int rows, threads, col_threads, g2, g3;
+    int rows, threads, col_threads, g2, g3, persistent, grid;
 } csd_batch_tuning;
 int csd_ring_batch_create(csd_ring* ring, const size_t* base, size_t r, size_t count, int64_t* const* cts,
                           int64_t* const* outs, const size_t* limbs, const csd_view* pt, csd_batch_plan** out);

@@ -92,16 +92,22 @@ def test_batch_graph_replays(setup, case):
         check_run(gpu, plan, dcts, douts, cts, expect, graph=True)
 
 
-@pytest.mark.parametrize("tune", [dict(rows=1, threads=128, col_threads=64, g2=1, g3=1),
-                                  dict(rows=2, threads=512, col_threads=128, g2=2, g3=2),
-                                  dict(rows=4, threads=384, col_threads=192, g2=3, g3=1),
-                                  dict(rows=0, threads=0, col_threads=256, g2=3, g3=2)])
+@pytest.mark.parametrize("tune", [dict(rows=1, threads=128, col_threads=64, g2=1, g3=1, persistent=0),
+                                  dict(rows=2, threads=512, col_threads=128, g2=2, g3=2, persistent=0),
+                                  dict(rows=4, threads=384, col_threads=192, g2=3, g3=1, persistent=0),
+                                  dict(rows=0, threads=0, col_threads=256, g2=3, g3=2, persistent=0),
+                                  dict(rows=1, g2=3, g3=2, persistent=1, grid=0),
+                                  dict(rows=2, g2=3, g3=2, persistent=1, grid=16),
+                                  dict(rows=3, g2=3, g3=2, persistent=1, grid=7)])
 def test_batch_launch_shapes(setup, case, tune):
-    """every launch shape and pass grouping computes the same residues"""
+    """every launch shape and pass grouping, and the persistent one-launch chain with grids
+    smaller than the slice count (each workgroup then loops over many slices), compute the same
+    residues"""
     gpu = setup[0]
     plan, dcts, douts, cts, expect = case
     plan.set_tuning(**tune)
-    assert plan.tuning() == tune
+    got = plan.tuning()
+    assert all(got[k] == v for k, v in tune.items())
     check_run(gpu, plan, dcts, douts, cts, expect, graph=False)
     check_run(gpu, plan, dcts, douts, cts, expect, graph=True)
 
@@ -113,7 +119,7 @@ def test_batch_autotune(setup, case):
     assert ms > 0
     assert plan.tuning() == chosen
     check_run(gpu, plan, dcts, douts, cts, expect, graph=True)
-    plan.set_tuning(rows=0, threads=0, col_threads=256, g2=3, g3=2)
+    plan.set_tuning(rows=0, threads=0, col_threads=256, g2=3, g3=2, persistent=0, grid=0)
 
 
 def test_batch_single_and_empty(setup, cpu):
@@ -149,4 +155,6 @@ def test_batch_errors(setup):
         plan.set_tuning(g2=4)
     with pytest.raises(pkg.HalError):
         plan.set_tuning(rows=7)  # 7 lines of 3072 do not fit the 160 KiB LDS
+    with pytest.raises(pkg.HalError):
+        plan.set_tuning(rows=1, persistent=1, g2=2)  # the persistent chain runs the default grouping only
     plan.close()
