@@ -148,6 +148,29 @@ static void launch_digits(csd_ctx* ctx, const int64_t* src, const VecLayout& L, 
 // consecutive k of k-group g; the output of a block sits in lane (v + 16 g) as rows 4 g .. 4 g + 3
 // of vector v. A and X use the same k assignment, so every product pairs equal k.
 
+// one K-step's operands of a wave: 8 digit planes of its 16 rows of A, and of its 2 x 16 vectors
+__device__ __forceinline__ void zp_load_step(v4i (&ra)[ZP_DIGITS], v4i (&rx)[2][ZP_DIGITS], const int8_t* A, const int8_t* X,
+                                             int64_t pa, int64_t px, int64_t kpad, int64_t k0) {
+#pragma unroll
+    for (int d = 0; d < ZP_DIGITS; d++) ra[d] = *(const v4i*)(A + d * pa + k0);
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int d = 0; d < ZP_DIGITS; d++) rx[u][d] = *(const v4i*)(X + d * px + u * 16 * kpad + k0);
+}
+
+// the 2 x 64 digit-pair MFMAs of one K-step, accumulated by shift
+__device__ __forceinline__ void zp_mfma_step(v4i (&acc)[2][2 * ZP_DIGITS - 1], const v4i (&ra)[ZP_DIGITS],
+                                             const v4i (&rx)[2][ZP_DIGITS]) {
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int da = 0; da < ZP_DIGITS; da++)
+#pragma unroll
+            for (int dx = 0; dx < ZP_DIGITS; dx++)
+                acc[u][da + dx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ra[da], rx[u][dx], acc[u][da + dx], 0, 0, 0);
+}
+
 struct MfmaArgs {
     const int8_t* a;  // [r][8][mpad][kpad]
     const int8_t* x;  // [r][8][npad][kpad]
@@ -185,22 +208,18 @@ __global__ __launch_bounds__(256) void k_zp_mfma(MfmaArgs a, ModTable mt) {
         for (int u = 0; u < 2; u++)
 #pragma unroll
             for (int s = 0; s < 2 * ZP_DIGITS - 1; s++) acc[u][s] = (v4i){0, 0, 0, 0};
-        for (int64_t k0 = kc; k0 < ke; k0 += 64) {
-            v4i av[ZP_DIGITS];
-#pragma unroll
-            for (int d = 0; d < ZP_DIGITS; d++) av[d] = *(const v4i*)(A + d * pa + k0);
-#pragma unroll
-            for (int u = 0; u < 2; u++) {
-                v4i xv[ZP_DIGITS];
-#pragma unroll
-                for (int d = 0; d < ZP_DIGITS; d++) xv[d] = *(const v4i*)(X + d * px + u * 16 * a.kpad + k0);
-#pragma unroll
-                for (int da = 0; da < ZP_DIGITS; da++)
-#pragma unroll
-                    for (int dx = 0; dx < ZP_DIGITS; dx++)
-                        acc[u][da + dx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[da], xv[dx], acc[u][da + dx], 0, 0, 0);
-            }
+        // software pipeline: the operands of a K-step are loaded one step ahead into the other
+        // register set, so their memory latency overlaps the 128 MFMAs of the step before
+        v4i ca[ZP_DIGITS], cx[2][ZP_DIGITS], na[ZP_DIGITS], nx[2][ZP_DIGITS];
+        zp_load_step(ca, cx, A, X, pa, px, a.kpad, kc);
+        int64_t k0 = kc;
+        for (; k0 + 128 <= ke; k0 += 128) {
+            zp_load_step(na, nx, A, X, pa, px, a.kpad, k0 + 64);
+            zp_mfma_step(acc, ca, cx);
+            if (k0 + 128 < ke) zp_load_step(ca, cx, A, X, pa, px, a.kpad, k0 + 128);
+            zp_mfma_step(acc, na, nx);
         }
+        if (k0 < ke) zp_mfma_step(acc, ca, cx);  // odd step count: the last step sits in ca
 #pragma unroll
         for (int u = 0; u < 2; u++)
 #pragma unroll
