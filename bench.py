@@ -69,8 +69,22 @@ def traffic_from_profiles(kernel):
     return None if entry is None else entry.get("bytes_per_launch")
 
 
-def cpu_baseline(primes_base, host_data, sample):
-    """The oracle (single-threaded C++ restatement of hal::cpu) on `sample` elements."""
+def cpu_threads():
+    """host threads for the CPU baseline: the cores this process may use, at most 16"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(primes_base, host_data, sample, threads, min_seconds=10.0):
+    """The oracle (the C++ restatement of hal::cpu) on a bounded sample: pb->CRT + CRT->pb of
+    `sample` elements of the batch, first on one thread, then on `threads` threads over disjoint
+    elements (every op is independent per element; the C++ calls release the GIL), repeated until
+    at least `min_seconds` of wall time."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from oracle import cpu_backend
     from oracle import ring as oring
 
@@ -79,13 +93,28 @@ def cpu_baseline(primes_base, host_data, sample):
     base = ring.initial_rns_base()
     assert [ring.primes[i] for i in base] == primes_base
     ring.transform_cache(base)
-    x = np.ascontiguousarray(host_data[:sample])
+    x = np.ascontiguousarray(host_data[:1])
     t0 = time.perf_counter()
     ring.pb_to_crt_tensor(x, base)
     ring.crt_to_pb_tensor(x, base)
-    dt = time.perf_counter() - t0
-    return {"value": round(2 * sample / dt, 3), "unit": "transforms/s", "cores": 1, "kind": "port",
-            "sample": f"{sample} ring elements of the same batch, pb->CRT + CRT->pb ({2 * sample} transforms, {dt:.1f} s)"}
+    one = 2 / (time.perf_counter() - t0)
+    chunks = [np.ascontiguousarray(host_data[i:i + 1]) for i in range(sample)]
+
+    def work(c):
+        ring.pb_to_crt_tensor(c, base)
+        ring.crt_to_pb_tensor(c, base)
+
+    done, t0 = 0, time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        while True:
+            list(ex.map(work, chunks))
+            done += 2 * len(chunks)
+            dt = time.perf_counter() - t0
+            if dt >= min_seconds:
+                break
+    return {"value": round(done / dt, 3), "unit": "transforms/s", "cores": threads, "kind": "port",
+            "sample": f"{sample} ring elements of the same batch, pb->CRT + CRT->pb on {threads} threads over disjoint "
+                      f"elements, {done} transforms in {dt:.1f} s; one thread: {one:.1f} transforms/s"}
 
 
 def bench_galois(pkg, be, steps, batch=32, seed=SEED, cpu=True):
@@ -494,7 +523,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--count", type=int, default=128, help="ring elements in the batch (2 per ciphertext)")
-    ap.add_argument("--cpu-sample", type=int, default=4, help="ring elements in the CPU-baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=64, help="ring elements in the CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workloads", default="hom_mul,batch,zp,galois",
                     help="comma-separated extra workloads beside the headline transforms: hom_mul, batch, zp, galois, "
@@ -611,7 +641,8 @@ def main():
         "wall_s": round(wall, 3),
     }
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(primes, host, min(args.cpu_sample, args.count))
+        result["cpu_baseline"] = cpu_baseline(primes, host, min(args.cpu_sample, args.count), cpu_threads(),
+                                              args.cpu_seconds)
     ring.close()
     del data
     workloads = {}
