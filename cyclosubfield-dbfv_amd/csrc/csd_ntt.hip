@@ -855,6 +855,36 @@ __device__ __forceinline__ void ct_pass(int64_t* __restrict__ lds, int rows, con
     __syncthreads();
 }
 
+// the last pass of a compiled shape, fused with the store: the register block goes straight to
+// global memory, through the optional Montgomery multiplier mul[row * ms2 + p * ms3]
+template <int L, int FI, int R, int G, int S, int SL, int S0, bool INV>
+__device__ __forceinline__ void ct_pass_store(const int64_t* __restrict__ lds, int rows, const DevPlan* __restrict__ P,
+                                              const CsdBarrett* __restrict__ TW, int dir, int64_t* __restrict__ out, int64_t st2,
+                                              const int64_t* __restrict__ mul, int64_t ms2, int64_t ms3, const CsdMod& md) {
+    constexpr int M = Pow<R>::of(G);
+    constexpr uint32_t sub = SL / M, NG = L / M, stride = sub * S, lline = L + L / 8 + 1;
+    const DevFactor& F = P->f[FI];
+    const uint32_t total = NG * (uint32_t)rows;
+    for (uint32_t w = threadIdx.x; w < total; w += blockDim.x) {
+        const uint32_t row = w / NG, gam = w % NG;
+        const uint32_t t = gam / S, suffix = gam % S;
+        const uint32_t blk0 = t / sub, jj = t % sub;
+        const uint32_t base = (blk0 * SL + jj) * S + suffix;
+        const int64_t* line = lds + row * lline;
+        int64_t v[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) v[m] = line[lds_slot(base + m * stride)];
+        ct_stages<R, G, S0, INV>(v, P, TW, F, dir, blk0);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const uint32_t p = base + m * stride;
+            int64_t x = v[m];
+            if (mul) x = csd_mont_mul(md, x, mul[row * ms2 + p * ms3]);
+            out[row * st2 + p] = x;
+        }
+    }
+}
+
 // The compiled shapes. SHAPE 1: factors [2, 3, 8, 64] (field_36864, deg_K = 3072); SHAPE 2:
 // [3, 64, 64] (field_49152, deg_K = 12288); both with the default grouping (3 radix-2 / 1 radix-3
 // stages per pass), forward passes in increasing and inverse passes in decreasing stage order.
@@ -869,35 +899,106 @@ struct CtShape<2> {
     static constexpr int L = 12288;
 };
 
-template <int SHAPE, bool INV>
+template <int SHAPE, bool INV, bool FUSED_HEAD = false, bool FUSED_TAIL = false>
 __device__ __forceinline__ void ct_passes(int64_t* lds, int rows, const DevPlan* P, const CsdBarrett* TW, int dir) {
     if constexpr (SHAPE == 1) {
         constexpr int L = 3072;
-        ct_pass<L, 0, 2, 1, 1536, 2, 0, INV>(lds, rows, P, TW, dir);
-        ct_pass<L, 1, 3, 1, 512, 3, 0, INV>(lds, rows, P, TW, dir);
+        if constexpr (!FUSED_HEAD) {
+            ct_pass<L, 0, 2, 1, 1536, 2, 0, INV>(lds, rows, P, TW, dir);
+            ct_pass<L, 1, 3, 1, 512, 3, 0, INV>(lds, rows, P, TW, dir);
+        }
         ct_pass<L, 2, 2, 3, 64, 8, 0, INV>(lds, rows, P, TW, dir);
         if constexpr (!INV) {
             ct_pass<L, 3, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
-            ct_pass<L, 3, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
+            if constexpr (!FUSED_TAIL) ct_pass<L, 3, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
         } else {
             ct_pass<L, 3, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
-            ct_pass<L, 3, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
+            if constexpr (!FUSED_TAIL) ct_pass<L, 3, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
         }
     } else {
         constexpr int L = 12288;
-        ct_pass<L, 0, 3, 1, 4096, 3, 0, INV>(lds, rows, P, TW, dir);
+        if constexpr (!FUSED_HEAD) ct_pass<L, 0, 3, 1, 4096, 3, 0, INV>(lds, rows, P, TW, dir);
         if constexpr (!INV) {
             ct_pass<L, 1, 2, 3, 64, 64, 0, INV>(lds, rows, P, TW, dir);
             ct_pass<L, 1, 2, 3, 64, 8, 3, INV>(lds, rows, P, TW, dir);
             ct_pass<L, 2, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
-            ct_pass<L, 2, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
+            if constexpr (!FUSED_TAIL) ct_pass<L, 2, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
         } else {
             ct_pass<L, 1, 2, 3, 64, 8, 3, INV>(lds, rows, P, TW, dir);
             ct_pass<L, 1, 2, 3, 64, 64, 0, INV>(lds, rows, P, TW, dir);
             ct_pass<L, 2, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir);
-            ct_pass<L, 2, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
+            if constexpr (!FUSED_TAIL) ct_pass<L, 2, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir);
         }
     }
+}
+
+// the pass ct_passes<..., FUSED_TAIL = true> leaves out, run with the store
+template <int SHAPE, bool INV>
+__device__ __forceinline__ void ct_store_tail(const int64_t* lds, int rows, const DevPlan* P, const CsdBarrett* TW, int dir,
+                                              int64_t* out, int64_t st2, const int64_t* mul, int64_t ms2, int64_t ms3,
+                                              const CsdMod& md) {
+    if constexpr (SHAPE == 1) {
+        if constexpr (!INV)
+            ct_pass_store<3072, 3, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir, out, st2, mul, ms2, ms3, md);
+        else
+            ct_pass_store<3072, 3, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir, out, st2, mul, ms2, ms3, md);
+    } else {
+        if constexpr (!INV)
+            ct_pass_store<12288, 2, 2, 3, 1, 8, 3, INV>(lds, rows, P, TW, dir, out, st2, mul, ms2, ms3, md);
+        else
+            ct_pass_store<12288, 2, 2, 3, 1, 64, 0, INV>(lds, rows, P, TW, dir, out, st2, mul, ms2, ms3, md);
+    }
+}
+
+// Global -> LDS load of `rows` lines fused with the leading single-stage passes of a compiled
+// shape: SHAPE 1 runs factor 0 (C_2, suffix 1536) and factor 1 (C_3, suffix 512) on the six
+// elements e + 512 m (m < 6) of each suffix e < 512 in registers; SHAPE 2 runs factor 0 (C_3,
+// suffix 4096) on the three elements e + 4096 m. The groups, twiddles (block 0, uniform) and
+// schedule bits are those of the corresponding ct_pass calls.
+template <int SHAPE, bool INV>
+__device__ __forceinline__ void ct_load_head(int64_t* __restrict__ lds, const int64_t* __restrict__ base, int64_t st2, int rows,
+                                             const DevPlan* __restrict__ P, const CsdBarrett* __restrict__ TW, int dir) {
+    if constexpr (SHAPE == 1) {
+        constexpr uint32_t L = 3072, E = 512, lline = L + L / 8 + 1;
+        const DevFactor& F0 = P->f[0];
+        const DevFactor& F1 = P->f[1];
+        for (uint32_t w = threadIdx.x; w < E * (uint32_t)rows; w += blockDim.x) {
+            const uint32_t r = w / E, e = w % E;
+            int64_t v[6];
+#pragma unroll
+            for (int m = 0; m < 6; m++) v[m] = base[r * st2 + e + E * m];
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                int64_t g[2] = {v[m], v[m + 3]};
+                ct_stages<2, 1, 0, INV>(g, P, TW, F0, dir, 0u);
+                v[m] = g[0];
+                v[m + 3] = g[1];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                int64_t g[3] = {v[3 * h], v[3 * h + 1], v[3 * h + 2]};
+                ct_stages<3, 1, 0, INV>(g, P, TW, F1, dir, 0u);
+                v[3 * h] = g[0];
+                v[3 * h + 1] = g[1];
+                v[3 * h + 2] = g[2];
+            }
+#pragma unroll
+            for (int m = 0; m < 6; m++) lds[r * lline + lds_slot(e + E * m)] = v[m];
+        }
+    } else {
+        constexpr uint32_t L = 12288, E = 4096, lline = L + L / 8 + 1;
+        const DevFactor& F0 = P->f[0];
+        for (uint32_t w = threadIdx.x; w < E * (uint32_t)rows; w += blockDim.x) {
+            const uint32_t r = w / E, e = w % E;
+            int64_t g[3];
+#pragma unroll
+            for (int m = 0; m < 3; m++) g[m] = base[r * st2 + e + E * m];
+            ct_stages<3, 1, 0, INV>(g, P, TW, F0, dir, 0u);
+#pragma unroll
+            for (int m = 0; m < 3; m++) lds[r * lline + lds_slot(e + E * m)] = g[m];
+        }
+    }
+    __syncthreads();
 }
 
 // the (factor, s0, g) pass list a compiled shape runs, for checking a plan against it
@@ -997,15 +1098,29 @@ __global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
     const uint32_t lline = lds_line((uint32_t)L);
     const int rows = (int)min((int64_t)a.rows, a.d2 - k0);
     const int total = rows * L;
-    for (int g = threadIdx.x; g < total; g += blockDim.x) {
-        const uint32_t r = a.rows == 1 ? 0u : (SHAPE ? (uint32_t)g / (uint32_t)L : fdiv((uint32_t)g, a.magic_L));
-        const int p = g - (int)r * L;
-        lds[r * lline + lds_slot(p)] = base[r * a.st[2] + p];
-    }
-    __syncthreads();
     const int dir = a.dir;
     if constexpr (SHAPE != 0) {
-        ct_passes<SHAPE, INV>(lds, rows, P, TW, dir);
+        ct_load_head<SHAPE, INV>(lds, base, a.st[2], rows, P, TW, dir);
+        if constexpr (INV) {
+            // the inverse ends on the stride-8 groups of the last factor's first stages: lanes
+            // hold interleaved runs, so its register blocks go to global memory directly, coalesced
+            ct_passes<SHAPE, INV, true, true>(lds, rows, P, TW, dir);
+            const int64_t* mul = a.mul ? a.mul + i * a.smul.s[0] + j * a.smul.s[1] + k0 * a.smul.s[2] : nullptr;
+            ct_store_tail<SHAPE, INV>(lds, rows, P, TW, dir, base, a.st[2], mul, a.smul.s[2], a.smul.s[3], mt.m[j]);
+            return;
+        } else {
+            // the forward ends on 8 consecutive elements per lane; its store goes through LDS below
+            ct_passes<SHAPE, INV, true, false>(lds, rows, P, TW, dir);
+        }
+    } else {
+        for (int g = threadIdx.x; g < total; g += blockDim.x) {
+            const uint32_t r = a.rows == 1 ? 0u : fdiv((uint32_t)g, a.magic_L);
+            const int p = g - (int)r * L;
+            lds[r * lline + lds_slot(p)] = base[r * a.st[2] + p];
+        }
+        __syncthreads();
+    }
+    if constexpr (SHAPE != 0) {
     } else if (P->nfactors == 0) {
         if (dir != 1) {
             const CsdBarrett m = dir == 2 ? P->inv_len : P->one;
