@@ -127,3 +127,65 @@ def bfv_hom_mul(ring, t, lhs, rhs, base, keys, digits):
     ring.crt_to_pb_tensor(prod, lifted)
     rescaled = rescale_to_base(ring, prod, lifted, base_len, t)
     return relin_crt(ring, rescaled, base, keys, digits)
+
+
+def _rows(t, first, step):
+    """rows first, first + step, ... of a batch tensor, as a strided view (no copy)"""
+    from .hal import DeviceTensor
+
+    n = (t.shape[0] - first + step - 1) // step
+    return DeviceTensor(t.backend, t.ptr + first * t.strides[0] * t.elem, (n,) + t.shape[1:],
+                        (t.strides[0] * step,) + t.strides[1:], t.elem, t._owner or t)
+
+
+def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
+    """bfv_hom_mul (bfv.rs:747-775) for B ciphertext pairs at once: lhs, rhs are [2B, r, ., .]
+    CRT-basis tensors with pair b in rows 2b, 2b + 1; the result has the same layout. Every HAL
+    call covers all B pairs, except the gadget base conversion, which takes one element
+    (bfv.rs:417-421) and runs per pair. Each element sees the same calls with the same operands,
+    in the same order, as in B calls of bfv_hom_mul, and the wide accumulators track the same
+    per-row bounds, so the residues are identical."""
+    be = ring.backend
+    B = lhs.shape[0] // 2
+    r = len(base)
+
+    def lift(ct):
+        ct_pb = ring.allocate(2 * B, base)
+        be.hal_assign(ct_pb, ct)
+        ring.crt_to_pb_tensor(ct_pb, base)
+        return lift_to_extended(ring, ct_pb, base, crt=ct)
+
+    lhs_l, lifted = lift(lhs)
+    rhs_l, _ = lift(rhs)
+    # hom_mul_raw_acc (bfv.rs:667-683) with the wide rows laid out [3][B]: row k B + b
+    acc = ring.allocate_wide(3 * B, lifted)
+    l0, l1, r0, r1 = _rows(lhs_l, 0, 2), _rows(lhs_l, 1, 2), _rows(rhs_l, 0, 2), _rows(rhs_l, 1, 2)
+    acc.fma(range(0, B), l0, 1, r0, 1)
+    acc.fma(range(B, 2 * B), l0, 1, r1, 1)
+    acc.fma(range(B, 2 * B), l1, 1, r0, 1)
+    acc.fma(range(2 * B, 3 * B), l1, 1, r1, 1)
+    prod = ring.reduce_wide(acc)
+    ring.crt_to_pb_tensor(prod, lifted)
+    rescaled = rescale_to_base(ring, prod, lifted, len(base), t)  # [3][B] over base
+    # relin (bfv.rs:699-741): gadget digits of c2 (row 2B + b) per pair, then all at once
+    dg = clip_digits(digits, r)
+    d = len(dg)
+    parts = ring.allocate(B * d, base)  # pair b, digit i at row b d + i
+    tmp = ring.allocate(1, base)
+    for b in range(B):
+        be.hal_assign(tmp, rescaled[2 * B + b:2 * B + b + 1])
+        ring.bconv_gadget_tensors(parts[b * d:(b + 1) * d], tmp, base, dg)
+    ring.pb_to_crt_tensor(parts, base)
+    acc2 = ring.allocate_wide(2 * B, base)  # row s B + b
+    if d:
+        for s in (0, 1):
+            be.hal_inner_product_i64(acc2.data[s * B:(s + 1) * B], [_rows(parts, i, d) for i in range(d)],
+                                     [keys[i][s:s + 1, :r] for i in range(d)])
+        acc2.record_step(range(0, 2 * B), d * acc2.max_p * acc2.max_p)
+    ring.pb_to_crt_tensor(rescaled[0:2 * B], base)
+    acc2.add(range(0, 2 * B), rescaled[0:2 * B], 1)
+    out = ring.reduce_wide(acc2)
+    result = ring.allocate(2 * B, base)
+    be.hal_assign(_rows(result, 0, 2), out[0:B])
+    be.hal_assign(_rows(result, 1, 2), out[B:2 * B])
+    return result

@@ -94,3 +94,25 @@ def test_select_digits_matches_oracle():
     for r in (1, 2, 17, 23, 37):
         for d in range(1, r + 1):
             assert pkg.bfv.select_digits(d, r) == R.select_digits(d, r)
+
+
+@pytest.mark.parametrize("field,log2_q,t_bits,ndigits,pairs", [("field_18", (100, 120), 4, 2, 3),
+                                                               ("field_36864", (960, 980), 76, 9, 3)])
+def test_hom_mul_batch_matches_per_pair(gpu, rings, field, log2_q, t_bits, ndigits, pairs):
+    """the batched ring multiplication gives, pair by pair, the residues of bfv_hom_mul"""
+    pkg = load()
+    g, c = rings(field, log2_q, t_bits)
+    base = g.initial_rns_base()
+    primes = [g.primes[i] for i in base]
+    rng = np.random.default_rng(55)
+    n2, n3 = g.info["deg_lprime"], g.info["deg_k"]
+    lhs = uniform_residues(rng, primes, (2 * pairs, len(base), n2, n3))
+    rhs = uniform_residues(rng, primes, (2 * pairs, len(base), n2, n3))
+    digits = R.select_digits(ndigits, len(base))
+    keys = [to_dev(gpu, uniform_residues(rng, primes, (2, len(base), n2, n3))) for _ in digits]
+    t = 1 << t_bits
+    got = from_dev(gpu, pkg.bfv.bfv_hom_mul_batch(g, t, to_dev(gpu, lhs), to_dev(gpu, rhs), base, keys, digits))
+    for b in range(pairs):
+        one = pkg.bfv.bfv_hom_mul(g, t, to_dev(gpu, lhs[2 * b:2 * b + 2]), to_dev(gpu, rhs[2 * b:2 * b + 2]), base, keys,
+                                  digits)
+        assert same(got[2 * b:2 * b + 2], from_dev(gpu, one)), b
