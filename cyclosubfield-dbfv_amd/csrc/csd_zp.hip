@@ -244,6 +244,110 @@ __global__ __launch_bounds__(256) void k_zp_mfma(MfmaArgs a, ModTable mt) {
     }
 }
 
+// The LDS-staged form: 8 waves share one 32-row x 64-vector tile. Each K-step's digit planes of
+// the tile (16 KiB of A, 32 KiB of X) are loaded from global memory once per workgroup, written
+// to one of two LDS buffers while the waves run the MFMAs of the other, and every wave keeps one
+// 16 x 16 output block (15 accumulators), so two waves per SIMD hide each other's LDS latency.
+// The operand layout in LDS is [plane][row][64 k] with the same k-group assignment as above.
+
+static constexpr int ZPL_THREADS = 512;
+static constexpr int ZPL_A_BYTES = ZP_DIGITS * 32 * 64;  // 16 KiB
+static constexpr int ZPL_X_BYTES = ZP_DIGITS * 64 * 64;  // 32 KiB
+
+__global__ __launch_bounds__(512) void k_zp_mfma_lds(MfmaArgs a, ModTable mt) {
+    extern __shared__ __align__(16) int8_t zs[];  // [2][A tile | X tile]
+    int64_t b = blockIdx.x;
+    if (a.nblocks % 8 == 0) b = (b % 8) * (a.nblocks / 8) + b / 8;
+    const int64_t nt = b % a.ntiles;
+    const int64_t mtile = (b / a.ntiles) % a.mtiles;
+    const int64_t l = b / (a.ntiles * a.mtiles);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x16 = lane & 15, g = lane >> 4;
+    const int rb = wave & 1, cb = wave >> 1;  // this wave's 16-row block and 16-vector block
+    const int64_t pa = a.mpad * a.kpad, px = a.npad * a.kpad;
+    const int8_t* Ag = a.a + (l * ZP_DIGITS * a.mpad + mtile * 32) * a.kpad;
+    const int8_t* Xg = a.x + (l * ZP_DIGITS * a.npad + nt * 64) * a.kpad;
+    // staging: A has 1024 16-byte chunks (2 per thread), X 2048 (4 per thread); chunk c of a tile
+    // is plane c / (rows * 4), row (c / 4) % rows, k-group c % 4
+    auto stage_load = [&](v4i (&ra)[2], v4i (&rx)[4], int64_t k0) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int c = tid + q * ZPL_THREADS;
+            const int pl = c >> 7, row = (c >> 2) & 31, kg = c & 3;
+            ra[q] = *(const v4i*)(Ag + pl * pa + row * a.kpad + k0 + kg * 16);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = tid + q * ZPL_THREADS;
+            const int pl = c >> 8, row = (c >> 2) & 63, kg = c & 3;
+            rx[q] = *(const v4i*)(Xg + pl * px + row * a.kpad + k0 + kg * 16);
+        }
+    };
+    auto stage_store = [&](const v4i (&ra)[2], const v4i (&rx)[4], int buf) {
+        int8_t* base = zs + buf * (ZPL_A_BYTES + ZPL_X_BYTES);
+#pragma unroll
+        for (int q = 0; q < 2; q++) *(v4i*)(base + (tid + q * ZPL_THREADS) * 16) = ra[q];
+#pragma unroll
+        for (int q = 0; q < 4; q++) *(v4i*)(base + ZPL_A_BYTES + (tid + q * ZPL_THREADS) * 16) = rx[q];
+    };
+    const CsdMod md = mt.m[l];
+    int64_t total[4] = {0, 0, 0, 0};
+    for (int64_t kc = 0; kc < a.kpad; kc += a.chunk) {
+        const int64_t ke = min(a.kpad, kc + a.chunk);
+        v4i acc[2 * ZP_DIGITS - 1];
+#pragma unroll
+        for (int s = 0; s < 2 * ZP_DIGITS - 1; s++) acc[s] = (v4i){0, 0, 0, 0};
+        v4i ra[2], rx[4];
+        stage_load(ra, rx, kc);
+        stage_store(ra, rx, 0);
+        __syncthreads();
+        int buf = 0;
+        for (int64_t k0 = kc; k0 < ke; k0 += 64) {
+            const bool more = k0 + 64 < ke;
+            if (more) stage_load(ra, rx, k0 + 64);
+            const int8_t* base = zs + buf * (ZPL_A_BYTES + ZPL_X_BYTES);
+            v4i av[ZP_DIGITS], xv[ZP_DIGITS];
+#pragma unroll
+            for (int d = 0; d < ZP_DIGITS; d++) {
+                av[d] = *(const v4i*)(base + ((d * 32 + rb * 16 + x16) * 4 + g) * 16);
+                xv[d] = *(const v4i*)(base + ZPL_A_BYTES + ((d * 64 + cb * 16 + x16) * 4 + g) * 16);
+            }
+#pragma unroll
+            for (int da = 0; da < ZP_DIGITS; da++)
+#pragma unroll
+                for (int dx = 0; dx < ZP_DIGITS; dx++)
+                    acc[da + dx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[da], xv[dx], acc[da + dx], 0, 0, 0);
+            if (more) stage_store(ra, rx, buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            u128 v = 0;
+#pragma unroll
+            for (int s = 0; s < 2 * ZP_DIGITS - 1; s++) v += (u128)(i128)acc[s][r] << (8 * s);
+            int64_t t = total[r] + zp_reduce(md, (i128)v);
+            total[r] = t >= md.N ? t - md.N : t;
+        }
+    }
+    const int64_t v = nt * 64 + cb * 16 + x16;
+    if (v < a.Ly.nvec) {
+        int64_t* out = a.y + l * a.Ly.s_limb + (v / a.Ly.nlo) * a.Ly.s_hi + (v % a.Ly.nlo) * a.Ly.s_lo;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t i = mtile * 32 + rb * 16 + 4 * g + r;
+            if (i < a.m) out[i * a.Ly.s_elem] = total[r];
+        }
+    }
+}
+
+static bool g_zp_lds = true;  // which MFMA kernel form csd_zp_apply_dim uses (csd_zp_set_kernel)
+
+extern "C" int csd_zp_set_kernel(int lds_staged) {
+    g_zp_lds = lds_staged != 0;
+    return 0;
+}
+
 // ---------------------------------------------------------------------------------------------
 // the plain form: 16 x 16 outputs per workgroup, 64-term slices of A and X staged in LDS
 
@@ -451,7 +555,17 @@ extern "C" int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* z, const csd_
         if (a.nblocks > (int64_t)INT32_MAX) throw CsdError("zp apply: too many tiles");
         TimedScope ts(ctx, "zp_mfma",
                       (double)z->r * ZP_DIGITS * (z->mpad * z->kpad + npad * z->kpad) + 8.0 * z->r * nvec * z->m);
-        hipLaunchKernelGGL(k_zp_mfma, dim3((unsigned)a.nblocks), dim3(256), 0, ctx->stream, a, mt);
+        if (g_zp_lds) {
+            const size_t lds = 2 * (size_t)(ZPL_A_BYTES + ZPL_X_BYTES);
+            static bool attr = false;
+            if (!attr) {
+                HIPCHECK(hipFuncSetAttribute((const void*)k_zp_mfma_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                attr = true;
+            }
+            hipLaunchKernelGGL(k_zp_mfma_lds, dim3((unsigned)a.nblocks), dim3(ZPL_THREADS), lds, ctx->stream, a, mt);
+        } else {
+            hipLaunchKernelGGL(k_zp_mfma, dim3((unsigned)a.nblocks), dim3(256), 0, ctx->stream, a, mt);
+        }
         HIPCHECK(hipGetLastError());
     } catch (...) {
         ctx->free_bytes(xd);

@@ -439,10 +439,16 @@ class HipBackend:
 
     def zp_apply_dim(self, mat, dst, src, dim, form="mfma", max_chunk=0):
         """dst = mat . src along dim (2 or 3), exact, result in [0, p): form "mfma" (int8 digits on
-        the matrix cores) or "plain" (64-bit VALU products)."""
+        the matrix cores, the kernel chosen by zp_set_kernel) or "plain" (64-bit VALU products)."""
         check(lib().csd_zp_apply_dim(self.ctx, ctypes.c_void_p(mat.handle), ctypes.byref(dst.view()),
                                      ctypes.byref(src.view()), ctypes.c_int(_DIM[dim]),
                                      ctypes.c_int({"mfma": 0, "plain": 1}[form]), ctypes.c_size_t(max_chunk)))
+
+    @staticmethod
+    def zp_set_kernel(lds_staged):
+        """which int8 MFMA kernel the "mfma" form runs (process-wide): LDS-staged tiles shared by 8
+        waves (True, the default) or operands loaded by 4 waves straight from L2 (False)"""
+        check(lib().csd_zp_set_kernel(ctypes.c_int(1 if lds_staged else 0)))
 
     def hal_create_permutation(self, inner_shape, shifts):
         n = len(inner_shape)

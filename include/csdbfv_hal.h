@@ -165,6 +165,10 @@ int csd_zp_matrix_create(csd_ctx* ctx, const csd_view* a, const csd_modulus* mod
 int csd_zp_matrix_free(csd_zp_matrix* m);
 int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* a, const csd_view* dst, const csd_view* src, int dim, int form,
                      size_t max_chunk);
+/* which int8 MFMA kernel form 0 uses: 1 (default) stages each K-step's digit planes in LDS for 8
+ * waves with one output block each; 0 has 4 waves load their operands from L2 directly, two
+ * output blocks each. Both compute identical results. */
+int csd_zp_set_kernel(int lds_staged);
 
 #ifdef __cplusplus
 }

@@ -58,14 +58,27 @@ def run(gpu, a, x, primes, dim, form, max_chunk=0):
     m = a.shape[2]
     shape = (x.shape[0], x.shape[1], m, x.shape[3]) if dim == 2 else (x.shape[0], x.shape[1], x.shape[2], m)
     dst = gpu.hal_allocate_i64(shape)
-    gpu.zp_apply_dim(mat, dst, to_dev(gpu, x), dim, form=form, max_chunk=max_chunk)
+    apply(gpu, mat, dst, to_dev(gpu, x), dim, form, max_chunk)
     return from_dev(gpu, dst)
+
+
+def apply(gpu, mat, dst, src, dim, form, max_chunk=0):
+    """form "mfma" runs the default (LDS-staged) MFMA kernel, "mfma_l2" the one that loads its
+    operands from L2 directly"""
+    if form != "mfma_l2":
+        gpu.zp_apply_dim(mat, dst, src, dim, form=form, max_chunk=max_chunk)
+        return
+    gpu.zp_set_kernel(False)
+    try:
+        gpu.zp_apply_dim(mat, dst, src, dim, form="mfma", max_chunk=max_chunk)
+    finally:
+        gpu.zp_set_kernel(True)
 
 
 @pytest.mark.parametrize("m,k,nvec,dim,mode", [
     (20, 70, 33, 3, "uniform"), (12, 12, 5, 2, "uniform"), (64, 128, 64, 3, "wide"), (3, 1, 2, 3, "extreme"),
     (33, 200, 65, 2, "extreme"), (16, 64, 16, 3, "max"), (100, 37, 7, 3, "wide")])
-@pytest.mark.parametrize("form", ["mfma", "plain"])
+@pytest.mark.parametrize("form", ["mfma", "mfma_l2", "plain"])
 def test_zp_exact_small(gpu, m, k, nvec, dim, mode, form):
     rng = np.random.default_rng(m * 1000 + k + nvec)
     primes = P57[:2] + SMALL[:1]
@@ -95,7 +108,7 @@ def test_zp_long_k_at_the_bound(gpu):
     x = np.full((1, 1, 64, 3072), (P57[0] - 1) // 2, dtype=np.int64)
     x[0, 0, :, 5::7] = -((P57[0] - 1) // 2)
     want = exact(a, x, primes, 3)
-    for form in ("mfma", "plain"):
+    for form in ("mfma", "mfma_l2", "plain"):
         assert (run(gpu, a, x, primes, 3, form) == want).all()
 
 
@@ -142,7 +155,7 @@ def test_transform_matrix_first_column_is_the_crt_kernel(gpu, rings, dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("form", ["mfma", "plain"])
+@pytest.mark.parametrize("form", ["mfma", "mfma_l2", "plain"])
 def test_dense_transform_equals_ntt_transform(gpu, rings, dim, form):
     """the dense map along one dimension equals that dimension's ntt_mul + intt pair mod p"""
     g, c = rings
@@ -153,7 +166,7 @@ def test_dense_transform_equals_ntt_transform(gpu, rings, dim, form):
     mods = [gpu.hal_create_modulus(p) for p in primes]
     zm = gpu.zp_matrix(mat, mods)
     dst = gpu.hal_allocate_i64(x.shape)
-    gpu.zp_apply_dim(zm, dst, to_dev(gpu, x), dim, form=form)
+    apply(gpu, zm, dst, to_dev(gpu, x), dim, form)
     got = from_dev(gpu, dst)
     # the oracle's own sequence for one dimension
     cache = c.transform_cache(BASE)
@@ -213,5 +226,5 @@ def test_homcrt_block_contraction(gpu):
     a = first[:, :, idx]
     x = entries(rng, primes, (1, 3, 512, 64), "uniform")
     want = exact(a, x, primes, 3)
-    for form in ("mfma", "plain"):
+    for form in ("mfma", "mfma_l2", "plain"):
         assert (run(gpu, a, x, primes, 3, form) == want).all()
