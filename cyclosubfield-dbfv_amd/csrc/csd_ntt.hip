@@ -615,10 +615,25 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt(NttArgs a, ModTable mt) {
 // ---------------------------------------------------------------------------------------------
 // register-blocked butterflies (the reference's butterflies, hal/ntt.rs:335-350, 378-407,
 // 450-476, 494-531, with the per-stage decisions read from DevPlan::info)
+//
+// The twiddle-1 fast path (skip) leaves a product out, as the reference does. With LANE set the
+// twiddles differ between the lanes of a wave: the product is then formed for every lane and the
+// skip is a select, so the twiddle's words are loaded together instead of behind a branch on z.
 
+template <bool LANE>
+__device__ __forceinline__ int64_t skip_or_mul(bool skip, const CsdBarrett& t, int64_t x, int64_t n) {
+    if constexpr (LANE) {
+        const int64_t p = csd_barrett(t, x, n);
+        return skip ? x : p;
+    } else {
+        return skip ? x : csd_barrett(t, x, n);
+    }
+}
+
+template <bool LANE = false>
 __device__ __forceinline__ void bf2_fwd(int64_t& x, int64_t& y, CsdBarrett t, uint8_t info, CsdBarrett one, int64_t n) {
     const bool skip = (info & INFO_SKIP) && t.z == 1;
-    const int64_t ty = skip ? y : csd_barrett(t, y, n);
+    const int64_t ty = skip_or_mul<LANE>(skip, t, y, n);
     int64_t o0 = (int64_t)((uint64_t)x + (uint64_t)ty);
     int64_t o1 = (int64_t)((uint64_t)x - (uint64_t)ty);
     if (info & INFO_FUSE) {
@@ -629,6 +644,7 @@ __device__ __forceinline__ void bf2_fwd(int64_t& x, int64_t& y, CsdBarrett t, ui
     y = o1;
 }
 
+template <bool LANE = false>
 __device__ __forceinline__ void bf2_inv(int64_t& x, int64_t& y, CsdBarrett t, uint8_t info, CsdBarrett inv_len, int64_t n) {
     const bool fuse = info & INFO_FUSE;
     const CsdBarrett ym = fuse ? inv_len : t;
@@ -636,16 +652,17 @@ __device__ __forceinline__ void bf2_inv(int64_t& x, int64_t& y, CsdBarrett t, ui
     const int64_t sum = (int64_t)((uint64_t)x + (uint64_t)y);
     const int64_t dif = (int64_t)((uint64_t)x - (uint64_t)y);
     x = fuse ? csd_barrett(ym, sum, n) : sum;
-    y = skip ? dif : csd_barrett(ym, dif, n);
+    y = skip_or_mul<LANE>(skip, ym, dif, n);
 }
 
+template <bool LANE = false>
 __device__ __forceinline__ void bf3_fwd(int64_t& x, int64_t& y, int64_t& z, const CsdBarrett* w6, uint8_t info, CsdBarrett one,
                                         int64_t n) {
     const bool skip = (info & INFO_SKIP) && w6[0].z == 1;
-    const int64_t tb = skip ? y : csd_barrett(w6[0], y, n);
+    const int64_t tb = skip_or_mul<LANE>(skip, w6[0], y, n);
     const int64_t tbz = csd_barrett(w6[1], y, n);
     const int64_t tbzz = csd_barrett(w6[2], y, n);
-    const int64_t tc = skip ? z : csd_barrett(w6[3], z, n);
+    const int64_t tc = skip_or_mul<LANE>(skip, w6[3], z, n);
     const int64_t tcz = csd_barrett(w6[4], z, n);
     const int64_t tczz = csd_barrett(w6[5], z, n);
     int64_t o0 = (int64_t)((uint64_t)x + (uint64_t)tb + (uint64_t)tc);
@@ -661,6 +678,7 @@ __device__ __forceinline__ void bf3_fwd(int64_t& x, int64_t& y, int64_t& z, cons
     z = o2;
 }
 
+template <bool LANE = false>
 __device__ __forceinline__ void bf3_inv(int64_t& x, int64_t& y, int64_t& z, const CsdBarrett* w2, uint8_t info,
                                         CsdBarrett inv_len, CsdBarrett zeta, CsdBarrett zeta2, int64_t n) {
     const bool fuse = info & INFO_FUSE;
@@ -675,8 +693,8 @@ __device__ __forceinline__ void bf3_inv(int64_t& x, int64_t& y, int64_t& z, cons
     const int64_t bb = (int64_t)((uint64_t)x + (uint64_t)y1zz + (uint64_t)y2z);
     const int64_t cc = (int64_t)((uint64_t)x + (uint64_t)y1z + (uint64_t)y2zz);
     x = fuse ? csd_barrett(ym, sum, n) : sum;
-    y = skip ? bb : csd_barrett(ym, bb, n);
-    z = skip ? cc : csd_barrett(zm, cc, n);
+    y = skip_or_mul<LANE>(skip, ym, bb, n);
+    z = skip_or_mul<LANE>(skip, zm, cc, n);
 }
 
 template <int R>
@@ -802,16 +820,21 @@ __device__ __forceinline__ void ct_level(int64_t (&v)[Pow<R>::of(G)], const DevP
 #pragma unroll
         for (int jj = 0; jj < D; jj++) {
             const int m0 = sb * D * R + jj;
+            // blocks starting at stage 0 have wave-uniform twiddles; the others vary per lane. The
+            // select form (LANE) measured faster in the forward passes and slower in the inverse
+            // ones (field_36864, MI355X), so the inverse keeps the branch.
+            constexpr bool LANE = S0 > 0 && !INV;
             if constexpr (R == 2) {
                 if constexpr (!INV)
-                    bf2_fwd(v[m0], v[m0 + D], TW[F.tw_off[s] + blk], info, P->one, n);
+                    bf2_fwd<LANE>(v[m0], v[m0 + D], TW[F.tw_off[s] + blk], info, P->one, n);
                 else
-                    bf2_inv(v[m0], v[m0 + D], TW[F.itw_off[s] + blk], info, P->inv_len, n);
+                    bf2_inv<LANE>(v[m0], v[m0 + D], TW[F.itw_off[s] + blk], info, P->inv_len, n);
             } else {
                 if constexpr (!INV)
-                    bf3_fwd(v[m0], v[m0 + D], v[m0 + 2 * D], TW + F.tw_off[s] + 6 * blk, info, P->one, n);
+                    bf3_fwd<LANE>(v[m0], v[m0 + D], v[m0 + 2 * D], TW + F.tw_off[s] + 6 * blk, info, P->one, n);
                 else
-                    bf3_inv(v[m0], v[m0 + D], v[m0 + 2 * D], TW + F.itw_off[s] + 2 * blk, info, P->inv_len, F.zeta, F.zeta2, n);
+                    bf3_inv<LANE>(v[m0], v[m0 + D], v[m0 + 2 * D], TW + F.itw_off[s] + 2 * blk, info, P->inv_len, F.zeta,
+                                  F.zeta2, n);
             }
         }
     }
@@ -1311,6 +1334,18 @@ static ColKernel pick_col_kernel(const DevPlan& P, bool inv) {
 
 static bool g_use_generic_ntt = false;  // forces the one-stage-per-LDS-pass kernel (tests)
 static bool g_use_plan_rows = false;    // keeps dim-3 transforms on the plan-driven row kernel (tests)
+static int g_row_lines = 0;             // dim-3 lines per workgroup, 0 = automatic (csd_ntt_set_row_shape)
+static int g_row_threads = 0;           // dim-3 threads per workgroup, 0 = automatic
+
+extern "C" int csd_ntt_set_row_shape(int lines, int threads) {
+    if (lines < 0 || lines > 8 || threads < 0 || threads > 512 || threads % 64 != 0) {
+        csd_set_error("csd_ntt_set_row_shape: lines must be 0-8, threads 0 or a multiple of 64 up to 512");
+        return 1;
+    }
+    g_row_lines = lines;
+    g_row_threads = threads;
+    return 0;
+}
 
 extern "C" int csd_ntt_force_generic(int on) {
     if (on < 0 || on > 2) {
@@ -1425,8 +1460,18 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         r.data = a.data;
         for (size_t jj = 0; jj < nplans; jj++) r.plans[jj] = a.plans[jj];
         r.dir = dir;
+        // Lines of 3072 (field_36864): one line per 256-thread workgroup, measured fastest on MI355X
+        // (27.7 KB of LDS, so five workgroups and 20 waves per CU hide the latency of the Barrett
+        // chains; two lines per 512 threads ran 8-14 % slower). Other lengths
+        // keep about 6144 coefficients per workgroup.
+        const bool measured_3072 = L == 3072;
         int rows = 1;
-        while (rows * 2 * L <= 6144 && rows * 2 <= a.d2 && rows < 256) rows *= 2;
+        while (!measured_3072 && rows * 2 * L <= 6144 && rows * 2 <= a.d2 && rows < 256) rows *= 2;
+        if (g_row_lines > 0) {
+            // the requested lines, capped by the batch and by what fits the 160 KiB of LDS
+            const int64_t fit = std::max<int64_t>(1, (160 * 1024) / ((int64_t)lds_line((uint32_t)L) * 8));
+            rows = (int)std::min<int64_t>(std::min<int64_t>(g_row_lines, a.d2), fit);
+        }
         r.rows = rows;
         // block size: the candidate with the best (resident threads per CU) x (fraction of
         // non-idle iterations over this plan's passes); measured on lines of 3072, two lines per
@@ -1455,6 +1500,8 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
                 }
             }
         }
+        if (measured_3072 && g_row_lines == 0) threads = 256;
+        if (g_row_threads > 0) threads = g_row_threads;
         r.magic_L = magic_for((uint32_t)L);
         const int Ms[5] = {2, 4, 8, 3, 9};
         for (int q = 0; q < 5; q++) r.magic_groups[q] = L % Ms[q] == 0 ? magic_for((uint32_t)(L / Ms[q])) : 0u;

@@ -244,6 +244,12 @@ class HipBackend:
         check(lib().csd_ctx_stream(self.ctx, ctypes.byref(s)))
         return s.value
 
+    @staticmethod
+    def ntt_set_row_shape(lines=0, threads=0):
+        """launch shape of dimension-3 transforms (process-wide): lines and threads per workgroup,
+        0 for the automatic choice; every shape computes identical results"""
+        check(lib().csd_ntt_set_row_shape(ctypes.c_int(lines), ctypes.c_int(threads)))
+
     def set_timing(self, on):
         """Records HIP events around the library's timed launches on the context stream."""
         check(lib().csd_ctx_set_timing(self.ctx, ctypes.c_int(1 if on else 0)))

@@ -132,6 +132,11 @@ int csd_hal_ntt_mul_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* 
  * kernel for every plan; 2 keeps the register-blocked kernels but reads every pass list from the
  * plan. All produce identical results; the switch exists to test and time each form. */
 int csd_ntt_force_generic(int on);
+/* Launch shape of dimension-3 transforms of lines that fit LDS: lines per workgroup (1-8, capped
+ * by the batch and by the 160 KiB of LDS) and threads per workgroup (a multiple of 64 up to 512);
+ * 0 keeps the automatic choice. Process-wide; every shape computes identical results. No
+ * counterpart in the reference (a tuning knob). */
+int csd_ntt_set_row_shape(int lines, int threads);
 
 /* ---- base-conversion corrections -------------------------------------------------------------- */
 int csd_hal_bconv_correct_mont64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs,

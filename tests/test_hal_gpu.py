@@ -376,6 +376,43 @@ def test_generic_ntt_kernel_parity(gpu, cpu, q, inner):
         pkg.lib().csd_ntt_force_generic(0)
 
 
+@pytest.mark.parametrize("lines,threads", [(0, 0), (2, 512), (1, 128), (3, 192), (4, 512), (1, 64), (8, 256)])
+def test_row_launch_shapes_parity(gpu, cpu, lines, threads):
+    """csd_ntt_set_row_shape: every dimension-3 launch shape of the compiled field_36864 kernels is
+    bit-exact with the oracle, including a ragged last workgroup (5 lines per limb)"""
+    pkg = load()
+    primes, _ = pkg.ring.sample_primes("field_36864", (960, 980), 76)
+    q = primes[5]
+    inner = [2, 3, 8, 64]
+    n = int(np.prod(inner))
+    mc, mg = cpu.hal_create_modulus(q), gpu.hal_create_modulus(q)
+    pc, pg = cpu.hal_create_ntt_plan(mc, inner), gpu.hal_create_ntt_plan(mg, inner)
+    rng = np.random.default_rng(lines * 1000 + threads)
+    gpu.ntt_set_row_shape(lines, threads)
+    try:
+        data = rand_i64(rng, (2, 1, 5, n), -2 * q, 2 * q)
+        kern = rand_i64(rng, (1, 1, 1, n), -q, q)
+        dc, dg = data.copy(), to_dev(gpu, data)
+        cpu.hal_ntt_mul_mont64_dim(dc, [pc], 3, kern, [mc])
+        gpu.hal_ntt_mul_mont64_dim(dg, [pg], 3, to_dev(gpu, kern), [mg])
+        assert same(from_dev(gpu, dg), dc)
+        cpu.hal_intt_mont64_dim(dc, [pc], 3)
+        gpu.hal_intt_mont64_dim(dg, [pg], 3)
+        assert same(from_dev(gpu, dg), dc)
+        cpu.hal_ntt_mont64_dim(dc, [pc], 3)
+        gpu.hal_ntt_mont64_dim(dg, [pg], 3)
+        assert same(from_dev(gpu, dg), dc)
+    finally:
+        gpu.ntt_set_row_shape(0, 0)
+
+
+def test_row_launch_shape_errors(gpu):
+    pkg = load()
+    for lines, threads in [(-1, 0), (9, 0), (1, 96), (1, 1024), (0, -64)]:
+        with pytest.raises(pkg.HalError):
+            gpu.ntt_set_row_shape(lines, threads)
+
+
 @pytest.mark.parametrize("field,log2_q,inner", [("field_36864", (960, 980), [2, 3, 8, 64]),
                                                 ("field_49152", (1280, 1300), [3, 64, 64])])
 @pytest.mark.parametrize("mode", [0, 2])
