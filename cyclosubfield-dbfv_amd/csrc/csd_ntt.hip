@@ -760,6 +760,13 @@ __device__ __forceinline__ void block_stages(int64_t (&v)[Pow<R>::of(G)], const 
 __device__ __forceinline__ uint32_t lds_slot(uint32_t e) { return e + (e >> 3); }
 __host__ __device__ __forceinline__ uint32_t lds_line(uint32_t L) { return L + (L >> 3) + 1; }
 
+// LDS layout of the compiled shapes: unpadded lines. A 3072-line then takes 24 KiB instead of
+// 27.7 KiB, so six workgroups fit a CU instead of five; on MI355X that measured 3-4 % faster than
+// the padded layout above (and than an XOR swizzle of the same size), the extra bank conflicts
+// costing less than the added occupancy gains.
+__host__ __device__ __forceinline__ constexpr uint32_t ct_slot(uint32_t e) { return e; }
+__host__ __device__ __forceinline__ constexpr uint32_t ct_line(uint32_t L) { return L; }
+
 // Runs one pass over all groups of `rows` LDS-resident lines of length L.
 template <int R, int G, bool INV>
 __device__ __forceinline__ void run_pass(int64_t* __restrict__ lds, int L, int rows, uint32_t magic_groups,
@@ -859,7 +866,7 @@ template <int L, int FI, int R, int G, int S, int SL, int S0, bool INV>
 __device__ __forceinline__ void ct_pass(int64_t* __restrict__ lds, int rows, const DevPlan* __restrict__ P,
                                         const CsdBarrett* __restrict__ TW, int dir) {
     constexpr int M = Pow<R>::of(G);
-    constexpr uint32_t sub = SL / M, NG = L / M, stride = sub * S, lline = L + L / 8 + 1;
+    constexpr uint32_t sub = SL / M, NG = L / M, stride = sub * S, lline = ct_line(L);
     const DevFactor& F = P->f[FI];
     const uint32_t total = NG * (uint32_t)rows;
     for (uint32_t w = threadIdx.x; w < total; w += blockDim.x) {
@@ -870,10 +877,10 @@ __device__ __forceinline__ void ct_pass(int64_t* __restrict__ lds, int rows, con
         int64_t* line = lds + row * lline;
         int64_t v[M];
 #pragma unroll
-        for (int m = 0; m < M; m++) v[m] = line[lds_slot(base + m * stride)];
+        for (int m = 0; m < M; m++) v[m] = line[ct_slot(base + m * stride)];
         ct_stages<R, G, S0, INV>(v, P, TW, F, dir, blk0);
 #pragma unroll
-        for (int m = 0; m < M; m++) line[lds_slot(base + m * stride)] = v[m];
+        for (int m = 0; m < M; m++) line[ct_slot(base + m * stride)] = v[m];
     }
     __syncthreads();
 }
@@ -885,7 +892,7 @@ __device__ __forceinline__ void ct_pass_store(const int64_t* __restrict__ lds, i
                                               const CsdBarrett* __restrict__ TW, int dir, int64_t* __restrict__ out, int64_t st2,
                                               const int64_t* __restrict__ mul, int64_t ms2, int64_t ms3, const CsdMod& md) {
     constexpr int M = Pow<R>::of(G);
-    constexpr uint32_t sub = SL / M, NG = L / M, stride = sub * S, lline = L + L / 8 + 1;
+    constexpr uint32_t sub = SL / M, NG = L / M, stride = sub * S, lline = ct_line(L);
     const DevFactor& F = P->f[FI];
     const uint32_t total = NG * (uint32_t)rows;
     for (uint32_t w = threadIdx.x; w < total; w += blockDim.x) {
@@ -896,7 +903,7 @@ __device__ __forceinline__ void ct_pass_store(const int64_t* __restrict__ lds, i
         const int64_t* line = lds + row * lline;
         int64_t v[M];
 #pragma unroll
-        for (int m = 0; m < M; m++) v[m] = line[lds_slot(base + m * stride)];
+        for (int m = 0; m < M; m++) v[m] = line[ct_slot(base + m * stride)];
         ct_stages<R, G, S0, INV>(v, P, TW, F, dir, blk0);
 #pragma unroll
         for (int m = 0; m < M; m++) {
@@ -982,7 +989,7 @@ template <int SHAPE, bool INV>
 __device__ __forceinline__ void ct_load_head(int64_t* __restrict__ lds, const int64_t* __restrict__ base, int64_t st2, int rows,
                                              const DevPlan* __restrict__ P, const CsdBarrett* __restrict__ TW, int dir) {
     if constexpr (SHAPE == 1) {
-        constexpr uint32_t L = 3072, E = 512, lline = L + L / 8 + 1;
+        constexpr uint32_t L = 3072, E = 512, lline = ct_line(L);
         const DevFactor& F0 = P->f[0];
         const DevFactor& F1 = P->f[1];
         for (uint32_t w = threadIdx.x; w < E * (uint32_t)rows; w += blockDim.x) {
@@ -1006,10 +1013,10 @@ __device__ __forceinline__ void ct_load_head(int64_t* __restrict__ lds, const in
                 v[3 * h + 2] = g[2];
             }
 #pragma unroll
-            for (int m = 0; m < 6; m++) lds[r * lline + lds_slot(e + E * m)] = v[m];
+            for (int m = 0; m < 6; m++) lds[r * lline + ct_slot(e + E * m)] = v[m];
         }
     } else {
-        constexpr uint32_t L = 12288, E = 4096, lline = L + L / 8 + 1;
+        constexpr uint32_t L = 12288, E = 4096, lline = ct_line(L);
         const DevFactor& F0 = P->f[0];
         for (uint32_t w = threadIdx.x; w < E * (uint32_t)rows; w += blockDim.x) {
             const uint32_t r = w / E, e = w % E;
@@ -1018,7 +1025,7 @@ __device__ __forceinline__ void ct_load_head(int64_t* __restrict__ lds, const in
             for (int m = 0; m < 3; m++) g[m] = base[r * st2 + e + E * m];
             ct_stages<3, 1, 0, INV>(g, P, TW, F0, dir, 0u);
 #pragma unroll
-            for (int m = 0; m < 3; m++) lds[r * lline + lds_slot(e + E * m)] = g[m];
+            for (int m = 0; m < 3; m++) lds[r * lline + ct_slot(e + E * m)] = g[m];
         }
     }
     __syncthreads();
@@ -1118,7 +1125,7 @@ __global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
         L = P->len;
     else
         L = CtShape<SHAPE>::L;
-    const uint32_t lline = lds_line((uint32_t)L);
+    const uint32_t lline = SHAPE ? ct_line((uint32_t)L) : lds_line((uint32_t)L);
     const int rows = (int)min((int64_t)a.rows, a.d2 - k0);
     const int total = rows * L;
     const int dir = a.dir;
@@ -1177,7 +1184,7 @@ __global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
     for (int g = threadIdx.x; g < total; g += blockDim.x) {
         const uint32_t r = a.rows == 1 ? 0u : (SHAPE ? (uint32_t)g / (uint32_t)L : fdiv((uint32_t)g, a.magic_L));
         const int p = g - (int)r * L;
-        int64_t x = lds[r * lline + lds_slot(p)];
+        int64_t x = lds[r * lline + (SHAPE ? ct_slot((uint32_t)p) : lds_slot((uint32_t)p))];
         if (a.mul)
             x = csd_mont_mul(m, x, a.mul[i * a.smul.s[0] + j * a.smul.s[1] + (k0 + r) * a.smul.s[2] + p * a.smul.s[3]]);
         base[r * a.st[2] + p] = x;
@@ -1361,7 +1368,7 @@ typedef void (*RowKernel)(RowArgs, ModTable);
 
 // the dim-3 row kernel for these plans: a compiled shape when every plan matches one, else the
 // plan-driven form; the dynamic LDS limit is raised once per kernel
-static RowKernel pick_row_kernel(csd_ntt_plan* const* plans, size_t nplans, int dir) {
+static RowKernel pick_row_kernel(csd_ntt_plan* const* plans, size_t nplans, int dir, int* shape_out = nullptr) {
     static bool attr = false;
     static const RowKernel ks[2][3] = {{k_ntt_rows<false, 0>, k_ntt_rows<false, 1>, k_ntt_rows<false, 2>},
                                        {k_ntt_rows<true, 0>, k_ntt_rows<true, 1>, k_ntt_rows<true, 2>}};
@@ -1382,8 +1389,12 @@ static RowKernel pick_row_kernel(csd_ntt_plan* const* plans, size_t nplans, int 
             }
         }
     }
+    if (shape_out) *shape_out = shape;
     return ks[dir == 2 ? 1 : 0][shape];
 }
+
+// LDS words per line of the row kernel of the given compiled shape (0: the plan-driven kernel)
+static uint32_t row_lds_line(int shape, uint32_t L) { return shape ? ct_line(L) : lds_line(L); }
 
 static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* plans, size_t nplans, int dim, int dir,
                        const csd_view* mul, const csd_modulus* mods, size_t nmods) {
@@ -1461,15 +1472,18 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         for (size_t jj = 0; jj < nplans; jj++) r.plans[jj] = a.plans[jj];
         r.dir = dir;
         // Lines of 3072 (field_36864): one line per 256-thread workgroup, measured fastest on MI355X
-        // (27.7 KB of LDS, so five workgroups and 20 waves per CU hide the latency of the Barrett
-        // chains; two lines per 512 threads ran 8-14 % slower). Other lengths
-        // keep about 6144 coefficients per workgroup.
+        // (24 KiB of LDS, so six workgroups and 24 waves per CU hide the latency of the Barrett
+        // chains; with the padded layout, two lines per 512 threads ran 8-14 % slower). Other
+        // lengths keep about 6144 coefficients per workgroup.
         const bool measured_3072 = L == 3072;
+        int shape = 0;
+        RowKernel rk = pick_row_kernel(plans, nplans, dir, &shape);
+        const uint32_t lline = row_lds_line(shape, (uint32_t)L);
         int rows = 1;
         while (!measured_3072 && rows * 2 * L <= 6144 && rows * 2 <= a.d2 && rows < 256) rows *= 2;
         if (g_row_lines > 0) {
             // the requested lines, capped by the batch and by what fits the 160 KiB of LDS
-            const int64_t fit = std::max<int64_t>(1, (160 * 1024) / ((int64_t)lds_line((uint32_t)L) * 8));
+            const int64_t fit = std::max<int64_t>(1, (160 * 1024) / ((int64_t)lline * 8));
             rows = (int)std::min<int64_t>(std::min<int64_t>(g_row_lines, a.d2), fit);
         }
         r.rows = rows;
@@ -1479,7 +1493,7 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         int threads = 256;
         {
             const DevPlan& P0 = plans[0]->host;
-            const size_t lds_block = (size_t)rows * lds_line((uint32_t)L) * 8;
+            const size_t lds_block = (size_t)rows * lline * 8;
             double best = -1;
             for (int T : {256, 512}) {
                 double busy = 0, slots = 0;
@@ -1510,9 +1524,8 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         r.smul = a.smul;
         int64_t blocks = a.d0 * a.d1 * r.chunks;
         if (blocks > (int64_t)INT32_MAX) throw CsdError("ntt: too many tiles");
-        size_t lds = (size_t)rows * lds_line((uint32_t)L) * 8;
+        size_t lds = (size_t)rows * lline * 8;
         if (lds > 160 * 1024) throw CsdError("ntt: line too long for LDS staging");
-        RowKernel rk = pick_row_kernel(plans, nplans, dir);
         TimedScope ts(ctx, names[1][dir], 16.0 * (double)numel(dst));
         hipLaunchKernelGGL(rk, dim3((unsigned)blocks), dim3(threads), lds, ctx->stream, r, t);
         HIPCHECK(hipGetLastError());
@@ -1620,12 +1633,13 @@ void csd_ntt_segments(csd_ctx* ctx, int64_t* const* seg_base, const int32_t* seg
     r.smul = smul;
     r.seg_base = seg_base;
     r.seg_limb = seg_limb;
-    size_t lds = (size_t)rows * lds_line((uint32_t)L) * 8;
+    int shape = 0;
+    RowKernel rk = pick_row_kernel(plans, nplans, dir, &shape);
+    size_t lds = (size_t)rows * row_lds_line(shape, (uint32_t)L) * 8;
     if (lds > 160 * 1024) throw CsdError("work-list ntt: line too long for LDS staging");
     const int threads = tune.threads > 0 ? tune.threads : (L * rows >= 8192 ? 512 : 256);
     if (threads % 64 || threads > 512) throw CsdError("work-list ntt: row block size must be a multiple of 64, at most 512");
     if (nslices * r.chunks > (int64_t)INT32_MAX) throw CsdError("work-list ntt: too many tiles");
-    RowKernel rk = pick_row_kernel(plans, nplans, dir);
     TimedScope ts(ctx, timer, bytes);
     hipLaunchKernelGGL(rk, dim3((unsigned)(nslices * r.chunks)), dim3(threads), lds, ctx->stream, r, t);
     HIPCHECK(hipGetLastError());
@@ -1696,20 +1710,20 @@ __device__ __forceinline__ void chain_rows(int64_t* __restrict__ lds, int64_t* _
                                            const DevPlan* __restrict__ P, int dir, const int64_t* __restrict__ mul_row,
                                            const int64_t* __restrict__ mul_mat, const CsdMod& m) {
     constexpr int L = CtShape<SHAPE>::L;
-    constexpr uint32_t lline = L + L / 8 + 1;
+    constexpr uint32_t lline = ct_line(L);
     const CsdBarrett* __restrict__ TW = (const CsdBarrett*)(P + 1);
     for (int k0 = 0; k0 < n2; k0 += rows_per) {
         const int rows = min(rows_per, n2 - k0);
         const int total = rows * L;
         for (int g = threadIdx.x; g < total; g += blockDim.x) {
             const int r = g / L, p = g - r * L;
-            lds[r * lline + lds_slot(p)] = base[(k0 + r) * L + p];
+            lds[r * lline + ct_slot(p)] = base[(k0 + r) * L + p];
         }
         __syncthreads();
         ct_passes<SHAPE, INV>(lds, rows, P, TW, dir);
         for (int g = threadIdx.x; g < total; g += blockDim.x) {
             const int r = g / L, p = g - r * L;
-            int64_t x = lds[r * lline + lds_slot(p)];
+            int64_t x = lds[r * lline + ct_slot(p)];
             if (mul_row) x = csd_mont_mul(m, x, mul_row[p]);
             if (mul_mat) x = csd_mont_mul(m, x, mul_mat[(k0 + r) * L + p]);
             base[(k0 + r) * L + p] = x;
@@ -1766,7 +1780,7 @@ int csd_batch_chain_occupancy(int rows) {
         attr = true;
     }
     int n = 0;
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_chain<4, 3, 1>, 256, (size_t)rows * lds_line(3072) * 8));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_chain<4, 3, 1>, 256, (size_t)rows * ct_line(3072) * 8));
     return std::max(n, 1);
 }
 
@@ -1793,7 +1807,7 @@ void csd_batch_chain_launch(csd_ctx* ctx, int64_t* const* seg_base, const int32_
     a.pt = pt;
     a.rows = rows;
     ModTable t = make_mod_table(mods, R, (int64_t)R);
-    const size_t lds = (size_t)rows * lds_line(3072) * 8;
+    const size_t lds = (size_t)rows * ct_line(3072) * 8;
     static bool attr = false;
     if (!attr) {
         HIPCHECK(hipFuncSetAttribute((const void*)k_batch_chain<4, 3, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
