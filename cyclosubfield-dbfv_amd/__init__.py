@@ -6,3 +6,4 @@ include/csdbfv_hal.h. The package directory name is not a Python identifier; loa
 from .hal import I128, DeviceTensor, HalError, HipBackend, lib  # noqa: F401
 from . import ring  # noqa: E402,F401
 from . import bfv  # noqa: E402,F401
+from . import dbfv  # noqa: E402,F401

@@ -1,0 +1,209 @@
+"""The dBFV digit reduction of cyclosubfield-dbfv/src/dbfv.rs as HAL call sequences on the
+MI355X: ``split_overflow`` (dbfv.rs:873-948) and the sequential carry chain ``dbfv_reduce``
+(dbfv.rs:950-1025). The constants of ``SplitOverflowData::new`` (dbfv.rs:785-841) are computed
+on the host with exact integers, as the reference computes them, and uploaded once; every pass
+over the ciphertexts runs in the backend (``hal.HipBackend``) through the ring (``ring.py``):
+one Montgomery multiply, one i128 matmul, one power-of-two base-conversion correction and one
+wide accumulation per digit.
+
+A dBFV ciphertext is a period-basis batch whose rows 2i, 2i + 1 hold the BFV ciphertext of
+digit i (dbfv.rs:32-38). ``dbfv_reduce_batch`` runs many ciphertexts through the same chain with
+every HAL call covering all of them; its rows are digit-major, so that each digit of all
+ciphertexts is one contiguous run of rows."""
+
+
+def _log2_ceil(x):
+    """feanor-math abs_log2_ceil for x >= 1"""
+    return (abs(x) - 1).bit_length()
+
+
+class DBFVModulus:
+    """DBFVModulus (dbfv.rs:56-80): digit base b = 2^log2_b, total precision t = 2^log2_t"""
+
+    def __init__(self, log2_b, log2_t):
+        if not (log2_b >= 1 and 1 <= log2_t <= 126):
+            raise ValueError("t = 2^log2_t must be between 2 and 2^126")
+        self.log2_b = log2_b
+        self.log2_t = log2_t
+
+    @classmethod
+    def new(cls, log2_b, digits):
+        return cls(log2_b, digits * log2_b)
+
+    @classmethod
+    def new_incomplete(cls, log2_b, log2_t):
+        return cls(log2_b, log2_t)
+
+    def t(self):
+        return 1 << self.log2_t
+
+    def b(self):
+        return 1 << self.log2_b
+
+    def digits(self):
+        return (self.log2_t - 1) // self.log2_b + 1
+
+
+def split_i128_limbs(x):
+    """split_i128_limbs (hal/mod.rs:66-76): x = hi 2^64 + lo, lo the low 64 bits of x as an i64"""
+    if abs(x) >= 1 << 126:
+        raise ValueError("split_i128_limbs needs |x| < 2^126")
+    lo = x & ((1 << 64) - 1)
+    lo = lo - (1 << 64) if lo >> 63 else lo
+    return lo, (x - lo) >> 64
+
+
+def rounded_div(a, b):
+    """round(a / b) for b > 0, halves away from zero (feanor-math rounded_div)"""
+    q, r = divmod(abs(a), b)
+    if 2 * r >= b:
+        q += 1
+    return q if a >= 0 else -q
+
+
+class SplitOverflowData:
+    """SplitOverflowData::new (dbfv.rs:785-841): the BEHZ-style constants that split a
+    coefficient into its remainder modulo b Delta and the overflow modulo R = t / b."""
+
+    def __init__(self, ring, modulus, base):
+        self.log2_b = modulus.log2_b
+        self.log2_t = modulus.log2_t
+        self.rns_base = list(base)
+        self.log2_r = modulus.log2_t - modulus.log2_b
+        if not 1 <= self.log2_r <= 126:
+            raise ValueError("t/b must be between 2 and 2^126")
+        limbs = self.limbs()
+        q = ring.modulus_of(base)
+        r_big = 1 << self.log2_r
+        primes = [ring.primes[i] for i in base]
+        max_p, min_p = max(primes), min(primes)
+        self.log2_gamma = max(64, _log2_ceil(4 * len(base) * max_p + 1))
+        gamma = 1 << self.log2_gamma
+        # neither inner product of the matmul may overflow an i128 (dbfv.rs:799-806)
+        if ((gamma // min_p) + 1) * max_p * len(base) >= 1 << 127:
+            raise ValueError("split_overflow: base too large for the i128 correction row")
+        q_over_p_mod_r = [(q // p) % r_big for p in primes]
+        values = [split_i128_limbs(c)[0] for c in q_over_p_mod_r]
+        if limbs == 2:
+            values += [split_i128_limbs(c)[1] for c in q_over_p_mod_r]
+        values += [rounded_div(gamma, p) for p in primes]
+        be = ring.backend
+        self.matrix = be.hal_allocate_i64([1, limbs + 1, len(base), 1])
+        be.hal_import(self.matrix, values)
+        self.q_mod_r = q % r_big
+        # the inverse modulo 2^128 (dbfv.rs:763-770), negated and reduced modulo R
+        self.neg_q_inv_mod_r = (-pow(self.q_mod_r, -1, 1 << 128)) % r_big
+        delta = rounded_div(q, modulus.t())
+        neg_b_delta = -(delta * modulus.b())
+        self.r_q_over_p_inv = ring.q_over_p_inv_tensor(base, r_big)
+        self.neg_b_delta_mont2 = [ring.scalar_tensor_big_mont2(base, neg_b_delta << (64 * k)) for k in range(limbs)]
+
+    def limbs(self):
+        """one i64 limb of overflow for R <= 2^62, else two (bconv_correct_pow2_wide)"""
+        return 1 if self.log2_r <= 62 else 2
+
+
+def split_overflow_data(ring, modulus, base):
+    """split_overflow_data (dbfv.rs:846-871): cached per (modulus, base) on the ring"""
+    key = ("split_overflow", modulus.log2_b, modulus.log2_t, tuple(base))
+    hit = ring._consts.get(key)
+    if hit is None:
+        hit = ring._consts[key] = SplitOverflowData(ring, modulus, base)
+    return hit
+
+
+def _assert_mul_mont_safe(ring, bound, base):
+    """assert_mul_mont_safe (ring.rs:955-960)"""
+    if bound * max(ring.primes[i] for i in base) > 1 << 63:
+        raise ValueError("representatives too large for a montgomery multiplication")
+
+
+def split_overflow(ring, data, x, x_bound, base):
+    """split_overflow (dbfv.rs:893-948): the period-basis rows `x` ([count, r, ., .]) become
+    x_lo in place, with x = x_lo + (b Delta) x_hi mod Q and |x_lo| <= 2 b Delta; returns x_hi as a
+    [count, limbs, ., .] tensor of plain values in (-R/2, R/2]."""
+    be = ring.backend
+    shape = x.shape
+    if len(base) != shape[1]:
+        raise ValueError("split_overflow: the tensor's limbs must match the base")
+    _assert_mul_mont_safe(ring, x_bound, base)
+    limbs = data.limbs()
+    mods = ring.mods_for(base)
+    # plain lifts of (x R) (Q/p)^-1 mod p, in (-p, p)
+    lifts = be.hal_allocate_i64(shape)
+    be.hal_assign(lifts, x)
+    be.hal_mul_mont64(lifts, data.r_q_over_p_inv, mods)
+    # per coefficient: sum_p lift_p (Q/p) mod R in limbs, and the gamma-scaled multiple-of-Q estimate
+    sums = be.hal_allocate_i128([shape[0], limbs + 1, shape[2], shape[3]])
+    be.hal_matmul_i64(sums, data.matrix, lifts)
+    overflow = be.hal_allocate_i64([shape[0], limbs, shape[2], shape[3]])
+    if limbs == 1:
+        be.hal_bconv_correct_pow2(overflow, sums, data.q_mod_r, data.neg_q_inv_mod_r, data.log2_gamma, data.log2_r)
+    else:
+        be.hal_bconv_correct_pow2_wide(overflow, sums, data.q_mod_r, data.neg_q_inv_mod_r, data.log2_gamma, data.log2_r)
+    # x_lo = x + overflow (-b Delta), the limbs folded into mont^2(-b Delta 2^(64 k))
+    acc = ring.allocate_wide(shape[0], base)
+    rows = range(shape[0])
+    for k in range(limbs):
+        acc.fma_unbounded(rows, overflow[:, k:k + 1], data.neg_b_delta_mont2[k])
+    acc.add(rows, x, x_bound)
+    ring.reduce_wide_into(x, acc)
+    return overflow
+
+
+def _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base):
+    """the carry chain of dbfv_reduce (dbfv.rs:990-1023) over `result`, whose digit i occupies
+    rows [i per_digit, (i + 1) per_digit) and whose first d digits came from the input"""
+    data = split_overflow_data(ring, modulus, base)
+    delta = rounded_div(ring.modulus_of(base), modulus.t())
+    delta_mont2 = [ring.scalar_tensor_big_mont2(base, delta << (64 * k)) for k in range(data.limbs())]
+    carry = None
+    rows = range(per_digit)
+    for i in range(modulus.digits()):
+        # rows copied from the (possibly unreduced) input have its bound; rows beyond it are zero
+        input_row_bound = ct_bound if i < d else 1
+        digit = result[i * per_digit:(i + 1) * per_digit]
+        if carry is not None:
+            # digit_i += Delta carry; the carry limbs are plain integers, hence the unbounded fma
+            acc = ring.allocate_wide(per_digit, base)
+            for k, factor in enumerate(delta_mont2):
+                acc.fma_unbounded(rows, carry[:, k:k + 1], factor)
+            acc.add(rows, digit, input_row_bound)
+            ring.reduce_wide_into(digit, acc)
+        row_bound = input_row_bound if carry is None else 1
+        carry = split_overflow(ring, data, digit, row_bound, base)
+
+
+def dbfv_reduce(ring, modulus, ct, ct_bound, base):
+    """dbfv_reduce (dbfv.rs:965-1025) of one period-basis dBFV ciphertext (rows 2i, 2i + 1 =
+    digit i): every digit is reduced modulo b Delta and its carry added to the next digit, the
+    carries of digits >= modulus.digits() dropped. Returns (result, bound)."""
+    if ct.shape[0] % 2:
+        raise ValueError("a dBFV ciphertext has two rows per digit")
+    d = ct.shape[0] // 2
+    if d > modulus.digits():
+        raise ValueError("ciphertext has more digits than the modulus")
+    if modulus.digits() == 1 or d == 0:
+        return ct, ct_bound
+    result = ring.allocate(2 * modulus.digits(), base)
+    ring.backend.hal_assign(result[0:2 * d], ct)
+    _reduce_chain(ring, modulus, result, d, 2, ct_bound, base)
+    return result, 1
+
+
+def dbfv_reduce_batch(ring, modulus, cts, count, ct_bound, base):
+    """dbfv_reduce for `count` ciphertexts of d digits at once. `cts` holds them digit-major:
+    row (2 i + s) count + c is row s of digit i of ciphertext c; the result ([2 digits count,
+    r, ., .]) has the same layout. Every HAL call covers all ciphertexts; each ciphertext sees
+    the calls and operand bounds of dbfv_reduce, so its residues are identical."""
+    if cts.shape[0] % (2 * count):
+        raise ValueError("dbfv_reduce_batch: rows must be 2 x digits x count")
+    d = cts.shape[0] // (2 * count)
+    if d > modulus.digits():
+        raise ValueError("ciphertexts have more digits than the modulus")
+    if modulus.digits() == 1 or d == 0:
+        return cts, ct_bound
+    result = ring.allocate(2 * modulus.digits() * count, base)
+    ring.backend.hal_assign(result[0:2 * d * count], cts)
+    _reduce_chain(ring, modulus, result, d, 2 * count, ct_bound, base)
+    return result, 1

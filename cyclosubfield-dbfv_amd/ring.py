@@ -212,6 +212,48 @@ class RingContext:
         self._consts[key] = t
         return t
 
+    def modulus_of(self, base):
+        """modulus_of (ring.rs): the product of the primes of `base`"""
+        q = 1
+        for i in base:
+            q *= self.primes[i]
+        return q
+
+    def _host_constants(self, key, base, values):
+        """a [1, r, 1, 1] tensor of host-computed per-prime constants, uploaded once per key"""
+        hit = self._consts.get(key)
+        if hit is None:
+            hit = self.backend.hal_allocate_i64([1, len(base), 1, 1])
+            self.backend.hal_import(hit, np.array(values, dtype=np.int64).reshape(1, len(base), 1, 1))
+            self._consts[key] = hit
+        return hit
+
+    def q_over_p_inv_tensor(self, base, scale):
+        """q_over_p_inv_tensor (ring.rs:1023-1043): the plain smallest lifts of
+        scale (Q/p)^-1 mod p per prime p of `base`"""
+        q = self.modulus_of(base)
+        vals = []
+        for i in base:
+            p = self.primes[i]
+            v = (pow((q // p) % p, -1, p) * (scale % p)) % p
+            vals.append(v - p if v > p // 2 else v)
+        return self._host_constants(("q_over_p_inv", tuple(base), scale), base, vals)
+
+    def scalar_tensor_big_mont2(self, base, scalar):
+        """scalar_tensor_big_mont2 (ring.rs:904-919): mont(mont(smallest_lift(scalar mod p)))
+        per prime, mont(x) = (x 2^64) rem p with the sign of x (Rust's remainder)"""
+        def mont(x, p):
+            r = abs(x << 64) % p
+            return -r if x < 0 else r
+
+        vals = []
+        for i in base:
+            p = self.primes[i]
+            v = scalar % p
+            v = v - p if v > p // 2 else v
+            vals.append(mont(mont(v, p), p))
+        return self._host_constants(("big_mont2", tuple(base), scalar), base, vals)
+
     def mods_for(self, base):
         key = ("mods", tuple(base))
         hit = self._consts.get(key)
@@ -225,15 +267,19 @@ class RingContext:
     def reduce_wide(self, wide):
         """reduce_wide (ring.rs:846-865): one reduction kind per run of rows."""
         out = self.allocate(wide.count(), wide.rns_base)
+        self.reduce_wide_into(out, wide)
+        return out
+
+    def reduce_wide_into(self, dst, wide):
+        """reduce_wide_into (ring.rs:829-838): the reduced rows written into `dst`"""
         start = 0
         while start < wide.count():
             full = wide.needs_full_reduction(start)
             end = start + 1
             while end < wide.count() and wide.needs_full_reduction(end) == full:
                 end += 1
-            self.reduce_wide_tensor(out[start:end], wide.data[start:end], wide.rns_base, full)
+            self.reduce_wide_tensor(dst[start:end], wide.data[start:end], wide.rns_base, full)
             start = end
-        return out
 
 
 class WideRingEls:
@@ -303,6 +349,11 @@ class WideRingEls:
             raise ValueError("inner_product overwrites: the rows must be fresh")
         self.ring.backend.hal_inner_product_i64(self.data[rows[0]:rows[-1] + 1], [t[0] for t in terms], [t[2] for t in terms])
         self.record_step(rows, sum(lb * rb * self.max_p * self.max_p for (_, lb, _, rb) in terms))
+
+    def fma_unbounded(self, rows, lhs, rhs):
+        """fma_unbounded (ring.rs:493-500): lhs arbitrary i64 (plain values), rhs bounded by max_p"""
+        self._accumulate(rows, lhs, rhs)
+        self.record_step(rows, self.max_p << 63)
 
     def add(self, rows, src, src_bound):
         self._accumulate(rows, src, self.one())

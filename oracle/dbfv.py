@@ -1,0 +1,172 @@
+"""ORACLE — test infrastructure only (imported by tests/ and by bench.py's cpu_baseline leg).
+
+A restatement of the dBFV digit reduction of cyclosubfield-dbfv/src/dbfv.rs over the oracle ring
+(``oracle.ring.RingContext`` on the ``CpuBackend``). It follows:
+
+* dbfv.rs:56-80    — ``DBFVModulus`` (b = 2^log2_b, t = 2^log2_t, digits);
+* hal/mod.rs:66-76 — ``split_i128_limbs``;
+* dbfv.rs:760-841  — ``SplitOverflowData::new`` (the BEHZ-style constants of the overflow split);
+* dbfv.rs:873-948  — ``split_overflow``: x -> (x_lo, x_hi) with x = x_lo + (b Delta) x_hi mod Q;
+* dbfv.rs:950-1025 — ``dbfv_reduce``: the sequential carry chain over the digits.
+
+Pins: ``split_overflow`` is checked against the reference's own property test
+(dbfv.rs:1155-1208: x = x_lo + b Delta x_hi mod Q, |x_lo| < 2 b Delta) at its three moduli,
+including two-limb R; ``dbfv_reduce`` is checked against the carry-chain invariant it must keep
+(every digit's carry recovered from the input and output is a small integer in (-R/2, R/2], and
+every output digit is below 2 b Delta), which a dropped, unscaled or misplaced carry breaks
+(tests/test_oracle_dbfv.py). The reference checks dbfv_reduce only through decryption
+(dbfv.rs:1324-1362), which needs key generation and encryption that this oracle does not restate.
+
+Big-integer constants use Python ints, as the reference's feanor-math ZZbig does.
+"""
+from .ring import _log2_ceil
+
+
+class DBFVModulus:
+    """dbfv.rs:56-80"""
+
+    def __init__(self, log2_b, log2_t):
+        if not (log2_b >= 1 and 1 <= log2_t <= 126):
+            raise ValueError("t = 2^log2_t must be between 2 and 2^126")
+        self.log2_b = log2_b
+        self.log2_t = log2_t
+
+    @classmethod
+    def new(cls, log2_b, digits):
+        return cls(log2_b, digits * log2_b)
+
+    @classmethod
+    def new_incomplete(cls, log2_b, log2_t):
+        return cls(log2_b, log2_t)
+
+    def t(self):
+        return 1 << self.log2_t
+
+    def b(self):
+        return 1 << self.log2_b
+
+    def digits(self):
+        return (self.log2_t - 1) // self.log2_b + 1
+
+
+def _as_i64(x):
+    x &= (1 << 64) - 1
+    return x - (1 << 64) if x >> 63 else x
+
+
+def split_i128_limbs(x):
+    """hal/mod.rs:66-76: x = hi 2^64 + lo with lo the low 64 bits of x as an i64"""
+    assert abs(x) < 1 << 126
+    lo = _as_i64(x)
+    hi = (x - lo) >> 64
+    return lo, hi
+
+
+def rounded_div(a, b):
+    """feanor-math rounded_div for b > 0: nearest integer, halves away from zero"""
+    q, r = divmod(abs(a), b)
+    if 2 * r >= b:
+        q += 1
+    return q if a >= 0 else -q
+
+
+class SplitOverflowData:
+    """SplitOverflowData::new (dbfv.rs:785-841)"""
+
+    def __init__(self, ring, modulus, base):
+        self.log2_b = modulus.log2_b
+        self.log2_t = modulus.log2_t
+        self.rns_base = list(base)
+        self.log2_r = modulus.log2_t - modulus.log2_b
+        if not 1 <= self.log2_r <= 126:
+            raise ValueError("t/b must be between 2 and 2^126")
+        limbs = self.limbs()
+        q = ring.modulus_of(base)
+        r_big = 1 << self.log2_r
+        primes = [ring.primes[i] for i in base]
+        max_p, min_p = max(primes), min(primes)
+        self.log2_gamma = max(64, _log2_ceil(4 * len(base) * max_p + 1))
+        gamma = 1 << self.log2_gamma
+        assert ((gamma // min_p) + 1) * max_p * len(base) < 1 << 127
+        q_over_p_mod_r = [(q // p) % r_big for p in primes]
+        gamma_over_p = [rounded_div(gamma, p) for p in primes]
+        values = [split_i128_limbs(c)[0] for c in q_over_p_mod_r]
+        if limbs == 2:
+            values += [split_i128_limbs(c)[1] for c in q_over_p_mod_r]
+        values += gamma_over_p
+        self.matrix = ring._import(values, [1, limbs + 1, len(base), 1])
+        self.q_mod_r = q % r_big
+        # inv_mod_2_128 (dbfv.rs:763-770) is the inverse modulo 2^128; its negation is reduced
+        # modulo 2^log2_r
+        self.neg_q_inv_mod_r = (-pow(self.q_mod_r, -1, 1 << 128)) % r_big
+        delta = rounded_div(q, modulus.t())
+        neg_b_delta = -(delta * modulus.b())
+        self.r_q_over_p_inv = ring.q_over_p_inv_tensor(base, r_big)
+        self.neg_b_delta_mont2 = [ring.scalar_tensor_big_mont2(base, neg_b_delta << (64 * k)) for k in range(limbs)]
+
+    def limbs(self):
+        return 1 if self.log2_r <= 62 else 2
+
+
+def assert_mul_mont_safe(ring, bound, base):
+    """ring.rs:955-960"""
+    max_p = max(ring.primes[i] for i in base)
+    assert bound * max_p <= 1 << 63, "representatives too large for a montgomery multiplication"
+
+
+def split_overflow(ring, data, x, x_bound, base):
+    """split_overflow (dbfv.rs:893-948): x (period basis, [count, r, ., .]) becomes x_lo in
+    place; returns x_hi as a [count, limbs, ., .] tensor of plain values in (-R/2, R/2]."""
+    be = ring.backend
+    shape = x.shape
+    assert len(base) == shape[1]
+    assert_mul_mont_safe(ring, x_bound, base)
+    limbs = data.limbs()
+    mods = ring.mods_for(base)
+    lifts = be.hal_allocate_i64(shape)
+    be.hal_assign(lifts, x)
+    be.hal_mul_mont64(lifts, data.r_q_over_p_inv, mods)
+    sums = be.hal_allocate_i128([shape[0], limbs + 1, shape[2], shape[3]])
+    be.hal_matmul_i64(sums, data.matrix, lifts)
+    overflow = be.hal_allocate_i64([shape[0], limbs, shape[2], shape[3]])
+    if limbs == 1:
+        be.hal_bconv_correct_pow2(overflow, sums, data.q_mod_r, data.neg_q_inv_mod_r, data.log2_gamma, data.log2_r)
+    else:
+        be.hal_bconv_correct_pow2_wide(overflow, sums, data.q_mod_r, data.neg_q_inv_mod_r, data.log2_gamma, data.log2_r)
+    acc = ring.allocate_wide(shape[0], base)
+    rows = range(shape[0])
+    for k in range(limbs):
+        acc.fma_unbounded(rows, overflow[:, k:k + 1], data.neg_b_delta_mont2[k])
+    acc.add(rows, x, x_bound)
+    ring.reduce_wide_into(x, acc)
+    return overflow
+
+
+def dbfv_reduce(ring, modulus, ct, ct_bound, base):
+    """dbfv_reduce (dbfv.rs:965-1025) on a period-basis dBFV ciphertext: rows 2i, 2i + 1 are the
+    BFV ciphertext of digit i. Returns (result, bound): `modulus.digits()` digits, each reduced
+    modulo b Delta with its carry added to the next digit; the last carry is dropped."""
+    be = ring.backend
+    assert ct.shape[0] % 2 == 0
+    d = ct.shape[0] // 2
+    assert d <= modulus.digits()
+    if modulus.digits() == 1 or d == 0:
+        return ct, ct_bound
+    data = SplitOverflowData(ring, modulus, base)
+    delta = rounded_div(ring.modulus_of(base), modulus.t())
+    delta_mont2 = [ring.scalar_tensor_big_mont2(base, delta << (64 * k)) for k in range(data.limbs())]
+    result = ring.allocate(2 * modulus.digits(), base)
+    be.hal_assign(result[0:2 * d], ct)
+    carry = None
+    for i in range(modulus.digits()):
+        input_row_bound = ct_bound if i < d else 1
+        rows = result[2 * i:2 * i + 2]
+        if carry is not None:
+            acc = ring.allocate_wide(2, base)
+            for k, factor in enumerate(delta_mont2):
+                acc.fma_unbounded(range(2), carry[:, k:k + 1], factor)
+            acc.add(range(2), rows, input_row_bound)
+            ring.reduce_wide_into(rows, acc)
+        row_bound = input_row_bound if carry is None else 1
+        carry = split_overflow(ring, data, rows, row_bound, base)
+    return result, 1

@@ -366,6 +366,11 @@ class WideRingEls:
         step = sum(lb * rb * self.max_p * self.max_p for (_, lb, _, rb) in terms)
         self.record_step(rows, step)
 
+    def fma_unbounded(self, rows, lhs, rhs):
+        """ring.rs:493-500: lhs arbitrary i64 (plain values), rhs bounded by max_p"""
+        self._accumulate(rows, lhs, rhs)
+        self.record_step(rows, self.max_p << 63)
+
     def add(self, rows, src, src_bound):
         self._accumulate(rows, src, self.one)
         self.record_step(rows, src_bound * self.max_p * self.max_p)
@@ -450,6 +455,14 @@ class RingContext:
         for i in base:
             p = self.primes[i]
             vals.append(mont_repr(smallest_lift(pow(scalar % p, -1, p), p), p))
+        return self._import(vals, [1, len(base), 1, 1])
+
+    def scalar_tensor_big_mont2(self, base, scalar):
+        """ring.rs:904-919: mont(mont(smallest_lift(scalar mod p))) per prime"""
+        vals = []
+        for i in base:
+            p = self.primes[i]
+            vals.append(mont_repr(mont_repr(smallest_lift(scalar, p), p), p))
         return self._import(vals, [1, len(base), 1, 1])
 
     def q_over_p_inv_tensor(self, base, scale):
@@ -604,6 +617,10 @@ class RingContext:
         out = self.allocate(wide.count(), wide.rns_base)
         self.reduce_wide_tensor(out, wide, range(wide.count()))
         return out
+
+    def reduce_wide_into(self, dst, wide):
+        """ring.rs:829-838"""
+        self.reduce_wide_tensor(dst, wide, range(wide.count()))
 
     # --- base conversion (ring.rs:1482-1852) -------------------------------------------------------
     def bconv_cache(self, src_base, dst_base):

@@ -1,0 +1,135 @@
+"""The dBFV digit reduction on the MI355X (split_overflow and the dbfv_reduce carry chain,
+dbfv.rs:873-1025), bit-exact against the oracle's restatement (oracle/dbfv.py, itself pinned by
+the reference's property test and the carry-chain invariant) on the same seeded inputs: one-limb
+and two-limb overflows (R = t/b up to 2^100), partial digit counts, unreduced inputs, and the
+batched form against the per-ciphertext form."""
+import numpy as np
+import pytest
+
+from _pkg import load
+from oracle import dbfv as OD
+from oracle import ring as R
+from test_hal_gpu import from_dev, same, to_dev
+from test_ring_gpu import uniform_residues
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    be = load().HipBackend(0)
+    yield be
+    be.close()
+
+
+@pytest.fixture(scope="module")
+def rings(gpu, cpu):
+    pkg = load()
+    made = {}
+
+    def get(field, log2_q, t_bits):
+        key = (field, log2_q, t_bits)
+        if key not in made:
+            made[key] = (pkg.ring.RingContext(gpu, field, log2_q, t_bits),
+                         R.RingContext(cpu, R.FieldDescriptor(field), log2_q, t_bits))
+        return made[key]
+
+    yield get
+    for g, _ in made.values():
+        g.close()
+
+
+def moduli(log2_b, log2_t):
+    pkg = load()
+    return pkg.dbfv.DBFVModulus(log2_b, log2_t), OD.DBFVModulus(log2_b, log2_t)
+
+
+def inputs(g, base, rows, seed, bound):
+    """uniform residues, or for bound 2 unreduced values in (-2p, 2p)"""
+    primes = [g.primes[i] for i in base]
+    rng = np.random.default_rng(seed)
+    shape = (rows, len(base), g.info["deg_lprime"], g.info["deg_k"])
+    x = uniform_residues(rng, primes, shape)
+    if bound == 2:
+        for j, p in enumerate(primes):
+            x[:, j] += rng.integers(-2, 1, size=x[:, j].shape, dtype=np.int64) * p
+    return x
+
+
+SPLIT_CASES = [("field_192", (200, 220), 8, 24), ("field_192", (200, 220), 8, 108), ("field_192", (200, 220), 27, 108),
+               ("field_36864", (960, 980), 27, 108), ("field_36864", (960, 980), 16, 64)]
+
+
+@pytest.mark.parametrize("field,log2_q,log2_b,log2_t", SPLIT_CASES)
+@pytest.mark.parametrize("bound", [1, 2])
+def test_split_overflow_parity(gpu, cpu, rings, field, log2_q, log2_b, log2_t, bound):
+    pkg = load()
+    g, c = rings(field, log2_q, log2_t)
+    base = g.initial_rns_base()
+    mg, mc = moduli(log2_b, log2_t)
+    x = inputs(g, base, 2, 100 + log2_t + bound, bound)
+    dc = OD.SplitOverflowData(c, mc, base)
+    dg = pkg.dbfv.SplitOverflowData(g, mg, base)
+    assert (dg.log2_gamma, dg.q_mod_r, dg.neg_q_inv_mod_r, dg.limbs()) == (dc.log2_gamma, dc.q_mod_r, dc.neg_q_inv_mod_r,
+                                                                           dc.limbs())
+    assert same(from_dev(gpu, dg.matrix), np.asarray(dc.matrix))
+    assert same(from_dev(gpu, dg.r_q_over_p_inv), np.asarray(dc.r_q_over_p_inv))
+    for k in range(dc.limbs()):
+        assert same(from_dev(gpu, dg.neg_b_delta_mont2[k]), np.asarray(dc.neg_b_delta_mont2[k]))
+    xc, xg = x.copy(), to_dev(gpu, x)
+    oc = OD.split_overflow(c, dc, xc, bound, base)
+    og = pkg.dbfv.split_overflow(g, dg, xg, bound, base)
+    assert same(from_dev(gpu, og), np.asarray(oc))
+    assert same(from_dev(gpu, xg), xc)
+
+
+REDUCE_CASES = [("field_192", (200, 220), 16, 64, 4), ("field_192", (200, 220), 16, 64, 2),
+                ("field_192", (260, 280), 27, 108, 4), ("field_192", (260, 280), 27, 108, 1),
+                ("field_36864", (960, 980), 27, 108, 4)]
+
+
+@pytest.mark.parametrize("field,log2_q,log2_b,log2_t,d", REDUCE_CASES)
+def test_dbfv_reduce_parity(gpu, cpu, rings, field, log2_q, log2_b, log2_t, d):
+    pkg = load()
+    g, c = rings(field, log2_q, log2_t)
+    base = g.initial_rns_base()
+    mg, mc = moduli(log2_b, log2_t)
+    x = inputs(g, base, 2 * d, 7 * d + log2_t, 2)
+    rc, bc = OD.dbfv_reduce(c, mc, x.copy(), 2, base)
+    rg, bg = pkg.dbfv.dbfv_reduce(g, mg, to_dev(gpu, x), 2, base)
+    assert bg == bc == 1 and rg.shape[0] == 2 * mg.digits()
+    assert same(from_dev(gpu, rg), np.asarray(rc))
+
+
+@pytest.mark.parametrize("field,log2_q,log2_b,log2_t,d,count", [("field_192", (200, 220), 16, 64, 3, 5),
+                                                              ("field_36864", (960, 980), 27, 108, 4, 3)])
+def test_dbfv_reduce_batch_matches_single(gpu, rings, field, log2_q, log2_b, log2_t, d, count):
+    """the digit-major batch gives every ciphertext the residues of its own dbfv_reduce"""
+    pkg = load()
+    g, _ = rings(field, log2_q, log2_t)
+    base = g.initial_rns_base()
+    mg, _ = moduli(log2_b, log2_t)
+    x = inputs(g, base, 2 * d * count, 31 + count, 1)  # ciphertext c row k at k count + c
+    rb, bb = pkg.dbfv.dbfv_reduce_batch(g, mg, to_dev(gpu, x), count, 1, base)
+    assert bb == 1 and rb.shape[0] == 2 * mg.digits() * count
+    out = from_dev(gpu, rb)
+    for cidx in range(count):
+        single, _ = pkg.dbfv.dbfv_reduce(g, mg, to_dev(gpu, x[cidx::count]), 1, base)
+        assert same(out[cidx::count], from_dev(gpu, single))
+
+
+def test_dbfv_reduce_edges(gpu, rings):
+    pkg = load()
+    g, _ = rings("field_192", (200, 220), 64)
+    base = g.initial_rns_base()
+    x = to_dev(gpu, inputs(g, base, 2, 5, 1))
+    one, _ = moduli(16, 16)
+    out, bound = pkg.dbfv.dbfv_reduce(g, one, x, 2, base)  # one digit: nothing to reduce
+    assert out is x and bound == 2
+    m4, _ = moduli(16, 64)
+    with pytest.raises(ValueError):
+        pkg.dbfv.dbfv_reduce(g, m4, to_dev(gpu, inputs(g, base, 3, 6, 1)), 1, base)
+    with pytest.raises(ValueError):
+        pkg.dbfv.dbfv_reduce(g, m4, to_dev(gpu, inputs(g, base, 10, 6, 1)), 1, base)
+    with pytest.raises(ValueError):
+        pkg.dbfv.DBFVModulus(8, 127)
