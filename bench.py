@@ -431,6 +431,81 @@ def bench_hom_mul(pkg, be, steps, pairs=64, ndigits=9, seed=SEED, cpu=True):
     return res
 
 
+def bench_dbfv(pkg, be, steps, count=16, log2_b=27, ndigits=4, seed=SEED, cpu=True):
+    """The dBFV digit reduction (dbfv_reduce, dbfv.rs:965-1025) at field_36864: `count`
+    period-basis ciphertexts of `ndigits` digits with b = 2^27, t = 2^108 (R = t/b = 2^81, two-limb
+    overflows), every digit split modulo b Delta with its carry added to the next. One step is the
+    whole chain for all ciphertexts (digit-major batch); the fused form runs one kernel per digit,
+    the other form the reference's HAL call sequence."""
+    log2_t = log2_b * ndigits
+    ring = pkg.ring.RingContext(be, FIELD, LOG2_Q, log2_t)
+    base = ring.initial_rns_base()
+    R = len(base)
+    n2, n3 = ring.info["deg_lprime"], ring.info["deg_k"]
+    modulus = pkg.dbfv.DBFVModulus.new(log2_b, ndigits)
+    cts = be.hal_allocate_i64((2 * ndigits * count, R, n2, n3))
+    ring.fill_uniform(cts, base, seed + 21)
+    stream = be.stream()
+    ev = HipEvents()
+    out_f, _ = pkg.dbfv.dbfv_reduce_batch(ring, modulus, cts, count, 1, base)  # warm-up, builds the constants
+    out_u, _ = pkg.dbfv.dbfv_reduce_batch(ring, modulus, cts, count, 1, base, fused=False)
+    if not (be.hal_export(out_f) == be.hal_export(out_u)).all():
+        raise SystemExit("bench: fused dbfv_reduce differs from the HAL call sequence")
+    del out_u
+    be.sync()
+    be.set_timing(True)
+    ev.start(stream)
+    for _ in range(steps):
+        pkg.dbfv.dbfv_reduce_batch(ring, modulus, cts, count, 1, base)
+    ms = ev.stop(stream) / steps
+    be.set_timing(False)
+    rep = be.timing_report().get("dbfv_split_overflow")
+    ev.start(stream)
+    for _ in range(steps):
+        pkg.dbfv.dbfv_reduce_batch(ring, modulus, cts, count, 1, base, fused=False)
+    ms_u = ev.stop(stream) / steps
+    coeff_bytes = 2 * count * R * n2 * n3 * 8  # one digit of every ciphertext
+    res = {
+        "workload": f"field_36864 dbfv_reduce: {count} ciphertexts x {ndigits} digits, b = 2^{log2_b}, t = 2^{log2_t} "
+                    f"(R = 2^{log2_t - log2_b}, two-limb carries), r = {R} primes, digit-major batch",
+        "value": round(count / (ms / 1000.0), 1), "unit": "ciphertext reductions/s", "ms_per_step": round(ms, 3),
+        "form": "fused: one kernel per digit (carry step + split_overflow), bit-identical to the HAL call sequence",
+        "hal_sequence": {"value": round(count / (ms_u / 1000.0), 1), "ms_per_step": round(ms_u, 3),
+                         "form": "the reference's HAL calls (mul_mont64, matmul_i64, bconv_correct_pow2_wide, wide fma, "
+                                 "reduce_wide), batched over the ciphertexts"},
+        "speedup_vs_hal_sequence": round(ms_u / ms, 2),
+    }
+    if rep is not None:
+        avg_ms = rep[1] / rep[0]
+        bpl = rep[2] / rep[0]
+        res["roofline"] = {"bound": "hbm", "kernel": "dbfv_split_overflow", "achieved": round(bpl / (avg_ms / 1000.0) / 1e9, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(bpl / (avg_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+                           "bytes_per_launch": bpl, "avg_launch_ms": round(avg_ms, 4),
+                           "note": f"per launch one digit of all ciphertexts: {coeff_bytes / 1e6:.0f} MB read and written, "
+                                   "plus the carries"}
+    if cpu:
+        from oracle import cpu_backend
+        from oracle import dbfv as odbfv
+        from oracle import ring as oring
+
+        cb = cpu_backend.CpuBackend()
+        orc = oring.RingContext(cb, oring.FieldDescriptor(FIELD), LOG2_Q, log2_t)
+        host = be.hal_export(cts).reshape(2 * ndigits, count, R, n2, n3)
+        ct0 = np.ascontiguousarray(host[:, 0])
+        t0 = time.perf_counter()
+        want, _ = odbfv.dbfv_reduce(orc, odbfv.DBFVModulus.new(log2_b, ndigits), ct0, 1, base)
+        dt = time.perf_counter() - t0
+        got = be.hal_export(out_f).reshape(2 * ndigits, count, R, n2, n3)[:, 0]
+        if not (got == np.asarray(want)).all():
+            raise SystemExit("bench: dbfv_reduce differs from the oracle")
+        res["cpu_baseline"] = {"value": round(1.0 / dt, 3), "unit": "ciphertext reductions/s", "cores": 1, "kind": "port",
+                               "sample": f"1 ciphertext of {ndigits} digits ({dt:.2f} s), also checked bit-exact"}
+    del cts, out_f
+    ring.close()
+    return res
+
+
 MFMA_INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
 
 
@@ -557,8 +632,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64, help="ring elements in the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="hom_mul,batch,zp,galois",
+    ap.add_argument("--workloads", default="hom_mul,batch,zp,galois,dbfv",
                     help="comma-separated extra workloads beside the headline transforms: hom_mul, batch, zp, galois, "
+                         "dbfv, "
                          "or none")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -685,6 +761,8 @@ def main():
         workloads["dense_zp_mfma"] = bench_zp(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
     if "galois" in args.workloads:
         workloads["galois_keyswitch"] = bench_galois(pkg, be, max(1, args.steps // 5), cpu=not args.no_cpu)
+    if "dbfv" in args.workloads:
+        workloads["dbfv_reduce"] = bench_dbfv(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
     if workloads:
         result["workloads"] = workloads
     be.close()

@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libcsdbfv_hal.so")
 
-SOURCES = ["csd_hal.hip", "csd_ntt.hip", "csd_ring.hip", "csd_fused.hip", "csd_zp.hip"]
+SOURCES = ["csd_hal.hip", "csd_ntt.hip", "csd_ring.hip", "csd_fused.hip", "csd_zp.hip", "csd_dbfv.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -84,6 +84,15 @@ CSD_HD int64_t csd_double_red128(const CsdMod& m, i128 x) {
 CSD_HD int64_t csd_to_mont64(const CsdMod& m, int64_t x) { return csd_mont_mul(m, x, m.r2); }
 CSD_HD int64_t csd_to_mont128(const CsdMod& m, i128 x) { return csd_mont_mul(m, csd_red128(m, x), m.r3); }
 
+// cmod_pow2 (ring.rs:86-91) as bconv_correct_pow2 uses it (cpu.rs:982-1057): x mod 2^k in
+// (-2^(k-1), 2^(k-1)]
+CSD_HD i128 csd_pow2_centre(i128 x, int k) {
+    u128 mask = ((u128)1 << k) - 1;
+    i128 reduced = (i128)((u128)x & mask);
+    i128 modulus = (i128)1 << k;
+    return reduced > modulus / 2 ? reduced - modulus : reduced;
+}
+
 // BarrettMul::mul (ntt.rs:149-152): |x| <= 2^63 -> |t| <= 3N/4, t = z x mod N
 CSD_HD int64_t csd_barrett(CsdBarrett b, int64_t x, int64_t n) {
     uint64_t lo;

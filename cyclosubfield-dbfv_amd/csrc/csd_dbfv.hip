@@ -1,0 +1,200 @@
+// The dBFV digit reduction (cyclosubfield-dbfv/src/dbfv.rs:873-1025) fused into one pass over
+// the coefficients per digit.
+//
+// The reference's split_overflow is a sequence of HAL calls over the whole batch:
+//   mul_mont64 (lifts of x R (Q/p)^-1), matmul_i64 (Sum_p lift_p (Q/p) mod R in limbs, and the
+//   gamma-scaled estimate), bconv_correct_pow2{,_wide} (the overflow), then a wide accumulation
+//   (overflow x mont^2(-b Delta 2^(64k)) + x x mont(1)) and reduce_wide;
+// dbfv_reduce adds Delta x carry to each digit first (another wide accumulation and reduction).
+// Every one of these steps is local to a coefficient position (all r primes of it), so one
+// thread per coefficient runs the whole chain in registers. x is read once and x_lo, the
+// overflow written once, against about ten passes over HBM for the call sequence. Each step
+// uses the same integer formulas as the kernel of its HAL call (csd_mont_mul, wrapping i128 sums,
+// the pow2 correction, csd_double_red128 / csd_red128), so the results are bit-identical.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "../../include/csdbfv_ring.h"
+#include "csd_internal.hpp"
+
+namespace {
+
+struct SoArgs {
+    int64_t count, r, n2, n3;
+    int64_t sx[3];   // strides of x (dims 0..2; dim 3 contiguous)
+    int64_t so[3];   // strides of the overflow
+    int64_t sc[3];   // strides of the carry-in (when present)
+    int64_t si[3];   // strides of the separate input (when present)
+    int64_t* x;
+    const int64_t* xin;  // the digit's input when it is not x itself (read-only), or null
+    int zero_in;         // the digit's input is zero (digits beyond the ciphertext's)
+    int64_t* ov;
+    const int64_t* cin;  // carry of the digit below, [count, limbs, n2, n3], or null
+    const int64_t* k;    // constant table, 3 + 3 limbs rows of r (row stride r)
+    i128 f0, f1;
+    int limbs, shift, log2_r;
+    int full_split, full_carry;
+};
+
+// rows of the constant table
+__device__ __forceinline__ int row_lift() { return 0; }                 // r (Q/p)^-1 mod p (plain)
+__device__ __forceinline__ int row_value(int k) { return 1 + k; }       // limb k of (Q/p) mod R
+__device__ __forceinline__ int row_corr(int limbs) { return 1 + limbs; }  // round(gamma / p)
+__device__ __forceinline__ int row_nbd(int limbs, int k) { return 2 + limbs + k; }  // mont^2(-b Delta 2^64k)
+__device__ __forceinline__ int row_one(int limbs) { return 2 + 2 * limbs; }           // mont(1)
+__device__ __forceinline__ int row_delta(int limbs, int k) { return 3 + 2 * limbs + k; }  // mont^2(Delta 2^64k)
+
+// One thread per coefficient position. Its r residues wait in its own LDS column (slot p at
+// p * blockDim.x + threadIdx.x) between the passes over the primes, so no barrier is needed and
+// the loops keep only the running sums in registers; the per-prime constants are wave-uniform
+// scalar loads.
+__global__ __launch_bounds__(256) void k_split_overflow(SoArgs a, ModTable mt) {
+    extern __shared__ int64_t xs[];
+    const int64_t per = a.n2 * a.n3;
+    const int64_t total = a.count * per;
+    const int r = (int)a.r;
+    const int limbs = a.limbs;
+    const int64_t* K = a.k;
+    int64_t* col = xs + threadIdx.x;
+    const int stride = (int)blockDim.x;
+    // one coefficient per thread and no grid-stride loop: a loop with a per-lane exit would make
+    // the prime loops below divergent, and their constants vector loads instead of scalar ones
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < total) {
+        const int64_t c = e / per, j = e - c * per;
+        const int64_t j2 = j / a.n3, j3 = j - j2 * a.n3;
+        int64_t* X = a.x + c * a.sx[0] + j2 * a.sx[2] + j3;
+        const int64_t* XI = a.xin ? a.xin + c * a.si[0] + j2 * a.si[2] + j3 : X;
+        const int64_t sxi = a.xin ? a.si[1] : a.sx[1];
+        int64_t cl0 = 0, cl1 = 0;
+        if (a.cin) {
+            const int64_t* C = a.cin + c * a.sc[0] + j2 * a.sc[2] + j3;
+            cl0 = C[0];
+            if (limbs == 2) cl1 = C[a.sc[1]];
+        }
+        // pass 1: the carry step (dbfv.rs:995-1003), then the lifts and the matmul rows
+        // (dbfv.rs:909-917)
+        u128 s0 = 0, s1 = 0, sg = 0;
+        for (int p = 0; p < r; p++) {
+            const CsdMod m = mt.m[p];
+            int64_t xv = a.zero_in ? 0 : XI[p * sxi];
+            if (a.cin) {
+                // wide fma of each carry limb with mont^2(Delta 2^64k) (mul for the first, fma
+                // after), add of the digit with mont(1), reduce_wide
+                u128 acc = (u128)((i128)cl0 * (i128)K[row_delta(limbs, 0) * r + p]);
+                if (limbs == 2) acc += (u128)((i128)cl1 * (i128)K[row_delta(limbs, 1) * r + p]);
+                acc += (u128)((i128)xv * (i128)K[row_one(limbs) * r + p]);
+                xv = a.full_carry ? csd_to_mont64(m, csd_double_red128(m, (i128)acc)) : csd_red128(m, (i128)acc);
+            }
+            col[p * stride] = xv;
+            const int64_t lift = csd_mont_mul(m, xv, K[row_lift() * r + p]);
+            s0 += (u128)((i128)K[row_value(0) * r + p] * (i128)lift);
+            if (limbs == 2) s1 += (u128)((i128)K[row_value(1) * r + p] * (i128)lift);
+            sg += (u128)((i128)K[row_corr(limbs) * r + p] * (i128)lift);
+        }
+        // the overflow (bconv_correct_pow2{,_wide}, dbfv.rs:919-938)
+        const u128 half = (u128)1 << (a.shift - 1);
+        const i128 quot = (i128)(sg + half) >> a.shift;
+        int64_t ov0, ov1 = 0;
+        if (limbs == 1) {
+            const i128 v = (i128)((s0 - (u128)quot * (u128)a.f0) * (u128)a.f1);
+            ov0 = (int64_t)csd_pow2_centre(v, a.log2_r);
+        } else {
+            const i128 v = (i128)(s0 + (s1 << 64));
+            const i128 cc = csd_pow2_centre((i128)(((u128)v - (u128)quot * (u128)a.f0) * (u128)a.f1), a.log2_r);
+            ov0 = (int64_t)cc;
+            ov1 = (int64_t)((i128)((u128)cc - (u128)(i128)ov0) >> 64);
+        }
+        int64_t* O = a.ov + c * a.so[0] + j2 * a.so[2] + j3;
+        O[0] = ov0;
+        if (limbs == 2) O[a.so[1]] = ov1;
+        // pass 2: x_lo = reduce(overflow mont^2(-b Delta) + x mont(1)) (dbfv.rs:940-947)
+        for (int p = 0; p < r; p++) {
+            const CsdMod m = mt.m[p];
+            const int64_t xv = col[p * stride];
+            u128 acc = (u128)((i128)ov0 * (i128)K[row_nbd(limbs, 0) * r + p]);
+            if (limbs == 2) acc += (u128)((i128)ov1 * (i128)K[row_nbd(limbs, 1) * r + p]);
+            acc += (u128)((i128)xv * (i128)K[row_one(limbs) * r + p]);
+            X[p * a.sx[1]] = a.full_split ? csd_to_mont64(m, csd_double_red128(m, (i128)acc)) : csd_red128(m, (i128)acc);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int csd_dbfv_split_overflow(csd_ctx* ctx, const csd_view* x, const csd_view* x_in, int zero_in,
+                                       const csd_view* overflow, const csd_view* carry_in,
+                                       const csd_view* consts, const csd_modulus* mods, size_t nmods, int64_t f0_lo,
+                                       int64_t f0_hi, int64_t f1_lo, int64_t f1_hi, size_t shift, size_t log2_r,
+                                       int full_split, int full_carry) {
+    CSD_TRY check_view(x, "dbfv_split_overflow x");
+    check_view(overflow, "dbfv_split_overflow overflow");
+    check_view(consts, "dbfv_split_overflow constants");
+    SoArgs a;
+    memset(&a, 0, sizeof(a));
+    a.count = x->shape[0];
+    a.r = x->shape[1];
+    a.n2 = x->shape[2];
+    a.n3 = x->shape[3];
+    const int limbs = (int)overflow->shape[1];
+    if (limbs != 1 && limbs != 2) throw CsdError("dbfv_split_overflow: the overflow has 1 or 2 limbs");
+    if (log2_r < 1 || log2_r > (limbs == 1 ? 62u : 126u)) throw CsdError("dbfv_split_overflow: log2_r out of range for the limbs");
+    if (shift < 1 || shift >= 127) throw CsdError("dbfv_split_overflow: shift out of range");
+    if (a.r < 1 || a.r > CSD_MAX_MODULI || (size_t)a.r != nmods) throw CsdError("dbfv_split_overflow: one modulus per limb of x");
+    if (overflow->shape[0] != a.count || overflow->shape[2] != a.n2 || overflow->shape[3] != a.n3)
+        throw CsdError("dbfv_split_overflow: overflow shape");
+    if (x->strides[3] != 1 || overflow->strides[3] != 1) throw CsdError("dbfv_split_overflow: dimension 3 must be contiguous");
+    const int64_t rows = 3 + 3 * (int64_t)limbs;
+    if (consts->shape[0] != 1 || consts->shape[1] != 1 || consts->shape[2] != rows || consts->shape[3] != a.r ||
+        consts->strides[2] != a.r || consts->strides[3] != 1)
+        throw CsdError("dbfv_split_overflow: constants must be a contiguous [1, 1, 3 + 3 limbs, r] table");
+    if (carry_in) {
+        check_view(carry_in, "dbfv_split_overflow carry");
+        if (carry_in->shape[0] != a.count || carry_in->shape[1] != limbs || carry_in->shape[2] != a.n2 ||
+            carry_in->shape[3] != a.n3 || carry_in->strides[3] != 1)
+            throw CsdError("dbfv_split_overflow: carry shape");
+        for (int d = 0; d < 3; d++) a.sc[d] = carry_in->strides[d];
+        a.cin = (const int64_t*)carry_in->ptr;
+    }
+    if (x_in) {
+        check_view(x_in, "dbfv_split_overflow input");
+        for (int d = 0; d < 4; d++)
+            if (x_in->shape[d] != x->shape[d]) throw CsdError("dbfv_split_overflow: input shape differs from x");
+        if (x_in->strides[3] != 1) throw CsdError("dbfv_split_overflow: dimension 3 must be contiguous");
+        for (int d = 0; d < 3; d++) a.si[d] = x_in->strides[d];
+        a.xin = (const int64_t*)x_in->ptr;
+    }
+    a.zero_in = zero_in ? 1 : 0;
+    for (int d = 0; d < 3; d++) {
+        a.sx[d] = x->strides[d];
+        a.so[d] = overflow->strides[d];
+    }
+    a.x = (int64_t*)x->ptr;
+    a.ov = (int64_t*)overflow->ptr;
+    a.k = (const int64_t*)consts->ptr;
+    a.f0 = (i128)(((u128)(uint64_t)f0_hi << 64) | (uint64_t)f0_lo);
+    a.f1 = (i128)(((u128)(uint64_t)f1_hi << 64) | (uint64_t)f1_lo);
+    a.limbs = limbs;
+    a.shift = (int)shift;
+    a.log2_r = (int)log2_r;
+    a.full_split = full_split ? 1 : 0;
+    a.full_carry = full_carry ? 1 : 0;
+    if (numel(x) == 0) return 0;
+    ModTable t = make_mod_table(mods, nmods, a.r);
+    TimedScope ts(ctx, "dbfv_split_overflow",
+                  8.0 * (double)a.count * a.n2 * a.n3 * ((zero_in ? 1.0 : 2.0) * a.r + limbs + (carry_in ? limbs : 0)));
+    const size_t lds = (size_t)a.r * 256 * sizeof(int64_t);  // at most 128 KiB (64 primes)
+    static bool attr = false;
+    if (!attr) {
+        HIPCHECK(hipFuncSetAttribute((const void*)k_split_overflow, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    const int64_t total = a.count * a.n2 * a.n3;
+    if ((total + 255) / 256 > (int64_t)INT32_MAX) throw CsdError("dbfv_split_overflow: too many coefficients for one launch");
+    hipLaunchKernelGGL(k_split_overflow, dim3((unsigned)((total + 255) / 256)), dim3(256), lds, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+    CSD_CATCH
+}

@@ -211,8 +211,7 @@ extern "C" int csd_ctx_stream(csd_ctx* ctx, void** stream) {
     CSD_CATCH
 }
 
-extern "C" int csd_hal_allocate(csd_ctx* ctx, int elem_bytes, const int64_t shape[4], void** out) {
-    CSD_TRY
+static void* allocate_tensor(csd_ctx* ctx, int elem_bytes, const int64_t shape[4], bool zero) {
     if (elem_bytes != 8 && elem_bytes != 16) throw CsdError("elem_bytes must be 8 or 16");
     size_t n = 1;
     for (int d = 0; d < 4; d++) {
@@ -220,8 +219,17 @@ extern "C" int csd_hal_allocate(csd_ctx* ctx, int elem_bytes, const int64_t shap
         n *= (size_t)shape[d];
     }
     void* p = ctx->alloc_bytes(n * elem_bytes);
-    csd_zero_fill(ctx, p, ((n * elem_bytes) + 7) & ~(size_t)7);
-    *out = p;
+    if (zero) csd_zero_fill(ctx, p, ((n * elem_bytes) + 7) & ~(size_t)7);
+    return p;
+}
+
+extern "C" int csd_hal_allocate(csd_ctx* ctx, int elem_bytes, const int64_t shape[4], void** out) {
+    CSD_TRY *out = allocate_tensor(ctx, elem_bytes, shape, true);
+    CSD_CATCH
+}
+
+extern "C" int csd_hal_allocate_uninit(csd_ctx* ctx, int elem_bytes, const int64_t shape[4], void** out) {
+    CSD_TRY *out = allocate_tensor(ctx, elem_bytes, shape, false);
     CSD_CATCH
 }
 
@@ -845,13 +853,6 @@ struct Pow2Args {
     int shift, log2_modulus, wide;
 };
 
-__device__ __forceinline__ i128 pow2_centre(i128 x, int k) {
-    u128 mask = ((u128)1 << k) - 1;
-    i128 reduced = (i128)((u128)x & mask);
-    i128 modulus = (i128)1 << k;
-    return reduced > modulus / 2 ? reduced - modulus : reduced;
-}
-
 __global__ void k_bconv_correct_pow2(Pow2Args a) {
     int64_t total = a.I * a.K * a.L;
     i128 half = (i128)1 << (a.shift - 1);
@@ -863,11 +864,11 @@ __global__ void k_bconv_correct_pow2(Pow2Args a) {
         if (!a.wide) {
             i128 quot = (i128)((u128)s[a.ss[1]] + (u128)half) >> a.shift;
             i128 v = (i128)(((u128)s[0] - (u128)quot * (u128)a.f0) * (u128)a.f1);
-            d[0] = (int64_t)pow2_centre(v, a.log2_modulus);
+            d[0] = (int64_t)csd_pow2_centre(v, a.log2_modulus);
         } else {
             i128 quot = (i128)((u128)s[2 * a.ss[1]] + (u128)half) >> a.shift;
             i128 v = (i128)((u128)s[0] + ((u128)s[a.ss[1]] << 64));
-            i128 c = pow2_centre((i128)(((u128)v - (u128)quot * (u128)a.f0) * (u128)a.f1), a.log2_modulus);
+            i128 c = csd_pow2_centre((i128)(((u128)v - (u128)quot * (u128)a.f0) * (u128)a.f1), a.log2_modulus);
             int64_t lo = (int64_t)c;
             int64_t hi = (int64_t)((i128)((u128)c - (u128)(i128)lo) >> 64);
             d[0] = lo;

@@ -97,6 +97,20 @@ class SplitOverflowData:
         neg_b_delta = -(delta * modulus.b())
         self.r_q_over_p_inv = ring.q_over_p_inv_tensor(base, r_big)
         self.neg_b_delta_mont2 = [ring.scalar_tensor_big_mont2(base, neg_b_delta << (64 * k)) for k in range(limbs)]
+        self.delta_mont2 = [ring.scalar_tensor_big_mont2(base, delta << (64 * k)) for k in range(limbs)]
+        # the same constants as one [1, 1, 3 + 3 limbs, r] table for the fused kernel
+        # (csd_dbfv_split_overflow): lifts, value limbs, gamma / p, -b Delta, mont(1), Delta
+        tables = [self.r_q_over_p_inv] + [None] * (limbs + 1) + self.neg_b_delta_mont2 + [ring.scalar_tensor(base, 1)]
+        tables += self.delta_mont2
+        host = []
+        for idx, t in enumerate(tables):
+            if t is None:
+                host.append(values[(idx - 1) * len(base):idx * len(base)])
+            else:
+                host.append([int(v) for v in be.hal_export(t)])
+        self.consts = be.hal_allocate_i64([1, 1, len(host), len(base)])
+        be.hal_import(self.consts, [v for row in host for v in row])
+        self.max_p, self.min_p = max_p, min_p
 
     def limbs(self):
         """one i64 limb of overflow for R <= 2^62, else two (bconv_correct_pow2_wide)"""
@@ -118,14 +132,49 @@ def _assert_mul_mont_safe(ring, bound, base):
         raise ValueError("representatives too large for a montgomery multiplication")
 
 
-def split_overflow(ring, data, x, x_bound, base):
+def _needs_full_reduction(data, steps):
+    """the bound tracking of a fresh WideRingEls over the accumulation steps (ring.rs:354-368):
+    whether reduce_wide takes the double reduction (ring.rs:843-865)"""
+    bound = 0
+    for s in steps:
+        bound += s
+        if bound > (1 << 127) - 1:
+            raise OverflowError("wide ring element overflow: too many accumulation steps")
+    return bound > (data.min_p << 63)
+
+
+def _split_fused(ring, data, x, x_bound, base, carry=None, carry_bound=1, x_in=None, zero_in=False):
+    """split_overflow, preceded by the carry step when `carry` is given, as one kernel; the
+    reduction kinds are those the wide accumulators of the call sequence would choose. The digit
+    comes from `x_in` (or is zero with `zero_in`) when given, and x_lo goes to `x`."""
+    be = ring.backend
+    limbs = data.limbs()
+    unbounded = [data.max_p << 63] * limbs
+    full_carry = False
+    if carry is not None:
+        _assert_mul_mont_safe(ring, carry_bound, base)
+        full_carry = _needs_full_reduction(data, unbounded + [carry_bound * data.max_p * data.max_p])
+        x_bound = 1
+    _assert_mul_mont_safe(ring, x_bound, base)
+    full_split = _needs_full_reduction(data, unbounded + [x_bound * data.max_p * data.max_p])
+    overflow = be.allocate_i64_uninit([x.shape[0], limbs, x.shape[2], x.shape[3]])  # written entirely
+    be.dbfv_split_overflow(x, overflow, carry, data.consts, ring.mods_for(base), data.q_mod_r, data.neg_q_inv_mod_r,
+                           data.log2_gamma, data.log2_r, full_split, full_carry, x_in=x_in, zero_in=zero_in)
+    return overflow
+
+
+def split_overflow(ring, data, x, x_bound, base, fused=True):
     """split_overflow (dbfv.rs:893-948): the period-basis rows `x` ([count, r, ., .]) become
     x_lo in place, with x = x_lo + (b Delta) x_hi mod Q and |x_lo| <= 2 b Delta; returns x_hi as a
-    [count, limbs, ., .] tensor of plain values in (-R/2, R/2]."""
+    [count, limbs, ., .] tensor of plain values in (-R/2, R/2]. `fused` runs the whole sequence
+    as one kernel (csd_dbfv_split_overflow); otherwise it is the reference's HAL call sequence.
+    Both give identical residues."""
     be = ring.backend
     shape = x.shape
     if len(base) != shape[1]:
         raise ValueError("split_overflow: the tensor's limbs must match the base")
+    if fused:
+        return _split_fused(ring, data, x, x_bound, base)
     _assert_mul_mont_safe(ring, x_bound, base)
     limbs = data.limbs()
     mods = ring.mods_for(base)
@@ -151,18 +200,25 @@ def split_overflow(ring, data, x, x_bound, base):
     return overflow
 
 
-def _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base):
+def _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base, fused, src=None):
     """the carry chain of dbfv_reduce (dbfv.rs:990-1023) over `result`, whose digit i occupies
-    rows [i per_digit, (i + 1) per_digit) and whose first d digits came from the input"""
+    rows [i per_digit, (i + 1) per_digit) and whose first d digits come from the input: already
+    copied into `result`, or read from `src` (same layout) by the fused kernels, which then
+    take the digits beyond the input as zero instead of reading zeroed rows"""
     data = split_overflow_data(ring, modulus, base)
-    delta = rounded_div(ring.modulus_of(base), modulus.t())
-    delta_mont2 = [ring.scalar_tensor_big_mont2(base, delta << (64 * k)) for k in range(data.limbs())]
+    delta_mont2 = data.delta_mont2
     carry = None
     rows = range(per_digit)
     for i in range(modulus.digits()):
         # rows copied from the (possibly unreduced) input have its bound; rows beyond it are zero
         input_row_bound = ct_bound if i < d else 1
         digit = result[i * per_digit:(i + 1) * per_digit]
+        if fused:
+            # the carry step and the split of this digit as one kernel
+            x_in = src[i * per_digit:(i + 1) * per_digit] if src is not None and i < d else None
+            zero_in = src is not None and i >= d
+            carry = _split_fused(ring, data, digit, input_row_bound, base, carry, input_row_bound, x_in, zero_in)
+            continue
         if carry is not None:
             # digit_i += Delta carry; the carry limbs are plain integers, hence the unbounded fma
             acc = ring.allocate_wide(per_digit, base)
@@ -171,13 +227,29 @@ def _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base):
             acc.add(rows, digit, input_row_bound)
             ring.reduce_wide_into(digit, acc)
         row_bound = input_row_bound if carry is None else 1
-        carry = split_overflow(ring, data, digit, row_bound, base)
+        carry = split_overflow(ring, data, digit, row_bound, base, fused=False)
 
 
-def dbfv_reduce(ring, modulus, ct, ct_bound, base):
+def _reduce_into_new(ring, modulus, src, d, per_digit, ct_bound, base, fused):
+    """the result batch of dbfv_reduce (dbfv.rs:985-988, 1023-1024). The call sequence allocates
+    it zeroed and copies the input digits in; the fused kernels read the input digits in place
+    and write every row, so the result needs neither the fill nor the copy."""
+    rows = modulus.digits() * per_digit
+    if fused:
+        result = ring.backend.allocate_i64_uninit([rows, len(base), ring.info["deg_lprime"], ring.info["deg_k"]])
+        _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base, True, src=src)
+    else:
+        result = ring.allocate(rows, base)
+        ring.backend.hal_assign(result[0:d * per_digit], src)
+        _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base, False)
+    return result, 1
+
+
+def dbfv_reduce(ring, modulus, ct, ct_bound, base, fused=True):
     """dbfv_reduce (dbfv.rs:965-1025) of one period-basis dBFV ciphertext (rows 2i, 2i + 1 =
     digit i): every digit is reduced modulo b Delta and its carry added to the next digit, the
-    carries of digits >= modulus.digits() dropped. Returns (result, bound)."""
+    carries of digits >= modulus.digits() dropped. Returns (result, bound). `fused` runs each
+    digit's carry step and split as one kernel; otherwise the reference's HAL call sequence."""
     if ct.shape[0] % 2:
         raise ValueError("a dBFV ciphertext has two rows per digit")
     d = ct.shape[0] // 2
@@ -185,13 +257,10 @@ def dbfv_reduce(ring, modulus, ct, ct_bound, base):
         raise ValueError("ciphertext has more digits than the modulus")
     if modulus.digits() == 1 or d == 0:
         return ct, ct_bound
-    result = ring.allocate(2 * modulus.digits(), base)
-    ring.backend.hal_assign(result[0:2 * d], ct)
-    _reduce_chain(ring, modulus, result, d, 2, ct_bound, base)
-    return result, 1
+    return _reduce_into_new(ring, modulus, ct, d, 2, ct_bound, base, fused)
 
 
-def dbfv_reduce_batch(ring, modulus, cts, count, ct_bound, base):
+def dbfv_reduce_batch(ring, modulus, cts, count, ct_bound, base, fused=True):
     """dbfv_reduce for `count` ciphertexts of d digits at once. `cts` holds them digit-major:
     row (2 i + s) count + c is row s of digit i of ciphertext c; the result ([2 digits count,
     r, ., .]) has the same layout. Every HAL call covers all ciphertexts; each ciphertext sees
@@ -203,7 +272,4 @@ def dbfv_reduce_batch(ring, modulus, cts, count, ct_bound, base):
         raise ValueError("ciphertexts have more digits than the modulus")
     if modulus.digits() == 1 or d == 0:
         return cts, ct_bound
-    result = ring.allocate(2 * modulus.digits() * count, base)
-    ring.backend.hal_assign(result[0:2 * d * count], cts)
-    _reduce_chain(ring, modulus, result, d, 2 * count, ct_bound, base)
-    return result, 1
+    return _reduce_into_new(ring, modulus, cts, d, 2 * count, ct_bound, base, fused)

@@ -272,11 +272,16 @@ class HipBackend:
         return out
 
     # --- allocation and transfer (hal/mod.rs:115-125) --------------------------------------------
-    def _alloc(self, shape, elem):
+    def _alloc(self, shape, elem, zero=True):
         shape = [int(x) for x in shape]
         ptr = ctypes.c_void_p()
-        check(lib().csd_hal_allocate(self.ctx, ctypes.c_int(elem), (ctypes.c_int64 * 4)(*shape), ctypes.byref(ptr)))
+        fn = lib().csd_hal_allocate if zero else lib().csd_hal_allocate_uninit
+        check(fn(self.ctx, ctypes.c_int(elem), (ctypes.c_int64 * 4)(*shape), ctypes.byref(ptr)))
         return _Allocation(self, ptr.value, shape, _row_major_strides(shape), elem)
+
+    def allocate_i64_uninit(self, shape):
+        """an i64 tensor without the zero fill, for outputs that the next call writes entirely"""
+        return self._alloc(shape, 8, zero=False)
 
     def hal_allocate_i64(self, shape):
         return self._alloc(shape, 8)
@@ -432,6 +437,28 @@ class HipBackend:
                                                     f0l, f0h, f1l, f1h, ctypes.c_size_t(shift), ctypes.c_size_t(log2_modulus)))
 
     # --- permutations (hal/mod.rs:339-349) ---------------------------------------------------------
+    def dbfv_split_overflow(self, x, overflow, carry_in, consts, mods, f0, f1, shift, log2_r, full_split, full_carry,
+                            x_in=None, zero_in=False):
+        """csd_dbfv_split_overflow: split_overflow (dbfv.rs:893-948) in one pass, optionally after
+        the carry step of dbfv_reduce; f0, f1 are i128 constants. The digit is read from `x_in`
+        (all zero with `zero_in`) when given, else from `x`; the result goes to `x`."""
+        def words(v):
+            v &= (1 << 128) - 1
+            lo, hi = v & ((1 << 64) - 1), v >> 64
+            s = lambda w: ctypes.c_int64(w - (1 << 64) if w >= 1 << 63 else w)
+            return s(lo), s(hi)
+
+        arr, n = _mods(mods)
+        f0l, f0h = words(f0)
+        f1l, f1h = words(f1)
+        cin = ctypes.byref(carry_in.view()) if carry_in is not None else None
+        xin = ctypes.byref(x_in.view()) if x_in is not None else None
+        check(lib().csd_dbfv_split_overflow(self.ctx, ctypes.byref(x.view()), xin, ctypes.c_int(1 if zero_in else 0),
+                                            ctypes.byref(overflow.view()), cin,
+                                            ctypes.byref(consts.view()), arr, n, f0l, f0h, f1l, f1h,
+                                            ctypes.c_size_t(shift), ctypes.c_size_t(log2_r),
+                                            ctypes.c_int(1 if full_split else 0), ctypes.c_int(1 if full_carry else 0)))
+
     def host_alloc(self, shape):
         """pinned host memory for a host-resident int64 tensor (PinnedHostTensor)"""
         return PinnedHostTensor(self, shape)

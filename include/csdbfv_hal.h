@@ -71,6 +71,9 @@ int csd_calibrate(csd_ctx* ctx, int kind, size_t n, double* out);
 /* hal_allocate_i64 / hal_allocate_i128 (hal/mod.rs:119-122): zero-initialised, row-major;
  * elem_bytes is 8 (i64) or 16 (i128). */
 int csd_hal_allocate(csd_ctx* ctx, int elem_bytes, const int64_t shape[4], void** out);
+/* The same allocation without the zero fill, for outputs whose every element the next call
+ * writes (no counterpart in the reference, whose allocations are always zeroed). */
+int csd_hal_allocate_uninit(csd_ctx* ctx, int elem_bytes, const int64_t shape[4], void** out);
 int csd_hal_free(csd_ctx* ctx, void* ptr);
 /* pinned host memory for host-resident tensors streamed to the device (no trait counterpart; the
  * CPU build's KeyStore::store / CpuTensor::offload, keystore.rs:50-127, cpu.rs:82-185, move key

@@ -84,6 +84,24 @@ int csd_ring_apply_galois(csd_ring* ring, const csd_view* dst, const csd_view* s
 int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t digits,
                               const csd_view* parts, const csd_view* c0, const uint64_t* gs, size_t ngal,
                               const csd_view* keys, const csd_view* out);
+/* split_overflow (dbfv.rs:893-948) of the period-basis tensor x ([count, r, n2, n3], Montgomery
+ * form, |x| <= x_bound p) in one pass over the coefficients: x becomes x_lo in place and overflow
+ * ([count, limbs, n2, n3], limbs 1 for log2_r <= 62, else 2) receives x_hi, with
+ * x = x_lo + (b Delta) x_hi mod Q. The digit is read from x_in (same shape) when it is given, or
+ * taken as zero when zero_in is set (digits beyond a ciphertext's), and x_lo is written to x
+ * either way. When carry_in ([count, limbs, n2, n3]) is given, the digit first
+ * becomes reduce_wide(carry_in * mont^2(Delta 2^64k) + x * mont(1)), the carry step of dbfv_reduce
+ * (dbfv.rs:995-1003). consts is the contiguous [1, 1, 3 + 3 limbs, r] table of
+ * SplitOverflowData (dbfv.rs:741-759), row by row: r (Q/p)^-1 mod p (plain lifts); the limbs of
+ * (Q/p) mod R; round(gamma / p); mont^2(-b Delta 2^64k) per limb; mont(1); mont^2(Delta 2^64k)
+ * per limb. f0 = Q mod R and f1 = -Q^-1 mod R are i128 (lo, hi words), shift = log2_gamma.
+ * full_split / full_carry select the double reduction of reduce_wide (ring.rs:843-865) as the
+ * bound tracking of the wide accumulators decides it. Bit-identical to the HAL call sequence. No
+ * counterpart in the reference's HAL (a fused form of its dbfv.rs sequence). */
+int csd_dbfv_split_overflow(csd_ctx* ctx, const csd_view* x, const csd_view* x_in, int zero_in, const csd_view* overflow,
+                            const csd_view* carry_in, const csd_view* consts, const csd_modulus* mods, size_t nmods,
+                            int64_t f0_lo, int64_t f0_hi, int64_t f1_lo, int64_t f1_hi, size_t shift, size_t log2_r,
+                            int full_split, int full_carry);
 /* The same key switch with the keys in pinned host memory instead of HBM, the fallback for key
  * sets larger than the device (the CPU build streams them from its NVMe KeyStore,
  * keystore.rs:50-127): host_keys[t] points to the contiguous [2*digits, r, ., .] key of rotation t

@@ -2,7 +2,8 @@
 dbfv.rs:873-1025), bit-exact against the oracle's restatement (oracle/dbfv.py, itself pinned by
 the reference's property test and the carry-chain invariant) on the same seeded inputs: one-limb
 and two-limb overflows (R = t/b up to 2^100), partial digit counts, unreduced inputs, and the
-batched form against the per-ciphertext form."""
+batched form against the per-ciphertext form; both the fused one-kernel-per-digit form and the
+reference's HAL call sequence."""
 import numpy as np
 import pytest
 
@@ -62,7 +63,8 @@ SPLIT_CASES = [("field_192", (200, 220), 8, 24), ("field_192", (200, 220), 8, 10
 
 @pytest.mark.parametrize("field,log2_q,log2_b,log2_t", SPLIT_CASES)
 @pytest.mark.parametrize("bound", [1, 2])
-def test_split_overflow_parity(gpu, cpu, rings, field, log2_q, log2_b, log2_t, bound):
+@pytest.mark.parametrize("fused", [True, False])
+def test_split_overflow_parity(gpu, cpu, rings, field, log2_q, log2_b, log2_t, bound, fused):
     pkg = load()
     g, c = rings(field, log2_q, log2_t)
     base = g.initial_rns_base()
@@ -78,7 +80,7 @@ def test_split_overflow_parity(gpu, cpu, rings, field, log2_q, log2_b, log2_t, b
         assert same(from_dev(gpu, dg.neg_b_delta_mont2[k]), np.asarray(dc.neg_b_delta_mont2[k]))
     xc, xg = x.copy(), to_dev(gpu, x)
     oc = OD.split_overflow(c, dc, xc, bound, base)
-    og = pkg.dbfv.split_overflow(g, dg, xg, bound, base)
+    og = pkg.dbfv.split_overflow(g, dg, xg, bound, base, fused=fused)
     assert same(from_dev(gpu, og), np.asarray(oc))
     assert same(from_dev(gpu, xg), xc)
 
@@ -89,14 +91,15 @@ REDUCE_CASES = [("field_192", (200, 220), 16, 64, 4), ("field_192", (200, 220), 
 
 
 @pytest.mark.parametrize("field,log2_q,log2_b,log2_t,d", REDUCE_CASES)
-def test_dbfv_reduce_parity(gpu, cpu, rings, field, log2_q, log2_b, log2_t, d):
+@pytest.mark.parametrize("fused", [True, False])
+def test_dbfv_reduce_parity(gpu, cpu, rings, field, log2_q, log2_b, log2_t, d, fused):
     pkg = load()
     g, c = rings(field, log2_q, log2_t)
     base = g.initial_rns_base()
     mg, mc = moduli(log2_b, log2_t)
     x = inputs(g, base, 2 * d, 7 * d + log2_t, 2)
     rc, bc = OD.dbfv_reduce(c, mc, x.copy(), 2, base)
-    rg, bg = pkg.dbfv.dbfv_reduce(g, mg, to_dev(gpu, x), 2, base)
+    rg, bg = pkg.dbfv.dbfv_reduce(g, mg, to_dev(gpu, x), 2, base, fused=fused)
     assert bg == bc == 1 and rg.shape[0] == 2 * mg.digits()
     assert same(from_dev(gpu, rg), np.asarray(rc))
 
@@ -104,7 +107,8 @@ def test_dbfv_reduce_parity(gpu, cpu, rings, field, log2_q, log2_b, log2_t, d):
 @pytest.mark.parametrize("field,log2_q,log2_b,log2_t,d,count", [("field_192", (200, 220), 16, 64, 3, 5),
                                                               ("field_36864", (960, 980), 27, 108, 4, 3)])
 def test_dbfv_reduce_batch_matches_single(gpu, rings, field, log2_q, log2_b, log2_t, d, count):
-    """the digit-major batch gives every ciphertext the residues of its own dbfv_reduce"""
+    """the digit-major batch gives every ciphertext the residues of its own dbfv_reduce (the
+    fused batch against the unfused call sequence per ciphertext)"""
     pkg = load()
     g, _ = rings(field, log2_q, log2_t)
     base = g.initial_rns_base()
@@ -114,7 +118,7 @@ def test_dbfv_reduce_batch_matches_single(gpu, rings, field, log2_q, log2_b, log
     assert bb == 1 and rb.shape[0] == 2 * mg.digits() * count
     out = from_dev(gpu, rb)
     for cidx in range(count):
-        single, _ = pkg.dbfv.dbfv_reduce(g, mg, to_dev(gpu, x[cidx::count]), 1, base)
+        single, _ = pkg.dbfv.dbfv_reduce(g, mg, to_dev(gpu, x[cidx::count]), 1, base, fused=False)
         assert same(out[cidx::count], from_dev(gpu, single))
 
 
@@ -133,3 +137,48 @@ def test_dbfv_reduce_edges(gpu, rings):
         pkg.dbfv.dbfv_reduce(g, m4, to_dev(gpu, inputs(g, base, 10, 6, 1)), 1, base)
     with pytest.raises(ValueError):
         pkg.dbfv.DBFVModulus(8, 127)
+
+
+def test_fused_split_errors(gpu, rings):
+    """csd_dbfv_split_overflow rejects tensors that do not match its constant table"""
+    pkg = load()
+    g, _ = rings("field_192", (200, 220), 64)
+    base = g.initial_rns_base()
+    mg, _ = moduli(16, 64)
+    data = pkg.dbfv.split_overflow_data(g, mg, base)
+    x = to_dev(gpu, inputs(g, base, 2, 9, 1))
+    mods = g.mods_for(base)
+    ok = gpu.hal_allocate_i64([2, 1, x.shape[2], x.shape[3]])
+    for ov, cin, consts in [(gpu.hal_allocate_i64([2, 3, x.shape[2], x.shape[3]]), None, data.consts),  # 3 limbs
+                            (gpu.hal_allocate_i64([3, 1, x.shape[2], x.shape[3]]), None, data.consts),  # count
+                            (ok, gpu.hal_allocate_i64([2, 2, x.shape[2], x.shape[3]]), data.consts),  # carry limbs
+                            (ok, None, gpu.hal_allocate_i64([1, 1, 5, len(base)]))]:  # table rows
+        with pytest.raises(pkg.HalError):
+            gpu.dbfv_split_overflow(x, ov, cin, consts, mods, data.q_mod_r, data.neg_q_inv_mod_r, data.log2_gamma,
+                                    data.log2_r, True, True)
+
+
+def test_fused_split_out_of_place(gpu, rings):
+    """the out-of-place form (digit read from x_in, or zero) writes what the in-place form writes,
+    and leaves its input untouched"""
+    pkg = load()
+    g, _ = rings("field_192", (200, 220), 108)
+    base = g.initial_rns_base()
+    mg, _ = moduli(27, 108)
+    data = pkg.dbfv.split_overflow_data(g, mg, base)
+    host = inputs(g, base, 2, 41, 1)
+    src = to_dev(gpu, host)
+    inplace = to_dev(gpu, host)
+    ov_a = pkg.dbfv._split_fused(g, data, inplace, 1, base)
+    out = gpu.allocate_i64_uninit(list(src.shape))
+    ov_b = pkg.dbfv._split_fused(g, data, out, 1, base, x_in=src)
+    assert same(from_dev(gpu, out), from_dev(gpu, inplace)) and same(from_dev(gpu, ov_a), from_dev(gpu, ov_b))
+    assert same(from_dev(gpu, src), host)
+    # a zero digit after a carry equals the carry step on zeroed rows
+    zero_rows = to_dev(gpu, np.zeros_like(host))
+    ov_c = pkg.dbfv._split_fused(g, data, zero_rows, 1, base, carry=ov_a, carry_bound=1)
+    out_z = gpu.allocate_i64_uninit(list(src.shape))
+    ov_d = pkg.dbfv._split_fused(g, data, out_z, 1, base, carry=ov_a, carry_bound=1, x_in=src, zero_in=True)
+    assert same(from_dev(gpu, out_z), from_dev(gpu, zero_rows)) and same(from_dev(gpu, ov_c), from_dev(gpu, ov_d))
+    with pytest.raises(pkg.HalError):
+        pkg.dbfv._split_fused(g, data, out, 1, base, x_in=to_dev(gpu, inputs(g, base, 3, 42, 1)))
