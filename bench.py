@@ -647,11 +647,16 @@ def main():
 
     pkg = load_package()
     be = pkg.HipBackend(0)
+    t_setup = time.perf_counter()
     ring = pkg.ring.RingContext(be, FIELD, LOG2_Q, T_BITS)
     base = ring.initial_rns_base()
     primes = [ring.primes[i] for i in base]
     info = ring.info
+    # compute_rns_factor_transform for every prime (ring.rs:1337-1415): prime sampling, then the CRT
+    # kernels' traces, NTTs and inverses on the device
     ring.precompute_rns_factor_transforms()
+    be.sync()
+    setup_s = time.perf_counter() - t_setup
     host = uniform_batch(primes, args.count, info["deg_lprime"], info["deg_k"], SEED)
     data = be.hal_allocate_i64(host.shape)
     be.hal_import(data, host)
@@ -746,6 +751,7 @@ def main():
         "overlap": "the batch runs as two halves on two streams, one pass apart (dim-2 passes beside dim-3 passes); "
                    "kernels and roofline are from a serial pass",
         "wall_s": round(wall, 3),
+        "setup_s": round(setup_s, 3),
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(primes, host, min(args.cpu_sample, args.count), cpu_threads(),

@@ -346,6 +346,55 @@ struct DevBuf {
     }
 };
 
+// ---------------------------------------------------------------------------------------------
+// CRT kernels on the device (compute_rns_factor_kernels, ring.rs:1337-1415). The trace of
+// zeta_p^(g^-1) over the Galois subgroup, for every transversal element g, is a sum of powers
+// zeta_p^e whose exponents e = (x g^-1 mod m) mod order do not depend on the prime; the host
+// computes them once per ring, and one thread per g sums its powers (square-and-multiply in
+// Montgomery form). The kernels' inverses are Fermat powers, one thread per entry. Both give the
+// canonical residues the host formulas give.
+
+__device__ __forceinline__ int64_t canon_mod(const CsdMod& m, int64_t v) {
+    if (v < 0) v += m.N;
+    if (v >= m.N) v -= m.N;
+    return v;
+}
+
+// base_m^e in Montgomery form (base_m and the result Montgomery representatives in [-N, N])
+__device__ int64_t mont_pow(const CsdMod& m, int64_t base_m, uint64_t e) {
+    int64_t r = csd_to_mont64(m, 1);
+    int64_t b = base_m;
+    while (e) {
+        if (e & 1) r = csd_mont_mul(m, r, b);
+        b = csd_mont_mul(m, b, b);
+        e >>= 1;
+    }
+    return r;
+}
+
+// out[t] = mont(sum_s zeta^E[t][s]) mod p, canonical: the Montgomery form of the trace
+__global__ void k_crt_traces(const uint32_t* __restrict__ E, int64_t T, int64_t S, CsdMod m, int64_t zeta_m,
+                             int64_t* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int64_t acc = 0;
+    for (int64_t s = 0; s < S; s++) {
+        acc += canon_mod(m, mont_pow(m, zeta_m, E[t * S + s]));
+        if (acc >= m.N) acc -= m.N;
+    }
+    out[t] = acc;
+}
+
+// x = mont(from_mont(x)^(p - 2)) mod p, canonical, for Montgomery-form entries of any
+// representative (the NTT-domain kernels); zero entries are rejected on the host first
+__global__ void k_crt_invert(int64_t* __restrict__ x, int64_t n, CsdMod m) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t v = x[i] % m.N;
+    if (v < 0) v += m.N;
+    x[i] = canon_mod(m, mont_pow(m, v, (uint64_t)m.N - 2));
+}
+
 csd_view make_view(void* p, int64_t s0, int64_t s1, int64_t s2, int64_t s3) {
     csd_view v;
     v.ptr = p;
@@ -458,7 +507,38 @@ struct csd_ring {
             if (i >= primes.size()) throw CsdError("RNS base index out of range");
     }
 
-    // compute_rns_factor_kernels (ring.rs:1337-1415); the forward NTTs run on the device
+    // the exponents of the trace sums of one part of the field (K or L'), shared by all primes:
+    // e[t][s] = (sub[s] g_t^-1 mod m) mod order for the transversal g_t and the Galois subgroup
+    struct TraceTable {
+        std::unique_ptr<DevBuf> e;
+        size_t T = 0, S = 0;
+    };
+    TraceTable tt_lp, tt_k;
+
+    const TraceTable& trace_table(bool k_part) {
+        TraceTable& tt = k_part ? tt_k : tt_lp;
+        if (tt.e) return tt;
+        const Field& F = *f;
+        const uint64_t m = F.m;
+        const uint64_t order = k_part ? F.m_k() : F.m_prime;
+        const std::vector<uint64_t>& sub = k_part ? F.gal_mk_k : F.gal_mp_lp;
+        const std::vector<uint64_t> tr = k_part ? F.kq_transversal() : F.lp_transversal();
+        if (order >= ((uint64_t)1 << 32)) throw CsdError("CRT kernel traces: cyclotomic order too large");
+        std::vector<uint32_t> e(tr.size() * sub.size());
+        for (size_t t = 0; t < tr.size(); t++) {
+            const uint64_t gi = inv_mod_u(tr[t], m);
+            for (size_t s = 0; s < sub.size(); s++) e[t * sub.size() + s] = (uint32_t)(mulmod_u(sub[s], gi, m) % order);
+        }
+        tt.T = tr.size();
+        tt.S = sub.size();
+        tt.e = std::make_unique<DevBuf>(ctx, std::max<size_t>(4, e.size() * 4));
+        HIPCHECK(hipMemcpyAsync(tt.e->p, e.data(), e.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        return tt;
+    }
+
+    // compute_rns_factor_kernels (ring.rs:1337-1415): traces, forward NTTs and inverses on the
+    // device
     const Kernels& factor_kernels(size_t idx, csd_ntt_plan* plan_lp, csd_ntt_plan* plan_k) {
         int64_t p = primes[idx];
         auto it = kernels.find(p);
@@ -475,51 +555,34 @@ struct csd_ring {
         }
         uint64_t z_mp = powmod_u(zeta, crt2(1, F.m_prime, 0, F.m_k()), (uint64_t)p);
         uint64_t z_mk = powmod_u(zeta, crt2(0, F.m_prime, 1, F.m_k()), (uint64_t)p);
-        auto traces = [&](uint64_t zp, uint64_t order, const std::vector<uint64_t>& sub_, const std::vector<uint64_t>& tr) {
-            std::vector<uint64_t> powers(order);
-            uint64_t cur = 1;
-            for (uint64_t e = 0; e < order; e++) {
-                powers[e] = cur;
-                cur = mulmod_u(cur, zp, (uint64_t)p);
-            }
-            std::vector<int64_t> out;
-            for (uint64_t g : tr) {
-                uint64_t gi = inv_mod_u(g, m), s = 0;
-                for (uint64_t x : sub_) s = (s + powers[mulmod_u(x, gi, m) % order]) % (uint64_t)p;
-                out.push_back((int64_t)s);
-            }
-            return out;
-        };
-        auto k_lp = traces(z_mp, F.m_prime, F.gal_mp_lp, F.lp_transversal());
-        auto k_k = traces(z_mk, F.m_k(), F.gal_mk_k, F.kq_transversal());
-        auto to_ntt = [&](const std::vector<int64_t>& kern, csd_ntt_plan* plan, int dim) {
-            std::vector<int64_t> mk;
-            for (int64_t x : kern) mk.push_back(mont_repr(x, p));
-            DevBuf b(ctx, mk.size() * 8);
-            csd_view v = dim == 2 ? make_view(b.p, 1, 1, (int64_t)mk.size(), 1) : make_view(b.p, 1, 1, 1, (int64_t)mk.size());
-            ok(csd_hal_import(ctx, &v, mk.data(), mk.size()));
+        const CsdMod md = make_mod_table(&mods[idx], 1, 1).m[0];
+        // traces (device), forward NTT (device), inverse (device); the host keeps the canonical
+        // NTT-domain values, checks them for zeros and caches them per prime
+        auto kernel_of = [&](uint64_t zp, const TraceTable& tt, csd_ntt_plan* plan, int dim, std::vector<int64_t>& fwd,
+                             std::vector<int64_t>& inv) {
+            const int64_t T = (int64_t)tt.T;
+            DevBuf b(ctx, (size_t)T * 8);
+            const int64_t zeta_m = mont_repr((int64_t)zp, p);
+            hipLaunchKernelGGL(k_crt_traces, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, ctx->stream,
+                               (const uint32_t*)tt.e->p, T, (int64_t)tt.S, md, zeta_m, (int64_t*)b.p);
+            HIPCHECK(hipGetLastError());
+            csd_view v = dim == 2 ? make_view(b.p, 1, 1, T, 1) : make_view(b.p, 1, 1, 1, T);
             ok(csd_hal_ntt_mont64_dim(ctx, &v, &plan, 1, dim));
-            std::vector<int64_t> out(mk.size());
-            ok(csd_hal_export(ctx, &v, out.data(), out.size()));
-            for (auto& x : out) x = (int64_t)mod_of(x, p);
-            return out;
+            fwd.resize((size_t)T);
+            ok(csd_hal_export(ctx, &v, fwd.data(), fwd.size()));
+            for (auto& x : fwd) {
+                x = (int64_t)mod_of(x, p);
+                if (x == 0) throw CsdError("CRT kernel is not invertible");  // from_mont(x) == 0
+            }
+            // from_mont_repr then invert then mont_repr (ring.rs:1395-1407)
+            hipLaunchKernelGGL(k_crt_invert, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, ctx->stream, (int64_t*)b.p, T, md);
+            HIPCHECK(hipGetLastError());
+            inv.resize((size_t)T);
+            ok(csd_hal_export(ctx, &v, inv.data(), inv.size()));
         };
         Kernels K;
-        K.lp = to_ntt(k_lp, plan_lp, 2);
-        K.k = to_ntt(k_k, plan_k, 3);
-        // from_mont_repr then invert then mont_repr (ring.rs:1395-1407)
-        uint64_t r_inv = powmod_u(mod_of(mont_repr(1, p), p), (uint64_t)p - 2, (uint64_t)p);
-        auto invert = [&](const std::vector<int64_t>& kern) {
-            std::vector<int64_t> out;
-            for (int64_t x : kern) {
-                uint64_t plain = mulmod_u(mod_of(x, p), r_inv, (uint64_t)p);
-                if (plain == 0) throw CsdError("CRT kernel is not invertible");
-                out.push_back(mont_repr((int64_t)powmod_u(plain, (uint64_t)p - 2, (uint64_t)p), p));
-            }
-            return out;
-        };
-        K.inv_lp = invert(K.lp);
-        K.inv_k = invert(K.k);
+        kernel_of(z_mp, trace_table(false), plan_lp, 2, K.lp, K.inv_lp);
+        kernel_of(z_mk, trace_table(true), plan_k, 3, K.k, K.inv_k);
         return kernels[p] = K;
     }
 

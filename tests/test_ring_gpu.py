@@ -74,9 +74,13 @@ def test_transform_parity(gpu, cpu, rings, field, log2_q, t_bits, count):
     assert same(from_dev(gpu, dg), dc)
 
 
-def test_crt_kernels_match_oracle(gpu, cpu, rings):
-    g, c = rings("field_36864", (960, 980), 76)
-    for idx in [0, 16, 36]:
+@pytest.mark.parametrize("field,log2_q,t_bits,idxs", [("field_36864", (960, 980), 76, [0, 16, 36]),
+                                                      ("field_49152", (1280, 1300), 76, [0, 22, 48]),
+                                                      ("field_192", (100, 120), 8, [0, 1]), ("field_18", (100, 120), 4, [0])])
+def test_crt_kernels_match_oracle(gpu, cpu, rings, field, log2_q, t_bits, idxs):
+    """the device-computed CRT kernels (traces, NTT, inverses) equal the oracle's host ones"""
+    g, c = rings(field, log2_q, t_bits)
+    for idx in idxs:
         k = g.crt_kernels(idx)
         tc = c.transform_cache([idx])
         exp = [c._kernels[c.primes[idx]][t] for t in range(4)]
