@@ -83,6 +83,8 @@ struct GksArgs {
     int64_t* out;  // [batch, 2, r, n2, n3]
     int full0, full1;
     int64_t chunks;  // position chunks of 256 along dim 3
+    int64_t groups;  // ciphertext groups of GKS_CB
+    int64_t nblocks;
     int64_t one[CSD_MAX_MODULI];  // mont(1) per limb
 };
 
@@ -92,14 +94,18 @@ __device__ __forceinline__ int64_t reduce_wide_entry(const CsdMod& m, i128 x, in
 }
 
 __global__ __launch_bounds__(256) void k_galois_keyswitch(GksArgs a, ModTable mt) {
-    // block -> (ciphertext group, j, k, l chunk)
+    // block -> (j, k, l chunk, ciphertext group), the group fastest: the groups of one position
+    // read the same key entries, so they run back to back, and the XCD-aware order (the
+    // hardware deals consecutive workgroups to the 8 XCDs in turn; tile t = (b mod 8) nblocks/8
+    // + b/8) keeps them on one XCD, whose L2 then serves the key to all but the first
     int64_t b = blockIdx.x;
+    if (a.nblocks % 8 == 0) b = (b % 8) * (a.nblocks / 8) + b / 8;
+    const int64_t cg = b % a.groups;
+    b /= a.groups;
     const int64_t c = b % a.chunks;
     b /= a.chunks;
     const int64_t k = b % a.n2;
-    b /= a.n2;
-    const int64_t j = b % a.r;
-    const int64_t cg = b / a.r;
+    const int64_t j = b / a.n2;
     const int64_t l = c * 256 + threadIdx.x;
     if (l >= a.n3) return;
     const int64_t plane = a.n2 * a.n3;
@@ -111,15 +117,48 @@ __global__ __launch_bounds__(256) void k_galois_keyswitch(GksArgs a, ModTable mt
     i128 acc0[GKS_CB], acc1[GKS_CB];
 #pragma unroll
     for (int q = 0; q < GKS_CB; q++) acc0[q] = acc1[q] = 0;
-    for (int i = 0; i < a.d; i++) {
-        const int64_t k0 = a.key[(int64_t)(2 * i) * limb + dst_pos];
-        const int64_t k1 = a.key[(int64_t)(2 * i + 1) * limb + dst_pos];
+    if (nc == GKS_CB) {
+        // a full group: the loads of a digit carry no per-ciphertext condition, so all ten are in
+        // flight together, and the next digit's loads are issued before this digit's products
+        const int64_t* P = a.parts + (int64_t)c_first * a.d * limb + src_pos;
+        const int64_t cstride = (int64_t)a.d * limb;
+        const int64_t* K = a.key + dst_pos;
+        int64_t pv[GKS_CB], k0 = 0, k1 = 0;
 #pragma unroll
-        for (int q = 0; q < GKS_CB; q++) {
-            if (q < nc) {
-                const int64_t p = a.parts[((int64_t)(c_first + q) * a.d + i) * limb + src_pos];
-                acc0[q] = (i128)((u128)acc0[q] + (u128)((i128)p * (i128)k0));
-                acc1[q] = (i128)((u128)acc1[q] + (u128)((i128)p * (i128)k1));
+        for (int q = 0; q < GKS_CB; q++) pv[q] = P[q * cstride];
+        k0 = K[0];
+        k1 = K[limb];
+        for (int i = 0; i < a.d; i++) {
+            int64_t pn[GKS_CB], k0n = 0, k1n = 0;
+            if (i + 1 < a.d) {
+#pragma unroll
+                for (int q = 0; q < GKS_CB; q++) pn[q] = P[q * cstride + (int64_t)(i + 1) * limb];
+                k0n = K[(int64_t)(2 * i + 2) * limb];
+                k1n = K[(int64_t)(2 * i + 3) * limb];
+            }
+#pragma unroll
+            for (int q = 0; q < GKS_CB; q++) {
+                acc0[q] = (i128)((u128)acc0[q] + (u128)((i128)pv[q] * (i128)k0));
+                acc1[q] = (i128)((u128)acc1[q] + (u128)((i128)pv[q] * (i128)k1));
+            }
+            if (i + 1 < a.d) {
+#pragma unroll
+                for (int q = 0; q < GKS_CB; q++) pv[q] = pn[q];
+                k0 = k0n;
+                k1 = k1n;
+            }
+        }
+    } else {
+        for (int i = 0; i < a.d; i++) {
+            const int64_t k0 = a.key[(int64_t)(2 * i) * limb + dst_pos];
+            const int64_t k1 = a.key[(int64_t)(2 * i + 1) * limb + dst_pos];
+#pragma unroll
+            for (int q = 0; q < GKS_CB; q++) {
+                if (q < nc) {
+                    const int64_t p = a.parts[((int64_t)(c_first + q) * a.d + i) * limb + src_pos];
+                    acc0[q] = (i128)((u128)acc0[q] + (u128)((i128)p * (i128)k0));
+                    acc1[q] = (i128)((u128)acc1[q] + (u128)((i128)p * (i128)k1));
+                }
             }
         }
     }
@@ -241,6 +280,8 @@ void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t
     int64_t groups = (batch + GKS_CB - 1) / GKS_CB;
     int64_t blocks = groups * r * n2 * a.chunks;
     if (blocks > (int64_t)INT32_MAX) throw CsdError("galois key switch: grid too large");
+    a.groups = groups;
+    a.nblocks = blocks;
     double bytes = 8.0 * (double)(batch * d * r * n2 * n3 + batch * r * n2 * n3 + 2 * batch * r * n2 * n3 + 2 * d * r * n2 * n3);
     TimedScope ts(ctx, "galois_keyswitch", bytes);
     hipLaunchKernelGGL(k_galois_keyswitch, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);

@@ -23,6 +23,7 @@ def gpu():
     ("field_18", (100, 120), 4, 2, 2, 3, 3),
     ("field_49152", (1280, 1300), 76, 3, 3, 10, 2),
     ("field_36864", (960, 980), 76, 17, 9, 2, 1),
+    ("field_36864", (960, 980), 76, 17, 9, 8, 1),  # one full group of 8 ciphertexts, 9 digits deep
 ])
 def test_galois_keyswitch_parity(gpu, cpu, field, log2_q, t_bits, r, digits, batch, ngal):
     pkg = load()
