@@ -341,10 +341,135 @@ __global__ __launch_bounds__(512) void k_zp_mfma_lds(MfmaArgs a, ModTable mt) {
     }
 }
 
-static bool g_zp_lds = true;  // which MFMA kernel form csd_zp_apply_dim uses (csd_zp_set_kernel)
+// The 64 x 64 form: 4 waves share one 64-row x 64-vector tile, and each wave keeps a 2 x 2
+// arrangement of 16 x 16 output blocks (60 accumulators, held in AGPRs). A K-step's operands of a
+// wave are 2 x 8 planes of A rows and 2 x 8 planes of X vectors, 32 LDS reads for 256 MFMAs:
+// half the LDS bytes per MFMA of the 32 x 64 form. Staging is as above: 32 KiB of A and 32 KiB of
+// X per K-step, double-buffered (128 KiB), so one workgroup (one wave per SIMD) runs per CU.
 
-extern "C" int csd_zp_set_kernel(int lds_staged) {
-    g_zp_lds = lds_staged != 0;
+static constexpr int ZPW_THREADS = 256;
+static constexpr int ZPW_A_BYTES = ZP_DIGITS * 64 * 64;  // 32 KiB
+static constexpr int ZPW_X_BYTES = ZP_DIGITS * 64 * 64;  // 32 KiB
+
+__global__ __launch_bounds__(256) void k_zp_mfma_wide(MfmaArgs a, ModTable mt) {
+    extern __shared__ __align__(16) int8_t zs[];  // [2][A tile | X tile]
+    int64_t b = blockIdx.x;
+    if (a.nblocks % 8 == 0) b = (b % 8) * (a.nblocks / 8) + b / 8;
+    const int64_t nt = b % a.ntiles;
+    const int64_t mtile = (b / a.ntiles) % a.mtiles;
+    const int64_t l = b / (a.ntiles * a.mtiles);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x16 = lane & 15, g = lane >> 4;
+    const int rb = wave & 1, cb = wave >> 1;  // this wave's 32-row half and 32-vector half
+    const int64_t pa = a.mpad * a.kpad, px = a.npad * a.kpad;
+    const int8_t* Ag = a.a + (l * ZP_DIGITS * a.mpad + mtile * 64) * a.kpad;
+    const int8_t* Xg = a.x + (l * ZP_DIGITS * a.npad + nt * 64) * a.kpad;
+    // staging: A and X have 2048 16-byte chunks each (8 per thread); chunk c of a tile is plane
+    // c / 256, row (c / 4) % 64, k-group c % 4, and lands at byte 16 c of the tile. The chunks go
+    // from global memory straight into LDS (global_load_lds_dwordx4: lane i of a wave writes its
+    // 16 bytes at the wave's base + 16 i), so no registers hold them in between.
+    auto stage_lds = [&](int64_t k0, int buf) {
+        const int off = buf * (ZPW_A_BYTES + ZPW_X_BYTES);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int c = tid + q * ZPW_THREADS;
+            const int pl = c >> 8, row = (c >> 2) & 63, kg = c & 3;
+            const int wbase = (q * ZPW_THREADS + wave * 64) * 16;
+            __builtin_amdgcn_global_load_lds((const void*)(Ag + pl * pa + row * a.kpad + k0 + kg * 16), zs + off + wbase, 16,
+                                             0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(Xg + pl * px + row * a.kpad + k0 + kg * 16),
+                                             zs + off + ZPW_A_BYTES + wbase, 16, 0, 0);
+        }
+    };
+    const CsdMod md = mt.m[l];
+    int64_t total[2][2][4];
+#pragma unroll
+    for (int ua = 0; ua < 2; ua++)
+#pragma unroll
+        for (int ux = 0; ux < 2; ux++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) total[ua][ux][r] = 0;
+    for (int64_t kc = 0; kc < a.kpad; kc += a.chunk) {
+        const int64_t ke = min(a.kpad, kc + a.chunk);
+        v4i acc[2][2][2 * ZP_DIGITS - 1];
+#pragma unroll
+        for (int ua = 0; ua < 2; ua++)
+#pragma unroll
+            for (int ux = 0; ux < 2; ux++)
+#pragma unroll
+                for (int s = 0; s < 2 * ZP_DIGITS - 1; s++) acc[ua][ux][s] = (v4i){0, 0, 0, 0};
+        stage_lds(kc, 0);
+        __builtin_amdgcn_s_waitcnt(0);  // the direct loads have landed in LDS
+        __syncthreads();
+        int buf = 0;
+        for (int64_t k0 = kc; k0 < ke; k0 += 64) {
+            const bool more = k0 + 64 < ke;
+            // the next K-step into the other buffer, whose last reads ended before the barrier
+            if (more) stage_lds(k0 + 64, buf ^ 1);
+            const int8_t* base = zs + buf * (ZPW_A_BYTES + ZPW_X_BYTES);
+            // both vector halves' operands once per K-step, then each row half against them
+            v4i xv[2][ZP_DIGITS];
+#pragma unroll
+            for (int ux = 0; ux < 2; ux++)
+#pragma unroll
+                for (int d = 0; d < ZP_DIGITS; d++)
+                    xv[ux][d] = *(const v4i*)(base + ZPW_A_BYTES + ((d * 64 + cb * 32 + ux * 16 + x16) * 4 + g) * 16);
+#pragma unroll
+            for (int ua = 0; ua < 2; ua++) {
+                v4i av[ZP_DIGITS];
+#pragma unroll
+                for (int d = 0; d < ZP_DIGITS; d++)
+                    av[d] = *(const v4i*)(base + ((d * 64 + rb * 32 + ua * 16 + x16) * 4 + g) * 16);
+#pragma unroll
+                for (int ux = 0; ux < 2; ux++)
+#pragma unroll
+                    for (int da = 0; da < ZP_DIGITS; da++)
+#pragma unroll
+                        for (int dx = 0; dx < ZP_DIGITS; dx++)
+                            acc[ua][ux][da + dx] =
+                                __builtin_amdgcn_mfma_i32_16x16x64_i8(av[da], xv[ux][dx], acc[ua][ux][da + dx], 0, 0, 0);
+            }
+            // each wave's LDS reads of this step and direct loads of the next have completed
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            buf ^= 1;
+        }
+#pragma unroll
+        for (int ua = 0; ua < 2; ua++)
+#pragma unroll
+            for (int ux = 0; ux < 2; ux++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    u128 v = 0;
+#pragma unroll
+                    for (int s = 0; s < 2 * ZP_DIGITS - 1; s++) v += (u128)(i128)acc[ua][ux][s][r] << (8 * s);
+                    int64_t t = total[ua][ux][r] + zp_reduce(md, (i128)v);
+                    total[ua][ux][r] = t >= md.N ? t - md.N : t;
+                }
+    }
+#pragma unroll
+    for (int ux = 0; ux < 2; ux++) {
+        const int64_t v = nt * 64 + cb * 32 + ux * 16 + x16;
+        if (v >= a.Ly.nvec) continue;
+        int64_t* out = a.y + l * a.Ly.s_limb + (v / a.Ly.nlo) * a.Ly.s_hi + (v % a.Ly.nlo) * a.Ly.s_lo;
+#pragma unroll
+        for (int ua = 0; ua < 2; ua++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t i = mtile * 64 + rb * 32 + ua * 16 + 4 * g + r;
+                if (i < a.m) out[i * a.Ly.s_elem] = total[ua][ux][r];
+            }
+    }
+}
+
+static int g_zp_kernel = 1;  // the MFMA kernel form 0 uses: 0 L2 operands, 1 LDS 32 x 64, 2 LDS 64 x 64
+
+extern "C" int csd_zp_set_kernel(int form) {
+    if (form < 0 || form > 2) {
+        csd_set_error("csd_zp_set_kernel: form must be 0, 1 or 2");
+        return 1;
+    }
+    g_zp_kernel = form;
     return 0;
 }
 
@@ -463,7 +588,7 @@ extern "C" int csd_zp_matrix_create(csd_ctx* ctx, const csd_view* a, const csd_m
     z->r = r;
     z->m = m;
     z->k = k;
-    z->mpad = round_up(m, 32);
+    z->mpad = round_up(m, 64);  // whole 64-row tiles (the 32-row kernels use half tiles)
     z->kpad = round_up(k, 64);
     z->mods.assign(mods, mods + nmods);
     try {
@@ -555,7 +680,18 @@ extern "C" int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* z, const csd_
         if (a.nblocks > (int64_t)INT32_MAX) throw CsdError("zp apply: too many tiles");
         TimedScope ts(ctx, "zp_mfma",
                       (double)z->r * ZP_DIGITS * (z->mpad * z->kpad + npad * z->kpad) + 8.0 * z->r * nvec * z->m);
-        if (g_zp_lds) {
+        if (g_zp_kernel == 2) {
+            // 64-row tiles over the same planes (mpad is a multiple of 64)
+            a.mtiles = z->mpad / 64;
+            a.nblocks = z->r * a.mtiles * a.ntiles;
+            const size_t lds = 2 * (size_t)(ZPW_A_BYTES + ZPW_X_BYTES);
+            static bool attr = false;
+            if (!attr) {
+                HIPCHECK(hipFuncSetAttribute((const void*)k_zp_mfma_wide, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                attr = true;
+            }
+            hipLaunchKernelGGL(k_zp_mfma_wide, dim3((unsigned)a.nblocks), dim3(ZPW_THREADS), lds, ctx->stream, a, mt);
+        } else if (g_zp_kernel == 1) {
             const size_t lds = 2 * (size_t)(ZPL_A_BYTES + ZPL_X_BYTES);
             static bool attr = false;
             if (!attr) {

@@ -478,10 +478,11 @@ class HipBackend:
                                      ctypes.c_int({"mfma": 0, "plain": 1}[form]), ctypes.c_size_t(max_chunk)))
 
     @staticmethod
-    def zp_set_kernel(lds_staged):
-        """which int8 MFMA kernel the "mfma" form runs (process-wide): LDS-staged tiles shared by 8
-        waves (True, the default) or operands loaded by 4 waves straight from L2 (False)"""
-        check(lib().csd_zp_set_kernel(ctypes.c_int(1 if lds_staged else 0)))
+    def zp_set_kernel(form):
+        """which int8 MFMA kernel the "mfma" form runs (process-wide, csd_zp_set_kernel): 1 (the
+        default) LDS-staged 32 x 64 tiles for 8 waves, 2 direct-to-LDS 64 x 64 tiles for 4 waves
+        with 2 x 2 blocks each, 0 operands loaded by 4 waves straight from L2"""
+        check(lib().csd_zp_set_kernel(ctypes.c_int(int(form))))
 
     def hal_create_permutation(self, inner_shape, shifts):
         n = len(inner_shape)
