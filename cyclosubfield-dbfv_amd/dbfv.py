@@ -265,8 +265,8 @@ def dbfv_reduce_batch(ring, modulus, cts, count, ct_bound, base, fused=True):
     row (2 i + s) count + c is row s of digit i of ciphertext c; the result ([2 digits count,
     r, ., .]) has the same layout. Every HAL call covers all ciphertexts; each ciphertext sees
     the calls and operand bounds of dbfv_reduce, so its residues are identical."""
-    if cts.shape[0] % (2 * count):
-        raise ValueError("dbfv_reduce_batch: rows must be 2 x digits x count")
+    if count < 1 or cts.shape[0] % (2 * count):
+        raise ValueError("dbfv_reduce_batch: count must be positive and the rows 2 x digits x count")
     d = cts.shape[0] // (2 * count)
     if d > modulus.digits():
         raise ValueError("ciphertexts have more digits than the modulus")

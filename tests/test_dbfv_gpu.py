@@ -137,6 +137,10 @@ def test_dbfv_reduce_edges(gpu, rings):
         pkg.dbfv.dbfv_reduce(g, m4, to_dev(gpu, inputs(g, base, 10, 6, 1)), 1, base)
     with pytest.raises(ValueError):
         pkg.dbfv.DBFVModulus(8, 127)
+    with pytest.raises(ValueError):
+        pkg.dbfv.dbfv_reduce_batch(g, m4, x, 0, 1, base)
+    with pytest.raises(ValueError):
+        pkg.dbfv.dbfv_reduce_batch(g, m4, to_dev(gpu, inputs(g, base, 6, 7, 1)), 4, 1, base)
 
 
 def test_fused_split_errors(gpu, rings):
