@@ -141,8 +141,8 @@ def _rows(t, first, step):
 def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     """bfv_hom_mul (bfv.rs:747-775) for B ciphertext pairs at once: lhs, rhs are [2B, r, ., .]
     CRT-basis tensors with pair b in rows 2b, 2b + 1; the result has the same layout. Every HAL
-    call covers all B pairs, except the gadget base conversion, which takes one element
-    (bfv.rs:417-421) and runs per pair. Each element sees the same calls with the same operands,
+    call covers all B pairs, the gadget base conversion (one element in bfv.rs:417-421) included.
+    Each element sees the same calls with the same operands,
     in the same order, as in B calls of bfv_hom_mul, and the wide accumulators track the same
     per-row bounds, so the residues are identical."""
     be = ring.backend
@@ -171,10 +171,7 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     dg = clip_digits(digits, r)
     d = len(dg)
     parts = ring.allocate(B * d, base)  # pair b, digit i at row b d + i
-    tmp = ring.allocate(1, base)
-    for b in range(B):
-        be.hal_assign(tmp, rescaled[2 * B + b:2 * B + b + 1])
-        ring.bconv_gadget_tensors(parts[b * d:(b + 1) * d], tmp, base, dg)
+    ring.bconv_gadget_tensors(parts, rescaled[2 * B:3 * B], base, dg)  # may destroy rows 2B.. (unused below)
     ring.pb_to_crt_tensor(parts, base)
     acc2 = ring.allocate_wide(2 * B, base)  # row s B + b
     if d:

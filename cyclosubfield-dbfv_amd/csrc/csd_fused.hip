@@ -287,3 +287,115 @@ void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t
     hipLaunchKernelGGL(k_galois_keyswitch, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
     HIPCHECK(hipGetLastError());
 }
+
+// ---------------------------------------------------------------------------------------------
+// gadget base conversion, one kernel (GadgetBConvCache::bconv, ring.rs:1809-1851)
+//
+// The HAL sequence multiplies the source by (D_i/p_u)^-1 in Montgomery form, forms the
+// [d, r + 1] i128 matrix product over chunks of `acc` source limbs (each chunk reduced with
+// double_red128 and added), adds round(corr / 2^shift) * (-D_i 2^64 mod p_j) through red128 and
+// converts to Montgomery form. Row i of the matrix is zero outside digit i, so thread (b, pos)
+// only multiplies the digit's own limbs; a zero product adds nothing to a wrapping i128 sum, and
+// every chunk, empty or not, is still reduced and added, so the residues are those of the
+// sequence. (Ring primes are below 2^57, so a chunk holds at least 2^13 products and up to 64
+// limbs always form one chunk.) The digit's lifted limbs sit in the thread's LDS column.
+
+struct GbArgs {
+    int64_t B, d, r, n3, P;  // source elements, digits, limbs, dim-3 length, positions n2 n3
+    int64_t ss[4], sd[4];    // source [B, r, n2, n3] and destination [B d, r, n2, n3] strides
+    const int64_t* src;
+    int64_t* dst;
+    const int64_t* dopi;  // [r]: (D_i / p_u)^-1 for u in digit i
+    const int64_t* mat;   // [d][r + 1][r]: (D_i / p_u) R^2 mod p_j, row r the correction weights
+    const int64_t* dmp;   // [d][r]: -D_i 2^64 mod p_j
+    int64_t acc;          // source limbs per i128 chunk
+    int shift;            // log2 of q~
+    int32_t lo[CSD_MAX_MODULI], hi[CSD_MAX_MODULI];
+};
+
+__device__ __forceinline__ i128 gb_mac(i128 acc, int64_t a, int64_t b) {
+    uint64_t lo;
+    int64_t hi;
+    csd_smul_wide(a, b, lo, hi);
+    return (i128)((u128)acc + (((u128)(uint64_t)hi << 64) | lo));
+}
+
+__global__ __launch_bounds__(256) void k_gadget_bconv(GbArgs a, ModTable mt) {
+    extern __shared__ int64_t gb_x[];
+    const int64_t chunks = (a.P + 255) / 256;
+    const int64_t b = blockIdx.x / chunks;
+    const int64_t pos = (blockIdx.x % chunks) * 256 + threadIdx.x;
+    if (pos >= a.P) return;  // the kernel has no barrier
+    const int64_t k = pos / a.n3, l = pos % a.n3;
+    const int64_t* s = a.src + b * a.ss[0] + k * a.ss[2] + l * a.ss[3];
+    int64_t* o = a.dst + b * a.d * a.sd[0] + k * a.sd[2] + l * a.sd[3];
+    int64_t* xs = gb_x + threadIdx.x;
+    const int r = (int)a.r;
+    const i128 half = (i128)1 << (a.shift - 1);
+    for (int i = 0; i < (int)a.d; i++) {
+        const int lo = a.lo[i], hi = a.hi[i];
+        const int64_t* mi = a.mat + (int64_t)i * (r + 1) * r;
+        i128 corr = 0;
+        for (int u = lo; u < hi; u++) {
+            int64_t x = csd_mont_mul(mt.m[u], s[u * a.ss[1]], a.dopi[u]);
+            xs[(u - lo) * 256] = x;
+            corr = gb_mac(corr, mi[(int64_t)r * r + u], x);
+        }
+        const int64_t q = (int64_t)((i128)((u128)corr + (u128)half) >> a.shift);  // |q| <= 2^63
+        for (int j = 0; j < r; j++) {
+            const int64_t* mj = mi + (int64_t)j * r;
+            int64_t v = 0;
+            for (int64_t c0 = 0; c0 < r; c0 += a.acc) {
+                const int u0 = (int)max((int64_t)lo, c0), u1 = (int)min((int64_t)hi, min(c0 + a.acc, (int64_t)r));
+                i128 sum = 0;
+                for (int u = u0; u < u1; u++) sum = gb_mac(sum, mj[u], xs[(u - lo) * 256]);
+                const int64_t red = csd_double_red128(mt.m[j], sum);
+                v = c0 == 0 ? red : (int64_t)((uint64_t)v + (uint64_t)red);
+            }
+            v = (int64_t)((uint64_t)v + (uint64_t)csd_mont_mul(mt.m[j], q, a.dmp[(int64_t)i * r + j]));
+            o[i * a.sd[0] + j * a.sd[1]] = csd_to_mont64(mt.m[j], v);
+        }
+    }
+}
+
+void csd_gadget_bconv_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, int64_t d, const int64_t* dopi,
+                             const int64_t* mat, const int64_t* dmp, const std::vector<size_t>& bounds, size_t acc,
+                             size_t shift, const csd_modulus* mods) {
+    GbArgs a;
+    memset(&a, 0, sizeof(a));
+    a.B = src.shape[0];
+    a.d = d;
+    a.r = src.shape[1];
+    a.n3 = src.shape[3];
+    a.P = src.shape[2] * src.shape[3];
+    for (int q = 0; q < 4; q++) {
+        a.ss[q] = src.strides[q];
+        a.sd[q] = dst.strides[q];
+    }
+    a.src = (const int64_t*)src.ptr;
+    a.dst = (int64_t*)dst.ptr;
+    a.dopi = dopi;
+    a.mat = mat;
+    a.dmp = dmp;
+    a.acc = (int64_t)std::min<size_t>(acc, (size_t)a.r);
+    a.shift = (int)shift;
+    int64_t max_len = 0;
+    for (int64_t i = 0; i < d; i++) {
+        a.lo[i] = (int32_t)bounds[2 * i];
+        a.hi[i] = (int32_t)bounds[2 * i + 1];
+        max_len = std::max<int64_t>(max_len, a.hi[i] - a.lo[i]);
+    }
+    if (a.B * a.P == 0) return;
+    const int64_t blocks = a.B * ((a.P + 255) / 256);
+    if (blocks > (int64_t)INT32_MAX) throw CsdError("gadget bconv: grid too large");
+    const size_t lds = (size_t)max_len * 256 * sizeof(int64_t);  // at most 128 KiB (64 limbs)
+    static bool attr = false;
+    if (!attr) {
+        HIPCHECK(hipFuncSetAttribute((const void*)k_gadget_bconv, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    ModTable t = make_mod_table(mods, (size_t)a.r, a.r);
+    TimedScope ts(ctx, "gadget_bconv", 8.0 * (double)a.B * a.P * a.r * (1.0 + (double)d));
+    hipLaunchKernelGGL(k_gadget_bconv, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+}

@@ -79,6 +79,11 @@ void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t
                                  const int64_t* c0, const int64_t* key, const uint32_t* perm2, const uint32_t* perm3,
                                  int64_t* out, const csd_modulus* mods, int full0, int full1);
 
+// GadgetBConvCache::bconv as one kernel over B source elements (see csd_fused.hip)
+void csd_gadget_bconv_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, int64_t d, const int64_t* dopi,
+                             const int64_t* mat, const int64_t* dmp, const std::vector<size_t>& bounds, size_t acc,
+                             size_t shift, const csd_modulus* mods);
+
 // launch shape of the work-list NTT kernels (0 = the built-in choice)
 struct CsdSegTune {
     int rows = 0;          // dim-3: lines per workgroup

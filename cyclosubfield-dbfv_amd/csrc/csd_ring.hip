@@ -461,6 +461,7 @@ struct GadgetCache {  // ring.rs:1714-1727
 };
 
 static bool g_overlap_halves = true;  // see csd_ring_set_overlap
+static bool g_gadget_fused = true;    // see csd_ring_set_gadget_form
 
 struct csd_ring {
     csd_ctx* ctx;
@@ -794,7 +795,13 @@ struct csd_ring {
         auto c = std::make_unique<GadgetCache>();
         c->log2_qtilde = std::max<size_t>(64, bits);
         u128 qtilde = (u128)1 << c->log2_qtilde;
-        c->acc = (size_t)((((u128)1 << 127) - 1) / ((u128)max_p * (u128)max_p));
+        // the overflow guards of ring.rs:1741-1759
+        u128 coeff = (qtilde - 1) / (u128)min_p + 1;
+        if (coeff > (u128)INT64_MAX) throw CsdError("coefficients of correction vector don't fit into an i64");
+        if ((u128)max_len * (u128)max_p > (((u128)1 << 127) - 1) / coeff) throw CsdError("correction vector inner product overflows");
+        c->acc = (size_t)((((u128)1 << 127) - 1) / ((u128)max_p * (u128)max_p));  // truncated like `as usize`
+        if (c->acc < 16) throw CsdError("to avoid overflow, modular reductions have to be performed too often to be efficient");
+        if ((u128)((r + c->acc - 1) / c->acc) * (u128)max_p > (u128)INT64_MAX) throw CsdError("overflow during accumulation");
         std::vector<int64_t> dopi(r, 0), mat(d * (r + 1) * r, 0), dmp;
         for (size_t i = 0; i < d; i++) {
             std::vector<size_t> dig(base.begin() + bounds[2 * i], base.begin() + bounds[2 * i + 1]);
@@ -824,37 +831,48 @@ struct csd_ring {
         return ref;
     }
 
-    // GadgetBConvCache::bconv (ring.rs:1809-1851); destroys src
+    // GadgetBConvCache::bconv (ring.rs:1809-1851) for B source elements at once: src [B, r, ., .],
+    // dst [B d, r, ., .] with element b, digit i at row b d + i (the reference takes B = 1). The
+    // fused form (default) is one kernel and leaves src as it is; the HAL sequence destroys src.
     void bconv_gadget(const csd_view& dst, const csd_view& src, const std::vector<size_t>& base, const std::vector<size_t>& bounds) {
         GadgetCache& c = gadget_cache(base, bounds);
         int64_t r = (int64_t)base.size(), d = (int64_t)bounds.size() / 2;
-        int64_t n2 = src.shape[2], n3 = src.shape[3];
-        if (src.shape[0] != 1 || src.shape[1] != r) throw CsdError("gadget bconv: source shape");
-        if (dst.shape[0] != d || dst.shape[1] != r || dst.shape[2] != n2 || dst.shape[3] != n3)
+        int64_t B = src.shape[0], n2 = src.shape[2], n3 = src.shape[3];
+        if (B < 1 || src.shape[1] != r) throw CsdError("gadget bconv: source shape");
+        if (dst.shape[0] != B * d || dst.shape[1] != r || dst.shape[2] != n2 || dst.shape[3] != n3)
             throw CsdError("gadget bconv: destination shape");
         auto m = mods_for(base);
+        if (g_gadget_fused) {
+            csd_gadget_bconv_launch(ctx, dst, src, d, (const int64_t*)c.d_over_p_inv->p, (const int64_t*)c.matrix->p,
+                                    (const int64_t*)c.d_mod_p->p, bounds, c.acc,
+                                    c.log2_qtilde, m.data());
+            return;
+        }
         csd_view dopi = make_view(c.d_over_p_inv->p, 1, r, 1, 1);
-        ok(csd_hal_mul_mont64(ctx, &src, &dopi, m.data(), r));
         DevBuf corr(ctx, (size_t)d * n2 * n3 * 16), tmp(ctx, (size_t)d * (r + 1) * n2 * n3 * 16);
         csd_view vcorr = make_view(corr.p, d, 1, n2, n3), vtmp = make_view(tmp.p, d, r + 1, n2, n3);
         csd_view mat = make_view(c.matrix->p, d, r + 1, r, 1);
-        int chunk = 0;
-        for (int64_t u = 0; u < r; u += (int64_t)c.acc, chunk++) {
-            int64_t e = std::min<int64_t>(u + (int64_t)c.acc, r);
-            csd_view ms = sub(mat, 2, u, e, 8), ss = sub(src, 1, u, e, 8);
-            ok(csd_hal_matmul_i64(ctx, &vtmp, &ms, &ss));
-            csd_view tv = sub(vtmp, 1, 0, r, 16), tc = sub(vtmp, 1, r, r + 1, 16);
-            if (chunk == 0) {
-                ok(csd_hal_red_mont64_double(ctx, &dst, &tv, m.data(), r));
-                ok(csd_hal_assign_i128(ctx, &vcorr, &tc));
-            } else {
-                ok(csd_hal_add_red_mont64_double(ctx, &dst, &tv, m.data(), r));
-                ok(csd_hal_add_i128(ctx, &vcorr, &tc));
-            }
-        }
         csd_view dm = make_view(c.d_mod_p->p, d, r, 1, 1);
-        ok(csd_hal_bconv_correct_mont64(ctx, &dst, &vcorr, &dm, m.data(), r, c.log2_qtilde));
-        ok(csd_hal_i64_to_mont64(ctx, &dst, m.data(), r));
+        for (int64_t b = 0; b < B; b++) {
+            csd_view sb = sub(src, 0, b, b + 1, 8), db = sub(dst, 0, b * d, (b + 1) * d, 8);
+            ok(csd_hal_mul_mont64(ctx, &sb, &dopi, m.data(), r));
+            int chunk = 0;
+            for (int64_t u = 0; u < r; u += (int64_t)c.acc, chunk++) {
+                int64_t e = std::min<int64_t>(u + (int64_t)c.acc, r);
+                csd_view ms = sub(mat, 2, u, e, 8), ss = sub(sb, 1, u, e, 8);
+                ok(csd_hal_matmul_i64(ctx, &vtmp, &ms, &ss));
+                csd_view tv = sub(vtmp, 1, 0, r, 16), tc = sub(vtmp, 1, r, r + 1, 16);
+                if (chunk == 0) {
+                    ok(csd_hal_red_mont64_double(ctx, &db, &tv, m.data(), r));
+                    ok(csd_hal_assign_i128(ctx, &vcorr, &tc));
+                } else {
+                    ok(csd_hal_add_red_mont64_double(ctx, &db, &tv, m.data(), r));
+                    ok(csd_hal_add_i128(ctx, &vcorr, &tc));
+                }
+            }
+            ok(csd_hal_bconv_correct_mont64(ctx, &db, &vcorr, &dm, m.data(), r, c.log2_qtilde));
+            ok(csd_hal_i64_to_mont64(ctx, &db, m.data(), r));
+        }
     }
 
     // inverse_scalar_tensor (ring.rs:1165-1177) for the scalar prod(primes at `of`)
@@ -1284,6 +1302,12 @@ extern "C" int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, siz
 extern "C" int csd_ring_set_overlap(int on) {
     g_overlap_halves = on != 0;
     return 0;
+}
+
+extern "C" int csd_ring_set_gadget_form(int form) {
+    CSD_TRY if (form != 0 && form != 1) throw CsdError("gadget bconv form must be 0 (HAL sequence) or 1 (fused)");
+    g_gadget_fused = form == 1;
+    CSD_CATCH
 }
 
 extern "C" int csd_ring_galois_keyswitch_streamed(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t digits,

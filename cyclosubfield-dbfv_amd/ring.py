@@ -56,6 +56,12 @@ def set_overlap(on):
     check(lib().csd_ring_set_overlap(ctypes.c_int(1 if on else 0)))
 
 
+def set_gadget_form(form):
+    """How bconv_gadget_tensors runs (csd_ring_set_gadget_form, process-wide): 1 (default) one
+    fused kernel, 0 the reference's HAL call sequence per element; residues are identical."""
+    check(lib().csd_ring_set_gadget_form(ctypes.c_int(int(form))))
+
+
 class RingContext:
     def __init__(self, backend, field, log2_q=None, t_bits=0, raw_primes=None):
         self.backend = backend
@@ -118,6 +124,8 @@ class RingContext:
         check(lib().csd_ring_bconv(self.handle, ctypes.byref(dst.view()), db, dn, ctypes.byref(src.view()), sb, sn))
 
     def bconv_gadget_tensors(self, dst, src, base, digits):
+        """bconv_gadget_tensors (ring.rs:1548-1557) for src [B, r, ., .] and dst [B d, r, ., .]
+        (element b, digit i at row b d + i); the reference takes B = 1."""
         b, n = _sizes(base)
         bounds = [x for d in digits for x in d]
         bb, _ = _sizes(bounds)

@@ -64,9 +64,15 @@ int csd_ring_set_overlap(int on);
 int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t* dst_base, size_t r_dst, const csd_view* src,
                    const size_t* src_base, size_t r_src);
 /* bconv_gadget_tensors (ring.rs:1548-1557, GadgetBConvCache::bconv 1809-1851); digits are
- * half-open position ranges [digit_bounds[2i], digit_bounds[2i+1]); destroys src */
+ * half-open position ranges [digit_bounds[2i], digit_bounds[2i+1]). src is [B, r, ., .] and dst
+ * [B ndigits, r, ., .], element b's digit i in row b ndigits + i; the reference takes B = 1, and
+ * each element gets the residues of its own call. The default form is one kernel that leaves src
+ * unchanged; the HAL-sequence form destroys src, like the reference. */
 int csd_ring_bconv_gadget(csd_ring* ring, const csd_view* dst, const csd_view* src, const size_t* base, size_t r,
                           const size_t* digit_bounds, size_t ndigits);
+/* 1 (default): csd_ring_bconv_gadget runs as one fused kernel; 0: as the reference's HAL call
+ * sequence (ring.rs:1809-1851), per element. Residues are identical (no reference counterpart). */
+int csd_ring_set_gadget_form(int form);
 /* mod_switch_down (bfv.rs:628-662): dst [count, new_len, ., .] from src [count, r, ., .] (period
  * basis); src is left unchanged */
 int csd_ring_mod_switch_down(csd_ring* ring, const csd_view* dst, const csd_view* src, const size_t* base, size_t r,

@@ -698,8 +698,11 @@ class RingContext:
         log2_qtilde = max(64, _log2_ceil(4 * max_len * max_p + 1))
         qtilde = 1 << log2_qtilde
         correct_max = (qtilde - 1) // min_p + 1
-        assert correct_max <= (1 << 63) - 1
-        acc_before_reduce = ((1 << 127) - 1) // (max_p * max_p)
+        assert correct_max <= (1 << 63) - 1, "coefficients of correction vector don't fit into an i64"
+        assert max_len * max_p * correct_max < 1 << 127, "correction vector inner product overflows"
+        acc_before_reduce = (((1 << 127) - 1) // (max_p * max_p)) % (1 << 64)  # `as usize` truncates
+        assert acc_before_reduce >= 16, "modular reductions would have to be performed too often"
+        assert -(-r // acc_before_reduce) * max_p <= (1 << 63) - 1, "overflow during accumulation"
         dmods = [self.modulus_of(base[a:b]) for a, b in digits]
         d_over_p_inv = [0] * r
         for (a, b), dm in zip(digits, dmods):

@@ -149,17 +149,56 @@ def test_bconv_parity(gpu, cpu, rings):
         assert same(from_dev(gpu, dg), dc)
 
 
-def test_gadget_bconv_parity(gpu, cpu, rings):
-    g, c = rings("field_36864", (960, 980), 76)
+GADGET_CASES = [("field_36864", (960, 980), 76, "select9", 1), ("field_36864", (960, 980), 76, "gaps", 3),
+                ("field_192", (200, 220), 8, "whole", 2), ("field_18", (100, 120), 4, "single", 4)]
+
+
+@pytest.mark.parametrize("form", [1, 0])
+@pytest.mark.parametrize("field,log2_q,t_bits,layout,count", GADGET_CASES)
+def test_gadget_bconv_parity(gpu, cpu, rings, field, log2_q, t_bits, layout, count, form):
+    """GadgetBConvCache::bconv (ring.rs:1809-1851) on `count` elements at once, against the
+    oracle's call per element: the fused kernel (form 1, which leaves its source unchanged) and the
+    HAL sequence (form 0); digit layouts of select_digits, with limbs in no digit, one digit over
+    the whole base, and one-limb digits"""
+    pkg = load()
+    g, c = rings(field, log2_q, t_bits)
     base = g.initial_rns_base()
-    digits = R.select_digits(9, len(base))
-    rng = np.random.default_rng(8)
-    src = uniform_residues(rng, [g.primes[i] for i in base], (1, len(base), 12, 3072))
-    dc = np.zeros((len(digits), len(base), 12, 3072), dtype=np.int64)
-    c.bconv_gadget_tensors(dc, src.copy(), base, digits)
+    r = len(base)
+    digits = {"select9": lambda: R.select_digits(9, r), "gaps": lambda: [(0, 3), (5, 9), (12, r)],
+              "whole": lambda: [(0, r)], "single": lambda: [(u, u + 1) for u in range(r)]}[layout]()
+    d = len(digits)
+    n2, n3 = g.info["deg_lprime"], g.info["deg_k"]
+    rng = np.random.default_rng(8 + count)
+    src = uniform_residues(rng, [g.primes[i] for i in base], (count, r, n2, n3))
+    src[0, :, 0, :8] = 0  # zero and small values too
+    src[-1, :, -1, -3:] = 1
+    dc = np.zeros((count * d, r, n2, n3), dtype=np.int64)
+    for b in range(count):
+        c.bconv_gadget_tensors(dc[b * d:(b + 1) * d], src[b:b + 1].copy(), base, digits)
+    sg = to_dev(gpu, src)
     dg = gpu.hal_allocate_i64(dc.shape)
-    g.bconv_gadget_tensors(dg, to_dev(gpu, src), base, digits)
+    pkg.ring.set_gadget_form(form)
+    try:
+        g.bconv_gadget_tensors(dg, sg, base, digits)
+    finally:
+        pkg.ring.set_gadget_form(1)
     assert same(from_dev(gpu, dg), dc)
+    if form == 1:
+        assert same(from_dev(gpu, sg), src)
+
+
+def test_gadget_bconv_errors(gpu, rings):
+    pkg = load()
+    g, _ = rings("field_18", (100, 120), 4)
+    base = g.initial_rns_base()
+    r = len(base)
+    n2, n3 = g.info["deg_lprime"], g.info["deg_k"]
+    src = gpu.hal_allocate_i64((2, r, n2, n3))
+    for shape in [(3, r, n2, n3), (4, r - 1, n2, n3), (4, r, n2, n3 - 1)]:  # rows must be count x digits
+        with pytest.raises(pkg.HalError):
+            g.bconv_gadget_tensors(gpu.hal_allocate_i64(shape), src, base, [(0, 1), (1, r)])
+    with pytest.raises(pkg.HalError):
+        pkg.ring.set_gadget_form(2)
 
 
 @pytest.mark.parametrize("r,new_len", [(17, 16), (17, 3), (9, 8), (3, 1)])
