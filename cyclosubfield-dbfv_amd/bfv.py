@@ -33,8 +33,8 @@ def lift_to_extended(ring, pb, base, crt=None):
     base_len = len(base)
     ext = ring.extension_rns_base_for(base)
     lifted = list(base) + ext
-    result = ring.allocate(pb.shape[0], lifted)
-    tmp = ring.allocate(pb.shape[0], base)
+    result = ring.allocate_uninit(pb.shape[0], lifted)  # every part is written below
+    tmp = ring.allocate_uninit(pb.shape[0], base)
     be.hal_assign(tmp, pb)  # bconv destroys its input
     ring.bconv_tensors(result[:, base_len:], ext, tmp, base)
     if crt is not None:
@@ -53,11 +53,11 @@ def rescale_to_base(ring, x, lifted, base_len, t):
     count = x.shape[0]
     base, ext = lifted[:base_len], lifted[base_len:]
     be.hal_mul_mont64(x, ring.scalar_tensor(lifted, t), ring.mods_for(lifted))   # mul_scalar
-    tmp = ring.allocate(count, ext)
+    tmp = ring.allocate_uninit(count, ext)
     ring.bconv_tensors(tmp, ext, x[:, :base_len], base)
     be.hal_sub_i64(x[:, base_len:], tmp)
     be.hal_mul_mont64(x[:, base_len:], ring.inverse_product_tensor(ext, base), ring.mods_for(ext))
-    result = ring.allocate(count, base)
+    result = ring.allocate_uninit(count, base)
     ring.bconv_tensors(result, base, x[:, base_len:], ext)
     return result
 
@@ -150,7 +150,7 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     r = len(base)
 
     def lift(ct):
-        ct_pb = ring.allocate(2 * B, base)
+        ct_pb = ring.allocate_uninit(2 * B, base)
         be.hal_assign(ct_pb, ct)
         ring.crt_to_pb_tensor(ct_pb, base)
         return lift_to_extended(ring, ct_pb, base, crt=ct)
@@ -158,7 +158,7 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     lhs_l, lifted = lift(lhs)
     rhs_l, _ = lift(rhs)
     # hom_mul_raw_acc (bfv.rs:667-683) with the wide rows laid out [3][B]: row k B + b
-    acc = ring.allocate_wide(3 * B, lifted)
+    acc = ring.allocate_wide(3 * B, lifted, zero=False)  # every row is written fresh first
     l0, l1, r0, r1 = _rows(lhs_l, 0, 2), _rows(lhs_l, 1, 2), _rows(rhs_l, 0, 2), _rows(rhs_l, 1, 2)
     acc.fma(range(0, B), l0, 1, r0, 1)
     acc.fma(range(B, 2 * B), l0, 1, r1, 1)
@@ -170,10 +170,10 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     # relin (bfv.rs:699-741): gadget digits of c2 (row 2B + b) per pair, then all at once
     dg = clip_digits(digits, r)
     d = len(dg)
-    parts = ring.allocate(B * d, base)  # pair b, digit i at row b d + i
+    parts = ring.allocate_uninit(B * d, base)  # pair b, digit i at row b d + i
     ring.bconv_gadget_tensors(parts, rescaled[2 * B:3 * B], base, dg)  # may destroy rows 2B.. (unused below)
     ring.pb_to_crt_tensor(parts, base)
-    acc2 = ring.allocate_wide(2 * B, base)  # row s B + b
+    acc2 = ring.allocate_wide(2 * B, base, zero=False)  # row s B + b
     if d:
         for s in (0, 1):
             be.hal_inner_product_i64(acc2.data[s * B:(s + 1) * B], [_rows(parts, i, d) for i in range(d)],
@@ -182,7 +182,7 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     ring.pb_to_crt_tensor(rescaled[0:2 * B], base)
     acc2.add(range(0, 2 * B), rescaled[0:2 * B], 1)
     out = ring.reduce_wide(acc2)
-    result = ring.allocate(2 * B, base)
+    result = ring.allocate_uninit(2 * B, base)
     be.hal_assign(_rows(result, 0, 2), out[0:B])
     be.hal_assign(_rows(result, 1, 2), out[B:2 * B])
     return result

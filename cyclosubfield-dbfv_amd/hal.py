@@ -283,6 +283,11 @@ class HipBackend:
         """an i64 tensor without the zero fill, for outputs that the next call writes entirely"""
         return self._alloc(shape, 8, zero=False)
 
+    def allocate_i128_uninit(self, shape):
+        """an i128 tensor without the zero fill, for accumulators whose rows are written before
+        they are read"""
+        return self._alloc(shape, 16, zero=False)
+
     def hal_allocate_i64(self, shape):
         return self._alloc(shape, 8)
 

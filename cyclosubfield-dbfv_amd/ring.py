@@ -94,6 +94,10 @@ class RingContext:
     def allocate(self, count, base):
         return self.backend.hal_allocate_i64([count, len(base), self.info["deg_lprime"], self.info["deg_k"]])
 
+    def allocate_uninit(self, count, base):
+        """`allocate` without the zero fill, for an output that the next call writes entirely"""
+        return self.backend.allocate_i64_uninit([count, len(base), self.info["deg_lprime"], self.info["deg_k"]])
+
     def precompute_rns_factor_transforms(self):
         check(lib().csd_ring_precompute_transforms(self.handle))
 
@@ -269,17 +273,21 @@ class RingContext:
             hit = self._consts[key] = [self.backend.hal_create_modulus(self.primes[i]) for i in base]
         return hit
 
-    def allocate_wide(self, count, base):
-        return WideRingEls(self, count, base)
+    def allocate_wide(self, count, base, zero=True):
+        """WideRingEls over `base`; with zero=False the i128 rows start unwritten, and reducing a
+        row that no operation has written since raises (a zeroed row would reduce to 0)"""
+        return WideRingEls(self, count, base, zero)
 
     def reduce_wide(self, wide):
         """reduce_wide (ring.rs:846-865): one reduction kind per run of rows."""
-        out = self.allocate(wide.count(), wide.rns_base)
+        out = self.allocate_uninit(wide.count(), wide.rns_base)
         self.reduce_wide_into(out, wide)
         return out
 
     def reduce_wide_into(self, dst, wide):
         """reduce_wide_into (ring.rs:829-838): the reduced rows written into `dst`"""
+        if not wide.zeroed and any(b == 0 for b in wide.bound):
+            raise ValueError("reduce_wide: an unzeroed wide row was never written")
         start = 0
         while start < wide.count():
             full = wide.needs_full_reduction(start)
@@ -294,10 +302,12 @@ class WideRingEls:
     """WideRingEls (ring.rs:330-532): i128 accumulators in the doubly-Montgomery domain with the
     reference's per-row bound tracking; fresh rows are overwritten (ring.rs:373-413)."""
 
-    def __init__(self, ring, count, base):
+    def __init__(self, ring, count, base, zero=True):
         self.ring = ring
         self.rns_base = list(base)
-        self.data = ring.backend.hal_allocate_i128([count, len(base), ring.info["deg_lprime"], ring.info["deg_k"]])
+        shape = [count, len(base), ring.info["deg_lprime"], ring.info["deg_k"]]
+        self.data = ring.backend.hal_allocate_i128(shape) if zero else ring.backend.allocate_i128_uninit(shape)
+        self.zeroed = zero
         self._one = None
         self._neg_one = None
         self.max_p = max(ring.primes[i] for i in base)

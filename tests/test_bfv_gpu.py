@@ -116,3 +116,18 @@ def test_hom_mul_batch_matches_per_pair(gpu, rings, field, log2_q, t_bits, ndigi
         one = pkg.bfv.bfv_hom_mul(g, t, to_dev(gpu, lhs[2 * b:2 * b + 2]), to_dev(gpu, rhs[2 * b:2 * b + 2]), base, keys,
                                   digits)
         assert same(got[2 * b:2 * b + 2], from_dev(gpu, one)), b
+
+
+def test_unzeroed_wide_rows_must_be_written(gpu, rings):
+    """an accumulator allocated without the zero fill refuses to reduce rows never written"""
+    g, _ = rings("field_18", (100, 120), 4)
+    base = g.initial_rns_base()
+    acc = g.allocate_wide(2, base, zero=False)
+    x = to_dev(gpu, uniform_residues(np.random.default_rng(4), [g.primes[i] for i in base],
+                                     (1, len(base), g.info["deg_lprime"], g.info["deg_k"])))
+    acc.add(range(0, 1), x, 1)
+    with pytest.raises(ValueError):
+        g.reduce_wide(acc)
+    acc.add(range(1, 2), x, 1)
+    out = from_dev(gpu, g.reduce_wide(acc))
+    assert same(out[0], out[1])
