@@ -142,12 +142,11 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     """bfv_hom_mul (bfv.rs:747-775) for B ciphertext pairs at once: lhs, rhs are [2B, r, ., .]
     CRT-basis tensors with pair b in rows 2b, 2b + 1; the result has the same layout. Every HAL
     call covers all B pairs, the gadget base conversion (one element in bfv.rs:417-421) included.
-    Each element sees the same calls with the same operands,
-    in the same order, as in B calls of bfv_hom_mul, and the wide accumulators track the same
-    per-row bounds, so the residues are identical."""
+    Each element sees the same calls with the same operands, in the same order, as in B calls of
+    bfv_hom_mul, and the wide accumulators track the same per-row bounds, so the residues are
+    identical."""
     be = ring.backend
     B = lhs.shape[0] // 2
-    r = len(base)
 
     def lift(ct):
         ct_pb = ring.allocate_uninit(2 * B, base)
@@ -167,22 +166,46 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     prod = ring.reduce_wide(acc)
     ring.crt_to_pb_tensor(prod, lifted)
     rescaled = rescale_to_base(ring, prod, lifted, len(base), t)  # [3][B] over base
-    # relin (bfv.rs:699-741): gadget digits of c2 (row 2B + b) per pair, then all at once
+    out, _ = relin_batch(ring, rescaled, B, base, keys, digits, "crt")
+    return out
+
+
+def relin_batch(ring, prod, K, base, keys, digits, output_basis, bound=1):
+    """relin (bfv.rs:699-741) of K products at once: `prod` is a [3K, r, ., .] period-basis
+    tensor (destroyed) with component c of product k in row c K + k, of bound `bound`. Returns
+    (ciphertexts [2K, r, ., .] with product k in rows 2k, 2k + 1, bound): CRT basis (bound 1), or
+    period basis (reduced key-switch part plus c0, c1: bound + 1). One fused gadget conversion
+    covers all K; each product sees the calls and operands of its own relin, so the residues
+    are identical."""
+    be = ring.backend
+    r = len(base)
     dg = clip_digits(digits, r)
     d = len(dg)
-    parts = ring.allocate_uninit(B * d, base)  # pair b, digit i at row b d + i
-    ring.bconv_gadget_tensors(parts, rescaled[2 * B:3 * B], base, dg)  # may destroy rows 2B.. (unused below)
+    if d == 0:
+        raise ValueError("relin: the key has no digits")
+    parts = ring.allocate_uninit(K * d, base)  # product k, digit i at row k d + i
+    ring.bconv_gadget_tensors(parts, prod[2 * K:3 * K], base, dg)  # may destroy rows 2K.. (unused below)
     ring.pb_to_crt_tensor(parts, base)
-    acc2 = ring.allocate_wide(2 * B, base, zero=False)  # row s B + b
-    if d:
-        for s in (0, 1):
-            be.hal_inner_product_i64(acc2.data[s * B:(s + 1) * B], [_rows(parts, i, d) for i in range(d)],
-                                     [keys[i][s:s + 1, :r] for i in range(d)])
-        acc2.record_step(range(0, 2 * B), d * acc2.max_p * acc2.max_p)
-    ring.pb_to_crt_tensor(rescaled[0:2 * B], base)
-    acc2.add(range(0, 2 * B), rescaled[0:2 * B], 1)
-    out = ring.reduce_wide(acc2)
-    result = ring.allocate_uninit(2 * B, base)
-    be.hal_assign(_rows(result, 0, 2), out[0:B])
-    be.hal_assign(_rows(result, 1, 2), out[B:2 * B])
-    return result
+    acc = ring.allocate_wide(2 * K, base, zero=False)  # row s K + k
+    for s in (0, 1):  # key_switch_acc (bfv.rs:498-528)
+        be.hal_inner_product_i64(acc.data[s * K:(s + 1) * K], [_rows(parts, i, d) for i in range(d)],
+                                 [keys[i][s:s + 1, :r] for i in range(d)])
+    acc.record_step(range(0, 2 * K), d * acc.max_p * acc.max_p)
+    if output_basis == "crt":
+        if bound > 2:
+            raise ValueError("basis transforms require representatives bounded by 2p")
+        ring.pb_to_crt_tensor(prod[0:2 * K], base)
+        acc.add(range(0, 2 * K), prod[0:2 * K], 1)
+        out = ring.reduce_wide(acc)
+        out_bound = 1
+    elif output_basis == "pb":
+        out = ring.reduce_wide(acc)
+        ring.crt_to_pb_tensor(out, base)
+        be.hal_add_i64(out, prod[0:2 * K])  # add_els (ring.rs:990-1023)
+        out_bound = 1 + bound
+    else:
+        raise ValueError("output_basis must be 'crt' or 'pb'")
+    result = ring.allocate_uninit(2 * K, base)
+    be.hal_assign(_rows(result, 0, 2), out[0:K])
+    be.hal_assign(_rows(result, 1, 2), out[K:2 * K])
+    return result, out_bound

@@ -273,3 +273,76 @@ def dbfv_reduce_batch(ring, modulus, cts, count, ct_bound, base, fused=True):
     if modulus.digits() == 1 or d == 0:
         return cts, ct_bound
     return _reduce_into_new(ring, modulus, cts, d, 2 * count, ct_bound, base, fused)
+
+
+def _mul_convolve(ring, modulus, lhs_l, rhs_l, lifted, base_len, keys, digits, output_basis):
+    """dbfv_mul_convolve (dbfv.rs:1078-1143): the digit products of two lifted CRT-basis dBFV
+    ciphertexts as a convolution, one inner product per output digit and component, in one wide
+    batch; one reduction, basis transform and rescale for all of them; then the relinearization
+    of every output digit. The wide rows are laid out component-major (row c d_out + k instead
+    of 3 k + c) so that the relinearizations run as one batch; each row sees the same operations
+    and bounds, so the residues are those of the reference's layout."""
+    from .bfv import relin_batch, rescale_to_base
+
+    d1, d2 = lhs_l.shape[0] // 2, rhs_l.shape[0] // 2
+    K = min(modulus.digits(), d1 + d2 - 1)
+    acc = ring.allocate_wide(3 * K, lifted, zero=False)  # every row is one inner product
+    for k in range(K):
+        t0, t1, t2 = [], [], []
+        for j in range(max(0, k - (d2 - 1)), min(k, d1 - 1) + 1):
+            l0, l1 = lhs_l[2 * j:2 * j + 1], lhs_l[2 * j + 1:2 * j + 2]
+            r0, r1 = rhs_l[2 * (k - j):2 * (k - j) + 1], rhs_l[2 * (k - j) + 1:2 * (k - j) + 2]
+            t0.append((l0, 1, r0, 1))
+            t1.append((l0, 1, r1, 1))
+            t1.append((l1, 1, r0, 1))
+            t2.append((l1, 1, r1, 1))
+        acc.inner_product(range(k, k + 1), t0)
+        acc.inner_product(range(K + k, K + k + 1), t1)
+        acc.inner_product(range(2 * K + k, 2 * K + k + 1), t2)
+    prod = ring.reduce_wide(acc)  # CRT basis
+    ring.crt_to_pb_tensor(prod, lifted)
+    rescaled = rescale_to_base(ring, prod, lifted, base_len, modulus.t())  # bound 1
+    return relin_batch(ring, rescaled, K, lifted[:base_len], keys, digits, output_basis, bound=1)
+
+
+def _reduced_operand(ring, modulus, ct, ct_bound, base, fused):
+    if ct.shape[0] % 2:
+        raise ValueError("a dBFV ciphertext has two rows per digit")
+    if ct.shape[0] // 2 > modulus.digits():
+        raise ValueError("ciphertext has more digits than the modulus")
+    red, bound = dbfv_reduce(ring, modulus, ct, ct_bound, base, fused=fused)
+    if red.shape[0] and bound > 2:
+        raise ValueError("lifting requires representatives bounded by 2p (basis transform input)")
+    return red, bound
+
+
+def dbfv_hom_mul(ring, modulus, lhs, lhs_bound, rhs, rhs_bound, base, keys, digits, output_basis="pb", fused=True):
+    """dbfv_hom_mul (dbfv.rs:1033-1056) of two period-basis dBFV ciphertexts over `base` (rows
+    2i, 2i + 1 = digit i, bounds lhs_bound, rhs_bound): both reduced (dbfv_reduce), lifted to
+    the extended base, multiplied as a digit convolution and relinearized with the key
+    (keys[i] the [2, r_key, ., .] CRT-basis key of digit i, `digits` its gadget digits).
+    Returns (ciphertext, bound); output_basis "pb" (bound 2, the reference's choice before a
+    reduction) or "crt" (bound 1). An operand without digits is returned as it is."""
+    from .bfv import lift_to_extended
+
+    lhs_red, lb = _reduced_operand(ring, modulus, lhs, lhs_bound, base, fused)
+    rhs_red, rb = _reduced_operand(ring, modulus, rhs, rhs_bound, base, fused)
+    if lhs_red.shape[0] == 0:
+        return lhs_red, lb
+    if rhs_red.shape[0] == 0:
+        return rhs_red, rb
+    lhs_l, lifted = lift_to_extended(ring, lhs_red, base)
+    rhs_l, _ = lift_to_extended(ring, rhs_red, base)
+    return _mul_convolve(ring, modulus, lhs_l, rhs_l, lifted, len(base), keys, digits, output_basis)
+
+
+def dbfv_hom_square(ring, modulus, ct, ct_bound, base, keys, digits, output_basis="pb", fused=True):
+    """dbfv_hom_square (dbfv.rs:1058-1076): dbfv_hom_mul of a ciphertext with itself, reduced
+    and lifted once."""
+    from .bfv import lift_to_extended
+
+    red, bound = _reduced_operand(ring, modulus, ct, ct_bound, base, fused)
+    if red.shape[0] == 0:
+        return red, bound
+    lifted_ct, lifted = lift_to_extended(ring, red, base)
+    return _mul_convolve(ring, modulus, lifted_ct, lifted_ct, lifted, len(base), keys, digits, output_basis)

@@ -8,6 +8,8 @@ A restatement of the dBFV digit reduction of cyclosubfield-dbfv/src/dbfv.rs over
 * dbfv.rs:760-841  — ``SplitOverflowData::new`` (the BEHZ-style constants of the overflow split);
 * dbfv.rs:873-948  — ``split_overflow``: x -> (x_lo, x_hi) with x = x_lo + (b Delta) x_hi mod Q;
 * dbfv.rs:950-1025 — ``dbfv_reduce``: the sequential carry chain over the digits.
+* dbfv.rs:1026-1143 — ``dbfv_hom_mul``, ``dbfv_hom_square``, ``dbfv_mul_convolve``: reduce,
+  lift, digit convolution in one wide batch, rescale, relinearization per output digit.
 
 Pins: ``split_overflow`` is checked against the reference's own property test
 (dbfv.rs:1155-1208: x = x_lo + b Delta x_hi mod Q, |x_lo| < 2 b Delta) at its three moduli,
@@ -16,6 +18,11 @@ including two-limb R; ``dbfv_reduce`` is checked against the carry-chain invaria
 every output digit is below 2 b Delta), which a dropped, unscaled or misplaced carry breaks
 (tests/test_oracle_dbfv.py). The reference checks dbfv_reduce only through decryption
 (dbfv.rs:1324-1362), which needs key generation and encryption that this oracle does not restate.
+
+``dbfv_hom_mul`` is a composition of the pinned ring steps (lift, inner product, wide reduction,
+rescale, relin); its composition is pinned by the property that a product with the digit
+ciphertext of a single-digit zero-key "encryption" decomposes digit-wise (tests/test_oracle_dbfv.py);
+beyond that it is parity unpinned, as the reference checks it only through decryption.
 
 Big-integer constants use Python ints, as the reference's feanor-math ZZbig does.
 """
@@ -170,3 +177,65 @@ def dbfv_reduce(ring, modulus, ct, ct_bound, base):
         row_bound = input_row_bound if carry is None else 1
         carry = split_overflow(ring, data, rows, row_bound, base)
     return result, 1
+
+
+def dbfv_mul_convolve(ring, modulus, lhs_l, rhs_l, lifted, base_len, keys, digits, output_basis):
+    """dbfv.rs:1078-1143: rows 3k..3k+3 of one wide batch hold the three components of output
+    digit k, each an inner product over the digit pairs (j, k - j); one reduction, transform and
+    rescale; then relin per output digit."""
+    from .ring import relin, rescale_to_base
+
+    d1, d2 = lhs_l.shape[0] // 2, rhs_l.shape[0] // 2
+    d_out = min(modulus.digits(), d1 + d2 - 1)
+    acc = ring.allocate_wide(3 * d_out, lifted)
+    for k in range(d_out):
+        t0, t1, t2 = [], [], []
+        for j in range(max(0, k - (d2 - 1)), min(k, d1 - 1) + 1):
+            i = k - j
+            t0.append((lhs_l[2 * j:2 * j + 1], 1, rhs_l[2 * i:2 * i + 1], 1))
+            t1.append((lhs_l[2 * j:2 * j + 1], 1, rhs_l[2 * i + 1:2 * i + 2], 1))
+            t1.append((lhs_l[2 * j + 1:2 * j + 2], 1, rhs_l[2 * i:2 * i + 1], 1))
+            t2.append((lhs_l[2 * j + 1:2 * j + 2], 1, rhs_l[2 * i + 1:2 * i + 2], 1))
+        acc.inner_product(range(3 * k, 3 * k + 1), t0)
+        acc.inner_product(range(3 * k + 1, 3 * k + 2), t1)
+        acc.inner_product(range(3 * k + 2, 3 * k + 3), t2)
+    prod = ring.reduce_wide(acc)
+    ring.crt_to_pb_tensor(prod, lifted)
+    base = lifted[:base_len]
+    rescaled = rescale_to_base(ring, prod, lifted, base_len, modulus.t())
+    result = ring.allocate(2 * d_out, base)
+    bound = 1
+    for k in range(d_out):
+        rel, bound = relin(ring, rescaled[3 * k:3 * k + 3], base, keys, digits, output_basis)
+        ring.backend.hal_assign(result[2 * k:2 * k + 2], rel)
+    return result, bound
+
+
+def dbfv_hom_mul(ring, modulus, lhs, lhs_bound, rhs, rhs_bound, base, keys, digits, output_basis="pb"):
+    """dbfv.rs:1033-1056"""
+    from .ring import lift_to_extended
+
+    assert lhs.shape[0] // 2 <= modulus.digits() and rhs.shape[0] // 2 <= modulus.digits()
+    lhs_red, lb = dbfv_reduce(ring, modulus, lhs, lhs_bound, base)
+    rhs_red, rb = dbfv_reduce(ring, modulus, rhs, rhs_bound, base)
+    if lhs_red.shape[0] == 0:
+        return lhs_red, lb
+    if rhs_red.shape[0] == 0:
+        return rhs_red, rb
+    assert lb <= 2 and rb <= 2
+    lhs_l, lifted = lift_to_extended(ring, lhs_red, base)
+    rhs_l, _ = lift_to_extended(ring, rhs_red, base)
+    return dbfv_mul_convolve(ring, modulus, lhs_l, rhs_l, lifted, len(base), keys, digits, output_basis)
+
+
+def dbfv_hom_square(ring, modulus, ct, ct_bound, base, keys, digits, output_basis="pb"):
+    """dbfv.rs:1058-1076"""
+    from .ring import lift_to_extended
+
+    assert ct.shape[0] // 2 <= modulus.digits()
+    red, bound = dbfv_reduce(ring, modulus, ct, ct_bound, base)
+    if red.shape[0] == 0:
+        return red, bound
+    assert bound <= 2
+    lifted_ct, lifted = lift_to_extended(ring, red, base)
+    return dbfv_mul_convolve(ring, modulus, lifted_ct, lifted_ct, lifted, len(base), keys, digits, output_basis)

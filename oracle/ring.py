@@ -886,6 +886,24 @@ def relin_crt(ring, prod, base, keys, digits):
     return ring.reduce_wide(acc)
 
 
+def relin(ring, prod, base, keys, digits, output_basis):
+    """relin (bfv.rs:699-741) of a [3, r, ., .] period-basis product (destroyed) of bound 1:
+    "crt" is relin_crt (bound 1); "pb" reduces the key-switch part, transforms it to period basis
+    and adds c0, c1 unreduced (add_els, ring.rs:990-1023). Returns (ciphertext, bound)."""
+    if output_basis == "crt":
+        return relin_crt(ring, prod, base, keys, digits), 1
+    assert output_basis == "pb"
+    r = len(base)
+    acc = ring.allocate_wide(2, base)
+    digits = clip_digits(digits, r)
+    parts = gadget_decompose(ring, prod[2:3], base, digits)
+    key_switch_inner_product(ring, acc, parts, [np.ascontiguousarray(k[:, :r]) for k in keys], digits)
+    switched = ring.reduce_wide(acc)
+    ring.crt_to_pb_tensor(switched, base)
+    ring.backend.hal_add_i64(switched, prod[0:2])
+    return switched, 2
+
+
 def bfv_hom_mul(ring, t, lhs, rhs, base, keys, digits):
     """bfv_hom_mul (bfv.rs:747-775) with CRT-basis output, for CRT-basis ciphertexts lhs, rhs
     ([2, r, ., .] each) over `base`."""

@@ -10,6 +10,7 @@ import pytest
 from _pkg import load
 from oracle import dbfv as OD
 from oracle import ring as R
+from oracle.ring import select_digits as OD_select
 from test_hal_gpu import from_dev, same, to_dev
 from test_ring_gpu import uniform_residues
 
@@ -186,3 +187,65 @@ def test_fused_split_out_of_place(gpu, rings):
     assert same(from_dev(gpu, out_z), from_dev(gpu, zero_rows)) and same(from_dev(gpu, ov_c), from_dev(gpu, ov_d))
     with pytest.raises(pkg.HalError):
         pkg.dbfv._split_fused(g, data, out, 1, base, x_in=to_dev(gpu, inputs(g, base, 3, 42, 1)))
+
+
+HOM_MUL_CASES = [("field_192", (200, 220), 16, 64, 4, 4, 2, "pb"), ("field_192", (200, 220), 16, 64, 2, 3, 4, "crt"),
+                 ("field_192", (260, 280), 27, 108, 4, 1, 3, "pb"), ("field_36864", (960, 980), 27, 108, 4, 4, 9, "pb")]
+
+
+@pytest.mark.parametrize("field,log2_q,log2_b,log2_t,d1,d2,ndigits,basis", HOM_MUL_CASES)
+@pytest.mark.parametrize("fused", [True, False])
+def test_dbfv_hom_mul_parity(gpu, cpu, rings, field, log2_q, log2_b, log2_t, d1, d2, ndigits, basis, fused):
+    """dbfv_hom_mul (dbfv.rs:1033-1143): reduction of both operands, lift, digit convolution,
+    rescale and the batched relinearization of every output digit, bit-exact against the
+    oracle's per-digit sequence; period- and CRT-basis outputs, uneven digit counts"""
+    pkg = load()
+    g, c = rings(field, log2_q, log2_t)
+    base = g.initial_rns_base()
+    mg, mc = moduli(log2_b, log2_t)
+    lhs = inputs(g, base, 2 * d1, 60 + d1, 2)
+    rhs = inputs(g, base, 2 * d2, 61 + d2, 1)
+    digits = OD_select(ndigits, len(base))
+    rng = np.random.default_rng(62)
+    keys = [uniform_residues(rng, [g.primes[i] for i in base], (2, len(base), g.info["deg_lprime"], g.info["deg_k"]))
+            for _ in digits]
+    oc, bc = OD.dbfv_hom_mul(c, mc, lhs.copy(), 2, rhs.copy(), 1, base, keys, digits, basis)
+    og, bg = pkg.dbfv.dbfv_hom_mul(g, mg, to_dev(gpu, lhs), 2, to_dev(gpu, rhs), 1, base,
+                                   [to_dev(gpu, k) for k in keys], digits, basis, fused=fused)
+    assert bg == bc == (2 if basis == "pb" else 1) and og.shape[0] == oc.shape[0]
+    assert same(from_dev(gpu, og), np.asarray(oc))
+
+
+def test_dbfv_hom_square_parity(gpu, cpu, rings):
+    pkg = load()
+    g, c = rings("field_192", (200, 220), 64)
+    base = g.initial_rns_base()
+    mg, mc = moduli(16, 64)
+    ct = inputs(g, base, 6, 70, 1)
+    digits = OD_select(3, len(base))
+    rng = np.random.default_rng(71)
+    keys = [uniform_residues(rng, [g.primes[i] for i in base], (2, len(base), g.info["deg_lprime"], g.info["deg_k"]))
+            for _ in digits]
+    oc, bc = OD.dbfv_hom_square(c, mc, ct.copy(), 1, base, keys, digits)
+    og, bg = pkg.dbfv.dbfv_hom_square(g, mg, to_dev(gpu, ct), 1, base, [to_dev(gpu, k) for k in keys], digits)
+    assert bg == bc == 2 and same(from_dev(gpu, og), np.asarray(oc))
+    # the square equals the product of the ciphertext with itself
+    om, _ = pkg.dbfv.dbfv_hom_mul(g, mg, to_dev(gpu, ct), 1, to_dev(gpu, ct), 1, base, [to_dev(gpu, k) for k in keys],
+                                  digits)
+    assert same(from_dev(gpu, om), from_dev(gpu, og))
+
+
+def test_dbfv_hom_mul_edges(gpu, rings):
+    pkg = load()
+    g, _ = rings("field_192", (200, 220), 64)
+    base = g.initial_rns_base()
+    mg, _ = moduli(16, 64)
+    ct = to_dev(gpu, inputs(g, base, 4, 72, 1))
+    empty = gpu.hal_allocate_i64([0, len(base), g.info["deg_lprime"], g.info["deg_k"]])
+    keys = [to_dev(gpu, inputs(g, base, 2, 73, 1))]
+    out, bound = pkg.dbfv.dbfv_hom_mul(g, mg, ct, 1, empty, 1, base, keys, [(0, len(base))])
+    assert out.shape[0] == 0 and bound == 1
+    with pytest.raises(ValueError):  # more digits than the modulus
+        pkg.dbfv.dbfv_hom_mul(g, mg, to_dev(gpu, inputs(g, base, 10, 74, 1)), 1, ct, 1, base, keys, [(0, len(base))])
+    with pytest.raises(ValueError):
+        pkg.dbfv.dbfv_hom_mul(g, mg, ct, 1, ct, 1, base, keys, [(0, len(base))], output_basis="coeff")

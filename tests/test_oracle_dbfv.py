@@ -151,3 +151,111 @@ def test_split_i128_limbs():
     for x in [0, 1, -1, (1 << 63), -(1 << 63) - 5, (1 << 125) + 12345, -(1 << 125)]:
         lo, hi = D.split_i128_limbs(x)
         assert -(1 << 63) <= lo < 1 << 63 and hi * (1 << 64) + lo == x
+
+
+def one_key(ring, base):
+    """the relinearization key of the zero secret key with key part 1: rows (0, mont(1)) in CRT
+    basis, one gadget digit over the whole base; relin then maps (c0, c1, c2) to (c0, c1 + c2)
+    mod Q"""
+    n2, n3 = ring.field.deg_lprime, ring.field.deg_k
+    key = ring.allocate(2, base)
+    for j, i in enumerate(base):
+        key[1, j] = R.montgomery_repr(1, ring.primes[i])
+    return [key], [(0, len(base))]
+
+
+def hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, out, tol=(1, 2)):
+    """The digit convolution of dbfv_hom_mul (dbfv.rs:1078-1143) under `one_key`: with T_k,c the
+    exact integer components sum_j lhs_j (x) rhs_(k-j) (c = 0: l0 r0, 1: l0 r1 + l1 r0, 2: l1 r1),
+    formed from the lifted operands in CRT basis and taken to period basis, output digit k must
+    be (round(t T_k,0 / Q), round(t T_k,1 / Q) + round(t T_k,2 / Q)) mod Q up to the rescale's
+    +-1 per term. Returns the offending (digit, component) pairs."""
+    ll, lifted = R.lift_to_extended(ring, lhs_red, base)
+    rl, _ = R.lift_to_extended(ring, rhs_red, base)
+    d1, d2 = ll.shape[0] // 2, rl.shape[0] // 2
+    K = min(modulus.digits(), d1 + d2 - 1)
+    T = ring.allocate(3 * K, lifted)
+    for j_pos, i in enumerate(lifted):
+        p = ring.primes[i]
+        rinv = pow(1 << 64, -1, p)
+        lv = [[[int(v) for v in ll[row, j_pos].reshape(-1)]] for row in range(2 * d1)]
+        rv = [[[int(v) for v in rl[row, j_pos].reshape(-1)]] for row in range(2 * d2)]
+        n = len(lv[0][0])
+        for k in range(K):
+            comps = [[0] * n for _ in range(3)]
+            for j in range(max(0, k - (d2 - 1)), min(k, d1 - 1) + 1):
+                a0, a1, b0, b1 = lv[2 * j][0], lv[2 * j + 1][0], rv[2 * (k - j)][0], rv[2 * (k - j) + 1][0]
+                for e in range(n):
+                    comps[0][e] += a0[e] * b0[e]
+                    comps[1][e] += a0[e] * b1[e] + a1[e] * b0[e]
+                    comps[2][e] += a1[e] * b1[e]
+            for c in range(3):
+                T[3 * k + c, j_pos] = np.array([x * rinv % p for x in comps[c]], dtype=np.int64).reshape(T.shape[2:])
+    ring.crt_to_pb_tensor(T, lifted)
+    X = lifts(ring, T, lifted)
+    q = ring.modulus_of(base)
+    t = modulus.t()
+    z = lifts(ring, out, base)
+    bad = []
+
+    def centred(v):
+        v %= q
+        return v - q if v > q // 2 else v
+
+    for k in range(K):
+        for c in range(2):
+            for e in range(len(z[0])):
+                want = D.rounded_div(t * X[3 * k + c][e], q)
+                if c == 1:
+                    want += D.rounded_div(t * X[3 * k + 2][e], q)
+                if abs(centred(z[2 * k + c][e] - want)) > tol[c]:
+                    bad.append((k, c))
+                    break
+    return bad
+
+
+@pytest.mark.parametrize("modulus,d1,d2,square", [(D.DBFVModulus.new(16, 4), 4, 4, False),
+                                                  (D.DBFVModulus.new(16, 4), 2, 3, False),
+                                                  (D.DBFVModulus.new(16, 3), 3, 3, True)])
+def test_dbfv_hom_mul_convolution(cpu, modulus, d1, d2, square):
+    ring = R.RingContext(cpu, R.FieldDescriptor("field_192"), (200, 220), modulus.log2_t)
+    base = ring.initial_rns_base()
+    rng = np.random.default_rng(40 + d1 + d2)
+    lhs = uniform(ring, 2 * d1, base, rng)
+    rhs = lhs if square else uniform(ring, 2 * d2, base, rng)
+    keys, digits = one_key(ring, base)
+    lhs_red, _ = D.dbfv_reduce(ring, modulus, np.array(lhs), 1, base)
+    rhs_red, _ = D.dbfv_reduce(ring, modulus, np.array(rhs), 1, base)
+    if square:
+        out, bound = D.dbfv_hom_square(ring, modulus, np.array(lhs), 1, base, keys, digits)
+    else:
+        out, bound = D.dbfv_hom_mul(ring, modulus, np.array(lhs), 1, np.array(rhs), 1, base, keys, digits)
+    K = min(modulus.digits(), lhs_red.shape[0] // 2 + rhs_red.shape[0] // 2 - 1)
+    assert bound == 2 and out.shape[0] == 2 * K
+    assert hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, out) == []
+    crt, cb = D.dbfv_hom_mul(ring, modulus, np.array(lhs), 1, np.array(rhs), 1, base, keys, digits, "crt")
+    pb = np.array(crt)
+    ring.crt_to_pb_tensor(pb, base)
+    assert cb == 1 and hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, pb) == []
+    # planted mistakes are caught: output digits swapped, components swapped, c2 not switched in
+    swapped = np.array(out)
+    swapped[0:2], swapped[2:4] = np.array(out[2:4]), np.array(out[0:2])
+    assert hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, swapped)
+    comps = np.array(out)
+    comps[0], comps[1] = np.array(out[1]), np.array(out[0])
+    assert hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, comps)
+    zero_keys = [ring.allocate(2, base)]
+    no_c2, _ = D.dbfv_hom_mul(ring, modulus, np.array(lhs), 1, np.array(rhs), 1, base, zero_keys, digits)
+    assert hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, no_c2)
+
+
+def test_dbfv_hom_mul_empty_operand(cpu):
+    """dbfv.rs:1047-1051: an operand without digits is the product"""
+    modulus = D.DBFVModulus.new(16, 4)
+    ring = R.RingContext(cpu, R.FieldDescriptor("field_192"), (200, 220), modulus.log2_t)
+    base = ring.initial_rns_base()
+    lhs = uniform(ring, 4, base, np.random.default_rng(5))
+    empty = ring.allocate(0, base)
+    keys, digits = one_key(ring, base)
+    out, bound = D.dbfv_hom_mul(ring, modulus, lhs, 1, empty, 1, base, keys, digits)
+    assert out.shape[0] == 0 and bound == 1
