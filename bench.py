@@ -510,6 +510,87 @@ def bench_dbfv(pkg, be, steps, count=16, log2_b=27, ndigits=4, seed=SEED, cpu=Tr
     return res
 
 
+def bench_dbfv_mul(pkg, be, steps, count=16, log2_b=27, ndigits=4, kdigits=9, seed=SEED, cpu=True):
+    """The dBFV multiplication (dbfv_hom_mul, dbfv.rs:1033-1143) at field_36864: `count` pairs of
+    period-basis ciphertexts of `ndigits` digits, b = 2^27, t = 2^108; both sides reduced, lifted
+    to the extended base, multiplied as a digit convolution, rescaled by t and relinearized with a
+    `kdigits`-digit key, period-basis output. One step multiplies all pairs (digit-major batch)."""
+    log2_t = log2_b * ndigits
+    ring = pkg.ring.RingContext(be, FIELD, LOG2_Q, log2_t)
+    base = ring.initial_rns_base()
+    R = len(base)
+    n2, n3 = ring.info["deg_lprime"], ring.info["deg_k"]
+    modulus = pkg.dbfv.DBFVModulus.new(log2_b, ndigits)
+    digits = pkg.bfv.select_digits(kdigits, R)
+    lhs = be.hal_allocate_i64((2 * ndigits * count, R, n2, n3))
+    rhs = be.hal_allocate_i64((2 * ndigits * count, R, n2, n3))
+    keys = be.hal_allocate_i64((2 * kdigits, R, n2, n3))
+    ring.fill_uniform(lhs, base, seed + 31)
+    ring.fill_uniform(rhs, base, seed + 32)
+    ring.fill_uniform(keys, base, seed + 33)
+    key_list = [keys[2 * i:2 * i + 2] for i in range(kdigits)]
+    ring.precompute_rns_factor_transforms()
+    stream = be.stream()
+    ev = HipEvents()
+
+    def step():
+        return pkg.dbfv.dbfv_hom_mul_batch(ring, modulus, lhs, 1, rhs, 1, count, base, key_list, digits)
+
+    out, bound = step()  # warm-up, builds the constants
+    host = be.hal_export(out).reshape(2 * ndigits, count, R, n2, n3)
+    # the last pair on its own, through the reference's HAL sequence for the reduction
+    last = count - 1
+    lhs_h = be.hal_export(lhs).reshape(2 * ndigits, count, R, n2, n3)
+    rhs_h = be.hal_export(rhs).reshape(2 * ndigits, count, R, n2, n3)
+    single, _ = pkg.dbfv.dbfv_hom_mul(ring, modulus, be_import(be, lhs_h[:, last]), 1, be_import(be, rhs_h[:, last]), 1,
+                                      base, key_list, digits, fused=False)
+    if not (be.hal_export(single).reshape(2 * ndigits, R, n2, n3) == host[:, last]).all():
+        raise SystemExit("bench: batched dbfv_hom_mul differs from the single-pair form")
+    del single, out
+    be.sync()
+    ev.start(stream)
+    for _ in range(steps):
+        step()
+    ms = ev.stop(stream) / steps
+    res = {
+        "workload": f"field_36864 dbfv_hom_mul: {count} ciphertext pairs x {ndigits} digits, b = 2^{log2_b}, "
+                    f"t = 2^{log2_t}, r = {R} primes lifted to {R + len(ring.extension_rns_base_for(base))}, "
+                    f"relinearization key of {kdigits} digits, period-basis output",
+        "value": round(count / (ms / 1000.0), 1), "unit": "dBFV multiplications/s", "ms_per_step": round(ms, 3),
+        "form": "digit-major batch: every HAL call covers all pairs; one batched relinearization; "
+                "bit-identical per pair",
+    }
+    if cpu:
+        from oracle import cpu_backend
+        from oracle import dbfv as odbfv
+        from oracle import ring as oring
+
+        cb = cpu_backend.CpuBackend()
+        orc = oring.RingContext(cb, oring.FieldDescriptor(FIELD), LOG2_Q, log2_t)
+        om = odbfv.DBFVModulus.new(log2_b, ndigits)
+        kh = be.hal_export(keys).reshape(2 * kdigits, R, n2, n3)
+        okeys = [np.ascontiguousarray(kh[2 * i:2 * i + 2]) for i in range(kdigits)]
+        l0, r0 = np.ascontiguousarray(lhs_h[:, 0]), np.ascontiguousarray(rhs_h[:, 0])
+        odbfv.dbfv_hom_mul(orc, om, l0.copy(), 1, r0.copy(), 1, base, okeys, digits)  # builds the caches
+        t0 = time.perf_counter()
+        want, _ = odbfv.dbfv_hom_mul(orc, om, l0.copy(), 1, r0.copy(), 1, base, okeys, digits)
+        dt = time.perf_counter() - t0
+        if not (host[:, 0] == np.asarray(want)).all():
+            raise SystemExit("bench: dbfv_hom_mul differs from the oracle")
+        res["cpu_baseline"] = {"value": round(1.0 / dt, 3), "unit": "dBFV multiplications/s", "cores": 1, "kind": "port",
+                               "sample": f"1 pair ({dt:.2f} s, caches built beforehand), also checked bit-exact"}
+    del lhs, rhs, keys
+    ring.close()
+    return res
+
+
+def be_import(be, host):
+    """a device tensor holding the host array `host`"""
+    t = be.hal_allocate_i64(host.shape)
+    be.hal_import(t, host)
+    return t
+
+
 MFMA_INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
 
 
@@ -641,10 +722,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64, help="ring elements in the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="hom_mul,batch,zp,galois,dbfv",
+    ap.add_argument("--workloads", default="hom_mul,batch,zp,galois,dbfv,dbfv_mul",
                     help="comma-separated extra workloads beside the headline transforms: hom_mul, batch, zp, galois, "
-                         "dbfv, "
-                         "or none")
+                         "dbfv, dbfv_mul, or none")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -768,16 +848,19 @@ def main():
     ring.close()
     del data
     workloads = {}
-    if "hom_mul" in args.workloads:
+    chosen = set(w.strip() for w in args.workloads.split(",") if w.strip())
+    if "hom_mul" in chosen:
         workloads["bfv_hom_mul"] = bench_hom_mul(pkg, be, max(1, args.steps // 5), cpu=not args.no_cpu)
-    if "batch" in args.workloads:
+    if "batch" in chosen:
         workloads["batch_uneven_chain"] = bench_batch(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
-    if "zp" in args.workloads:
+    if "zp" in chosen:
         workloads["dense_zp_mfma"] = bench_zp(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
-    if "galois" in args.workloads:
+    if "galois" in chosen:
         workloads["galois_keyswitch"] = bench_galois(pkg, be, max(1, args.steps // 5), cpu=not args.no_cpu)
-    if "dbfv" in args.workloads:
+    if "dbfv" in chosen:
         workloads["dbfv_reduce"] = bench_dbfv(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
+    if "dbfv_mul" in chosen:
+        workloads["dbfv_hom_mul"] = bench_dbfv_mul(pkg, be, max(1, args.steps // 5), cpu=not args.no_cpu)
     if workloads:
         result["workloads"] = workloads
     be.close()

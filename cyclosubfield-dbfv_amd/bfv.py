@@ -170,13 +170,13 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     return out
 
 
-def relin_batch(ring, prod, K, base, keys, digits, output_basis, bound=1):
+def relin_batch(ring, prod, K, base, keys, digits, output_basis, bound=1, interleave=True):
     """relin (bfv.rs:699-741) of K products at once: `prod` is a [3K, r, ., .] period-basis
     tensor (destroyed) with component c of product k in row c K + k, of bound `bound`. Returns
-    (ciphertexts [2K, r, ., .] with product k in rows 2k, 2k + 1, bound): CRT basis (bound 1), or
-    period basis (reduced key-switch part plus c0, c1: bound + 1). One fused gadget conversion
-    covers all K; each product sees the calls and operands of its own relin, so the residues
-    are identical."""
+    (ciphertexts [2K, r, ., .] with product k in rows 2k, 2k + 1, or with interleave=False in
+    rows k, K + k; bound): CRT basis (bound 1), or period basis (reduced key-switch part plus
+    c0, c1: bound + 1). One fused gadget conversion covers all K; each product sees the calls and
+    operands of its own relin, so the residues are identical."""
     be = ring.backend
     r = len(base)
     dg = clip_digits(digits, r)
@@ -205,6 +205,8 @@ def relin_batch(ring, prod, K, base, keys, digits, output_basis, bound=1):
         out_bound = 1 + bound
     else:
         raise ValueError("output_basis must be 'crt' or 'pb'")
+    if not interleave:
+        return out, out_bound
     result = ring.allocate_uninit(2 * K, base)
     be.hal_assign(_rows(result, 0, 2), out[0:K])
     be.hal_assign(_rows(result, 1, 2), out[K:2 * K])

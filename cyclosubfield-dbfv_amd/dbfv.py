@@ -9,7 +9,12 @@ wide accumulation per digit.
 A dBFV ciphertext is a period-basis batch whose rows 2i, 2i + 1 hold the BFV ciphertext of
 digit i (dbfv.rs:32-38). ``dbfv_reduce_batch`` runs many ciphertexts through the same chain with
 every HAL call covering all of them; its rows are digit-major, so that each digit of all
-ciphertexts is one contiguous run of rows."""
+ciphertexts is one contiguous run of rows.
+
+Above the reduction sits the dBFV multiplication (dbfv.rs:1026-1143): ``dbfv_hom_mul``,
+``dbfv_hom_square`` and ``dbfv_hom_mul_batch`` (many pairs, digit-major) reduce both sides, lift
+them to the extended base, form the digit convolution in one wide batch, rescale by t and
+relinearize every output digit in one batch (``bfv.relin_batch``)."""
 
 
 def _log2_ceil(x):
@@ -275,65 +280,89 @@ def dbfv_reduce_batch(ring, modulus, cts, count, ct_bound, base, fused=True):
     return _reduce_into_new(ring, modulus, cts, d, 2 * count, ct_bound, base, fused)
 
 
-def _mul_convolve(ring, modulus, lhs_l, rhs_l, lifted, base_len, keys, digits, output_basis):
-    """dbfv_mul_convolve (dbfv.rs:1078-1143): the digit products of two lifted CRT-basis dBFV
-    ciphertexts as a convolution, one inner product per output digit and component, in one wide
-    batch; one reduction, basis transform and rescale for all of them; then the relinearization
-    of every output digit. The wide rows are laid out component-major (row c d_out + k instead
-    of 3 k + c) so that the relinearizations run as one batch; each row sees the same operations
-    and bounds, so the residues are those of the reference's layout."""
+def _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, base_len, keys, digits, output_basis):
+    """dbfv_mul_convolve (dbfv.rs:1078-1143) of `count` pairs of lifted CRT-basis dBFV
+    ciphertexts (digit-major: row (2 i + s) count + c is row s of digit i of pair c): the digit
+    products as a convolution, one inner product per output digit and component covering all
+    pairs, in one wide batch; one reduction, basis transform and rescale for all of them; then
+    one batched relinearization of every output digit of every pair. The wide rows are laid out
+    component-major (row (c d_out + k) count + pair instead of 3 k + c) so that the
+    relinearizations run as one batch; each row sees the same operations and bounds, so the
+    residues are those of the reference's layout. The result is digit-major like the inputs."""
     from .bfv import relin_batch, rescale_to_base
 
-    d1, d2 = lhs_l.shape[0] // 2, rhs_l.shape[0] // 2
+    be = ring.backend
+    n = count
+    d1, d2 = lhs_l.shape[0] // (2 * n), rhs_l.shape[0] // (2 * n)
     K = min(modulus.digits(), d1 + d2 - 1)
-    acc = ring.allocate_wide(3 * K, lifted, zero=False)  # every row is one inner product
+
+    def blk(t, row):  # rows row n .. row n + n
+        return t[row * n:(row + 1) * n]
+
+    acc = ring.allocate_wide(3 * K * n, lifted, zero=False)  # every row is one inner product
     for k in range(K):
         t0, t1, t2 = [], [], []
         for j in range(max(0, k - (d2 - 1)), min(k, d1 - 1) + 1):
-            l0, l1 = lhs_l[2 * j:2 * j + 1], lhs_l[2 * j + 1:2 * j + 2]
-            r0, r1 = rhs_l[2 * (k - j):2 * (k - j) + 1], rhs_l[2 * (k - j) + 1:2 * (k - j) + 2]
+            l0, l1 = blk(lhs_l, 2 * j), blk(lhs_l, 2 * j + 1)
+            r0, r1 = blk(rhs_l, 2 * (k - j)), blk(rhs_l, 2 * (k - j) + 1)
             t0.append((l0, 1, r0, 1))
             t1.append((l0, 1, r1, 1))
             t1.append((l1, 1, r0, 1))
             t2.append((l1, 1, r1, 1))
-        acc.inner_product(range(k, k + 1), t0)
-        acc.inner_product(range(K + k, K + k + 1), t1)
-        acc.inner_product(range(2 * K + k, 2 * K + k + 1), t2)
+        for c, terms in enumerate((t0, t1, t2)):
+            acc.inner_product(range((c * K + k) * n, (c * K + k + 1) * n), terms)
     prod = ring.reduce_wide(acc)  # CRT basis
     ring.crt_to_pb_tensor(prod, lifted)
     rescaled = rescale_to_base(ring, prod, lifted, base_len, modulus.t())  # bound 1
-    return relin_batch(ring, rescaled, K, lifted[:base_len], keys, digits, output_basis, bound=1)
+    out, bound = relin_batch(ring, rescaled, K * n, lifted[:base_len], keys, digits, output_basis, bound=1,
+                             interleave=False)  # product (k, pair) component s at row s K n + k n + pair
+    result = ring.allocate_uninit(2 * K * n, lifted[:base_len])
+    for k in range(K):
+        for s_ in (0, 1):
+            be.hal_assign(blk(result, 2 * k + s_), out[(s_ * K + k) * n:(s_ * K + k + 1) * n])
+    return result, bound
 
 
-def _reduced_operand(ring, modulus, ct, ct_bound, base, fused):
-    if ct.shape[0] % 2:
-        raise ValueError("a dBFV ciphertext has two rows per digit")
-    if ct.shape[0] // 2 > modulus.digits():
+def _reduced_operand(ring, modulus, ct, ct_bound, count, base, fused):
+    if count < 1 or ct.shape[0] % (2 * count):
+        raise ValueError("dBFV ciphertexts need two rows per digit and pair")
+    if ct.shape[0] // (2 * count) > modulus.digits():
         raise ValueError("ciphertext has more digits than the modulus")
-    red, bound = dbfv_reduce(ring, modulus, ct, ct_bound, base, fused=fused)
+    red, bound = dbfv_reduce_batch(ring, modulus, ct, count, ct_bound, base, fused=fused)
     if red.shape[0] and bound > 2:
         raise ValueError("lifting requires representatives bounded by 2p (basis transform input)")
     return red, bound
 
 
-def dbfv_hom_mul(ring, modulus, lhs, lhs_bound, rhs, rhs_bound, base, keys, digits, output_basis="pb", fused=True):
-    """dbfv_hom_mul (dbfv.rs:1033-1056) of two period-basis dBFV ciphertexts over `base` (rows
-    2i, 2i + 1 = digit i, bounds lhs_bound, rhs_bound): both reduced (dbfv_reduce), lifted to
-    the extended base, multiplied as a digit convolution and relinearized with the key
-    (keys[i] the [2, r_key, ., .] CRT-basis key of digit i, `digits` its gadget digits).
-    Returns (ciphertext, bound); output_basis "pb" (bound 2, the reference's choice before a
-    reduction) or "crt" (bound 1). An operand without digits is returned as it is."""
+def dbfv_hom_mul_batch(ring, modulus, lhs, lhs_bound, rhs, rhs_bound, count, base, keys, digits, output_basis="pb",
+                       fused=True):
+    """dbfv_hom_mul (dbfv.rs:1033-1056) of `count` pairs of period-basis dBFV ciphertexts over
+    `base`, digit-major (row (2 i + s) count + c is row s of digit i of pair c; bounds lhs_bound,
+    rhs_bound): both sides reduced (dbfv_reduce), lifted to the extended base, multiplied as a
+    digit convolution and relinearized with the key (keys[i] the [2, r_key, ., .] CRT-basis key
+    of digit i, `digits` its gadget digits). Returns (ciphertexts, bound), digit-major;
+    output_basis "pb" (bound 2, the reference's choice before a reduction) or "crt" (bound 1).
+    Every HAL call covers all pairs, and each pair gets the residues of its own dbfv_hom_mul. A
+    side without digits is returned as it is."""
     from .bfv import lift_to_extended
 
-    lhs_red, lb = _reduced_operand(ring, modulus, lhs, lhs_bound, base, fused)
-    rhs_red, rb = _reduced_operand(ring, modulus, rhs, rhs_bound, base, fused)
+    if output_basis not in ("pb", "crt"):
+        raise ValueError("output_basis must be 'crt' or 'pb'")
+    lhs_red, lb = _reduced_operand(ring, modulus, lhs, lhs_bound, count, base, fused)
+    rhs_red, rb = _reduced_operand(ring, modulus, rhs, rhs_bound, count, base, fused)
     if lhs_red.shape[0] == 0:
         return lhs_red, lb
     if rhs_red.shape[0] == 0:
         return rhs_red, rb
     lhs_l, lifted = lift_to_extended(ring, lhs_red, base)
     rhs_l, _ = lift_to_extended(ring, rhs_red, base)
-    return _mul_convolve(ring, modulus, lhs_l, rhs_l, lifted, len(base), keys, digits, output_basis)
+    return _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, len(base), keys, digits, output_basis)
+
+
+def dbfv_hom_mul(ring, modulus, lhs, lhs_bound, rhs, rhs_bound, base, keys, digits, output_basis="pb", fused=True):
+    """dbfv_hom_mul (dbfv.rs:1033-1056) of one pair: rows 2i, 2i + 1 = digit i; see
+    dbfv_hom_mul_batch (one pair is the batch of one)."""
+    return dbfv_hom_mul_batch(ring, modulus, lhs, lhs_bound, rhs, rhs_bound, 1, base, keys, digits, output_basis, fused)
 
 
 def dbfv_hom_square(ring, modulus, ct, ct_bound, base, keys, digits, output_basis="pb", fused=True):
@@ -341,8 +370,10 @@ def dbfv_hom_square(ring, modulus, ct, ct_bound, base, keys, digits, output_basi
     and lifted once."""
     from .bfv import lift_to_extended
 
-    red, bound = _reduced_operand(ring, modulus, ct, ct_bound, base, fused)
+    if output_basis not in ("pb", "crt"):
+        raise ValueError("output_basis must be 'crt' or 'pb'")
+    red, bound = _reduced_operand(ring, modulus, ct, ct_bound, 1, base, fused)
     if red.shape[0] == 0:
         return red, bound
     lifted_ct, lifted = lift_to_extended(ring, red, base)
-    return _mul_convolve(ring, modulus, lifted_ct, lifted_ct, lifted, len(base), keys, digits, output_basis)
+    return _mul_convolve(ring, modulus, lifted_ct, lifted_ct, 1, lifted, len(base), keys, digits, output_basis)

@@ -249,3 +249,26 @@ def test_dbfv_hom_mul_edges(gpu, rings):
         pkg.dbfv.dbfv_hom_mul(g, mg, to_dev(gpu, inputs(g, base, 10, 74, 1)), 1, ct, 1, base, keys, [(0, len(base))])
     with pytest.raises(ValueError):
         pkg.dbfv.dbfv_hom_mul(g, mg, ct, 1, ct, 1, base, keys, [(0, len(base))], output_basis="coeff")
+
+
+@pytest.mark.parametrize("field,log2_q,log2_b,log2_t,d1,d2,count,basis",
+                         [("field_192", (200, 220), 16, 64, 4, 3, 5, "pb"), ("field_36864", (960, 980), 27, 108, 4, 4, 3, "crt")])
+def test_dbfv_hom_mul_batch_matches_single(gpu, rings, field, log2_q, log2_b, log2_t, d1, d2, count, basis):
+    """the digit-major batch gives every pair the residues of its own dbfv_hom_mul"""
+    pkg = load()
+    g, _ = rings(field, log2_q, log2_t)
+    base = g.initial_rns_base()
+    mg, _ = moduli(log2_b, log2_t)
+    lhs = inputs(g, base, 2 * d1 * count, 80 + count, 2)  # pair c, row k at k count + c
+    rhs = inputs(g, base, 2 * d2 * count, 81 + count, 1)
+    digits = OD_select(min(5, len(base)), len(base))
+    keys = [to_dev(gpu, inputs(g, base, 2, 82 + i, 1)) for i in range(len(digits))]
+    out, bound = pkg.dbfv.dbfv_hom_mul_batch(g, mg, to_dev(gpu, lhs), 2, to_dev(gpu, rhs), 1, count, base, keys, digits,
+                                             basis)
+    got = from_dev(gpu, out)
+    K = out.shape[0] // (2 * count)
+    assert K == min(mg.digits(), 2 * mg.digits() - 1) and bound == (2 if basis == "pb" else 1)
+    for c in range(count):
+        one, _ = pkg.dbfv.dbfv_hom_mul(g, mg, to_dev(gpu, lhs[c::count]), 2, to_dev(gpu, rhs[c::count]), 1, base, keys,
+                                       digits, basis, fused=False)
+        assert same(got[c::count], from_dev(gpu, one)), c
