@@ -102,3 +102,20 @@ CSD_HD int64_t csd_barrett(CsdBarrett b, int64_t x, int64_t n) {
     uint64_t q = (uint64_t)hi + (lo >> 63);
     return (int64_t)((uint64_t)b.z * (uint64_t)x + q * (uint64_t)n);
 }
+
+// The same product with the sign correction of u folded into the constant. The signed high word
+// is hi(u x) = hi_uu - [u < 0] x - [x < 0] u (mod 2^64), hi_uu that of the unsigned product, so
+//   z x + q N = (z - [u < 0] N) x + (hi_uu + [lo >= 2^63] - [x < 0] u) N   (mod 2^64).
+// The device tables hold z' = z - [u < 0] N (csd_barrett_fold); csd_barrett_folded returns the
+// bits csd_barrett returns for (z, u), with one select and one 64-bit add fewer.
+CSD_HD CsdBarrett csd_barrett_fold(CsdBarrett b, int64_t n) {
+    return CsdBarrett{b.u < 0 ? (int64_t)((uint64_t)b.z - (uint64_t)n) : b.z, b.u};
+}
+CSD_HD int64_t csd_barrett_folded(CsdBarrett f, int64_t x, int64_t n) {
+    const u128 p = (u128)(uint64_t)f.u * (uint64_t)x;
+    const uint64_t lo = (uint64_t)p;
+    const uint64_t q = (uint64_t)(p >> 64) + (lo >> 63) - (x < 0 ? (uint64_t)f.u : 0);
+    return (int64_t)((uint64_t)f.z * (uint64_t)x + q * (uint64_t)n);
+}
+// the folded form of the constant 1 (z = 1 has u < 0)
+CSD_HD bool csd_folded_is_one(const CsdBarrett& f, int64_t n) { return f.z == (int64_t)(1 - (uint64_t)n); }

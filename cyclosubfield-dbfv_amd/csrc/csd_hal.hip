@@ -1028,15 +1028,16 @@ __global__ __launch_bounds__(256) void k_calib_copy(const int4* __restrict__ src
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
-// `iters` dependent Barrett multiplications per element, 4 independent chains per thread
+// `iters` dependent Barrett multiplications per element (the NTT kernels' folded form, `b`
+// folded), 4 independent chains per thread
 __global__ __launch_bounds__(256) void k_calib_barrett(int64_t* data, CsdBarrett b, int64_t n, int iters) {
     size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     int64_t x0 = data[i], x1 = data[i + 1], x2 = data[i + 2], x3 = data[i + 3];
     for (int k = 0; k < iters; k++) {
-        x0 = csd_barrett(b, x0, n);
-        x1 = csd_barrett(b, x1, n);
-        x2 = csd_barrett(b, x2, n);
-        x3 = csd_barrett(b, x3, n);
+        x0 = csd_barrett_folded(b, x0, n);
+        x1 = csd_barrett_folded(b, x1, n);
+        x2 = csd_barrett_folded(b, x2, n);
+        x3 = csd_barrett_folded(b, x3, n);
     }
     data[i] = x0;
     data[i + 1] = x1;
@@ -1072,7 +1073,7 @@ extern "C" int csd_calibrate(csd_ctx* ctx, int kind, size_t n, double* out) {
         void* a = ctx->alloc_bytes(elems * 8);
         csd_zero_fill(ctx, a, elems * 8);
         const int64_t N = 144115176243904321;
-        CsdBarrett bm = csd_host_barrett(123456789123, N);
+        CsdBarrett bm = csd_barrett_fold(csd_host_barrett(123456789123, N), N);
         unsigned blocks = (unsigned)(elems / 4 / 256);
         hipLaunchKernelGGL(k_calib_barrett, dim3(blocks), dim3(256), 0, ctx->stream, (int64_t*)a, bm, N, 8);
         HIPCHECK(hipEventRecord(e0, ctx->stream));

@@ -131,6 +131,19 @@ struct DevPlan {
     // followed in memory by the Barrett twiddle table
 };
 
+// The device copy of a plan header: every Barrett constant in the folded form of
+// csd_barrett_fold, which the kernels' csd_barrett_folded expects.
+static DevPlan device_header(const DevPlan& P) {
+    DevPlan D = P;
+    D.one = csd_barrett_fold(D.one, P.n);
+    D.inv_len = csd_barrett_fold(D.inv_len, P.n);
+    for (int i = 0; i < NTT_MAX_FACTORS; i++) {
+        D.f[i].zeta = csd_barrett_fold(D.f[i].zeta, P.n);
+        D.f[i].zeta2 = csd_barrett_fold(D.f[i].zeta2, P.n);
+    }
+    return D;
+}
+
 struct csd_ntt_plan {
     csd_ctx* ctx;
     DevPlan host;
@@ -347,8 +360,11 @@ extern "C" int csd_hal_create_ntt_plan(csd_ctx* ctx, const csd_modulus* modulus,
     size_t bytes = sizeof(DevPlan) + tw.size() * sizeof(CsdBarrett);
     plan->dev = ctx->alloc_bytes(bytes);
     std::vector<char> blob(bytes);
-    memcpy(blob.data(), &P, sizeof(DevPlan));
-    if (!tw.empty()) memcpy(blob.data() + sizeof(DevPlan), tw.data(), tw.size() * sizeof(CsdBarrett));
+    const DevPlan D = device_header(P);
+    std::vector<CsdBarrett> dtw(tw.size());
+    for (size_t i = 0; i < tw.size(); i++) dtw[i] = csd_barrett_fold(tw[i], n);
+    memcpy(blob.data(), &D, sizeof(DevPlan));
+    if (!dtw.empty()) memcpy(blob.data() + sizeof(DevPlan), dtw.data(), dtw.size() * sizeof(CsdBarrett));
     HIPCHECK(hipMemcpyAsync(plan->dev, blob.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     plan->dev_tw = (const CsdBarrett*)((const char*)plan->dev + sizeof(DevPlan));
@@ -404,7 +420,8 @@ extern "C" int csd_ntt_plan_mults(const csd_ntt_plan* plan, int dir, uint64_t* o
 extern "C" int csd_ntt_plan_set_grouping(csd_ntt_plan* plan, int g2, int g3) {
     CSD_TRY if (g2 < 1 || g2 > 3 || g3 < 1 || g3 > 2) throw CsdError("grouping: 1 <= g2 <= 3 and 1 <= g3 <= 2");
     for (int dir = 0; dir < 3; dir++) build_passes(plan->host, dir, g2, g3);
-    HIPCHECK(hipMemcpyAsync(plan->dev, &plan->host, sizeof(DevPlan), hipMemcpyHostToDevice, plan->ctx->stream));
+    const DevPlan D = device_header(plan->host);
+    HIPCHECK(hipMemcpyAsync(plan->dev, &D, sizeof(DevPlan), hipMemcpyHostToDevice, plan->ctx->stream));
     HIPCHECK(hipStreamSynchronize(plan->ctx->stream));
     CSD_CATCH
 }
@@ -482,7 +499,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt(NttArgs a, ModTable mt) {
         const DevStep s = P->steps[a.dir][sidx];
         if (s.kind == STEP_REDUCE || s.kind == STEP_SCALE_ONE || s.kind == STEP_SCALE_INV) {
             const CsdBarrett m = s.kind == STEP_SCALE_INV ? P->inv_len : P->one;
-            for (int g = threadIdx.x; g < total; g += NTT_THREADS) lds[g] = csd_barrett(m, lds[g], n);
+            for (int g = threadIdx.x; g < total; g += NTT_THREADS) lds[g] = csd_barrett_folded(m, lds[g], n);
         } else {
             const DevFactor& F = P->f[s.factor];
             const int st = s.stage;
@@ -506,24 +523,24 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt(NttArgs a, ModTable mt) {
                     const int64_t x = lds[x_i], y = lds[y_i];
                     const CsdBarrett t = tw[blk];
                     if (s.kind == STEP_FWD) {
-                        const bool skip = (s.flags & 1) && t.z == 1;
-                        const int64_t ty = skip ? y : csd_barrett(t, y, n);
+                        const bool skip = (s.flags & 1) && csd_folded_is_one(t, n);
+                        const int64_t ty = skip ? y : csd_barrett_folded(t, y, n);
                         int64_t o0 = (int64_t)((uint64_t)x + (uint64_t)ty);
                         int64_t o1 = (int64_t)((uint64_t)x - (uint64_t)ty);
                         if (s.flags & 2) {
-                            o0 = csd_barrett(P->one, o0, n);
-                            o1 = csd_barrett(P->one, o1, n);
+                            o0 = csd_barrett_folded(P->one, o0, n);
+                            o1 = csd_barrett_folded(P->one, o1, n);
                         }
                         lds[x_i] = o0;
                         lds[y_i] = o1;
                     } else {
                         const bool fuse = s.flags & 1;
                         const CsdBarrett ym = fuse ? P->inv_len : t;
-                        const bool skip = !fuse && ym.z == 1;
+                        const bool skip = !fuse && csd_folded_is_one(ym, n);
                         const int64_t sum = (int64_t)((uint64_t)x + (uint64_t)y);
                         const int64_t dif = (int64_t)((uint64_t)x - (uint64_t)y);
-                        lds[x_i] = fuse ? csd_barrett(ym, sum, n) : sum;
-                        lds[y_i] = skip ? dif : csd_barrett(ym, dif, n);
+                        lds[x_i] = fuse ? csd_barrett_folded(ym, sum, n) : sum;
+                        lds[y_i] = skip ? dif : csd_barrett_folded(ym, dif, n);
                     }
                 }
             } else {
@@ -542,20 +559,20 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt(NttArgs a, ModTable mt) {
                     const int64_t x = lds[x_i], y = lds[y_i], z = lds[z_i];
                     if (s.kind == STEP_FWD) {
                         const CsdBarrett* w6 = TW + F.tw_off[st] + 6 * blk;
-                        const bool skip = (s.flags & 1) && w6[0].z == 1;
-                        const int64_t tb = skip ? y : csd_barrett(w6[0], y, n);
-                        const int64_t tbz = csd_barrett(w6[1], y, n);
-                        const int64_t tbzz = csd_barrett(w6[2], y, n);
-                        const int64_t tc = skip ? z : csd_barrett(w6[3], z, n);
-                        const int64_t tcz = csd_barrett(w6[4], z, n);
-                        const int64_t tczz = csd_barrett(w6[5], z, n);
+                        const bool skip = (s.flags & 1) && csd_folded_is_one(w6[0], n);
+                        const int64_t tb = skip ? y : csd_barrett_folded(w6[0], y, n);
+                        const int64_t tbz = csd_barrett_folded(w6[1], y, n);
+                        const int64_t tbzz = csd_barrett_folded(w6[2], y, n);
+                        const int64_t tc = skip ? z : csd_barrett_folded(w6[3], z, n);
+                        const int64_t tcz = csd_barrett_folded(w6[4], z, n);
+                        const int64_t tczz = csd_barrett_folded(w6[5], z, n);
                         int64_t o0 = (int64_t)((uint64_t)x + (uint64_t)tb + (uint64_t)tc);
                         int64_t o1 = (int64_t)((uint64_t)x + (uint64_t)tbz + (uint64_t)tczz);
                         int64_t o2 = (int64_t)((uint64_t)x + (uint64_t)tbzz + (uint64_t)tcz);
                         if (s.flags & 2) {
-                            o0 = csd_barrett(P->one, o0, n);
-                            o1 = csd_barrett(P->one, o1, n);
-                            o2 = csd_barrett(P->one, o2, n);
+                            o0 = csd_barrett_folded(P->one, o0, n);
+                            o1 = csd_barrett_folded(P->one, o1, n);
+                            o2 = csd_barrett_folded(P->one, o2, n);
                         }
                         lds[x_i] = o0;
                         lds[y_i] = o1;
@@ -565,17 +582,17 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt(NttArgs a, ModTable mt) {
                         const bool fuse = s.flags & 1;
                         const CsdBarrett ym = fuse ? P->inv_len : w2[0];
                         const CsdBarrett zm = fuse ? P->inv_len : w2[1];
-                        const bool skip = !fuse && ym.z == 1;
-                        const int64_t y1z = csd_barrett(F.zeta, y, n);
-                        const int64_t y1zz = csd_barrett(F.zeta2, y, n);
-                        const int64_t y2z = csd_barrett(F.zeta, z, n);
-                        const int64_t y2zz = csd_barrett(F.zeta2, z, n);
+                        const bool skip = !fuse && csd_folded_is_one(ym, n);
+                        const int64_t y1z = csd_barrett_folded(F.zeta, y, n);
+                        const int64_t y1zz = csd_barrett_folded(F.zeta2, y, n);
+                        const int64_t y2z = csd_barrett_folded(F.zeta, z, n);
+                        const int64_t y2zz = csd_barrett_folded(F.zeta2, z, n);
                         const int64_t sum = (int64_t)((uint64_t)x + (uint64_t)y + (uint64_t)z);
                         const int64_t bb = (int64_t)((uint64_t)x + (uint64_t)y1zz + (uint64_t)y2z);
                         const int64_t cc = (int64_t)((uint64_t)x + (uint64_t)y1z + (uint64_t)y2zz);
-                        lds[x_i] = fuse ? csd_barrett(ym, sum, n) : sum;
-                        lds[y_i] = skip ? bb : csd_barrett(ym, bb, n);
-                        lds[z_i] = skip ? cc : csd_barrett(zm, cc, n);
+                        lds[x_i] = fuse ? csd_barrett_folded(ym, sum, n) : sum;
+                        lds[y_i] = skip ? bb : csd_barrett_folded(ym, bb, n);
+                        lds[z_i] = skip ? cc : csd_barrett_folded(zm, cc, n);
                     }
                 }
             }
@@ -623,22 +640,22 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt(NttArgs a, ModTable mt) {
 template <bool LANE>
 __device__ __forceinline__ int64_t skip_or_mul(bool skip, const CsdBarrett& t, int64_t x, int64_t n) {
     if constexpr (LANE) {
-        const int64_t p = csd_barrett(t, x, n);
+        const int64_t p = csd_barrett_folded(t, x, n);
         return skip ? x : p;
     } else {
-        return skip ? x : csd_barrett(t, x, n);
+        return skip ? x : csd_barrett_folded(t, x, n);
     }
 }
 
 template <bool LANE = false>
 __device__ __forceinline__ void bf2_fwd(int64_t& x, int64_t& y, CsdBarrett t, uint8_t info, CsdBarrett one, int64_t n) {
-    const bool skip = (info & INFO_SKIP) && t.z == 1;
+    const bool skip = (info & INFO_SKIP) && csd_folded_is_one(t, n);
     const int64_t ty = skip_or_mul<LANE>(skip, t, y, n);
     int64_t o0 = (int64_t)((uint64_t)x + (uint64_t)ty);
     int64_t o1 = (int64_t)((uint64_t)x - (uint64_t)ty);
     if (info & INFO_FUSE) {
-        o0 = csd_barrett(one, o0, n);
-        o1 = csd_barrett(one, o1, n);
+        o0 = csd_barrett_folded(one, o0, n);
+        o1 = csd_barrett_folded(one, o1, n);
     }
     x = o0;
     y = o1;
@@ -648,30 +665,30 @@ template <bool LANE = false>
 __device__ __forceinline__ void bf2_inv(int64_t& x, int64_t& y, CsdBarrett t, uint8_t info, CsdBarrett inv_len, int64_t n) {
     const bool fuse = info & INFO_FUSE;
     const CsdBarrett ym = fuse ? inv_len : t;
-    const bool skip = !fuse && ym.z == 1;
+    const bool skip = !fuse && csd_folded_is_one(ym, n);
     const int64_t sum = (int64_t)((uint64_t)x + (uint64_t)y);
     const int64_t dif = (int64_t)((uint64_t)x - (uint64_t)y);
-    x = fuse ? csd_barrett(ym, sum, n) : sum;
+    x = fuse ? csd_barrett_folded(ym, sum, n) : sum;
     y = skip_or_mul<LANE>(skip, ym, dif, n);
 }
 
 template <bool LANE = false>
 __device__ __forceinline__ void bf3_fwd(int64_t& x, int64_t& y, int64_t& z, const CsdBarrett* w6, uint8_t info, CsdBarrett one,
                                         int64_t n) {
-    const bool skip = (info & INFO_SKIP) && w6[0].z == 1;
+    const bool skip = (info & INFO_SKIP) && csd_folded_is_one(w6[0], n);
     const int64_t tb = skip_or_mul<LANE>(skip, w6[0], y, n);
-    const int64_t tbz = csd_barrett(w6[1], y, n);
-    const int64_t tbzz = csd_barrett(w6[2], y, n);
+    const int64_t tbz = csd_barrett_folded(w6[1], y, n);
+    const int64_t tbzz = csd_barrett_folded(w6[2], y, n);
     const int64_t tc = skip_or_mul<LANE>(skip, w6[3], z, n);
-    const int64_t tcz = csd_barrett(w6[4], z, n);
-    const int64_t tczz = csd_barrett(w6[5], z, n);
+    const int64_t tcz = csd_barrett_folded(w6[4], z, n);
+    const int64_t tczz = csd_barrett_folded(w6[5], z, n);
     int64_t o0 = (int64_t)((uint64_t)x + (uint64_t)tb + (uint64_t)tc);
     int64_t o1 = (int64_t)((uint64_t)x + (uint64_t)tbz + (uint64_t)tczz);
     int64_t o2 = (int64_t)((uint64_t)x + (uint64_t)tbzz + (uint64_t)tcz);
     if (info & INFO_FUSE) {
-        o0 = csd_barrett(one, o0, n);
-        o1 = csd_barrett(one, o1, n);
-        o2 = csd_barrett(one, o2, n);
+        o0 = csd_barrett_folded(one, o0, n);
+        o1 = csd_barrett_folded(one, o1, n);
+        o2 = csd_barrett_folded(one, o2, n);
     }
     x = o0;
     y = o1;
@@ -684,15 +701,15 @@ __device__ __forceinline__ void bf3_inv(int64_t& x, int64_t& y, int64_t& z, cons
     const bool fuse = info & INFO_FUSE;
     const CsdBarrett ym = fuse ? inv_len : w2[0];
     const CsdBarrett zm = fuse ? inv_len : w2[1];
-    const bool skip = !fuse && ym.z == 1;
-    const int64_t y1z = csd_barrett(zeta, y, n);
-    const int64_t y1zz = csd_barrett(zeta2, y, n);
-    const int64_t y2z = csd_barrett(zeta, z, n);
-    const int64_t y2zz = csd_barrett(zeta2, z, n);
+    const bool skip = !fuse && csd_folded_is_one(ym, n);
+    const int64_t y1z = csd_barrett_folded(zeta, y, n);
+    const int64_t y1zz = csd_barrett_folded(zeta2, y, n);
+    const int64_t y2z = csd_barrett_folded(zeta, z, n);
+    const int64_t y2zz = csd_barrett_folded(zeta2, z, n);
     const int64_t sum = (int64_t)((uint64_t)x + (uint64_t)y + (uint64_t)z);
     const int64_t bb = (int64_t)((uint64_t)x + (uint64_t)y1zz + (uint64_t)y2z);
     const int64_t cc = (int64_t)((uint64_t)x + (uint64_t)y1z + (uint64_t)y2zz);
-    x = fuse ? csd_barrett(ym, sum, n) : sum;
+    x = fuse ? csd_barrett_folded(ym, sum, n) : sum;
     y = skip_or_mul<LANE>(skip, ym, bb, n);
     z = skip_or_mul<LANE>(skip, zm, cc, n);
 }
@@ -716,7 +733,7 @@ __device__ __forceinline__ void block_level(int64_t (&v)[Pow<R>::of(G)], const D
     const int64_t n = P->n;
     if (info & INFO_REDUCE) {
 #pragma unroll
-        for (int m = 0; m < M; m++) v[m] = csd_barrett(P->one, v[m], n);
+        for (int m = 0; m < M; m++) v[m] = csd_barrett_folded(P->one, v[m], n);
     }
     const uint32_t nbk = F.nblocks[s];
 #pragma unroll
@@ -819,7 +836,7 @@ __device__ __forceinline__ void ct_level(int64_t (&v)[Pow<R>::of(G)], const DevP
     const int64_t n = P->n;
     if (info & INFO_REDUCE) {
 #pragma unroll
-        for (int m = 0; m < M; m++) v[m] = csd_barrett(P->one, v[m], n);
+        for (int m = 0; m < M; m++) v[m] = csd_barrett_folded(P->one, v[m], n);
     }
 #pragma unroll
     for (int sb = 0; sb < NSB; sb++) {
@@ -1157,7 +1174,7 @@ __global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
             for (int g = threadIdx.x; g < total; g += blockDim.x) {
                 const uint32_t r = a.rows == 1 ? 0u : fdiv((uint32_t)g, a.magic_L);
                 const uint32_t s = r * lline + lds_slot(g - r * L);
-                lds[s] = csd_barrett(m, lds[s], P->n);
+                lds[s] = csd_barrett_folded(m, lds[s], P->n);
             }
         }
     } else {
@@ -1211,7 +1228,7 @@ __device__ __forceinline__ void col_stage(int64_t (&v)[F * S * PRE], const DevPl
     const uint8_t info = P->info[dir][f][s];
     if (info & INFO_REDUCE) {
 #pragma unroll
-        for (int e = 0; e < F * S * PRE; e++) v[e] = csd_barrett(P->one, v[e], n);
+        for (int e = 0; e < F * S * PRE; e++) v[e] = csd_barrett_folded(P->one, v[e], n);
     }
 #pragma unroll
     for (int b = 0; b < F / SL; b++) {
@@ -1299,7 +1316,7 @@ __global__ __launch_bounds__(256) void k_ntt_cols(ColArgs a, ModTable mt) {
     for (int k = 0; k < L; k++) v[k] = base[k * a.st[2]];
     const int dir = a.dir;
     if constexpr (L == 1) {
-        if (dir != 1) v[0] = csd_barrett(dir == 2 ? P->inv_len : P->one, v[0], P->n);
+        if (dir != 1) v[0] = csd_barrett_folded(dir == 2 ? P->inv_len : P->one, v[0], P->n);
     } else {
         // factor 0 acts on the most significant digit (suffix F1), then factor 1 (suffix 1)
         col_factor<F0, F1, 1, INV>(v, P, TW, 0, dir);

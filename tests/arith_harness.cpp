@@ -68,12 +68,20 @@ int main(int argc, char** argv) {
             }
             if (N < ((int64_t)1 << 62)) {
                 int64_t z, u;
-                orc_barrett_new((int64_t)(rnd() % (uint64_t)N), N, &z, &u);
+                const int64_t value = i % 64 == 0 ? 1 : (int64_t)(rnd() % (uint64_t)N);
+                orc_barrett_new(value, N, &z, &u);
                 int64_t xx = i < 100 ? edge((int)i) : (int64_t)rnd();
                 orc_barrett_mul(z, u, xx, N, &want);
                 got = csd_barrett(CsdBarrett{z, u}, xx, N);
                 if (want != got) {
                     printf("barrett mismatch N=%lld z=%lld x=%lld\n", (long long)N, (long long)z, (long long)xx);
+                    return 1;
+                }
+                // the device tables' folded constant gives the same bits; only the constant 1 reads as one
+                const CsdBarrett f = csd_barrett_fold(CsdBarrett{z, u}, N);
+                got = csd_barrett_folded(f, xx, N);
+                if (want != got || csd_folded_is_one(f, N) != (value == 1)) {
+                    printf("folded barrett mismatch N=%lld z=%lld x=%lld\n", (long long)N, (long long)z, (long long)xx);
                     return 1;
                 }
             }

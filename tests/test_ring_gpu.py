@@ -261,3 +261,24 @@ def test_transform_overlap_halves(gpu, cpu, rings):
     cpu.hal_mul_mont64(dc, other, c.mods_for(base))
     c.crt_to_pb_tensor(dc, base)
     assert same(outs[0][[0, 13]], dc)
+
+
+def test_transform_round_trip_full_batch(gpu, cpu, rings):
+    """The benchmark's full batch ([128, 17, 12, 3072], 642 MB): CRT->pb after pb->CRT gives
+    every residue back mod p (a size-independent property), and three elements spread over both
+    halves of the two-stream overlap equal the oracle's transform bit for bit."""
+    g, c = rings("field_36864", (960, 980), 76)
+    base = g.initial_rns_base()
+    primes = np.array([g.primes[i] for i in base], dtype=np.int64).reshape(1, -1, 1, 1)
+    rng = np.random.default_rng(128)
+    data = uniform_residues(rng, primes.reshape(-1).tolist(), (128, len(base), 12, 3072))
+    dg = to_dev(gpu, data)
+    g.pb_to_crt_tensor(dg, base)
+    crt = from_dev(gpu, dg)
+    picks = [0, 63, 127]
+    dc = data[picks].copy()
+    c.pb_to_crt_tensor(dc, base)
+    assert same(crt[picks], dc)
+    g.crt_to_pb_tensor(dg, base)
+    back = from_dev(gpu, dg)
+    assert (np.mod(back, primes) == data).all()
