@@ -20,9 +20,11 @@ every output digit is below 2 b Delta), which a dropped, unscaled or misplaced c
 (dbfv.rs:1324-1362), which needs key generation and encryption that this oracle does not restate.
 
 ``dbfv_hom_mul`` is a composition of the pinned ring steps (lift, inner product, wide reduction,
-rescale, relin); its composition is pinned by the property that a product with the digit
-ciphertext of a single-digit zero-key "encryption" decomposes digit-wise (tests/test_oracle_dbfv.py);
-beyond that it is parity unpinned, as the reference checks it only through decryption.
+rescale, relin). The composition is pinned by the convolution identity under the zero secret key
+(tests/test_oracle_dbfv.py): with key rows (0, g_i), g_i the CRT recomposition factor of gadget
+digit i, output digit k is (round(t T_k0 / Q), round(t T_k1 / Q) + round(t T_k2 / Q)) mod Q up to
++-1 per term, T_kc the exact convolution components of the lifted operands. Keys of a nonzero
+secret are not restated (the reference checks those only through decryption).
 
 Big-integer constants use Python ints, as the reference's feanor-math ZZbig does.
 """

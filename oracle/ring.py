@@ -15,8 +15,11 @@ path serves, on top of any backend with the ``hal::Backend`` method names (here:
 
 Pins beyond the reference's KATs: lift_to_extended, rescale_to_base and gadget_decompose are
 checked against their closed forms by CRT reconstruction (tests/test_oracle_ring.py). The
-compositions relin_crt and bfv_hom_mul are parity unpinned beyond their pinned steps (the
-reference checks them only through decryption).
+compositions relin_crt and bfv_hom_mul are pinned by the identity they satisfy under the zero
+secret key (tests/test_oracle_dbfv.py::test_bfv_hom_mul_identity): with key rows (0, g_i), g_i the
+CRT recomposition factor of gadget digit i, the product is (round(t l0 r0 / Q), round(t (l0 r1 +
+l1 r0) / Q) + round(t l1 r1 / Q)) mod Q up to +-1 per term. Keys of a nonzero secret are not
+restated (the reference checks those only through decryption).
 
 Big-integer constants use Python ints, which is what the reference's feanor-math BigInt/Zn
 arithmetic computes exactly. Primality (feanor-math's Miller-Rabin, ring.rs:139) is decided with

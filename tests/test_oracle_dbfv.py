@@ -153,38 +153,43 @@ def test_split_i128_limbs():
         assert -(1 << 63) <= lo < 1 << 63 and hi * (1 << 64) + lo == x
 
 
-def one_key(ring, base):
-    """the relinearization key of the zero secret key with key part 1: rows (0, mont(1)) in CRT
-    basis, one gadget digit over the whole base; relin then maps (c0, c1, c2) to (c0, c1 + c2)
-    mod Q"""
-    n2, n3 = ring.field.deg_lprime, ring.field.deg_k
-    key = ring.allocate(2, base)
-    for j, i in enumerate(base):
-        key[1, j] = R.montgomery_repr(1, ring.primes[i])
-    return [key], [(0, len(base))]
+def one_key(ring, base, ndigits=1):
+    """the relinearization key of the zero secret key with key part g_i = (Q/D_i) ((Q/D_i)^-1
+    mod D_i) for gadget digit i (the digits of select_digits): rows (0, g_i) in CRT basis. The
+    gadget digits d_i of c2 recompose to sum_i d_i g_i = c2 mod Q, so relin maps (c0, c1, c2) to
+    (c0, c1 + c2) mod Q"""
+    digits = R.select_digits(ndigits, len(base))
+    q = ring.modulus_of(base)
+    keys = []
+    for a, b in digits:
+        di = ring.modulus_of(base[a:b])
+        g = (q // di) * pow((q // di) % di, -1, di) % q
+        key = ring.allocate(2, base)
+        for j, i in enumerate(base):
+            key[1, j] = R.montgomery_repr(g % ring.primes[i], ring.primes[i])
+        keys.append(key)
+    return keys, digits
 
 
-def hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, out, tol=(1, 2)):
-    """The digit convolution of dbfv_hom_mul (dbfv.rs:1078-1143) under `one_key`: with T_k,c the
-    exact integer components sum_j lhs_j (x) rhs_(k-j) (c = 0: l0 r0, 1: l0 r1 + l1 r0, 2: l1 r1),
-    formed from the lifted operands in CRT basis and taken to period basis, output digit k must
-    be (round(t T_k,0 / Q), round(t T_k,1 / Q) + round(t T_k,2 / Q)) mod Q up to the rescale's
-    +-1 per term. Returns the offending (digit, component) pairs."""
-    ll, lifted = R.lift_to_extended(ring, lhs_red, base)
-    rl, _ = R.lift_to_extended(ring, rhs_red, base)
+def convolution_failures(ring, t, base, ll, rl, lifted, out_pb, K, tol=(1, 2)):
+    """The digit convolution identity under `one_key`: with T_k,c the exact integer components
+    sum_j ll_j (x) rl_(k-j) (c = 0: l0 r0, 1: l0 r1 + l1 r0, 2: l1 r1) of the lifted CRT-basis
+    operands (two rows per digit), formed here and taken to period basis by the oracle's
+    transforms, output digit k (period basis) must be (round(t T_k,0 / Q), round(t T_k,1 / Q) +
+    round(t T_k,2 / Q)) mod Q up to the rescale's +-1 per term. Returns the offending
+    (digit, component) pairs."""
     d1, d2 = ll.shape[0] // 2, rl.shape[0] // 2
-    K = min(modulus.digits(), d1 + d2 - 1)
     T = ring.allocate(3 * K, lifted)
     for j_pos, i in enumerate(lifted):
         p = ring.primes[i]
         rinv = pow(1 << 64, -1, p)
-        lv = [[[int(v) for v in ll[row, j_pos].reshape(-1)]] for row in range(2 * d1)]
-        rv = [[[int(v) for v in rl[row, j_pos].reshape(-1)]] for row in range(2 * d2)]
-        n = len(lv[0][0])
+        lv = [[int(v) for v in ll[row, j_pos].reshape(-1)] for row in range(2 * d1)]
+        rv = [[int(v) for v in rl[row, j_pos].reshape(-1)] for row in range(2 * d2)]
+        n = len(lv[0])
         for k in range(K):
             comps = [[0] * n for _ in range(3)]
             for j in range(max(0, k - (d2 - 1)), min(k, d1 - 1) + 1):
-                a0, a1, b0, b1 = lv[2 * j][0], lv[2 * j + 1][0], rv[2 * (k - j)][0], rv[2 * (k - j) + 1][0]
+                a0, a1, b0, b1 = lv[2 * j], lv[2 * j + 1], rv[2 * (k - j)], rv[2 * (k - j) + 1]
                 for e in range(n):
                     comps[0][e] += a0[e] * b0[e]
                     comps[1][e] += a0[e] * b1[e] + a1[e] * b0[e]
@@ -194,8 +199,7 @@ def hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, out, tol=(1, 2)):
     ring.crt_to_pb_tensor(T, lifted)
     X = lifts(ring, T, lifted)
     q = ring.modulus_of(base)
-    t = modulus.t()
-    z = lifts(ring, out, base)
+    z = lifts(ring, out_pb, base)
     bad = []
 
     def centred(v):
@@ -214,6 +218,14 @@ def hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, out, tol=(1, 2)):
     return bad
 
 
+def hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, out):
+    """convolution_failures for dbfv_hom_mul (dbfv.rs:1078-1143), from the reduced operands"""
+    ll, lifted = R.lift_to_extended(ring, lhs_red, base)
+    rl, _ = R.lift_to_extended(ring, rhs_red, base)
+    K = min(modulus.digits(), ll.shape[0] // 2 + rl.shape[0] // 2 - 1)
+    return convolution_failures(ring, modulus.t(), base, ll, rl, lifted, out, K)
+
+
 @pytest.mark.parametrize("modulus,d1,d2,square", [(D.DBFVModulus.new(16, 4), 4, 4, False),
                                                   (D.DBFVModulus.new(16, 4), 2, 3, False),
                                                   (D.DBFVModulus.new(16, 3), 3, 3, True)])
@@ -223,7 +235,7 @@ def test_dbfv_hom_mul_convolution(cpu, modulus, d1, d2, square):
     rng = np.random.default_rng(40 + d1 + d2)
     lhs = uniform(ring, 2 * d1, base, rng)
     rhs = lhs if square else uniform(ring, 2 * d2, base, rng)
-    keys, digits = one_key(ring, base)
+    keys, digits = one_key(ring, base, 2)
     lhs_red, _ = D.dbfv_reduce(ring, modulus, np.array(lhs), 1, base)
     rhs_red, _ = D.dbfv_reduce(ring, modulus, np.array(rhs), 1, base)
     if square:
@@ -244,7 +256,7 @@ def test_dbfv_hom_mul_convolution(cpu, modulus, d1, d2, square):
     comps = np.array(out)
     comps[0], comps[1] = np.array(out[1]), np.array(out[0])
     assert hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, comps)
-    zero_keys = [ring.allocate(2, base)]
+    zero_keys = [ring.allocate(2, base) for _ in digits]
     no_c2, _ = D.dbfv_hom_mul(ring, modulus, np.array(lhs), 1, np.array(rhs), 1, base, zero_keys, digits)
     assert hom_mul_failures(ring, modulus, base, lhs_red, rhs_red, no_c2)
 
@@ -259,3 +271,41 @@ def test_dbfv_hom_mul_empty_operand(cpu):
     keys, digits = one_key(ring, base)
     out, bound = D.dbfv_hom_mul(ring, modulus, lhs, 1, empty, 1, base, keys, digits)
     assert out.shape[0] == 0 and bound == 1
+
+
+@pytest.mark.parametrize("field,log2_q,t_bits", [("field_18", (100, 120), 4), ("field_192", (200, 220), 40)])
+def test_bfv_hom_mul_identity(cpu, field, log2_q, t_bits):
+    """The oracle's bfv_hom_mul and relin_crt (bfv.rs:699-775), pinned by the one-digit case of
+    the same identity: under the zero secret key the product of two CRT-basis ciphertexts is
+    (round(t l0 r0 / Q), round(t (l0 r1 + l1 r0) / Q) + round(t l1 r1 / Q)) mod Q up to +-1 per
+    term; it rejects swapped components and a c2 that is not switched in."""
+    ring = R.RingContext(cpu, R.FieldDescriptor(field), log2_q, t_bits)
+    base = ring.initial_rns_base()
+    rng = np.random.default_rng(90 + t_bits)
+    lhs, rhs = uniform(ring, 2, base, rng), uniform(ring, 2, base, rng)
+    keys, digits = one_key(ring, base, 2)
+    t = 1 << t_bits
+    out = R.bfv_hom_mul(ring, t, np.array(lhs), np.array(rhs), base, keys, digits)
+
+    def lifted(ct):
+        pb = np.array(ct)
+        ring.crt_to_pb_tensor(pb, base)
+        return R.lift_to_extended(ring, pb, base, crt=np.array(ct))
+
+    ll, lb = lifted(lhs)
+    rl, _ = lifted(rhs)
+
+    def fails(o):
+        pb = np.array(o)
+        ring.crt_to_pb_tensor(pb, base)
+        return convolution_failures(ring, t, base, ll, rl, lb, pb, 1)
+
+    assert fails(out) == []
+    swapped = np.array(out)
+    swapped[0], swapped[1] = np.array(out[1]), np.array(out[0])
+    assert fails(swapped)
+    no_c2 = R.bfv_hom_mul(ring, t, np.array(lhs), np.array(rhs), base, [ring.allocate(2, base) for _ in digits], digits)
+    assert fails(no_c2)
+    # the key digits paired with the wrong gadget digits
+    crossed = R.bfv_hom_mul(ring, t, np.array(lhs), np.array(rhs), base, keys[::-1], digits)
+    assert fails(crossed)
