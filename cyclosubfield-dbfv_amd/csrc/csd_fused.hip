@@ -289,26 +289,29 @@ void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t
 }
 
 // ---------------------------------------------------------------------------------------------
-// gadget base conversion, one kernel (GadgetBConvCache::bconv, ring.rs:1809-1851)
+// base conversions as one kernel: GadgetBConvCache::bconv (ring.rs:1809-1851) and
+// BConvCache::bconv (ring.rs:1663-1701)
 //
 // The HAL sequence multiplies the source by (D_i/p_u)^-1 in Montgomery form, forms the
-// [d, r + 1] i128 matrix product over chunks of `acc` source limbs (each chunk reduced with
+// [d, r' + 1] i128 matrix product over chunks of `acc` source limbs (each chunk reduced with
 // double_red128 and added), adds round(corr / 2^shift) * (-D_i 2^64 mod p_j) through red128 and
-// converts to Montgomery form. Row i of the matrix is zero outside digit i, so thread (b, pos)
-// only multiplies the digit's own limbs; a zero product adds nothing to a wrapping i128 sum, and
-// every chunk, empty or not, is still reduced and added, so the residues are those of the
-// sequence. (Ring primes are below 2^57, so a chunk holds at least 2^13 products and up to 64
-// limbs always form one chunk.) The digit's lifted limbs sit in the thread's LDS column.
+// converts to Montgomery form. The BEHZ conversion is the case of one digit over the whole source
+// base, with a target base of its own. Row i of the gadget matrix is zero outside digit i, so
+// thread (b, pos) only multiplies the digit's own limbs; a zero product adds nothing to a
+// wrapping i128 sum, and every chunk, empty or not, is still reduced and added, so the residues
+// are those of the sequence. (Ring primes are below 2^57, so a chunk holds at least 2^13
+// products and up to 64 limbs always form one chunk.)
 
 struct GbArgs {
-    int64_t B, d, r, n3, P;  // source elements, digits, limbs, dim-3 length, positions n2 n3
-    int64_t ss[4], sd[4];    // source [B, r, n2, n3] and destination [B d, r, n2, n3] strides
+    int64_t B, d, r, rd, rdp, n3, P;  // elements, digits, source limbs, target limbs (padded), dim-3 length, positions
+    int64_t ss[4], sd[4];             // source [B, r, n2, n3] and destination [B d, rd, n2, n3] strides
     const int64_t* src;
     int64_t* dst;
     const int64_t* dopi;  // [r]: (D_i / p_u)^-1 for u in digit i
-    const int64_t* mat;   // [d][r + 1][r]: (D_i / p_u) R^2 mod p_j, row r the correction weights
-    const int64_t* dmp;   // [d][r]: -D_i 2^64 mod p_j
-    int64_t acc;          // source limbs per i128 chunk
+    const int64_t* mat;   // [d][rd + 1][r]: row rd holds the correction weights
+    const int64_t* matT;  // [d][r][rdp]: (D_i / p_u) R^2 mod p_j, transposed, zero beyond rd
+    const int64_t* dmp;   // [d][rd]: -D_i 2^64 mod p_j
+    const CsdMod* smods;  // source moduli (device), or null when the source base is the target base
     int shift;            // log2 of q~
     int32_t lo[CSD_MAX_MODULI], hi[CSD_MAX_MODULI];
 };
@@ -320,8 +323,13 @@ __device__ __forceinline__ i128 gb_mac(i128 acc, int64_t a, int64_t b) {
     return (i128)((u128)acc + (((u128)(uint64_t)hi << 64) | lo));
 }
 
-__global__ __launch_bounds__(256) void k_gadget_bconv(GbArgs a, ModTable mt) {
-    extern __shared__ int64_t gb_x[];
+#define GB_GROUP 8  // target limbs per pass over the digit's source limbs (2 and 4 measured slower)
+
+// One thread per (element, position). Launched only when every digit's source limbs fit one i128
+// chunk, which the ring's primes (below 2^57) guarantee; so each target limb's sum is reduced once.
+// Every pass over GB_GROUP target limbs lifts the digit's source limbs again (the source stays in
+// L2); parking them in LDS instead measured the same, and costs occupancy for long digits.
+__global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
     const int64_t chunks = (a.P + 255) / 256;
     const int64_t b = blockIdx.x / chunks;
     const int64_t pos = (blockIdx.x % chunks) * 256 + threadIdx.x;
@@ -329,43 +337,55 @@ __global__ __launch_bounds__(256) void k_gadget_bconv(GbArgs a, ModTable mt) {
     const int64_t k = pos / a.n3, l = pos % a.n3;
     const int64_t* s = a.src + b * a.ss[0] + k * a.ss[2] + l * a.ss[3];
     int64_t* o = a.dst + b * a.d * a.sd[0] + k * a.sd[2] + l * a.sd[3];
-    int64_t* xs = gb_x + threadIdx.x;
-    const int r = (int)a.r;
+    const int r = (int)a.r, rd = (int)a.rd;
     const i128 half = (i128)1 << (a.shift - 1);
+    auto lift = [&](int u) {
+        const CsdMod mu = a.smods ? a.smods[u] : mt.m[u];
+        return csd_mont_mul(mu, s[u * a.ss[1]], a.dopi[u]);
+    };
     for (int i = 0; i < (int)a.d; i++) {
         const int lo = a.lo[i], hi = a.hi[i];
-        const int64_t* mi = a.mat + (int64_t)i * (r + 1) * r;
-        i128 corr = 0;
-        for (int u = lo; u < hi; u++) {
-            int64_t x = csd_mont_mul(mt.m[u], s[u * a.ss[1]], a.dopi[u]);
-            xs[(u - lo) * 256] = x;
-            corr = gb_mac(corr, mi[(int64_t)r * r + u], x);
-        }
-        const int64_t q = (int64_t)((i128)((u128)corr + (u128)half) >> a.shift);  // |q| <= 2^63
-        for (int j = 0; j < r; j++) {
-            const int64_t* mj = mi + (int64_t)j * r;
-            int64_t v = 0;
-            for (int64_t c0 = 0; c0 < r; c0 += a.acc) {
-                const int u0 = (int)max((int64_t)lo, c0), u1 = (int)min((int64_t)hi, min(c0 + a.acc, (int64_t)r));
-                i128 sum = 0;
-                for (int u = u0; u < u1; u++) sum = gb_mac(sum, mj[u], xs[(u - lo) * 256]);
-                const int64_t red = csd_double_red128(mt.m[j], sum);
-                v = c0 == 0 ? red : (int64_t)((uint64_t)v + (uint64_t)red);
+        const int64_t* corr_row = a.mat + ((int64_t)i * (rd + 1) + rd) * r;
+        const int64_t* wi = a.matT + (int64_t)i * r * a.rdp;
+        int64_t q = 0;
+        for (int j0 = 0; j0 < rd; j0 += GB_GROUP) {
+            i128 sum[GB_GROUP];
+#pragma unroll
+            for (int g = 0; g < GB_GROUP; g++) sum[g] = 0;
+            i128 corr = 0;
+            for (int u = lo; u < hi; u++) {
+                const int64_t xv = lift(u);
+                if (j0 == 0) corr = gb_mac(corr, corr_row[u], xv);
+                const int64_t* w = wi + (int64_t)u * a.rdp + j0;
+#pragma unroll
+                for (int g = 0; g < GB_GROUP; g++)
+                    if (j0 + g < rd) sum[g] = gb_mac(sum[g], w[g], xv);
             }
-            v = (int64_t)((uint64_t)v + (uint64_t)csd_mont_mul(mt.m[j], q, a.dmp[(int64_t)i * r + j]));
-            o[i * a.sd[0] + j * a.sd[1]] = csd_to_mont64(mt.m[j], v);
+            if (j0 == 0) q = (int64_t)((i128)((u128)corr + (u128)half) >> a.shift);  // |q| <= 2^63
+#pragma unroll
+            for (int g = 0; g < GB_GROUP; g++) {
+                const int j = j0 + g;
+                if (j < rd) {
+                    int64_t v = csd_double_red128(mt.m[j], sum[g]);
+                    v = (int64_t)((uint64_t)v + (uint64_t)csd_mont_mul(mt.m[j], q, a.dmp[(int64_t)i * rd + j]));
+                    o[i * a.sd[0] + j * a.sd[1]] = csd_to_mont64(mt.m[j], v);
+                }
+            }
         }
     }
 }
 
-void csd_gadget_bconv_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, int64_t d, const int64_t* dopi,
-                             const int64_t* mat, const int64_t* dmp, const std::vector<size_t>& bounds, size_t acc,
-                             size_t shift, const csd_modulus* mods) {
+void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, int64_t d, const int64_t* dopi,
+                            const int64_t* mat, const int64_t* matT, int64_t rdp, const int64_t* dmp,
+                            const std::vector<size_t>& bounds, size_t shift, const csd_modulus* dmods,
+                            const CsdMod* smods_dev, const char* timer) {
     GbArgs a;
     memset(&a, 0, sizeof(a));
     a.B = src.shape[0];
     a.d = d;
     a.r = src.shape[1];
+    a.rd = dst.shape[1];
+    a.rdp = rdp;
     a.n3 = src.shape[3];
     a.P = src.shape[2] * src.shape[3];
     for (int q = 0; q < 4; q++) {
@@ -376,26 +396,31 @@ void csd_gadget_bconv_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& 
     a.dst = (int64_t*)dst.ptr;
     a.dopi = dopi;
     a.mat = mat;
+    a.matT = matT;
     a.dmp = dmp;
-    a.acc = (int64_t)std::min<size_t>(acc, (size_t)a.r);
+    a.smods = smods_dev;
     a.shift = (int)shift;
-    int64_t max_len = 0;
+    if (!smods_dev && a.rd != a.r) throw CsdError("fused bconv: source moduli missing");
+    if (rdp < a.rd || rdp % GB_GROUP) throw CsdError("fused bconv: transposed weights not padded");
     for (int64_t i = 0; i < d; i++) {
         a.lo[i] = (int32_t)bounds[2 * i];
         a.hi[i] = (int32_t)bounds[2 * i + 1];
-        max_len = std::max<int64_t>(max_len, a.hi[i] - a.lo[i]);
     }
     if (a.B * a.P == 0) return;
     const int64_t blocks = a.B * ((a.P + 255) / 256);
-    if (blocks > (int64_t)INT32_MAX) throw CsdError("gadget bconv: grid too large");
-    const size_t lds = (size_t)max_len * 256 * sizeof(int64_t);  // at most 128 KiB (64 limbs)
-    static bool attr = false;
-    if (!attr) {
-        HIPCHECK(hipFuncSetAttribute((const void*)k_gadget_bconv, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr = true;
-    }
-    ModTable t = make_mod_table(mods, (size_t)a.r, a.r);
-    TimedScope ts(ctx, "gadget_bconv", 8.0 * (double)a.B * a.P * a.r * (1.0 + (double)d));
-    hipLaunchKernelGGL(k_gadget_bconv, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, a, t);
+    if (blocks > (int64_t)INT32_MAX) throw CsdError("fused bconv: grid too large");
+    ModTable t = make_mod_table(dmods, (size_t)a.rd, a.rd);
+    TimedScope ts(ctx, timer, 8.0 * (double)a.B * a.P * ((double)a.r + (double)d * a.rd));
+    hipLaunchKernelGGL(k_bconv_fused, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
     HIPCHECK(hipGetLastError());
+}
+
+// the padded, transposed weight layout csd_bconv_fused_launch reads: out[i][u][j] = mat[i][j][u]
+std::vector<int64_t> csd_bconv_transpose(const std::vector<int64_t>& mat, int64_t d, int64_t rd, int64_t r, int64_t* rdp) {
+    *rdp = (rd + GB_GROUP - 1) / GB_GROUP * GB_GROUP;
+    std::vector<int64_t> t((size_t)(d * r * *rdp), 0);
+    for (int64_t i = 0; i < d; i++)
+        for (int64_t j = 0; j < rd; j++)
+            for (int64_t u = 0; u < r; u++) t[(size_t)((i * r + u) * *rdp + j)] = mat[(size_t)((i * (rd + 1) + j) * r + u)];
+    return t;
 }

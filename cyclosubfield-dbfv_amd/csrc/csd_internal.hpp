@@ -79,10 +79,14 @@ void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t
                                  const int64_t* c0, const int64_t* key, const uint32_t* perm2, const uint32_t* perm3,
                                  int64_t* out, const csd_modulus* mods, int full0, int full1);
 
-// GadgetBConvCache::bconv as one kernel over B source elements (see csd_fused.hip)
-void csd_gadget_bconv_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, int64_t d, const int64_t* dopi,
-                             const int64_t* mat, const int64_t* dmp, const std::vector<size_t>& bounds, size_t acc,
-                             size_t shift, const csd_modulus* mods);
+// GadgetBConvCache::bconv / BConvCache::bconv as one kernel over B source elements (see
+// csd_fused.hip), for digits whose limbs fit one i128 chunk; smods_dev holds the source moduli,
+// or is null when they are the target moduli. matT is csd_bconv_transpose of mat.
+void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, int64_t d, const int64_t* dopi,
+                            const int64_t* mat, const int64_t* matT, int64_t rdp, const int64_t* dmp,
+                            const std::vector<size_t>& bounds, size_t shift, const csd_modulus* dmods,
+                            const CsdMod* smods_dev, const char* timer);
+std::vector<int64_t> csd_bconv_transpose(const std::vector<int64_t>& mat, int64_t d, int64_t rd, int64_t r, int64_t* rdp);
 
 // launch shape of the work-list NTT kernels (0 = the built-in choice)
 struct CsdSegTune {

@@ -452,16 +452,47 @@ struct TransformCache {  // ring.rs:578-592
 struct BConvCache {  // ring.rs:1596-1604
     size_t log2_qtilde, acc;
     std::unique_ptr<DevBuf> q_over_p_inv, matrix, q_mod_p;
+    std::unique_ptr<DevBuf> src_mods;  // the source moduli as CsdMod, for the fused kernel
+    std::unique_ptr<DevBuf> matrix_t;  // the matrix rows transposed and padded (csd_bconv_transpose)
+    int64_t rdp = 0;
     std::vector<int64_t> host_qopi, host_matrix, host_qm;  // the same constants, kept for fused kernels
 };
 
 struct GadgetCache {  // ring.rs:1714-1727
     size_t log2_qtilde, acc;
     std::unique_ptr<DevBuf> d_over_p_inv, matrix, d_mod_p;
+    std::unique_ptr<DevBuf> matrix_t;  // the matrix rows transposed and padded (csd_bconv_transpose)
+    int64_t rdp = 0;
 };
 
 static bool g_overlap_halves = true;  // see csd_ring_set_overlap
 static bool g_gadget_fused = true;    // see csd_ring_set_gadget_form
+static bool g_bconv_fused = true;     // see csd_ring_set_bconv_form
+
+// whether the memory spans of two views (first to last element) intersect
+static bool views_overlap(const csd_view& a, const csd_view& b, int elem) {
+    auto span = [elem](const csd_view& v, uintptr_t& lo, uintptr_t& hi) {
+        int64_t last = 0;
+        for (int q = 0; q < 4; q++) {
+            if (v.shape[q] == 0) {
+                lo = hi = 0;
+                return false;
+            }
+            if (v.strides[q] < 0) {  // not produced by this library's views; treat as overlapping
+                lo = 0;
+                hi = UINTPTR_MAX;
+                return true;
+            }
+            last += (v.shape[q] - 1) * v.strides[q];
+        }
+        lo = (uintptr_t)v.ptr;
+        hi = lo + (uintptr_t)(last + 1) * (uintptr_t)elem;
+        return true;
+    };
+    uintptr_t alo, ahi, blo, bhi;
+    if (!span(a, alo, ahi) || !span(b, blo, bhi)) return false;
+    return alo < bhi && blo < ahi;
+}
 
 struct csd_ring {
     csd_ctx* ctx;
@@ -726,6 +757,14 @@ struct csd_ring {
         c->q_over_p_inv = upload(ctx, qopi);
         c->matrix = upload(ctx, mat);
         c->q_mod_p = upload(ctx, qm);
+        {
+            auto sm = mods_for(src);
+            ModTable t = make_mod_table(sm.data(), sm.size(), (int64_t)sm.size());
+            std::vector<int64_t> flat;
+            for (size_t u = 0; u < r; u++) flat.insert(flat.end(), {t.m[u].N, t.m[u].ninv, t.m[u].r2, t.m[u].r3});
+            c->src_mods = upload(ctx, flat);
+        }
+        c->matrix_t = upload(ctx, csd_bconv_transpose(mat, 1, (int64_t)rd, (int64_t)r, &c->rdp));
         c->host_qopi = qopi;
         c->host_matrix = mat;
         c->host_qm = qm;
@@ -734,7 +773,13 @@ struct csd_ring {
         return ref;
     }
 
-    // BConvCache::bconv (ring.rs:1663-1701); destroys src
+    // whether bconv runs the fused kernel (which leaves src unchanged) for these operands
+    static bool fused_bconv_applies(const BConvCache& c, int64_t r, const csd_view& dst, const csd_view& src) {
+        return g_bconv_fused && c.acc >= (size_t)r && !views_overlap(dst, src, 8);
+    }
+
+    // BConvCache::bconv (ring.rs:1663-1701). The fused form (default, when src and dst do not
+    // overlap) is one kernel and leaves src as it is; the HAL sequence destroys src.
     void bconv(const csd_view& dst, const std::vector<size_t>& dbase, const csd_view& src, const std::vector<size_t>& sbase) {
         BConvCache& c = bconv_cache(sbase, dbase);
         int64_t r = (int64_t)sbase.size(), rd = (int64_t)dbase.size();
@@ -743,6 +788,13 @@ struct csd_ring {
         if (dst.shape[0] != cnt || dst.shape[1] != rd || dst.shape[2] != n2 || dst.shape[3] != n3)
             throw CsdError("bconv: destination shape");
         auto smods = mods_for(sbase), dmods = mods_for(dbase);
+        if (fused_bconv_applies(c, r, dst, src)) {
+            const std::vector<size_t> whole = {0, (size_t)r};
+            csd_bconv_fused_launch(ctx, dst, src, 1, (const int64_t*)c.q_over_p_inv->p, (const int64_t*)c.matrix->p,
+                                   (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.q_mod_p->p, whole,
+                                   c.log2_qtilde, dmods.data(), (const CsdMod*)c.src_mods->p, "bconv");
+            return;
+        }
         csd_view qopi = make_view(c.q_over_p_inv->p, 1, r, 1, 1);
         ok(csd_hal_mul_mont64(ctx, &src, &qopi, smods.data(), r));
         DevBuf corr(ctx, (size_t)cnt * n2 * n3 * 16), tmp(ctx, (size_t)cnt * (rd + 1) * n2 * n3 * 16);
@@ -826,6 +878,7 @@ struct csd_ring {
         c->d_over_p_inv = upload(ctx, dopi);
         c->matrix = upload(ctx, mat);
         c->d_mod_p = upload(ctx, dmp);
+        c->matrix_t = upload(ctx, csd_bconv_transpose(mat, (int64_t)d, (int64_t)r, (int64_t)r, &c->rdp));
         auto& ref = *c;
         gcache[key] = std::move(c);
         return ref;
@@ -842,10 +895,10 @@ struct csd_ring {
         if (dst.shape[0] != B * d || dst.shape[1] != r || dst.shape[2] != n2 || dst.shape[3] != n3)
             throw CsdError("gadget bconv: destination shape");
         auto m = mods_for(base);
-        if (g_gadget_fused) {
-            csd_gadget_bconv_launch(ctx, dst, src, d, (const int64_t*)c.d_over_p_inv->p, (const int64_t*)c.matrix->p,
-                                    (const int64_t*)c.d_mod_p->p, bounds, c.acc,
-                                    c.log2_qtilde, m.data());
+        if (g_gadget_fused && c.acc >= (size_t)r && !views_overlap(dst, src, 8)) {
+            csd_bconv_fused_launch(ctx, dst, src, d, (const int64_t*)c.d_over_p_inv->p, (const int64_t*)c.matrix->p,
+                                   (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.d_mod_p->p, bounds,
+                                   c.log2_qtilde, m.data(), nullptr, "gadget_bconv");
             return;
         }
         csd_view dopi = make_view(c.d_over_p_inv->p, 1, r, 1, 1);
@@ -909,11 +962,17 @@ struct csd_ring {
         ok(csd_hal_assign(ctx, &dst, &src_kept));
         if (dropped.empty()) return;
         int64_t rdr = (int64_t)dropped.size();
-        DevBuf dpart(ctx, (size_t)cnt * rdr * n2 * n3 * 8), tmp(ctx, (size_t)cnt * nl * n2 * n3 * 8);
-        csd_view vd = make_view(dpart.p, cnt, rdr, n2, n3), vt = make_view(tmp.p, cnt, nl, n2, n3);
+        DevBuf tmp(ctx, (size_t)cnt * nl * n2 * n3 * 8);
+        csd_view vt = make_view(tmp.p, cnt, nl, n2, n3);
         csd_view src_dropped = sub(src, 1, nl, r, 8);
-        ok(csd_hal_assign(ctx, &vd, &src_dropped));
-        bconv(vt, kept, vd, dropped);
+        if (fused_bconv_applies(bconv_cache(dropped, kept), rdr, vt, src_dropped)) {
+            bconv(vt, kept, src_dropped, dropped);  // the fused conversion leaves its source unchanged
+        } else {
+            DevBuf dpart(ctx, (size_t)cnt * rdr * n2 * n3 * 8);
+            csd_view vd = make_view(dpart.p, cnt, rdr, n2, n3);
+            ok(csd_hal_assign(ctx, &vd, &src_dropped));
+            bconv(vt, kept, vd, dropped);
+        }
         ok(csd_hal_sub_i64(ctx, &dst, &vt));
         DevBuf& inv = inverse_product_tensor(kept, dropped);
         csd_view vinv = make_view(inv.p, 1, nl, 1, 1);
@@ -1302,6 +1361,12 @@ extern "C" int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, siz
 extern "C" int csd_ring_set_overlap(int on) {
     g_overlap_halves = on != 0;
     return 0;
+}
+
+extern "C" int csd_ring_set_bconv_form(int form) {
+    CSD_TRY if (form != 0 && form != 1) throw CsdError("bconv form must be 0 (HAL sequence) or 1 (fused)");
+    g_bconv_fused = form == 1;
+    CSD_CATCH
 }
 
 extern "C" int csd_ring_set_gadget_form(int form) {

@@ -56,6 +56,13 @@ def set_overlap(on):
     check(lib().csd_ring_set_overlap(ctypes.c_int(1 if on else 0)))
 
 
+def set_bconv_form(form):
+    """How bconv_tensors (and the conversions inside the ring sequences) run
+    (csd_ring_set_bconv_form, process-wide): 1 (default) one fused kernel, 0 the reference's HAL
+    call sequence; residues are identical."""
+    check(lib().csd_ring_set_bconv_form(ctypes.c_int(int(form))))
+
+
 def set_gadget_form(form):
     """How bconv_gadget_tensors runs (csd_ring_set_gadget_form, process-wide): 1 (default) one
     fused kernel, 0 the reference's HAL call sequence per element; residues are identical."""

@@ -60,9 +60,15 @@ int csd_ring_crt_to_pb(csd_ring* ring, const csd_view* data, const size_t* base,
  * VALU-bound dim-3 passes of the other (default on). Every element sees the same passes in the
  * same order, so results are identical either way; 0 turns it off (no reference counterpart). */
 int csd_ring_set_overlap(int on);
-/* bconv_tensors (ring.rs:1526-1535, BConvCache::bconv 1663-1701); destroys src */
+/* bconv_tensors (ring.rs:1526-1535, BConvCache::bconv 1663-1701). The default form is one kernel
+ * that leaves src unchanged (used when src and dst do not overlap); the HAL-sequence form destroys
+ * src, like the reference. */
 int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t* dst_base, size_t r_dst, const csd_view* src,
                    const size_t* src_base, size_t r_src);
+/* 1 (default): csd_ring_bconv and the conversions inside the ring sequences run as one fused
+ * kernel; 0: as the reference's HAL call sequence (ring.rs:1663-1701). Residues are identical
+ * (no reference counterpart). */
+int csd_ring_set_bconv_form(int form);
 /* bconv_gadget_tensors (ring.rs:1548-1557, GadgetBConvCache::bconv 1809-1851); digits are
  * half-open position ranges [digit_bounds[2i], digit_bounds[2i+1]). src is [B, r, ., .] and dst
  * [B ndigits, r, ., .], element b's digit i in row b ndigits + i; the reference takes B = 1, and

@@ -112,7 +112,16 @@ def test_field18_square_kat_on_device(gpu, rings):
             assert int(out[blk * 6 + j]) % p == R.montgomery_repr(exp[j], p)
 
 
-def test_bconv_kat_on_device(gpu):
+@pytest.fixture(params=[1, 0], ids=["fused", "hal_sequence"])
+def bconv_form(request):
+    """runs a test with the fused conversion kernel and with the reference's HAL sequence"""
+    pkg = load()
+    pkg.ring.set_bconv_form(request.param)
+    yield request.param
+    pkg.ring.set_bconv_form(1)
+
+
+def test_bconv_kat_on_device(gpu, bconv_form):
     """ring.rs:1859-1917, on the device."""
     ring = load().ring.RingContext(gpu, "field_2", raw_primes=BCONV_PRIMES)
     frm, to = BCONV_PRIMES[:7], BCONV_PRIMES[7:]
@@ -131,8 +140,10 @@ def test_bconv_kat_on_device(gpu):
     ring.close()
 
 
-def test_bconv_parity(gpu, cpu, rings):
-    """The lift q -> extension of field_36864 (17 -> 20 primes) and its reverse, bit-exact."""
+def test_bconv_parity(gpu, cpu, rings, bconv_form):
+    """The lift q -> extension of field_36864 (17 -> 20 primes) and its reverse, bit-exact; also
+    from a limb slice of a larger tensor into a limb slice (strided views), and with unreduced
+    (-2p, 2p) inputs. The fused form leaves its source unchanged."""
     g, c = rings("field_36864", (960, 980), 76)
     base = g.initial_rns_base()
     ext = g.extension_rns_base_for(base)
@@ -142,11 +153,25 @@ def test_bconv_parity(gpu, cpu, rings):
         primes = [g.primes[i] for i in sb]
         src = uniform_residues(rng, primes, (2, len(sb), 12, 3072))
         src[0, :, 0, :8] = 0  # zero and small values too
+        src[1, :, 1, :64] -= np.array(primes, dtype=np.int64).reshape(-1, 1) * rng.integers(0, 2, (len(sb), 64))
         dc = np.zeros((2, len(db), 12, 3072), dtype=np.int64)
         c.bconv_tensors(dc, db, src.copy(), sb)
         dg = gpu.hal_allocate_i64(dc.shape)
-        g.bconv_tensors(dg, db, to_dev(gpu, src), sb)
+        sg = to_dev(gpu, src)
+        g.bconv_tensors(dg, db, sg, sb)
         assert same(from_dev(gpu, dg), dc)
+        if bconv_form == 1:
+            assert same(from_dev(gpu, sg), src)
+    # strided views: limbs 1.. of a [2, 1 + r, ., .] tensor into limbs 2.. of a [2, 2 + r', ., .] one
+    sb, db = base, ext
+    big = np.zeros((2, 1 + len(sb), 12, 3072), dtype=np.int64)
+    big[:, 1:] = uniform_residues(rng, [g.primes[i] for i in sb], (2, len(sb), 12, 3072))
+    dc = np.zeros((2, len(db), 12, 3072), dtype=np.int64)
+    c.bconv_tensors(dc, db, big[:, 1:].copy(), sb)
+    bg = to_dev(gpu, big)
+    out = gpu.hal_allocate_i64((2, 2 + len(db), 12, 3072))
+    g.bconv_tensors(out[:, 2:], db, bg[:, 1:], sb)
+    assert same(from_dev(gpu, out)[:, 2:], dc)
 
 
 GADGET_CASES = [("field_36864", (960, 980), 76, "select9", 1), ("field_36864", (960, 980), 76, "gaps", 3),
