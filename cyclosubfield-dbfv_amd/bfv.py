@@ -156,14 +156,9 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
 
     lhs_l, lifted = lift(lhs)
     rhs_l, _ = lift(rhs)
-    # hom_mul_raw_acc (bfv.rs:667-683) with the wide rows laid out [3][B]: row k B + b
-    acc = ring.allocate_wide(3 * B, lifted, zero=False)  # every row is written fresh first
-    l0, l1, r0, r1 = _rows(lhs_l, 0, 2), _rows(lhs_l, 1, 2), _rows(rhs_l, 0, 2), _rows(rhs_l, 1, 2)
-    acc.fma(range(0, B), l0, 1, r0, 1)
-    acc.fma(range(B, 2 * B), l0, 1, r1, 1)
-    acc.fma(range(B, 2 * B), l1, 1, r0, 1)
-    acc.fma(range(2 * B, 3 * B), l1, 1, r1, 1)
-    prod = ring.reduce_wide(acc)
+    # hom_mul_raw_acc (bfv.rs:667-683) and reduce_wide as one kernel, rows laid out [3][B]:
+    # component k of pair b in row k B + b (operand rows 2b + s)
+    prod = ring.convolve_reduce(lhs_l, (0, 1, 2), rhs_l, (0, 1, 2), B, 1, 1, 1, lifted)
     ring.crt_to_pb_tensor(prod, lifted)
     rescaled = rescale_to_base(ring, prod, lifted, len(base), t)  # [3][B] over base
     out, _ = relin_batch(ring, rescaled, B, base, keys, digits, "crt")

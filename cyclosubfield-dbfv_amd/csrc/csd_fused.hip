@@ -424,3 +424,107 @@ std::vector<int64_t> csd_bconv_transpose(const std::vector<int64_t>& mat, int64_
             for (int64_t u = 0; u < r; u++) t[(size_t)((i * r + u) * *rdp + j)] = mat[(size_t)((i * (rd + 1) + j) * r + u)];
     return t;
 }
+
+// ---------------------------------------------------------------------------------------------
+// digit convolution and wide reduction in one kernel: the inner products of dbfv_mul_convolve
+// (dbfv.rs:1096-1127), or of hom_mul_raw_acc (bfv.rs:667-683) as its one-digit case, followed by
+// reduce_wide (ring.rs:846-865)
+//
+// For output digit k and pair b the three components are
+//   c = 0: sum_j l_j0 r_(k-j)0,  c = 1: sum_j (l_j0 r_(k-j)1 + l_j1 r_(k-j)0),  c = 2: sum_j l_j1 r_(k-j)1
+// over j = max(0, k - d2 + 1) .. min(k, d1 - 1). Each is formed as the HAL forms it, an i128 sum
+// with wrapping adds (the order of wrapping adds does not change it), and reduced in registers by
+// the kind the wide element's bound picks (red128, or double_red128 and to_mont64), so the
+// int64 rows are those of the accumulate-then-reduce sequence and no i128 row reaches HBM.
+
+struct CvArgs {
+    int64_t n, r, P, n3;    // pairs, limbs, positions n2 n3, dim-3 length
+    int32_t d1, d2, K;
+    int64_t lrow[3], rrow[3];  // row of (digit, component, pair) = digit lrow[0] + comp lrow[1] + pair lrow[2]
+    int64_t sl[4], sr[4], sd[4];
+    const int64_t* lhs;
+    const int64_t* rhs;
+    int64_t* dst;  // row (c K + k) n + pair
+    uint8_t full[3 * CSD_MAX_MODULI];
+};
+
+__global__ __launch_bounds__(256) void k_convolve_reduce(CvArgs a, ModTable mt) {
+    const int64_t chunks = (a.P + 255) / 256;
+    int64_t blk = blockIdx.x;
+    const int64_t c0 = blk % chunks;
+    blk /= chunks;
+    const int64_t jl = blk % a.r, b = blk / a.r;  // limb, pair
+    const int64_t pos = c0 * 256 + threadIdx.x;
+    if (pos >= a.P) return;
+    const int64_t k2 = pos / a.n3, l3 = pos % a.n3;
+    const CsdMod m = mt.m[jl];
+    const int64_t* L = a.lhs + jl * a.sl[1] + k2 * a.sl[2] + l3 * a.sl[3];
+    const int64_t* R = a.rhs + jl * a.sr[1] + k2 * a.sr[2] + l3 * a.sr[3];
+    int64_t* D = a.dst + jl * a.sd[1] + k2 * a.sd[2] + l3 * a.sd[3];
+    for (int k = 0; k < a.K; k++) {
+        i128 s0 = 0, s1 = 0, s2 = 0;
+        const int j_lo = k - (a.d2 - 1) > 0 ? k - (a.d2 - 1) : 0, j_hi = k < a.d1 - 1 ? k : a.d1 - 1;
+        for (int j = j_lo; j <= j_hi; j++) {
+            const int64_t lr = j * a.lrow[0] + b * a.lrow[2], rr = (k - j) * a.rrow[0] + b * a.rrow[2];
+            const int64_t l0 = L[lr * a.sl[0]], l1 = L[(lr + a.lrow[1]) * a.sl[0]];
+            const int64_t r0 = R[rr * a.sr[0]], r1 = R[(rr + a.rrow[1]) * a.sr[0]];
+            s0 = gb_mac(s0, l0, r0);
+            s1 = gb_mac(gb_mac(s1, l0, r1), l1, r0);
+            s2 = gb_mac(s2, l1, r1);
+        }
+        const i128 sums[3] = {s0, s1, s2};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int row = c * a.K + k;
+            const int64_t v = a.full[row] ? csd_to_mont64(m, csd_double_red128(m, sums[c])) : csd_red128(m, sums[c]);
+            D[((int64_t)row * a.n + b) * a.sd[0]] = v;
+        }
+    }
+}
+
+extern "C" int csd_convolve_reduce(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const int64_t* lhs_rows,
+                                   const csd_view* rhs, const int64_t* rhs_rows, size_t count, size_t d1, size_t d2,
+                                   size_t K, const uint8_t* full, const csd_modulus* mods, size_t nmods) {
+    CSD_TRY check_view(dst, "convolve_reduce dst");
+    check_view(lhs, "convolve_reduce lhs");
+    check_view(rhs, "convolve_reduce rhs");
+    if (count == 0 || d1 == 0 || d2 == 0 || K == 0) throw CsdError("convolve_reduce: empty operands");
+    if (K > d1 + d2 - 1 || 3 * K > 3 * (size_t)CSD_MAX_MODULI) throw CsdError("convolve_reduce: too many output digits");
+    CvArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n = (int64_t)count;
+    a.r = dst->shape[1];
+    a.n3 = dst->shape[3];
+    a.P = dst->shape[2] * dst->shape[3];
+    a.d1 = (int32_t)d1;
+    a.d2 = (int32_t)d2;
+    a.K = (int32_t)K;
+    if (dst->shape[0] != (int64_t)(3 * K * count)) throw CsdError("convolve_reduce: dst rows must be 3 K count");
+    auto max_row = [&](const int64_t* rows, size_t d) { return (int64_t)(d - 1) * rows[0] + rows[1] + (int64_t)(count - 1) * rows[2]; };
+    for (int q = 0; q < 3; q++) {
+        if (lhs_rows[q] < 0 || rhs_rows[q] < 0) throw CsdError("convolve_reduce: negative row strides");
+        a.lrow[q] = lhs_rows[q];
+        a.rrow[q] = rhs_rows[q];
+    }
+    if (max_row(lhs_rows, d1) >= lhs->shape[0] || max_row(rhs_rows, d2) >= rhs->shape[0])
+        throw CsdError("convolve_reduce: operand rows out of range");
+    for (int q = 1; q < 4; q++)
+        if (lhs->shape[q] != dst->shape[q] || rhs->shape[q] != dst->shape[q]) throw CsdError("convolve_reduce: shapes differ");
+    for (int q = 0; q < 4; q++) {
+        a.sl[q] = lhs->strides[q];
+        a.sr[q] = rhs->strides[q];
+        a.sd[q] = dst->strides[q];
+    }
+    a.lhs = (const int64_t*)lhs->ptr;
+    a.rhs = (const int64_t*)rhs->ptr;
+    a.dst = (int64_t*)dst->ptr;
+    for (size_t q = 0; q < 3 * K; q++) a.full[q] = full[q] ? 1 : 0;
+    ModTable t = make_mod_table(mods, nmods, a.r);
+    if (a.P == 0) return 0;
+    const int64_t blocks = a.n * a.r * ((a.P + 255) / 256);
+    if (blocks > (int64_t)INT32_MAX) throw CsdError("convolve_reduce: grid too large");
+    TimedScope ts(ctx, "convolve_reduce", 8.0 * (double)a.n * a.r * a.P * (2.0 * (d1 + d2) + 3.0 * K));
+    hipLaunchKernelGGL(k_convolve_reduce, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+    CSD_CATCH
+}

@@ -285,10 +285,12 @@ def _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, base_len, keys, di
     ciphertexts (digit-major: row (2 i + s) count + c is row s of digit i of pair c): the digit
     products as a convolution, one inner product per output digit and component covering all
     pairs, in one wide batch; one reduction, basis transform and rescale for all of them; then
-    one batched relinearization of every output digit of every pair. The wide rows are laid out
-    component-major (row (c d_out + k) count + pair instead of 3 k + c) so that the
-    relinearizations run as one batch; each row sees the same operations and bounds, so the
-    residues are those of the reference's layout. The result is digit-major like the inputs."""
+    one batched relinearization of every output digit of every pair. The inner products and their
+    wide reduction run as one kernel (csd_convolve_reduce), which forms each row's exact i128 sum
+    and reduces it by the kind its WideRingEls bound selects. The rows are laid out component-major
+    (row (c d_out + k) count + pair instead of 3 k + c) so that the relinearizations run as one
+    batch; each row gets the same value, so the residues are those of the reference's layout. The
+    result is digit-major like the inputs."""
     from .bfv import relin_batch, rescale_to_base
 
     be = ring.backend
@@ -299,19 +301,8 @@ def _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, base_len, keys, di
     def blk(t, row):  # rows row n .. row n + n
         return t[row * n:(row + 1) * n]
 
-    acc = ring.allocate_wide(3 * K * n, lifted, zero=False)  # every row is one inner product
-    for k in range(K):
-        t0, t1, t2 = [], [], []
-        for j in range(max(0, k - (d2 - 1)), min(k, d1 - 1) + 1):
-            l0, l1 = blk(lhs_l, 2 * j), blk(lhs_l, 2 * j + 1)
-            r0, r1 = blk(rhs_l, 2 * (k - j)), blk(rhs_l, 2 * (k - j) + 1)
-            t0.append((l0, 1, r0, 1))
-            t1.append((l0, 1, r1, 1))
-            t1.append((l1, 1, r0, 1))
-            t2.append((l1, 1, r1, 1))
-        for c, terms in enumerate((t0, t1, t2)):
-            acc.inner_product(range((c * K + k) * n, (c * K + k + 1) * n), terms)
-    prod = ring.reduce_wide(acc)  # CRT basis
+    # the inner products of every output digit and component and reduce_wide, as one kernel
+    prod = ring.convolve_reduce(lhs_l, (2 * n, n, 1), rhs_l, (2 * n, n, 1), n, d1, d2, K, lifted)  # CRT basis
     ring.crt_to_pb_tensor(prod, lifted)
     rescaled = rescale_to_base(ring, prod, lifted, base_len, modulus.t())  # bound 1
     out, bound = relin_batch(ring, rescaled, K * n, lifted[:base_len], keys, digits, output_basis, bound=1,

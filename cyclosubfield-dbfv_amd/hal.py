@@ -353,6 +353,17 @@ class HipBackend:
         rv = (CsdView * len(rhs))(*[x.view() for x in rhs])
         check(lib().csd_hal_inner_product_i64(self.ctx, ctypes.byref(dst.view()), lv, rv, ctypes.c_size_t(len(lhs))))
 
+    def convolve_reduce(self, dst, lhs, lhs_rows, rhs, rhs_rows, count, d1, d2, K, full, mods):
+        """csd_convolve_reduce: the digit convolution (or one-digit tensor product) and its wide
+        reduction as one kernel; `full` holds the reduction kind of each of the 3 K output rows."""
+        lr = (ctypes.c_int64 * 3)(*[int(x) for x in lhs_rows])
+        rr = (ctypes.c_int64 * 3)(*[int(x) for x in rhs_rows])
+        fl = (ctypes.c_uint8 * len(full))(*[1 if f else 0 for f in full])
+        arr, n = _mods(mods)
+        check(lib().csd_convolve_reduce(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()), lr,
+                                        ctypes.byref(rhs.view()), rr, ctypes.c_size_t(count), ctypes.c_size_t(d1),
+                                        ctypes.c_size_t(d2), ctypes.c_size_t(K), fl, arr, n))
+
     def hal_matmul_i64(self, dst, lhs, rhs):
         check(lib().csd_hal_matmul_i64(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()), ctypes.byref(rhs.view())))
 

@@ -280,6 +280,27 @@ class RingContext:
             hit = self._consts[key] = [self.backend.hal_create_modulus(self.primes[i]) for i in base]
         return hit
 
+    def convolve_reduce(self, lhs, lhs_rows, rhs, rhs_rows, count, d1, d2, K, base, lhs_bound=1, rhs_bound=1):
+        """The digit convolution of dbfv_mul_convolve (dbfv.rs:1096-1127), or with d1 = d2 = K = 1 the
+        tensor product of hom_mul_raw_acc (bfv.rs:667-683), reduced as reduce_wide (ring.rs:846-865)
+        would reduce the WideRingEls rows the reference accumulates: one kernel, the same int64
+        rows. Operand row of (digit i, component s, pair b) is i rows[0] + s rows[1] + b rows[2];
+        the result ([3 K count, r, ., .], CRT basis) holds component c of output digit k of pair b
+        in row (c K + k) count + b."""
+        max_p = max(self.primes[i] for i in base)
+        min_p = min(self.primes[i] for i in base)
+        full = []
+        for c in range(3):
+            for k in range(K):
+                pairs = min(k, d1 - 1) - max(0, k - (d2 - 1)) + 1
+                bound = (2 if c == 1 else 1) * pairs * lhs_bound * rhs_bound * max_p * max_p  # WideRingEls.record_step
+                if bound > (1 << 127) - 1:
+                    raise OverflowError("wide ring element overflow: too many accumulation steps")
+                full.append(bound > (min_p << 63))
+        out = self.allocate_uninit(3 * K * count, base)
+        self.backend.convolve_reduce(out, lhs, lhs_rows, rhs, rhs_rows, count, d1, d2, K, full, self.mods_for(base))
+        return out
+
     def allocate_wide(self, count, base, zero=True):
         """WideRingEls over `base`; with zero=False the i128 rows start unwritten, and reducing a
         row that no operation has written since raises (a zeroed row would reduce to 0)"""

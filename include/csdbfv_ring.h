@@ -79,6 +79,17 @@ int csd_ring_bconv_gadget(csd_ring* ring, const csd_view* dst, const csd_view* s
 /* 1 (default): csd_ring_bconv_gadget runs as one fused kernel; 0: as the reference's HAL call
  * sequence (ring.rs:1809-1851), per element. Residues are identical (no reference counterpart). */
 int csd_ring_set_gadget_form(int form);
+/* The digit convolution of dbfv_mul_convolve (dbfv.rs:1096-1127) and, with d1 = d2 = K = 1, the
+ * tensor product of hom_mul_raw_acc (bfv.rs:667-683), each followed by reduce_wide (ring.rs:846-865),
+ * as one kernel (no reference counterpart; it replaces 3 K inner products or four wide fma calls
+ * plus the reduction, with identical residues and no i128 rows in HBM). Operand row of (digit i,
+ * component s, pair b) is i rows[0] + s rows[1] + b rows[2]; dst row (c K + k) count + b holds
+ * component c of output digit k of pair b, reduced by red128 (full[c K + k] = 0) or by
+ * double_red128 and to_mont64 (full = 1), the kinds the wide rows' bounds select. dst must not
+ * overlap the operands. */
+int csd_convolve_reduce(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const int64_t* lhs_rows,
+                        const csd_view* rhs, const int64_t* rhs_rows, size_t count, size_t d1, size_t d2, size_t K,
+                        const uint8_t* full, const csd_modulus* mods, size_t nmods);
 /* mod_switch_down (bfv.rs:628-662): dst [count, new_len, ., .] from src [count, r, ., .] (period
  * basis); src is left unchanged */
 int csd_ring_mod_switch_down(csd_ring* ring, const csd_view* dst, const csd_view* src, const size_t* base, size_t r,

@@ -272,3 +272,19 @@ def test_dbfv_hom_mul_batch_matches_single(gpu, rings, field, log2_q, log2_b, lo
         one, _ = pkg.dbfv.dbfv_hom_mul(g, mg, to_dev(gpu, lhs[c::count]), 2, to_dev(gpu, rhs[c::count]), 1, base, keys,
                                        digits, basis, fused=False)
         assert same(got[c::count], from_dev(gpu, one)), c
+
+
+def test_convolve_reduce_errors(gpu, rings):
+    """csd_convolve_reduce rejects destinations and operand row maps that do not fit"""
+    pkg = load()
+    g, _ = rings("field_192", (200, 220), 64)
+    base = g.initial_rns_base()
+    shape = (g.info["deg_lprime"], g.info["deg_k"])
+    lhs = gpu.hal_allocate_i64((8, len(base)) + shape)
+    mods = g.mods_for(base)
+    for dst_rows, lrows, d1, K in [(12, (2, 1, 8), 4, 4),     # pair rows beyond the operand
+                                   (11, (2, 1, 0), 4, 4),     # dst rows not 3 K count
+                                   (24, (2, 1, 0), 4, 8)]:    # more output digits than d1 + d2 - 1
+        dst = gpu.hal_allocate_i64((dst_rows, len(base)) + shape)
+        with pytest.raises(pkg.HalError):
+            gpu.convolve_reduce(dst, lhs, lrows, lhs, lrows, 1 if lrows[2] == 0 else 2, d1, 1, K, [0] * (3 * K), mods)
