@@ -170,8 +170,9 @@ def relin_batch(ring, prod, K, base, keys, digits, output_basis, bound=1, interl
     tensor (destroyed) with component c of product k in row c K + k, of bound `bound`. Returns
     (ciphertexts [2K, r, ., .] with product k in rows 2k, 2k + 1, or with interleave=False in
     rows k, K + k; bound): CRT basis (bound 1), or period basis (reduced key-switch part plus
-    c0, c1: bound + 1). One fused gadget conversion covers all K; each product sees the calls and
-    operands of its own relin, so the residues are identical."""
+    c0, c1: bound + 1). One fused gadget conversion covers all K, and one kernel forms the key
+    switch of both key rows with its wide reduction (csd_keyswitch_reduce); each product gets the
+    values of its own relin, so the residues are identical."""
     be = ring.backend
     r = len(base)
     dg = clip_digits(digits, r)
@@ -181,28 +182,37 @@ def relin_batch(ring, prod, K, base, keys, digits, output_basis, bound=1, interl
     parts = ring.allocate_uninit(K * d, base)  # product k, digit i at row k d + i
     ring.bconv_gadget_tensors(parts, prod[2 * K:3 * K], base, dg)  # may destroy rows 2K.. (unused below)
     ring.pb_to_crt_tensor(parts, base)
-    acc = ring.allocate_wide(2 * K, base, zero=False)  # row s K + k
-    for s in (0, 1):  # key_switch_acc (bfv.rs:498-528)
-        be.hal_inner_product_i64(acc.data[s * K:(s + 1) * K], [_rows(parts, i, d) for i in range(d)],
-                                 [keys[i][s:s + 1, :r] for i in range(d)])
-    acc.record_step(range(0, 2 * K), d * acc.max_p * acc.max_p)
+    # key_switch_acc (bfv.rs:498-528) for both key rows, the c0, c1 add of the CRT branch and
+    # reduce_wide as one kernel; the reduction kind follows the WideRingEls bound
+    keys = list(keys[:d])
+    if any(k.strides != keys[0].strides for k in keys):  # the kernel takes one stride set for all keys
+        copies = []
+        for k in keys:  # contiguous copies of the limbs the base uses
+            c = be.allocate_i64_uninit([2, r] + list(k.shape[2:]))
+            be.hal_assign(c, k[:, :r])
+            copies.append(c)
+        keys = copies
+    max_p = max(ring.primes[i] for i in base)
+    min_p = min(ring.primes[i] for i in base)
+    wide_bound = d * max_p * max_p
+    out_rows = (1, 2) if interleave else (K, 1)  # row of (key row s, product k) = s rows[0] + k rows[1]
+    out = ring.allocate_uninit(2 * K, base)
     if output_basis == "crt":
         if bound > 2:
             raise ValueError("basis transforms require representatives bounded by 2p")
         ring.pb_to_crt_tensor(prod[0:2 * K], base)
-        acc.add(range(0, 2 * K), prod[0:2 * K], 1)
-        out = ring.reduce_wide(acc)
-        out_bound = 1
-    elif output_basis == "pb":
-        out = ring.reduce_wide(acc)
-        ring.crt_to_pb_tensor(out, base)
-        be.hal_add_i64(out, prod[0:2 * K])  # add_els (ring.rs:990-1023)
-        out_bound = 1 + bound
-    else:
+        wide_bound += max_p * max_p  # acc.add(rows, c0 c1, 1)
+        be.keyswitch_reduce(out, out_rows, parts, K, d, keys, prod[0:2 * K], ring.scalar_tensor(base, 1),
+                            wide_bound > (min_p << 63), ring.mods_for(base))
+        return out, 1
+    if output_basis != "pb":
         raise ValueError("output_basis must be 'crt' or 'pb'")
+    be.keyswitch_reduce(out, (K, 1), parts, K, d, keys, None, None, wide_bound > (min_p << 63), ring.mods_for(base))
+    ring.crt_to_pb_tensor(out, base)
+    be.hal_add_i64(out, prod[0:2 * K])  # add_els (ring.rs:990-1023)
     if not interleave:
-        return out, out_bound
+        return out, 1 + bound
     result = ring.allocate_uninit(2 * K, base)
     be.hal_assign(_rows(result, 0, 2), out[0:K])
     be.hal_assign(_rows(result, 1, 2), out[K:2 * K])
-    return result, out_bound
+    return result, 1 + bound

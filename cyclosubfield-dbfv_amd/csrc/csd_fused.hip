@@ -528,3 +528,116 @@ extern "C" int csd_convolve_reduce(csd_ctx* ctx, const csd_view* dst, const csd_
     HIPCHECK(hipGetLastError());
     CSD_CATCH
 }
+
+// ---------------------------------------------------------------------------------------------
+// relinearization key switch in one kernel: key_switch_acc's inner products (bfv.rs:498-528) for
+// both key rows, relin's acc.add of c0, c1 (bfv.rs:722-726, CRT output) and reduce_wide
+// (ring.rs:846-865)
+//
+// For product k, key row s: sum_i parts[k d + i] * key_i[s] (+ prod[s K + k] * one), an i128 with
+// wrapping adds as the HAL forms it, reduced in registers by the kind the WideRingEls bound
+// selects. Each digit row of `parts` is read once for both key rows.
+
+struct KsArgs {
+    int64_t K, d, r, P, n3;
+    int64_t sp[4], sk[4], sa[4], so[4];  // parts [K d, r, ...], keys [2, r_key, ...], add [2K, r, ...], out
+    int64_t orow_s, orow_k;              // out row of (s, k) = s orow_s + k orow_k
+    const int64_t* parts;
+    const int64_t* keys[CSD_MAX_MODULI];
+    const int64_t* add;  // prod rows s K + k, or null
+    const int64_t* one;  // [r] the multiplier of `add` (the WideRingEls one), or null
+    int64_t* out;
+    int full;
+};
+
+__global__ __launch_bounds__(256) void k_keyswitch_reduce(KsArgs a, ModTable mt) {
+    const int64_t chunks = (a.P + 255) / 256;
+    int64_t blk = blockIdx.x;
+    const int64_t c0 = blk % chunks;
+    blk /= chunks;
+    const int64_t j = blk % a.r, k = blk / a.r;
+    const int64_t pos = c0 * 256 + threadIdx.x;
+    if (pos >= a.P) return;
+    const int64_t p2 = pos / a.n3, p3 = pos % a.n3;
+    const CsdMod m = mt.m[j];
+    const int64_t* P = a.parts + k * a.d * a.sp[0] + j * a.sp[1] + p2 * a.sp[2] + p3 * a.sp[3];
+    const int64_t koff = j * a.sk[1] + p2 * a.sk[2] + p3 * a.sk[3];
+    i128 s0 = 0, s1 = 0;
+    for (int i = 0; i < (int)a.d; i++) {
+        const int64_t x = P[i * a.sp[0]];
+        const int64_t* key = a.keys[i] + koff;
+        s0 = gb_mac(s0, x, key[0]);
+        s1 = gb_mac(s1, x, key[a.sk[0]]);
+    }
+    if (a.add) {
+        const int64_t* A = a.add + j * a.sa[1] + p2 * a.sa[2] + p3 * a.sa[3];
+        const int64_t one = a.one[j];
+        s0 = gb_mac(s0, A[k * a.sa[0]], one);
+        s1 = gb_mac(s1, A[(a.K + k) * a.sa[0]], one);
+    }
+    int64_t* O = a.out + j * a.so[1] + p2 * a.so[2] + p3 * a.so[3];
+    const i128 sums[2] = {s0, s1};
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+        O[(s * a.orow_s + k * a.orow_k) * a.so[0]] = a.full ? csd_to_mont64(m, csd_double_red128(m, sums[s]))
+                                                             : csd_red128(m, sums[s]);
+}
+
+extern "C" int csd_keyswitch_reduce(csd_ctx* ctx, const csd_view* out, const int64_t* out_rows, const csd_view* parts,
+                                    size_t K, size_t d, const csd_view* keys, const csd_view* add, const csd_view* one,
+                                    int full, const csd_modulus* mods, size_t nmods) {
+    CSD_TRY check_view(out, "keyswitch_reduce out");
+    check_view(parts, "keyswitch_reduce parts");
+    if (K == 0 || d == 0 || d > CSD_MAX_MODULI) throw CsdError("keyswitch_reduce: 1 <= digits <= CSD_MAX_MODULI and K >= 1");
+    KsArgs a;
+    memset(&a, 0, sizeof(a));
+    a.K = (int64_t)K;
+    a.d = (int64_t)d;
+    a.r = out->shape[1];
+    a.n3 = out->shape[3];
+    a.P = out->shape[2] * out->shape[3];
+    if (parts->shape[0] != (int64_t)(K * d)) throw CsdError("keyswitch_reduce: parts rows must be K x digits");
+    for (int q = 1; q < 4; q++)
+        if (parts->shape[q] != out->shape[q]) throw CsdError("keyswitch_reduce: parts shape");
+    if (out_rows[0] < 0 || out_rows[1] < 0 || out_rows[0] + (int64_t)(K - 1) * out_rows[1] >= out->shape[0])
+        throw CsdError("keyswitch_reduce: output rows out of range");
+    a.orow_s = out_rows[0];
+    a.orow_k = out_rows[1];
+    for (int q = 0; q < 4; q++) {
+        a.sp[q] = parts->strides[q];
+        a.so[q] = out->strides[q];
+        a.sk[q] = keys[0].strides[q];
+    }
+    for (size_t i = 0; i < d; i++) {
+        check_view(&keys[i], "keyswitch_reduce key");
+        if (keys[i].shape[0] != 2 || keys[i].shape[1] < a.r || keys[i].shape[2] != out->shape[2] ||
+            keys[i].shape[3] != out->shape[3])
+            throw CsdError("keyswitch_reduce: a key is not [2, >= r, ., .]");
+        for (int q = 0; q < 4; q++)
+            if (keys[i].strides[q] != a.sk[q]) throw CsdError("keyswitch_reduce: keys must share their strides");
+        a.keys[i] = (const int64_t*)keys[i].ptr;
+    }
+    if (add) {
+        check_view(add, "keyswitch_reduce add");
+        check_view(one, "keyswitch_reduce one");
+        if (add->shape[0] != (int64_t)(2 * K) || add->shape[1] != a.r || add->shape[2] != out->shape[2] ||
+            add->shape[3] != out->shape[3])
+            throw CsdError("keyswitch_reduce: add must be [2 K, r, ., .]");
+        if (one->shape[0] != 1 || one->shape[1] != a.r || one->shape[2] != 1 || one->shape[3] != 1 || one->strides[1] != 1)
+            throw CsdError("keyswitch_reduce: one must be a contiguous [1, r, 1, 1] tensor");
+        for (int q = 0; q < 4; q++) a.sa[q] = add->strides[q];
+        a.add = (const int64_t*)add->ptr;
+        a.one = (const int64_t*)one->ptr;
+    }
+    a.parts = (const int64_t*)parts->ptr;
+    a.out = (int64_t*)out->ptr;
+    a.full = full ? 1 : 0;
+    ModTable t = make_mod_table(mods, nmods, a.r);
+    if (a.P == 0) return 0;
+    const int64_t blocks = a.K * a.r * ((a.P + 255) / 256);
+    if (blocks > (int64_t)INT32_MAX) throw CsdError("keyswitch_reduce: grid too large");
+    TimedScope ts(ctx, "keyswitch_reduce", 8.0 * (double)a.K * a.r * a.P * ((double)d + 2.0 + (add ? 2.0 : 0.0)));
+    hipLaunchKernelGGL(k_keyswitch_reduce, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+    CSD_CATCH
+}

@@ -364,6 +364,18 @@ class HipBackend:
                                         ctypes.byref(rhs.view()), rr, ctypes.c_size_t(count), ctypes.c_size_t(d1),
                                         ctypes.c_size_t(d2), ctypes.c_size_t(K), fl, arr, n))
 
+    def keyswitch_reduce(self, out, out_rows, parts, K, d, keys, add, one, full, mods):
+        """csd_keyswitch_reduce: relin's key-switch inner products for both key rows (plus c0, c1
+        times `one` when `add` is given) and their wide reduction as one kernel."""
+        rows = (ctypes.c_int64 * 2)(*[int(x) for x in out_rows])
+        kv = (CsdView * len(keys))(*[k.view() for k in keys])
+        arr, n = _mods(mods)
+        check(lib().csd_keyswitch_reduce(self.ctx, ctypes.byref(out.view()), rows, ctypes.byref(parts.view()),
+                                         ctypes.c_size_t(K), ctypes.c_size_t(d), kv,
+                                         ctypes.byref(add.view()) if add is not None else None,
+                                         ctypes.byref(one.view()) if one is not None else None,
+                                         ctypes.c_int(1 if full else 0), arr, n))
+
     def hal_matmul_i64(self, dst, lhs, rhs):
         check(lib().csd_hal_matmul_i64(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()), ctypes.byref(rhs.view())))
 

@@ -90,6 +90,15 @@ int csd_ring_set_gadget_form(int form);
 int csd_convolve_reduce(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const int64_t* lhs_rows,
                         const csd_view* rhs, const int64_t* rhs_rows, size_t count, size_t d1, size_t d2, size_t K,
                         const uint8_t* full, const csd_modulus* mods, size_t nmods);
+/* relin's key switch (bfv.rs:498-528, 699-741) for K products as one kernel (no reference
+ * counterpart): for key row s and product k, sum_i parts[k d + i] keys[i][s] (keys[i] a [2, >= r,
+ * ., .] CRT-basis key restricted to the first r limbs), plus add[s K + k] one when `add` is given
+ * (the CRT-basis c0, c1 with the WideRingEls constant one, [1, r, 1, 1]), reduced as reduce_wide
+ * would (full = 1: double_red128 and to_mont64; 0: red128), into out row s out_rows[0] +
+ * k out_rows[1]. Residues are those of the inner products, the wide add and the reduction. */
+int csd_keyswitch_reduce(csd_ctx* ctx, const csd_view* out, const int64_t* out_rows, const csd_view* parts, size_t K,
+                         size_t d, const csd_view* keys, const csd_view* add, const csd_view* one, int full,
+                         const csd_modulus* mods, size_t nmods);
 /* mod_switch_down (bfv.rs:628-662): dst [count, new_len, ., .] from src [count, r, ., .] (period
  * basis); src is left unchanged */
 int csd_ring_mod_switch_down(csd_ring* ring, const csd_view* dst, const csd_view* src, const size_t* base, size_t r,

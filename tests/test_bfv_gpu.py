@@ -131,3 +131,34 @@ def test_unzeroed_wide_rows_must_be_written(gpu, rings):
     acc.add(range(1, 2), x, 1)
     out = from_dev(gpu, g.reduce_wide(acc))
     assert same(out[0], out[1])
+
+
+def test_relin_batch_key_layouts(gpu, rings):
+    """relin_batch gives the same residues when the key digits are views with different strides
+    (one a limb prefix of a larger key), in both output bases; and csd_keyswitch_reduce rejects
+    operands that do not fit"""
+    pkg = load()
+    g, _ = rings("field_192", (100, 120), 8)
+    base = g.initial_rns_base()
+    r = len(base)
+    primes = [g.primes[i] for i in base]
+    n2, n3 = g.info["deg_lprime"], g.info["deg_k"]
+    rng = np.random.default_rng(77)
+    digits = R.select_digits(2, r)
+    K = 3
+    prod = uniform_residues(rng, primes, (3 * K, r, n2, n3))
+    keys = [uniform_residues(rng, primes, (2, r, n2, n3)) for _ in digits]
+    wide = np.zeros((2, r + 2, n2, n3), dtype=np.int64)
+    wide[:, :r] = keys[1]
+    for basis in ("crt", "pb"):
+        same_layout, _ = pkg.bfv.relin_batch(g, to_dev(gpu, prod), K, base, [to_dev(gpu, k) for k in keys], digits, basis)
+        mixed = [to_dev(gpu, keys[0]), to_dev(gpu, wide)]  # the second key has r + 2 limbs
+        other, _ = pkg.bfv.relin_batch(g, to_dev(gpu, prod), K, base, mixed, digits, basis)
+        assert same(from_dev(gpu, same_layout), from_dev(gpu, other))
+    parts = gpu.hal_allocate_i64((K * 2, r, n2, n3))
+    out = gpu.hal_allocate_i64((2 * K, r, n2, n3))
+    kd = [to_dev(gpu, k) for k in keys]
+    with pytest.raises(pkg.HalError):  # parts rows are not K x digits
+        gpu.keyswitch_reduce(out, (K, 1), parts, K, 3, kd + kd[:1], None, None, False, g.mods_for(base))
+    with pytest.raises(pkg.HalError):  # output rows out of range
+        gpu.keyswitch_reduce(out, (K, 2), parts, K, 2, kd, None, None, False, g.mods_for(base))
