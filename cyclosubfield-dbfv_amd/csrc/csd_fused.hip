@@ -323,13 +323,40 @@ __device__ __forceinline__ i128 gb_mac(i128 acc, int64_t a, int64_t b) {
     return (i128)((u128)acc + (((u128)(uint64_t)hi << 64) | lo));
 }
 
-#define GB_GROUP 8  // target limbs per pass over the digit's source limbs (2 and 4 measured slower)
+#define GB_GROUP 8  // target limbs per pass over the digit's source limbs
+
+// The sums sum_u w_u x_u of the conversions are exact integers: every weight is a smallest lift
+// mod a ring prime (|w| < 2^56) and every lifted limb a Montgomery product (|x| <= N < 2^57), so
+// with w = w1 2^28 + w0 and x = x1 2^29 + x0 (w0 in [0, 2^28), x0 in [0, 2^29)) each of the four
+// partial sums over at most 64 limbs stays below 2^63 in magnitude. They accumulate in int64 with
+// one signed 32 x 32 -> 64 multiply-add each, and the exact i128 sum (below 2^119, so equal to
+// the HAL's wrapping one) is put together once per target limb.
+struct GbSplit {
+    int64_t a00, a01, a10, a11;  // sum w0 x0, w0 x1, w1 x0, w1 x1
+};
+
+__device__ __forceinline__ void gb_split_mac(GbSplit& s, int64_t w, int32_t x0, int32_t x1) {
+    const int32_t w0 = (int32_t)(w & ((1 << 28) - 1)), w1 = (int32_t)(w >> 28);
+    s.a00 += (int64_t)w0 * x0;
+    s.a01 += (int64_t)w0 * x1;
+    s.a10 += (int64_t)w1 * x0;
+    s.a11 += (int64_t)w1 * x1;
+}
+
+__device__ __forceinline__ i128 gb_split_sum(const GbSplit& s) {
+    return (i128)s.a00 + ((i128)s.a01 << 29) + ((i128)s.a10 << 28) + ((i128)s.a11 << 57);
+}
 
 // One thread per (element, position). Launched only when every digit's source limbs fit one i128
 // chunk, which the ring's primes (below 2^57) guarantee; so each target limb's sum is reduced once.
-// Every pass over GB_GROUP target limbs lifts the digit's source limbs again (the source stays in
-// L2); parking them in LDS instead measured the same, and costs occupancy for long digits.
+// The digit's source limbs are loaded eight at a time (so that their loads overlap), lifted, and
+// parked in the thread's LDS column for the passes over GB_GROUP target limbs. SPLIT accumulates
+// in the four int64 partial sums (worth it for long digits; short ones pay more for the final
+// i128 assembly than they save).
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
+    extern __shared__ int64_t gb_x[];
+    int64_t* xs = gb_x + threadIdx.x;
     const int64_t chunks = (a.P + 255) / 256;
     const int64_t b = blockIdx.x / chunks;
     const int64_t pos = (blockIdx.x % chunks) * 256 + threadIdx.x;
@@ -339,34 +366,55 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
     int64_t* o = a.dst + b * a.d * a.sd[0] + k * a.sd[2] + l * a.sd[3];
     const int r = (int)a.r, rd = (int)a.rd;
     const i128 half = (i128)1 << (a.shift - 1);
-    auto lift = [&](int u) {
-        const CsdMod mu = a.smods ? a.smods[u] : mt.m[u];
-        return csd_mont_mul(mu, s[u * a.ss[1]], a.dopi[u]);
-    };
     for (int i = 0; i < (int)a.d; i++) {
         const int lo = a.lo[i], hi = a.hi[i];
         const int64_t* corr_row = a.mat + ((int64_t)i * (rd + 1) + rd) * r;
         const int64_t* wi = a.matT + (int64_t)i * r * a.rdp;
-        int64_t q = 0;
+        i128 corr = 0;
+        for (int u0 = lo; u0 < hi; u0 += 8) {
+            int64_t raw[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                if (u0 + t < hi) raw[t] = s[(u0 + t) * a.ss[1]];
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                if (u0 + t < hi) {
+                    const int u = u0 + t;
+                    const CsdMod mu = a.smods ? a.smods[u] : mt.m[u];
+                    const int64_t xv = csd_mont_mul(mu, raw[t], a.dopi[u]);
+                    xs[(u - lo) * 256] = xv;
+                    corr = gb_mac(corr, corr_row[u], xv);
+                }
+        }
+        const int64_t q = (int64_t)((i128)((u128)corr + (u128)half) >> a.shift);  // |q| <= 2^63
         for (int j0 = 0; j0 < rd; j0 += GB_GROUP) {
-            i128 sum[GB_GROUP];
+            GbSplit ssum[GB_GROUP];
+            i128 wsum[GB_GROUP];
 #pragma unroll
-            for (int g = 0; g < GB_GROUP; g++) sum[g] = 0;
-            i128 corr = 0;
-            for (int u = lo; u < hi; u++) {
-                const int64_t xv = lift(u);
-                if (j0 == 0) corr = gb_mac(corr, corr_row[u], xv);
-                const int64_t* w = wi + (int64_t)u * a.rdp + j0;
-#pragma unroll
-                for (int g = 0; g < GB_GROUP; g++)
-                    if (j0 + g < rd) sum[g] = gb_mac(sum[g], w[g], xv);
+            for (int g = 0; g < GB_GROUP; g++) {
+                ssum[g] = GbSplit{0, 0, 0, 0};
+                wsum[g] = 0;
             }
-            if (j0 == 0) q = (int64_t)((i128)((u128)corr + (u128)half) >> a.shift);  // |q| <= 2^63
+            for (int u = lo; u < hi; u++) {
+                const int64_t xv = xs[(u - lo) * 256];
+                const int64_t* w = wi + (int64_t)u * a.rdp + j0;
+                if (SPLIT) {
+                    const int32_t x0 = (int32_t)(xv & ((1 << 29) - 1)), x1 = (int32_t)(xv >> 29);
+#pragma unroll
+                    for (int g = 0; g < GB_GROUP; g++)
+                        if (j0 + g < rd) gb_split_mac(ssum[g], w[g], x0, x1);
+                } else {
+#pragma unroll
+                    for (int g = 0; g < GB_GROUP; g++)
+                        if (j0 + g < rd) wsum[g] = gb_mac(wsum[g], w[g], xv);
+                }
+            }
 #pragma unroll
             for (int g = 0; g < GB_GROUP; g++) {
                 const int j = j0 + g;
                 if (j < rd) {
-                    int64_t v = csd_double_red128(mt.m[j], sum[g]);
+                    const i128 sum = SPLIT ? gb_split_sum(ssum[g]) : wsum[g];
+                    int64_t v = csd_double_red128(mt.m[j], sum);
                     v = (int64_t)((uint64_t)v + (uint64_t)csd_mont_mul(mt.m[j], q, a.dmp[(int64_t)i * rd + j]));
                     o[i * a.sd[0] + j * a.sd[1]] = csd_to_mont64(mt.m[j], v);
                 }
@@ -411,17 +459,34 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
     if (blocks > (int64_t)INT32_MAX) throw CsdError("fused bconv: grid too large");
     ModTable t = make_mod_table(dmods, (size_t)a.rd, a.rd);
     TimedScope ts(ctx, timer, 8.0 * (double)a.B * a.P * ((double)a.r + (double)d * a.rd));
-    hipLaunchKernelGGL(k_bconv_fused, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
+    int64_t max_len = 0;
+    for (int64_t i = 0; i < d; i++) max_len = std::max<int64_t>(max_len, a.hi[i] - a.lo[i]);
+    const size_t lds = (size_t)max_len * 256 * sizeof(int64_t);  // at most 128 KiB (64 limbs)
+    static bool attr = false;
+    if (!attr) {
+        HIPCHECK(hipFuncSetAttribute((const void*)k_bconv_fused<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        HIPCHECK(hipFuncSetAttribute((const void*)k_bconv_fused<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    if (max_len >= 4)  // measured: 17-20 limb digits gain 20 %, 2-limb gadget digits lose 8 %
+        hipLaunchKernelGGL(k_bconv_fused<true>, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, a, t);
+    else
+        hipLaunchKernelGGL(k_bconv_fused<false>, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, a, t);
     HIPCHECK(hipGetLastError());
 }
 
 // the padded, transposed weight layout csd_bconv_fused_launch reads: out[i][u][j] = mat[i][j][u]
 std::vector<int64_t> csd_bconv_transpose(const std::vector<int64_t>& mat, int64_t d, int64_t rd, int64_t r, int64_t* rdp) {
+    if (r > 64) throw CsdError("fused bconv: at most 64 source limbs");
     *rdp = (rd + GB_GROUP - 1) / GB_GROUP * GB_GROUP;
     std::vector<int64_t> t((size_t)(d * r * *rdp), 0);
     for (int64_t i = 0; i < d; i++)
         for (int64_t j = 0; j < rd; j++)
-            for (int64_t u = 0; u < r; u++) t[(size_t)((i * r + u) * *rdp + j)] = mat[(size_t)((i * (rd + 1) + j) * r + u)];
+            for (int64_t u = 0; u < r; u++) {
+                const int64_t w = mat[(size_t)((i * (rd + 1) + j) * r + u)];
+                if (w >= ((int64_t)1 << 56) || w < -((int64_t)1 << 56)) throw CsdError("fused bconv: weight out of range");
+                t[(size_t)((i * r + u) * *rdp + j)] = w;
+            }
     return t;
 }
 
