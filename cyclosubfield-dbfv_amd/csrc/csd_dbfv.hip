@@ -75,12 +75,24 @@ __global__ __launch_bounds__(256) void k_split_overflow(SoArgs a, ModTable mt) {
             cl0 = C[0];
             if (limbs == 2) cl1 = C[a.sc[1]];
         }
+        // the digit's residues, eight loads in flight at a time, parked in the LDS column
+        if (!a.zero_in) {
+            for (int p0 = 0; p0 < r; p0 += 8) {
+                int64_t raw[8];
+#pragma unroll
+                for (int t = 0; t < 8; t++)
+                    if (p0 + t < r) raw[t] = XI[(p0 + t) * sxi];
+#pragma unroll
+                for (int t = 0; t < 8; t++)
+                    if (p0 + t < r) col[(p0 + t) * stride] = raw[t];
+            }
+        }
         // pass 1: the carry step (dbfv.rs:995-1003), then the lifts and the matmul rows
         // (dbfv.rs:909-917)
         u128 s0 = 0, s1 = 0, sg = 0;
         for (int p = 0; p < r; p++) {
             const CsdMod m = mt.m[p];
-            int64_t xv = a.zero_in ? 0 : XI[p * sxi];
+            int64_t xv = a.zero_in ? 0 : col[p * stride];
             if (a.cin) {
                 // wide fma of each carry limb with mont^2(Delta 2^64k) (mul for the first, fma
                 // after), add of the digit with mont(1), reduce_wide

@@ -628,7 +628,22 @@ __global__ __launch_bounds__(256) void k_keyswitch_reduce(KsArgs a, ModTable mt)
     const int64_t* P = a.parts + k * a.d * a.sp[0] + j * a.sp[1] + p2 * a.sp[2] + p3 * a.sp[3];
     const int64_t koff = j * a.sk[1] + p2 * a.sk[2] + p3 * a.sk[3];
     i128 s0 = 0, s1 = 0;
-    for (int i = 0; i < (int)a.d; i++) {
+    int i = 0;
+    for (; i + 4 <= (int)a.d; i += 4) {  // four digits' loads in flight together
+        int64_t x[4], k0[4], k1[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            x[t] = P[(i + t) * a.sp[0]];
+            k0[t] = a.keys[i + t][koff];
+            k1[t] = a.keys[i + t][koff + a.sk[0]];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            s0 = gb_mac(s0, x[t], k0[t]);
+            s1 = gb_mac(s1, x[t], k1[t]);
+        }
+    }
+    for (; i < (int)a.d; i++) {
         const int64_t x = P[i * a.sp[0]];
         const int64_t* key = a.keys[i] + koff;
         s0 = gb_mac(s0, x, key[0]);
