@@ -465,7 +465,7 @@ struct GadgetCache {  // ring.rs:1714-1727
     int64_t rdp = 0;
 };
 
-static bool g_overlap_halves = true;  // see csd_ring_set_overlap
+static bool g_overlap_halves = false;  // see csd_ring_set_overlap
 static bool g_gadget_fused = true;    // see csd_ring_set_gadget_form
 static bool g_bconv_fused = true;     // see csd_ring_set_bconv_form
 

@@ -51,8 +51,8 @@ def sample_primes(name, log2_q, t_bits):
 
 
 def set_overlap(on):
-    """Two-stream half-batch overlap of the transforms (csd_ring_set_overlap); results are
-    identical either way."""
+    """Two-stream half-batch overlap of the transforms (csd_ring_set_overlap, off by default);
+    results are identical either way."""
     check(lib().csd_ring_set_overlap(ctypes.c_int(1 if on else 0)))
 
 

@@ -55,10 +55,11 @@ int csd_ring_extension_base(csd_ring* ring, const size_t* base, size_t r, size_t
 /* pb_to_crt_tensor / crt_to_pb_tensor (ring.rs:1117-1161), in place, plain form (four HAL calls). */
 int csd_ring_pb_to_crt(csd_ring* ring, const csd_view* data, const size_t* base, size_t r);
 int csd_ring_crt_to_pb(csd_ring* ring, const csd_view* data, const size_t* base, size_t r);
-/* Batches of at least 2 elements and 64 MiB are transformed as two halves on two streams, the
- * second half one pass behind, so that the memory-bound dim-2 passes of one half overlap the
- * VALU-bound dim-3 passes of the other (default on). Every element sees the same passes in the
- * same order, so results are identical either way; 0 turns it off (no reference counterpart). */
+/* 1: batches of at least 2 elements and 64 MiB are transformed as two halves on two streams, the
+ * second half one pass behind, so that the dim-2 passes of one half overlap the dim-3 passes of
+ * the other. Every element sees the same passes in the same order, so results are identical
+ * either way. Default 0 (serial): with both pass kinds VALU-heavy the overlap measures 1-3 %
+ * slower (no reference counterpart). */
 int csd_ring_set_overlap(int on);
 /* bconv_tensors (ring.rs:1526-1535, BConvCache::bconv 1663-1701). The default form is one kernel
  * that leaves src unchanged (used when src and dst do not overlap); the HAL-sequence form destroys

@@ -279,7 +279,7 @@ def test_transform_overlap_halves(gpu, cpu, rings):
             g.crt_to_pb_tensor(dg, base)
             outs.append(from_dev(gpu, dg))
         finally:
-            pkg.ring.set_overlap(True)
+            pkg.ring.set_overlap(False)
     assert same(outs[0], outs[1])
     dc = data[[0, 13]].copy()
     c.pb_to_crt_tensor(dc, base)
