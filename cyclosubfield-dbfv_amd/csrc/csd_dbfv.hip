@@ -199,11 +199,7 @@ extern "C" int csd_dbfv_split_overflow(csd_ctx* ctx, const csd_view* x, const cs
     TimedScope ts(ctx, "dbfv_split_overflow",
                   8.0 * (double)a.count * a.n2 * a.n3 * ((zero_in ? 1.0 : 2.0) * a.r + limbs + (carry_in ? limbs : 0)));
     const size_t lds = (size_t)a.r * 256 * sizeof(int64_t);  // at most 128 KiB (64 primes)
-    static bool attr = false;
-    if (!attr) {
-        HIPCHECK(hipFuncSetAttribute((const void*)k_split_overflow, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr = true;
-    }
+    csd_allow_lds((const void*)k_split_overflow, 160 * 1024);
     const int64_t total = a.count * a.n2 * a.n3;
     if ((total + 255) / 256 > (int64_t)INT32_MAX) throw CsdError("dbfv_split_overflow: too many coefficients for one launch");
     hipLaunchKernelGGL(k_split_overflow, dim3((unsigned)((total + 255) / 256)), dim3(256), lds, ctx->stream, a, t);

@@ -462,12 +462,8 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
     int64_t max_len = 0;
     for (int64_t i = 0; i < d; i++) max_len = std::max<int64_t>(max_len, a.hi[i] - a.lo[i]);
     const size_t lds = (size_t)max_len * 256 * sizeof(int64_t);  // at most 128 KiB (64 limbs)
-    static bool attr = false;
-    if (!attr) {
-        HIPCHECK(hipFuncSetAttribute((const void*)k_bconv_fused<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        HIPCHECK(hipFuncSetAttribute((const void*)k_bconv_fused<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr = true;
-    }
+    csd_allow_lds((const void*)k_bconv_fused<true>, 160 * 1024);
+    csd_allow_lds((const void*)k_bconv_fused<false>, 160 * 1024);
     if (max_len >= 4)  // measured: 17-20 limb digits gain 20 %, 2-limb gadget digits lose 8 %
         hipLaunchKernelGGL(k_bconv_fused<true>, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, a, t);
     else

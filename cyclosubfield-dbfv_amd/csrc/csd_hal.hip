@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -277,6 +278,17 @@ Str4 own_strides(const csd_view* v) {
     Str4 s;
     for (int d = 0; d < 4; d++) s.s[d] = v->shape[d] == 1 ? 0 : v->strides[d];
     return s;
+}
+
+void csd_allow_lds(const void* fn, int bytes) {
+    static std::mutex mu;
+    static std::set<std::pair<const void*, int>> done;
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if (done.count({fn, dev})) return;
+    HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    done.insert({fn, dev});
 }
 
 ModTable make_mod_table(const csd_modulus* mods, size_t nmods, int64_t d1) {

@@ -136,6 +136,9 @@ void check_view(const csd_view* v, const char* what);
 Str4 bcast_strides(const csd_view* src, const csd_view* dst, const char* what);
 Str4 own_strides(const csd_view* v);
 ModTable make_mod_table(const csd_modulus* mods, size_t nmods, int64_t d1);
+// Raises kernel `fn`'s dynamic-LDS limit to `bytes` on the current device, once per (kernel,
+// device): the attribute is per device, and this keeps first calls from several threads safe.
+void csd_allow_lds(const void* fn, int bytes);
 void csd_zero_fill(csd_ctx* ctx, void* p, size_t bytes);
 
 // host-side modular helpers (plan and constant construction)

@@ -12,6 +12,8 @@
 // control flow. The kernel therefore produces the reference's representatives bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -1356,10 +1358,10 @@ static ColKernel pick_col_kernel(const DevPlan& P, bool inv) {
     }
 }
 
-static bool g_use_generic_ntt = false;  // forces the one-stage-per-LDS-pass kernel (tests)
-static bool g_use_plan_rows = false;    // keeps dim-3 transforms on the plan-driven row kernel (tests)
-static int g_row_lines = 0;             // dim-3 lines per workgroup, 0 = automatic (csd_ntt_set_row_shape)
-static int g_row_threads = 0;           // dim-3 threads per workgroup, 0 = automatic
+static std::atomic<bool> g_use_generic_ntt{false};  // forces the one-stage-per-LDS-pass kernel (tests)
+static std::atomic<bool> g_use_plan_rows{false};    // keeps dim-3 transforms on the plan-driven row kernel (tests)
+static std::atomic<int> g_row_lines{0};             // dim-3 lines per workgroup, 0 = automatic (csd_ntt_set_row_shape)
+static std::atomic<int> g_row_threads{0};           // dim-3 threads per workgroup, 0 = automatic
 
 extern "C" int csd_ntt_set_row_shape(int lines, int threads) {
     if (lines < 0 || lines > 8 || threads < 0 || threads > 512 || threads % 64 != 0) {
@@ -1386,15 +1388,10 @@ typedef void (*RowKernel)(RowArgs, ModTable);
 // the dim-3 row kernel for these plans: a compiled shape when every plan matches one, else the
 // plan-driven form; the dynamic LDS limit is raised once per kernel
 static RowKernel pick_row_kernel(csd_ntt_plan* const* plans, size_t nplans, int dir, int* shape_out = nullptr) {
-    static bool attr = false;
     static const RowKernel ks[2][3] = {{k_ntt_rows<false, 0>, k_ntt_rows<false, 1>, k_ntt_rows<false, 2>},
                                        {k_ntt_rows<true, 0>, k_ntt_rows<true, 1>, k_ntt_rows<true, 2>}};
-    if (!attr) {
-        for (auto& row : ks)
-            for (RowKernel k : row)
-                HIPCHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr = true;
-    }
+    for (auto& row : ks)
+        for (RowKernel k : row) csd_allow_lds((const void*)k, 160 * 1024);
     int shape = 0;
     if (!g_use_plan_rows && nplans > 0) {
         for (int c : {1, 2}) {
@@ -1560,11 +1557,7 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
     if (blocks > (int64_t)INT32_MAX) throw CsdError("ntt: too many tiles");
     size_t lds = (size_t)L * V * 8;
     if (lds > 160 * 1024) throw CsdError("ntt: transform too long for LDS staging");
-    static bool attr_set = false;
-    if (!attr_set) {
-        HIPCHECK(hipFuncSetAttribute((const void*)k_ntt, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
-    }
+    csd_allow_lds((const void*)k_ntt, 160 * 1024);
     static const char* gnames[2][3] = {{"generic_ntt_dim2", "generic_ntt_mul_dim2", "generic_intt_dim2"},
                                        {"generic_ntt_dim3", "generic_ntt_mul_dim3", "generic_intt_dim3"}};
     TimedScope ts(ctx, gnames[dim - 2][dir], 16.0 * (double)numel(dst));
@@ -1791,11 +1784,7 @@ bool csd_batch_chain_supported(csd_ntt_plan* const* plp, csd_ntt_plan* const* pk
 
 // workgroups of the persistent chain that fit one CU at once with `rows` lines of LDS
 int csd_batch_chain_occupancy(int rows) {
-    static bool attr = false;
-    if (!attr) {
-        HIPCHECK(hipFuncSetAttribute((const void*)k_batch_chain<4, 3, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr = true;
-    }
+    csd_allow_lds((const void*)k_batch_chain<4, 3, 1>, 160 * 1024);
     int n = 0;
     HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_chain<4, 3, 1>, 256, (size_t)rows * ct_line(3072) * 8));
     return std::max(n, 1);
@@ -1825,11 +1814,7 @@ void csd_batch_chain_launch(csd_ctx* ctx, int64_t* const* seg_base, const int32_
     a.rows = rows;
     ModTable t = make_mod_table(mods, R, (int64_t)R);
     const size_t lds = (size_t)rows * ct_line(3072) * 8;
-    static bool attr = false;
-    if (!attr) {
-        HIPCHECK(hipFuncSetAttribute((const void*)k_batch_chain<4, 3, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr = true;
-    }
+    csd_allow_lds((const void*)k_batch_chain<4, 3, 1>, 160 * 1024);
     if (nslices == 0) return;
     TimedScope ts(ctx, "batch_chain_persistent", bytes);
     hipLaunchKernelGGL((k_batch_chain<4, 3, 1>), dim3((unsigned)std::min<int64_t>(grid, nslices)), dim3(256), lds, ctx->stream, a, t);

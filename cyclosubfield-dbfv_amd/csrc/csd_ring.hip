@@ -7,6 +7,8 @@
 // reference's HAL call sequences (ring.rs:1117-1161, 1663-1701, 1809-1851; bfv.rs:628-662).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -465,9 +467,9 @@ struct GadgetCache {  // ring.rs:1714-1727
     int64_t rdp = 0;
 };
 
-static bool g_overlap_halves = false;  // see csd_ring_set_overlap
-static bool g_gadget_fused = true;    // see csd_ring_set_gadget_form
-static bool g_bconv_fused = true;     // see csd_ring_set_bconv_form
+static std::atomic<bool> g_overlap_halves{false};  // see csd_ring_set_overlap
+static std::atomic<bool> g_gadget_fused{true};    // see csd_ring_set_gadget_form
+static std::atomic<bool> g_bconv_fused{true};     // see csd_ring_set_bconv_form
 
 // whether the memory spans of two views (first to last element) intersect
 static bool views_overlap(const csd_view& a, const csd_view& b, int elem) {

@@ -16,6 +16,8 @@
 // The plain form computes the same sums with 64 x 64 -> 128-bit products on the VALU.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -462,7 +464,7 @@ __global__ __launch_bounds__(256) void k_zp_mfma_wide(MfmaArgs a, ModTable mt) {
     }
 }
 
-static int g_zp_kernel = 1;  // the MFMA kernel form 0 uses: 0 L2 operands, 1 LDS 32 x 64, 2 LDS 64 x 64
+static std::atomic<int> g_zp_kernel{1};  // the MFMA kernel form 0 uses: 0 L2 operands, 1 LDS 32 x 64, 2 LDS 64 x 64
 
 extern "C" int csd_zp_set_kernel(int form) {
     if (form < 0 || form > 2) {
@@ -685,19 +687,11 @@ extern "C" int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* z, const csd_
             a.mtiles = z->mpad / 64;
             a.nblocks = z->r * a.mtiles * a.ntiles;
             const size_t lds = 2 * (size_t)(ZPW_A_BYTES + ZPW_X_BYTES);
-            static bool attr = false;
-            if (!attr) {
-                HIPCHECK(hipFuncSetAttribute((const void*)k_zp_mfma_wide, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                attr = true;
-            }
+            csd_allow_lds((const void*)k_zp_mfma_wide, (int)lds);
             hipLaunchKernelGGL(k_zp_mfma_wide, dim3((unsigned)a.nblocks), dim3(ZPW_THREADS), lds, ctx->stream, a, mt);
         } else if (g_zp_kernel == 1) {
             const size_t lds = 2 * (size_t)(ZPL_A_BYTES + ZPL_X_BYTES);
-            static bool attr = false;
-            if (!attr) {
-                HIPCHECK(hipFuncSetAttribute((const void*)k_zp_mfma_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                attr = true;
-            }
+            csd_allow_lds((const void*)k_zp_mfma_lds, (int)lds);
             hipLaunchKernelGGL(k_zp_mfma_lds, dim3((unsigned)a.nblocks), dim3(ZPL_THREADS), lds, ctx->stream, a, mt);
         } else {
             hipLaunchKernelGGL(k_zp_mfma, dim3((unsigned)a.nblocks), dim3(256), 0, ctx->stream, a, mt);
