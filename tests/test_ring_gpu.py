@@ -174,8 +174,25 @@ def test_bconv_parity(gpu, cpu, rings, bconv_form):
     assert same(from_dev(gpu, out)[:, 2:], dc)
 
 
+def test_bconv_parity_field49152(gpu, cpu, rings, bconv_form):
+    """field_49152's bases (23 primes lifted by its extension, and back), bit-exact with the oracle"""
+    g, c = rings("field_49152", (1280, 1300), 76)
+    base = g.initial_rns_base()
+    ext = g.extension_rns_base_for(base)
+    n2, n3 = g.info["deg_lprime"], g.info["deg_k"]
+    rng = np.random.default_rng(49)
+    for sb, db in [(base, ext), (ext, base)]:
+        src = uniform_residues(rng, [g.primes[i] for i in sb], (1, len(sb), n2, n3))
+        dc = np.zeros((1, len(db), n2, n3), dtype=np.int64)
+        c.bconv_tensors(dc, db, src.copy(), sb)
+        dg = gpu.hal_allocate_i64(dc.shape)
+        g.bconv_tensors(dg, db, to_dev(gpu, src), sb)
+        assert same(from_dev(gpu, dg), dc)
+
+
 GADGET_CASES = [("field_36864", (960, 980), 76, "select9", 1), ("field_36864", (960, 980), 76, "gaps", 3),
-                ("field_192", (200, 220), 8, "whole", 2), ("field_18", (100, 120), 4, "single", 4)]
+                ("field_192", (200, 220), 8, "whole", 2), ("field_18", (100, 120), 4, "single", 4),
+                ("field_49152", (1280, 1300), 76, "select9", 2)]
 
 
 @pytest.mark.parametrize("form", [1, 0])
