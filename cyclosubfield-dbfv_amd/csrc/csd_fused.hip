@@ -313,6 +313,7 @@ struct GbArgs {
     const int64_t* dmp;   // [d][rd]: -D_i 2^64 mod p_j
     const CsdMod* smods;  // source moduli (device), or null when the source base is the target base
     int shift;            // log2 of q~
+    int64_t xs_rows;      // rows of the LDS limb columns (the longest digit)
     int32_t lo[CSD_MAX_MODULI], hi[CSD_MAX_MODULI];
 };
 
@@ -352,16 +353,19 @@ __device__ __forceinline__ i128 gb_split_sum(const GbSplit& s) {
 // The digit's source limbs are loaded eight at a time (so that their loads overlap), lifted, and
 // parked in the thread's LDS column for the passes over GB_GROUP target limbs. SPLIT accumulates
 // in the four int64 partial sums (worth it for long digits; short ones pay more for the final
-// i128 assembly than they save).
-template <bool SPLIT>
+// i128 assembly than they save). WV = 1 stages the digit's weights in LDS once per workgroup
+// (broadcast reads); WV = 0 reads them with scalar loads, for weight tables beyond the LDS.
+template <bool SPLIT, int WV>
 __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
     extern __shared__ int64_t gb_x[];
     int64_t* xs = gb_x + threadIdx.x;
+    int64_t* wl = gb_x + a.xs_rows * 256;  // WV == 1: the digit's weights [len][rdp], shared by the block
     const int64_t chunks = (a.P + 255) / 256;
     const int64_t b = blockIdx.x / chunks;
     const int64_t pos = (blockIdx.x % chunks) * 256 + threadIdx.x;
-    if (pos >= a.P) return;  // the kernel has no barrier
-    const int64_t k = pos / a.n3, l = pos % a.n3;
+    const bool active = pos < a.P;
+    if (WV == 0 && !active) return;  // that form has no barrier
+    const int64_t k = active ? pos / a.n3 : 0, l = active ? pos % a.n3 : 0;
     const int64_t* s = a.src + b * a.ss[0] + k * a.ss[2] + l * a.ss[3];
     int64_t* o = a.dst + b * a.d * a.sd[0] + k * a.sd[2] + l * a.sd[3];
     const int r = (int)a.r, rd = (int)a.rd;
@@ -370,6 +374,12 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
         const int lo = a.lo[i], hi = a.hi[i];
         const int64_t* corr_row = a.mat + ((int64_t)i * (rd + 1) + rd) * r;
         const int64_t* wi = a.matT + (int64_t)i * r * a.rdp;
+        if (WV == 1) {
+            __syncthreads();  // the previous digit's weights are no longer read
+            for (int e = threadIdx.x; e < (hi - lo) * a.rdp; e += 256) wl[e] = wi[(int64_t)lo * a.rdp + e];
+            __syncthreads();
+        }
+        if (!active) continue;
         i128 corr = 0;
         for (int u0 = lo; u0 < hi; u0 += 8) {
             int64_t raw[8];
@@ -395,9 +405,10 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
                 ssum[g] = GbSplit{0, 0, 0, 0};
                 wsum[g] = 0;
             }
+#pragma unroll 2
             for (int u = lo; u < hi; u++) {
                 const int64_t xv = xs[(u - lo) * 256];
-                const int64_t* w = wi + (int64_t)u * a.rdp + j0;
+                const int64_t* w = WV == 1 ? wl + (int64_t)(u - lo) * a.rdp + j0 : wi + (int64_t)u * a.rdp + j0;
                 if (SPLIT) {
                     const int32_t x0 = (int32_t)(xv & ((1 << 29) - 1)), x1 = (int32_t)(xv >> 29);
 #pragma unroll
@@ -461,13 +472,21 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
     TimedScope ts(ctx, timer, 8.0 * (double)a.B * a.P * ((double)a.r + (double)d * a.rd));
     int64_t max_len = 0;
     for (int64_t i = 0; i < d; i++) max_len = std::max<int64_t>(max_len, a.hi[i] - a.lo[i]);
-    const size_t lds = (size_t)max_len * 256 * sizeof(int64_t);  // at most 128 KiB (64 limbs)
-    csd_allow_lds((const void*)k_bconv_fused<true>, 160 * 1024);
-    csd_allow_lds((const void*)k_bconv_fused<false>, 160 * 1024);
-    if (max_len >= 4)  // measured: 17-20 limb digits gain 20 %, 2-limb gadget digits lose 8 %
-        hipLaunchKernelGGL(k_bconv_fused<true>, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, a, t);
-    else
-        hipLaunchKernelGGL(k_bconv_fused<false>, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, a, t);
+    const size_t lds0 = (size_t)max_len * 256 * sizeof(int64_t);
+    a.xs_rows = max_len;
+    const size_t lds1 = (size_t)max_len * 256 * sizeof(int64_t) + (size_t)max_len * a.rdp * sizeof(int64_t);
+    csd_allow_lds((const void*)k_bconv_fused<true, 0>, 160 * 1024);
+    csd_allow_lds((const void*)k_bconv_fused<false, 0>, 160 * 1024);
+    csd_allow_lds((const void*)k_bconv_fused<true, 1>, 160 * 1024);
+    csd_allow_lds((const void*)k_bconv_fused<false, 1>, 160 * 1024);
+    const bool split = max_len >= 4;
+    if (lds1 <= 160 * 1024) {  // the digit's weights in LDS: 15-20 % faster than scalar loads
+        if (split) hipLaunchKernelGGL((k_bconv_fused<true, 1>), dim3((unsigned)blocks), dim3(256), lds1, ctx->stream, a, t);
+        else hipLaunchKernelGGL((k_bconv_fused<false, 1>), dim3((unsigned)blocks), dim3(256), lds1, ctx->stream, a, t);
+    } else {
+        if (split) hipLaunchKernelGGL((k_bconv_fused<true, 0>), dim3((unsigned)blocks), dim3(256), lds0, ctx->stream, a, t);
+        else hipLaunchKernelGGL((k_bconv_fused<false, 0>), dim3((unsigned)blocks), dim3(256), lds0, ctx->stream, a, t);
+    }
     HIPCHECK(hipGetLastError());
 }
 
