@@ -48,9 +48,9 @@ __device__ __forceinline__ int row_one(int limbs) { return 2 + 2 * limbs; }     
 __device__ __forceinline__ int row_delta(int limbs, int k) { return 3 + 2 * limbs + k; }  // mont^2(Delta 2^64k)
 
 // One thread per coefficient position. Its r residues wait in its own LDS column (slot p at
-// p * blockDim.x + threadIdx.x) between the passes over the primes, so no barrier is needed and
-// the loops keep only the running sums in registers; the per-prime constants are wave-uniform
-// scalar loads.
+// p * blockDim.x + threadIdx.x) between the passes over the primes, so the loops keep only the
+// running sums in registers. The constant table is staged in LDS once per workgroup (broadcast
+// reads, faster here than per-prime scalar loads).
 __global__ __launch_bounds__(256) void k_split_overflow(SoArgs a, ModTable mt) {
     extern __shared__ int64_t xs[];
     const int64_t per = a.n2 * a.n3;
@@ -60,8 +60,15 @@ __global__ __launch_bounds__(256) void k_split_overflow(SoArgs a, ModTable mt) {
     const int64_t* K = a.k;
     int64_t* col = xs + threadIdx.x;
     const int stride = (int)blockDim.x;
+    {
+        int64_t* kl = xs + (int64_t)r * stride;  // the table, after the columns
+        const int rows = 3 + 3 * limbs;
+        for (int q = threadIdx.x; q < rows * r; q += stride) kl[q] = a.k[q];
+        __syncthreads();
+        K = kl;
+    }
     // one coefficient per thread and no grid-stride loop: a loop with a per-lane exit would make
-    // the prime loops below divergent, and their constants vector loads instead of scalar ones
+    // the prime loops below divergent
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < total) {
         const int64_t c = e / per, j = e - c * per;
@@ -198,7 +205,7 @@ extern "C" int csd_dbfv_split_overflow(csd_ctx* ctx, const csd_view* x, const cs
     ModTable t = make_mod_table(mods, nmods, a.r);
     TimedScope ts(ctx, "dbfv_split_overflow",
                   8.0 * (double)a.count * a.n2 * a.n3 * ((zero_in ? 1.0 : 2.0) * a.r + limbs + (carry_in ? limbs : 0)));
-    const size_t lds = (size_t)a.r * 256 * sizeof(int64_t);  // at most 128 KiB (64 primes)
+    const size_t lds = ((size_t)a.r * 256 + (size_t)(3 + 3 * a.limbs) * a.r) * sizeof(int64_t);  // <= 128 + 4.5 KiB
     csd_allow_lds((const void*)k_split_overflow, 160 * 1024);
     const int64_t total = a.count * a.n2 * a.n3;
     if ((total + 255) / 256 > (int64_t)INT32_MAX) throw CsdError("dbfv_split_overflow: too many coefficients for one launch");
