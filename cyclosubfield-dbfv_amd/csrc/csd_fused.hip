@@ -201,11 +201,18 @@ struct MsArgs {
 };
 
 __global__ __launch_bounds__(256) void k_mod_switch_one(MsArgs a, ModTable mt) {
+    __shared__ int64_t cm[3 * CSD_MAX_MODULI];  // the level's M, qm, inv, staged once per workgroup
     const int64_t e = blockIdx.x / a.chunks;
     const int64_t pos = (blockIdx.x % a.chunks) * 256 + threadIdx.x;
-    if (pos >= a.npos) return;
     const MsLevel& lv = a.levels[a.level[e]];
     const int r = lv.r;
+    for (int q = threadIdx.x; q < r - 1; q += 256) {
+        cm[q] = lv.M[q];
+        cm[CSD_MAX_MODULI + q] = lv.qm[q];
+        cm[2 * CSD_MAX_MODULI + q] = lv.inv[q];
+    }
+    __syncthreads();
+    if (pos >= a.npos) return;
     const int64_t* in = a.in[e];
     int64_t* out = a.out[e];
     const CsdMod md = mt.m[r - 1];
@@ -214,11 +221,11 @@ __global__ __launch_bounds__(256) void k_mod_switch_one(MsArgs a, ModTable mt) {
     const i128 q = (i128)((u128)corr + ((u128)1 << (lv.lq - 1))) >> lv.lq;
     for (int i = 0; i < r - 1; i++) {
         const CsdMod m = mt.m[i];
-        int64_t d = csd_double_red128(m, (i128)lv.M[i] * (i128)x1);
-        d = (int64_t)((uint64_t)d + (uint64_t)csd_red128(m, (i128)((u128)q * (u128)(i128)lv.qm[i])));
+        int64_t d = csd_double_red128(m, (i128)cm[i] * (i128)x1);
+        d = (int64_t)((uint64_t)d + (uint64_t)csd_red128(m, (i128)((u128)q * (u128)(i128)cm[CSD_MAX_MODULI + i])));
         d = csd_to_mont64(m, d);
         const int64_t diff = (int64_t)((uint64_t)in[(int64_t)i * a.npos + pos] - (uint64_t)d);
-        out[(int64_t)i * a.npos + pos] = csd_mont_mul(m, diff, lv.inv[i]);
+        out[(int64_t)i * a.npos + pos] = csd_mont_mul(m, diff, cm[2 * CSD_MAX_MODULI + i]);
     }
 }
 
