@@ -6,6 +6,8 @@ representatives are almost reduced (bound 1) unless stated. Each function names 
 lines it follows; the HAL call sequence is the reference's, so results are bit-identical with any
 backend that is bit-identical per call."""
 
+from .ring import bconv_preserves_source
+
 
 def select_digits(digits, rns_base_len):
     """select_digits (bfv.rs:388-401): `digits` contiguous prime ranges, the first ones longer."""
@@ -34,8 +36,11 @@ def lift_to_extended(ring, pb, base, crt=None):
     ext = ring.extension_rns_base_for(base)
     lifted = list(base) + ext
     result = ring.allocate_uninit(pb.shape[0], lifted)  # every part is written below
-    tmp = ring.allocate_uninit(pb.shape[0], base)
-    be.hal_assign(tmp, pb)  # bconv destroys its input
+    if bconv_preserves_source():  # the fused conversion leaves pb (a separate buffer) unchanged
+        tmp = pb
+    else:
+        tmp = ring.allocate_uninit(pb.shape[0], base)
+        be.hal_assign(tmp, pb)  # bconv destroys its input
     ring.bconv_tensors(result[:, base_len:], ext, tmp, base)
     if crt is not None:
         ring.pb_to_crt_tensor(result[:, base_len:], ext)

@@ -1365,6 +1365,8 @@ extern "C" int csd_ring_set_overlap(int on) {
     return 0;
 }
 
+extern "C" int csd_ring_get_bconv_form(void) { return g_bconv_fused ? 1 : 0; }
+
 extern "C" int csd_ring_set_bconv_form(int form) {
     CSD_TRY if (form != 0 && form != 1) throw CsdError("bconv form must be 0 (HAL sequence) or 1 (fused)");
     g_bconv_fused = form == 1;

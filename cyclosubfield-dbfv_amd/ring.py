@@ -63,6 +63,12 @@ def set_bconv_form(form):
     check(lib().csd_ring_set_bconv_form(ctypes.c_int(int(form))))
 
 
+def bconv_preserves_source():
+    """Whether bconv_tensors leaves its source unchanged (the fused form, when source and
+    destination do not overlap); the HAL sequence destroys it, like the reference."""
+    return lib().csd_ring_get_bconv_form() == 1
+
+
 def set_gadget_form(form):
     """How bconv_gadget_tensors runs (csd_ring_set_gadget_form, process-wide): 1 (default) one
     fused kernel, 0 the reference's HAL call sequence per element; residues are identical."""

@@ -70,6 +70,8 @@ int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t* dst_base, 
  * kernel; 0: as the reference's HAL call sequence (ring.rs:1663-1701). Residues are identical
  * (no reference counterpart). */
 int csd_ring_set_bconv_form(int form);
+/* the current csd_ring_set_bconv_form setting (1 fused, 0 HAL sequence) */
+int csd_ring_get_bconv_form(void);
 /* bconv_gadget_tensors (ring.rs:1548-1557, GadgetBConvCache::bconv 1809-1851); digits are
  * half-open position ranges [digit_bounds[2i], digit_bounds[2i+1]). src is [B, r, ., .] and dst
  * [B ndigits, r, ., .], element b's digit i in row b ndigits + i; the reference takes B = 1, and
