@@ -69,6 +69,24 @@ def traffic_from_profiles(kernel):
     return None if entry is None else entry.get("bytes_per_launch")
 
 
+def valu_issue_from_profiles(kernel, avg_ms):
+    """The VALU issue fraction of `kernel` at `avg_ms` per launch: its wave instructions per launch
+    (the committed rocprofv3 SQ_INSTS_VALU summary) x 4 issue cycles, over the 1024 SIMDs at 2.4 GHz
+    for the launch time; None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "valu_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    entry = data.get(kernel)
+    if entry is None or not avg_ms:
+        return None
+    cycles = entry["sq_insts_valu"] * data["cycles_per_wave_inst"] / data["simds"]
+    return {"wave_insts_per_launch": entry["sq_insts_valu"],
+            "frac": round(cycles / (data["clock_GHz"] * 1e9 * avg_ms / 1000.0), 4),
+            "note": "VALU issue cycles per SIMD over the launch time at 2.4 GHz: the bound of this kernel"}
+
+
 def cpu_threads():
     """host threads for the CPU baseline: the cores this process may use, at most 16"""
     try:
@@ -840,7 +858,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 4),
                      "valu": {"achieved": per_kernel[name].get("Gmul/s"), "ceiling": round(barrett_gps, 1),
-                              "unit": "G modmul/s", "frac": per_kernel[name].get("valu_frac")}},
+                              "unit": "G modmul/s", "frac": per_kernel[name].get("valu_frac")},
+                     "valu_issue": valu_issue_from_profiles(name, avg_ms)},
         "ceilings": {"hbm_copy_GB/s": round(copy_gbs, 1), "barrett_Gmul/s": round(barrett_gps, 1)},
         "kernels": per_kernel,
         "two_stream_ms_per_step": round(two_stream_ms, 3),
