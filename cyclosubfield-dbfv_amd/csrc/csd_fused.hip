@@ -638,11 +638,17 @@ struct KsArgs {
 };
 
 __global__ __launch_bounds__(256) void k_keyswitch_reduce(KsArgs a, ModTable mt) {
+    // block -> (chunk, limb, product), the product fastest: the products of one position read the
+    // same key entries, so they run back to back, and the XCD-aware order (the hardware deals
+    // consecutive workgroups to the 8 XCDs in turn; tile t = (b mod 8) nblocks/8 + b/8) keeps them
+    // on one XCD, whose L2 then serves the key to all but the first
     const int64_t chunks = (a.P + 255) / 256;
+    const int64_t nblocks = chunks * a.r * a.K;
     int64_t blk = blockIdx.x;
-    const int64_t c0 = blk % chunks;
-    blk /= chunks;
-    const int64_t j = blk % a.r, k = blk / a.r;
+    if (nblocks % 8 == 0) blk = (blk % 8) * (nblocks / 8) + blk / 8;
+    const int64_t k = blk % a.K;
+    blk /= a.K;
+    const int64_t j = blk % a.r, c0 = blk / a.r;
     const int64_t pos = c0 * 256 + threadIdx.x;
     if (pos >= a.P) return;
     const int64_t p2 = pos / a.n3, p3 = pos % a.n3;
