@@ -54,14 +54,11 @@ def lift_to_extended(ring, pb, base, crt=None):
 def rescale_to_base(ring, x, lifted, base_len, t):
     """rescale_to_base (bfv.rs:601-622): round(t x / q') over q' = base for the period-basis
     batch `x` over `lifted` (destroyed)."""
-    be = ring.backend
     count = x.shape[0]
     base, ext = lifted[:base_len], lifted[base_len:]
-    be.hal_mul_mont64(x, ring.scalar_tensor(lifted, t), ring.mods_for(lifted))   # mul_scalar
-    tmp = ring.allocate_uninit(count, ext)
-    ring.bconv_tensors(tmp, ext, x[:, :base_len], base)
-    be.hal_sub_i64(x[:, base_len:], tmp)
-    be.hal_mul_mont64(x[:, base_len:], ring.inverse_product_tensor(ext, base), ring.mods_for(ext))
+    s = ring.scalar_tensor(lifted, t)   # mul_scalar, then bconv, sub and the inverse's mul in one step
+    ring.rescale_step(x[:, base_len:], ext, x[:, :base_len], base, s[:, :base_len], s[:, base_len:],
+                      ring.inverse_product_tensor(ext, base))
     result = ring.allocate_uninit(count, base)
     ring.bconv_tensors(result, base, x[:, base_len:], ext)
     return result

@@ -321,6 +321,12 @@ struct GbArgs {
     const CsdMod* smods;  // source moduli (device), or null when the source base is the target base
     int shift;            // log2 of q~
     int64_t xs_rows;      // rows of the LDS limb columns (the longest digit)
+    // rescale form (all null otherwise): each source limb u is first taken to mont(x, pre[u]), and
+    // each target row j of dst, read before it is written, becomes mont(mont(dst, post_s[j]) - v, post_v[j])
+    // for the converted value v
+    const int64_t* pre;
+    const int64_t* post_s;
+    const int64_t* post_v;
     int32_t lo[CSD_MAX_MODULI], hi[CSD_MAX_MODULI];
 };
 
@@ -398,7 +404,8 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
                 if (u0 + t < hi) {
                     const int u = u0 + t;
                     const CsdMod mu = a.smods ? a.smods[u] : mt.m[u];
-                    const int64_t xv = csd_mont_mul(mu, raw[t], a.dopi[u]);
+                    const int64_t xin = a.pre ? csd_mont_mul(mu, raw[t], a.pre[u]) : raw[t];
+                    const int64_t xv = csd_mont_mul(mu, xin, a.dopi[u]);
                     xs[(u - lo) * 256] = xv;
                     corr = gb_mac(corr, corr_row[u], xv);
                 }
@@ -434,7 +441,13 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
                     const i128 sum = SPLIT ? gb_split_sum(ssum[g]) : wsum[g];
                     int64_t v = csd_double_red128(mt.m[j], sum);
                     v = (int64_t)((uint64_t)v + (uint64_t)csd_mont_mul(mt.m[j], q, a.dmp[(int64_t)i * rd + j]));
-                    o[i * a.sd[0] + j * a.sd[1]] = csd_to_mont64(mt.m[j], v);
+                    v = csd_to_mont64(mt.m[j], v);
+                    int64_t* oj = o + i * a.sd[0] + j * a.sd[1];
+                    if (a.post_s) {
+                        const int64_t e = csd_mont_mul(mt.m[j], *oj, a.post_s[j]);
+                        v = csd_mont_mul(mt.m[j], (int64_t)((uint64_t)e - (uint64_t)v), a.post_v[j]);
+                    }
+                    *oj = v;
                 }
             }
         }
@@ -444,7 +457,8 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
 void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, int64_t d, const int64_t* dopi,
                             const int64_t* mat, const int64_t* matT, int64_t rdp, const int64_t* dmp,
                             const std::vector<size_t>& bounds, size_t shift, const csd_modulus* dmods,
-                            const CsdMod* smods_dev, const char* timer) {
+                            const CsdMod* smods_dev, const char* timer, const int64_t* pre, const int64_t* post_s,
+                            const int64_t* post_v) {
     GbArgs a;
     memset(&a, 0, sizeof(a));
     a.B = src.shape[0];
@@ -466,6 +480,10 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
     a.dmp = dmp;
     a.smods = smods_dev;
     a.shift = (int)shift;
+    a.pre = pre;
+    a.post_s = post_s;
+    a.post_v = post_v;
+    if ((post_s == nullptr) != (post_v == nullptr) || (post_s && d != 1)) throw CsdError("fused bconv: rescale form");
     if (!smods_dev && a.rd != a.r) throw CsdError("fused bconv: source moduli missing");
     if (rdp < a.rd || rdp % GB_GROUP) throw CsdError("fused bconv: transposed weights not padded");
     for (int64_t i = 0; i < d; i++) {
@@ -476,7 +494,7 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
     const int64_t blocks = a.B * ((a.P + 255) / 256);
     if (blocks > (int64_t)INT32_MAX) throw CsdError("fused bconv: grid too large");
     ModTable t = make_mod_table(dmods, (size_t)a.rd, a.rd);
-    TimedScope ts(ctx, timer, 8.0 * (double)a.B * a.P * ((double)a.r + (double)d * a.rd));
+    TimedScope ts(ctx, timer, 8.0 * (double)a.B * a.P * ((double)a.r + (double)d * a.rd * (post_s ? 2 : 1)));
     int64_t max_len = 0;
     for (int64_t i = 0; i < d; i++) max_len = std::max<int64_t>(max_len, a.hi[i] - a.lo[i]);
     const size_t lds0 = (size_t)max_len * 256 * sizeof(int64_t);

@@ -471,6 +471,28 @@ static std::atomic<bool> g_overlap_halves{false};  // see csd_ring_set_overlap
 static std::atomic<bool> g_gadget_fused{true};    // see csd_ring_set_gadget_form
 static std::atomic<bool> g_bconv_fused{true};     // see csd_ring_set_bconv_form
 
+// Whether a and b are disjoint limb ranges of one tensor (or do not overlap at all), so that
+// a kernel whose thread owns one (element, position) of both may read b and rewrite a in place:
+// equal shapes and strides apart from dim 1, the dim-1 rows of both within one element's
+// stride, and the two row ranges apart.
+static bool views_overlap(const csd_view& a, const csd_view& b, int elem);
+static bool limb_disjoint(const csd_view& a, const csd_view& b) {
+    if (!views_overlap(a, b, 8)) return true;
+    for (int q = 0; q < 4; q++) {
+        if (a.strides[q] != b.strides[q] || a.strides[q] <= 0) return false;
+        if (q != 1 && a.shape[q] != b.shape[q]) return false;
+    }
+    const int64_t s1 = a.strides[1];
+    if (a.shape[2] > 1 && a.strides[2] < a.strides[3] * a.shape[3]) return false;          // positions alias
+    if (a.strides[2] * (a.shape[2] - 1) + a.strides[3] * (a.shape[3] - 1) >= s1) return false;  // rows interleave
+    const intptr_t diff = (intptr_t)b.ptr - (intptr_t)a.ptr;
+    if (diff % (intptr_t)(8 * s1)) return false;
+    const int64_t m = (int64_t)(diff / (intptr_t)(8 * s1));  // b's first row, counted in a's rows
+    const int64_t lo = std::min<int64_t>(0, m), hi = std::max<int64_t>(a.shape[1], m + b.shape[1]);
+    if ((hi - lo) * s1 > a.strides[0] && a.shape[0] > 1) return false;  // rows reach the next element
+    return m >= a.shape[1] || m + b.shape[1] <= 0;
+}
+
 // whether the memory spans of two views (first to last element) intersect
 static bool views_overlap(const csd_view& a, const csd_view& b, int elem) {
     auto span = [elem](const csd_view& v, uintptr_t& lo, uintptr_t& hi) {
@@ -820,6 +842,41 @@ struct csd_ring {
         csd_view qm = make_view(c.q_mod_p->p, 1, rd, 1, 1);
         ok(csd_hal_bconv_correct_mont64(ctx, &dst, &vcorr, &qm, dmods.data(), rd, c.log2_qtilde));
         ok(csd_hal_i64_to_mont64(ctx, &dst, dmods.data(), rd));
+    }
+
+    // The middle of rescale_to_base (bfv.rs:601-622): with s the scalar t and v the inverse of
+    // the source base's product (one Montgomery value per limb),
+    //   ext <- mont(mont(ext, s_ext) - bconv(mont(src, s_src)), v)
+    // that is mul_mont64 by s, bconv, sub_i64 and mul_mont64 by v. The fused form is one kernel that
+    // leaves src as it is; the HAL sequence (when the fused bconv does not apply) destroys src.
+    void rescale_step(const csd_view& ext, const std::vector<size_t>& ebase, const csd_view& src,
+                      const std::vector<size_t>& sbase, const csd_view& s_src, const csd_view& s_ext, const csd_view& inv) {
+        BConvCache& c = bconv_cache(sbase, ebase);
+        int64_t r = (int64_t)sbase.size(), re = (int64_t)ebase.size();
+        int64_t cnt = src.shape[0], n2 = src.shape[2], n3 = src.shape[3];
+        if (src.shape[1] != r) throw CsdError("rescale: source doesn't match its base");
+        if (ext.shape[0] != cnt || ext.shape[1] != re || ext.shape[2] != n2 || ext.shape[3] != n3)
+            throw CsdError("rescale: extension shape");
+        auto scalar_ok = [](const csd_view& v, int64_t n) {
+            return v.shape[0] == 1 && v.shape[1] == n && v.shape[2] == 1 && v.shape[3] == 1 && (n < 2 || v.strides[1] == 1);
+        };
+        if (!scalar_ok(s_src, r) || !scalar_ok(s_ext, re) || !scalar_ok(inv, re)) throw CsdError("rescale: scalar tensor shape");
+        auto smods = mods_for(sbase), emods = mods_for(ebase);
+        if (g_bconv_fused && c.acc >= (size_t)r && limb_disjoint(ext, src)) {
+            const std::vector<size_t> whole = {0, (size_t)r};
+            csd_bconv_fused_launch(ctx, ext, src, 1, (const int64_t*)c.q_over_p_inv->p, (const int64_t*)c.matrix->p,
+                                   (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.q_mod_p->p, whole,
+                                   c.log2_qtilde, emods.data(), (const CsdMod*)c.src_mods->p, "rescale_step",
+                                   (const int64_t*)s_src.ptr, (const int64_t*)s_ext.ptr, (const int64_t*)inv.ptr);
+            return;
+        }
+        ok(csd_hal_mul_mont64(ctx, &src, &s_src, smods.data(), r));
+        ok(csd_hal_mul_mont64(ctx, &ext, &s_ext, emods.data(), re));
+        DevBuf tmp(ctx, (size_t)cnt * re * n2 * n3 * 8);
+        csd_view vt = make_view(tmp.p, cnt, re, n2, n3);
+        bconv(vt, ebase, src, sbase);
+        ok(csd_hal_sub_i64(ctx, &ext, &vt));
+        ok(csd_hal_mul_mont64(ctx, &ext, &inv, emods.data(), re));
     }
 
     GadgetCache& gadget_cache(const std::vector<size_t>& base, const std::vector<size_t>& bounds) {  // ring.rs:1730-1807
@@ -1325,6 +1382,18 @@ extern "C" int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t*
     CSD_TRY check_view(dst, "bconv dst");
     check_view(src, "bconv src");
     ring->bconv(*dst, to_vec(dst_base, rd), *src, to_vec(src_base, rs));
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_rescale_step(csd_ring* ring, const csd_view* ext, const size_t* ext_base, size_t re,
+                                     const csd_view* src, const size_t* src_base, size_t rs, const csd_view* s_src,
+                                     const csd_view* s_ext, const csd_view* inv) {
+    CSD_TRY check_view(ext, "rescale ext");
+    check_view(src, "rescale src");
+    check_view(s_src, "rescale scalar");
+    check_view(s_ext, "rescale scalar");
+    check_view(inv, "rescale inverse");
+    ring->rescale_step(*ext, to_vec(ext_base, re), *src, to_vec(src_base, rs), *s_src, *s_ext, *inv);
     CSD_CATCH
 }
 

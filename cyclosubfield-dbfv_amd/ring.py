@@ -140,6 +140,17 @@ class RingContext:
         sb, sn = _sizes(src_base)
         check(lib().csd_ring_bconv(self.handle, ctypes.byref(dst.view()), db, dn, ctypes.byref(src.view()), sb, sn))
 
+    def rescale_step(self, ext, ext_base, src, src_base, s_src, s_ext, inv):
+        """The middle of rescale_to_base (bfv.rs:601-622) for a batch: ext <- mont(mont(ext, s_ext)
+        - bconv(mont(src, s_src)), inv), that is mul_mont64, bconv, sub_i64 and mul_mont64, as one
+        kernel when the fused conversion applies (src is then left as it is; the HAL sequence, the
+        other form, destroys it). src and ext may be disjoint limb ranges of one tensor."""
+        eb, en = _sizes(ext_base)
+        sb, sn = _sizes(src_base)
+        check(lib().csd_ring_rescale_step(self.handle, ctypes.byref(ext.view()), eb, en, ctypes.byref(src.view()), sb, sn,
+                                          ctypes.byref(s_src.view()), ctypes.byref(s_ext.view()),
+                                          ctypes.byref(inv.view())))
+
     def bconv_gadget_tensors(self, dst, src, base, digits):
         """bconv_gadget_tensors (ring.rs:1548-1557) for src [B, r, ., .] and dst [B d, r, ., .]
         (element b, digit i at row b d + i); the reference takes B = 1."""

@@ -66,6 +66,17 @@ int csd_ring_set_overlap(int on);
  * src, like the reference. */
 int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t* dst_base, size_t r_dst, const csd_view* src,
                    const size_t* src_base, size_t r_src);
+/* Lines 607-616 of rescale_to_base (bfv.rs:601-622) for a batch of count elements: mul_scalar by t
+ * (mul_mont64 with s_src / s_ext, the [1, r, 1, 1] scalar tensor of t over the source and extension
+ * bases), bconv of src into the extension, sub_i64 and mul_mont64 by inv (the inverse scalar
+ * tensor of the source base's product over the extension):
+ *   ext <- mont(mont(ext, s_ext) - bconv(mont(src, s_src)), inv).
+ * src [count, r_src, ., .] and ext [count, r_ext, ., .] may be disjoint limb ranges of one tensor.
+ * With the fused conversion form this is one kernel and src is left as it is; otherwise it is the
+ * reference's HAL call sequence and src is destroyed. ext's residues are identical either way. */
+int csd_ring_rescale_step(csd_ring* ring, const csd_view* ext, const size_t* ext_base, size_t r_ext,
+                          const csd_view* src, const size_t* src_base, size_t r_src, const csd_view* s_src,
+                          const csd_view* s_ext, const csd_view* inv);
 /* 1 (default): csd_ring_bconv and the conversions inside the ring sequences run as one fused
  * kernel; 0: as the reference's HAL call sequence (ring.rs:1663-1701). Residues are identical
  * (no reference counterpart). */

@@ -162,3 +162,69 @@ def test_relin_batch_key_layouts(gpu, rings):
         gpu.keyswitch_reduce(out, (K, 1), parts, K, 3, kd + kd[:1], None, None, False, g.mods_for(base))
     with pytest.raises(pkg.HalError):  # output rows out of range
         gpu.keyswitch_reduce(out, (K, 2), parts, K, 2, kd, None, None, False, g.mods_for(base))
+
+
+def _oracle_rescale_step(c, x, lifted, base_len, t):
+    """Lines 607-616 of rescale_to_base (bfv.rs:601-622) on the oracle: the extension rows of x."""
+    be = c.backend
+    base, ext = lifted[:base_len], lifted[base_len:]
+    be.hal_mul_mont64(x, c.scalar_tensor(lifted, t), c.mods_for(lifted))
+    tmp = c.allocate(x.shape[0], ext)
+    c.bconv_tensors(tmp, ext, x[:, :base_len], base)
+    be.hal_sub_i64(x[:, base_len:], tmp)
+    be.hal_mul_mont64(x[:, base_len:], c.inverse_scalar_tensor(ext, c.modulus_of(base)), c.mods_for(ext))
+    return x[:, base_len:]
+
+
+@pytest.mark.parametrize("form", [1, 0])
+@pytest.mark.parametrize("layout", ["split", "separate", "ext_first"])
+@pytest.mark.parametrize("field,log2_q,t_bits,ndigits", CASES[1:])
+def test_rescale_step_parity(gpu, cpu, rings, field, log2_q, t_bits, ndigits, layout, form):
+    """csd_ring_rescale_step against the oracle's sequence, bit for bit, in both conversion forms:
+    source and extension as limb ranges of one tensor (either order) or as separate tensors, the
+    extension a strided view of a wider one."""
+    pkg = load()
+    g, c, base, lhs, rhs, digits, keys = setup(rings, field, log2_q, t_bits, ndigits, 13)
+    _, lifted = R.lift_to_extended(c, lhs.copy(), base)
+    bl, el = len(base), len(lifted) - len(base)
+    x = uniform_residues(np.random.default_rng(14), [g.primes[i] for i in lifted], (3, len(lifted)) + lhs.shape[2:])
+    t = 1 << t_bits
+    want = _oracle_rescale_step(c, x.copy(), lifted, bl, t)
+    s = g.scalar_tensor(lifted, t)
+    inv = g.inverse_product_tensor(lifted[bl:], base)
+    if layout == "split":
+        xd = to_dev(gpu, x)
+        src, ext = xd[:, :bl], xd[:, bl:]
+    elif layout == "ext_first":  # extension rows first, then the source rows, in one tensor
+        xd = to_dev(gpu, np.concatenate([x[:, bl:], x[:, :bl]], axis=1))
+        ext, src = xd[:, :el], xd[:, el:]
+    else:
+        wide = np.zeros((3, el + 2) + x.shape[2:], dtype=np.int64)
+        wide[:, 1:el + 1] = x[:, bl:]
+        xd = to_dev(gpu, wide)
+        src, ext = to_dev(gpu, np.ascontiguousarray(x[:, :bl])), xd[:, 1:el + 1]
+    src_before = from_dev(gpu, src)
+    pkg.ring.set_bconv_form(form)
+    try:
+        g.rescale_step(ext, lifted[bl:], src, base, s[:, :bl], s[:, bl:], inv)
+    finally:
+        pkg.ring.set_bconv_form(1)
+    assert same(from_dev(gpu, ext), want)
+    if form == 1:
+        assert same(from_dev(gpu, src), src_before)  # the fused form leaves the source as it is
+    if layout == "separate":  # the rows beside the extension view stay untouched
+        assert not from_dev(gpu, xd[:, :1]).any() and not from_dev(gpu, xd[:, el + 1:]).any()
+
+
+def test_rescale_step_errors(gpu, rings):
+    pkg = load()
+    g, c, base, lhs, rhs, digits, keys = setup(rings, "field_192", (100, 120), 8, 2, 15)
+    _, lifted = R.lift_to_extended(c, lhs.copy(), base)
+    bl = len(base)
+    x = g.allocate(2, lifted)
+    s = g.scalar_tensor(lifted, 1 << 8)
+    inv = g.inverse_product_tensor(lifted[bl:], base)
+    with pytest.raises(pkg.hal.HalError):  # scalar tensor over the wrong base
+        g.rescale_step(x[:, bl:], lifted[bl:], x[:, :bl], base, s, s[:, bl:], inv)
+    with pytest.raises(pkg.hal.HalError):  # extension of another element count
+        g.rescale_step(x[:1, bl:], lifted[bl:], x[:, :bl], base, s[:, :bl], s[:, bl:], inv)
