@@ -646,11 +646,12 @@ def bench_zp(pkg, be, steps, nvec=512, seed=SEED, cpu=True):
     mfma_ms = ev.stop(stream) / steps
     be.set_timing(False)
     kern = be.timing_report()
-    # the other MFMA kernels: 0 (4 waves loading their operands from L2 directly) and 2 (64 x 64
-    # tiles loaded straight into LDS, 2 x 2 blocks per wave)
+    # the other MFMA kernels: 0 (4 waves loading their operands from L2 directly), 1 (32 x 64 tiles
+    # staged in LDS, one block per wave) and 2 (64 x 64 tiles loaded straight into LDS, 2 x 2 blocks
+    # per wave)
     ym = be.hal_export(y_mfma)
     other_ms = {}
-    for form in (0, 2):
+    for form in (0, 1, 2):
         be.zp_set_kernel(form)
         try:
             be.zp_apply_dim(mat, y_plain, x, 3, form="mfma")
@@ -659,7 +660,7 @@ def bench_zp(pkg, be, steps, nvec=512, seed=SEED, cpu=True):
                 be.zp_apply_dim(mat, y_plain, x, 3, form="mfma")
             other_ms[form] = ev.stop(stream) / steps
         finally:
-            be.zp_set_kernel(1)
+            be.zp_set_kernel(3)
         if not (ym == be.hal_export(y_plain)).all():
             raise SystemExit(f"bench: MFMA kernel form {form} differs from the default")
     l2_ms = other_ms[0]
@@ -699,6 +700,7 @@ def bench_zp(pkg, be, steps, nvec=512, seed=SEED, cpu=True):
         "value": round(R * nvec / (mfma_ms / 1000.0), 1), "unit": "vector-limb maps/s", "ms_per_step": round(mfma_ms, 3),
         "mfma_l2_operands": {"ms_per_step": round(l2_ms, 3), "value": round(R * nvec / (l2_ms / 1000.0), 1)},
         "mfma_64x64_direct_lds": {"ms_per_step": round(other_ms[2], 3), "value": round(R * nvec / (other_ms[2] / 1000.0), 1)},
+        "mfma_32x64_lds": {"ms_per_step": round(other_ms[1], 3), "value": round(R * nvec / (other_ms[1] / 1000.0), 1)},
         "plain_valu": {"ms_per_step": round(plain_ms, 3), "value": round(R * nvec / (plain_ms / 1000.0), 1)},
         "ntt_path": {"ms_per_step": round(ntt_ms, 3), "value": round(R * nvec / (ntt_ms / 1000.0), 1),
                      "note": "the O(n log n) NTT form of the same map (equal mod p)"},

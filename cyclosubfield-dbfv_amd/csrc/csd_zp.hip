@@ -343,6 +343,110 @@ __global__ __launch_bounds__(512) void k_zp_mfma_lds(MfmaArgs a, ModTable mt) {
     }
 }
 
+// The 64 x 64 form on 8 waves: a 64-row x 64-vector tile, each wave one 32-row x 16-vector strip
+// (two 16 x 16 output blocks, 30 accumulators). A K-step stages 32 KiB of A and 32 KiB of X
+// (double-buffered, 128 KiB): 1.5 x the MFMAs per staged byte of the 32 x 64 form, with two waves
+// per SIMD still hiding each other's LDS reads, and half the accumulators per wave of the
+// 4-wave 64 x 64 form.
+
+static constexpr int ZP8_THREADS = 512;
+static constexpr int ZP8_A_BYTES = ZP_DIGITS * 64 * 64;  // 32 KiB
+static constexpr int ZP8_X_BYTES = ZP_DIGITS * 64 * 64;  // 32 KiB
+
+__global__ __launch_bounds__(512) void k_zp_mfma_lds64(MfmaArgs a, ModTable mt) {
+    extern __shared__ __align__(16) int8_t zs[];  // [2][A tile | X tile]
+    int64_t b = blockIdx.x;
+    if (a.nblocks % 8 == 0) b = (b % 8) * (a.nblocks / 8) + b / 8;
+    const int64_t nt = b % a.ntiles;
+    const int64_t mtile = (b / a.ntiles) % a.mtiles;
+    const int64_t l = b / (a.ntiles * a.mtiles);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x16 = lane & 15, g = lane >> 4;
+    const int rh = wave & 1, cb = wave >> 1;  // this wave's 32-row half and 16-vector block
+    const int64_t pa = a.mpad * a.kpad, px = a.npad * a.kpad;
+    const int8_t* Ag = a.a + (l * ZP_DIGITS * a.mpad + mtile * 64) * a.kpad;
+    const int8_t* Xg = a.x + (l * ZP_DIGITS * a.npad + nt * 64) * a.kpad;
+    // staging: each tile has 2048 16-byte chunks (4 per thread); chunk c is plane c / 256,
+    // row (c / 4) % 64, k-group c % 4
+    auto stage_load = [&](v4i (&ra)[4], v4i (&rx)[4], int64_t k0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = tid + q * ZP8_THREADS;
+            const int pl = c >> 8, row = (c >> 2) & 63, kg = c & 3;
+            ra[q] = *(const v4i*)(Ag + pl * pa + row * a.kpad + k0 + kg * 16);
+            rx[q] = *(const v4i*)(Xg + pl * px + row * a.kpad + k0 + kg * 16);
+        }
+    };
+    auto stage_store = [&](const v4i (&ra)[4], const v4i (&rx)[4], int buf) {
+        int8_t* base = zs + buf * (ZP8_A_BYTES + ZP8_X_BYTES);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            *(v4i*)(base + (tid + q * ZP8_THREADS) * 16) = ra[q];
+            *(v4i*)(base + ZP8_A_BYTES + (tid + q * ZP8_THREADS) * 16) = rx[q];
+        }
+    };
+    const CsdMod md = mt.m[l];
+    int64_t total[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    for (int64_t kc = 0; kc < a.kpad; kc += a.chunk) {
+        const int64_t ke = min(a.kpad, kc + a.chunk);
+        v4i acc[2][2 * ZP_DIGITS - 1];
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int s = 0; s < 2 * ZP_DIGITS - 1; s++) acc[u][s] = (v4i){0, 0, 0, 0};
+        v4i ra[4], rx[4];
+        stage_load(ra, rx, kc);
+        stage_store(ra, rx, 0);
+        __syncthreads();
+        int buf = 0;
+        for (int64_t k0 = kc; k0 < ke; k0 += 64) {
+            const bool more = k0 + 64 < ke;
+            if (more) stage_load(ra, rx, k0 + 64);
+            const int8_t* base = zs + buf * (ZP8_A_BYTES + ZP8_X_BYTES);
+            v4i xv[ZP_DIGITS];
+#pragma unroll
+            for (int d = 0; d < ZP_DIGITS; d++)
+                xv[d] = *(const v4i*)(base + ZP8_A_BYTES + ((d * 64 + cb * 16 + x16) * 4 + g) * 16);
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                v4i av[ZP_DIGITS];
+#pragma unroll
+                for (int d = 0; d < ZP_DIGITS; d++)
+                    av[d] = *(const v4i*)(base + ((d * 64 + rh * 32 + u * 16 + x16) * 4 + g) * 16);
+#pragma unroll
+                for (int da = 0; da < ZP_DIGITS; da++)
+#pragma unroll
+                    for (int dx = 0; dx < ZP_DIGITS; dx++)
+                        acc[u][da + dx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[da], xv[dx], acc[u][da + dx], 0, 0, 0);
+            }
+            if (more) stage_store(ra, rx, buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                u128 v = 0;
+#pragma unroll
+                for (int s = 0; s < 2 * ZP_DIGITS - 1; s++) v += (u128)(i128)acc[u][s][r] << (8 * s);
+                int64_t t = total[u][r] + zp_reduce(md, (i128)v);
+                total[u][r] = t >= md.N ? t - md.N : t;
+            }
+    }
+    const int64_t v = nt * 64 + cb * 16 + x16;
+    if (v < a.Ly.nvec) {
+        int64_t* out = a.y + l * a.Ly.s_limb + (v / a.Ly.nlo) * a.Ly.s_hi + (v % a.Ly.nlo) * a.Ly.s_lo;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t i = mtile * 64 + rh * 32 + u * 16 + 4 * g + r;
+                if (i < a.m) out[i * a.Ly.s_elem] = total[u][r];
+            }
+    }
+}
+
 // The 64 x 64 form: 4 waves share one 64-row x 64-vector tile, and each wave keeps a 2 x 2
 // arrangement of 16 x 16 output blocks (60 accumulators, held in AGPRs). A K-step's operands of a
 // wave are 2 x 8 planes of A rows and 2 x 8 planes of X vectors, 32 LDS reads for 256 MFMAs:
@@ -464,11 +568,12 @@ __global__ __launch_bounds__(256) void k_zp_mfma_wide(MfmaArgs a, ModTable mt) {
     }
 }
 
-static std::atomic<int> g_zp_kernel{1};  // the MFMA kernel form 0 uses: 0 L2 operands, 1 LDS 32 x 64, 2 LDS 64 x 64
+static std::atomic<int> g_zp_kernel{3};  // the MFMA kernel form 0 uses: 0 L2 operands, 1 LDS 32 x 64,
+                                         // 2 LDS 64 x 64 on 4 waves, 3 (default) LDS 64 x 64 on 8 waves
 
 extern "C" int csd_zp_set_kernel(int form) {
-    if (form < 0 || form > 2) {
-        csd_set_error("csd_zp_set_kernel: form must be 0, 1 or 2");
+    if (form < 0 || form > 3) {
+        csd_set_error("csd_zp_set_kernel: form must be 0, 1, 2 or 3");
         return 1;
     }
     g_zp_kernel = form;
@@ -682,7 +787,13 @@ extern "C" int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* z, const csd_
         if (a.nblocks > (int64_t)INT32_MAX) throw CsdError("zp apply: too many tiles");
         TimedScope ts(ctx, "zp_mfma",
                       (double)z->r * ZP_DIGITS * (z->mpad * z->kpad + npad * z->kpad) + 8.0 * z->r * nvec * z->m);
-        if (g_zp_kernel == 2) {
+        if (g_zp_kernel == 3) {
+            a.mtiles = z->mpad / 64;  // mpad is a multiple of 64
+            a.nblocks = z->r * a.mtiles * a.ntiles;
+            const size_t lds = 2 * (size_t)(ZP8_A_BYTES + ZP8_X_BYTES);
+            csd_allow_lds((const void*)k_zp_mfma_lds64, (int)lds);
+            hipLaunchKernelGGL(k_zp_mfma_lds64, dim3((unsigned)a.nblocks), dim3(ZP8_THREADS), lds, ctx->stream, a, mt);
+        } else if (g_zp_kernel == 2) {
             // 64-row tiles over the same planes (mpad is a multiple of 64)
             a.mtiles = z->mpad / 64;
             a.nblocks = z->r * a.mtiles * a.ntiles;

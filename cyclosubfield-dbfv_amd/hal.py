@@ -507,9 +507,10 @@ class HipBackend:
 
     @staticmethod
     def zp_set_kernel(form):
-        """which int8 MFMA kernel the "mfma" form runs (process-wide, csd_zp_set_kernel): 1 (the
-        default) LDS-staged 32 x 64 tiles for 8 waves, 2 direct-to-LDS 64 x 64 tiles for 4 waves
-        with 2 x 2 blocks each, 0 operands loaded by 4 waves straight from L2"""
+        """which int8 MFMA kernel the "mfma" form runs (process-wide, csd_zp_set_kernel): 3 (the
+        default) LDS-staged 64 x 64 tiles for 8 waves with 2 blocks each, 1 LDS-staged 32 x 64 tiles
+        for 8 waves, 2 direct-to-LDS 64 x 64 tiles for 4 waves with 2 x 2 blocks each, 0 operands
+        loaded by 4 waves straight from L2"""
         check(lib().csd_zp_set_kernel(ctypes.c_int(int(form))))
 
     def hal_create_permutation(self, inner_shape, shifts):

@@ -173,10 +173,11 @@ int csd_zp_matrix_create(csd_ctx* ctx, const csd_view* a, const csd_modulus* mod
 int csd_zp_matrix_free(csd_zp_matrix* m);
 int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* a, const csd_view* dst, const csd_view* src, int dim, int form,
                      size_t max_chunk);
-/* which int8 MFMA kernel form 0 uses (process-wide): 1 (default) stages each K-step's digit planes
- * of a 32 x 64 tile in LDS for 8 waves with one 16 x 16 output block each; 2 loads them straight
- * into LDS (global_load_lds) for a 64 x 64 tile of 4 waves with 2 x 2 blocks each; 0 has 4 waves
- * load their operands from L2 directly, two blocks each. All compute identical results. */
+/* which int8 MFMA kernel form 0 uses (process-wide): 3 (default) stages each K-step's digit planes
+ * of a 64 x 64 tile in LDS for 8 waves with two 16 x 16 output blocks each; 1 stages a 32 x 64
+ * tile for 8 waves with one block each; 2 loads them straight into LDS (global_load_lds) for a
+ * 64 x 64 tile of 4 waves with 2 x 2 blocks each; 0 has 4 waves load their operands from L2
+ * directly, two blocks each. All compute identical results. */
 int csd_zp_set_kernel(int form);
 
 #ifdef __cplusplus

@@ -62,12 +62,13 @@ def run(gpu, a, x, primes, dim, form, max_chunk=0):
     return from_dev(gpu, dst)
 
 
-KERNELS = {"mfma_l2": 0, "mfma_wide": 2}
+KERNELS = {"mfma_l2": 0, "mfma_lds32": 1, "mfma_wide": 2}
 
 
 def apply(gpu, mat, dst, src, dim, form, max_chunk=0):
-    """form "mfma" runs the default MFMA kernel, "mfma_l2" the one that loads its operands from L2
-    directly, "mfma_wide" the 64 x 64 direct-to-LDS one"""
+    """form "mfma" runs the default MFMA kernel (64 x 64 tiles staged in LDS for 8 waves), "mfma_l2"
+    the one that loads its operands from L2 directly, "mfma_lds32" the 32 x 64 LDS-staged one,
+    "mfma_wide" the 64 x 64 direct-to-LDS one on 4 waves"""
     if form not in KERNELS:
         gpu.zp_apply_dim(mat, dst, src, dim, form=form, max_chunk=max_chunk)
         return
@@ -75,13 +76,13 @@ def apply(gpu, mat, dst, src, dim, form, max_chunk=0):
     try:
         gpu.zp_apply_dim(mat, dst, src, dim, form="mfma", max_chunk=max_chunk)
     finally:
-        gpu.zp_set_kernel(1)
+        gpu.zp_set_kernel(3)
 
 
 @pytest.mark.parametrize("m,k,nvec,dim,mode", [
     (20, 70, 33, 3, "uniform"), (12, 12, 5, 2, "uniform"), (64, 128, 64, 3, "wide"), (3, 1, 2, 3, "extreme"),
     (33, 200, 65, 2, "extreme"), (16, 64, 16, 3, "max"), (100, 37, 7, 3, "wide")])
-@pytest.mark.parametrize("form", ["mfma", "mfma_l2", "mfma_wide", "plain"])
+@pytest.mark.parametrize("form", ["mfma", "mfma_l2", "mfma_wide", "mfma_lds32", "plain"])
 def test_zp_exact_small(gpu, m, k, nvec, dim, mode, form):
     rng = np.random.default_rng(m * 1000 + k + nvec)
     primes = P57[:2] + SMALL[:1]
@@ -100,7 +101,7 @@ def test_zp_chunked(gpu, max_chunk):
     a = entries(rng, primes, (1, 3, 40, 300), "extreme")
     x = entries(rng, primes, (1, 3, 50, 300), "wide")
     want = exact(a, x, primes, 3)
-    for form in ("mfma", "mfma_wide"):
+    for form in ("mfma", "mfma_wide", "mfma_lds32"):
         assert (run(gpu, a, x, primes, 3, form, max_chunk) == want).all()
 
 
@@ -112,7 +113,7 @@ def test_zp_long_k_at_the_bound(gpu):
     x = np.full((1, 1, 64, 3072), (P57[0] - 1) // 2, dtype=np.int64)
     x[0, 0, :, 5::7] = -((P57[0] - 1) // 2)
     want = exact(a, x, primes, 3)
-    for form in ("mfma", "mfma_l2", "mfma_wide", "plain"):
+    for form in ("mfma", "mfma_l2", "mfma_wide", "mfma_lds32", "plain"):
         assert (run(gpu, a, x, primes, 3, form) == want).all()
 
 
@@ -159,7 +160,7 @@ def test_transform_matrix_first_column_is_the_crt_kernel(gpu, rings, dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("form", ["mfma", "mfma_l2", "mfma_wide", "plain"])
+@pytest.mark.parametrize("form", ["mfma", "mfma_l2", "mfma_wide", "mfma_lds32", "plain"])
 def test_dense_transform_equals_ntt_transform(gpu, rings, dim, form):
     """the dense map along one dimension equals that dimension's ntt_mul + intt pair mod p"""
     g, c = rings
@@ -230,5 +231,5 @@ def test_homcrt_block_contraction(gpu):
     a = first[:, :, idx]
     x = entries(rng, primes, (1, 3, 512, 64), "uniform")
     want = exact(a, x, primes, 3)
-    for form in ("mfma", "mfma_l2", "mfma_wide", "plain"):
+    for form in ("mfma", "mfma_l2", "mfma_wide", "mfma_lds32", "plain"):
         assert (run(gpu, a, x, primes, 3, form) == want).all()
