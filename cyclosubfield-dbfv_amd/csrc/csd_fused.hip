@@ -368,7 +368,7 @@ __device__ __forceinline__ i128 gb_split_sum(const GbSplit& s) {
 // in the four int64 partial sums (worth it for long digits; short ones pay more for the final
 // i128 assembly than they save). WV = 1 stages the digit's weights in LDS once per workgroup
 // (broadcast reads); WV = 0 reads them with scalar loads, for weight tables beyond the LDS.
-template <bool SPLIT, int WV>
+template <bool SPLIT, int WV, bool RS>
 __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
     extern __shared__ int64_t gb_x[];
     int64_t* xs = gb_x + threadIdx.x;
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
                 if (u0 + t < hi) {
                     const int u = u0 + t;
                     const CsdMod mu = a.smods ? a.smods[u] : mt.m[u];
-                    const int64_t xin = a.pre ? csd_mont_mul(mu, raw[t], a.pre[u]) : raw[t];
+                    const int64_t xin = RS ? csd_mont_mul(mu, raw[t], a.pre[u]) : raw[t];
                     const int64_t xv = csd_mont_mul(mu, xin, a.dopi[u]);
                     xs[(u - lo) * 256] = xv;
                     corr = gb_mac(corr, corr_row[u], xv);
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
                     v = (int64_t)((uint64_t)v + (uint64_t)csd_mont_mul(mt.m[j], q, a.dmp[(int64_t)i * rd + j]));
                     v = csd_to_mont64(mt.m[j], v);
                     int64_t* oj = o + i * a.sd[0] + j * a.sd[1];
-                    if (a.post_s) {
+                    if (RS) {
                         const int64_t e = csd_mont_mul(mt.m[j], *oj, a.post_s[j]);
                         v = csd_mont_mul(mt.m[j], (int64_t)((uint64_t)e - (uint64_t)v), a.post_v[j]);
                     }
@@ -500,17 +500,20 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
     const size_t lds0 = (size_t)max_len * 256 * sizeof(int64_t);
     a.xs_rows = max_len;
     const size_t lds1 = (size_t)max_len * 256 * sizeof(int64_t) + (size_t)max_len * a.rdp * sizeof(int64_t);
-    csd_allow_lds((const void*)k_bconv_fused<true, 0>, 160 * 1024);
-    csd_allow_lds((const void*)k_bconv_fused<false, 0>, 160 * 1024);
-    csd_allow_lds((const void*)k_bconv_fused<true, 1>, 160 * 1024);
-    csd_allow_lds((const void*)k_bconv_fused<false, 1>, 160 * 1024);
-    const bool split = max_len >= 4;
-    if (lds1 <= 160 * 1024) {  // the digit's weights in LDS: 15-20 % faster than scalar loads
-        if (split) hipLaunchKernelGGL((k_bconv_fused<true, 1>), dim3((unsigned)blocks), dim3(256), lds1, ctx->stream, a, t);
-        else hipLaunchKernelGGL((k_bconv_fused<false, 1>), dim3((unsigned)blocks), dim3(256), lds1, ctx->stream, a, t);
+    const bool split = max_len >= 4, wv = lds1 <= 160 * 1024;  // the digit's weights in LDS: 15-20 % faster
+    auto launch = [&](auto kern) {
+        csd_allow_lds((const void*)kern, 160 * 1024);
+        hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), wv ? lds1 : lds0, ctx->stream, a, t);
+    };
+    if (post_s) {  // the rescale form: one digit of many limbs
+        if (wv) launch(k_bconv_fused<true, 1, true>);
+        else launch(k_bconv_fused<true, 0, true>);
+    } else if (wv) {
+        if (split) launch(k_bconv_fused<true, 1, false>);
+        else launch(k_bconv_fused<false, 1, false>);
     } else {
-        if (split) hipLaunchKernelGGL((k_bconv_fused<true, 0>), dim3((unsigned)blocks), dim3(256), lds0, ctx->stream, a, t);
-        else hipLaunchKernelGGL((k_bconv_fused<false, 0>), dim3((unsigned)blocks), dim3(256), lds0, ctx->stream, a, t);
+        if (split) launch(k_bconv_fused<true, 0, false>);
+        else launch(k_bconv_fused<false, 0, false>);
     }
     HIPCHECK(hipGetLastError());
 }
