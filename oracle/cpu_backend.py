@@ -330,3 +330,25 @@ def plan_apply(plan, data, es=1, vl=1, inverse=False, reduce_output=True):
     assert data.dtype == np.int64 and data.flags["C_CONTIGUOUS"]
     _check(lib().orc_plan_apply(plan.handle, data.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(es),
                                 ctypes.c_size_t(vl), ctypes.c_int(1 if inverse else 0), ctypes.c_int(1 if reduce_output else 0)))
+
+
+def plan_apply_traced(plan, data, es=1, vl=1, inverse=False, reduce_output=True):
+    """plan_apply with the oracle's schedule trace on. Returns (events, barrett_mults): events are
+    dicts with kind ("reduce", "fwd", "inv"), factor, stage, skip_ok, skipped (blocks whose
+    twiddle multiplication was skipped) and fused, in execution order, recorded once per factor
+    (its first line); barrett_mults counts every Barrett multiplication of the whole apply."""
+    _check(lib().orc_ntt_trace(ctypes.c_int(1)))
+    try:
+        plan_apply(plan, data, es, vl, inverse, reduce_output)
+        cap = 4096
+        buf = (ctypes.c_int32 * (6 * cap))()
+        count, mults = ctypes.c_size_t(), ctypes.c_uint64()
+        _check(lib().orc_ntt_trace_get(buf, ctypes.c_size_t(cap), ctypes.byref(count), ctypes.byref(mults)))
+    finally:
+        _check(lib().orc_ntt_trace(ctypes.c_int(0)))
+    kinds = {0: "reduce", 1: "fwd", 2: "inv"}
+    events = []
+    for i in range(count.value):
+        k, f, s, ok, sk, fu = buf[6 * i:6 * i + 6]
+        events.append(dict(kind=kinds[k], factor=f, stage=s, skip_ok=bool(ok), skipped=sk, fused=bool(fu)))
+    return events, mults.value

@@ -164,7 +164,28 @@ static Barrett barrett_new(int64_t value, int64_t n) {
     require(u >= (i128)INT64_MIN && u <= (i128)INT64_MAX, "Barrett constant out of range");
     return Barrett{z, (int64_t)u};
 }
+// Schedule trace of the lazy NTTs (test instrumentation, off by default): while g_trace is set,
+// the first call of each factor in an apply records its reduce passes and per-stage decisions,
+// and every Barrett multiplication of the apply is counted.
+struct TraceEvent {
+    int32_t kind;     // 0 reduce pass, 1 forward stage, 2 inverse stage
+    int32_t factor;   // factor index within the plan
+    int32_t stage;    // stage index (the tree level; inverse stages count down)
+    int32_t skip_ok;  // forward: the twiddle-1 skip was allowed in this stage
+    int32_t skipped;  // number of blocks whose Barrett multiplication was skipped
+    int32_t fused;    // forward: reduce fused into the stage; inverse: len^-1 fused into the stage
+};
+static bool g_trace = false;
+static bool g_trace_rec = false;
+static int32_t g_trace_factor = 0;
+static uint64_t g_trace_mults = 0;
+static std::vector<TraceEvent> g_trace_events;
+static inline void trace_event(int32_t kind, int32_t stage, int32_t skip_ok, int32_t skipped, int32_t fused) {
+    if (g_trace_rec) g_trace_events.push_back(TraceEvent{kind, g_trace_factor, stage, skip_ok, skipped, fused});
+}
+
 static inline int64_t barrett_mul(Barrett b, int64_t x, int64_t n) {
+    if (__builtin_expect(g_trace, 0)) g_trace_mults++;
     int64_t q = (int64_t)(wadd128((i128)b.u * (i128)x, (i128)1 << 63) >> 64);
     return wadd64(wmul64(b.z, x), wmul64(q, n));
 }
@@ -173,6 +194,7 @@ static const u128 I64MAX = (u128)INT64_MAX;
 
 // reduce_pass (ntt.rs:164-173)
 static void reduce_pass(Barrett one, int64_t n, int64_t* data, size_t es, size_t len, size_t vl) {
+    trace_event(0, -1, 0, 0, 0);
     for (size_t i = 0; i < len; i++)
         for (size_t l = 0; l < vl; l++) data[i * es + l] = barrett_mul(one, data[i * es + l], n);
 }
@@ -273,6 +295,11 @@ struct FactorNTT {
                 bool skip_ok = 2 * bound <= I64MAX;
                 u128 nb = fuse ? (u128)n : (skip_ok ? 2 * bound : bound + (u128)n);
                 size_t half = stage_len / 2;
+                if (g_trace_rec) {
+                    int32_t sk = 0;
+                    for (size_t block = 0; block * stage_len < len; block++) sk += (skip_ok && tw[s][block].z == 1);
+                    trace_event(1, (int32_t)s, skip_ok, sk, fuse);
+                }
                 for (size_t block = 0, bs = 0; bs < len; block++, bs += stage_len) {
                     Barrett v = tw[s][block];
                     bool skip = skip_ok && v.z == 1;
@@ -300,6 +327,11 @@ struct FactorNTT {
                 bool skip_ok = 3 * bound <= I64MAX;
                 u128 nb = fuse ? (u128)n : (skip_ok ? 3 * bound : bound + 2 * (u128)n);
                 size_t third = stage_len / 3;
+                if (g_trace_rec) {
+                    int32_t sk = 0;
+                    for (size_t block = 0; block * stage_len < len; block++) sk += (skip_ok && tw[s][6 * block].z == 1);
+                    trace_event(1, (int32_t)s, skip_ok, sk, fuse);
+                }
                 for (size_t block = 0, bs = 0; bs < len; block++, bs += stage_len) {
                     const Barrett* w = &tw[s][6 * block];
                     bool skip = skip_ok && w[0].z == 1;
@@ -342,6 +374,11 @@ struct FactorNTT {
                 bound = fs ? (u128)n : 2 * bound;
                 size_t stage_len = len >> si;
                 size_t half = stage_len / 2;
+                if (g_trace_rec) {
+                    int32_t sk = 0;
+                    for (size_t block = 0; !fs && block * stage_len < len; block++) sk += (itw[si][block].z == 1);
+                    trace_event(2, (int32_t)si, 1, sk, fs != nullptr);
+                }
                 for (size_t block = 0, bs = 0; bs < len; block++, bs += stage_len) {
                     bool fuse = fs != nullptr;
                     Barrett ym = fuse ? *fs : itw[si][block];
@@ -368,6 +405,11 @@ struct FactorNTT {
                 size_t stage_len = len;
                 for (size_t t = 0; t < si; t++) stage_len /= 3;
                 size_t third = stage_len / 3;
+                if (g_trace_rec) {
+                    int32_t sk = 0;
+                    for (size_t block = 0; !fs && block * stage_len < len; block++) sk += (itw[si][2 * block].z == 1);
+                    trace_event(2, (int32_t)si, 1, sk, fs != nullptr);
+                }
                 for (size_t block = 0, bs = 0; bs < len; block++, bs += stage_len) {
                     bool fuse = fs != nullptr;
                     Barrett ym = fuse ? *fs : itw[si][2 * block];
@@ -427,20 +469,24 @@ struct OrcNTTPlan {
             bool fuse_fwd = last && reduce_output;
             suffix_len /= f.len;
             u128 nb = bound;
+            g_trace_factor = (int32_t)fi;
             if (vl == es) {
                 size_t inner = suffix_len * es;
                 for (size_t p = 0; p < prefix_len; p++) {
+                    g_trace_rec = g_trace && p == 0;
                     int64_t* block = data + p * f.len * inner;
                     nb = inverse ? f.backward(block, inner, inner, bound, fs) : f.forward(block, inner, inner, bound, fuse_fwd);
                 }
             } else {
                 for (size_t p = 0; p < prefix_len; p++)
                     for (size_t s = 0; s < suffix_len; s++) {
+                        g_trace_rec = g_trace && p == 0 && s == 0;
                         size_t off = (p * f.len * suffix_len + s) * es;
                         nb = inverse ? f.backward(data + off, suffix_len * es, vl, bound, fs)
                                      : f.forward(data + off, suffix_len * es, vl, bound, fuse_fwd);
                     }
             }
+            g_trace_rec = false;
             prefix_len *= f.len;
             bound = nb;
         }
@@ -568,6 +614,29 @@ int orc_free_ntt_plan(void* plan) {
 }
 int orc_plan_apply(void* plan, int64_t* data, size_t es, size_t vl, int inverse, int reduce_output) {
     ORC_TRY((OrcNTTPlan*)plan)->apply(data, es, vl, inverse != 0, reduce_output != 0);
+    ORC_CATCH
+}
+
+// Schedule trace (test instrumentation): orc_ntt_trace(1) clears and enables it, orc_ntt_trace(0)
+// disables it; orc_ntt_trace_get copies the recorded events (6 int32 each) and the number of
+// Barrett multiplications since it was enabled.
+int orc_ntt_trace(int on) {
+    ORC_TRY g_trace = on != 0;
+    g_trace_rec = false;
+    g_trace_events.clear();
+    g_trace_mults = 0;
+    ORC_CATCH
+}
+
+int orc_ntt_trace_get(int32_t* out, size_t cap, size_t* count, uint64_t* mults) {
+    ORC_TRY* count = g_trace_events.size();
+    *mults = g_trace_mults;
+    require(g_trace_events.size() <= cap, "trace buffer too small");
+    for (size_t i = 0; i < g_trace_events.size(); i++) {
+        const TraceEvent& e = g_trace_events[i];
+        int32_t v[6] = {e.kind, e.factor, e.stage, e.skip_ok, e.skipped, e.fused};
+        for (int k = 0; k < 6; k++) out[6 * i + k] = v[k];
+    }
     ORC_CATCH
 }
 

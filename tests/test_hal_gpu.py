@@ -446,3 +446,22 @@ def test_compiled_row_kernels_parity(gpu, cpu, field, log2_q, inner, mode):
         assert same(from_dev(gpu, dg), dc)
     finally:
         pkg.lib().csd_ntt_force_generic(0)
+
+
+@pytest.mark.parametrize("shape,prime", [((4, 3), P36864[0]), ((2, 3, 8, 64), P36864[0]), ((3, 64, 64), None),
+                                         ((4,), None), ((8, 9, 2), None)])
+def test_plan_schedule_counts_match_oracle(gpu, shape, prime):
+    """The device plan's schedule (csd_ntt_plan_mults) performs exactly the Barrett
+    multiplications the oracle executes per transform, in each direction; the oracle's schedule
+    is pinned against the reference's stated rules in test_oracle_hal."""
+    from test_oracle_hal import _largest_57bit_prime
+
+    n = prime or _largest_57bit_prime(int(np.lcm.reduce(shape)))
+    plan = gpu.hal_create_ntt_plan(gpu.hal_create_modulus(n), list(shape))
+    oplan = cb.CpuNTTPlan(cb.CpuBackend().hal_create_modulus(n), shape)
+    rng = np.random.default_rng(3)
+    for d, (inverse, reduce_output) in {"fwd": (False, True), "fwd_unreduced": (False, False),
+                                        "inv": (True, False)}.items():
+        data = rng.integers(-2 * n, 2 * n, size=oplan.len, dtype=np.int64)
+        _, mults = cb.plan_apply_traced(oplan, data, 1, 1, inverse, reduce_output)
+        assert plan.mults(d) == mults, (shape, d)

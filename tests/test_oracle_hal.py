@@ -616,3 +616,209 @@ def test_oracle_errors(cpu):
         cpu.hal_mul_mont64(t, t, [cpu.hal_create_modulus(97)])
     with pytest.raises(RuntimeError):
         cpu.hal_create_modulus(96)
+
+
+# ---------------------------------------------------------------------------------------------
+# The lazy schedule against the facts the reference states about it (ntt.rs:18-81)
+
+def _is_prime(n):
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % p == 0:
+            return n == p
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d, s = d // 2, s + 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def _largest_57bit_prime(order):
+    """the largest prime below 2^57 that is 1 mod `order` (so NTTs of that order exist)"""
+    k = ((1 << 57) - 2) // order
+    while not _is_prime(k * order + 1):
+        k -= 1
+    return k * order + 1
+
+
+def _schedule_model(shape, n, inverse, reduce_output):
+    """The schedule ntt.rs:18-81 describes, stage by stage, as (kind, factor, stage, skip_ok,
+    fused) tuples: entries start bounded by 2N; a forward stage of radix R adds (R - 1) N to the
+    bound, or multiplies it by R in the one twiddle-1 block, which skips its reduction only
+    while R * bound fits an i64; a reduce pass (back to N) comes only when the next stage could
+    exceed i64::MAX; the last forward stage reduces its output unless unreduced output is asked
+    for. The inverse multiplies the bound by R per stage (the unmultiplied y0 + y1 branch), so it
+    reduces when R * bound would not fit, and fuses len^-1 into its last stage."""
+    MAX = (1 << 63) - 1
+    factors = [(L, 2 if L & (L - 1) == 0 else 3) for L in shape if L > 1]
+    ev = []
+    bound = 2 * n
+    for fi, (L, R) in enumerate(factors):
+        stages = 0
+        while R ** stages < L:
+            stages += 1
+        last = fi + 1 == len(factors)
+        order = range(stages - 1, -1, -1) if inverse else range(stages)
+        for s in order:
+            if not inverse:
+                if bound + (R - 1) * n > MAX:
+                    ev.append(("reduce", fi))
+                    bound = n
+                fuse = reduce_output and last and s == stages - 1
+                ok = R * bound <= MAX
+                ev.append(("fwd", fi, s, ok, fuse))
+                bound = n if fuse else (R * bound if ok else bound + (R - 1) * n)
+            else:
+                if R * bound > MAX:
+                    ev.append(("reduce", fi))
+                    bound = n
+                fuse = last and s == 0
+                ev.append(("inv", fi, s, True, fuse))
+                bound = n if fuse else R * bound
+    return ev
+
+
+def _traced(shape, n, inverse, reduce_output, seed=1):
+    be = cb.CpuBackend()
+    plan = cb.CpuNTTPlan(be.hal_create_modulus(n), shape)
+    rng = np.random.default_rng(seed)
+    data = rng.integers(-2 * n, 2 * n, size=plan.len, dtype=np.int64)
+    events, mults = cb.plan_apply_traced(plan, data, 1, 1, inverse, reduce_output)
+    return events, mults
+
+
+SCHEDULE_PLANS = [
+    # field_36864: dim 2 [4, 3] and dim 3 [2, 3, 8, 64] (field.rs:535-536); field_49152: [4] and
+    # [3, 64, 64]; long one-radix plans for the inverse reduce cadence
+    ((4, 3), 144115176243904321),
+    ((2, 3, 8, 64), 144115176243904321),
+    ((4,), None),
+    ((3, 64, 64), None),
+    ((1 << 14,), None),
+    ((3 ** 8,), None),
+    ((64, 64, 3), None),
+]
+
+
+@pytest.mark.parametrize("shape,prime", SCHEDULE_PLANS)
+def test_ntt_schedule_matches_stated_facts(shape, prime):
+    n = prime or _largest_57bit_prime(int(np.lcm.reduce(shape)))
+    assert (1 << 56) < n < (1 << 57)
+    for inverse, reduce_output in [(False, True), (False, False), (True, False)]:
+        events, mults = _traced(shape, n, inverse, reduce_output)
+        model = _schedule_model(shape, n, inverse, reduce_output)
+        got = [("reduce", e["factor"]) if e["kind"] == "reduce" else
+               (e["kind"], e["factor"], e["stage"], e["skip_ok"], e["fused"]) for e in events]
+        assert got == model, (shape, inverse, reduce_output)
+        # 57-bit moduli: the fields' forward transforms never need a reduce pass (ntt.rs:32-34).
+        # Long one-radix plans do: their twiddle-1 skips double the bound for 5 stages, after
+        # which additive growth crosses i64::MAX, as the rules above imply
+        if not inverse and (prime is not None or shape in [(4,), (3, 64, 64)]):
+            assert not any(e["kind"] == "reduce" for e in events)
+        # every stage has exactly one twiddle-1 block (ntt.rs:39-41); the forward skips it while
+        # allowed, the inverse always (except in the stage that applies len^-1)
+        for e in events:
+            if e["kind"] == "fwd":
+                assert e["skipped"] == (1 if e["skip_ok"] else 0)
+            elif e["kind"] == "inv":
+                assert e["skipped"] == (0 if e["fused"] else 1)
+        # the observed Barrett multiplications equal the count implied by those decisions
+        L = int(np.prod(shape))
+        want = 0
+        for e in events:
+            if e["kind"] == "reduce":
+                want += L
+                continue
+            F = [f for f in shape if f > 1][e["factor"]]
+            R = 2 if F & (F - 1) == 0 else 3
+            nblocks = R ** e["stage"]
+            part = F // nblocks // R
+            lines = L // F
+            if e["kind"] == "fwd":
+                per = part * ((nblocks - e["skipped"]) if R == 2 else (6 * nblocks - 2 * e["skipped"]))
+                per += F if e["fused"] else 0
+            else:
+                if R == 2:
+                    per = F if e["fused"] else part * (nblocks - e["skipped"])
+                else:
+                    per = part * nblocks * 7 if e["fused"] else part * (6 * nblocks - 2 * e["skipped"])
+            want += per * lines
+        assert mults == want, (shape, inverse, reduce_output)
+
+
+def test_inverse_reduce_every_six_radix2_stages():
+    """ntt.rs:73-75: for 57-bit moduli the inverse needs a reduction pass every 6 radix-2 stages"""
+    n = _largest_57bit_prime(1 << 14)
+    events, _ = _traced((1 << 14,), n, True, False)
+    pos, stage_no = [], 0
+    for e in events:
+        if e["kind"] == "reduce":
+            pos.append(stage_no + 1)
+        else:
+            stage_no += 1
+    assert pos == [6, 12]
+    # radix 3 (the bound triples, ntt.rs:67-69): from 2N the first pass comes before stage 4
+    # (3 * 54 N > 2^63), then from N every 3 stages (3 * 27 N > 2^63 > 27 N)
+    n3 = _largest_57bit_prime(3 ** 8)
+    events, _ = _traced((3 ** 8,), n3, True, False)
+    pos, stage_no = [], 0
+    for e in events:
+        if e["kind"] == "reduce":
+            pos.append(stage_no + 1)
+        else:
+            stage_no += 1
+    assert pos == [4, 7]
+
+
+def test_schedule_model_rejects_planted_mistakes():
+    """the comparison above fails for a schedule with an extra reduce pass, a dropped twiddle-1
+    skip or a fused stage moved"""
+    n = 144115176243904321
+    shape = (2, 3, 8, 64)
+    events, _ = _traced(shape, n, False, True)
+    got = [("reduce", e["factor"]) if e["kind"] == "reduce" else
+           (e["kind"], e["factor"], e["stage"], e["skip_ok"], e["fused"]) for e in events]
+    model = _schedule_model(shape, n, False, True)
+    assert got == model
+    assert got[:3] + [("reduce", 1)] + got[3:] != model
+    dropped = list(got)
+    dropped[3] = dropped[3][:3] + (False,) + dropped[3][4:]
+    assert dropped != model
+    moved = list(got)
+    moved[-1] = moved[-1][:4] + (False,)
+    moved[-2] = moved[-2][:4] + (True,)
+    assert moved != model
+
+
+def test_forward_twiddle_one_representatives():
+    """closed forms of the lazy representatives on length 2 (ntt.rs:36-56, 5-16): the only
+    stage's twiddle-1 butterfly adds and subtracts b without reducing it; with reduced output the
+    stage Barrett-multiplies both outputs by 1, i.e. x + round(u x / 2^64) N with
+    u = round(-2^64 / N) (halves away from zero)"""
+    n = Q57
+    be = cb.CpuBackend()
+    u = -((2 * (1 << 64) + n) // (2 * n))
+    plan2 = cb.CpuNTTPlan(be.hal_create_modulus(n), (2,))
+
+    def red(x):
+        q = (u * x + (1 << 63)) >> 64
+        return x + q * n
+
+    for a, b in [(2 * n - 1, 2 * n - 3), (-2 * n, 2 * n), (12345, -(2 * n - 7))]:
+        x = np.array([a, b], dtype=np.int64)
+        cb.plan_apply(plan2, x, 1, 1, False, False)
+        assert x.tolist() == [a + b, a - b]
+        y = np.array([a, b], dtype=np.int64)
+        cb.plan_apply(plan2, y, 1, 1, False, True)
+        assert y.tolist() == [red(a + b), red(a - b)]
+        assert all(abs(v) <= n for v in y.tolist())

@@ -377,3 +377,86 @@ def test_gadget_decompose_recomposes(cpu):
             g = (D // Di) * pow((D // Di) % Di, -1, Di)
             total += d * g
         assert (total - x) % D == 0
+
+
+def _uniform_crt(ring, count, base, rng):
+    t = ring.allocate(count, base)
+    for j, i in enumerate(base):
+        t[:, j] = rng.integers(0, ring.primes[i], size=t[:, j].shape, dtype=np.int64)
+    return t
+
+
+def _zero_secret_galois_keys(ring, base, ndigits):
+    """Galois key of the zero secret key: for gadget digit i the rows (0, g_i) in CRT basis with
+    g_i = (Q/D_i) ((Q/D_i)^-1 mod D_i), the CRT recomposition factor of the digit, so that
+    sum_i d_i g_i = c1 mod Q for the gadget digits d_i of c1."""
+    digits = R.select_digits(ndigits, len(base))
+    q = ring.modulus_of(base)
+    keys = []
+    for a, b in digits:
+        di = ring.modulus_of(base[a:b])
+        g = (q // di) * pow((q // di) % di, -1, di) % q
+        key = ring.allocate(2, base)
+        for j, i in enumerate(base):
+            key[1, j] = R.montgomery_repr(g % ring.primes[i], ring.primes[i])
+        keys.append(key)
+    return keys, digits
+
+
+def _galois_identity_failures(ring, ct, base, g, out):
+    """Under the zero secret key's Galois key, one rotation of bfv_hom_gal (bfv.rs:816-839) maps
+    (c0, c1) to (sigma_g(c0), sigma_g(c1)) mod Q: the key-switch part sum_i sigma_g(d_i) (0, g_i)
+    is (0, sigma_g(sum_i d_i g_i)) = (0, sigma_g(c1)), and sigma_g(c0) is added to row 0.
+    sigma_g is the oracle's apply_galois, pinned by ring.rs:2171-2212. Returns the rows whose
+    residues differ mod p in some limb."""
+    want = ring.allocate(2, base)
+    ring.apply_galois(want, ct, g)
+    bad = []
+    for row in range(2):
+        for j, i in enumerate(base):
+            p = ring.primes[i]
+            d = (out[row, j].astype(object) - want[row, j].astype(object)) % p
+            if any(int(v) != 0 for v in d.reshape(-1)):
+                bad.append(row)
+                break
+    return bad
+
+
+@pytest.mark.parametrize("field,bits,ndigits", [("field_18", (160, 180), 2), ("field_192", (200, 240), 3)])
+def test_hom_gal_one_zero_secret_identity(cpu, field, bits, ndigits):
+    """Pins the oracle's hom_gal_one (the checker of the device Galois key switch): with the zero
+    secret key it must give (sigma_g(c0), sigma_g(c1)) mod Q for every rotation of the homcrt
+    set. The same check rejects the planted mistakes: c0 added unpermuted, the key rows swapped,
+    and gadget digits paired with the wrong key digit."""
+    ring = R.RingContext(cpu, R.FieldDescriptor(field), bits, 4)
+    base = ring.initial_rns_base()
+    assert len(base) >= ndigits
+    rng = np.random.default_rng(11)
+    keys, digits = _zero_secret_galois_keys(ring, base, ndigits)
+    m = ring.field.m
+    gens = [g for g, _ in ring.field.lq_cyclic]
+    gs = [gens[0], gens[-1], pow(gens[-1], 3, m), gens[0] * gens[-1] % m]
+    for trial, g in enumerate(gs):
+        ct = _uniform_crt(ring, 2, base, rng)
+        c1 = ct[1:2].copy()
+        ring.crt_to_pb_tensor(c1, base)
+        parts = R.gadget_decompose(ring, c1, base, digits)
+        out = R.hom_gal_one(ring, ct, base, g, parts, keys, digits)
+        assert _galois_identity_failures(ring, ct, base, g, out) == [], (field, g)
+        if trial:
+            continue
+        # planted: c0 added without the automorphism
+        gal_parts = ring.allocate(parts.shape[0], base)
+        ring.apply_galois(gal_parts, parts, g)
+        acc = ring.allocate_wide(2, base)
+        R.key_switch_inner_product(ring, acc, gal_parts, keys, digits)
+        acc.add(range(0, 1), ct[0:1], 1)
+        assert 0 in _galois_identity_failures(ring, ct, base, g, ring.reduce_wide(acc))
+        # planted: key rows swapped
+        swapped = [np.ascontiguousarray(k[::-1]) for k in keys]
+        out_sw = R.hom_gal_one(ring, ct, base, g, parts, swapped, digits)
+        assert _galois_identity_failures(ring, ct, base, g, out_sw) != []
+        # planted: digits crossed with the next digit's key
+        crossed = keys[1:] + keys[:1]
+        out_cr = R.hom_gal_one(ring, ct, base, g, parts, crossed, digits)
+        assert 1 in _galois_identity_failures(ring, ct, base, g, out_cr)
