@@ -397,8 +397,8 @@ def bench_hom_mul(pkg, be, steps, pairs=64, ndigits=9, seed=SEED, cpu=True):
             out = pkg.bfv.bfv_hom_mul(ring, t, cts[4 * k:4 * k + 2], cts[4 * k + 2:4 * k + 4], base, key_list, digits)
         return out
 
-    pkg.ring.set_gadget_form(0)  # the per-pair form runs the reference's HAL sequences throughout
-    pkg.ring.set_bconv_form(0)
+    ring.set_gadget_form(0)  # the per-pair form runs the reference's HAL sequences throughout
+    ring.set_bconv_form(0)
     try:
         first = step()  # warm-up (also builds the bconv and transform caches)
         be.sync()
@@ -407,8 +407,8 @@ def bench_hom_mul(pkg, be, steps, pairs=64, ndigits=9, seed=SEED, cpu=True):
             step()
         ms = ev.stop(stream) / steps
     finally:
-        pkg.ring.set_gadget_form(1)
-        pkg.ring.set_bconv_form(1)
+        ring.set_gadget_form(1)
+        ring.set_bconv_form(1)
     # the batched form: every HAL call covers all pairs, the fused gadget conversion included
     # pair k sits at rows 4k, 4k + 1 (lhs) and 4k + 2, 4k + 3 (rhs) of `cts`; gather the two sides
     lhs_all = be.hal_allocate_i64((2 * pairs, R, n2, n3))
@@ -797,14 +797,14 @@ def main():
     be.set_timing(False)
     kernels = be.timing_report()
     # the two-stream half-batch overlap (off by default), timed beside the default serial passes
-    pkg.ring.set_overlap(True)
+    ring.set_overlap(True)
     step()  # warm-up: creates the second stream
     be.sync()
     ev.start(stream)
     for _ in range(max(2, args.steps // 2)):
         step()
     two_stream_ms = ev.stop(stream) / max(2, args.steps // 2)
-    pkg.ring.set_overlap(False)
+    ring.set_overlap(False)
 
     # size-independent check: every step is a round trip, so the batch is unchanged mod p
     out = be.hal_export(data[0:1]).reshape(host[0:1].shape)

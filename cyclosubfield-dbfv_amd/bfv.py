@@ -6,9 +6,6 @@ representatives are almost reduced (bound 1) unless stated. Each function names 
 lines it follows; the HAL call sequence is the reference's, so results are bit-identical with any
 backend that is bit-identical per call."""
 
-from .ring import bconv_preserves_source
-
-
 def select_digits(digits, rns_base_len):
     """select_digits (bfv.rs:388-401): `digits` contiguous prime ranges, the first ones longer."""
     if not 0 < digits <= rns_base_len:
@@ -36,12 +33,9 @@ def lift_to_extended(ring, pb, base, crt=None):
     ext = ring.extension_rns_base_for(base)
     lifted = list(base) + ext
     result = ring.allocate_uninit(pb.shape[0], lifted)  # every part is written below
-    if bconv_preserves_source():  # the fused conversion leaves pb (a separate buffer) unchanged
-        tmp = pb
-    else:
-        tmp = ring.allocate_uninit(pb.shape[0], base)
-        be.hal_assign(tmp, pb)  # bconv destroys its input
-    ring.bconv_tensors(result[:, base_len:], ext, tmp, base)
+    # the reference converts a copy of pb (bconv destroys its input); preserve_src leaves pb
+    # unchanged in every conversion form, copying only where the form needs it
+    ring.bconv_tensors(result[:, base_len:], ext, pb, base, preserve_src=True)
     if crt is not None:
         ring.pb_to_crt_tensor(result[:, base_len:], ext)
         be.hal_assign(result[:, :base_len], crt)
@@ -51,11 +45,15 @@ def lift_to_extended(ring, pb, base, crt=None):
     return result, lifted
 
 
-def rescale_to_base(ring, x, lifted, base_len, t):
+def rescale_to_base(ring, x, lifted, base_len, t, bound=1):
     """rescale_to_base (bfv.rs:601-622): round(t x / q') over q' = base for the period-basis
-    batch `x` over `lifted` (destroyed)."""
+    batch `x` over `lifted` (destroyed), whose representatives are bounded by `bound` p. Like the
+    reference's mul_scalar (ring.rs:923-924, assert_mul_mont_safe) it rejects bounds with
+    bound * max p > 2^63."""
     count = x.shape[0]
     base, ext = lifted[:base_len], lifted[base_len:]
+    if bound * max(ring.primes[i] for i in lifted) > 1 << 63:
+        raise ValueError("representatives too large for a montgomery multiplication")
     s = ring.scalar_tensor(lifted, t)   # mul_scalar, then bconv, sub and the inverse's mul in one step
     ring.rescale_step(x[:, base_len:], ext, x[:, :base_len], base, s[:, :base_len], s[:, base_len:],
                       ring.inverse_product_tensor(ext, base))

@@ -483,7 +483,9 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
     a.pre = pre;
     a.post_s = post_s;
     a.post_v = post_v;
-    if ((post_s == nullptr) != (post_v == nullptr) || (post_s && d != 1)) throw CsdError("fused bconv: rescale form");
+    // the rescale form takes pre, post_s and post_v together (the kernel reads all three)
+    if ((pre == nullptr) != (post_s == nullptr) || (post_s == nullptr) != (post_v == nullptr) || (post_s && d != 1))
+        throw CsdError("fused bconv: the rescale form needs pre, post_s and post_v together, and one digit");
     if (!smods_dev && a.rd != a.r) throw CsdError("fused bconv: source moduli missing");
     if (rdp < a.rd || rdp % GB_GROUP) throw CsdError("fused bconv: transposed weights not padded");
     for (int64_t i = 0; i < d; i++) {

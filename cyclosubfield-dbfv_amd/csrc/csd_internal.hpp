@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -51,6 +52,12 @@ struct csd_ctx {
         double bytes;
     };
     std::vector<TimedLaunch> timed;
+    // launch-form settings of this context (no reference counterpart; every setting computes
+    // identical results): csd_ntt_force_generic, csd_ntt_set_row_shape, csd_zp_set_kernel
+    std::atomic<int> ntt_mode{0};
+    std::atomic<int> row_lines{0};
+    std::atomic<int> row_threads{0};
+    std::atomic<int> zp_kernel{3};
 
     void* alloc_bytes(size_t bytes);
     void free_bytes(void* p);

@@ -1358,42 +1358,36 @@ static ColKernel pick_col_kernel(const DevPlan& P, bool inv) {
     }
 }
 
-static std::atomic<bool> g_use_generic_ntt{false};  // forces the one-stage-per-LDS-pass kernel (tests)
-static std::atomic<bool> g_use_plan_rows{false};    // keeps dim-3 transforms on the plan-driven row kernel (tests)
-static std::atomic<int> g_row_lines{0};             // dim-3 lines per workgroup, 0 = automatic (csd_ntt_set_row_shape)
-static std::atomic<int> g_row_threads{0};           // dim-3 threads per workgroup, 0 = automatic
-
-extern "C" int csd_ntt_set_row_shape(int lines, int threads) {
-    if (lines < 0 || lines > 8 || threads < 0 || threads > 512 || threads % 64 != 0) {
-        csd_set_error("csd_ntt_set_row_shape: lines must be 0-8, threads 0 or a multiple of 64 up to 512");
-        return 1;
-    }
-    g_row_lines = lines;
-    g_row_threads = threads;
-    return 0;
+// ctx->row_lines / row_threads: dim-3 lines and threads per workgroup, 0 = automatic
+extern "C" int csd_ntt_set_row_shape(csd_ctx* ctx, int lines, int threads) {
+    CSD_TRY if (!ctx) throw CsdError("null context");
+    if (lines < 0 || lines > 8 || threads < 0 || threads > 512 || threads % 64 != 0)
+        throw CsdError("csd_ntt_set_row_shape: lines must be 0-8, threads 0 or a multiple of 64 up to 512");
+    ctx->row_lines = lines;
+    ctx->row_threads = threads;
+    CSD_CATCH
 }
 
-extern "C" int csd_ntt_force_generic(int on) {
-    if (on < 0 || on > 2) {
-        csd_set_error("csd_ntt_force_generic: mode must be 0, 1 or 2");
-        return 1;
-    }
-    g_use_generic_ntt = on == 1;
-    g_use_plan_rows = on == 2;
-    return 0;
+// ctx->ntt_mode: 1 forces the one-stage-per-LDS-pass kernel, 2 keeps dim-3 transforms on the
+// plan-driven row kernel (tests)
+extern "C" int csd_ntt_force_generic(csd_ctx* ctx, int mode) {
+    CSD_TRY if (!ctx) throw CsdError("null context");
+    if (mode < 0 || mode > 2) throw CsdError("csd_ntt_force_generic: mode must be 0, 1 or 2");
+    ctx->ntt_mode = mode;
+    CSD_CATCH
 }
 
 typedef void (*RowKernel)(RowArgs, ModTable);
 
 // the dim-3 row kernel for these plans: a compiled shape when every plan matches one, else the
 // plan-driven form; the dynamic LDS limit is raised once per kernel
-static RowKernel pick_row_kernel(csd_ntt_plan* const* plans, size_t nplans, int dir, int* shape_out = nullptr) {
+static RowKernel pick_row_kernel(csd_ctx* ctx, csd_ntt_plan* const* plans, size_t nplans, int dir, int* shape_out = nullptr) {
     static const RowKernel ks[2][3] = {{k_ntt_rows<false, 0>, k_ntt_rows<false, 1>, k_ntt_rows<false, 2>},
                                        {k_ntt_rows<true, 0>, k_ntt_rows<true, 1>, k_ntt_rows<true, 2>}};
     for (auto& row : ks)
         for (RowKernel k : row) csd_allow_lds((const void*)k, 160 * 1024);
     int shape = 0;
-    if (!g_use_plan_rows && nplans > 0) {
+    if (ctx->ntt_mode != 2 && nplans > 0) {
         for (int c : {1, 2}) {
             bool all = true;
             for (size_t jj = 0; jj < nplans && all; jj++) all = ct_shape_matches(plans[jj]->host, dir, c);
@@ -1443,7 +1437,7 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
     if (numel(dst) == 0) return;
     static const char* names[2][3] = {{"ntt_dim2", "ntt_mul_dim2", "intt_dim2"}, {"ntt_dim3", "ntt_mul_dim3", "intt_dim3"}};
     const bool inv = dir == 2;
-    if (!g_use_generic_ntt && dim == 2) {
+    if (ctx->ntt_mode != 1 && dim == 2) {
         // register-resident columns, if every plan has the same supported factor list
         ColKernel k = pick_col_kernel(plans[0]->host, inv);
         for (size_t jj = 1; jj < nplans && k; jj++) {
@@ -1474,7 +1468,7 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
             return;
         }
     }
-    if (!g_use_generic_ntt && dim == 3 && L <= 16384) {
+    if (ctx->ntt_mode != 1 && dim == 3 && L <= 16384) {
         RowArgs r;
         memset(&r, 0, sizeof(r));
         r.d0 = a.d0;
@@ -1491,14 +1485,15 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         // lengths keep about 6144 coefficients per workgroup.
         const bool measured_3072 = L == 3072;
         int shape = 0;
-        RowKernel rk = pick_row_kernel(plans, nplans, dir, &shape);
+        RowKernel rk = pick_row_kernel(ctx, plans, nplans, dir, &shape);
         const uint32_t lline = row_lds_line(shape, (uint32_t)L);
         int rows = 1;
         while (!measured_3072 && rows * 2 * L <= 6144 && rows * 2 <= a.d2 && rows < 256) rows *= 2;
-        if (g_row_lines > 0) {
+        const int row_lines = ctx->row_lines, row_threads = ctx->row_threads;
+        if (row_lines > 0) {
             // the requested lines, capped by the batch and by what fits the 160 KiB of LDS
             const int64_t fit = std::max<int64_t>(1, (160 * 1024) / ((int64_t)lline * 8));
-            rows = (int)std::min<int64_t>(std::min<int64_t>(g_row_lines, a.d2), fit);
+            rows = (int)std::min<int64_t>(std::min<int64_t>(row_lines, a.d2), fit);
         }
         r.rows = rows;
         // block size: the candidate with the best (resident threads per CU) x (fraction of
@@ -1528,8 +1523,8 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
                 }
             }
         }
-        if (measured_3072 && g_row_lines == 0) threads = 256;
-        if (g_row_threads > 0) threads = g_row_threads;
+        if (measured_3072 && row_lines == 0) threads = 256;
+        if (row_threads > 0) threads = row_threads;
         r.magic_L = magic_for((uint32_t)L);
         const int Ms[5] = {2, 4, 8, 3, 9};
         for (int q = 0; q < 5; q++) r.magic_groups[q] = L % Ms[q] == 0 ? magic_for((uint32_t)(L / Ms[q])) : 0u;
@@ -1644,7 +1639,7 @@ void csd_ntt_segments(csd_ctx* ctx, int64_t* const* seg_base, const int32_t* seg
     r.seg_base = seg_base;
     r.seg_limb = seg_limb;
     int shape = 0;
-    RowKernel rk = pick_row_kernel(plans, nplans, dir, &shape);
+    RowKernel rk = pick_row_kernel(ctx, plans, nplans, dir, &shape);
     size_t lds = (size_t)rows * row_lds_line(shape, (uint32_t)L) * 8;
     if (lds > 160 * 1024) throw CsdError("work-list ntt: line too long for LDS staging");
     const int threads = tune.threads > 0 ? tune.threads : (L * rows >= 8192 ? 512 : 256);

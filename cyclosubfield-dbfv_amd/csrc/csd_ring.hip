@@ -467,9 +467,6 @@ struct GadgetCache {  // ring.rs:1714-1727
     int64_t rdp = 0;
 };
 
-static std::atomic<bool> g_overlap_halves{false};  // see csd_ring_set_overlap
-static std::atomic<bool> g_gadget_fused{true};    // see csd_ring_set_gadget_form
-static std::atomic<bool> g_bconv_fused{true};     // see csd_ring_set_bconv_form
 
 // Whether a and b are disjoint limb ranges of one tensor (or do not overlap at all), so that
 // a kernel whose thread owns one (element, position) of both may read b and rewrite a in place:
@@ -532,6 +529,11 @@ struct csd_ring {
     std::map<std::pair<std::vector<size_t>, std::vector<size_t>>, std::unique_ptr<GadgetCache>> gcache;
     std::map<std::vector<int64_t>, std::pair<csd_perm_plan*, csd_perm_plan*>> pcache;
     std::map<std::pair<std::vector<size_t>, std::vector<size_t>>, std::unique_ptr<DevBuf>> inv_scalar_cache;
+    // the ring's execution-form settings (every form computes identical residues): see
+    // csd_ring_set_overlap, csd_ring_set_bconv_form and csd_ring_set_gadget_form
+    std::atomic<bool> overlap_halves{false};
+    std::atomic<bool> bconv_fused{true};
+    std::atomic<bool> gadget_fused{true};
 
     ~csd_ring() {
         if (aux_stream) {
@@ -690,7 +692,7 @@ struct csd_ring {
         };
         const int64_t cnt = data.shape[0];
         const int64_t slice_bytes = r * (int64_t)f->deg_lp * (int64_t)f->deg_k * 8;
-        if (!g_overlap_halves || cnt < 2 || cnt * slice_bytes < ((int64_t)64 << 20)) {
+        if (!overlap_halves || cnt < 2 || cnt * slice_bytes < ((int64_t)64 << 20)) {
             for (int q = 0; q < 4; q++) step(data, q);
             return;
         }
@@ -797,27 +799,32 @@ struct csd_ring {
         return ref;
     }
 
-    // whether bconv runs the fused kernel (which leaves src unchanged) for these operands
-    static bool fused_bconv_applies(const BConvCache& c, int64_t r, const csd_view& dst, const csd_view& src) {
-        return g_bconv_fused && c.acc >= (size_t)r && !views_overlap(dst, src, 8);
-    }
-
     // BConvCache::bconv (ring.rs:1663-1701). The fused form (default, when src and dst do not
-    // overlap) is one kernel and leaves src as it is; the HAL sequence destroys src.
-    void bconv(const csd_view& dst, const std::vector<size_t>& dbase, const csd_view& src, const std::vector<size_t>& sbase) {
+    // overlap) is one kernel and leaves src as it is; the HAL sequence destroys src, like the
+    // reference, unless preserve_src is set, when it runs on a copy. The form is decided once per
+    // call, so what happens to src never depends on a later change of the setting.
+    void bconv(const csd_view& dst, const std::vector<size_t>& dbase, const csd_view& src_in, const std::vector<size_t>& sbase,
+               bool preserve_src = false) {
         BConvCache& c = bconv_cache(sbase, dbase);
         int64_t r = (int64_t)sbase.size(), rd = (int64_t)dbase.size();
-        int64_t cnt = src.shape[0], n2 = src.shape[2], n3 = src.shape[3];
-        if (src.shape[1] != r) throw CsdError("bconv: source doesn't match its base");
+        int64_t cnt = src_in.shape[0], n2 = src_in.shape[2], n3 = src_in.shape[3];
+        if (src_in.shape[1] != r) throw CsdError("bconv: source doesn't match its base");
         if (dst.shape[0] != cnt || dst.shape[1] != rd || dst.shape[2] != n2 || dst.shape[3] != n3)
             throw CsdError("bconv: destination shape");
         auto smods = mods_for(sbase), dmods = mods_for(dbase);
-        if (fused_bconv_applies(c, r, dst, src)) {
+        if (bconv_fused && c.acc >= (size_t)r && !views_overlap(dst, src_in, 8)) {
             const std::vector<size_t> whole = {0, (size_t)r};
-            csd_bconv_fused_launch(ctx, dst, src, 1, (const int64_t*)c.q_over_p_inv->p, (const int64_t*)c.matrix->p,
+            csd_bconv_fused_launch(ctx, dst, src_in, 1, (const int64_t*)c.q_over_p_inv->p, (const int64_t*)c.matrix->p,
                                    (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.q_mod_p->p, whole,
                                    c.log2_qtilde, dmods.data(), (const CsdMod*)c.src_mods->p, "bconv");
             return;
+        }
+        std::unique_ptr<DevBuf> copy;
+        csd_view src = src_in;
+        if (preserve_src) {
+            copy.reset(new DevBuf(ctx, (size_t)cnt * r * n2 * n3 * 8));
+            src = make_view(copy->p, cnt, r, n2, n3);
+            ok(csd_hal_assign(ctx, &src, &src_in));
         }
         csd_view qopi = make_view(c.q_over_p_inv->p, 1, r, 1, 1);
         ok(csd_hal_mul_mont64(ctx, &src, &qopi, smods.data(), r));
@@ -862,7 +869,7 @@ struct csd_ring {
         };
         if (!scalar_ok(s_src, r) || !scalar_ok(s_ext, re) || !scalar_ok(inv, re)) throw CsdError("rescale: scalar tensor shape");
         auto smods = mods_for(sbase), emods = mods_for(ebase);
-        if (g_bconv_fused && c.acc >= (size_t)r && limb_disjoint(ext, src)) {
+        if (bconv_fused && c.acc >= (size_t)r && limb_disjoint(ext, src)) {
             const std::vector<size_t> whole = {0, (size_t)r};
             csd_bconv_fused_launch(ctx, ext, src, 1, (const int64_t*)c.q_over_p_inv->p, (const int64_t*)c.matrix->p,
                                    (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.q_mod_p->p, whole,
@@ -954,7 +961,7 @@ struct csd_ring {
         if (dst.shape[0] != B * d || dst.shape[1] != r || dst.shape[2] != n2 || dst.shape[3] != n3)
             throw CsdError("gadget bconv: destination shape");
         auto m = mods_for(base);
-        if (g_gadget_fused && c.acc >= (size_t)r && !views_overlap(dst, src, 8)) {
+        if (gadget_fused && c.acc >= (size_t)r && !views_overlap(dst, src, 8)) {
             csd_bconv_fused_launch(ctx, dst, src, d, (const int64_t*)c.d_over_p_inv->p, (const int64_t*)c.matrix->p,
                                    (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.d_mod_p->p, bounds,
                                    c.log2_qtilde, m.data(), nullptr, "gadget_bconv");
@@ -1024,14 +1031,7 @@ struct csd_ring {
         DevBuf tmp(ctx, (size_t)cnt * nl * n2 * n3 * 8);
         csd_view vt = make_view(tmp.p, cnt, nl, n2, n3);
         csd_view src_dropped = sub(src, 1, nl, r, 8);
-        if (fused_bconv_applies(bconv_cache(dropped, kept), rdr, vt, src_dropped)) {
-            bconv(vt, kept, src_dropped, dropped);  // the fused conversion leaves its source unchanged
-        } else {
-            DevBuf dpart(ctx, (size_t)cnt * rdr * n2 * n3 * 8);
-            csd_view vd = make_view(dpart.p, cnt, rdr, n2, n3);
-            ok(csd_hal_assign(ctx, &vd, &src_dropped));
-            bconv(vt, kept, vd, dropped);
-        }
+        bconv(vt, kept, src_dropped, dropped, true);  // src is left unchanged
         ok(csd_hal_sub_i64(ctx, &dst, &vt));
         DevBuf& inv = inverse_product_tensor(kept, dropped);
         csd_view vinv = make_view(inv.p, 1, nl, 1, 1);
@@ -1378,10 +1378,10 @@ extern "C" int csd_ring_crt_to_pb(csd_ring* ring, const csd_view* data, const si
 }
 
 extern "C" int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t* dst_base, size_t rd, const csd_view* src,
-                              const size_t* src_base, size_t rs) {
+                              const size_t* src_base, size_t rs, int preserve_src) {
     CSD_TRY check_view(dst, "bconv dst");
     check_view(src, "bconv src");
-    ring->bconv(*dst, to_vec(dst_base, rd), *src, to_vec(src_base, rs));
+    ring->bconv(*dst, to_vec(dst_base, rd), *src, to_vec(src_base, rs), preserve_src != 0);
     CSD_CATCH
 }
 
@@ -1429,22 +1429,25 @@ extern "C" int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, siz
     CSD_CATCH
 }
 
-extern "C" int csd_ring_set_overlap(int on) {
-    g_overlap_halves = on != 0;
-    return 0;
-}
-
-extern "C" int csd_ring_get_bconv_form(void) { return g_bconv_fused ? 1 : 0; }
-
-extern "C" int csd_ring_set_bconv_form(int form) {
-    CSD_TRY if (form != 0 && form != 1) throw CsdError("bconv form must be 0 (HAL sequence) or 1 (fused)");
-    g_bconv_fused = form == 1;
+extern "C" int csd_ring_set_overlap(csd_ring* ring, int on) {
+    CSD_TRY if (!ring) throw CsdError("null ring");
+    ring->overlap_halves = on != 0;
     CSD_CATCH
 }
 
-extern "C" int csd_ring_set_gadget_form(int form) {
-    CSD_TRY if (form != 0 && form != 1) throw CsdError("gadget bconv form must be 0 (HAL sequence) or 1 (fused)");
-    g_gadget_fused = form == 1;
+extern "C" int csd_ring_get_bconv_form(const csd_ring* ring) { return ring && ring->bconv_fused ? 1 : 0; }
+
+extern "C" int csd_ring_set_bconv_form(csd_ring* ring, int form) {
+    CSD_TRY if (!ring) throw CsdError("null ring");
+    if (form != 0 && form != 1) throw CsdError("bconv form must be 0 (HAL sequence) or 1 (fused)");
+    ring->bconv_fused = form == 1;
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_set_gadget_form(csd_ring* ring, int form) {
+    CSD_TRY if (!ring) throw CsdError("null ring");
+    if (form != 0 && form != 1) throw CsdError("gadget bconv form must be 0 (HAL sequence) or 1 (fused)");
+    ring->gadget_fused = form == 1;
     CSD_CATCH
 }
 

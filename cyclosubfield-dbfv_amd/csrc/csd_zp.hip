@@ -568,16 +568,13 @@ __global__ __launch_bounds__(256) void k_zp_mfma_wide(MfmaArgs a, ModTable mt) {
     }
 }
 
-static std::atomic<int> g_zp_kernel{3};  // the MFMA kernel form 0 uses: 0 L2 operands, 1 LDS 32 x 64,
-                                         // 2 LDS 64 x 64 on 4 waves, 3 (default) LDS 64 x 64 on 8 waves
-
-extern "C" int csd_zp_set_kernel(int form) {
-    if (form < 0 || form > 3) {
-        csd_set_error("csd_zp_set_kernel: form must be 0, 1, 2 or 3");
-        return 1;
-    }
-    g_zp_kernel = form;
-    return 0;
+// ctx->zp_kernel: the MFMA kernel form 0 uses: 0 L2 operands, 1 LDS 32 x 64, 2 LDS 64 x 64 on 4
+// waves, 3 (default) LDS 64 x 64 on 8 waves
+extern "C" int csd_zp_set_kernel(csd_ctx* ctx, int form) {
+    CSD_TRY if (!ctx) throw CsdError("null context");
+    if (form < 0 || form > 3) throw CsdError("csd_zp_set_kernel: form must be 0, 1, 2 or 3");
+    ctx->zp_kernel = form;
+    CSD_CATCH
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -787,20 +784,21 @@ extern "C" int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* z, const csd_
         if (a.nblocks > (int64_t)INT32_MAX) throw CsdError("zp apply: too many tiles");
         TimedScope ts(ctx, "zp_mfma",
                       (double)z->r * ZP_DIGITS * (z->mpad * z->kpad + npad * z->kpad) + 8.0 * z->r * nvec * z->m);
-        if (g_zp_kernel == 3) {
+        const int zk = ctx->zp_kernel;
+        if (zk == 3) {
             a.mtiles = z->mpad / 64;  // mpad is a multiple of 64
             a.nblocks = z->r * a.mtiles * a.ntiles;
             const size_t lds = 2 * (size_t)(ZP8_A_BYTES + ZP8_X_BYTES);
             csd_allow_lds((const void*)k_zp_mfma_lds64, (int)lds);
             hipLaunchKernelGGL(k_zp_mfma_lds64, dim3((unsigned)a.nblocks), dim3(ZP8_THREADS), lds, ctx->stream, a, mt);
-        } else if (g_zp_kernel == 2) {
+        } else if (zk == 2) {
             // 64-row tiles over the same planes (mpad is a multiple of 64)
             a.mtiles = z->mpad / 64;
             a.nblocks = z->r * a.mtiles * a.ntiles;
             const size_t lds = 2 * (size_t)(ZPW_A_BYTES + ZPW_X_BYTES);
             csd_allow_lds((const void*)k_zp_mfma_wide, (int)lds);
             hipLaunchKernelGGL(k_zp_mfma_wide, dim3((unsigned)a.nblocks), dim3(ZPW_THREADS), lds, ctx->stream, a, mt);
-        } else if (g_zp_kernel == 1) {
+        } else if (zk == 1) {
             const size_t lds = 2 * (size_t)(ZPL_A_BYTES + ZPL_X_BYTES);
             csd_allow_lds((const void*)k_zp_mfma_lds, (int)lds);
             hipLaunchKernelGGL(k_zp_mfma_lds, dim3((unsigned)a.nblocks), dim3(ZPL_THREADS), lds, ctx->stream, a, mt);

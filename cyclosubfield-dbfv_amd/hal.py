@@ -244,11 +244,16 @@ class HipBackend:
         check(lib().csd_ctx_stream(self.ctx, ctypes.byref(s)))
         return s.value
 
-    @staticmethod
-    def ntt_set_row_shape(lines=0, threads=0):
-        """launch shape of dimension-3 transforms (process-wide): lines and threads per workgroup,
+    def ntt_set_row_shape(self, lines=0, threads=0):
+        """launch shape of this context's dimension-3 transforms: lines and threads per workgroup,
         0 for the automatic choice; every shape computes identical results"""
-        check(lib().csd_ntt_set_row_shape(ctypes.c_int(lines), ctypes.c_int(threads)))
+        check(lib().csd_ntt_set_row_shape(self.ctx, ctypes.c_int(lines), ctypes.c_int(threads)))
+
+    def ntt_force_generic(self, mode):
+        """NTT kernel selection of this context (csd_ntt_force_generic): 0 the register-blocked
+        kernels with compiled shapes, 1 the one-stage-per-pass kernel, 2 the plan-driven row
+        kernel; all compute identical results"""
+        check(lib().csd_ntt_force_generic(self.ctx, ctypes.c_int(int(mode))))
 
     def set_timing(self, on):
         """Records HIP events around the library's timed launches on the context stream."""
@@ -505,13 +510,12 @@ class HipBackend:
                                      ctypes.byref(src.view()), ctypes.c_int(_DIM[dim]),
                                      ctypes.c_int({"mfma": 0, "plain": 1}[form]), ctypes.c_size_t(max_chunk)))
 
-    @staticmethod
-    def zp_set_kernel(form):
-        """which int8 MFMA kernel the "mfma" form runs (process-wide, csd_zp_set_kernel): 3 (the
+    def zp_set_kernel(self, form):
+        """which int8 MFMA kernel this context's "mfma" form runs (csd_zp_set_kernel): 3 (the
         default) LDS-staged 64 x 64 tiles for 8 waves with 2 blocks each, 1 LDS-staged 32 x 64 tiles
         for 8 waves, 2 direct-to-LDS 64 x 64 tiles for 4 waves with 2 x 2 blocks each, 0 operands
         loaded by 4 waves straight from L2"""
-        check(lib().csd_zp_set_kernel(ctypes.c_int(int(form))))
+        check(lib().csd_zp_set_kernel(self.ctx, ctypes.c_int(int(form))))
 
     def hal_create_permutation(self, inner_shape, shifts):
         n = len(inner_shape)

@@ -50,31 +50,6 @@ def sample_primes(name, log2_q, t_bits):
     return [out[i] for i in range(n.value)], il.value
 
 
-def set_overlap(on):
-    """Two-stream half-batch overlap of the transforms (csd_ring_set_overlap, off by default);
-    results are identical either way."""
-    check(lib().csd_ring_set_overlap(ctypes.c_int(1 if on else 0)))
-
-
-def set_bconv_form(form):
-    """How bconv_tensors (and the conversions inside the ring sequences) run
-    (csd_ring_set_bconv_form, process-wide): 1 (default) one fused kernel, 0 the reference's HAL
-    call sequence; residues are identical."""
-    check(lib().csd_ring_set_bconv_form(ctypes.c_int(int(form))))
-
-
-def bconv_preserves_source():
-    """Whether bconv_tensors leaves its source unchanged (the fused form, when source and
-    destination do not overlap); the HAL sequence destroys it, like the reference."""
-    return lib().csd_ring_get_bconv_form() == 1
-
-
-def set_gadget_form(form):
-    """How bconv_gadget_tensors runs (csd_ring_set_gadget_form, process-wide): 1 (default) one
-    fused kernel, 0 the reference's HAL call sequence per element; residues are identical."""
-    check(lib().csd_ring_set_gadget_form(ctypes.c_int(int(form))))
-
-
 class RingContext:
     def __init__(self, backend, field, log2_q=None, t_bits=0, raw_primes=None):
         self.backend = backend
@@ -135,10 +110,34 @@ class RingContext:
         b, n = _sizes(base)
         check(lib().csd_ring_crt_to_pb(self.handle, ctypes.byref(data.view()), b, n))
 
-    def bconv_tensors(self, dst, dst_base, src, src_base):
+    def bconv_tensors(self, dst, dst_base, src, src_base, preserve_src=False):
+        """bconv_tensors (ring.rs:1526-1535). Like the reference it may destroy src; with
+        preserve_src it leaves src unchanged in every form (the fused kernel does anyway, the HAL
+        sequence then runs on a copy)."""
         db, dn = _sizes(dst_base)
         sb, sn = _sizes(src_base)
-        check(lib().csd_ring_bconv(self.handle, ctypes.byref(dst.view()), db, dn, ctypes.byref(src.view()), sb, sn))
+        check(lib().csd_ring_bconv(self.handle, ctypes.byref(dst.view()), db, dn, ctypes.byref(src.view()), sb, sn,
+                                   ctypes.c_int(1 if preserve_src else 0)))
+
+    def set_overlap(self, on):
+        """Two-stream half-batch overlap of this ring's transforms (csd_ring_set_overlap, off by
+        default); results are identical either way."""
+        check(lib().csd_ring_set_overlap(self.handle, ctypes.c_int(1 if on else 0)))
+
+    def set_bconv_form(self, form):
+        """How this ring's bconv_tensors (and the conversions inside its sequences) run
+        (csd_ring_set_bconv_form): 1 (default) one fused kernel, 0 the reference's HAL call
+        sequence; residues are identical."""
+        check(lib().csd_ring_set_bconv_form(self.handle, ctypes.c_int(int(form))))
+
+    def bconv_form(self):
+        """the current csd_ring_set_bconv_form setting of this ring (1 fused, 0 HAL sequence)"""
+        return lib().csd_ring_get_bconv_form(self.handle)
+
+    def set_gadget_form(self, form):
+        """How this ring's bconv_gadget_tensors runs (csd_ring_set_gadget_form): 1 (default) one
+        fused kernel, 0 the reference's HAL call sequence per element; residues are identical."""
+        check(lib().csd_ring_set_gadget_form(self.handle, ctypes.c_int(int(form))))
 
     def rescale_step(self, ext, ext_base, src, src_base, s_src, s_ext, inv):
         """The middle of rescale_to_base (bfv.rs:601-622) for a batch: ext <- mont(mont(ext, s_ext)

@@ -134,12 +134,12 @@ int csd_hal_ntt_mul_mont64_dim(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* 
  * field_36864 / field_49152 K parts where the plans match them; 1 forces the one-stage-per-LDS-pass
  * kernel for every plan; 2 keeps the register-blocked kernels but reads every pass list from the
  * plan. All produce identical results; the switch exists to test and time each form. */
-int csd_ntt_force_generic(int on);
+int csd_ntt_force_generic(csd_ctx* ctx, int mode); /* per context */
 /* Launch shape of dimension-3 transforms of lines that fit LDS: lines per workgroup (1-8, capped
  * by the batch and by the 160 KiB of LDS) and threads per workgroup (a multiple of 64 up to 512);
- * 0 keeps the automatic choice. Process-wide; every shape computes identical results. No
+ * 0 keeps the automatic choice. Per context; every shape computes identical results. No
  * counterpart in the reference (a tuning knob). */
-int csd_ntt_set_row_shape(int lines, int threads);
+int csd_ntt_set_row_shape(csd_ctx* ctx, int lines, int threads);
 
 /* ---- base-conversion corrections -------------------------------------------------------------- */
 int csd_hal_bconv_correct_mont64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs,
@@ -173,12 +173,12 @@ int csd_zp_matrix_create(csd_ctx* ctx, const csd_view* a, const csd_modulus* mod
 int csd_zp_matrix_free(csd_zp_matrix* m);
 int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* a, const csd_view* dst, const csd_view* src, int dim, int form,
                      size_t max_chunk);
-/* which int8 MFMA kernel form 0 uses (process-wide): 3 (default) stages each K-step's digit planes
+/* which int8 MFMA kernel form 0 uses (per context): 3 (default) stages each K-step's digit planes
  * of a 64 x 64 tile in LDS for 8 waves with two 16 x 16 output blocks each; 1 stages a 32 x 64
  * tile for 8 waves with one block each; 2 loads them straight into LDS (global_load_lds) for a
  * 64 x 64 tile of 4 waves with 2 x 2 blocks each; 0 has 4 waves load their operands from L2
  * directly, two blocks each. All compute identical results. */
-int csd_zp_set_kernel(int form);
+int csd_zp_set_kernel(csd_ctx* ctx, int form);
 
 #ifdef __cplusplus
 }

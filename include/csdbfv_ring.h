@@ -60,12 +60,13 @@ int csd_ring_crt_to_pb(csd_ring* ring, const csd_view* data, const size_t* base,
  * the other. Every element sees the same passes in the same order, so results are identical
  * either way. Default 0 (serial): with both pass kinds VALU-heavy the overlap measures 1-3 %
  * slower (no reference counterpart). */
-int csd_ring_set_overlap(int on);
+int csd_ring_set_overlap(csd_ring* ring, int on);
 /* bconv_tensors (ring.rs:1526-1535, BConvCache::bconv 1663-1701). The default form is one kernel
  * that leaves src unchanged (used when src and dst do not overlap); the HAL-sequence form destroys
- * src, like the reference. */
+ * src, like the reference, when preserve_src is 0, and works on a copy when it is 1. With
+ * preserve_src = 1, src is unchanged whatever form the ring's setting selects. */
 int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t* dst_base, size_t r_dst, const csd_view* src,
-                   const size_t* src_base, size_t r_src);
+                   const size_t* src_base, size_t r_src, int preserve_src);
 /* Lines 607-616 of rescale_to_base (bfv.rs:601-622) for a batch of count elements: mul_scalar by t
  * (mul_mont64 with s_src / s_ext, the [1, r, 1, 1] scalar tensor of t over the source and extension
  * bases), bconv of src into the extension, sub_i64 and mul_mont64 by inv (the inverse scalar
@@ -80,9 +81,9 @@ int csd_ring_rescale_step(csd_ring* ring, const csd_view* ext, const size_t* ext
 /* 1 (default): csd_ring_bconv and the conversions inside the ring sequences run as one fused
  * kernel; 0: as the reference's HAL call sequence (ring.rs:1663-1701). Residues are identical
  * (no reference counterpart). */
-int csd_ring_set_bconv_form(int form);
+int csd_ring_set_bconv_form(csd_ring* ring, int form);
 /* the current csd_ring_set_bconv_form setting (1 fused, 0 HAL sequence) */
-int csd_ring_get_bconv_form(void);
+int csd_ring_get_bconv_form(const csd_ring* ring);
 /* bconv_gadget_tensors (ring.rs:1548-1557, GadgetBConvCache::bconv 1809-1851); digits are
  * half-open position ranges [digit_bounds[2i], digit_bounds[2i+1]). src is [B, r, ., .] and dst
  * [B ndigits, r, ., .], element b's digit i in row b ndigits + i; the reference takes B = 1, and
@@ -92,7 +93,7 @@ int csd_ring_bconv_gadget(csd_ring* ring, const csd_view* dst, const csd_view* s
                           const size_t* digit_bounds, size_t ndigits);
 /* 1 (default): csd_ring_bconv_gadget runs as one fused kernel; 0: as the reference's HAL call
  * sequence (ring.rs:1809-1851), per element. Residues are identical (no reference counterpart). */
-int csd_ring_set_gadget_form(int form);
+int csd_ring_set_gadget_form(csd_ring* ring, int form);
 /* The digit convolution of dbfv_mul_convolve (dbfv.rs:1096-1127) and, with d1 = d2 = K = 1, the
  * tensor product of hom_mul_raw_acc (bfv.rs:667-683), each followed by reduce_wide (ring.rs:846-865),
  * as one kernel (no reference counterpart; it replaces 3 K inner products or four wide fma calls

@@ -204,11 +204,11 @@ def test_rescale_step_parity(gpu, cpu, rings, field, log2_q, t_bits, ndigits, la
         xd = to_dev(gpu, wide)
         src, ext = to_dev(gpu, np.ascontiguousarray(x[:, :bl])), xd[:, 1:el + 1]
     src_before = from_dev(gpu, src)
-    pkg.ring.set_bconv_form(form)
+    g.set_bconv_form(form)
     try:
         g.rescale_step(ext, lifted[bl:], src, base, s[:, :bl], s[:, bl:], inv)
     finally:
-        pkg.ring.set_bconv_form(1)
+        g.set_bconv_form(1)
     assert same(from_dev(gpu, ext), want)
     if form == 1:
         assert same(from_dev(gpu, src), src_before)  # the fused form leaves the source as it is

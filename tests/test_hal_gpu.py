@@ -360,7 +360,7 @@ def test_generic_ntt_kernel_parity(gpu, cpu, q, inner):
     mc, mg = cpu.hal_create_modulus(q), gpu.hal_create_modulus(q)
     pc, pg = cpu.hal_create_ntt_plan(mc, inner), gpu.hal_create_ntt_plan(mg, inner)
     rng = np.random.default_rng(31)
-    pkg.lib().csd_ntt_force_generic(1)
+    gpu.ntt_force_generic(1)
     try:
         for dim in (2, 3):
             shape = (2, 1, n, 9) if dim == 2 else (2, 1, 3, n)
@@ -373,7 +373,7 @@ def test_generic_ntt_kernel_parity(gpu, cpu, q, inner):
             gpu.hal_intt_mont64_dim(dg, [pg], dim)
             assert same(from_dev(gpu, dg), dc)
     finally:
-        pkg.lib().csd_ntt_force_generic(0)
+        gpu.ntt_force_generic(0)
 
 
 @pytest.mark.parametrize("lines,threads", [(0, 0), (2, 512), (1, 128), (3, 192), (4, 512), (1, 64), (8, 256)])
@@ -427,7 +427,7 @@ def test_compiled_row_kernels_parity(gpu, cpu, field, log2_q, inner, mode):
     mc, mg = cpu.hal_create_modulus(q), gpu.hal_create_modulus(q)
     pc, pg = cpu.hal_create_ntt_plan(mc, inner), gpu.hal_create_ntt_plan(mg, inner)
     rng = np.random.default_rng(n + mode)
-    pkg.lib().csd_ntt_force_generic(mode)
+    gpu.ntt_force_generic(mode)
     try:
         data = rand_i64(rng, (2, 1, 3, n), -2 * q, 2 * q)
         kern = rand_i64(rng, (1, 1, 1, n), -q, q)
@@ -445,7 +445,7 @@ def test_compiled_row_kernels_parity(gpu, cpu, field, log2_q, inner, mode):
         gpu.hal_intt_mont64_dim(dg, [pg], 3)
         assert same(from_dev(gpu, dg), dc)
     finally:
-        pkg.lib().csd_ntt_force_generic(0)
+        gpu.ntt_force_generic(0)
 
 
 @pytest.mark.parametrize("shape,prime", [((4, 3), P36864[0]), ((2, 3, 8, 64), P36864[0]), ((3, 64, 64), None),

@@ -114,16 +114,24 @@ def test_field18_square_kat_on_device(gpu, rings):
 
 @pytest.fixture(params=[1, 0], ids=["fused", "hal_sequence"])
 def bconv_form(request):
-    """runs a test with the fused conversion kernel and with the reference's HAL sequence"""
-    pkg = load()
-    pkg.ring.set_bconv_form(request.param)
-    yield request.param
-    pkg.ring.set_bconv_form(1)
+    """runs a test with the fused conversion kernel and with the reference's HAL sequence: the
+    test calls the yielded function on each ring it uses, which is reset to the default after"""
+    used = []
+
+    def apply(ring):
+        ring.set_bconv_form(request.param)
+        used.append(ring)
+        return request.param
+
+    yield apply
+    for ring in used:
+        ring.set_bconv_form(1)
 
 
 def test_bconv_kat_on_device(gpu, bconv_form):
     """ring.rs:1859-1917, on the device."""
     ring = load().ring.RingContext(gpu, "field_2", raw_primes=BCONV_PRIMES)
+    bconv_form(ring)
     frm, to = BCONV_PRIMES[:7], BCONV_PRIMES[7:]
     d1 = [0, 1, 2, 3, 4, 5, 6, 7]
     d2 = [1 << 64, 3 << 100, -7 << 100, 9 << 120, -1 << 120, 5 << 120, 0, 7 << 120]
@@ -145,6 +153,7 @@ def test_bconv_parity(gpu, cpu, rings, bconv_form):
     from a limb slice of a larger tensor into a limb slice (strided views), and with unreduced
     (-2p, 2p) inputs. The fused form leaves its source unchanged."""
     g, c = rings("field_36864", (960, 980), 76)
+    form = bconv_form(g)
     base = g.initial_rns_base()
     ext = g.extension_rns_base_for(base)
     assert ext == c.extension_rns_base_for(base)
@@ -160,7 +169,7 @@ def test_bconv_parity(gpu, cpu, rings, bconv_form):
         sg = to_dev(gpu, src)
         g.bconv_tensors(dg, db, sg, sb)
         assert same(from_dev(gpu, dg), dc)
-        if bconv_form == 1:
+        if form == 1:
             assert same(from_dev(gpu, sg), src)
     # strided views: limbs 1.. of a [2, 1 + r, ., .] tensor into limbs 2.. of a [2, 2 + r', ., .] one
     sb, db = base, ext
@@ -177,6 +186,7 @@ def test_bconv_parity(gpu, cpu, rings, bconv_form):
 def test_bconv_parity_field49152(gpu, cpu, rings, bconv_form):
     """field_49152's bases (23 primes lifted by its extension, and back), bit-exact with the oracle"""
     g, c = rings("field_49152", (1280, 1300), 76)
+    bconv_form(g)
     base = g.initial_rns_base()
     ext = g.extension_rns_base_for(base)
     n2, n3 = g.info["deg_lprime"], g.info["deg_k"]
@@ -219,11 +229,11 @@ def test_gadget_bconv_parity(gpu, cpu, rings, field, log2_q, t_bits, layout, cou
         c.bconv_gadget_tensors(dc[b * d:(b + 1) * d], src[b:b + 1].copy(), base, digits)
     sg = to_dev(gpu, src)
     dg = gpu.hal_allocate_i64(dc.shape)
-    pkg.ring.set_gadget_form(form)
+    g.set_gadget_form(form)
     try:
         g.bconv_gadget_tensors(dg, sg, base, digits)
     finally:
-        pkg.ring.set_gadget_form(1)
+        g.set_gadget_form(1)
     assert same(from_dev(gpu, dg), dc)
     if form == 1:
         assert same(from_dev(gpu, sg), src)
@@ -240,7 +250,7 @@ def test_gadget_bconv_errors(gpu, rings):
         with pytest.raises(pkg.HalError):
             g.bconv_gadget_tensors(gpu.hal_allocate_i64(shape), src, base, [(0, 1), (1, r)])
     with pytest.raises(pkg.HalError):
-        pkg.ring.set_gadget_form(2)
+        g.set_gadget_form(2)
 
 
 @pytest.mark.parametrize("r,new_len", [(17, 16), (17, 3), (9, 8), (3, 1)])
@@ -288,7 +298,7 @@ def test_transform_overlap_halves(gpu, cpu, rings):
     mods = [gpu.hal_create_modulus(p) for p in primes]
     outs = []
     for on in (True, False):
-        pkg.ring.set_overlap(on)
+        g.set_overlap(on)
         try:
             dg = to_dev(gpu, data)
             g.pb_to_crt_tensor(dg, base)
@@ -296,7 +306,7 @@ def test_transform_overlap_halves(gpu, cpu, rings):
             g.crt_to_pb_tensor(dg, base)
             outs.append(from_dev(gpu, dg))
         finally:
-            pkg.ring.set_overlap(False)
+            g.set_overlap(False)
     assert same(outs[0], outs[1])
     dc = data[[0, 13]].copy()
     c.pb_to_crt_tensor(dc, base)
@@ -324,3 +334,66 @@ def test_transform_round_trip_full_batch(gpu, cpu, rings):
     g.crt_to_pb_tensor(dg, base)
     back = from_dev(gpu, dg)
     assert (np.mod(back, primes) == data).all()
+
+
+@pytest.mark.parametrize("form", [1, 0])
+def test_bconv_preserve_src(gpu, cpu, rings, form):
+    """csd_ring_bconv with preserve_src leaves the source unchanged in both conversion forms, with
+    the same residues as the destroying call"""
+    g, c = rings("field_36864", (960, 980), 76)
+    base = g.initial_rns_base()
+    ext = g.extension_rns_base_for(base)
+    rng = np.random.default_rng(77)
+    src = uniform_residues(rng, [g.primes[i] for i in base], (2, len(base), 12, 3072))
+    dc = np.zeros((2, len(ext), 12, 3072), dtype=np.int64)
+    c.bconv_tensors(dc, ext, src.copy(), base)
+    g.set_bconv_form(form)
+    try:
+        sg = to_dev(gpu, src)
+        dg = gpu.hal_allocate_i64(dc.shape)
+        g.bconv_tensors(dg, ext, sg, base, preserve_src=True)
+        assert same(from_dev(gpu, dg), dc)
+        assert same(from_dev(gpu, sg), src)
+    finally:
+        g.set_bconv_form(1)
+
+
+def test_forms_are_per_ring_and_context(gpu, cpu):
+    """Two contexts (and their rings) in one process hold different execution forms, used
+    interleaved: ring A with the HAL-sequence conversions, ring B fused; context A with the
+    one-stage-per-pass NTT kernel, context B with the compiled shapes. Every call keeps its own
+    context's form (the HAL sequence destroys the source, the fused form does not) and all give the
+    oracle's residues."""
+    pkg = load()
+    be_a, be_b = pkg.HipBackend(0), pkg.HipBackend(0)
+    ra = pkg.ring.RingContext(be_a, "field_192", (200, 240), 8)
+    rb = pkg.ring.RingContext(be_b, "field_192", (200, 240), 8)
+    oc = R.RingContext(cpu, R.FieldDescriptor("field_192"), (200, 240), 8)
+    try:
+        ra.set_bconv_form(0)
+        be_a.ntt_force_generic(1)
+        assert ra.bconv_form() == 0 and rb.bconv_form() == 1
+        base = ra.initial_rns_base()
+        ext = ra.extension_rns_base_for(base)
+        n2, n3 = ra.info["deg_lprime"], ra.info["deg_k"]
+        rng = np.random.default_rng(5)
+        for it in range(3):
+            src = uniform_residues(rng, [ra.primes[i] for i in base], (2, len(base), n2, n3))
+            want = np.zeros((2, len(ext), n2, n3), dtype=np.int64)
+            oc.bconv_tensors(want, ext, src.copy(), base)
+            wt = src.copy()
+            oc.pb_to_crt_tensor(wt, base)
+            for be, ring, destroys in [(be_a, ra, True), (be_b, rb, False)]:
+                sg = to_dev(be, src)
+                dg = be.hal_allocate_i64(want.shape)
+                ring.bconv_tensors(dg, ext, sg, base)
+                assert same(from_dev(be, dg), want)
+                assert same(from_dev(be, sg), src) != destroys
+                tg = to_dev(be, src)
+                ring.pb_to_crt_tensor(tg, base)
+                assert same(from_dev(be, tg), wt)
+    finally:
+        ra.close()
+        rb.close()
+        be_a.close()
+        be_b.close()
