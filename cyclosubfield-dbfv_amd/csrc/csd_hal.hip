@@ -149,6 +149,7 @@ extern "C" int csd_ctx_create(int device, csd_ctx** out) {
     HIPCHECK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) throw CsdError("no such HIP device");
     HIPCHECK(hipSetDevice(device));
+    csd_zp_scratch_setup(device);  // the no-allocation entry point's scratch on this device
     csd_ctx* c = new csd_ctx();
     c->device = device;
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

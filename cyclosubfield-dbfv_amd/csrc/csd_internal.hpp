@@ -151,6 +151,9 @@ ModTable make_mod_table(const csd_modulus* mods, size_t nmods, int64_t d1);
 // device): the attribute is per device, and this keeps first calls from several threads safe.
 void csd_allow_lds(const void* fn, int bytes);
 void csd_zero_fill(csd_ctx* ctx, void* p, size_t bytes);
+// Resolves the address of this device's copy of the dense maps' scratch array (csd_zp.hip); call
+// once per device with that device current, before csd_zp_apply_dim_noalloc runs on it.
+void csd_zp_scratch_setup(int device);
 
 // host-side modular helpers (plan and constant construction)
 int64_t csd_host_mulmod(int64_t a, int64_t b, int64_t q);

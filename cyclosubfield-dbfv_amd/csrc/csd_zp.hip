@@ -16,6 +16,10 @@
 // The plain form computes the same sums with 64 x 64 -> 128-bit products on the VALU.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <string>
+
 #include <atomic>
 
 #include <algorithm>
@@ -728,6 +732,91 @@ extern "C" int csd_zp_matrix_free(csd_zp_matrix* z) {
     CSD_CATCH
 }
 
+// ---------------------------------------------------------------------------------------------
+// The scratch of csd_zp_apply_dim_noalloc: the source vectors' digit planes live in this file-scope
+// device array instead of a pooled allocation. Every GPU the library runs on holds its own copy
+// (CSD_ZP_SCRATCH_BYTES) until the process ends. Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in): csd_zp_scratch_setup resolves the device address of
+// each GPU's copy once, with that GPU current, and the entry point uses that address.
+static __device__ __align__(256) int8_t scratch[CSD_ZP_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, void*> g_scratch_ptr;  // device -> address of that device's copy of `scratch`
+
+void csd_zp_scratch_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(device)) return;
+    int cur = -1;
+    HIPCHECK(hipGetDevice(&cur));
+    if (cur != device) throw CsdError("zp scratch setup: the device must be current");
+    void* ptr = nullptr;
+    HIPCHECK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw CsdError("zp scratch setup: no device address for the scratch array");
+    g_scratch_ptr[device] = ptr;
+}
+
+static void* zp_scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it == g_scratch_ptr.end()) throw CsdError("zp scratch: not set up for this device");
+    return it->second;
+}
+
+// The K-chunk of the int8 contraction: at most ZP_MAX_CHUNK terms, or max_chunk rounded down to
+// a multiple of 64 when given.
+static int64_t zp_chunk(size_t max_chunk) {
+    int64_t chunk = ZP_MAX_CHUNK;
+    if (max_chunk) chunk = std::min<int64_t>(chunk, (int64_t)max_chunk / 64 * 64);
+    if (chunk < 64) throw CsdError("zp apply: chunk must hold at least 64 terms");
+    return chunk;
+}
+
+// The int8-MFMA form of a dense map: splits the source vectors into their digit planes in xd
+// ([r][8][npad][kpad] bytes, every byte written here, padding included) and runs the context's
+// MFMA kernel over them.
+static void zp_mfma_run(csd_ctx* ctx, const csd_zp_matrix* z, const csd_view* src, const VecLayout& Lx, const csd_view* dst,
+                        const VecLayout& Ly, int64_t nvec, int64_t npad, int64_t chunk, int8_t* xd, const ModTable& mt) {
+    launch_digits(ctx, (const int64_t*)src->ptr, Lx, z->r, npad, z->kpad, xd, mt);
+    MfmaArgs a;
+    a.a = z->planes;
+    a.x = xd;
+    a.y = (int64_t*)dst->ptr;
+    a.Ly = Ly;
+    a.m = z->m;
+    a.mpad = z->mpad;
+    a.npad = npad;
+    a.kpad = z->kpad;
+    a.chunk = chunk;
+    a.mtiles = z->mpad / 32;
+    a.ntiles = npad / 64;
+    a.nblocks = z->r * a.mtiles * a.ntiles;
+    if (a.nblocks > (int64_t)INT32_MAX) throw CsdError("zp apply: too many tiles");
+    TimedScope ts(ctx, "zp_mfma",
+                  (double)z->r * ZP_DIGITS * (z->mpad * z->kpad + npad * z->kpad) + 8.0 * z->r * nvec * z->m);
+    const int zk = ctx->zp_kernel;
+    if (zk == 3) {
+        a.mtiles = z->mpad / 64;  // mpad is a multiple of 64
+        a.nblocks = z->r * a.mtiles * a.ntiles;
+        const size_t lds = 2 * (size_t)(ZP8_A_BYTES + ZP8_X_BYTES);
+        csd_allow_lds((const void*)k_zp_mfma_lds64, (int)lds);
+        hipLaunchKernelGGL(k_zp_mfma_lds64, dim3((unsigned)a.nblocks), dim3(ZP8_THREADS), lds, ctx->stream, a, mt);
+    } else if (zk == 2) {
+        // 64-row tiles over the same planes (mpad is a multiple of 64)
+        a.mtiles = z->mpad / 64;
+        a.nblocks = z->r * a.mtiles * a.ntiles;
+        const size_t lds = 2 * (size_t)(ZPW_A_BYTES + ZPW_X_BYTES);
+        csd_allow_lds((const void*)k_zp_mfma_wide, (int)lds);
+        hipLaunchKernelGGL(k_zp_mfma_wide, dim3((unsigned)a.nblocks), dim3(ZPW_THREADS), lds, ctx->stream, a, mt);
+    } else if (zk == 1) {
+        const size_t lds = 2 * (size_t)(ZPL_A_BYTES + ZPL_X_BYTES);
+        csd_allow_lds((const void*)k_zp_mfma_lds, (int)lds);
+        hipLaunchKernelGGL(k_zp_mfma_lds, dim3((unsigned)a.nblocks), dim3(ZPL_THREADS), lds, ctx->stream, a, mt);
+    } else {
+        hipLaunchKernelGGL(k_zp_mfma, dim3((unsigned)a.nblocks), dim3(256), 0, ctx->stream, a, mt);
+    }
+    HIPCHECK(hipGetLastError());
+}
+
 extern "C" int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* z, const csd_view* dst, const csd_view* src, int dim, int form,
                                 size_t max_chunk) {
     CSD_TRY check_view(dst, "zp apply dst");
@@ -761,55 +850,47 @@ extern "C" int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* z, const csd_
         HIPCHECK(hipGetLastError());
         return 0;
     }
-    int64_t chunk = ZP_MAX_CHUNK;
-    if (max_chunk) chunk = std::min<int64_t>(chunk, (int64_t)max_chunk / 64 * 64);
-    if (chunk < 64) throw CsdError("zp apply: chunk must hold at least 64 terms");
+    const int64_t chunk = zp_chunk(max_chunk);
     const int64_t npad = round_up(nvec, 64);
     int8_t* xd = (int8_t*)ctx->alloc_bytes((size_t)z->r * ZP_DIGITS * npad * z->kpad);
     try {
-        launch_digits(ctx, (const int64_t*)src->ptr, Lx, z->r, npad, z->kpad, xd, mt);
-        MfmaArgs a;
-        a.a = z->planes;
-        a.x = xd;
-        a.y = (int64_t*)dst->ptr;
-        a.Ly = Ly;
-        a.m = z->m;
-        a.mpad = z->mpad;
-        a.npad = npad;
-        a.kpad = z->kpad;
-        a.chunk = chunk;
-        a.mtiles = z->mpad / 32;
-        a.ntiles = npad / 64;
-        a.nblocks = z->r * a.mtiles * a.ntiles;
-        if (a.nblocks > (int64_t)INT32_MAX) throw CsdError("zp apply: too many tiles");
-        TimedScope ts(ctx, "zp_mfma",
-                      (double)z->r * ZP_DIGITS * (z->mpad * z->kpad + npad * z->kpad) + 8.0 * z->r * nvec * z->m);
-        const int zk = ctx->zp_kernel;
-        if (zk == 3) {
-            a.mtiles = z->mpad / 64;  // mpad is a multiple of 64
-            a.nblocks = z->r * a.mtiles * a.ntiles;
-            const size_t lds = 2 * (size_t)(ZP8_A_BYTES + ZP8_X_BYTES);
-            csd_allow_lds((const void*)k_zp_mfma_lds64, (int)lds);
-            hipLaunchKernelGGL(k_zp_mfma_lds64, dim3((unsigned)a.nblocks), dim3(ZP8_THREADS), lds, ctx->stream, a, mt);
-        } else if (zk == 2) {
-            // 64-row tiles over the same planes (mpad is a multiple of 64)
-            a.mtiles = z->mpad / 64;
-            a.nblocks = z->r * a.mtiles * a.ntiles;
-            const size_t lds = 2 * (size_t)(ZPW_A_BYTES + ZPW_X_BYTES);
-            csd_allow_lds((const void*)k_zp_mfma_wide, (int)lds);
-            hipLaunchKernelGGL(k_zp_mfma_wide, dim3((unsigned)a.nblocks), dim3(ZPW_THREADS), lds, ctx->stream, a, mt);
-        } else if (zk == 1) {
-            const size_t lds = 2 * (size_t)(ZPL_A_BYTES + ZPL_X_BYTES);
-            csd_allow_lds((const void*)k_zp_mfma_lds, (int)lds);
-            hipLaunchKernelGGL(k_zp_mfma_lds, dim3((unsigned)a.nblocks), dim3(ZPL_THREADS), lds, ctx->stream, a, mt);
-        } else {
-            hipLaunchKernelGGL(k_zp_mfma, dim3((unsigned)a.nblocks), dim3(256), 0, ctx->stream, a, mt);
-        }
-        HIPCHECK(hipGetLastError());
+        zp_mfma_run(ctx, z, src, Lx, dst, Ly, nvec, npad, chunk, xd, mt);
     } catch (...) {
         ctx->free_bytes(xd);
         throw;
     }
     ctx->free_bytes(xd);
+    CSD_CATCH
+}
+
+// csd_zp_apply_dim's int8-MFMA form (form 0) without any allocation: the digit planes of the
+// source vectors go to the device's scratch array. Calls on one GPU share that array, so calls
+// that overlap on one GPU (other streams or contexts of the same device) race; calls on one
+// context's stream are ordered and safe. Nothing needs clearing: the digit kernel writes every
+// byte the contraction reads, padding included, before the contraction runs.
+extern "C" int csd_zp_apply_dim_noalloc(csd_ctx* ctx, const csd_zp_matrix* z, const csd_view* dst, const csd_view* src,
+                                        int dim, size_t max_chunk) {
+    CSD_TRY if (!ctx || !z) throw CsdError("zp apply: null context or matrix");
+    check_view(dst, "zp apply dst");
+    check_view(src, "zp apply src");
+    const VecLayout Lx = vec_layout(src, dim), Ly = vec_layout(dst, dim);
+    if (src->shape[1] != z->r || dst->shape[1] != z->r) throw CsdError("zp apply: dimension 1 must equal the number of moduli");
+    if (Lx.nelem != z->k) throw CsdError("zp apply: source vectors must have the matrix's column count");
+    if (Ly.nelem != z->m) throw CsdError("zp apply: destination vectors must have the matrix's row count");
+    if (Lx.nvec != Ly.nvec || src->shape[0] != dst->shape[0] || Lx.nlo != Ly.nlo)
+        throw CsdError("zp apply: source and destination must hold the same vectors");
+    if (dst->ptr == src->ptr) throw CsdError("zp apply: destination must not alias the source");
+    const int64_t nvec = Lx.nvec;
+    if (nvec == 0) return 0;
+    const int64_t chunk = zp_chunk(max_chunk);
+    const int64_t npad = round_up(nvec, 64);
+    const double need = (double)z->r * ZP_DIGITS * (double)npad * (double)z->kpad;
+    if (need > (double)CSD_ZP_SCRATCH_BYTES)
+        throw CsdError("zp apply (no allocation): the source digit planes need " + std::to_string((long long)need) +
+                       " bytes (moduli x 8 x vectors rounded up to 64 x columns rounded up to 64), more than the " +
+                       std::to_string((long long)CSD_ZP_SCRATCH_BYTES) + "-byte scratch");
+    ModTable mt = make_mod_table(z->mods.data(), z->mods.size(), z->r);
+    int8_t* xd = (int8_t*)zp_scratch_of(ctx->device);
+    zp_mfma_run(ctx, z, src, Lx, dst, Ly, nvec, npad, chunk, xd, mt);
     CSD_CATCH
 }

@@ -510,6 +510,14 @@ class HipBackend:
                                      ctypes.byref(src.view()), ctypes.c_int(_DIM[dim]),
                                      ctypes.c_int({"mfma": 0, "plain": 1}[form]), ctypes.c_size_t(max_chunk)))
 
+    def zp_apply_dim_noalloc(self, mat, dst, src, dim, max_chunk=0):
+        """zp_apply_dim's "mfma" form without any allocation (csd_zp_apply_dim_noalloc): the source
+        digit planes go to the device's fixed scratch array; inputs needing more than it holds are
+        rejected"""
+        check(lib().csd_zp_apply_dim_noalloc(self.ctx, ctypes.c_void_p(mat.handle), ctypes.byref(dst.view()),
+                                             ctypes.byref(src.view()), ctypes.c_int(_DIM[dim]),
+                                             ctypes.c_size_t(max_chunk)))
+
     def zp_set_kernel(self, form):
         """which int8 MFMA kernel this context's "mfma" form runs (csd_zp_set_kernel): 3 (the
         default) LDS-staged 64 x 64 tiles for 8 waves with 2 blocks each, 1 LDS-staged 32 x 64 tiles

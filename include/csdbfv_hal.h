@@ -179,6 +179,17 @@ int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* a, const csd_view* dst, 
  * 64 x 64 tile of 4 waves with 2 x 2 blocks each; 0 has 4 waves load their operands from L2
  * directly, two blocks each. All compute identical results. */
 int csd_zp_set_kernel(csd_ctx* ctx, int form);
+/* csd_zp_apply_dim's int8-MFMA form (form 0), allocating nothing. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The source vectors' digit planes go to a
+ * file-scope device array of CSD_ZP_SCRATCH_BYTES per GPU (held until the process ends) instead
+ * of a library allocation; calls needing more (moduli x 8 x vectors rounded up to 64 x columns
+ * rounded up to 64 bytes) are rejected with an error. All calls on one GPU share that array:
+ * calls that overlap on one GPU (other streams or contexts of the device) race. Results are
+ * those of csd_zp_apply_dim(form 0). */
+#define CSD_ZP_SCRATCH_BYTES (64u << 20)
+int csd_zp_apply_dim_noalloc(csd_ctx* ctx, const csd_zp_matrix* a, const csd_view* dst, const csd_view* src, int dim,
+                             size_t max_chunk);
 
 #ifdef __cplusplus
 }
