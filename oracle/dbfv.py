@@ -241,3 +241,209 @@ def dbfv_hom_square(ring, modulus, ct, ct_bound, base, keys, digits, output_basi
     assert bound <= 2
     lifted_ct, lifted = lift_to_extended(ring, red, base)
     return dbfv_mul_convolve(ring, modulus, lifted_ct, lifted_ct, lifted, len(base), keys, digits, output_basis)
+
+
+# ---------------------------------------------------------------------------------------------
+# dBFV accumulators and exact division (dbfv.rs:82-198, 300-408, 464-533)
+
+def cmod(a, modulus):
+    """cmod (ring.rs:93-104): Rust's truncating remainder moved into (-modulus/2, modulus/2]"""
+    from .ring import _trunc_mod
+
+    half = modulus // 2
+    r = _trunc_mod(a, modulus)
+    if abs(r) >= half:
+        if r < 0:
+            r += modulus
+        elif r > half:
+            r -= modulus
+    return r
+
+
+def balanced_digits(x, b, max_digits):
+    """balanced_digits (dbfv.rs:82-94): at most max_digits balanced base-b digits, stopping at 0;
+    the part of x beyond max_digits digits is dropped"""
+    out = []
+    for _ in range(max_digits):
+        if x == 0:
+            break
+        rem = cmod(x, b)
+        out.append(rem)
+        x = (x - rem) // b
+    return out
+
+
+class Ct:
+    """A dBFV ciphertext (dbfv.rs:35): rows 2i, 2i + 1 are digit i; with its RNS base, basis
+    ("pb" or "crt") and representative bound, as the reference's RingEls carries them."""
+
+    def __init__(self, data, base, basis="pb", bound=1):
+        self.data, self.base, self.basis, self.bound = data, list(base), basis, bound
+
+    def digits(self):
+        assert self.data.shape[0] % 2 == 0
+        return self.data.shape[0] // 2
+
+
+class DBFVAccumulator:
+    """DBFVAccumulator (dbfv.rs:121-147): a wide batch of 2 digits() rows, the modulus, the basis
+    and the number of digits used so far"""
+
+    def __init__(self, ring, modulus, base, basis):
+        self.acc = ring.allocate_wide(2 * modulus.digits(), base)
+        self.modulus, self.basis, self.used_digits = modulus, basis, 0
+
+    def capacity(self):
+        return self.modulus.digits()
+
+    def rns_base(self):
+        return self.acc.rns_base
+
+
+def dbfv_allocate_accumulator(ring, modulus, base, basis):
+    """dbfv.rs:150-157"""
+    return DBFVAccumulator(ring, modulus, base, basis)
+
+
+def dbfv_acc_add(ring, acc, ct):
+    """dbfv.rs:160-171"""
+    d = ct.digits()
+    if d == 0:
+        return
+    assert d <= acc.capacity() and acc.basis == ct.basis and acc.rns_base() == ct.base
+    acc.acc.add(range(0, 2 * d), ct.data, ct.bound)
+    acc.used_digits = max(acc.used_digits, d)
+
+
+def dbfv_acc_sub(ring, acc, ct):
+    """dbfv.rs:185-196"""
+    d = ct.digits()
+    if d == 0:
+        return
+    assert d <= acc.capacity() and acc.basis == ct.basis and acc.rns_base() == ct.base
+    acc.acc.sub(range(0, 2 * d), ct.data, ct.bound)
+    acc.used_digits = max(acc.used_digits, d)
+
+
+def scalar_tensor_big(ring, base, scalar):
+    """scalar_tensor_big (ring.rs:890-903): mont(smallest_lift(scalar mod p)) per prime"""
+    from .ring import mont_repr, smallest_lift
+
+    return ring._import([mont_repr(smallest_lift(scalar, ring.primes[i]), ring.primes[i]) for i in base],
+                        [1, len(base), 1, 1])
+
+
+def dbfv_acc_add_constant(ring, acc, scalar):
+    """dbfv.rs:174-182: adds scalar Delta (times the period-basis sign) to row 0; the digit count
+    is left as it is"""
+    delta = rounded_div(ring.modulus_of(acc.rns_base()), acc.modulus.t())
+    if acc.basis == "pb":
+        scalar *= ring.field.scale_q_to_m_period_basis()
+    tensor = scalar_tensor_big(ring, acc.rns_base(), scalar * delta)
+    acc.acc.add(range(0, 1), tensor, 1)
+
+
+def dbfv_acc_inner_product_tensors(ring, acc, terms):
+    """dbfv.rs:300-348: for each output digit k one inner product over the convolution diagonal
+    i + j = k of all terms (lhs Ct, list of factor tensors); the rows written must be fresh"""
+    capacity = acc.capacity()
+    max_new = 0
+    for lhs, rhs in terms:
+        ld = lhs.digits()
+        if ld == 0 or not rhs:
+            continue
+        assert ld <= capacity and acc.basis == lhs.basis and acc.rns_base() == lhs.base
+        max_new = max(max_new, min(capacity, ld + len(rhs) - 1))
+    if max_new == 0:
+        return
+    for k in range(max_new):
+        k_terms = []
+        for lhs, rhs in terms:
+            ld = lhs.digits()
+            if ld == 0 or not rhs:
+                continue
+            if k >= min(capacity, ld + len(rhs) - 1):
+                continue
+            for j in range(len(rhs)):
+                if k < j:
+                    break
+                i = k - j
+                if i >= ld:
+                    continue
+                k_terms.append((lhs.data[2 * i:2 * i + 2], lhs.bound, rhs[j], 1))
+        if k_terms:
+            acc.acc.inner_product(range(2 * k, 2 * k + 2), k_terms)
+    acc.used_digits = max(acc.used_digits, max_new)
+
+
+def dbfv_acc_inner_product_scalar(ring, acc, terms):
+    """dbfv.rs:356-373: terms are (lhs Ct, scalar); each scalar in balanced base-b digits"""
+    m = acc.modulus
+    prepared = [(lhs, [ring.scalar_tensor(acc.rns_base(), dg) for dg in balanced_digits(s, m.b(), m.digits())])
+                for lhs, s in terms]
+    dbfv_acc_inner_product_tensors(ring, acc, prepared)
+
+
+def dbfv_acc_fma_scalar(ring, acc, lhs, rhs):
+    """dbfv_acc_fma_scalar (dbfv.rs:236-248) over dbfv_acc_fma_tensors (dbfv.rs:202-232)"""
+    m = acc.modulus
+    rhs_t = [ring.scalar_tensor(acc.rns_base(), dg) for dg in balanced_digits(rhs, m.b(), m.digits())]
+    ld = lhs.digits()
+    if ld == 0 or not rhs_t:
+        return
+    assert ld <= acc.capacity() and acc.basis == lhs.basis and acc.rns_base() == lhs.base
+    new_digits = min(acc.capacity(), ld + len(rhs_t) - 1)
+    for i in range(ld):
+        for j in range(len(rhs_t)):
+            k = i + j
+            if k < new_digits:
+                acc.acc.fma(range(2 * k, 2 * k + 2), lhs.data[2 * i:2 * i + 2], lhs.bound, rhs_t[j], 1)
+    acc.used_digits = max(acc.used_digits, new_digits)
+
+
+def dbfv_finish_accumulator(ring, acc):
+    """dbfv.rs:402-406: the used rows reduced (reduce_wide_rows), bound 1"""
+    rows = 2 * acc.used_digits
+    out = ring.allocate(rows, acc.rns_base())
+    if rows:
+        ring.reduce_wide_tensor(out, acc.acc, range(rows))
+    return Ct(out, acc.rns_base(), acc.basis, 1)
+
+
+def dbfv_exact_div_pow2(ring, initial_modulus, val, log2_div):
+    """dbfv_exact_div_pow2 (dbfv.rs:500-533): digit 0 becomes sum_(i <= min(k, d - 1)) b^i y_i,
+    digit j >= 1 becomes b^k y_(j + k), k = ceil(log2_div / log2_b)"""
+    be = ring.backend
+    d = val.digits()
+    assert d <= initial_modulus.digits()
+    assert log2_div < initial_modulus.log2_t, "the result modulus t' = t / 2^log2_div must be at least 2"
+    if log2_div == 0 or d <= 1:
+        return val
+    log2_b = initial_modulus.log2_b
+    k = (log2_div - 1) // log2_b + 1
+    base = val.base
+    assert_mul_mont_safe(ring, val.bound, base)
+    out_digits = max(d - k, 1) if d > k else 1
+    result = ring.allocate(2 * out_digits, base)
+    if out_digits > 1:
+        b_pow_k = scalar_tensor_big(ring, base, 1 << (k * log2_b))
+        be.hal_assign(result[2:], val.data[2 * (k + 1):])
+        be.hal_mul_mont64(result[2:], b_pow_k, ring.mods_for(base))
+    factors = [scalar_tensor_big(ring, base, 1 << (i * log2_b)) for i in range(min(k, d - 1) + 1)]
+    terms = [(val.data[2 * i:2 * i + 2], val.bound, f, 1) for i, f in enumerate(factors)]
+    acc = ring.allocate_wide(2, base)
+    acc.inner_product(range(0, 2), terms)
+    ring.reduce_wide_tensor(result[0:2], acc, range(2))
+    return Ct(result, base, val.basis, 1)
+
+
+def mod_switch_ct(ring, ct, new_len):
+    """mod_switch_down (bfv.rs:628-662) of a period-basis dBFV ciphertext: bound 1 over the
+    length-new_len prefix of its base (a copy when nothing is dropped, with its bound)"""
+    from .ring import mod_switch_down
+
+    assert ct.basis == "pb" and new_len <= len(ct.base)
+    if new_len == len(ct.base):
+        return Ct(ct.data.copy(), ct.base, "pb", ct.bound)
+    assert_mul_mont_safe(ring, ct.bound + 1, ct.base[:new_len])
+    return Ct(mod_switch_down(ring, ct.data, ct.base, new_len), ct.base[:new_len], "pb", 1)

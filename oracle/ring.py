@@ -204,6 +204,15 @@ FIELD_DATA = {
 }
 
 
+# (log2_pb_normalized_pb_op_norm_estimate, log2_sk_pb_op_norm_estimate) of each descriptor
+# (field.rs:537-767)
+NOISE_ESTIMATES = {
+    "field_36864": (8., 6.), "field_32768": (6., 5.), "field_32768_ex": (6., 5.), "field_24576": (5., 4.),
+    "field_65536": (6., 5.), "field_49152": (10., 8.), "field_49152_ex": (8., 6.), "field_192": (0., 0.),
+    "field_32": (0., 0.), "field_18": (0., 0.), "field_16": (0., 0.), "field_2": (0., 0.),
+}
+
+
 class FieldDescriptor:
     """field.rs:128-300 (the parts the hot path uses)."""
 
@@ -237,6 +246,19 @@ class FieldDescriptor:
         for i, g in enumerate(self.lq_transversal()):
             assert g not in self.transversal_index
             self.transversal_index[g] = i
+
+    def scale_q_to_m_period_basis(self):
+        """field.rs:447: (-1)^(number of prime factors of m)"""
+        return -1 if len(self.m_factors) % 2 == 1 else 1
+
+    def noise_growth_hom_mul_lower_estimate(self, t):
+        """field.rs:464-467 (for the secret key weight 128 the reference fixes)"""
+        pb_norm, sk_norm = NOISE_ESTIMATES[self.name]
+        return math.log2(float(t)) + sk_norm + pb_norm
+
+    def noise_growth_hom_mul_scalar_lower_estimate(self, t):
+        """field.rs:483"""
+        return math.log2(float(t))
 
     @property
     def m_k(self):
