@@ -7,3 +7,4 @@ from .hal import I128, DeviceTensor, HalError, HipBackend, lib  # noqa: F401
 from . import ring  # noqa: E402,F401
 from . import bfv  # noqa: E402,F401
 from . import dbfv  # noqa: E402,F401
+from . import digitex  # noqa: E402,F401

@@ -368,3 +368,239 @@ def dbfv_hom_square(ring, modulus, ct, ct_bound, base, keys, digits, output_basi
         return red, bound
     lifted_ct, lifted = lift_to_extended(ring, red, base)
     return _mul_convolve(ring, modulus, lifted_ct, lifted_ct, 1, lifted, len(base), keys, digits, output_basis)
+
+
+def dbfv_hom_square_batch(ring, modulus, ct, ct_bound, count, base, keys, digits, output_basis="pb", fused=True):
+    """dbfv_hom_square (dbfv.rs:1058-1076) of `count` period-basis dBFV ciphertexts, digit-major
+    (row (2 i + s) count + c is row s of digit i of ciphertext c): one reduction and one lift for
+    all of them, then the digit convolution of each with itself and one batched
+    relinearization. Each ciphertext gets the residues of its own dbfv_hom_square."""
+    from .bfv import lift_to_extended
+
+    if output_basis not in ("pb", "crt"):
+        raise ValueError("output_basis must be 'crt' or 'pb'")
+    red, bound = _reduced_operand(ring, modulus, ct, ct_bound, count, base, fused)
+    if red.shape[0] == 0:
+        return red, bound
+    lifted_ct, lifted = lift_to_extended(ring, red, base)
+    return _mul_convolve(ring, modulus, lifted_ct, lifted_ct, count, lifted, len(base), keys, digits, output_basis)
+
+
+# ---------------------------------------------------------------------------------------------
+# dBFV ciphertext batches, accumulators and exact division (dbfv.rs:82-198, 300-408, 464-533)
+
+def cmod(a, modulus):
+    """cmod (ring.rs:93-104): Rust's truncating remainder moved into (-modulus/2, modulus/2]"""
+    r = abs(a) % modulus
+    r = -r if a < 0 else r
+    half = modulus // 2
+    if abs(r) >= half:
+        if r < 0:
+            r += modulus
+        elif r > half:
+            r -= modulus
+    return r
+
+
+def balanced_digits(x, b, max_digits):
+    """balanced_digits (dbfv.rs:82-94): at most max_digits balanced base-b digits of x, stopping
+    at zero (what lies beyond max_digits digits is dropped)"""
+    out = []
+    for _ in range(max_digits):
+        if x == 0:
+            break
+        rem = cmod(x, b)
+        out.append(rem)
+        x = (x - rem) // b
+    return out
+
+
+def scale_q_to_m_period_basis(ring):
+    """field.rs:447: (-1)^(number of distinct prime factors of m)"""
+    m, p, k = ring.info["m"], 2, 0
+    while p * p <= m:
+        if m % p == 0:
+            k += 1
+            while m % p == 0:
+                m //= p
+        p += 1
+    if m > 1:
+        k += 1
+    return -1 if k % 2 else 1
+
+
+class DBFVBatch:
+    """`count` dBFV ciphertexts (dbfv.rs:35) over one RNS base, digit-major: row (2 i + s) count + c
+    is component s of digit i of ciphertext c, so digit i of every ciphertext is the contiguous
+    run of rows [2 i count, (2 i + 2) count). With the basis ("pb" or "crt") and the
+    representative bound a RingEls carries."""
+
+    def __init__(self, data, base, count, basis="pb", bound=1):
+        if count < 1 or data.shape[0] % (2 * count):
+            raise ValueError("a dBFV batch has 2 x digits x count rows")
+        self.data, self.base, self.count, self.basis, self.bound = data, list(base), count, basis, bound
+
+    def digits(self):
+        return self.data.shape[0] // (2 * self.count)
+
+    def digit_rows(self, i, j=None):
+        """rows of digits i .. j - 1 (j = i + 1 by default) of every ciphertext"""
+        j = i + 1 if j is None else j
+        return self.data[2 * i * self.count:2 * j * self.count]
+
+
+class DBFVAccumulator:
+    """DBFVAccumulator (dbfv.rs:121-147) for a batch of `count` ciphertexts: a wide batch with
+    rows for every digit of the modulus, digit-major like DBFVBatch, the basis and the number of
+    digits used so far. Its i128 rows start unwritten: finishing reduces only the used rows,
+    which every sequence writes first (the fresh rows of the reference are overwritten too)."""
+
+    def __init__(self, ring, modulus, base, basis, count):
+        self.ring, self.modulus, self.basis, self.count = ring, modulus, basis, count
+        self.acc = ring.allocate_wide(2 * modulus.digits() * count, base, zero=False)
+        self.used_digits = 0
+
+    def capacity(self):
+        return self.modulus.digits()
+
+    def rns_base(self):
+        return self.acc.rns_base
+
+    def rows(self, i, j):
+        """the wide rows of digits i .. j - 1"""
+        return range(2 * i * self.count, 2 * j * self.count)
+
+
+def dbfv_allocate_accumulator(ring, modulus, base, basis, count=1):
+    """dbfv_allocate_accumulator (dbfv.rs:150-157)"""
+    return DBFVAccumulator(ring, modulus, base, basis, count)
+
+
+def _check_operand(acc, ct):
+    if ct.count != acc.count or ct.basis != acc.basis or ct.base != acc.rns_base():
+        raise ValueError("accumulator and ciphertexts differ in batch size, basis or RNS base")
+    if ct.digits() > acc.capacity():
+        raise ValueError("ciphertext has more digits than the accumulator")
+
+
+def dbfv_acc_add(ring, acc, ct):
+    """dbfv_acc_add (dbfv.rs:160-171), digit-wise over the batch"""
+    d = ct.digits()
+    if d == 0:
+        return
+    _check_operand(acc, ct)
+    acc.acc.add(acc.rows(0, d), ct.data, ct.bound)
+    acc.used_digits = max(acc.used_digits, d)
+
+
+def dbfv_acc_sub(ring, acc, ct):
+    """dbfv_acc_sub (dbfv.rs:185-196)"""
+    d = ct.digits()
+    if d == 0:
+        return
+    _check_operand(acc, ct)
+    acc.acc.sub(acc.rows(0, d), ct.data, ct.bound)
+    acc.used_digits = max(acc.used_digits, d)
+
+
+def dbfv_acc_add_constant(ring, acc, scalar):
+    """dbfv_acc_add_constant (dbfv.rs:174-182): scalar Delta (times the period-basis sign) into
+    component 0 of digit 0 of every ciphertext; the digit count is left as it is"""
+    delta = rounded_div(ring.modulus_of(acc.rns_base()), acc.modulus.t())
+    if acc.basis == "pb":
+        scalar *= scale_q_to_m_period_basis(ring)
+    acc.acc.add(range(0, acc.count), ring.scalar_tensor_big(acc.rns_base(), scalar * delta), 1)
+
+
+def dbfv_acc_inner_product_tensors(ring, acc, terms):
+    """dbfv_acc_inner_product_tensors (dbfv.rs:300-348): for every output digit k one inner
+    product over the convolution diagonal i + j = k of all terms (a DBFVBatch, its factor
+    tensors), covering every ciphertext of the batch; the rows written must be fresh"""
+    capacity = acc.capacity()
+    max_new = 0
+    for lhs, rhs in terms:
+        ld = lhs.digits()
+        if ld == 0 or not rhs:
+            continue
+        _check_operand(acc, lhs)
+        max_new = max(max_new, min(capacity, ld + len(rhs) - 1))
+    if max_new == 0:
+        return
+    for k in range(max_new):
+        k_terms = []
+        for lhs, rhs in terms:
+            ld = lhs.digits()
+            if ld == 0 or not rhs or k >= min(capacity, ld + len(rhs) - 1):
+                continue
+            for j in range(len(rhs)):
+                if k < j:
+                    break
+                i = k - j
+                if i < ld:
+                    k_terms.append((lhs.digit_rows(i), lhs.bound, rhs[j], 1))
+        if k_terms:
+            acc.acc.inner_product(acc.rows(k, k + 1), k_terms)
+    acc.used_digits = max(acc.used_digits, max_new)
+
+
+def dbfv_acc_inner_product_scalar(ring, acc, terms):
+    """dbfv_acc_inner_product_scalar (dbfv.rs:356-373): terms (DBFVBatch, scalar), each scalar in
+    balanced base-b digits"""
+    m = acc.modulus
+    prepared = [(lhs, [ring.scalar_tensor(acc.rns_base(), dg) for dg in balanced_digits(s, m.b(), m.digits())])
+                for lhs, s in terms]
+    dbfv_acc_inner_product_tensors(ring, acc, prepared)
+
+
+def dbfv_finish_accumulator(ring, acc):
+    """dbfv_finish_accumulator (dbfv.rs:402-406): the used rows reduced, bound 1"""
+    rows = 2 * acc.used_digits * acc.count
+    out = ring.allocate_uninit(rows, acc.rns_base())
+    if rows:
+        ring.reduce_wide_rows_into(out, acc.acc, rows)
+    return DBFVBatch(out, acc.rns_base(), acc.count, acc.basis, 1)
+
+
+def dbfv_exact_div_pow2(ring, initial_modulus, val, log2_div):
+    """dbfv_exact_div_pow2 (dbfv.rs:500-533) of a batch: digit 0 becomes
+    sum_(i <= min(k, d - 1)) b^i y_i and digit j >= 1 becomes b^k y_(j + k), k = ceil(log2_div /
+    log2_b); every call covers all ciphertexts"""
+    be = ring.backend
+    d = val.digits()
+    if d > initial_modulus.digits():
+        raise ValueError("ciphertext has more digits than the modulus")
+    if log2_div >= initial_modulus.log2_t:
+        raise ValueError("the result modulus t' = t / 2^log2_div must be at least 2")
+    if log2_div == 0 or d <= 1:
+        return val
+    log2_b = initial_modulus.log2_b
+    k = (log2_div - 1) // log2_b + 1
+    base = val.base
+    _assert_mul_mont_safe(ring, val.bound, base)
+    out_digits = max(d - k, 1)
+    n = val.count
+    result = ring.allocate_uninit(2 * out_digits * n, base)
+    if out_digits > 1:
+        be.hal_assign(result[2 * n:], val.digit_rows(k + 1, d))
+        be.hal_mul_mont64(result[2 * n:], ring.scalar_tensor_big(base, 1 << (k * log2_b)), ring.mods_for(base))
+    terms = [(val.digit_rows(i), val.bound, ring.scalar_tensor_big(base, 1 << (i * log2_b)), 1)
+             for i in range(min(k, d - 1) + 1)]
+    acc = ring.allocate_wide(2 * n, base, zero=False)
+    acc.inner_product(range(0, 2 * n), terms)
+    ring.reduce_wide_into(result[0:2 * n], acc)
+    return DBFVBatch(result, base, n, val.basis, 1)
+
+
+def dbfv_mod_switch_down(ring, ct, new_len):
+    """mod_switch_down (bfv.rs:628-662) of a period-basis batch to the length-new_len prefix of
+    its base (bound 1), one call for every ciphertext"""
+    if ct.basis != "pb" or not 1 <= new_len <= len(ct.base):
+        raise ValueError("mod_switch_down needs period basis and a prefix of the base")
+    if new_len == len(ct.base):
+        out = ring.allocate_uninit(ct.data.shape[0], ct.base)
+        ring.backend.hal_assign(out, ct.data)
+        return DBFVBatch(out, ct.base, ct.count, "pb", ct.bound)
+    _assert_mul_mont_safe(ring, ct.bound + 1, ct.base[:new_len])
+    out = ring.allocate_uninit(ct.data.shape[0], ct.base[:new_len])
+    ring.mod_switch_down(out, ct.data, ct.base, new_len)
+    return DBFVBatch(out, ct.base[:new_len], ct.count, "pb", 1)

@@ -274,6 +274,18 @@ class RingContext:
             vals.append(v - p if v > p // 2 else v)
         return self._host_constants(("q_over_p_inv", tuple(base), scale), base, vals)
 
+    def scalar_tensor_big(self, base, scalar):
+        """scalar_tensor_big (ring.rs:890-903): mont(smallest_lift(scalar mod p)) per prime, for any
+        integer scalar, mont(x) = (x 2^64) rem p with the sign of x (Rust's remainder)"""
+        vals = []
+        for i in base:
+            p = self.primes[i]
+            v = scalar % p
+            v = v - p if v > p // 2 else v
+            r = abs(v << 64) % p
+            vals.append(-r if v < 0 else r)
+        return self._host_constants(("big", tuple(base), scalar), base, vals)
+
     def scalar_tensor_big_mont2(self, base, scalar):
         """scalar_tensor_big_mont2 (ring.rs:904-919): mont(mont(smallest_lift(scalar mod p)))
         per prime, mont(x) = (x 2^64) rem p with the sign of x (Rust's remainder)"""
@@ -337,6 +349,21 @@ class RingContext:
             full = wide.needs_full_reduction(start)
             end = start + 1
             while end < wide.count() and wide.needs_full_reduction(end) == full:
+                end += 1
+            self.reduce_wide_tensor(dst[start:end], wide.data[start:end], wide.rns_base, full)
+            start = end
+
+
+    def reduce_wide_rows_into(self, dst, wide, rows):
+        """reduce_wide_rows (ring.rs:820-824) of wide rows 0 .. rows - 1 into dst: one reduction
+        kind per run of rows; every reduced row must have been written"""
+        if any(wide.bound[i] == 0 for i in range(rows)) and not wide.zeroed:
+            raise ValueError("reduce_wide: an unzeroed wide row was never written")
+        start = 0
+        while start < rows:
+            full = wide.needs_full_reduction(start)
+            end = start + 1
+            while end < rows and wide.needs_full_reduction(end) == full:
                 end += 1
             self.reduce_wide_tensor(dst[start:end], wide.data[start:end], wide.rns_base, full)
             start = end

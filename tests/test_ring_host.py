@@ -105,3 +105,19 @@ def test_host_errors():
         pkg.ring.field_info("field_7")
     with pytest.raises(pkg.HalError):
         pkg.ring.sample_primes("field_2", (230, 200), 0)
+
+
+def test_floordiv_schedule_matches_oracle():
+    """the device package's floor-division schedule (host logic) equals the oracle's, which is
+    pinned against the reference's noise model"""
+    from oracle import dbfv as OD
+    from oracle import digitex as OX
+    from oracle import ring as R
+
+    pkg = load()
+    for field, init, final, r in [("field_49152", (8, 76), (8, 64), 23), ("field_36864", (8, 76), (8, 64), 17),
+                                  ("field_192", (4, 16), (4, 12), 14), ("field_192", (8, 36), (8, 24), 16),
+                                  ("field_49152", (8, 8), (8, 7), 13)]:
+        got = pkg.digitex.floordiv_schedule(field, pkg.dbfv.DBFVModulus(*init), pkg.dbfv.DBFVModulus(*final), r)
+        want = OX.floordiv_schedule(R.FieldDescriptor(field), OD.DBFVModulus(*init), OD.DBFVModulus(*final), r)
+        assert got == want, field
