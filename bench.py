@@ -135,10 +135,70 @@ def cpu_baseline(primes_base, host_data, sample, threads, min_seconds=10.0):
                       f"elements, {done} transforms in {dt:.1f} s; one thread: {one:.1f} transforms/s"}
 
 
-def bench_galois(pkg, be, steps, batch=32, seed=SEED, cpu=True):
-    """Bootstrapping's Galois key switching at field_49152 (m = 35297665, n = 49152, 23-prime base,
-    23 gadget digits): the full homcrt rotation set (40 rotations, keys resident in HBM, 16.6 GB) on
-    a batch of `batch` ciphertexts with hoisted digits; one step = every rotation of every ciphertext."""
+def _zero_secret_rk(pkg, be, ring, base, ndigits):
+    """relinearization key of the zero secret key (gadget digit i: rows (0, g_i) in CRT basis, g_i
+    the CRT recomposition factor of the digit), so that a Delta-scaled plaintext in c0 decrypts
+    as round(t c0 / Q) through every product"""
+    digits = pkg.bfv.select_digits(ndigits, len(base))
+    q = ring.modulus_of(base)
+    n2, n3 = ring.info["deg_lprime"], ring.info["deg_k"]
+    keys = []
+    for a, b in digits:
+        di = ring.modulus_of(base[a:b])
+        g = (q // di) * pow((q // di) % di, -1, di) % q
+        host = np.zeros((2, len(base), n2, n3), dtype=np.int64)
+        for j, i in enumerate(base):
+            p = ring.primes[i]
+            host[1, j] = (g % p) * ((1 << 64) % p) % p
+        k = be.hal_allocate_i64(host.shape)
+        be.hal_import(k, host)
+        keys.append(k)
+    return keys, digits
+
+
+def _mont_table(val, p):
+    """Montgomery representatives ((val + e) 2^64 mod p, in [0, p)) for e = -3 .. 3"""
+    r = (1 << 64) % p
+    return np.array([(val + e) % p * r % p for e in range(-3, 4)], dtype=np.int64)
+
+
+def _decrypt_scalar_coeffs(pkg, be, ring, modulus, ct, c, ncoef):
+    """dbfv_dec (dbfv.rs:656-680) under the zero secret key of ciphertext c of a period-basis batch,
+    on its first ncoef coefficients (round(t c0 / Q) per digit, recombined)"""
+    D = pkg.dbfv
+    base = ct.base
+    primes = [ring.primes[i] for i in base]
+    q = ring.modulus_of(base)
+    t = modulus.t()
+    total = [0] * ncoef
+    for i in range(ct.digits()):
+        row = ct.data[2 * i * ct.count + c:2 * i * ct.count + c + 1]
+        h = be.hal_export(row).reshape(len(base), -1)[:, :ncoef]
+        for k in range(ncoef):
+            x = 0
+            for j, p in enumerate(primes):
+                v = int(h[j, k]) * pow(1 << 64, -1, p) % p
+                m = q // p
+                x += v * m * pow(m % p, -1, p)
+            x %= q
+            x = x - q if x > q // 2 else x
+            part = D.cmod(D.rounded_div(x * t, q), t)
+            total[k] += D.cmod(part, 1 << (modulus.log2_t - i * modulus.log2_b)) << (i * modulus.log2_b)
+    vals = set(D.cmod(v, t) for v in total)
+    return vals
+
+
+def bench_bootstrap_step(pkg, be, batch=32, seed=SEED, cpu=True):
+    """The dBFV bootstrapping step at field_49152 (m = 35297665, n = 49152, 23-prime base, t_bits
+    76) on `batch` ciphertexts, end to end on the device: homcrt's Galois key switching over the
+    full rotation set (40 rotations, 23 gadget digits, keys resident in HBM, 16.6 GB;
+    bfv.rs:777-841, homcrt.rs:336-429) followed by the digit extraction of the bootstrap,
+    hom_floordiv_dbfv from the intermediate modulus (b = 2^8, t = 2^76) to t = 2^64 (v = 12 levels
+    of the 8-product circuit, bootstrap/mod.rs:369-382, digitex.rs:267-367) with the relinearization
+    key of ceil(23 / 2) = 12 digits (bootstrap/mod.rs:535). The key-switch inputs are uniform
+    residues (hoisted digits and c0); the floor division's inputs are dBFV encryptions under the
+    zero secret key of random 76-bit scalars, so the result's decryption is checked."""
+    D = pkg.dbfv
     field, log2_q, t_bits = "field_49152", (1280, 1300), 76
     ring = pkg.ring.RingContext(be, field, log2_q, t_bits)
     base = ring.initial_rns_base()
@@ -146,6 +206,9 @@ def bench_galois(pkg, be, steps, batch=32, seed=SEED, cpu=True):
     d = r
     gs, _ = pkg.ring.homcrt_galois_elements(field)
     n2, n3 = ring.info["deg_lprime"], ring.info["deg_k"]
+    stream = be.stream()
+    ev = HipEvents()
+    # key switching operands
     parts = be.hal_allocate_i64((batch * d, r, n2, n3))
     c0 = be.hal_allocate_i64((batch, r, n2, n3))
     keys = [be.hal_allocate_i64((2 * d, r, n2, n3)) for _ in gs]
@@ -154,30 +217,98 @@ def bench_galois(pkg, be, steps, batch=32, seed=SEED, cpu=True):
     ring.fill_uniform(c0, base, seed + 2)
     for t, k in enumerate(keys):
         ring.fill_uniform(k, base, seed + 100 + t)
-    stream = be.stream()
-    ev = HipEvents()
-    ring.galois_keyswitch(base, batch, d, parts, c0, gs, keys, out)  # warm-up
+    # floor-division operands: `batch` dBFV ciphertexts of 10 digits, digit-major
+    init, final = D.DBFVModulus(8, 76), D.DBFVModulus(8, 64)
+    rk, rk_digits = _zero_secret_rk(pkg, be, ring, base, (r - 1) // 2 + 1)
+    rng = np.random.default_rng(seed + 3)
+    msgs = [int(rng.integers(0, 1 << 62)) << 14 | int(rng.integers(0, 1 << 14)) for _ in range(batch)]
+    q = ring.modulus_of(base)
+    delta = D.rounded_div(q, init.t())
+    sign = D.scale_q_to_m_period_basis(ring)
+    nd = init.digits()
+    # the floor division runs on chunks of `chunk` ciphertexts: its v + 1 i128 accumulators take
+    # 4.7 GB per ciphertext at the full base and a product's temporaries about as much again, beside
+    # the keys and the key switch's outputs
+    chunk = min(batch, 8)
+    nchunks = (batch + chunk - 1) // chunk
+    chunks = []
+    for k in range(nchunks):
+        cnt = min(chunk, batch - k * chunk)
+        cts = be.hal_allocate_i64((2 * nd * cnt, r, n2, n3))
+        ring.fill_uniform(cts, base, seed + 4 + k)  # the c1 rows: uniform (a zero-secret decryption never reads them)
+        for i in range(nd):  # c0 of digit i: Delta m + e for digit 0, e for the others, |e| <= 3
+            host = np.empty((cnt, r, n2, n3), dtype=np.int64)
+            for c in range(cnt):
+                err = rng.integers(-3, 4, size=(n2, n3)) + 3
+                val = D.cmod(msgs[k * chunk + c] * sign, init.t()) * delta if i == 0 else 0
+                for j, idx in enumerate(base):
+                    host[c, j] = _mont_table(val, ring.primes[idx])[err]
+            be.hal_import(cts[2 * i * cnt:(2 * i + 1) * cnt], host)
+            del host
+        chunks.append(D.DBFVBatch(cts, base, cnt, "pb", 1))
+    # warm-up (builds the conversion and transform caches of every level's base) on ciphertext 0
+    first = chunks[0]
+    one = D.DBFVBatch(be.hal_allocate_i64((2 * nd, r, n2, n3)), base, 1, "pb", 1)
+    for row in range(2 * nd):
+        be.hal_assign(one.data[row:row + 1], first.data[row * first.count:row * first.count + 1])
+    single = pkg.digitex.hom_floordiv_dbfv(ring, init, final, one, rk, rk_digits)
+    ring.galois_keyswitch(base, batch, d, parts, c0, gs, keys, out)
     be.sync()
-    be.set_timing(True)
+    # the timed step: every rotation of every ciphertext, then the floor division of the batch
+    ev_ks = HipEvents()
     ev.start(stream)
-    for _ in range(steps):
-        ring.galois_keyswitch(base, batch, d, parts, c0, gs, keys, out)
+    ev_ks.start(stream)
+    ring.galois_keyswitch(base, batch, d, parts, c0, gs, keys, out)
+    ks_ms = ev_ks.stop(stream)
+    t0 = time.perf_counter()
+    results = [pkg.digitex.hom_floordiv_dbfv(ring, init, final, ch, rk, rk_digits) for ch in chunks]
     ms = ev.stop(stream)
+    fd_wall = time.perf_counter() - t0
+    # checks: the batch gives ciphertext 0 the residues of its own floor division, and the results
+    # decrypt to floor(m / 2^12) mod 2^64 (first 16 coefficients of the first and last ciphertext)
+    res0 = results[0]
+    for row in range(single.data.shape[0]):
+        if not (be.hal_export(single.data[row:row + 1]) ==
+                be.hal_export(res0.data[row * res0.count:row * res0.count + 1])).all():
+            raise SystemExit("bench: batched floor division differs from the single-ciphertext one")
+    for k, c in [(0, 0), (nchunks - 1, results[-1].count - 1)]:
+        got = _decrypt_scalar_coeffs(pkg, be, ring, final, results[k], c, 16)
+        want = {D.cmod(D.cmod((msgs[k * chunk + c] % init.t()) >> 12, final.t()) * sign, final.t())}
+        if got != want:
+            raise SystemExit(f"bench: floor division of ciphertext {k * chunk + c} decrypts to {got}, want {want}")
+    del results, res0
+    # per-kernel view of the key switch (HIP events around each launch)
+    be.set_timing(True)
+    ring.galois_keyswitch(base, batch, d, parts, c0, gs, keys, out)
+    be.sync()
     be.set_timing(False)
     rep = be.timing_report()["galois_keyswitch"]
     avg_ms = rep[1] / rep[0]
     bytes_per_launch = rep[2] / rep[0]
-    res = {
-        "workload": f"field_49152 Galois key switching: {len(gs)} homcrt rotations x {batch} ciphertexts, r = {r} primes, "
-                    f"{d} gadget digits, keys {len(gs) * 2 * d * r * n2 * n3 * 8 / 2**30:.1f} GiB resident in HBM",
-        "value": round(len(gs) * batch * steps / (ms / 1000.0), 1), "unit": "key switches/s",
-        "ms_per_step": round(ms / steps, 3),
-        "kernel": {"name": "galois_keyswitch", "avg_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch,
-                   "GB/s": round(bytes_per_launch / (avg_ms / 1000.0) / 1e9, 1),
-                   "frac": round(bytes_per_launch / (avg_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)},
+    v, mults, _ = pkg.digitex.floordiv_schedule(field, init, final, r)
+    products = v * mults
+    res_json = {
+        "workload": f"field_49152 dBFV bootstrapping step on {batch} ciphertexts: Galois key switching over the {len(gs)} "
+                    f"homcrt rotations (r = {r} primes, {d} gadget digits, keys "
+                    f"{len(gs) * 2 * d * r * n2 * n3 * 8 / 2**30:.1f} GiB resident in HBM), then hom_floordiv_dbfv "
+                    f"t = 2^76 -> 2^64 (b = 2^8, 10 digits, v = 12 levels x 8 products, relinearization key of "
+                    f"{len(rk_digits)} digits; chunks of {chunk} ciphertexts)",
+        "value": round(ms, 1), "unit": "ms per step", "higher_is_better": False,
+        "ciphertexts_per_s": round(batch / (ms / 1000.0), 2),
+        "ms_per_ciphertext": round(ms / batch, 2),
+        "key_switch": {"ms": round(ks_ms, 3), "value": round(len(gs) * batch / (ks_ms / 1000.0), 1),
+                       "unit": "key switches/s",
+                       "kernel": {"name": "galois_keyswitch", "avg_ms": round(avg_ms, 4),
+                                  "bytes_per_launch": bytes_per_launch,
+                                  "GB/s": round(bytes_per_launch / (avg_ms / 1000.0) / 1e9, 1),
+                                  "frac": round(bytes_per_launch / (avg_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)}},
+        "floor_division": {"ms": round(ms - ks_ms, 1), "dbfv_products": products * batch,
+                           "ms_per_product_per_ciphertext": round((ms - ks_ms) / (products * batch), 3),
+                           "host_wall_s": round(fd_wall, 2)},
+        "checks": "batch == single-ciphertext floor division (bit-identical); decryptions of ciphertexts 0 and "
+                  f"{batch - 1} under the zero secret key equal floor(m / 2^12) mod 2^64",
     }
     # the fallback for key sets beyond HBM: keys in pinned host memory, streamed per rotation
-    # (copy of key t + 1 overlapping the kernel of key t); a subset of the rotations
     nstream = min(8, len(gs))
     host = []
     for k in keys[:nstream]:
@@ -188,43 +319,93 @@ def bench_galois(pkg, be, steps, batch=32, seed=SEED, cpu=True):
     ring.galois_keyswitch_streamed(base, batch, d, parts, c0, gs[:nstream], host, out_s)  # warm-up
     be.sync()
     ev.start(stream)
-    for _ in range(steps):
-        ring.galois_keyswitch_streamed(base, batch, d, parts, c0, gs[:nstream], host, out_s)
-    ms_s = ev.stop(stream) / steps
+    ring.galois_keyswitch_streamed(base, batch, d, parts, c0, gs[:nstream], host, out_s)
+    ms_s = ev.stop(stream)
     if not (be.hal_export(out_s[0:2]) == be.hal_export(out[0:2])).all():
         raise SystemExit("bench: streamed-key key switch differs from the resident one")
     key_bytes = 2 * d * r * n2 * n3 * 8
-    res["streamed_keys"] = {
+    res_json["key_switch"]["streamed_keys"] = {
         "rotations": nstream, "ms": round(ms_s, 3), "value": round(nstream * batch / (ms_s / 1000.0), 1),
         "unit": "key switches/s", "host_to_device_GB/s": round(nstream * key_bytes / (ms_s / 1000.0) / 1e9, 1),
-        "note": "keys in pinned host memory, double-buffered over PCIe; the resident form above reads them from HBM",
+        "note": "keys in pinned host memory, double-buffered over PCIe; the resident form reads them from HBM",
     }
     for h in host:
         h.free()
     del out_s
     if cpu:
-        # oracle: the unfused reference sequence for one (ciphertext, rotation) on the same data,
-        # which doubles as a full-size spot check of the device result
-        from oracle import cpu_backend
-        from oracle import ring as oring
-
-        cb = cpu_backend.CpuBackend()
-        oc = oring.RingContext(cb, oring.FieldDescriptor(field), log2_q, t_bits)
-        hp = be.hal_export(parts[0:d]).reshape(d, r, n2, n3)
-        hc = np.zeros((2, r, n2, n3), dtype=np.int64)
-        hc[0] = be.hal_export(c0[0:1]).reshape(r, n2, n3)
-        hk = be.hal_export(keys[0]).reshape(2 * d, r, n2, n3)
-        t0 = time.perf_counter()
-        exp = oring.hom_gal_one(oc, hc, base, gs[0], hp, [hk[2 * i:2 * i + 2] for i in range(d)], [(i, i + 1) for i in range(d)])
-        dt = time.perf_counter() - t0
-        got = be.hal_export(out[0:2]).reshape(2, r, n2, n3)
-        if not (got == exp).all():
-            raise SystemExit("bench: galois key switch differs from the oracle")
-        res["cpu_baseline"] = {"value": round(1.0 / dt, 3), "unit": "key switches/s", "cores": 1, "kind": "port",
-                               "sample": f"1 ciphertext x 1 rotation, same data, in RAM ({dt:.2f} s); also checked bit-exact"}
-    del parts, c0, keys, out
+        res_json["cpu_baseline"] = _bootstrap_cpu_baseline(pkg, be, ring, base, parts, c0, keys[0], out, gs[0], d,
+                                                           first, batch, rk_digits, len(gs), products)
+    del parts, c0, keys, out, chunks, first, single, one
     ring.close()
-    return res
+    return res_json
+
+
+def _bootstrap_cpu_baseline(pkg, be, ring, base, parts, c0, key0, out, g0, d, first, batch, rk_digits, nrot, products):
+    """The oracle on the host, one thread: one hoisted Galois key switch with the key read through
+    keystore semantics (keystore.rs:50-127: written to an unlinked scratch file and read through a
+    shared mapping; the page cache holds it, as for the reference after its first read), checked
+    bit-exact against the device; and one dbfv_hom_square of a 10-digit ciphertext of the batch at
+    the full base (the product the floor division runs 96 times per ciphertext). The step estimate
+    scales both to the device step: batch x rotations key switches plus batch x 96 products."""
+    import mmap
+    import tempfile
+
+    from oracle import cpu_backend
+    from oracle import dbfv as OD
+    from oracle import ring as oring
+
+    r = len(base)
+    n2, n3 = ring.info["deg_lprime"], ring.info["deg_k"]
+    cb = cpu_backend.CpuBackend()
+    oc = oring.RingContext(cb, oring.FieldDescriptor("field_49152"), (1280, 1300), 76)
+    hp = be.hal_export(parts[0:d]).reshape(d, r, n2, n3)
+    hc = np.zeros((2, r, n2, n3), dtype=np.int64)
+    hc[0] = be.hal_export(c0[0:1]).reshape(r, n2, n3)
+    hk = be.hal_export(key0).reshape(2 * d, r, n2, n3)
+    fd, path = tempfile.mkstemp(prefix="csd_keystore_")
+    os.unlink(path)
+    try:
+        os.write(fd, hk.tobytes())
+        os.fsync(fd)
+        mm = mmap.mmap(fd, hk.nbytes, mmap.MAP_SHARED, mmap.PROT_READ)
+        if hasattr(mm, "madvise") and hasattr(mmap, "MADV_SEQUENTIAL"):
+            mm.madvise(mmap.MADV_SEQUENTIAL)
+        kv = np.frombuffer(mm, dtype=np.int64).reshape(hk.shape)
+        del hk
+        t0 = time.perf_counter()
+        exp = oring.hom_gal_one(oc, hc, base, g0, hp, [kv[2 * i:2 * i + 2] for i in range(d)],
+                                [(i, i + 1) for i in range(d)])
+        ks_dt = time.perf_counter() - t0
+        del kv
+        mm.close()
+    finally:
+        os.close(fd)
+    if not (be.hal_export(out[0:2]).reshape(2, r, n2, n3) == exp).all():
+        raise SystemExit("bench: galois key switch differs from the oracle")
+    # one dBFV square of ciphertext 0 of the batch at the full base
+    nd = first.digits()
+    ct0 = np.empty((2 * nd, r, n2, n3), dtype=np.int64)
+    for row in range(2 * nd):
+        ct0[row] = be.hal_export(first.data[row * first.count:row * first.count + 1]).reshape(r, n2, n3)
+    okeys = []
+    q = oc.modulus_of(base)
+    for a, b in rk_digits:
+        di = oc.modulus_of(base[a:b])
+        gq = (q // di) * pow((q // di) % di, -1, di) % q
+        k = oc.allocate(2, base)
+        for j, i in enumerate(base):
+            k[1, j] = oring.montgomery_repr(gq % oc.primes[i], oc.primes[i])
+        okeys.append(k)
+    t0 = time.perf_counter()
+    OD.dbfv_hom_square(oc, OD.DBFVModulus(8, 76), ct0, 1, base, okeys, rk_digits, "pb")
+    sq_dt = time.perf_counter() - t0
+    step_s = batch * nrot * ks_dt + batch * products * sq_dt
+    return {"value": round(step_s * 1000.0, 1), "unit": "ms per step (estimate)", "cores": 1, "kind": "port",
+            "key_switches/s": round(1.0 / ks_dt, 3), "dbfv_products/s": round(1.0 / sq_dt, 4),
+            "sample": f"1 hoisted key switch (23 digits, key mmap'd from an unlinked scratch file: {ks_dt:.2f} s, checked "
+                      f"bit-exact) and 1 dbfv_hom_square of a 10-digit ciphertext over 23 primes ({sq_dt:.1f} s), one "
+                      f"thread; value = {batch} x {nrot} key switches + {batch} x {products} products at those rates "
+                      f"(an extrapolation, the products of later levels run on shorter bases)"}
 
 
 def bench_batch(pkg, be, steps, count=256, seed=SEED, cpu=True):
@@ -744,8 +925,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64, help="ring elements in the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="hom_mul,batch,zp,galois,dbfv,dbfv_mul",
-                    help="comma-separated extra workloads beside the headline transforms: hom_mul, batch, zp, galois, "
+    ap.add_argument("--workloads", default="hom_mul,batch,zp,bootstrap,dbfv,dbfv_mul",
+                    help="comma-separated extra workloads beside the headline transforms: hom_mul, batch, zp, bootstrap, "
                          "dbfv, dbfv_mul, or none")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -883,8 +1064,8 @@ def main():
         workloads["batch_uneven_chain"] = bench_batch(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
     if "zp" in chosen:
         workloads["dense_zp_mfma"] = bench_zp(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
-    if "galois" in chosen:
-        workloads["galois_keyswitch"] = bench_galois(pkg, be, max(1, args.steps // 5), cpu=not args.no_cpu)
+    if "bootstrap" in chosen:
+        workloads["bootstrap_step_c5"] = bench_bootstrap_step(pkg, be, cpu=not args.no_cpu)
     if "dbfv" in chosen:
         workloads["dbfv_reduce"] = bench_dbfv(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
     if "dbfv_mul" in chosen:

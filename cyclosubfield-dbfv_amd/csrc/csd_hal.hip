@@ -52,6 +52,14 @@ void* csd_ctx::alloc_bytes(size_t bytes) {
         p = it->second.back();
         it->second.pop_back();
     } else {
+        // keep a reserve free for the runtime (kernel arguments, scratch): when a new block would
+        // cut into it, the cached blocks go back to the device first
+        size_t free_b = 0, total_b = 0;
+        if (!pool.empty() && hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b < bytes + ((size_t)2 << 30)) {
+            for (auto& kv : pool)
+                for (void* q : kv.second) (void)hipFree(q);
+            pool.clear();
+        }
         hipError_t e = hipMalloc(&p, bytes);
         if (e != hipSuccess) {
             // release cached blocks and retry once
