@@ -46,9 +46,14 @@ void* csd_ctx::alloc_bytes(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
     if (bytes == 0) bytes = 256;
     std::lock_guard<std::mutex> g(mu);
-    auto it = pool.find(bytes);
+    // the smallest cached block of at least `bytes`, if it is at most 1.5 x as large: sequences
+    // whose temporaries shrink step by step (bases losing limbs level by level) reuse blocks
+    // instead of growing the cache with every size
+    auto it = pool.lower_bound(bytes);
+    while (it != pool.end() && it->second.empty()) ++it;
     void* p = nullptr;
-    if (it != pool.end() && !it->second.empty()) {
+    if (it != pool.end() && it->first <= bytes + bytes / 2) {
+        bytes = it->first;
         p = it->second.back();
         it->second.pop_back();
     } else {

@@ -1210,6 +1210,66 @@ __global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// field_49152's K part ([3, 64, 64], lines of 12288) through 32 KiB of LDS. Factor 0 (C_3, suffix
+// 4096) is a single stage whose butterfly links the elements e, e + 4096, e + 8192; after it,
+// factors 1 and 2 act inside each 4096-element sub-line (their twiddle blocks are taken mod the
+// stage's block count, so the prefix does not enter). The workgroup of a line runs the C_3
+// butterflies on the load, keeps sub-line 0 in LDS and writes sub-lines 1 and 2 back in place (the
+// line is its own), then transforms the three sub-lines one after another, reloading 1 and 2 from
+// L2. A whole line in LDS (96 KiB) leaves one workgroup per CU; sub-lines leave five.
+template <bool INV>
+__global__ __launch_bounds__(256) void k_ntt_rows_split3(RowArgs a, ModTable mt) {
+    extern __shared__ int64_t lds[];
+    constexpr uint32_t E = 4096;
+    constexpr int L = E;
+    const int64_t row = blockIdx.x % a.d2;
+    const int64_t ij = blockIdx.x / a.d2;
+    const int64_t j = ij % a.d1, i = ij / a.d1;
+    int64_t* base = a.data + i * a.st[0] + j * a.st[1] + row * a.st[2];
+    const DevPlan* __restrict__ P = a.plans[j];
+    const CsdBarrett* __restrict__ TW = (const CsdBarrett*)(P + 1);
+    const int dir = a.dir;
+    {
+        const DevFactor& F0 = P->f[0];
+        for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
+            int64_t g[3] = {base[e], base[E + e], base[2 * E + e]};
+            ct_stages<3, 1, 0, INV>(g, P, TW, F0, dir, 0u);
+            lds[e] = g[0];
+            base[E + e] = g[1];
+            base[2 * E + e] = g[2];
+        }
+        __syncthreads();
+    }
+    const CsdMod m = mt.m[j];
+    for (int k = 0; k < 3; k++) {
+        if (k > 0) {
+            for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) lds[e] = base[k * E + e];
+            __syncthreads();
+        }
+        int64_t* out = base + (int64_t)k * E;
+        const int64_t* mul = a.mul ? a.mul + i * a.smul.s[0] + j * a.smul.s[1] + row * a.smul.s[2] + (int64_t)k * E * a.smul.s[3]
+                                   : nullptr;
+        if constexpr (!INV) {
+            ct_pass<L, 1, 2, 3, 64, 64, 0, INV>(lds, 1, P, TW, dir);
+            ct_pass<L, 1, 2, 3, 64, 8, 3, INV>(lds, 1, P, TW, dir);
+            ct_pass<L, 2, 2, 3, 1, 64, 0, INV>(lds, 1, P, TW, dir);
+            ct_pass<L, 2, 2, 3, 1, 8, 3, INV>(lds, 1, P, TW, dir);
+            for (uint32_t p = threadIdx.x; p < E; p += blockDim.x) {
+                int64_t x = lds[p];
+                if (mul) x = csd_mont_mul(m, x, mul[p * a.smul.s[3]]);
+                out[p] = x;
+            }
+        } else {
+            ct_pass<L, 1, 2, 3, 64, 8, 3, INV>(lds, 1, P, TW, dir);
+            ct_pass<L, 1, 2, 3, 64, 64, 0, INV>(lds, 1, P, TW, dir);
+            ct_pass<L, 2, 2, 3, 1, 8, 3, INV>(lds, 1, P, TW, dir);
+            ct_pass_store<L, 2, 2, 3, 1, 64, 0, INV>(lds, 1, P, TW, dir, out, 0, mul, 0, a.smul.s[3], m);
+        }
+        __syncthreads();  // the next sub-line overwrites the LDS the store read
+    }
+}
+
 // dim-2 transforms of short length: one thread owns a whole column (F0 * F1 values) in registers.
 // F0, F1 are the factor lengths (F1 = 1 for single-factor plans, F0 = F1 = 1 for the identity).
 template <int F>
@@ -1369,10 +1429,10 @@ extern "C" int csd_ntt_set_row_shape(csd_ctx* ctx, int lines, int threads) {
 }
 
 // ctx->ntt_mode: 1 forces the one-stage-per-LDS-pass kernel, 2 keeps dim-3 transforms on the
-// plan-driven row kernel (tests)
+// plan-driven row kernel, 3 runs field_49152's lines whole instead of as sub-lines (tests)
 extern "C" int csd_ntt_force_generic(csd_ctx* ctx, int mode) {
     CSD_TRY if (!ctx) throw CsdError("null context");
-    if (mode < 0 || mode > 2) throw CsdError("csd_ntt_force_generic: mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) throw CsdError("csd_ntt_force_generic: mode must be 0, 1, 2 or 3");
     ctx->ntt_mode = mode;
     CSD_CATCH
 }
@@ -1486,6 +1546,22 @@ static void launch_ntt(csd_ctx* ctx, const csd_view* dst, csd_ntt_plan* const* p
         const bool measured_3072 = L == 3072;
         int shape = 0;
         RowKernel rk = pick_row_kernel(ctx, plans, nplans, dir, &shape);
+        if (shape == 2 && ctx->ntt_mode == 0 && ctx->row_lines == 0 && ctx->row_threads == 0) {
+            // field_49152's lines through sub-lines of 32 KiB (k_ntt_rows_split3)
+            r.mul = a.mul;
+            r.smul = a.smul;
+            const int64_t grid = a.d0 * a.d1 * a.d2;
+            if (grid > (int64_t)INT32_MAX) throw CsdError("ntt: too many tiles");
+            csd_allow_lds((const void*)k_ntt_rows_split3<false>, 32 * 1024);
+            csd_allow_lds((const void*)k_ntt_rows_split3<true>, 32 * 1024);
+            TimedScope ts(ctx, names[1][dir], 16.0 * (double)numel(dst));
+            if (dir == 2)
+                hipLaunchKernelGGL(k_ntt_rows_split3<true>, dim3((unsigned)grid), dim3(256), 32 * 1024, ctx->stream, r, t);
+            else
+                hipLaunchKernelGGL(k_ntt_rows_split3<false>, dim3((unsigned)grid), dim3(256), 32 * 1024, ctx->stream, r, t);
+            HIPCHECK(hipGetLastError());
+            return;
+        }
         const uint32_t lline = row_lds_line(shape, (uint32_t)L);
         int rows = 1;
         while (!measured_3072 && rows * 2 * L <= 6144 && rows * 2 <= a.d2 && rows < 256) rows *= 2;

@@ -415,7 +415,7 @@ def test_row_launch_shape_errors(gpu):
 
 @pytest.mark.parametrize("field,log2_q,inner", [("field_36864", (960, 980), [2, 3, 8, 64]),
                                                 ("field_49152", (1280, 1300), [3, 64, 64])])
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode", [0, 2, 3])
 def test_compiled_row_kernels_parity(gpu, cpu, field, log2_q, inner, mode):
     """Dimension-3 transforms of the benchmark fields' K parts: the compiled-shape row kernels
     (mode 0, the default) and the plan-driven row kernel (mode 2) are both bit-exact with the

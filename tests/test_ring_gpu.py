@@ -51,7 +51,8 @@ def uniform_residues(rng, primes, shape):
 
 
 @pytest.mark.parametrize("field,log2_q,t_bits,count", [("field_2", (200, 230), 0, 16), ("field_18", (100, 120), 4, 3),
-                                                       ("field_192", (100, 120), 8, 2), ("field_36864", (960, 980), 76, 2)])
+                                                       ("field_192", (100, 120), 8, 2), ("field_36864", (960, 980), 76, 2),
+                                                       ("field_49152", (1280, 1300), 76, 2)])
 def test_transform_parity(gpu, cpu, rings, field, log2_q, t_bits, count):
     """Config C1 (field_2, 4 primes, 16 elements) and larger fields: pb->CRT, a pointwise
     Montgomery product, CRT->pb; every residue bit-exact with the oracle."""
