@@ -518,34 +518,56 @@ def bench_batch(pkg, be, steps, count=256, seed=SEED, cpu=True):
         "kernels": per_kernel,
     }
     if cpu:
-        # oracle: the reference sequence on the two smallest ciphertexts, checked against the device
+        # oracle: the reference sequence on a sample that follows the batch's limb distribution
+        # (16 ciphertexts at evenly spaced ranks of the sorted limb counts), checked against the
+        # device; one thread, then the host's threads over disjoint ciphertexts
+        from concurrent.futures import ThreadPoolExecutor
+
         from oracle import cpu_backend
         from oracle import ring as oring
 
         cb = cpu_backend.CpuBackend()
         orc = oring.RingContext(cb, oring.FieldDescriptor(FIELD), LOG2_Q, T_BITS)
-        idx = sorted(range(count), key=lambda c: limbs[c])[:2]
+        order = sorted(range(count), key=lambda c: limbs[c])
+        nsamp = min(16, count)
+        idx = [order[(2 * k + 1) * count // (2 * nsamp)] for k in range(nsamp)]
         reset()
         hosts = [be.hal_export(cts[c]).reshape(cts[c].shape) for c in idx]
         hpt = be.hal_export(pt).reshape(pt.shape)
         plan.run(graph=True)
-        dt = 0.0
-        for c, h in zip(idx, hosts):
+        for r in sorted(set(limbs[c] for c in idx)):
+            orc.transform_cache(list(range(r)))
+
+        def chain(k):
+            c, h = idx[k], hosts[k]
             r = limbs[c]
             b = list(range(r))
-            orc.transform_cache(b)
             x = h.copy()
-            t0 = time.perf_counter()
             orc.pb_to_crt_tensor(x, b)
             cb.hal_mul_mont64(x, np.ascontiguousarray(hpt[:, :r]), orc.mods_for(b))
             orc.crt_to_pb_tensor(x, b)
-            exp = oring.mod_switch_down(orc, x.copy(), b, r - 1)
-            dt += time.perf_counter() - t0
-            if not (be.hal_export(outs[c]).reshape(outs[c].shape) == exp).all():
+            return oring.mod_switch_down(orc, x.copy(), b, r - 1)
+
+        mid = nsamp // 2
+        t0 = time.perf_counter()
+        exp0 = chain(mid)
+        one = time.perf_counter() - t0
+        threads = cpu_threads()
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            exps = list(ex.map(chain, range(nsamp)))
+        dt = time.perf_counter() - t0
+        if not (exps[mid] == exp0).all():
+            raise SystemExit("bench: oracle chain is not deterministic")
+        for k, c in enumerate(idx):
+            if not (be.hal_export(outs[c]).reshape(outs[c].shape) == exps[k]).all():
                 raise SystemExit("bench: batched chain differs from the oracle")
-        res["cpu_baseline"] = {"value": round(len(idx) / dt, 3), "unit": "ciphertexts/s", "cores": 1, "kind": "port",
-                               "sample": f"{len(idx)} ciphertexts at {[limbs[c] for c in idx]} limbs ({dt:.2f} s), "
-                                         "also checked bit-exact; the batch mean is higher"}
+        mean_s = float(np.mean([limbs[c] for c in idx]))
+        res["cpu_baseline"] = {"value": round(nsamp / dt, 3), "unit": "ciphertexts/s", "cores": threads, "kind": "port",
+                               "sample": f"{nsamp} ciphertexts at evenly spaced ranks of the batch's limb counts (mean "
+                                         f"{mean_s:.2f}, batch mean {np.mean(limbs):.2f}) on {threads} threads over disjoint "
+                                         f"ciphertexts ({dt:.2f} s), each checked bit-exact; one thread: "
+                                         f"{1.0 / one:.2f} ciphertexts/s at {limbs[idx[mid]]} limbs"}
     plan.close()
     del cts, outs, ct_arena, out_arena, pt
     ring.close()
@@ -785,6 +807,42 @@ def bench_dbfv_mul(pkg, be, steps, count=16, log2_b=27, ndigits=4, kdigits=9, se
     return res
 
 
+def bench_mfma_form(pkg, be, ring, base, data, host, primes, stream, steps):
+    """The int8-MFMA form of the same step on the same batch (mfma_transform: the K part as F_64,
+    F_48, the kernel's diagonal and their inverses on the matrix cores, the L' part by the NTT
+    passes), equal mod p to the plain form (L1) and checked so; per-kernel times from HIP events."""
+    mf = pkg.mfma_transform.MfmaTransforms(ring, base)
+    ev = HipEvents()
+    x = be.allocate_i64_uninit(list(data.shape))
+    be.hal_import(x, host)
+    mf.pb_to_crt_tensor(x)
+    mf.crt_to_pb_tensor(x)
+    be.sync()
+    ev.start(stream)
+    for _ in range(steps):
+        mf.pb_to_crt_tensor(x)
+        mf.crt_to_pb_tensor(x)
+    ms = ev.stop(stream) / steps
+    be.set_timing(True)
+    mf.pb_to_crt_tensor(x)
+    mf.crt_to_pb_tensor(x)
+    be.sync()
+    be.set_timing(False)
+    kern = {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in be.timing_report().items()}
+    # every step is a round trip: the batch is unchanged mod p
+    out = be.hal_export(x[0:1]).reshape(host[0:1].shape)
+    pvec = np.array(primes, dtype=object).reshape(1, -1, 1, 1)
+    if not (((out.astype(object) - host[0:1].astype(object)) % pvec) == 0).all():
+        raise SystemExit("bench: MFMA-form round trip failed")
+    del x, mf
+    transforms = 2 * data.shape[0]
+    return {"value": round(transforms / (ms / 1000.0), 1), "unit": "transforms/s", "ms_per_step": round(ms, 3),
+            "parity": "L1: canonical residues equal mod p to the plain form",
+            "form": "K part as F_64 (64 x 64), F_48 (48 x 48), diag(F k), F_48^-1, F_64^-1 on int8 MFMA "
+                    "(csd_zp_apply_dim, 224 Z_p-MACs per coefficient), L' part by the NTT passes",
+            "kernels_per_step": kern}
+
+
 def be_import(be, host):
     """a device tensor holding the host array `host`"""
     t = be.hal_allocate_i64(host.shape)
@@ -986,6 +1044,7 @@ def main():
         step()
     two_stream_ms = ev.stop(stream) / max(2, args.steps // 2)
     ring.set_overlap(False)
+    mfma = bench_mfma_form(pkg, be, ring, base, data, host, primes, stream, max(2, args.steps // 2))
 
     # size-independent check: every step is a round trip, so the batch is unchanged mod p
     out = be.hal_export(data[0:1]).reshape(host[0:1].shape)
@@ -1046,6 +1105,7 @@ def main():
         "ceilings": {"hbm_copy_GB/s": round(copy_gbs, 1), "barrett_Gmul/s": round(barrett_gps, 1)},
         "kernels": per_kernel,
         "two_stream_ms_per_step": round(two_stream_ms, 3),
+        "mfma_form": mfma,
         "overlap": "the passes run serially (the default); two_stream_ms_per_step is the optional two-stream "
                    "half-batch overlap, one pass apart, which measures 1-3 % slower",
         "wall_s": round(wall, 3),
