@@ -13,6 +13,7 @@ the oracle (the C++ restatement of hal::cpu, single thread) timed on a bounded s
 """
 import argparse
 import ctypes
+import hashlib
 import json
 import os
 import sys
@@ -58,27 +59,45 @@ def uniform_batch(primes, count, deg_lp, deg_k, seed):
     return out
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if present."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+# the sources the headline transform kernels are built from; the committed counter summaries
+# record their hash and are used only while it matches
+PROFILED_SOURCES = ("cyclosubfield-dbfv_amd/csrc/csd_ntt.hip", "cyclosubfield-dbfv_amd/csrc/csd_arith.hpp")
+
+
+def profiled_sources_sha256():
+    h = hashlib.sha256()
+    for rel in PROFILED_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def profile_summary(name, count):
+    """The committed rocprofv3 counter summary profiles/<name>, or None unless it was measured on a
+    batch of `count` ring elements and on the kernel sources as they are now (same hash)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         data = json.load(f)
-    entry = data.get(kernel)
+    if data.get("count") != count or data.get("source_sha256") != profiled_sources_sha256():
+        return None
+    return data
+
+
+def traffic_from_profiles(kernel, count):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if it applies."""
+    data = profile_summary("hbm_traffic.json", count)
+    entry = None if data is None else data.get(kernel)
     return None if entry is None else entry.get("bytes_per_launch")
 
 
-def valu_issue_from_profiles(kernel, avg_ms):
+def valu_issue_from_profiles(kernel, avg_ms, count):
     """The VALU issue fraction of `kernel` at `avg_ms` per launch: its wave instructions per launch
     (the committed rocprofv3 SQ_INSTS_VALU summary) x 4 issue cycles, over the 1024 SIMDs at 2.4 GHz
-    for the launch time; None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "valu_counters.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        data = json.load(f)
-    entry = data.get(kernel)
+    for the launch time; None when the summary is absent or does not apply."""
+    data = profile_summary("valu_counters.json", count)
+    entry = None if data is None else data.get(kernel)
     if entry is None or not avg_ms:
         return None
     cycles = entry["sq_insts_valu"] * data["cycles_per_wave_inst"] / data["simds"]
@@ -514,6 +533,8 @@ def bench_batch(pkg, be, steps, count=256, seed=SEED, cpu=True):
         "tuning": tuning, "autotune_s": round(tune_s, 2),
         ("staged_ms_per_step" if chosen["persistent"] else "persistent_ms_per_step"): round(other_ms, 3),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "headline_frac": round(value / headline_peak, 4),
+                     "headline_peak": round(headline_peak, 1),
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": round(total_ms / launches, 4)},
         "kernels": per_kernel,
     }
@@ -1054,6 +1075,9 @@ def main():
 
     ms_per_step = ms / args.steps
     value = transforms_per_step / (ms_per_step / 1000.0)
+    # the whole transform at the HBM roofline: one read and one write of every residue of the
+    # element (16 B x primes x coefficients) at the 8 TB/s peak
+    headline_peak = HBM_PEAK_GBS * 1e9 / (16 * len(primes) * info["deg_lprime"] * info["deg_k"])
     dom = max(kernels.items(), key=lambda kv: kv[1][1])
     name, (launches, total_ms, total_bytes) = dom
     avg_ms = total_ms / launches
@@ -1097,12 +1121,17 @@ def main():
                                f"x 17 primes x 36864 coefficients, [{args.count}, 17, 12, 3072] int64 in HBM",
                    "field": FIELD, "rns_primes": len(primes), "elements": args.count, "form": "plain"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
+                     "headline_frac": round(value / headline_peak, 4),
+                     "headline_peak": round(headline_peak, 1),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name, args.count),
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 4),
                      "valu": {"achieved": per_kernel[name].get("Gmul/s"), "ceiling": round(barrett_gps, 1),
                               "unit": "G modmul/s", "frac": per_kernel[name].get("valu_frac")},
-                     "valu_issue": valu_issue_from_profiles(name, avg_ms)},
-        "ceilings": {"hbm_copy_GB/s": round(copy_gbs, 1), "barrett_Gmul/s": round(barrett_gps, 1)},
+                     "valu_issue": valu_issue_from_profiles(name, avg_ms, args.count)},
+        "ceilings": {"barrett_Gmul/s": round(barrett_gps, 1)},
+        "calibration": {"copy_kernel_GB/s": round(copy_gbs, 1),
+                        "note": "a plain 1 GiB int4 copy kernel, run so that rocprofv3 PMC runs of this command can "
+                                "scale FETCH_SIZE by a known read volume; not a bandwidth ceiling"},
         "kernels": per_kernel,
         "two_stream_ms_per_step": round(two_stream_ms, 3),
         "mfma_form": mfma,
