@@ -1100,6 +1100,11 @@ static bool ct_shape_matches(const DevPlan& P, int dir, int shape) {
     return true;
 }
 
+// two consecutive residues, for 16-byte LDS and global accesses
+struct __align__(16) I64x2 {
+    int64_t a, b;
+};
+
 struct RowArgs {
     int64_t d0, d1, d2, d3;
     int64_t st[3];
@@ -1170,6 +1175,31 @@ __global__ __launch_bounds__(512) void k_ntt_rows(RowArgs a, ModTable mt) {
         __syncthreads();
     }
     if constexpr (SHAPE != 0) {
+        // The forward's store. The lines sit in LDS in natural order; when the lines and the
+        // multiplier rows are 16-byte aligned with unit stride, pairs of consecutive elements move
+        // as 16-byte accesses from row addresses formed once per line (the per-element form below
+        // spends more instructions on its 64-bit index arithmetic than on the product).
+        const int64_t* mrow = a.mul ? a.mul + i * a.smul.s[0] + j * a.smul.s[1] + k0 * a.smul.s[2] : nullptr;
+        const bool pairs = (((uintptr_t)base | (uintptr_t)(a.st[2] * 8)) & 15) == 0 &&
+                           (!mrow || ((((uintptr_t)mrow | (uintptr_t)(a.smul.s[2] * 8)) & 15) == 0 && a.smul.s[3] == 1));
+        if (pairs) {
+            const CsdMod m = mt.m[j];
+            for (int r = 0; r < rows; r++) {
+                const I64x2* src = (const I64x2*)(lds + r * lline);
+                I64x2* dst = (I64x2*)(base + r * a.st[2]);
+                const I64x2* mr = mrow ? (const I64x2*)(mrow + r * a.smul.s[2]) : nullptr;
+                for (int q = threadIdx.x; q < L / 2; q += blockDim.x) {
+                    I64x2 x = src[q];
+                    if (mr) {
+                        const I64x2 b = mr[q];
+                        x.a = csd_mont_mul(m, x.a, b.a);
+                        x.b = csd_mont_mul(m, x.b, b.b);
+                    }
+                    dst[q] = x;
+                }
+            }
+            return;
+        }
     } else if (P->nfactors == 0) {
         if (dir != 1) {
             const CsdBarrett m = dir == 2 ? P->inv_len : P->one;
