@@ -1415,6 +1415,17 @@ __global__ __launch_bounds__(256) void k_ntt_cols(ColArgs a, ModTable mt) {
         if constexpr (F1 > 1) col_factor<F1, 1, F0, INV>(v, P, TW, 1, dir);
     }
     const CsdMod m = mt.m[j];
+    if (a.mul && a.smul.s[3] == 0) {
+        // multipliers constant along dim 3 (the ring's L' kernels): the workgroup's L values are
+        // wave-uniform, loaded together before the products instead of one per product
+        const int64_t* __restrict__ mrow = a.mul + i * a.smul.s[0] + j * a.smul.s[1];
+        int64_t mk[L];
+#pragma unroll
+        for (int k = 0; k < L; k++) mk[k] = mrow[k * a.smul.s[2]];
+#pragma unroll
+        for (int k = 0; k < L; k++) base[k * a.st[2]] = csd_mont_mul(m, v[k], mk[k]);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < L; k++) {
         int64_t x = v[k];
