@@ -874,102 +874,162 @@ def be_import(be, host):
 MFMA_INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
 
 
-def bench_zp(pkg, be, steps, nvec=512, seed=SEED, cpu=True):
-    """The dense K-part matrix of the period->CRT transform at field_36864 (the 3072 x 3072
-    group-circulant of the CRT kernel, one per prime of the 17-prime base) applied to `nvec`
-    vectors of 3072 residues per prime, exactly: the int8-digit MFMA contraction, the plain VALU
-    contraction, and the NTT path (ntt_mul + intt along dimension 3) for the same map."""
+def bench_zp(pkg, be, steps, nelem=512, seed=SEED, cpu=True):
+    """C4: the period->CRT map of field_36864 as dense matrices over Z_p, Y = C_L' X C_K^T per
+    prime, applied exactly to `nelem` ring elements of the 17-prime base ([nelem, 17, 12, 3072]):
+    the 3072 x 3072 group-circulant K-part matrix along dimension 3 and the 12 x 12 L'-part matrix
+    along dimension 2, each as the int8-digit MFMA contraction. 113,688,576 Z_p multiply-adds per
+    (element, prime); checked equal mod p to pb_to_crt_tensor of the same elements. The other MFMA
+    kernels and the 64-bit VALU contraction run the K part for comparison."""
     ring = pkg.ring.RingContext(be, FIELD, LOG2_Q, T_BITS)
     base = ring.initial_rns_base()
     R = len(base)
-    n = ring.info["deg_k"]
+    n2, n3 = ring.info["deg_lprime"], ring.info["deg_k"]
     primes = [ring.primes[i] for i in base]
     mods = [be.hal_create_modulus(p) for p in primes]
     t0 = time.perf_counter()
-    a = ring.transform_matrix(base, 3)
-    mat = be.zp_matrix(a, mods)
+    ak = ring.transform_matrix(base, 3)
+    mat_k = be.zp_matrix(ak, mods)
+    al = ring.transform_matrix(base, 2)
+    mat_l = be.zp_matrix(al, mods)
     be.sync()
     setup_s = time.perf_counter() - t0
-    del a
-    x = be.hal_allocate_i64((1, R, nvec, n))
+    del ak, al
+    x = be.hal_allocate_i64((nelem, R, n2, n3))
     ring.fill_uniform(x, base, seed + 7)
-    y_mfma = be.hal_allocate_i64(x.shape)
-    y_plain = be.hal_allocate_i64(x.shape)
+    t = be.hal_allocate_i64(x.shape)
+    y = be.hal_allocate_i64(x.shape)
     stream = be.stream()
     ev = HipEvents()
-    be.zp_apply_dim(mat, y_mfma, x, 3, form="mfma")  # warm-up
+
+    # the L' side is 12-term dot products: the plain form's one-thread-per-vector kernel, not the
+    # matrix cores (which would pad each 12-term sum to 64)
+    def step():
+        be.zp_apply_dim(mat_k, t, x, 3, form="mfma")
+        be.zp_apply_dim(mat_l, y, t, 2, form="plain")
+
+    step()  # warm-up
     be.sync()
+    ev.start(stream)
+    for _ in range(steps):
+        step()
+    mfma_ms = ev.stop(stream) / steps
+    # each side alone, with the kernels' own durations
     be.set_timing(True)
     ev.start(stream)
     for _ in range(steps):
-        be.zp_apply_dim(mat, y_mfma, x, 3, form="mfma")
-    mfma_ms = ev.stop(stream) / steps
+        be.zp_apply_dim(mat_k, t, x, 3, form="mfma")
+    k_ms = ev.stop(stream) / steps
+    kern_k = be.timing_report()
+    be.set_timing(True)
+    ev.start(stream)
+    for _ in range(steps):
+        be.zp_apply_dim(mat_l, y, t, 2, form="plain")
+    l_ms = ev.stop(stream) / steps
     be.set_timing(False)
-    kern = be.timing_report()
-    # the other MFMA kernels: 0 (4 waves loading their operands from L2 directly), 1 (32 x 64 tiles
-    # staged in LDS, one block per wave) and 2 (64 x 64 tiles loaded straight into LDS, 2 x 2 blocks
-    # per wave)
-    ym = be.hal_export(y_mfma)
+    kern_l = be.timing_report()
+    ev.start(stream)
+    be.zp_apply_dim(mat_l, y, t, 2, form="mfma")
+    l_mfma_ms = ev.stop(stream)
+
+    # parity: the first and the last element equal pb_to_crt_tensor's output mod p
+    pv = np.array(primes, dtype=object).reshape(1, -1, 1, 1)
+    for e in (0, nelem - 1):
+        xe = be.hal_allocate_i64((1, R, n2, n3))
+        be.hal_assign(xe, x[e:e + 1])
+        ring.pb_to_crt_tensor(xe, base)
+        want = be.hal_export(xe).reshape(1, R, n2, n3).astype(object) % pv
+        got = be.hal_export(y[e:e + 1]).reshape(1, R, n2, n3).astype(object)
+        if not (got == want).all():
+            raise SystemExit(f"bench: the dense C4 map differs from pb_to_crt mod p (element {e})")
+        del xe
+
+    y0 = be.hal_export(y[0:1]).reshape(1, R, n2, n3).astype(object)  # kept for the CPU check below
+    # the other MFMA kernels on the K part: 0 (4 waves loading their operands from L2), 1 (32 x 64
+    # tiles staged in LDS, one block per wave), 2 (64 x 64 tiles loaded straight into LDS); each
+    # must give the default kernel's bits
+    tk = be.hal_export(t[0:1])
     other_ms = {}
     for form in (0, 1, 2):
         be.zp_set_kernel(form)
         try:
-            be.zp_apply_dim(mat, y_plain, x, 3, form="mfma")
+            be.zp_apply_dim(mat_k, y, x, 3, form="mfma")
             ev.start(stream)
-            for _ in range(steps):
-                be.zp_apply_dim(mat, y_plain, x, 3, form="mfma")
-            other_ms[form] = ev.stop(stream) / steps
+            be.zp_apply_dim(mat_k, y, x, 3, form="mfma")
+            other_ms[form] = ev.stop(stream)
         finally:
             be.zp_set_kernel(3)
-        if not (ym == be.hal_export(y_plain)).all():
+        if not (tk == be.hal_export(y[0:1])).all():
             raise SystemExit(f"bench: MFMA kernel form {form} differs from the default")
-    l2_ms = other_ms[0]
-    plain_steps = max(1, steps // 4)
+    # the 64-bit VALU contraction on the K part of 16 elements, per element
+    sub = 16
     ev.start(stream)
-    for _ in range(plain_steps):
-        be.zp_apply_dim(mat, y_plain, x, 3, form="plain")
-    plain_ms = ev.stop(stream) / plain_steps
-    if not (ym == be.hal_export(y_plain)).all():
-        raise SystemExit("bench: MFMA contraction differs from the plain kernel")
-    # the NTT path for the same map: ntt_mul along dim 3 by the NTT-domain CRT kernel, then intt
-    orders = ring.info["orders"][ring.info["lprime_factor_count"]:]
-    plans = [be.hal_create_ntt_plan(m, orders) for m in mods]
-    kk = np.stack([ring.crt_kernels(i)[1] for i in base]).reshape(1, R, 1, n)
-    kdev = be.hal_allocate_i64(kk.shape)
-    be.hal_import(kdev, kk)
+    be.zp_apply_dim(mat_k, y[0:sub], x[0:sub], 3, form="plain")
+    plain_k_ms = ev.stop(stream) * nelem / sub
+    if not (tk == be.hal_export(y[0:1])).all():
+        raise SystemExit("bench: the plain K-part contraction differs from the MFMA one")
+    # homcrt's cyclic transforms (bootstrap/homcrt.rs:191-214) as J_p-block maps: field_36864's
+    # factors give blocks of 4, 6, 24 and 64 slots; each block a dense J x J circulant with
+    # entries uniform mod p, over the same batch viewed as [elements, primes, 36864 / J, J]
+    rng = np.random.default_rng(seed + 8)
+    homcrt = {}
+    for J in (4, 6, 24, 64):
+        first = np.stack([rng.integers(0, p, size=J, dtype=np.int64) for p in primes]).reshape(1, R, J)
+        idx = (np.arange(J)[:, None] - np.arange(J)[None, :]) % J
+        ah = be.hal_allocate_i64((1, R, J, J))
+        be.hal_import(ah, first[:, :, idx])
+        mat_h = be.zp_matrix(ah, mods)
+        shape = (nelem, R, n2 * n3 // J, J)
+        xv = pkg.hal.DeviceTensor(be, x.ptr, shape, (x.strides[0], x.strides[1], J, 1), 8, x)
+        yv = pkg.hal.DeviceTensor(be, y.ptr, shape, (y.strides[0], y.strides[1], J, 1), 8, y)
+        form = "plain" if J <= 32 else "mfma"
+        be.zp_apply_dim(mat_h, yv, xv, 3, form=form)
+        ev.start(stream)
+        for _ in range(steps):
+            be.zp_apply_dim(mat_h, yv, xv, 3, form=form)
+        hms = ev.stop(stream) / steps
+        ops = 2 * 64 * R * nelem * (n2 * n3 // J) * J * J
+        homcrt[f"J={J}"] = {"ms_per_step": round(hms, 3), "form": form, "int8_ops_equivalent": ops,
+                            "tops_equivalent": round(ops / (hms / 1000.0) / 1e12, 1)}
+        del mat_h, ah
+    # the NTT path of the same map: pb_to_crt_tensor on the whole batch
     xn = be.hal_allocate_i64(x.shape)
     be.hal_assign(xn, x)
-    be.hal_ntt_mul_mont64_dim(xn, plans, 3, kdev, mods)
-    be.hal_intt_mont64_dim(xn, plans, 3)
-    pv = np.array(primes, dtype=np.int64).reshape(-1, 1)
-    if not (np.mod(be.hal_export(xn).reshape(R, -1), pv) == ym.reshape(R, -1)).all():
-        raise SystemExit("bench: MFMA contraction differs from the NTT transform mod p")
+    ring.pb_to_crt_tensor(xn, base)
     ev.start(stream)
     for _ in range(steps):
-        be.hal_ntt_mul_mont64_dim(xn, plans, 3, kdev, mods)
-        be.hal_intt_mont64_dim(xn, plans, 3)
+        ring.pb_to_crt_tensor(xn, base)
     ntt_ms = ev.stop(stream) / steps
-    zp_macs = R * n * n * nvec
-    int8_ops = 2 * 64 * zp_macs
-    mk = kern["zp_mfma"]
+
+    ops_k = 2 * 64 * R * n2 * n3 * n3 * nelem
+    ops_l = 2 * 64 * R * n3 * n2 * n2 * nelem
+    mk = kern_k["zp_mfma"]
     mfma_kernel_ms = mk[1] / mk[0]
-    achieved = int8_ops / (mfma_kernel_ms / 1000.0) / 1e12
+    achieved = ops_k / (mfma_kernel_ms / 1000.0) / 1e12
+    units = R * nelem
     res = {
-        "workload": f"field_36864 dense K-part transform matrix ({n} x {n} group-circulant per prime, {R} primes) "
-                    f"applied to {nvec} vectors per prime, exact over Z_p",
-        "value": round(R * nvec / (mfma_ms / 1000.0), 1), "unit": "vector-limb maps/s", "ms_per_step": round(mfma_ms, 3),
-        "mfma_l2_operands": {"ms_per_step": round(l2_ms, 3), "value": round(R * nvec / (l2_ms / 1000.0), 1)},
-        "mfma_64x64_direct_lds": {"ms_per_step": round(other_ms[2], 3), "value": round(R * nvec / (other_ms[2] / 1000.0), 1)},
-        "mfma_32x64_lds": {"ms_per_step": round(other_ms[1], 3), "value": round(R * nvec / (other_ms[1] / 1000.0), 1)},
-        "plain_valu": {"ms_per_step": round(plain_ms, 3), "value": round(R * nvec / (plain_ms / 1000.0), 1)},
-        "ntt_path": {"ms_per_step": round(ntt_ms, 3), "value": round(R * nvec / (ntt_ms / 1000.0), 1),
-                     "note": "the O(n log n) NTT form of the same map (equal mod p)"},
-        "speedup_vs_plain_valu": round(plain_ms / mfma_ms, 2),
-        "zp_macs_per_s": round(zp_macs / (mfma_ms / 1000.0) / 1e12, 3), "zp_macs_unit": "T Z_p-MAC/s",
-        "roofline": {"bound": "mfma", "kernel": "zp_mfma", "achieved": round(achieved, 1), "peak": MFMA_INT8_PEAK_TOPS,
+        "workload": f"C4: field_36864 period->CRT map as dense matrices, Y = C_L' X C_K^T per prime ({n3} x {n3} "
+                    f"group-circulant and {n2} x {n2}), {nelem} ring elements x {R} primes ([{nelem}, {R}, {n2}, {n3}]), "
+                    "exact over Z_p, int8-digit MFMA",
+        "value": round(units / (mfma_ms / 1000.0), 1), "unit": "element-limb maps/s", "ms_per_step": round(mfma_ms, 3),
+        "int8_ops_per_step": ops_k + ops_l,
+        "tops": round((ops_k + ops_l) / (mfma_ms / 1000.0) / 1e12, 1),
+        "k_part": {"ms_per_step": round(k_ms, 3), "int8_ops": ops_k},
+        "l_part": {"ms_per_step": round(l_ms, 3), "int8_ops": ops_l, "form": "plain (one thread per 12-element vector)",
+                   "mfma_form_ms": round(l_mfma_ms, 3),
+                   "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in kern_l.items()}},
+        "homcrt_blocks": homcrt,
+        "parity": "L1: elements 0 and last equal pb_to_crt_tensor mod p; every MFMA kernel and the VALU form bit-identical",
+        "k_part_other_kernels_ms": {"mfma_l2_operands": round(other_ms[0], 3), "mfma_32x64_lds": round(other_ms[1], 3),
+                                    "mfma_64x64_direct_lds": round(other_ms[2], 3),
+                                    "plain_valu": round(plain_k_ms, 3)},
+        "ntt_path": {"ms_per_step": round(ntt_ms, 3), "value": round(units / (ntt_ms / 1000.0), 1),
+                     "note": "the O(n log n) transform of the same map (pb_to_crt_tensor), equal mod p"},
+        "speedup_vs_plain_valu": round(plain_k_ms / k_ms, 2),
+        "roofline": {"bound": "mfma", "kernel": "zp_mfma (K part)", "achieved": round(achieved, 1), "peak": MFMA_INT8_PEAK_TOPS,
                      "unit": "TOPS int8", "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 4),
-                     "ops_per_launch": int8_ops, "avg_launch_ms": round(mfma_kernel_ms, 4)},
-        "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in kern.items()},
+                     "ops_per_launch": ops_k, "avg_launch_ms": round(mfma_kernel_ms, 4)},
+        "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in kern_k.items()},
         "setup_s": round(setup_s, 2),
     }
     if cpu:
@@ -978,19 +1038,17 @@ def bench_zp(pkg, be, steps, nvec=512, seed=SEED, cpu=True):
 
         cb = cpu_backend.CpuBackend()
         orc = oring.RingContext(cb, oring.FieldDescriptor(FIELD), LOG2_Q, T_BITS)
-        cache = orc.transform_cache(base)
-        sample = 12
-        hx = be.hal_export(x[:, :, 0:sample]).reshape(1, R, sample, n)
+        orc.transform_cache(base)
+        hx = be.hal_export(x[0:1]).reshape(1, R, n2, n3)
         t0 = time.perf_counter()
-        cb.hal_ntt_mul_mont64_dim(hx, cache["ntts_k"], 3, cache["crt_kernel_k"], cache["mods"])
-        cb.hal_intt_mont64_dim(hx, cache["ntts_k"], 3)
+        orc.pb_to_crt_tensor(hx, base)
         dt = time.perf_counter() - t0
-        if not (np.mod(hx.reshape(R, -1), pv) == ym.reshape(R, nvec, n)[:, :sample].reshape(R, -1)).all():
-            raise SystemExit("bench: MFMA contraction differs from the oracle")
-        res["cpu_baseline"] = {"value": round(R * sample / dt, 1), "unit": "vector-limb maps/s", "cores": 1, "kind": "port",
-                               "sample": f"{sample} vectors x {R} primes through the oracle's NTT form ({dt:.2f} s), "
+        if not ((hx.astype(object) % pv) == y0).all():
+            raise SystemExit("bench: the dense C4 map differs from the oracle")
+        res["cpu_baseline"] = {"value": round(R / dt, 1), "unit": "element-limb maps/s", "cores": 1, "kind": "port",
+                               "sample": f"1 element x {R} primes through the oracle's pb_to_crt_tensor ({dt:.2f} s), "
                                          "also checked equal mod p"}
-    del mat, x, y_mfma, y_plain, xn
+    del mat_k, mat_l, x, t, y, xn
     ring.close()
     return res
 
@@ -1152,7 +1210,7 @@ def main():
     if "batch" in chosen:
         workloads["batch_uneven_chain"] = bench_batch(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
     if "zp" in chosen:
-        workloads["dense_zp_mfma"] = bench_zp(pkg, be, max(2, args.steps // 2), cpu=not args.no_cpu)
+        workloads["dense_zp_mfma"] = bench_zp(pkg, be, max(2, args.steps // 4), cpu=not args.no_cpu)
     if "bootstrap" in chosen:
         workloads["bootstrap_step_c5"] = bench_bootstrap_step(pkg, be, cpu=not args.no_cpu)
     if "dbfv" in chosen:

@@ -634,6 +634,64 @@ __global__ __launch_bounds__(256) void k_zp_plain(PlainArgs a, ModTable mt) {
         a.y[l * a.Ly.s_limb + (v / a.Ly.nlo) * a.Ly.s_hi + (v % a.Ly.nlo) * a.Ly.s_lo + i * a.Ly.s_elem] = total;
 }
 
+// Maps of at most 32 x 32 (the L' part's 12 x 12, homcrt's small J_p blocks): one thread per
+// vector, its K centred entries in registers, the limb's matrix staged in LDS. Centred entries are
+// below 2^56 in magnitude, so with a = a1 2^28 + a0 and x = x1 2^28 + x0 (a0, x0 in [0, 2^28)) the
+// four partial sums of 32 x 32-bit products over K <= 32 terms stay below 2^62; the exact sum is
+// assembled once per output and reduced as k_zp_plain reduces it (one chunk), so the results are
+// its bits.
+#define ZP_SMALL_MAX 32
+
+template <int K>
+__global__ __launch_bounds__(256) void k_zp_small(PlainArgs a, ModTable mt) {
+    __shared__ int32_t sa[ZP_SMALL_MAX * ZP_SMALL_MAX][2];  // (a0, a1) of entry [i][k]
+    const int64_t chunks = (a.Lx.nvec + 255) / 256;
+    const int64_t l = blockIdx.x / chunks;
+    const int64_t v = (blockIdx.x % chunks) * 256 + threadIdx.x;
+    const CsdMod md = mt.m[l];
+    const int64_t p = md.N;
+    const int m = (int)a.m;
+    for (int e = threadIdx.x; e < m * K; e += 256) {
+        const int64_t c = zp_centre(a.a[l * a.m * K + e], p);
+        sa[e][0] = (int32_t)(c & ((1 << 28) - 1));
+        sa[e][1] = (int32_t)(c >> 28);
+    }
+    __syncthreads();
+    if (v >= a.Lx.nvec) return;
+    const int64_t* __restrict__ xv = a.x + l * a.Lx.s_limb + (v / a.Lx.nlo) * a.Lx.s_hi + (v % a.Lx.nlo) * a.Lx.s_lo;
+    int64_t* __restrict__ yv = a.y + l * a.Ly.s_limb + (v / a.Ly.nlo) * a.Ly.s_hi + (v % a.Ly.nlo) * a.Ly.s_lo;
+    int32_t x0[K], x1[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int64_t c = zp_centre(xv[k * a.Lx.s_elem], p);
+        x0[k] = (int32_t)(c & ((1 << 28) - 1));
+        x1[k] = (int32_t)(c >> 28);
+    }
+    for (int i = 0; i < m; i++) {
+        int64_t s00 = 0, s01 = 0, s10 = 0, s11 = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int32_t a0 = sa[i * K + k][0], a1 = sa[i * K + k][1];
+            s00 += (int64_t)a0 * x0[k];
+            s01 += (int64_t)a0 * x1[k];
+            s10 += (int64_t)a1 * x0[k];
+            s11 += (int64_t)a1 * x1[k];
+        }
+        const i128 sum = (i128)s00 + ((i128)(s01 + s10) << 28) + ((i128)s11 << 56);
+        yv[i * a.Ly.s_elem] = zp_reduce(md, sum);
+    }
+}
+
+typedef void (*ZpSmallKernel)(PlainArgs, ModTable);
+template <int K>
+static ZpSmallKernel zp_small_for(int k) {
+    if constexpr (K > ZP_SMALL_MAX) {
+        return nullptr;
+    } else {
+        return k == K ? k_zp_small<K> : zp_small_for<K + 1>(k);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // group-circulant matrices: out[l][a][b] = kern[l][a - b], the difference taken in the group
 // C_o0 x ... x C_o(f-1) with mixed-radix (row-major) indices
@@ -832,6 +890,23 @@ extern "C" int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* z, const csd_
     ModTable mt = make_mod_table(z->mods.data(), z->mods.size(), z->r);
     const int64_t nvec = Lx.nvec;
     if (nvec == 0) return 0;
+    if (form == 1 && z->m <= ZP_SMALL_MAX && z->k <= ZP_SMALL_MAX) {
+        PlainArgs a;
+        memset(&a, 0, sizeof(a));
+        a.a = z->values;
+        a.x = (const int64_t*)src->ptr;
+        a.y = (int64_t*)dst->ptr;
+        a.Lx = Lx;
+        a.Ly = Ly;
+        a.m = z->m;
+        a.k = z->k;
+        const int64_t blocks = z->r * ((nvec + 255) / 256);
+        if (blocks > (int64_t)INT32_MAX) throw CsdError("zp apply: too many tiles");
+        TimedScope ts(ctx, "zp_small", 8.0 * z->r * (z->m * z->k + nvec * (z->k + z->m)));
+        hipLaunchKernelGGL(zp_small_for<1>((int)z->k), dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, mt);
+        HIPCHECK(hipGetLastError());
+        return 0;
+    }
     if (form == 1) {
         PlainArgs a;
         a.a = z->values;

@@ -81,7 +81,10 @@ def apply(gpu, mat, dst, src, dim, form, max_chunk=0):
 
 @pytest.mark.parametrize("m,k,nvec,dim,mode", [
     (20, 70, 33, 3, "uniform"), (12, 12, 5, 2, "uniform"), (64, 128, 64, 3, "wide"), (3, 1, 2, 3, "extreme"),
-    (33, 200, 65, 2, "extreme"), (16, 64, 16, 3, "max"), (100, 37, 7, 3, "wide")])
+    (33, 200, 65, 2, "extreme"), (16, 64, 16, 3, "max"), (100, 37, 7, 3, "wide"),
+    # maps of at most 32 x 32 take the one-thread-per-vector plain kernel: its largest size and
+    # non-square ones
+    (32, 32, 300, 2, "extreme"), (5, 9, 41, 3, "max"), (24, 17, 70, 3, "extreme")])
 @pytest.mark.parametrize("form", ["mfma", "mfma_l2", "mfma_wide", "mfma_lds32", "plain"])
 def test_zp_exact_small(gpu, m, k, nvec, dim, mode, form):
     rng = np.random.default_rng(m * 1000 + k + nvec)
