@@ -890,6 +890,9 @@ def bench_zp(pkg, be, steps, nelem=512, seed=SEED, cpu=True):
     t0 = time.perf_counter()
     ak = ring.transform_matrix(base, 3)
     mat_k = be.zp_matrix(ak, mods)
+    # the K-part matrix is the group-circulant of the CRT kernel over C_2 x C_3 x C_8 x C_64: the
+    # default kernel then builds its A operands from the kernel's digit planes (checked on the device)
+    mat_k.set_circulant(ring.info["orders"][ring.info["lprime_factor_count"]:])
     al = ring.transform_matrix(base, 2)
     mat_l = be.zp_matrix(al, mods)
     be.sync()
@@ -946,11 +949,12 @@ def bench_zp(pkg, be, steps, nelem=512, seed=SEED, cpu=True):
 
     y0 = be.hal_export(y[0:1]).reshape(1, R, n2, n3).astype(object)  # kept for the CPU check below
     # the other MFMA kernels on the K part: 0 (4 waves loading their operands from L2), 1 (32 x 64
-    # tiles staged in LDS, one block per wave), 2 (64 x 64 tiles loaded straight into LDS); each
-    # must give the default kernel's bits
+    # tiles staged in LDS, one block per wave), 2 (64 x 64 tiles loaded straight into LDS), 4 (the
+    # default's 64 x 64 tiles with the matrix's own digit planes staged); each must give the
+    # default kernel's bits
     tk = be.hal_export(t[0:1])
     other_ms = {}
-    for form in (0, 1, 2):
+    for form in (0, 1, 2, 4):
         be.zp_set_kernel(form)
         try:
             be.zp_apply_dim(mat_k, y, x, 3, form="mfma")
@@ -1022,11 +1026,12 @@ def bench_zp(pkg, be, steps, nelem=512, seed=SEED, cpu=True):
         "parity": "L1: elements 0 and last equal pb_to_crt_tensor mod p; every MFMA kernel and the VALU form bit-identical",
         "k_part_other_kernels_ms": {"mfma_l2_operands": round(other_ms[0], 3), "mfma_32x64_lds": round(other_ms[1], 3),
                                     "mfma_64x64_direct_lds": round(other_ms[2], 3),
+                                    "mfma_64x64_dense_planes": round(other_ms[4], 3),
                                     "plain_valu": round(plain_k_ms, 3)},
         "ntt_path": {"ms_per_step": round(ntt_ms, 3), "value": round(units / (ntt_ms / 1000.0), 1),
                      "note": "the O(n log n) transform of the same map (pb_to_crt_tensor), equal mod p"},
         "speedup_vs_plain_valu": round(plain_k_ms / k_ms, 2),
-        "roofline": {"bound": "mfma", "kernel": "zp_mfma (K part)", "achieved": round(achieved, 1), "peak": MFMA_INT8_PEAK_TOPS,
+        "roofline": {"bound": "mfma", "kernel": "zp_mfma (K part, group-circulant form)", "achieved": round(achieved, 1), "peak": MFMA_INT8_PEAK_TOPS,
                      "unit": "TOPS int8", "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 4),
                      "ops_per_launch": ops_k, "avg_launch_ms": round(mfma_kernel_ms, 4)},
         "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in kern_k.items()},

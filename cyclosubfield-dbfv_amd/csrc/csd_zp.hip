@@ -40,6 +40,10 @@ struct csd_zp_matrix {
     int8_t* planes;   // [r][8][mpad][kpad] digit planes
     int64_t* values;  // [r][m][k] the entries as given
     std::vector<csd_modulus> mods;
+    // csd_zp_matrix_set_circulant: the generator's digit planes [r][8][k] and the leading factors
+    int8_t* gen = nullptr;
+    int ntop = 0;
+    int32_t top_orders[7] = {0}, top_strides[7] = {0};
 };
 
 // the vectors of a view along `dim`: vector v = (hi, lo) = (v / nlo, v % nlo), element k
@@ -451,6 +455,152 @@ __global__ __launch_bounds__(512) void k_zp_mfma_lds64(MfmaArgs a, ModTable mt) 
     }
 }
 
+// The group-circulant form of k_zp_mfma_lds64. For C[a][b] = g[a - b] (difference in
+// C_o0 x ... x C_o(f-1), mixed-radix indices, last order 64), the 64 x 64 tile (row block A,
+// column block B) is the circulant of the 64 generator entries g[S 64 .. S 64 + 63], S = A - B in
+// the group of the leading factors: C[64 A + i][64 B + j] = g[S 64 + (i - j) mod 64]. A K-step
+// stages the tile's X planes as k_zp_mfma_lds64 does, but for A only those 64 entries' 8 digit
+// planes, reversed and written twice (h[t] = g[S 64 + (63 - t) mod 64], t < 128, 1 KiB). Lane
+// (x16, k-group q) of row i then needs the bytes h[start .. start + 15], start = (63 - i + 16 q)
+// mod 64: five aligned dword reads and four byte-aligns. The operand bytes are those of the dense
+// digit planes, so the results are k_zp_mfma_lds64's bits.
+
+struct CircMfmaArgs {
+    MfmaArgs m;
+    const int8_t* gen;  // [r][8][n] digit planes of the generator (column 0 of the matrix)
+    int ntop;           // leading factors (all but the last, which is 64)
+    int32_t top_orders[7], top_strides[7];  // strides in units of 64-blocks
+};
+
+static constexpr int ZPC_H_BYTES = ZP_DIGITS * 128;  // one K-step's reversed, doubled generator planes
+
+__device__ __forceinline__ int32_t zp_block_sub(const CircMfmaArgs& c, int32_t A, int32_t B) {
+    int32_t s = 0;
+    for (int f = 0; f < c.ntop; f++) {
+        const int32_t o = c.top_orders[f], st = c.top_strides[f];
+        const int32_t da = (A / st) % o, db = (B / st) % o;
+        s += ((da - db + o) % o) * st;
+    }
+    return s;
+}
+
+__global__ __launch_bounds__(512) void k_zp_mfma_circ(CircMfmaArgs c, ModTable mt) {
+    extern __shared__ __align__(16) int8_t zs[];  // [2][X tile | H planes]
+    const MfmaArgs& a = c.m;
+    int64_t b = blockIdx.x;
+    if (a.nblocks % 8 == 0) b = (b % 8) * (a.nblocks / 8) + b / 8;
+    const int64_t nt = b % a.ntiles;
+    const int64_t mtile = (b / a.ntiles) % a.mtiles;
+    const int64_t l = b / (a.ntiles * a.mtiles);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x16 = lane & 15, g = lane >> 4;
+    const int rh = wave & 1, cb = wave >> 1;
+    const int64_t px = a.npad * a.kpad;
+    const int8_t* Xg = a.x + (l * ZP_DIGITS * a.npad + nt * 64) * a.kpad;
+    const int8_t* Gg = c.gen + l * ZP_DIGITS * a.kpad;
+    constexpr int BUF = ZP8_X_BYTES + ZPC_H_BYTES;
+    // staging: the X tile's 2048 16-byte chunks (4 per thread, as in k_zp_mfma_lds64) and, from
+    // threads 0-127, one dword of h each (plane tid / 16, bytes 4 (tid % 16) .. + 3, both copies)
+    auto stage_load = [&](v4i (&rx)[4], uint32_t& hw, int64_t k0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int cc = tid + q * ZP8_THREADS;
+            const int pl = cc >> 8, row = (cc >> 2) & 63, kg = cc & 3;
+            rx[q] = *(const v4i*)(Xg + pl * px + row * a.kpad + k0 + kg * 16);
+        }
+        if (tid < 128) {
+            const int d = tid >> 4, q = tid & 15;
+            const int32_t S = zp_block_sub(c, (int32_t)mtile, (int32_t)(k0 >> 6));
+            const uint32_t w = *(const uint32_t*)(Gg + d * a.kpad + (int64_t)S * 64 + 60 - 4 * q);
+            hw = __builtin_amdgcn_perm(w, w, 0x00010203u);  // bytes reversed: g[63 - 4q] first
+        }
+    };
+    auto stage_store = [&](const v4i (&rx)[4], uint32_t hw, int buf) {
+        int8_t* base = zs + buf * BUF;
+#pragma unroll
+        for (int q = 0; q < 4; q++) *(v4i*)(base + (tid + q * ZP8_THREADS) * 16) = rx[q];
+        if (tid < 128) {
+            const int d = tid >> 4, q = tid & 15;
+            uint32_t* h = (uint32_t*)(base + ZP8_X_BYTES + d * 128);
+            h[q] = hw;
+            h[16 + q] = hw;
+        }
+    };
+    const CsdMod md = mt.m[l];
+    int64_t total[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    for (int64_t kc = 0; kc < a.kpad; kc += a.chunk) {
+        const int64_t ke = min(a.kpad, kc + a.chunk);
+        v4i acc[2][2 * ZP_DIGITS - 1];
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int s2 = 0; s2 < 2 * ZP_DIGITS - 1; s2++) acc[u][s2] = (v4i){0, 0, 0, 0};
+        v4i rx[4];
+        uint32_t hw = 0;
+        stage_load(rx, hw, kc);
+        stage_store(rx, hw, 0);
+        __syncthreads();
+        int buf = 0;
+        for (int64_t k0 = kc; k0 < ke; k0 += 64) {
+            const bool more = k0 + 64 < ke;
+            if (more) stage_load(rx, hw, k0 + 64);
+            const int8_t* base = zs + buf * BUF;
+            v4i xv[ZP_DIGITS];
+#pragma unroll
+            for (int d = 0; d < ZP_DIGITS; d++)
+                xv[d] = *(const v4i*)(base + ((d * 64 + cb * 16 + x16) * 4 + g) * 16);
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int row = rh * 32 + u * 16 + x16;
+                const int start = (63 - row + 16 * g) & 63;
+                const int w0 = start >> 2;
+                const uint32_t sh = (uint32_t)(start & 3);
+                v4i av[ZP_DIGITS];
+#pragma unroll
+                for (int d = 0; d < ZP_DIGITS; d++) {
+                    const uint32_t* h = (const uint32_t*)(base + ZP8_X_BYTES + d * 128) + w0;
+                    uint32_t hv[5];
+#pragma unroll
+                    for (int e = 0; e < 5; e++) hv[e] = h[e];
+                    v4i o;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) o[e] = (int)__builtin_amdgcn_alignbyte(hv[e + 1], hv[e], sh);
+                    av[d] = o;
+                }
+#pragma unroll
+                for (int da = 0; da < ZP_DIGITS; da++)
+#pragma unroll
+                    for (int dx = 0; dx < ZP_DIGITS; dx++)
+                        acc[u][da + dx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[da], xv[dx], acc[u][da + dx], 0, 0, 0);
+            }
+            if (more) stage_store(rx, hw, buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                u128 v = 0;
+#pragma unroll
+                for (int s2 = 0; s2 < 2 * ZP_DIGITS - 1; s2++) v += (u128)(i128)acc[u][s2][r] << (8 * s2);
+                int64_t t = total[u][r] + zp_reduce(md, (i128)v);
+                total[u][r] = t >= md.N ? t - md.N : t;
+            }
+    }
+    const int64_t v = nt * 64 + cb * 16 + x16;
+    if (v < a.Ly.nvec) {
+        int64_t* out = a.y + l * a.Ly.s_limb + (v / a.Ly.nlo) * a.Ly.s_hi + (v % a.Ly.nlo) * a.Ly.s_lo;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t i = mtile * 64 + rh * 32 + u * 16 + 4 * g + r;
+                if (i < a.m) out[i * a.Ly.s_elem] = total[u][r];
+            }
+    }
+}
+
 // The 64 x 64 form: 4 waves share one 64-row x 64-vector tile, and each wave keeps a 2 x 2
 // arrangement of 16 x 16 output blocks (60 accumulators, held in AGPRs). A K-step's operands of a
 // wave are 2 x 8 planes of A rows and 2 x 8 planes of X vectors, 32 LDS reads for 256 MFMAs:
@@ -576,7 +726,7 @@ __global__ __launch_bounds__(256) void k_zp_mfma_wide(MfmaArgs a, ModTable mt) {
 // waves, 3 (default) LDS 64 x 64 on 8 waves
 extern "C" int csd_zp_set_kernel(csd_ctx* ctx, int form) {
     CSD_TRY if (!ctx) throw CsdError("null context");
-    if (form < 0 || form > 3) throw CsdError("csd_zp_set_kernel: form must be 0, 1, 2 or 3");
+    if (form < 0 || form > 4) throw CsdError("csd_zp_set_kernel: form must be 0, 1, 2, 3 or 4");
     ctx->zp_kernel = form;
     CSD_CATCH
 }
@@ -786,7 +936,97 @@ extern "C" int csd_zp_matrix_free(csd_zp_matrix* z) {
     HIPCHECK(hipStreamSynchronize(z->ctx->stream));
     z->ctx->free_bytes(z->planes);
     z->ctx->free_bytes(z->values);
+    if (z->gen) z->ctx->free_bytes(z->gen);
     delete z;
+    CSD_CATCH
+}
+
+// ---------------------------------------------------------------------------------------------
+// group-circulant structure of a square matrix (csd_zp_matrix_set_circulant)
+
+struct CircCheckArgs {
+    const int64_t* values;  // [r][n][n]
+    int64_t n, r;
+    int nf;
+    int64_t orders[8], strides[8];
+    int64_t* bad;           // set to 1 (vector store) where an entry differs from g[a - b]
+};
+
+// every entry against column 0 at the group difference of its indices
+__global__ __launch_bounds__(256) void k_zp_check_circulant(CircCheckArgs c) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.r * c.n * c.n) return;
+    const int64_t bcol = t % c.n, arow = (t / c.n) % c.n, l = t / (c.n * c.n);
+    int64_t idx = 0;
+    for (int f = 0; f < c.nf; f++) {
+        const int64_t da = (arow / c.strides[f]) % c.orders[f], db = (bcol / c.strides[f]) % c.orders[f];
+        idx += ((da - db + c.orders[f]) % c.orders[f]) * c.strides[f];
+    }
+    if (c.values[t] != c.values[(l * c.n + idx) * c.n]) c.bad[0] = 1;
+}
+
+// the generator (column 0) as 8 balanced digit planes [r][8][n], k_zp_digits' digits
+__global__ __launch_bounds__(256) void k_zp_gen_digits(const int64_t* values, int64_t n, int64_t r, int8_t* gen, ModTable mt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= r * n) return;
+    const int64_t a = t % n, l = t / n;
+    int64_t c = zp_centre(values[(l * n + a) * n], mt.m[l].N);
+#pragma unroll
+    for (int d = 0; d < ZP_DIGITS; d++) {
+        int32_t lo = (int32_t)(c & 255);
+        lo = lo >= 128 ? lo - 256 : lo;
+        c = (c - lo) >> 8;
+        gen[(l * ZP_DIGITS + d) * n + a] = (int8_t)lo;
+    }
+}
+
+extern "C" int csd_zp_matrix_set_circulant(csd_zp_matrix* z, const size_t* orders, size_t norders) {
+    CSD_TRY if (!z) throw CsdError("null zp matrix");
+    if (norders < 1 || norders > 8) throw CsdError("zp circulant: 1 to 8 cyclic factors");
+    int64_t n = 1;
+    for (size_t f = 0; f < norders; f++) n *= (int64_t)orders[f];
+    if (z->m != n || z->k != n) throw CsdError("zp circulant: the matrix must be n x n, n the group order");
+    if (orders[norders - 1] != 64)
+        throw CsdError("zp circulant: the last cyclic factor must have order 64 (the 64 x 64 tiles are its circulants)");
+    csd_ctx* ctx = z->ctx;
+    CircCheckArgs c;
+    memset(&c, 0, sizeof(c));
+    c.values = z->values;
+    c.n = n;
+    c.r = z->r;
+    c.nf = (int)norders;
+    int64_t st = 1;
+    for (size_t f = norders; f-- > 0;) {
+        c.orders[f] = (int64_t)orders[f];
+        c.strides[f] = st;
+        st *= (int64_t)orders[f];
+    }
+    int64_t* bad = (int64_t*)ctx->alloc_bytes(sizeof(int64_t));  // csd_zero_fill clears whole words
+    int64_t h_bad = 0;
+    try {
+        csd_zero_fill(ctx, bad, sizeof(int64_t));
+        c.bad = bad;
+        const int64_t total = z->r * n * n;
+        hipLaunchKernelGGL(k_zp_check_circulant, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, c);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(&h_bad, bad, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        ctx->free_bytes(bad);
+        throw;
+    }
+    ctx->free_bytes(bad);
+    if (h_bad) throw CsdError("zp circulant: the matrix is not the group-circulant of its first column");
+    if (!z->gen) z->gen = (int8_t*)ctx->alloc_bytes((size_t)z->r * ZP_DIGITS * n);
+    ModTable mt = make_mod_table(z->mods.data(), z->mods.size(), z->r);
+    hipLaunchKernelGGL(k_zp_gen_digits, dim3((unsigned)((z->r * n + 255) / 256)), dim3(256), 0, ctx->stream, z->values, n, z->r,
+                       z->gen, mt);
+    HIPCHECK(hipGetLastError());
+    z->ntop = (int)norders - 1;
+    for (int f = 0; f < z->ntop; f++) {
+        z->top_orders[f] = (int32_t)orders[f];
+        z->top_strides[f] = (int32_t)(c.strides[f] / 64);
+    }
     CSD_CATCH
 }
 
@@ -852,7 +1092,23 @@ static void zp_mfma_run(csd_ctx* ctx, const csd_zp_matrix* z, const csd_view* sr
     TimedScope ts(ctx, "zp_mfma",
                   (double)z->r * ZP_DIGITS * (z->mpad * z->kpad + npad * z->kpad) + 8.0 * z->r * nvec * z->m);
     const int zk = ctx->zp_kernel;
-    if (zk == 3) {
+    if (zk == 3 && z->gen && z->mpad == z->kpad && z->m == z->k) {
+        // the group-circulant form: the A operands from the generator's planes
+        CircMfmaArgs c;
+        memset(&c, 0, sizeof(c));
+        c.m = a;
+        c.m.mtiles = z->mpad / 64;
+        c.m.nblocks = z->r * c.m.mtiles * c.m.ntiles;
+        c.gen = z->gen;
+        c.ntop = z->ntop;
+        for (int f = 0; f < z->ntop; f++) {
+            c.top_orders[f] = z->top_orders[f];
+            c.top_strides[f] = z->top_strides[f];
+        }
+        const size_t lds = 2 * (size_t)(ZP8_X_BYTES + ZPC_H_BYTES);
+        csd_allow_lds((const void*)k_zp_mfma_circ, (int)lds);
+        hipLaunchKernelGGL(k_zp_mfma_circ, dim3((unsigned)c.m.nblocks), dim3(ZP8_THREADS), lds, ctx->stream, c, mt);
+    } else if (zk == 3 || zk == 4) {
         a.mtiles = z->mpad / 64;  // mpad is a multiple of 64
         a.nblocks = z->r * a.mtiles * a.ntiles;
         const size_t lds = 2 * (size_t)(ZP8_A_BYTES + ZP8_X_BYTES);

@@ -202,6 +202,14 @@ class ZpMatrix:
             _lib.csd_zp_matrix_free(ctypes.c_void_p(self.handle))
             self.handle = None
 
+    def set_circulant(self, orders):
+        """Declares the matrices group-circulant over C_o0 x ... x C_o(f-1) (last order 64):
+        entry [a][b] = entry [a - b][0]. Checked on the device (an error if any entry differs);
+        the default MFMA kernel then builds its A operands from the first column's digit planes
+        (csd_zp_matrix_set_circulant). Results are unchanged."""
+        arr = (ctypes.c_size_t * len(orders))(*[int(o) for o in orders])
+        check(lib().csd_zp_matrix_set_circulant(ctypes.c_void_p(self.handle), arr, ctypes.c_size_t(len(orders))))
+
 
 def _row_major_strides(shape):
     st = [1, 1, 1, 1]
@@ -520,9 +528,10 @@ class HipBackend:
 
     def zp_set_kernel(self, form):
         """which int8 MFMA kernel this context's "mfma" form runs (csd_zp_set_kernel): 3 (the
-        default) LDS-staged 64 x 64 tiles for 8 waves with 2 blocks each, 1 LDS-staged 32 x 64 tiles
-        for 8 waves, 2 direct-to-LDS 64 x 64 tiles for 4 waves with 2 x 2 blocks each, 0 operands
-        loaded by 4 waves straight from L2"""
+        default) LDS-staged 64 x 64 tiles for 8 waves with 2 blocks each, the A operands built
+        from the generator when the matrix is declared group-circulant; 4 the same without that;
+        1 LDS-staged 32 x 64 tiles for 8 waves, 2 direct-to-LDS 64 x 64 tiles for 4 waves with
+        2 x 2 blocks each, 0 operands loaded by 4 waves straight from L2"""
         check(lib().csd_zp_set_kernel(self.ctx, ctypes.c_int(int(form))))
 
     def hal_create_permutation(self, inner_shape, shifts):

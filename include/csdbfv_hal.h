@@ -174,11 +174,20 @@ int csd_zp_matrix_free(csd_zp_matrix* m);
 int csd_zp_apply_dim(csd_ctx* ctx, const csd_zp_matrix* a, const csd_view* dst, const csd_view* src, int dim, int form,
                      size_t max_chunk);
 /* which int8 MFMA kernel form 0 uses (per context): 3 (default) stages each K-step's digit planes
- * of a 64 x 64 tile in LDS for 8 waves with two 16 x 16 output blocks each; 1 stages a 32 x 64
- * tile for 8 waves with one block each; 2 loads them straight into LDS (global_load_lds) for a
- * 64 x 64 tile of 4 waves with 2 x 2 blocks each; 0 has 4 waves load their operands from L2
- * directly, two blocks each. All compute identical results. */
+ * of a 64 x 64 tile in LDS for 8 waves with two 16 x 16 output blocks each, taking the A operands
+ * from the generator when the matrix was declared group-circulant (csd_zp_matrix_set_circulant);
+ * 4 is form 3 without the circulant path; 1 stages a 32 x 64 tile for 8 waves with one block
+ * each; 2 loads them straight into LDS (global_load_lds) for a 64 x 64 tile of 4 waves with 2 x 2
+ * blocks each; 0 has 4 waves load their operands from L2 directly, two blocks each. All compute
+ * identical results. */
 int csd_zp_set_kernel(csd_ctx* ctx, int form);
+/* Declares m's matrices group-circulant over C_o0 x ... x C_o(f-1), mixed-radix (row-major)
+ * indices, last order 64: A[a][b] = A[a - b][0], the difference taken in the group (the period->
+ * CRT matrices of the K part, ring.rs:1118-1141, are such matrices of the CRT kernel). The
+ * structure is checked on the device (an error, and m unchanged, if any entry differs); the
+ * default MFMA kernel then builds each 64 x 64 tile's A operands from the first column's digit
+ * planes instead of streaming the matrix's. Results are unchanged (no reference counterpart). */
+int csd_zp_matrix_set_circulant(csd_zp_matrix* m, const size_t* orders, size_t norders);
 /* csd_zp_apply_dim's int8-MFMA form (form 0), allocating nothing. No counterpart in the reference:
  * it is for callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc). The source vectors' digit planes go to a

@@ -62,7 +62,7 @@ def run(gpu, a, x, primes, dim, form, max_chunk=0):
     return from_dev(gpu, dst)
 
 
-KERNELS = {"mfma_l2": 0, "mfma_lds32": 1, "mfma_wide": 2}
+KERNELS = {"mfma_l2": 0, "mfma_lds32": 1, "mfma_wide": 2, "mfma_dense64": 4}
 
 
 def apply(gpu, mat, dst, src, dim, form, max_chunk=0):
@@ -236,3 +236,59 @@ def test_homcrt_block_contraction(gpu):
     want = exact(a, x, primes, 3)
     for form in ("mfma", "mfma_l2", "mfma_wide", "mfma_lds32", "plain"):
         assert (run(gpu, a, x, primes, 3, form) == want).all()
+
+
+def _group_circulant(first, orders):
+    """[1, r, n, n] with entry [a][b] = first[:, a - b], the difference in C_o0 x ... (row-major)"""
+    n = int(np.prod(orders))
+    strides = [int(np.prod(orders[f + 1:])) for f in range(len(orders))]
+    idx = np.zeros((n, n), dtype=np.int64)
+    ar = np.arange(n)[:, None]
+    bc = np.arange(n)[None, :]
+    for o, st in zip(orders, strides):
+        idx += (((ar // st) % o - (bc // st) % o) % o) * st
+    return first[:, :, idx]
+
+
+@pytest.mark.parametrize("orders,nvec,mode", [((64,), 70, "uniform"), ((2, 64), 129, "extreme"), ((3, 2, 64), 64, "max"),
+                                              ((2, 3, 64), 200, "uniform")])
+def test_circulant_form_matches_dense(gpu, orders, nvec, mode):
+    """the group-circulant MFMA form (A operands from the generator) gives the dense kernels' bits
+    and the exact sums"""
+    rng = np.random.default_rng(sum(orders) + nvec)
+    primes = P57[:2] + SMALL[:1]
+    n = int(np.prod(orders))
+    first = entries(rng, primes, (1, len(primes), n, 1), mode)[..., 0]
+    a = _group_circulant(first, list(orders))
+    x = entries(rng, primes, (1, len(primes), nvec, n), mode)
+    mods = [gpu.hal_create_modulus(p) for p in primes]
+    zm = gpu.zp_matrix(to_dev(gpu, a), mods)
+    want = exact(a, x, primes, 3)
+    dense = gpu.hal_allocate_i64(x.shape)
+    apply(gpu, zm, dense, to_dev(gpu, x), 3, "mfma_dense64")
+    zm.set_circulant(list(orders))
+    circ = gpu.hal_allocate_i64(x.shape)
+    gpu.zp_apply_dim(zm, circ, to_dev(gpu, x), 3)
+    assert (from_dev(gpu, dense) == want).all()
+    assert (from_dev(gpu, circ) == want).all()
+    # chunked K (two chunks of 64 terms for n = 128) takes the same path
+    if n >= 128:
+        gpu.zp_apply_dim(zm, circ, to_dev(gpu, x), 3, max_chunk=64)
+        assert (from_dev(gpu, circ) == want).all()
+
+
+def test_circulant_declaration_is_checked(gpu):
+    """a matrix that is not the group-circulant of its first column, or of the wrong shape, is refused"""
+    rng = np.random.default_rng(3)
+    primes = P57[:1]
+    first = entries(rng, primes, (1, 1, 128, 1), "uniform")[..., 0]
+    a = _group_circulant(first, [2, 64])
+    a[0, 0, 5, 77] = (a[0, 0, 5, 77] + 1) % primes[0]
+    mods = [gpu.hal_create_modulus(p) for p in primes]
+    zm = gpu.zp_matrix(to_dev(gpu, a), mods)
+    with pytest.raises(load().HalError):
+        zm.set_circulant([2, 64])
+    with pytest.raises(load().HalError):
+        zm.set_circulant([4, 32])
+    with pytest.raises(load().HalError):
+        zm.set_circulant([64])
