@@ -473,6 +473,9 @@ struct CircMfmaArgs {
 };
 
 static constexpr int ZPC_H_BYTES = ZP_DIGITS * 128;  // one K-step's reversed, doubled generator planes
+#ifndef ZPC_NCB
+#define ZPC_NCB 4  // 16-vector blocks per workgroup of the circulant form (2 measured the same)
+#endif
 
 __device__ __forceinline__ int32_t zp_block_sub(const CircMfmaArgs& c, int32_t A, int32_t B) {
     int32_t s = 0;
@@ -484,7 +487,11 @@ __device__ __forceinline__ int32_t zp_block_sub(const CircMfmaArgs& c, int32_t A
     return s;
 }
 
-__global__ __launch_bounds__(512) void k_zp_mfma_circ(CircMfmaArgs c, ModTable mt) {
+// NCB 16-vector blocks per workgroup (2 NCB waves, a 64-row x 16 NCB-vector tile)
+template <int NCB>
+__global__ __launch_bounds__(128 * NCB, 2) void k_zp_mfma_circ(CircMfmaArgs c, ModTable mt) {
+    constexpr int THREADS = 128 * NCB, TW = 16 * NCB;  // threads, tile width in vectors
+    constexpr int XB = ZP_DIGITS * TW * 64;             // X tile bytes per K-step
     extern __shared__ __align__(16) int8_t zs[];  // [2][X tile | H planes]
     const MfmaArgs& a = c.m;
     int64_t b = blockIdx.x;
@@ -496,16 +503,17 @@ __global__ __launch_bounds__(512) void k_zp_mfma_circ(CircMfmaArgs c, ModTable m
     const int x16 = lane & 15, g = lane >> 4;
     const int rh = wave & 1, cb = wave >> 1;
     const int64_t px = a.npad * a.kpad;
-    const int8_t* Xg = a.x + (l * ZP_DIGITS * a.npad + nt * 64) * a.kpad;
+    const int8_t* Xg = a.x + (l * ZP_DIGITS * a.npad + nt * TW) * a.kpad;
     const int8_t* Gg = c.gen + l * ZP_DIGITS * a.kpad;
-    constexpr int BUF = ZP8_X_BYTES + ZPC_H_BYTES;
-    // staging: the X tile's 2048 16-byte chunks (4 per thread, as in k_zp_mfma_lds64) and, from
-    // threads 0-127, one dword of h each (plane tid / 16, bytes 4 (tid % 16) .. + 3, both copies)
+    constexpr int BUF = XB + ZPC_H_BYTES;
+    // staging: the X tile's 16-byte chunks (4 per thread; chunk cc is plane cc / (4 TW), vector
+    // (cc / 4) % TW, k-group cc % 4) and, from threads 0-127, one dword of h each (plane tid / 16,
+    // bytes 4 (tid % 16) .. + 3, both copies)
     auto stage_load = [&](v4i (&rx)[4], uint32_t& hw, int64_t k0) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int cc = tid + q * ZP8_THREADS;
-            const int pl = cc >> 8, row = (cc >> 2) & 63, kg = cc & 3;
+            const int cc = tid + q * THREADS;
+            const int pl = cc / (4 * TW), row = (cc >> 2) % TW, kg = cc & 3;
             rx[q] = *(const v4i*)(Xg + pl * px + row * a.kpad + k0 + kg * 16);
         }
         if (tid < 128) {
@@ -518,10 +526,10 @@ __global__ __launch_bounds__(512) void k_zp_mfma_circ(CircMfmaArgs c, ModTable m
     auto stage_store = [&](const v4i (&rx)[4], uint32_t hw, int buf) {
         int8_t* base = zs + buf * BUF;
 #pragma unroll
-        for (int q = 0; q < 4; q++) *(v4i*)(base + (tid + q * ZP8_THREADS) * 16) = rx[q];
+        for (int q = 0; q < 4; q++) *(v4i*)(base + (tid + q * THREADS) * 16) = rx[q];
         if (tid < 128) {
             const int d = tid >> 4, q = tid & 15;
-            uint32_t* h = (uint32_t*)(base + ZP8_X_BYTES + d * 128);
+            uint32_t* h = (uint32_t*)(base + XB + d * 128);
             h[q] = hw;
             h[16 + q] = hw;
         }
@@ -548,7 +556,7 @@ __global__ __launch_bounds__(512) void k_zp_mfma_circ(CircMfmaArgs c, ModTable m
             v4i xv[ZP_DIGITS];
 #pragma unroll
             for (int d = 0; d < ZP_DIGITS; d++)
-                xv[d] = *(const v4i*)(base + ((d * 64 + cb * 16 + x16) * 4 + g) * 16);
+                xv[d] = *(const v4i*)(base + ((d * TW + cb * 16 + x16) * 4 + g) * 16);
 #pragma unroll
             for (int u = 0; u < 2; u++) {
                 const int row = rh * 32 + u * 16 + x16;
@@ -558,7 +566,7 @@ __global__ __launch_bounds__(512) void k_zp_mfma_circ(CircMfmaArgs c, ModTable m
                 v4i av[ZP_DIGITS];
 #pragma unroll
                 for (int d = 0; d < ZP_DIGITS; d++) {
-                    const uint32_t* h = (const uint32_t*)(base + ZP8_X_BYTES + d * 128) + w0;
+                    const uint32_t* h = (const uint32_t*)(base + XB + d * 128) + w0;
                     uint32_t hv[5];
 #pragma unroll
                     for (int e = 0; e < 5; e++) hv[e] = h[e];
@@ -588,7 +596,7 @@ __global__ __launch_bounds__(512) void k_zp_mfma_circ(CircMfmaArgs c, ModTable m
                 total[u][r] = t >= md.N ? t - md.N : t;
             }
     }
-    const int64_t v = nt * 64 + cb * 16 + x16;
+    const int64_t v = nt * TW + cb * 16 + x16;
     if (v < a.Ly.nvec) {
         int64_t* out = a.y + l * a.Ly.s_limb + (v / a.Ly.nlo) * a.Ly.s_hi + (v % a.Ly.nlo) * a.Ly.s_lo;
 #pragma unroll
@@ -1105,9 +1113,12 @@ static void zp_mfma_run(csd_ctx* ctx, const csd_zp_matrix* z, const csd_view* sr
             c.top_orders[f] = z->top_orders[f];
             c.top_strides[f] = z->top_strides[f];
         }
-        const size_t lds = 2 * (size_t)(ZP8_X_BYTES + ZPC_H_BYTES);
-        csd_allow_lds((const void*)k_zp_mfma_circ, (int)lds);
-        hipLaunchKernelGGL(k_zp_mfma_circ, dim3((unsigned)c.m.nblocks), dim3(ZP8_THREADS), lds, ctx->stream, c, mt);
+        constexpr int NCB = ZPC_NCB;
+        c.m.ntiles = npad / (16 * NCB);
+        c.m.nblocks = z->r * c.m.mtiles * c.m.ntiles;
+        const size_t lds = 2 * (size_t)(ZP_DIGITS * 16 * NCB * 64 + ZPC_H_BYTES);
+        csd_allow_lds((const void*)k_zp_mfma_circ<NCB>, (int)lds);
+        hipLaunchKernelGGL(k_zp_mfma_circ<NCB>, dim3((unsigned)c.m.nblocks), dim3(128 * NCB), lds, ctx->stream, c, mt);
     } else if (zk == 3 || zk == 4) {
         a.mtiles = z->mpad / 64;  // mpad is a multiple of 64
         a.nblocks = z->r * a.mtiles * a.ntiles;
