@@ -533,8 +533,6 @@ def bench_batch(pkg, be, steps, count=256, seed=SEED, cpu=True):
         "tuning": tuning, "autotune_s": round(tune_s, 2),
         ("staged_ms_per_step" if chosen["persistent"] else "persistent_ms_per_step"): round(other_ms, 3),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "headline_frac": round(value / headline_peak, 4),
-                     "headline_peak": round(headline_peak, 1),
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": round(total_ms / launches, 4)},
         "kernels": per_kernel,
     }
