@@ -17,6 +17,7 @@
 
 #include "../../include/csdbfv_ring.h"
 #include "csd_internal.hpp"
+#include "csd_mont_fast.hpp"
 
 // ---------------------------------------------------------------------------------------------
 // synthetic residues
@@ -411,6 +412,9 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
                 }
         }
         const int64_t q = (int64_t)((i128)((u128)corr + (u128)half) >> a.shift);  // |q| <= 2^63
+        // the fast product needs |q| < 2^63 - 2^31 (the ring's conversions have |q| < 2^8); the
+        // wave takes the general one if any of its positions is beyond 2^62
+        const bool qfast = __all(q < ((int64_t)1 << 62) && q > -((int64_t)1 << 62));
         for (int j0 = 0; j0 < rd; j0 += GB_GROUP) {
             GbSplit ssum[GB_GROUP];
             i128 wsum[GB_GROUP];
@@ -439,13 +443,19 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
                 const int j = j0 + g;
                 if (j < rd) {
                     const i128 sum = SPLIT ? gb_split_sum(ssum[g]) : wsum[g];
-                    int64_t v = csd_double_red128(mt.m[j], sum);
-                    v = (int64_t)((uint64_t)v + (uint64_t)csd_mont_mul(mt.m[j], q, a.dmp[(int64_t)i * rd + j]));
-                    v = csd_to_mont64(mt.m[j], v);
+                    // the fast Montgomery forms (csd_mont_fast.hpp): |sum| < 2^119, and the values
+                    // below stay within [-2N, 2N]
+                    const CsdMod& mj = mt.m[j];
+                    const CsdFastMod f = csd_fast_mod(mj.N, mj.ninv);
+                    const int64_t dm = a.dmp[(int64_t)i * rd + j];
+                    int64_t v = csd_double_red_fast(f, (uint64_t)sum, (int64_t)(sum >> 64));
+                    const int64_t c = qfast ? csd_mont_mul_fc(f, q, csd_fast_const(dm, mj.ninv)) : csd_mont_mul(mj, q, dm);
+                    v = (int64_t)((uint64_t)v + (uint64_t)c);
+                    v = csd_mont_mul_fc(f, v, csd_fast_const(mj.r2, mj.ninv));  // to_mont64
                     int64_t* oj = o + i * a.sd[0] + j * a.sd[1];
                     if (RS) {
-                        const int64_t e = csd_mont_mul(mt.m[j], *oj, a.post_s[j]);
-                        v = csd_mont_mul(mt.m[j], (int64_t)((uint64_t)e - (uint64_t)v), a.post_v[j]);
+                        const int64_t e = csd_mont_mul(mj, *oj, a.post_s[j]);  // any i64
+                        v = csd_mont_mul_fc(f, (int64_t)((uint64_t)e - (uint64_t)v), csd_fast_const(a.post_v[j], mj.ninv));
                     }
                     *oj = v;
                 }
@@ -533,6 +543,312 @@ std::vector<int64_t> csd_bconv_transpose(const std::vector<int64_t>& mat, int64_
                 t[(size_t)((i * r + u) * *rdp + j)] = w;
             }
     return t;
+}
+
+// ---------------------------------------------------------------------------------------------
+// The BEHZ conversion (one digit over the whole source base; BConvCache::bconv, ring.rs:1663-1701,
+// and the rescale form above) with its matrix product on the int8 matrix cores.
+//
+// The sums S_j = sum_u W[j][u] x_u are exact integers, and so is the correction sum, which rides
+// along as matrix row rd. Every weight (|w| < 2^63) and every lifted limb (|x| <= N < 2^57) is
+// written in eight balanced base-256 digits, w = sum_a w_a 256^a and x = sum_b x_b 256^b with
+// w_a, x_b in [-128, 127]: the bytes of w + B, each minus 128, for B = sum_a 128 256^a. The 64
+// digit pairs (a, b) are 64 v_mfma_i32_32x32x32_i8, each over all source limbs at once (the K
+// slice of 32 bytes holds limb u in lane half u / 16, byte u % 16), accumulated by shift class
+// s = a + b: |C_s| <= 8 * 32 * 2^14 = 2^22 in int32. The classes are then combined exactly,
+// c_t = C_2t + 256 C_(2t+1) (|c_t| < 2^31) and R_t = c_t + (R_(t-1) >> 16), which gives
+// S = sum_(t<7) (R_t mod 2^16) 2^(16t) + R_7 2^112, |S| < 2^125 and |R_7| <= 2^13, in four
+// 32-bit words. The tail (double_red128, the correction q (-D 2^64 mod p_j), to_mont64 and the
+// rescale form's post-multiplications) is that of k_bconv_fused, so the residues are identical.
+//
+// Workgroup: 4 waves, each owning 64 positions of one element. Phase 1 (lane = position) lifts the
+// source limbs and writes their digit bytes to the wave's LDS slice as [digit][limb quad][64];
+// phase 2, per block of 32 positions, reads the B operands, runs the 64 MFMAs and the carry chain;
+// phase 3 runs the tail in the MFMA output layout (lane l, element 4i + t: row 8i + 4 (l / 32) + t,
+// position l % 32), where rows 0 .. rd - 1 leave at most a quarter of the lanes idle.
+
+typedef int gm_v4i __attribute__((ext_vector_type(4)));
+typedef int gm_v16i __attribute__((ext_vector_type(16)));
+
+#define GM_WAVES 4
+#define GM_MAX_SRC 32  // source limbs: one 32-byte K slice
+#define GM_MAX_ROWS 32  // target limbs plus the correction row
+static constexpr uint64_t GM_BIAS = 0x8080808080808080ull;
+// per target row j, from the host: N, ninv, n0 | n1 << 32, n1b (csd_fast_mod), dmp_j balanced
+// (lo | hi << 32) and dmp_j ninv, r2 balanced and r2 ninv; in LDS, for the rescale form, also
+// post_s[j], post_v[j] balanced and post_v[j] ninv
+#define GM_ROW_HOST 8
+#define GM_ROW 12
+
+__device__ __forceinline__ CsdFastConst gm_fc(int64_t packed, int64_t cn) {
+    return CsdFastConst{{(int32_t)(uint32_t)(uint64_t)packed, (int32_t)(packed >> 32)}, (uint64_t)cn};
+}
+
+struct GmArgs {
+    int64_t P, n3, chunks;
+    int32_t r, rd, nq, shift;
+    int64_t ss[4], sd[4];
+    const int64_t* src;
+    int64_t* dst;
+    const int64_t* dopi;   // [r]
+    const CsdMod* smods;   // [r] source moduli
+    const gm_v4i* atab;    // [8][64]: lane l's 16 bytes of weight digit a (csd_bconv_mfma_tables)
+    const int64_t* rows;   // [rd][GM_ROW_HOST]: target limb j's constants (csd_bconv_mfma_tables)
+    const int64_t* pre;    // rescale form (null otherwise): as in GbArgs
+    const int64_t* post_s;
+    const int64_t* post_v;
+};
+
+// 4 x 4 byte transpose: out[b] byte e = byte b of in[e]
+__device__ __forceinline__ void gm_transpose4(const uint32_t (&in)[4], uint32_t (&out)[4]) {
+    const uint32_t lo01 = __builtin_amdgcn_perm(in[1], in[0], 0x05010400u), hi01 = __builtin_amdgcn_perm(in[1], in[0], 0x07030602u);
+    const uint32_t lo23 = __builtin_amdgcn_perm(in[3], in[2], 0x05010400u), hi23 = __builtin_amdgcn_perm(in[3], in[2], 0x07030602u);
+    out[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+    out[1] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+    out[2] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+    out[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+}
+
+// ROWG groups of 8 matrix rows hold rows 0 .. rd (8 ROWG >= rd + 1)
+template <bool RS, int ROWG>
+__global__ __launch_bounds__(64 * GM_WAVES) void k_bconv_mfma(GmArgs a) {
+    extern __shared__ uint32_t gm_lds[];
+    int64_t* rowc = (int64_t*)gm_lds;  // [GM_MAX_ROWS][GM_ROW]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t* xt = gm_lds + GM_MAX_ROWS * GM_ROW * 2 + wave * (8 * a.nq * 64);  // [digit][quad][64 positions]
+    for (int j = threadIdx.x; j < a.rd; j += 64 * GM_WAVES) {
+        int64_t* rj = rowc + GM_ROW * j;
+#pragma unroll
+        for (int f = 0; f < GM_ROW_HOST; f++) rj[f] = a.rows[GM_ROW_HOST * j + f];
+        if (RS) {
+            const int64_t pv = a.post_v[j];
+            const CsdBal bv = csd_bal(pv);
+            rj[8] = a.post_s[j];
+            rj[9] = (int64_t)(((uint64_t)(uint32_t)bv.hi << 32) | (uint32_t)bv.lo);
+            rj[10] = (int64_t)((uint64_t)pv * (uint64_t)rj[1]);
+        }
+    }
+    const int64_t b = blockIdx.x / a.chunks;
+    const int64_t p0 = (blockIdx.x % a.chunks) * (64 * GM_WAVES) + wave * 64;
+    const int64_t* sb = a.src + b * a.ss[0];
+    int64_t* db = a.dst + b * a.sd[0];
+
+    // phase 1: lane = position p0 + lane; lift every source limb, split into digit bytes (lanes
+    // past the end read the last position and contribute zeros)
+    {
+        const int64_t pos = p0 + lane;
+        const bool act = pos < a.P;
+        const int64_t pc = act ? pos : a.P - 1;
+        const int64_t* s = sb + (pc / a.n3) * a.ss[2] + (pc % a.n3) * a.ss[3];
+        int64_t raw[GM_MAX_SRC];  // every limb's load in flight at once
+#pragma unroll
+        for (int u = 0; u < GM_MAX_SRC; u++) raw[u] = u < a.r ? s[u * a.ss[1]] : 0;
+#pragma unroll
+        for (int q = 0; q < GM_MAX_SRC / 4; q++) {
+            if (q >= a.nq) break;
+            uint32_t lo[4], hi[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int u = min(4 * q + e, a.r - 1);
+                const CsdMod mu = a.smods[u];
+                const int64_t xin = RS ? csd_mont_mul(mu, raw[4 * q + e], a.pre[u]) : raw[4 * q + e];
+                int64_t xv = csd_mont_mul(mu, xin, a.dopi[u]);
+                xv = (act && 4 * q + e < a.r) ? xv : 0;
+                const uint64_t d = ((uint64_t)xv + GM_BIAS) ^ GM_BIAS;
+                lo[e] = (uint32_t)d;
+                hi[e] = (uint32_t)(d >> 32);
+            }
+            uint32_t dl[4], dh[4];
+            gm_transpose4(lo, dl);
+            gm_transpose4(hi, dh);
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                xt[(g * a.nq + q) * 64 + lane] = dl[g];
+                xt[((g + 4) * a.nq + q) * 64 + lane] = dh[g];
+            }
+        }
+    }
+    __syncthreads();
+
+    const int h = lane >> 5, n = lane & 31;
+    // the correction row rd sits in lane half hc, element ec
+    const int hc = (a.rd >> 2) & 1, ec = 4 * (a.rd >> 3) + (a.rd & 3);
+    const i128 half = (i128)1 << (a.shift - 1);
+#pragma unroll 1
+    for (int blk = 0; blk < 2; blk++) {
+        // phase 2: B operands of this block's 32 positions, the 64 digit-pair MFMAs, the carry chain
+        // (the weight operands are read again per block, so they are not live during the tail)
+        gm_v4i A[8], Bv[8];
+#pragma unroll
+        for (int d = 0; d < 8; d++) A[d] = a.atab[d * 64 + lane];
+#pragma unroll
+        for (int d = 0; d < 8; d++)
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) {
+                // quads past the source limbs meet zero weights: any bytes do
+                const int q = min(4 * h + qq, a.nq - 1);
+                Bv[d][qq] = (int)xt[(d * a.nq + q) * 64 + 32 * blk + n];
+            }
+        int32_t R[4 * ROWG], low[4 * ROWG];
+        uint32_t w[4 * ROWG][4];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            gm_v16i ce = {0}, co = {0};
+#pragma unroll
+            for (int da = 0; da < 8; da++) {
+                const int de = 2 * t - da, dodd = 2 * t + 1 - da;
+                if (de >= 0 && de < 8) ce = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[da], Bv[de], ce, 0, 0, 0);
+                if (dodd >= 0 && dodd < 8) co = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[da], Bv[dodd], co, 0, 0, 0);
+            }
+#pragma unroll
+            for (int e = 0; e < 4 * ROWG; e++) {
+                const int32_t c = (int32_t)(((uint32_t)co[e] << 8) + (uint32_t)ce[e]);
+                if (t == 0) {
+                    R[e] = c;
+                } else {
+                    const int32_t prev = R[e];
+                    R[e] = c + (prev >> 16);
+                    if ((t - 1) % 2 == 0) low[e] = prev & 0xffff;
+                    else w[e][(t - 1) / 2] = ((uint32_t)prev << 16) | (uint32_t)low[e];
+                }
+                // pin the chain here: otherwise it is sunk towards the tail and every class
+                // accumulator stays live until then
+                asm volatile("" : "+v"(R[e]));
+            }
+            __builtin_amdgcn_sched_barrier(0);  // one class pair's accumulators live at a time
+        }
+#pragma unroll
+        for (int e = 0; e < 4 * ROWG; e++) w[e][3] = ((uint32_t)R[e] << 16) | (uint32_t)low[e];
+
+        // phase 3: the correction q of each position, then the tail of every target row
+        int64_t qv = 0;
+#pragma unroll
+        for (int e = 0; e < 4 * ROWG; e++)
+            if (e == ec) {
+                const i128 corr = (i128)(((u128)w[e][3] << 96) | ((u128)w[e][2] << 64) | ((u128)w[e][1] << 32) | (u128)w[e][0]);
+                qv = (int64_t)((i128)((u128)corr + (u128)half) >> a.shift);
+            }
+        const int src_lane = n + 32 * hc;
+        const int64_t q = (int64_t)(((uint64_t)(uint32_t)__shfl((int)(uint32_t)((uint64_t)qv >> 32), src_lane) << 32) |
+                                    (uint32_t)__shfl((int)(uint32_t)qv, src_lane));
+        const int64_t pos = p0 + 32 * blk + n;
+        const bool pact = pos < a.P;
+        const int64_t pc = pact ? pos : a.P - 1;
+        int64_t* o = db + (pc / a.n3) * a.sd[2] + (pc % a.n3) * a.sd[3];
+#pragma unroll
+        for (int e = 0; e < 4 * ROWG; e++) {
+            if (8 * (e >> 2) + (e & 3) >= a.rd) continue;  // no target row in either lane half
+            const int j = 8 * (e >> 2) + 4 * h + (e & 3);
+            const int jc = min(j, a.rd - 1);
+            const int64_t* rc = rowc + GM_ROW * jc;
+            const uint64_t np = (uint64_t)rc[2];
+            const CsdFastMod f = {(uint64_t)rc[1], (uint32_t)np, (uint32_t)(np >> 32), (uint32_t)rc[3]};
+            // |S| < 2^125, |q| < 2^61 and the values below within [-2N, 2N]: the fast forms apply
+            const uint64_t lo64 = ((uint64_t)w[e][1] << 32) | w[e][0];
+            const int64_t hi64 = (int64_t)(((uint64_t)w[e][3] << 32) | w[e][2]);
+            int64_t v = csd_double_red_fast(f, lo64, hi64);
+            v = (int64_t)((uint64_t)v + (uint64_t)csd_mont_mul_fc(f, q, gm_fc(rc[4], rc[5])));
+            v = csd_mont_mul_fc(f, v, gm_fc(rc[6], rc[7]));  // to_mont64
+            int64_t* oj = o + jc * a.sd[1];
+            if (RS) {
+                // the destination holds any i64: the general product for it
+                const int64_t ev = csd_mont_mul(CsdMod{rc[0], rc[1], 0, 0}, *oj, rc[8]);
+                v = csd_mont_mul_fc(f, (int64_t)((uint64_t)ev - (uint64_t)v), gm_fc(rc[9], rc[10]));
+            }
+            if (pact && j < a.rd) *oj = v;
+        }
+    }
+}
+
+bool csd_bconv_mfma_tables(const std::vector<int64_t>& mat, int64_t rd, int64_t r, const std::vector<CsdMod>& dmods,
+                           const std::vector<int64_t>& dmp, std::vector<int8_t>& atab, std::vector<int64_t>& rows) {
+    if (r < 1 || r > GM_MAX_SRC || rd < 1 || rd + 1 > GM_MAX_ROWS) return false;
+    if ((int64_t)dmods.size() != rd || (int64_t)dmp.size() != rd || (int64_t)mat.size() != (rd + 1) * r) return false;
+    // the balanced 8-digit range is [-B, 2^64 - 1 - B]: below, every i64 is inside (B > 2^63);
+    // above, w + B must not wrap
+    const int64_t wmax = (int64_t)(~GM_BIAS);
+    for (int64_t w : mat)
+        if (w > wmax) return false;
+    atab.assign(8 * 64 * 16, 0);
+    for (int l = 0; l < 64; l++) {
+        const int64_t j = l % 32, hh = l / 32;
+        for (int e = 0; e < 16; e++) {
+            const int64_t u = 16 * hh + e;
+            if (j > rd || u >= r) continue;
+            const uint64_t d = (uint64_t)mat[(size_t)(j * r + u)] + GM_BIAS;
+            for (int dg = 0; dg < 8; dg++) atab[(size_t)((dg * 64 + l) * 16 + e)] = (int8_t)(uint8_t)(((d >> (8 * dg)) & 255) ^ 128);
+        }
+    }
+    rows.assign((size_t)(GM_ROW_HOST * rd), 0);
+    auto packed = [](CsdBal b) { return (int64_t)(((uint64_t)(uint32_t)b.hi << 32) | (uint32_t)b.lo); };
+    for (int64_t j = 0; j < rd; j++) {
+        const CsdMod& m = dmods[(size_t)j];
+        if (m.N <= 0 || m.N >= ((int64_t)1 << 62)) return false;
+        const CsdFastMod f = csd_fast_mod(m.N, m.ninv);
+        int64_t* rj = rows.data() + GM_ROW_HOST * j;
+        rj[0] = m.N;
+        rj[1] = m.ninv;
+        rj[2] = (int64_t)(((uint64_t)f.n1 << 32) | f.n0);
+        rj[3] = (int64_t)f.n1b;
+        rj[4] = packed(csd_bal(dmp[(size_t)j]));
+        rj[5] = (int64_t)((uint64_t)dmp[(size_t)j] * (uint64_t)m.ninv);
+        rj[6] = packed(csd_bal(m.r2));
+        rj[7] = (int64_t)((uint64_t)m.r2 * (uint64_t)m.ninv);
+    }
+    return true;
+}
+
+void csd_bconv_mfma_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, const int64_t* dopi, const int8_t* atab,
+                           const int64_t* rows, size_t shift, const CsdMod* smods_dev, const char* timer, const int64_t* pre,
+                           const int64_t* post_s, const int64_t* post_v) {
+    GmArgs a;
+    memset(&a, 0, sizeof(a));
+    const int64_t B = src.shape[0];
+    a.r = (int32_t)src.shape[1];
+    a.rd = (int32_t)dst.shape[1];
+    a.n3 = src.shape[3];
+    a.P = src.shape[2] * src.shape[3];
+    if (a.r < 1 || a.r > GM_MAX_SRC || a.rd < 1 || a.rd + 1 > GM_MAX_ROWS) throw CsdError("mfma bconv: base sizes out of range");
+    if (dst.shape[0] != B || dst.shape[2] != src.shape[2] || dst.shape[3] != src.shape[3]) throw CsdError("mfma bconv: shapes");
+    if ((pre == nullptr) != (post_s == nullptr) || (post_s == nullptr) != (post_v == nullptr))
+        throw CsdError("mfma bconv: the rescale form needs pre, post_s and post_v together");
+    if (shift < 1 || shift > 125) throw CsdError("mfma bconv: correction shift out of range");
+    for (int q = 0; q < 4; q++) {
+        a.ss[q] = src.strides[q];
+        a.sd[q] = dst.strides[q];
+    }
+    a.src = (const int64_t*)src.ptr;
+    a.dst = (int64_t*)dst.ptr;
+    a.dopi = dopi;
+    a.smods = smods_dev;
+    a.atab = (const gm_v4i*)atab;
+    a.rows = rows;
+    a.shift = (int32_t)shift;
+    a.nq = (a.r + 3) / 4;
+    a.pre = pre;
+    a.post_s = post_s;
+    a.post_v = post_v;
+    if (B * a.P == 0) return;
+    a.chunks = (a.P + 64 * GM_WAVES - 1) / (64 * GM_WAVES);
+    const int64_t blocks = B * a.chunks;
+    if (blocks > (int64_t)INT32_MAX) throw CsdError("mfma bconv: grid too large");
+    TimedScope ts(ctx, timer, 8.0 * (double)B * a.P * ((double)a.r + (double)a.rd * (post_s ? 2 : 1)));
+    const size_t lds = (size_t)GM_MAX_ROWS * GM_ROW * 8 + (size_t)GM_WAVES * 8 * a.nq * 64 * sizeof(uint32_t);
+    const int rowg = (a.rd + 1 + 7) / 8;
+    auto launch = [&](auto kern) {
+        csd_allow_lds((const void*)kern, (int)lds);
+        hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * GM_WAVES), lds, ctx->stream, a);
+    };
+    if (post_s) {
+        if (rowg <= 2) launch(k_bconv_mfma<true, 2>);
+        else if (rowg == 3) launch(k_bconv_mfma<true, 3>);
+        else launch(k_bconv_mfma<true, 4>);
+    } else {
+        if (rowg <= 2) launch(k_bconv_mfma<false, 2>);
+        else if (rowg == 3) launch(k_bconv_mfma<false, 3>);
+        else launch(k_bconv_mfma<false, 4>);
+    }
+    HIPCHECK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -98,6 +98,15 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
                             const CsdMod* smods_dev, const char* timer, const int64_t* pre = nullptr,
                             const int64_t* post_s = nullptr, const int64_t* post_v = nullptr);
 std::vector<int64_t> csd_bconv_transpose(const std::vector<int64_t>& mat, int64_t d, int64_t rd, int64_t r, int64_t* rdp);
+// The BEHZ conversion (one digit over the whole source base) with its matrix product on the int8
+// matrix cores (csd_fused.hip); residues identical to csd_bconv_fused_launch's. Tables from
+// csd_bconv_mfma_tables: mat [rd + 1][r] (row rd the correction weights), the target moduli and
+// -D 2^64 mod p_j; false when r > 32, rd > 31 or a weight is outside eight balanced digits.
+bool csd_bconv_mfma_tables(const std::vector<int64_t>& mat, int64_t rd, int64_t r, const std::vector<CsdMod>& dmods,
+                           const std::vector<int64_t>& dmp, std::vector<int8_t>& atab, std::vector<int64_t>& rows);
+void csd_bconv_mfma_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, const int64_t* dopi, const int8_t* atab,
+                           const int64_t* rows, size_t shift, const CsdMod* smods_dev, const char* timer,
+                           const int64_t* pre = nullptr, const int64_t* post_s = nullptr, const int64_t* post_v = nullptr);
 
 // launch shape of the work-list NTT kernels (0 = the built-in choice)
 struct CsdSegTune {

@@ -458,6 +458,9 @@ struct BConvCache {  // ring.rs:1596-1604
     std::unique_ptr<DevBuf> matrix_t;  // the matrix rows transposed and padded (csd_bconv_transpose)
     int64_t rdp = 0;
     std::vector<int64_t> host_qopi, host_matrix, host_qm;  // the same constants, kept for fused kernels
+    // the matrix-core form (csd_bconv_mfma_tables): weight digit operands and target-row constants,
+    // null when the bases or weights are outside what that kernel covers
+    std::unique_ptr<DevBuf> mfma_atab, mfma_rows;
 };
 
 struct GadgetCache {  // ring.rs:1714-1727
@@ -532,7 +535,7 @@ struct csd_ring {
     // the ring's execution-form settings (every form computes identical residues): see
     // csd_ring_set_overlap, csd_ring_set_bconv_form and csd_ring_set_gadget_form
     std::atomic<bool> overlap_halves{false};
-    std::atomic<bool> bconv_fused{true};
+    std::atomic<int> bconv_form{2};  // csd_ring_set_bconv_form: 0 HAL sequence, 1 fused, 2 fused on the matrix cores
     std::atomic<bool> gadget_fused{true};
 
     ~csd_ring() {
@@ -791,6 +794,18 @@ struct csd_ring {
             c->src_mods = upload(ctx, flat);
         }
         c->matrix_t = upload(ctx, csd_bconv_transpose(mat, 1, (int64_t)rd, (int64_t)r, &c->rdp));
+        {
+            auto dm = mods_for(dst);
+            ModTable t = make_mod_table(dm.data(), dm.size(), (int64_t)dm.size());
+            std::vector<CsdMod> dmods(t.m, t.m + rd);
+            std::vector<int8_t> atab;
+            std::vector<int64_t> rows;
+            if (csd_bconv_mfma_tables(mat, (int64_t)rd, (int64_t)r, dmods, qm, atab, rows)) {
+                c->mfma_atab.reset(new DevBuf(ctx, atab.size()));
+                HIPCHECK(hipMemcpy(c->mfma_atab->p, atab.data(), atab.size(), hipMemcpyHostToDevice));
+                c->mfma_rows = upload(ctx, rows);
+            }
+        }
         c->host_qopi = qopi;
         c->host_matrix = mat;
         c->host_qm = qm;
@@ -812,7 +827,13 @@ struct csd_ring {
         if (dst.shape[0] != cnt || dst.shape[1] != rd || dst.shape[2] != n2 || dst.shape[3] != n3)
             throw CsdError("bconv: destination shape");
         auto smods = mods_for(sbase), dmods = mods_for(dbase);
-        if (bconv_fused && c.acc >= (size_t)r && !views_overlap(dst, src_in, 8)) {
+        const int form = bconv_form;
+        if (form == 2 && c.mfma_atab && c.acc >= (size_t)r && !views_overlap(dst, src_in, 8)) {
+            csd_bconv_mfma_launch(ctx, dst, src_in, (const int64_t*)c.q_over_p_inv->p, (const int8_t*)c.mfma_atab->p,
+                                  (const int64_t*)c.mfma_rows->p, c.log2_qtilde, (const CsdMod*)c.src_mods->p, "bconv_mfma");
+            return;
+        }
+        if (form >= 1 && c.acc >= (size_t)r && !views_overlap(dst, src_in, 8)) {
             const std::vector<size_t> whole = {0, (size_t)r};
             csd_bconv_fused_launch(ctx, dst, src_in, 1, (const int64_t*)c.q_over_p_inv->p, (const int64_t*)c.matrix->p,
                                    (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.q_mod_p->p, whole,
@@ -869,7 +890,14 @@ struct csd_ring {
         };
         if (!scalar_ok(s_src, r) || !scalar_ok(s_ext, re) || !scalar_ok(inv, re)) throw CsdError("rescale: scalar tensor shape");
         auto smods = mods_for(sbase), emods = mods_for(ebase);
-        if (bconv_fused && c.acc >= (size_t)r && limb_disjoint(ext, src)) {
+        const int form = bconv_form;
+        if (form == 2 && c.mfma_atab && c.acc >= (size_t)r && limb_disjoint(ext, src)) {
+            csd_bconv_mfma_launch(ctx, ext, src, (const int64_t*)c.q_over_p_inv->p, (const int8_t*)c.mfma_atab->p,
+                                  (const int64_t*)c.mfma_rows->p, c.log2_qtilde, (const CsdMod*)c.src_mods->p, "rescale_step_mfma",
+                                  (const int64_t*)s_src.ptr, (const int64_t*)s_ext.ptr, (const int64_t*)inv.ptr);
+            return;
+        }
+        if (form >= 1 && c.acc >= (size_t)r && limb_disjoint(ext, src)) {
             const std::vector<size_t> whole = {0, (size_t)r};
             csd_bconv_fused_launch(ctx, ext, src, 1, (const int64_t*)c.q_over_p_inv->p, (const int64_t*)c.matrix->p,
                                    (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.q_mod_p->p, whole,
@@ -1435,12 +1463,12 @@ extern "C" int csd_ring_set_overlap(csd_ring* ring, int on) {
     CSD_CATCH
 }
 
-extern "C" int csd_ring_get_bconv_form(const csd_ring* ring) { return ring && ring->bconv_fused ? 1 : 0; }
+extern "C" int csd_ring_get_bconv_form(const csd_ring* ring) { return ring ? ring->bconv_form.load() : 0; }
 
 extern "C" int csd_ring_set_bconv_form(csd_ring* ring, int form) {
     CSD_TRY if (!ring) throw CsdError("null ring");
-    if (form != 0 && form != 1) throw CsdError("bconv form must be 0 (HAL sequence) or 1 (fused)");
-    ring->bconv_fused = form == 1;
+    if (form < 0 || form > 2) throw CsdError("bconv form must be 0 (HAL sequence), 1 (fused) or 2 (fused, matrix cores)");
+    ring->bconv_form = form;
     CSD_CATCH
 }
 

@@ -176,11 +176,11 @@ def _oracle_rescale_step(c, x, lifted, base_len, t):
     return x[:, base_len:]
 
 
-@pytest.mark.parametrize("form", [1, 0])
+@pytest.mark.parametrize("form", [2, 1, 0])
 @pytest.mark.parametrize("layout", ["split", "separate", "ext_first"])
 @pytest.mark.parametrize("field,log2_q,t_bits,ndigits", CASES[1:])
 def test_rescale_step_parity(gpu, cpu, rings, field, log2_q, t_bits, ndigits, layout, form):
-    """csd_ring_rescale_step against the oracle's sequence, bit for bit, in both conversion forms:
+    """csd_ring_rescale_step against the oracle's sequence, bit for bit, in every conversion form:
     source and extension as limb ranges of one tensor (either order) or as separate tensors, the
     extension a strided view of a wider one."""
     pkg = load()
@@ -208,10 +208,10 @@ def test_rescale_step_parity(gpu, cpu, rings, field, log2_q, t_bits, ndigits, la
     try:
         g.rescale_step(ext, lifted[bl:], src, base, s[:, :bl], s[:, bl:], inv)
     finally:
-        g.set_bconv_form(1)
+        g.set_bconv_form(2)
     assert same(from_dev(gpu, ext), want)
-    if form == 1:
-        assert same(from_dev(gpu, src), src_before)  # the fused form leaves the source as it is
+    if form >= 1:
+        assert same(from_dev(gpu, src), src_before)  # the fused forms leave the source as it is
     if layout == "separate":  # the rows beside the extension view stay untouched
         assert not from_dev(gpu, xd[:, :1]).any() and not from_dev(gpu, xd[:, el + 1:]).any()
 

@@ -113,10 +113,11 @@ def test_field18_square_kat_on_device(gpu, rings):
             assert int(out[blk * 6 + j]) % p == R.montgomery_repr(exp[j], p)
 
 
-@pytest.fixture(params=[1, 0], ids=["fused", "hal_sequence"])
+@pytest.fixture(params=[2, 1, 0], ids=["mfma", "fused", "hal_sequence"])
 def bconv_form(request):
-    """runs a test with the fused conversion kernel and with the reference's HAL sequence: the
-    test calls the yielded function on each ring it uses, which is reset to the default after"""
+    """runs a test with the matrix-core conversion kernel, the fused VALU kernel and the reference's
+    HAL sequence: the test calls the yielded function on each ring it uses, which is reset to the
+    default after"""
     used = []
 
     def apply(ring):
@@ -127,7 +128,7 @@ def bconv_form(request):
     yield apply
     for ring in used:
         if ring.handle:  # rings a test closed itself are gone
-            ring.set_bconv_form(1)
+            ring.set_bconv_form(2)
 
 
 def test_bconv_kat_on_device(gpu, bconv_form):
@@ -171,7 +172,7 @@ def test_bconv_parity(gpu, cpu, rings, bconv_form):
         sg = to_dev(gpu, src)
         g.bconv_tensors(dg, db, sg, sb)
         assert same(from_dev(gpu, dg), dc)
-        if form == 1:
+        if form >= 1:
             assert same(from_dev(gpu, sg), src)
     # strided views: limbs 1.. of a [2, 1 + r, ., .] tensor into limbs 2.. of a [2, 2 + r', ., .] one
     sb, db = base, ext
@@ -183,6 +184,39 @@ def test_bconv_parity(gpu, cpu, rings, bconv_form):
     out = gpu.hal_allocate_i64((2, 2 + len(db), 12, 3072))
     g.bconv_tensors(out[:, 2:], db, bg[:, 1:], sb)
     assert same(from_dev(gpu, out)[:, 2:], dc)
+
+
+def test_bconv_matrix_core_path_is_taken(gpu, rings):
+    """The default conversion form runs the matrix-core kernel (its own timer name) for the lift of
+    field_36864 and field_49152 to their extensions, the reverse conversions and the rescale step;
+    form 1 runs the vector-ALU kernel for the same calls."""
+    for field, log2_q in (("field_36864", (960, 980)), ("field_49152", (1280, 1300))):
+        g, _ = rings(field, log2_q, 76)
+        assert g.bconv_form() == 2
+        base = g.initial_rns_base()
+        ext = g.extension_rns_base_for(base)
+        n2, n3 = g.info["deg_lprime"], g.info["deg_k"]
+        lifted = base + ext
+        x = g.allocate(1, lifted)
+        g.fill_uniform(x, lifted, 3)
+        d = gpu.hal_allocate_i64((1, len(base), n2, n3))
+        s = g.scalar_tensor(lifted, 1 << 20)
+        inv = g.inverse_product_tensor(ext, base)
+        for form, names in ((2, {"bconv_mfma", "rescale_step_mfma"}), (1, {"bconv", "rescale_step"})):
+            g.set_bconv_form(form)
+            gpu.timing_report()  # clears the record
+            gpu.set_timing(True)
+            try:
+                g.bconv_tensors(x[:, len(base):], ext, x[:, :len(base)], base)
+                g.bconv_tensors(d, base, x[:, len(base):], ext)
+                g.rescale_step(x[:, len(base):], ext, x[:, :len(base)], base, s[:, :len(base)], s[:, len(base):], inv)
+                gpu.sync()
+            finally:
+                gpu.set_timing(False)
+                g.set_bconv_form(2)
+            rep = gpu.timing_report()
+            assert names <= set(rep), (form, sorted(rep))
+            assert rep.get("bconv_mfma" if form == 2 else "bconv", (0,))[0] == 2
 
 
 def test_bconv_parity_field49152(gpu, cpu, rings, bconv_form):
@@ -338,9 +372,9 @@ def test_transform_round_trip_full_batch(gpu, cpu, rings):
     assert (np.mod(back, primes) == data).all()
 
 
-@pytest.mark.parametrize("form", [1, 0])
+@pytest.mark.parametrize("form", [2, 1, 0])
 def test_bconv_preserve_src(gpu, cpu, rings, form):
-    """csd_ring_bconv with preserve_src leaves the source unchanged in both conversion forms, with
+    """csd_ring_bconv with preserve_src leaves the source unchanged in every conversion form, with
     the same residues as the destroying call"""
     g, c = rings("field_36864", (960, 980), 76)
     base = g.initial_rns_base()
@@ -357,7 +391,7 @@ def test_bconv_preserve_src(gpu, cpu, rings, form):
         assert same(from_dev(gpu, dg), dc)
         assert same(from_dev(gpu, sg), src)
     finally:
-        g.set_bconv_form(1)
+        g.set_bconv_form(2)
 
 
 def test_forms_are_per_ring_and_context(gpu, cpu):
@@ -374,7 +408,7 @@ def test_forms_are_per_ring_and_context(gpu, cpu):
     try:
         ra.set_bconv_form(0)
         be_a.ntt_force_generic(1)
-        assert ra.bconv_form() == 0 and rb.bconv_form() == 1
+        assert ra.bconv_form() == 0 and rb.bconv_form() == 2
         base = ra.initial_rns_base()
         ext = ra.extension_rns_base_for(base)
         n2, n3 = ra.info["deg_lprime"], ra.info["deg_k"]
