@@ -9,3 +9,4 @@ from . import bfv  # noqa: E402,F401
 from . import dbfv  # noqa: E402,F401
 from . import digitex  # noqa: E402,F401
 from . import mfma_transform  # noqa: E402,F401
+from . import homcrt  # noqa: E402,F401

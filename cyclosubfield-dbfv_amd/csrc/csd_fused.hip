@@ -78,10 +78,12 @@ struct GksArgs {
     int d, batch;
     const int64_t* parts;  // [batch, d, r, n2, n3]
     const int64_t* c0;     // [batch, r, n2, n3]
-    const int64_t* key;    // [d, 2, r, n2, n3]
+    const int64_t* key;    // row 2i + s of digit i at key + (2i + s) krow, limbs contiguous planes
+    int64_t krow;
     const uint32_t* perm2;
     const uint32_t* perm3;
-    int64_t* out;  // [batch, 2, r, n2, n3]
+    int64_t* out;  // element e = h grp + l: row h o_hi + l o_lo + s o_s
+    int64_t grp, c0_hi, c0_lo, o_hi, o_lo, o_s;  // c0 of element e at row h c0_hi + l c0_lo
     int full0, full1;
     int64_t chunks;  // position chunks of 256 along dim 3
     int64_t groups;  // ciphertext groups of GKS_CB
@@ -124,18 +126,19 @@ __global__ __launch_bounds__(256) void k_galois_keyswitch(GksArgs a, ModTable mt
         const int64_t* P = a.parts + (int64_t)c_first * a.d * limb + src_pos;
         const int64_t cstride = (int64_t)a.d * limb;
         const int64_t* K = a.key + dst_pos;
+        const int64_t krow = a.krow;
         int64_t pv[GKS_CB], k0 = 0, k1 = 0;
 #pragma unroll
         for (int q = 0; q < GKS_CB; q++) pv[q] = P[q * cstride];
         k0 = K[0];
-        k1 = K[limb];
+        k1 = K[krow];
         for (int i = 0; i < a.d; i++) {
             int64_t pn[GKS_CB], k0n = 0, k1n = 0;
             if (i + 1 < a.d) {
 #pragma unroll
                 for (int q = 0; q < GKS_CB; q++) pn[q] = P[q * cstride + (int64_t)(i + 1) * limb];
-                k0n = K[(int64_t)(2 * i + 2) * limb];
-                k1n = K[(int64_t)(2 * i + 3) * limb];
+                k0n = K[(int64_t)(2 * i + 2) * krow];
+                k1n = K[(int64_t)(2 * i + 3) * krow];
             }
 #pragma unroll
             for (int q = 0; q < GKS_CB; q++) {
@@ -151,8 +154,8 @@ __global__ __launch_bounds__(256) void k_galois_keyswitch(GksArgs a, ModTable mt
         }
     } else {
         for (int i = 0; i < a.d; i++) {
-            const int64_t k0 = a.key[(int64_t)(2 * i) * limb + dst_pos];
-            const int64_t k1 = a.key[(int64_t)(2 * i + 1) * limb + dst_pos];
+            const int64_t k0 = a.key[(int64_t)(2 * i) * a.krow + dst_pos];
+            const int64_t k1 = a.key[(int64_t)(2 * i + 1) * a.krow + dst_pos];
 #pragma unroll
             for (int q = 0; q < GKS_CB; q++) {
                 if (q < nc) {
@@ -168,11 +171,12 @@ __global__ __launch_bounds__(256) void k_galois_keyswitch(GksArgs a, ModTable mt
 #pragma unroll
     for (int q = 0; q < GKS_CB; q++) {
         if (q < nc) {
-            const int64_t x0 = a.c0[(int64_t)(c_first + q) * limb + src_pos];
+            const int64_t e = c_first + q, h = e / a.grp, l = e - h * a.grp;
+            const int64_t x0 = a.c0[(h * a.c0_hi + l * a.c0_lo) * limb + src_pos];
             const i128 s0 = (i128)((u128)acc0[q] + (u128)((i128)x0 * (i128)one));
-            int64_t* o = a.out + (int64_t)(c_first + q) * 2 * limb + dst_pos;
+            int64_t* o = a.out + (h * a.o_hi + l * a.o_lo) * limb + dst_pos;
             o[0] = reduce_wide_entry(m, s0, a.full0);
-            o[limb] = reduce_wide_entry(m, acc1[q], a.full1);
+            o[a.o_s * limb] = reduce_wide_entry(m, acc1[q], a.full1);
         }
     }
 }
@@ -265,8 +269,9 @@ void csd_mod_switch_fill_level(void* dst, int r, const int64_t* M, const int64_t
 
 // exported through the ring (which owns the permutations); see csd_ring.hip
 void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t r, int64_t n2, int64_t n3, const int64_t* parts,
-                                 const int64_t* c0, const int64_t* key, const uint32_t* perm2, const uint32_t* perm3,
-                                 int64_t* out, const csd_modulus* mods, int full0, int full1) {
+                                 const int64_t* c0, const int64_t* key, int64_t krow, const uint32_t* perm2,
+                                 const uint32_t* perm3, int64_t* out, const GksRows& rows, const csd_modulus* mods, int full0,
+                                 int full1) {
     GksArgs a;
     memset(&a, 0, sizeof(a));
     a.r = r;
@@ -277,9 +282,16 @@ void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t
     a.parts = parts;
     a.c0 = c0;
     a.key = key;
+    a.krow = krow;
     a.perm2 = perm2;
     a.perm3 = perm3;
     a.out = out;
+    a.grp = rows.grp;
+    a.c0_hi = rows.c0_hi;
+    a.c0_lo = rows.c0_lo;
+    a.o_hi = rows.o_hi;
+    a.o_lo = rows.o_lo;
+    a.o_s = rows.o_s;
     a.full0 = full0;
     a.full1 = full1;
     a.chunks = (n3 + 255) / 256;

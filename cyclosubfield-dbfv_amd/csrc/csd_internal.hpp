@@ -82,9 +82,19 @@ struct csd_perm_plan {
     std::vector<uint32_t> host;
 };
 
+// Row addressing of the batched Galois key switch: element e = h grp + l (0 <= l < grp) reads c0
+// row h c0_hi + l c0_lo and writes output row s at h o_hi + l o_lo + s o_s (rows of r limb planes).
+// The plain batch is grp = batch, c0_lo = 1, o_lo = 2, o_s = 1; a digit-major dBFV batch of n
+// ciphertexts is grp = n, c0_hi = o_hi = 2 n, c0_lo = o_lo = 1, o_s = n.
+struct GksRows {
+    int64_t grp, c0_hi, c0_lo, o_hi, o_lo, o_s;
+};
+
+// key: row 2i + s of key digit i at key + (2i + s) krow (a restricted key keeps its row stride)
 void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t r, int64_t n2, int64_t n3, const int64_t* parts,
-                                 const int64_t* c0, const int64_t* key, const uint32_t* perm2, const uint32_t* perm3,
-                                 int64_t* out, const csd_modulus* mods, int full0, int full1);
+                                 const int64_t* c0, const int64_t* key, int64_t krow, const uint32_t* perm2,
+                                 const uint32_t* perm3, int64_t* out, const GksRows& rows, const csd_modulus* mods, int full0,
+                                 int full1);
 
 // GadgetBConvCache::bconv / BConvCache::bconv as one kernel over B source elements (see
 // csd_fused.hip), for digits whose limbs fit one i128 chunk; smods_dev holds the source moduli,

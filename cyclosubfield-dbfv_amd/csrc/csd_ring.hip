@@ -1094,20 +1094,33 @@ struct csd_ring {
     }
 
     // The key-switch part of bfv_hom_gal (bfv.rs:816-839) for a batch of ciphertexts whose
-    // gadget digits were hoisted, over a set of rotations; see csd_fused.hip.
-    void galois_keyswitch(const std::vector<size_t>& base, int64_t batch, int64_t d, const csd_view& parts, const csd_view& c0,
-                          const std::vector<uint64_t>& gs, const csd_view* keys, const csd_view& out) {
+    // gadget digits were hoisted, over a set of rotations; see csd_fused.hip. Element e reads c0 and
+    // writes outs[t] through the row addressing `rows` (GksRows); keys[t] may be a restriction of a
+    // key over a longer base (rows 2i + s at any row stride, limbs as contiguous planes).
+    void galois_keyswitch_rows(const std::vector<size_t>& base, int64_t batch, int64_t d, const csd_view& parts,
+                               const csd_view& c0, const std::vector<uint64_t>& gs, const csd_view* keys, const csd_view* outs,
+                               const GksRows& rows) {
         int64_t r = (int64_t)base.size(), n2 = (int64_t)f->deg_lp, n3 = (int64_t)f->deg_k;
+        const int64_t plane = n2 * n3;
         auto contiguous = [](const csd_view& v, int64_t s0, int64_t s1, int64_t s2, int64_t s3) {
             csd_view e = make_view(v.ptr, s0, s1, s2, s3);
             for (int t = 0; t < 4; t++)
                 if (v.shape[t] != e.shape[t] || (e.shape[t] > 1 && v.strides[t] != e.strides[t])) return false;
             return true;
         };
+        if (batch < 1 || d < 1 || rows.grp < 1) throw CsdError("galois key switch: empty batch or digits");
         if (!contiguous(parts, batch * d, r, n2, n3)) throw CsdError("galois key switch: parts must be a contiguous [batch*d, r, n2, n3] tensor");
-        if (!contiguous(c0, batch, r, n2, n3)) throw CsdError("galois key switch: c0 must be a contiguous [batch, r, n2, n3] tensor");
-        if (!contiguous(out, (int64_t)gs.size() * batch * 2, r, n2, n3))
-            throw CsdError("galois key switch: out must be a contiguous [ngal*batch*2, r, n2, n3] tensor");
+        // the largest row an element reaches, for the bounds of c0 and the outputs
+        const int64_t hmax = (batch - 1) / rows.grp, lmax = std::min<int64_t>(batch, rows.grp) - 1;
+        auto max_row = [&](int64_t hi, int64_t lo) {
+            if (hi < 0 || lo < 0) throw CsdError("galois key switch: negative row strides");
+            return hmax * hi + lmax * lo;
+        };
+        const int64_t c0_rows = max_row(rows.c0_hi, rows.c0_lo) + 1;
+        if (!contiguous(c0, c0.shape[0], r, n2, n3) || c0.shape[0] < c0_rows)
+            throw CsdError("galois key switch: c0 must be a contiguous [rows, r, n2, n3] tensor covering every element");
+        if (rows.o_s < 0) throw CsdError("galois key switch: negative row strides");
+        const int64_t o_rows = max_row(rows.o_hi, rows.o_lo) + rows.o_s + 1;
         auto m = mods_for(base);
         // WideRingEls bounds (ring.rs:354-365, 441-455): d terms of bound-1 operands, plus the
         // bound-1 c0 on row 0; a row needs the full reduction above 2^63 min_p (ring.rs:366)
@@ -1118,14 +1131,40 @@ struct csd_ring {
         }
         u128 b1 = (u128)d * max_p * max_p, b0 = b1 + max_p * max_p;
         int full0 = b0 > (min_p << 63), full1 = b1 > (min_p << 63);
-        int64_t per_g = batch * 2 * r * n2 * n3;
         for (size_t gi = 0; gi < gs.size(); gi++) {
-            if (!contiguous(keys[gi], 2 * d, r, n2, n3)) throw CsdError("galois key switch: keys must be contiguous [2*d, r, n2, n3]");
+            const csd_view& k = keys[gi];
+            if (k.shape[0] < 2 * d || k.shape[1] != r || k.shape[2] != n2 || k.shape[3] != n3 || k.strides[3] != 1 ||
+                (n2 > 1 && k.strides[2] != n3) || (r > 1 && k.strides[1] != plane) || k.strides[0] < r * plane)
+                throw CsdError("galois key switch: keys must be [>= 2*d, r, n2, n3] with contiguous limb planes");
+            if (!contiguous(outs[gi], outs[gi].shape[0], r, n2, n3) || outs[gi].shape[0] < o_rows)
+                throw CsdError("galois key switch: outputs must be contiguous [rows, r, n2, n3] tensors covering every element");
             auto perms = galois_perms(gs[gi]);
             csd_galois_keyswitch_launch(ctx, batch, d, r, n2, n3, (const int64_t*)parts.ptr, (const int64_t*)c0.ptr,
-                                        (const int64_t*)keys[gi].ptr, perms.first->dperm, perms.second->dperm,
-                                        (int64_t*)out.ptr + (int64_t)gi * per_g, m.data(), full0, full1);
+                                        (const int64_t*)k.ptr, k.strides[0], perms.first->dperm, perms.second->dperm,
+                                        (int64_t*)outs[gi].ptr, rows, m.data(), full0, full1);
         }
+    }
+
+    void galois_keyswitch(const std::vector<size_t>& base, int64_t batch, int64_t d, const csd_view& parts, const csd_view& c0,
+                          const std::vector<uint64_t>& gs, const csd_view* keys, const csd_view& out) {
+        int64_t r = (int64_t)base.size(), n2 = (int64_t)f->deg_lp, n3 = (int64_t)f->deg_k;
+        csd_view e = make_view(out.ptr, (int64_t)gs.size() * batch * 2, r, n2, n3);
+        for (int t = 0; t < 4; t++)
+            if (out.shape[t] != e.shape[t] || (e.shape[t] > 1 && out.strides[t] != e.strides[t]))
+                throw CsdError("galois key switch: out must be a contiguous [ngal*batch*2, r, n2, n3] tensor");
+        if (c0.shape[0] != batch) throw CsdError("galois key switch: c0 must be a contiguous [batch, r, n2, n3] tensor");
+        for (size_t gi = 0; gi < gs.size(); gi++) {
+            const csd_view& k = keys[gi];
+            csd_view ek = make_view(k.ptr, 2 * d, r, n2, n3);
+            for (int t = 0; t < 4; t++)
+                if (k.shape[t] != ek.shape[t] || (ek.shape[t] > 1 && k.strides[t] != ek.strides[t]))
+                    throw CsdError("galois key switch: keys must be contiguous [2*d, r, n2, n3]");
+        }
+        std::vector<csd_view> outs;
+        for (size_t gi = 0; gi < gs.size(); gi++)
+            outs.push_back(sub(out, 0, (int64_t)gi * batch * 2, (int64_t)(gi + 1) * batch * 2, 8));
+        GksRows rows{batch, 0, 1, 0, 2, 1};
+        galois_keyswitch_rows(base, batch, d, parts, c0, gs, keys, outs.data(), rows);
     }
 
     // The same key switch with the keys in pinned host memory (the role of the CPU build's
@@ -1217,13 +1256,17 @@ extern "C" int csd_field_gal_to_shift(const char* name, uint64_t g, int64_t* shi
     CSD_CATCH
 }
 
-// required_galois_elements (bootstrap/homcrt.rs:336-349): per prime p | m_K, the generator of the
-// inertia group I_p (field.rs:200-213) and the baby-step/giant-step split of the cyclic
-// transform of length |J_p| = (p - 1) / |I_p ∩ Gal(M|L)| (homcrt.rs:320-334)
-static std::vector<uint64_t> homcrt_galois_elements(const Field& f, std::vector<uint64_t>* orders = nullptr) {
+// Per prime p | m_K (field.rs:217-243, m_k_factors field.rs:271): the generator i_p_gen of the
+// inertia group I_p (the smallest generator of (Z/pZ)*, embedded as 1 mod m/p) and the length
+// |J_p| = (p - 1) / |I_p ∩ Gal(M|L)| of the cyclic transform factor (homcrt.rs:79-81)
+struct HomcrtFactor {
+    uint64_t p, gen, len;
+};
+
+static std::vector<HomcrtFactor> homcrt_factors(const Field& f) {
     std::unordered_map<uint64_t, bool> ml;
     for (uint64_t h : f.gal_ml) ml[h] = true;
-    std::vector<uint64_t> res;
+    std::vector<HomcrtFactor> res;
     for (uint64_t p : f.m_factors) {
         if (p == f.m_prime && f.m_prime != 1) continue;
         std::vector<uint64_t> pf;
@@ -1246,7 +1289,17 @@ static std::vector<uint64_t> homcrt_galois_elements(const Field& f, std::vector<
             if (ml.count(x)) hp++;
             x = mulmod_u(x, ipg, f.m);
         }
-        uint64_t len = (p - 1) / hp;
+        res.push_back({p, ipg, (p - 1) / hp});
+    }
+    return res;
+}
+
+// required_galois_elements (bootstrap/homcrt.rs:336-349): per factor the baby-step/giant-step
+// split of its length (homcrt.rs:320-324), the baby-step powers of i_p_gen and the giant step
+static std::vector<uint64_t> homcrt_galois_elements(const Field& f, std::vector<uint64_t>* orders = nullptr) {
+    std::vector<uint64_t> res;
+    for (const HomcrtFactor& fac : homcrt_factors(f)) {
+        const uint64_t len = fac.len, ipg = fac.gen;
         if (orders) orders->push_back(len);
         if (len <= 1) continue;
         uint64_t best = 1, best_cost = UINT64_MAX;
@@ -1264,6 +1317,19 @@ static std::vector<uint64_t> homcrt_galois_elements(const Field& f, std::vector<
     std::sort(res.begin(), res.end());
     res.erase(std::unique(res.begin(), res.end()), res.end());
     return res;
+}
+
+extern "C" int csd_field_homcrt_factors(const char* name, uint64_t* p, uint64_t* gen, uint64_t* len, size_t cap, size_t* n) {
+    CSD_TRY auto f = make_field(name);
+    auto facs = homcrt_factors(*f);
+    if (facs.size() > cap) throw CsdError("buffer too small");
+    for (size_t i = 0; i < facs.size(); i++) {
+        p[i] = facs[i].p;
+        gen[i] = facs[i].gen;
+        len[i] = facs[i].len;
+    }
+    *n = facs.size();
+    CSD_CATCH
 }
 
 extern "C" int csd_field_homcrt_galois_elements(const char* name, uint64_t* out, size_t cap, size_t* n, uint64_t* jp_orders,
@@ -1399,6 +1465,21 @@ extern "C" int csd_ring_pb_to_crt(csd_ring* ring, const csd_view* data, const si
     CSD_CATCH
 }
 
+// The L' half of pb_to_crt_tensor along dim 2 (ntt_mul with the L' CRT kernel, then intt) on a
+// [count, r, deg_L', w] tensor of any width w: what lprime_element_tensor (ring.rs:1227-1250)
+// applies to the [1, r, deg_L', 1] montgomery-form coefficients of an element of L'.
+extern "C" int csd_ring_lprime_transform(csd_ring* ring, const csd_view* data, const size_t* base, size_t r) {
+    CSD_TRY check_view(data, "lprime_transform");
+    auto b = to_vec(base, r);
+    TransformCache& c = ring->transform_cache(b);
+    if (data->shape[1] != (int64_t)r || data->shape[2] != (int64_t)ring->f->deg_lp)
+        throw CsdError("lprime_transform: tensor shape doesn't match [count, r, deg_L', .]");
+    csd_view klp = make_view(c.k_lp->p, 1, (int64_t)r, (int64_t)ring->f->deg_lp, 1);
+    ok(csd_hal_ntt_mul_mont64_dim(ring->ctx, data, c.plans_lp.data(), r, CSD_DIM2, &klp, c.mods.data(), r));
+    ok(csd_hal_intt_mont64_dim(ring->ctx, data, c.plans_lp.data(), r, CSD_DIM2));
+    CSD_CATCH
+}
+
 extern "C" int csd_ring_crt_to_pb(csd_ring* ring, const csd_view* data, const size_t* base, size_t r) {
     CSD_TRY check_view(data, "crt_to_pb");
     ring->transform(*data, to_vec(base, r), true);
@@ -1454,6 +1535,17 @@ extern "C" int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, siz
     CSD_TRY if (batch == 0 || digits == 0 || ngal == 0) throw CsdError("galois key switch: empty batch, digits or rotations");
     ring->galois_keyswitch(to_vec(base, r), (int64_t)batch, (int64_t)digits, *parts, *c0, std::vector<uint64_t>(gs, gs + ngal),
                            keys, *out);
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_galois_keyswitch_rows(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t group,
+                                              size_t digits, const csd_view* parts, const csd_view* c0, int64_t c0_hi,
+                                              int64_t c0_lo, const uint64_t* gs, size_t ngal, const csd_view* keys,
+                                              const csd_view* outs, int64_t o_hi, int64_t o_lo, int64_t o_s) {
+    CSD_TRY if (batch == 0 || digits == 0 || ngal == 0 || group == 0) throw CsdError("galois key switch: empty batch, digits or rotations");
+    GksRows rows{(int64_t)group, c0_hi, c0_lo, o_hi, o_lo, o_s};
+    ring->galois_keyswitch_rows(to_vec(base, r), (int64_t)batch, (int64_t)digits, *parts, *c0, std::vector<uint64_t>(gs, gs + ngal),
+                                keys, outs, rows);
     CSD_CATCH
 }
 

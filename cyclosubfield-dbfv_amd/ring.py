@@ -42,6 +42,15 @@ def homcrt_galois_elements(name):
     return [out[i] for i in range(n.value)], [jp[i] for i in range(njp.value)]
 
 
+def homcrt_factors(name):
+    """(p, i_p_gen, |J_p|) per prime p | m_K (field.rs:217-243, 271): the cyclic factors of the
+    homomorphic CRT transform (bootstrap/homcrt.rs:29-40)."""
+    p, g, n = (ctypes.c_uint64 * 16)(), (ctypes.c_uint64 * 16)(), (ctypes.c_uint64 * 16)()
+    cnt = ctypes.c_size_t()
+    check(lib().csd_field_homcrt_factors(name.encode(), p, g, n, ctypes.c_size_t(16), ctypes.byref(cnt)))
+    return [(p[i], g[i], n[i]) for i in range(cnt.value)]
+
+
 def sample_primes(name, log2_q, t_bits):
     out = (ctypes.c_int64 * 128)()
     n, il = ctypes.c_size_t(), ctypes.c_size_t()
@@ -188,6 +197,48 @@ class RingContext:
         check(lib().csd_ring_galois_keyswitch_streamed(self.handle, b, n, ctypes.c_size_t(batch), ctypes.c_size_t(digits),
                                                        ctypes.byref(parts.view()), ctypes.byref(c0.view()), garr,
                                                        ctypes.c_size_t(len(gs)), karr, ctypes.byref(out.view())))
+
+    def galois_keyswitch_rows(self, base, batch, group, digits, parts, c0, c0_rows, gs, keys, outs, out_rows):
+        """csd_ring_galois_keyswitch_rows: the batched key switch with element e = h group + l
+        reading c0 row h c0_rows[0] + l c0_rows[1] and writing row s of rotation t to row
+        h out_rows[0] + l out_rows[1] + s out_rows[2] of outs[t]; keys may be restricted views."""
+        b, n = _sizes(base)
+        garr = (ctypes.c_uint64 * len(gs))(*gs)
+        karr = (type(parts.view()) * len(keys))(*[k.view() for k in keys])
+        oarr = (type(parts.view()) * len(outs))(*[o.view() for o in outs])
+        check(lib().csd_ring_galois_keyswitch_rows(self.handle, b, n, ctypes.c_size_t(batch), ctypes.c_size_t(group),
+                                                   ctypes.c_size_t(digits), ctypes.byref(parts.view()),
+                                                   ctypes.byref(c0.view()), ctypes.c_int64(c0_rows[0]),
+                                                   ctypes.c_int64(c0_rows[1]), garr, ctypes.c_size_t(len(gs)), karr, oarr,
+                                                   ctypes.c_int64(out_rows[0]), ctypes.c_int64(out_rows[1]),
+                                                   ctypes.c_int64(out_rows[2])))
+
+    def lprime_element_tensor(self, base, coeffs):
+        """lprime_element_tensor (ring.rs:1227-1250): the element of L' with period-basis
+        coefficients `coeffs` (basis induced by M) as a [1, r, deg_L', 1] CRT-basis factor tensor:
+        mont((c rem p) as i64) per limb and coefficient, then the L' half of the CRT transform on
+        the device. Kept per (base, coefficients), like the other constant tensors."""
+        coeffs = tuple(int(c) for c in coeffs)
+        if len(coeffs) != self.info["deg_lprime"]:
+            raise ValueError("an element of L' has deg_L' coefficients")
+        key = ("lprime", tuple(base), coeffs)
+        hit = self._consts.get(key)
+        if hit is not None:
+            return hit
+        vals = []
+        for i in base:
+            p = self.primes[i]
+            for c in coeffs:
+                x = abs(c) % p
+                x = -x if c < 0 else x  # Rust's remainder keeps the sign of c
+                m = (abs(x) << 64) % p
+                vals.append(-m if x < 0 else m)
+        t = self.backend.hal_allocate_i64([1, len(base), self.info["deg_lprime"], 1])
+        self.backend.hal_import(t, np.array(vals, dtype=np.int64))
+        b, n = _sizes(base)
+        check(lib().csd_ring_lprime_transform(self.handle, ctypes.byref(t.view()), b, n))
+        self._consts[key] = t
+        return t
 
     def fill_uniform(self, dst, base, seed):
         """Synthetic residues uniform in [0, p) per limb (splitmix64 of the flat index)."""

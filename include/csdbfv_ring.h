@@ -29,6 +29,11 @@ int csd_field_gal_to_shift(const char* field_name, uint64_t g, int64_t* shifts, 
  * and the cyclic transform lengths |J_p| */
 int csd_field_homcrt_galois_elements(const char* field_name, uint64_t* out, size_t cap, size_t* n, uint64_t* jp_orders,
                                      size_t jp_cap, size_t* n_jp);
+/* Per prime p | m_K (FieldDescriptor::m_k_factors, field.rs:271, in that order): p, the
+ * generator i_p_gen of the inertia group (PrimeInfo, field.rs:217-243) and the length
+ * |J_p| = PrimeInfo::j_p_order (field.rs:123) of the cyclic factor of the homomorphic CRT
+ * transform (CRTTransformFactor, bootstrap/homcrt.rs:29-40). */
+int csd_field_homcrt_factors(const char* field_name, uint64_t* p, uint64_t* gen, uint64_t* len, size_t cap, size_t* n);
 /* the prime sampler of RingContext::new (ring.rs:136-183, 620-650): initial base and the
  * extended pool, no device work. */
 int csd_sample_primes(const char* field_name, size_t log2_q_lo, size_t log2_q_hi, size_t t_bits, int64_t* out,
@@ -133,6 +138,22 @@ int csd_ring_apply_galois(csd_ring* ring, const csd_view* dst, const csd_view* s
 int csd_ring_galois_keyswitch(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t digits,
                               const csd_view* parts, const csd_view* c0, const uint64_t* gs, size_t ngal,
                               const csd_view* keys, const csd_view* out);
+/* The same key switch over a batch laid out in rows (no reference counterpart; the layout of
+ * hom_cyclic_transform_dbfv's baby steps, bootstrap/homcrt.rs:380-392, run for all digits and
+ * ciphertexts at once): element e = h group + l (0 <= l < group) takes its gadget digits from
+ * parts[e*digits + i] (contiguous), its c0 from row h c0_hi + l c0_lo of c0 (contiguous rows) and
+ * writes row s of rotation t to row h o_hi + l o_lo + s o_s of outs[t] (contiguous rows). keys[t]
+ * is a [>= 2*digits, r, ., .] view whose rows 2i + s may be those of a key over a longer base
+ * (KeySwitchKey restricted with restrict_base, bfv.rs:825): any row stride, limbs as contiguous
+ * planes. Bit-identical to csd_ring_galois_keyswitch element by element. */
+int csd_ring_galois_keyswitch_rows(csd_ring* ring, const size_t* base, size_t r, size_t batch, size_t group,
+                                   size_t digits, const csd_view* parts, const csd_view* c0, int64_t c0_hi, int64_t c0_lo,
+                                   const uint64_t* gs, size_t ngal, const csd_view* keys, const csd_view* outs, int64_t o_hi,
+                                   int64_t o_lo, int64_t o_s);
+/* The L' half of pb_to_crt_tensor along dim 2 (ring.rs:1118-1141: ntt_mul with the L' CRT
+ * kernel, then intt) on a [count, r, deg_L', w] tensor; with w = 1 and montgomery-form
+ * coefficients it is lprime_element_tensor (ring.rs:1227-1250). */
+int csd_ring_lprime_transform(csd_ring* ring, const csd_view* data, const size_t* base, size_t r);
 /* split_overflow (dbfv.rs:893-948) of the period-basis tensor x ([count, r, n2, n3], Montgomery
  * form, |x| <= x_bound p) in one pass over the coefficients: x becomes x_lo in place and overflow
  * ([count, limbs, n2, n3], limbs 1 for log2_r <= 62, else 2) receives x_hi, with

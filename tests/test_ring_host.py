@@ -121,3 +121,17 @@ def test_floordiv_schedule_matches_oracle():
         got = pkg.digitex.floordiv_schedule(field, pkg.dbfv.DBFVModulus(*init), pkg.dbfv.DBFVModulus(*final), r)
         want = OX.floordiv_schedule(R.FieldDescriptor(field), OD.DBFVModulus(*init), OD.DBFVModulus(*final), r)
         assert got == want, field
+
+
+@pytest.mark.parametrize("name", ["field_18", "field_192", "field_36864", "field_49152"])
+def test_homcrt_factors_match_oracle(name):
+    """csd_field_homcrt_factors and the key set of the homomorphic CRT transform (homcrt.rs:336-349)
+    against the oracle's per-prime data (field.rs:217-243)"""
+    from oracle import homcrt as OH
+
+    pkg = load()
+    f = R.FieldDescriptor(name)
+    assert pkg.ring.homcrt_factors(name) == OH.homcrt_factors(f)
+    want = sorted({g for _, gen, j in OH.homcrt_factors(f) for g in OH.factor_galois_elements(f.m, gen, j)})
+    assert pkg.homcrt.required_galois_elements(name) == want
+    assert pkg.ring.homcrt_galois_elements(name)[0] == want
