@@ -10,3 +10,7 @@ from . import dbfv  # noqa: E402,F401
 from . import digitex  # noqa: E402,F401
 from . import mfma_transform  # noqa: E402,F401
 from . import homcrt  # noqa: E402,F401
+from . import field  # noqa: E402,F401
+from . import galoisring  # noqa: E402,F401
+from . import keys  # noqa: E402,F401
+from . import bootstrap  # noqa: E402,F401

@@ -1319,6 +1319,36 @@ static std::vector<uint64_t> homcrt_galois_elements(const Field& f, std::vector<
     return res;
 }
 
+extern "C" int csd_field_data(const char* name, uint64_t* m, uint64_t* m_prime, uint64_t* mp_gens, size_t* n_mp,
+                              uint64_t* mk_gens, size_t* n_mk, uint64_t* cyc_gens, uint64_t* cyc_orders, size_t* n_cyc,
+                              size_t* n_lprime, size_t cap) {
+    CSD_TRY for (const FieldData& d : field_table()) {
+        if (name == nullptr || std::string(name) != d.name) continue;
+        if (d.mp_gens.size() > cap || d.mk_gens.size() > cap || d.lq_cyc.size() + d.kq_cyc.size() > cap)
+            throw CsdError("buffer too small");
+        *m = d.m;
+        *m_prime = d.m_prime;
+        std::copy(d.mp_gens.begin(), d.mp_gens.end(), mp_gens);
+        *n_mp = d.mp_gens.size();
+        std::copy(d.mk_gens.begin(), d.mk_gens.end(), mk_gens);
+        *n_mk = d.mk_gens.size();
+        size_t k = 0;
+        for (auto& c : d.lq_cyc) {
+            cyc_gens[k] = c.first;
+            cyc_orders[k++] = c.second;
+        }
+        for (auto& c : d.kq_cyc) {
+            cyc_gens[k] = c.first;
+            cyc_orders[k++] = c.second;
+        }
+        *n_cyc = k;
+        *n_lprime = d.lq_cyc.size();
+        return 0;
+    }
+    throw CsdError("unknown field descriptor");
+    CSD_CATCH
+}
+
 extern "C" int csd_field_homcrt_factors(const char* name, uint64_t* p, uint64_t* gen, uint64_t* len, size_t cap, size_t* n) {
     CSD_TRY auto f = make_field(name);
     auto facs = homcrt_factors(*f);

@@ -210,3 +210,145 @@ def slots_to_coeffs_dbfv(ring, factors, modulus, ct, gks):
 
 
 coeffs_to_slots_dbfv = slots_to_coeffs_dbfv
+
+
+# ---------------------------------------------------------------------------------------------
+# the transform's coefficients (host set-up, homcrt.rs:29-176)
+
+class CRTTransformFactor:
+    """CRTTransformFactor::new (homcrt.rs:70-152): the coefficients u_p (slots to coefficients) and
+    inv_u_p of the J_p-cyclic factor for the prime p | m_K, as elements of L' modulo t (period-basis
+    coefficients w.r.t. the basis induced by M, balanced), found as Gauss sums in GR(2^e, D),
+    D = lcm(f_p, ord_m'(2)), and pulled back to L' by solving against the period basis of L' under
+    every embedding of L' (one block of D rows per prime of L' above 2)."""
+
+    def __init__(self, t, p, field, u_p=None, inv_u_p=None, gen_jp=None):
+        if u_p is not None:  # reduce_t
+            self.p, self.t, self.u_p, self.inv_u_p, self.gen_jp = p, t, u_p, inv_u_p, gen_jp
+            return
+        from .galoisring import GaloisRing, solve_mod_2e
+        from .field import inv_mod
+
+        if t < 2 or t & (t - 1):
+            raise ValueError("t must be a power of two")
+        e = (t - 1).bit_length()
+        m = field.m
+        info = field.prime_info(p)
+        n2 = field.deg_lprime
+        fo = field.mprime_frobenius_order()
+        import math
+
+        d = info.f_p * fo // math.gcd(info.f_p, fo)
+        gr = GaloisRing(e, d)
+        rou_mp = gr.root_of_unity(field.m_prime)
+        scale = field.scale_mprime_to_m_period_basis()
+        embeddings = field.lprime_embedding_transversal()
+        rows = len(embeddings) * d
+        matrix = [0] * (rows * n2)
+        mp_pows = [gr.one()]
+        for _ in range(1, max(field.m_prime, 1)):
+            mp_pows.append(gr.mul(mp_pows[-1], rou_mp))
+        for block, gamma in enumerate(embeddings):
+            for j, g in enumerate(field.lprime_q_transversal()):
+                period = gr.zero()
+                for h in field.gal_mprime_lprime:
+                    conj = gamma * g % m
+                    power = conj * h % m % field.m_prime
+                    period = gr.add(period, mp_pows[power])
+                period = gr.mul_scalar(period, scale)
+                for i in range(d):
+                    matrix[(block * d + i) * n2 + j] = period[i]
+        rou_p = gr.root_of_unity(p)
+        p_pows = [gr.one()]
+        for _ in range(1, p):
+            p_pows.append(gr.mul(p_pows[-1], rou_p))
+        gen = info.i_p_gen
+        gen_inv = inv_mod(gen, m)
+        jp = info.j_p_order()
+        hp = len(info.h_p)
+
+        def orbit_sum(base, i, conjugate):
+            pg = pow(base, i, m)
+            step = m - 1 if conjugate else 1
+            s = gr.zero()
+            for h in info.h_p:
+                el = h * pg % m * step % m
+                s = gr.add(s, p_pows[el % p])
+            return s
+
+        p_inv = gr.inv(gr.from_scalar(p))
+        rhs = [0] * (rows * 2 * jp)
+
+        def set_rhs(col, coeff):
+            for block in range(len(embeddings)):
+                for row in range(d):
+                    rhs[(block * d + row) * 2 * jp + col] = coeff[row]
+
+        for i in range(jp):
+            set_rhs(i, orbit_sum(gen_inv, i, False))
+            c = gr.sub(orbit_sum(gen, i, True), gr.from_scalar(hp))
+            set_rhs(jp + i, gr.mul(c, p_inv))
+        sol = solve_mod_2e(e, rows, n2, 2 * jp, matrix, rhs)
+
+        def extract(col):
+            return [gr.smallest_lift(sol[c * 2 * jp + col]) for c in range(n2)]
+
+        self.p, self.t, self.gen_jp = p, t, gen
+        self.u_p = [extract(i) for i in range(jp)]
+        self.inv_u_p = [extract(jp + i) for i in range(jp)]
+
+    def reduce_t(self, new_t):
+        """homcrt.rs:164-176"""
+        if self.t % new_t:
+            raise ValueError("new t must divide t")
+
+        def red(cs):
+            return [[D.cmod(c, new_t) for c in v] for v in cs]
+
+        return CRTTransformFactor(new_t, self.p, None, red(self.u_p), red(self.inv_u_p), self.gen_jp)
+
+    def hom_cost(self):
+        """homcrt.rs:180-185: (hoisted, unhoisted) automorphisms of one homomorphic evaluation"""
+        b, g = bsgs_split(len(self.u_p))
+        return max(b - 2, 0), g - 1 + (1 if b > 1 else 0)
+
+
+class UnsplitCRTTransform:
+    """UnsplitCRTTransform (homcrt.rs:214-304): one factor per prime p | m_K, in m_k_factors order"""
+
+    def __init__(self, t, field, factors=None):
+        self.t = t
+        self.factors = factors if factors is not None else [CRTTransformFactor(t, p, field) for p in field.m_k_factors()]
+
+    def reduce_t(self, new_t):
+        return UnsplitCRTTransform(new_t, None, [f.reduce_t(new_t) for f in self.factors])
+
+    def forward(self):
+        """(generator, coefficients) per factor for slots_to_coeffs_dbfv"""
+        return [(f.gen_jp, f.u_p) for f in self.factors]
+
+    def inverse(self):
+        """(generator, coefficients) per factor for coeffs_to_slots_dbfv"""
+        return [(f.gen_jp, f.inv_u_p) for f in self.factors]
+
+    def hom_cost(self):
+        h = u = 0
+        for f in self.factors:
+            a, b = f.hom_cost()
+            h, u = h + a, u + b
+        return h, u
+
+
+def apply_cyclic_transform_plain(ring, data, base, automorphism, coeffs):
+    """apply_cyclic_transform_plain (homcrt.rs:188-211) on the device: sum_i coeffs[i] sigma^i(x)
+    of the CRT-basis elements `data` ([count, r, ., .], bound 1), reduced (bound 1)"""
+    count = data.shape[0]
+    acc = ring.allocate_wide(count, base)
+    current = data
+    for i, c in enumerate(coeffs):
+        acc.fma(range(count), current, 1, ring.lprime_element_tensor(base, c), 1)
+        if i + 1 < len(coeffs):
+            nxt = ring.allocate_uninit(count, base)
+            ring.apply_galois(nxt, current, automorphism)
+            current = nxt
+    return ring.reduce_wide(acc)

@@ -29,6 +29,12 @@ int csd_field_gal_to_shift(const char* field_name, uint64_t g, int64_t* shifts, 
  * and the cyclic transform lengths |J_p| */
 int csd_field_homcrt_galois_elements(const char* field_name, uint64_t* out, size_t cap, size_t* n, uint64_t* jp_orders,
                                      size_t jp_cap, size_t* n_jp);
+/* The descriptor data of FieldDescriptorData (field.rs:529-769): m, m', the generators of
+ * Gal(M'|L') and Gal(M_K|K) in (Z/mZ)*, and the cyclic decomposition of Gal(L|Q) (generator,
+ * order), the first n_lprime factors being those of L'. Every list holds at most cap entries. */
+int csd_field_data(const char* field_name, uint64_t* m, uint64_t* m_prime, uint64_t* mp_gens, size_t* n_mp,
+                   uint64_t* mk_gens, size_t* n_mk, uint64_t* cyc_gens, uint64_t* cyc_orders, size_t* n_cyc,
+                   size_t* n_lprime, size_t cap);
 /* Per prime p | m_K (FieldDescriptor::m_k_factors, field.rs:271, in that order): p, the
  * generator i_p_gen of the inertia group (PrimeInfo, field.rs:217-243) and the length
  * |J_p| = PrimeInfo::j_p_order (field.rs:123) of the cyclic factor of the homomorphic CRT
