@@ -5,10 +5,11 @@ the L'-correction and the trace to K, the homomorphic evaluation of the noisy de
 the dBFV encryption of the sparse key, coefficients-to-slots and the floor division that removes
 the noise (``digitex``).
 
-Every ring operation runs on the device, batched over the ciphertexts where the reference's steps
-are per ciphertext; the two host steps of the reference stay on the host: decoding the two
-components after the trace (decode_pb_scaled, an exact CRT lift per coefficient) and encoding
-the decoded plaintexts for the homomorphic decryption."""
+Every step runs on the device, batched over the ciphertexts where the reference's steps are per
+ciphertext. The reference's two host steps are device kernels here too: decoding both components
+after the trace (decode_pb_scaled, exact: csd_ring_decode_scaled) and encoding the decoded
+plaintexts, the second as its balanced base-b digit vectors (csd_ring_encode_digits), for the
+homomorphic decryption; nothing returns to the host inside a bootstrap except the digit count."""
 import math
 
 import numpy as np
@@ -145,28 +146,16 @@ def key_switch(ring, c0_crt, c1_pb, base, ksk):
     return ring.reduce_wide(acc)
 
 
-def dbfv_acc_fma_plain_batch(ring, acc, lhs, rhs_per_ct):
-    """dbfv_acc_fma_plain (dbfv.rs:262-289) of one dBFV ciphertext `lhs` (a batch of 1, CRT basis)
-    with a plaintext per ciphertext of the accumulator's batch: acc digit k of ciphertext c +=
-    sum_(i + j = k) lhs digit i * (digit j of rhs_c), rhs_c in balanced base-b digits encoded and
-    transformed to CRT basis. Every ciphertext of the batch gets its own plaintext's digits."""
-    m = acc.modulus
+def dbfv_acc_fma_tensors_batch(ring, acc, lhs, tensors):
+    """dbfv_acc_fma_tensors (dbfv.rs:193-224) of one dBFV ciphertext `lhs` (a batch of 1, CRT basis)
+    with per-ciphertext factor digits: tensors[j] is [count, r, ., .], row c the j-th digit of
+    ciphertext c's factor, so acc digit k of ciphertext c += sum_(i + j = k) lhs digit i *
+    tensors[j][c]"""
     n = acc.count
-    base = acc.rns_base()
-    digit_vecs = [H.balanced_digits_vec(rhs, m.b(), m.digits()) for rhs in rhs_per_ct]
-    nd = max(len(v) for v in digit_vecs)
-    if nd == 0 or lhs.digits() == 0:
-        return
-    tensors = []
-    zero = [0] * len(rhs_per_ct[0])
-    for j in range(nd):
-        t = ring.allocate_uninit(n, base)
-        for c, v in enumerate(digit_vecs):
-            ring.backend.hal_assign(t[c:c + 1], K.encode_pb_unscaled(ring, base, m.t(), v[j] if j < len(v) else zero))
-        ring.pb_to_crt_tensor(t, base)
-        tensors.append(t)
     ld = lhs.digits()
-    new_digits = min(acc.capacity(), ld + nd - 1)
+    if ld == 0 or not tensors:
+        return
+    new_digits = min(acc.capacity(), ld + len(tensors) - 1)
     for i in range(ld):
         for j, t in enumerate(tensors):
             k = i + j
@@ -234,19 +223,21 @@ def bootstrap_bfv_dbfv(ring, field, crt_transform, modulus, ct, rk, gks, encaps,
     be.hal_mul_mont64(ks, ring.lprime_element_tensor(ske, field.lprime_trace_one_coeffs()), ring.mods_for(ske))
     trace_to_k(ring, field, ks, ske)
     ring.crt_to_pb_tensor(ks, ske)
-    plain0 = [K.decode_pb_scaled(ring, inter_t, ks[c:c + 1], ske) for c in range(n)]
-    plain1 = [K.decode_pb_scaled(ring, inter_t, ks[n + c:n + c + 1], ske) for c in range(n)]
+    # decode_pb_scaled of both components (rows c: c0, n + c: c1) on the device, exact
+    plain = ring.decode_scaled(ks, ske, log2_inter_t)
     tick("keyswitch_trace_decode")
 
-    # the noisy decryption c0 + c1 s_sparse, plus the rounding constant
+    # the noisy decryption c0 + c1 s_sparse, plus the rounding constant; the plaintexts are encoded
+    # over the initial base on the device (c1 as its balanced base-b digit vectors)
     inter = D.DBFVModulus.new_incomplete(modulus.log2_b, log2_inter_t)
+    planes, nd = ring.encode_digits(plain[n:2 * n], base, modulus.log2_b, inter.digits())
     macc = D.dbfv_allocate_accumulator(ring, inter, base, "crt", n)
-    dbfv_acc_fma_plain_batch(ring, macc, encaps.enc, plain1)
+    if nd:
+        ring.pb_to_crt_tensor(planes[0:nd * n], base)
+        dbfv_acc_fma_tensors_batch(ring, macc, encaps.enc, [planes[j * n:(j + 1) * n] for j in range(nd)])
     noisy = D.dbfv_finish_accumulator(ring, macc)
     delta = D.rounded_div(ring.modulus_of(base), inter_t)
-    const = ring.allocate_uninit(n, base)
-    for c in range(n):
-        be.hal_assign(const[c:c + 1], K.encode_pb_unscaled(ring, base, inter_t, plain0[c]))
+    const, _ = ring.encode_digits(plain[0:n], base)
     ring.pb_to_crt_tensor(const, base)
     wide = ring.allocate_wide(n, base)
     wide.fma(range(n), const, 1, ring.scalar_tensor_big(base, delta), 1)

@@ -213,3 +213,233 @@ extern "C" int csd_dbfv_split_overflow(csd_ctx* ctx, const csd_view* x, const cs
     HIPCHECK(hipGetLastError());
     CSD_CATCH
 }
+
+// ---------------------------------------------------------------------------------------------
+// Decoding and encoding of plaintexts on the device (the host steps of bootstrap_bfv_dbfv,
+// bootstrap/mod.rs:378-395: decode_pb_scaled of both components after the trace, then
+// encode_pb_unscaled of the first and of the balanced base-b digits of the second).
+//
+// decode_pb_scaled (bfv.rs:262-274) is round(t x / Q) mod t of the smallest lift x of every
+// coefficient. With t = 2^e and y_j = x (Q/p_j)^-1 mod p_j, x = sum_j y_j (Q/p_j) - k Q, so
+//   t x / Q = sum_j y_j 2^e / p_j  (mod t),   y_j 2^e = q_j p_j + r_j,
+//   round(t x / Q) = sum_j q_j + round(sum_j r_j / p_j)  (mod t).
+// r_j = x c_j mod p_j with c_j = (Q/p_j)^-1 2^e mod p_j (one Montgomery product of the
+// montgomery-form residue), q_j = -r_j p_j^-1 mod 2^e (q_j p_j = y_j 2^e - r_j), and
+// round(sum_j r_j / p_j) counts the k with 2 sum_j r_j (Q/p_j) >= (2k + 1) Q, compared exactly in
+// four 64-bit words (2 sum r_j Q/p_j and (2k + 1) Q are integers of different parity, so no
+// coefficient sits on a tie). The result is exact for every input, as the reference's big-integer
+// decoding is.
+
+#define DEC_MAX_R 4
+
+struct DecArgs {
+    int64_t count, n2, n3;
+    int64_t s0, s1, s2;  // strides of the source (dims 0..2; dim 3 contiguous)
+    const int64_t* src;
+    i128* out;  // [count, 1, n2, n3], contiguous
+    int r, e;
+    int64_t c[DEC_MAX_R];
+    u128 pinv[DEC_MAX_R];
+    uint64_t M[DEC_MAX_R][4];  // Q / p_j, little-endian words
+    uint64_t T[DEC_MAX_R][4];  // (2k + 1) Q for k < r
+    CsdMod m[DEC_MAX_R];
+};
+
+__device__ __forceinline__ bool words_geq(const uint64_t* a, const uint64_t* b) {
+    for (int w = 3; w >= 0; w--)
+        if (a[w] != b[w]) return a[w] > b[w];
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_decode_scaled(DecArgs a) {
+    const int64_t plane = a.n2 * a.n3;
+    const int64_t pos = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pos >= a.count * plane) return;
+    const int64_t i = pos / plane, rest = pos - i * plane, k = rest / a.n3, l = rest - k * a.n3;
+    const int64_t* x = a.src + i * a.s0 + k * a.s2 + l;
+    u128 q = 0;
+    uint64_t W[4] = {0, 0, 0, 0};
+    for (int j = 0; j < a.r; j++) {
+        const CsdMod& m = a.m[j];
+        int64_t rj = csd_mont_mul(m, x[j * a.s1], a.c[j]);
+        if (rj < 0) rj += m.N;
+        if (rj >= m.N) rj -= m.N;
+        q -= (u128)(uint64_t)rj * a.pinv[j];
+        const uint64_t mult = 2 * (uint64_t)rj;
+        u128 carry = 0;
+        for (int w = 0; w < 4; w++) {
+            u128 t = (u128)a.M[j][w] * mult + W[w] + carry;
+            W[w] = (uint64_t)t;
+            carry = t >> 64;
+        }
+    }
+    int rounded = 0;
+    for (int kk = 0; kk < a.r; kk++) rounded += words_geq(W, a.T[kk]) ? 1 : 0;
+    q += (u128)rounded;
+    const u128 mask = ((u128)1 << a.e) - 1;
+    u128 v = q & mask;
+    i128 out = (i128)v;
+    if (v > ((u128)1 << (a.e - 1))) out -= (i128)((u128)1 << a.e);  // balanced: (-t/2, t/2]
+    a.out[pos] = out;
+}
+
+// encode_pb_unscaled (bfv.rs:226-235) with the reference's representatives: mont_repr((c rem p) as
+// i64, p) is c R reduced with the sign of c (Rust's remainder), |.| < p.
+__device__ __forceinline__ int64_t encode_repr(const CsdMod& m, i128 c) {
+    const i128 mag = c < 0 ? -c : c;
+    int64_t z = csd_to_mont128(m, mag);
+    if (z < 0) z += m.N;
+    if (z >= m.N) z -= m.N;
+    return c < 0 ? -z : z;
+}
+
+struct EncArgs {
+    int64_t count, n2, n3, R, D;
+    const i128* v;  // [count, 1, n2, n3], contiguous
+    int64_t* dst;   // [D count, R, n2, n3], contiguous: plane i of element c at row i count + c
+    int log2_b;     // 0: the values themselves (one plane)
+    int* ndig;      // the largest digit count (balanced_digits_vec stops when all are zero)
+};
+
+__global__ __launch_bounds__(256) void k_encode_digits(EncArgs a, ModTable mt) {
+    const int64_t plane = a.n2 * a.n3;
+    const int64_t pos = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pos >= a.count * plane) return;
+    const int64_t c = pos / plane, p = pos - c * plane;
+    i128 cur = a.v[pos];
+    if (a.log2_b == 0) {
+        for (int64_t j = 0; j < a.R; j++) a.dst[(c * a.R + j) * plane + p] = encode_repr(mt.m[j], cur);
+        return;
+    }
+    const int64_t b = (int64_t)1 << a.log2_b;
+    int cnt = 0;
+    for (int64_t i = 0; i < a.D; i++) {
+        int64_t d = 0;
+        if (cur != 0) {
+            d = (int64_t)((u128)cur & (u128)(b - 1));  // cmod(cur, b) in (-b/2, b/2] (dbfv.rs:83-95)
+            if (d > b / 2) d -= b;
+            cur = (cur - d) >> a.log2_b;
+            cnt = (int)i + 1;
+        }
+        int64_t* o = a.dst + ((i * a.count + c) * a.R) * plane + p;
+        for (int64_t j = 0; j < a.R; j++) o[j * plane] = encode_repr(mt.m[j], (i128)d);
+    }
+    if (cnt) atomicMax(a.ndig, cnt);
+}
+
+// host multiword helpers (little-endian 64-bit words)
+static void words_mul_small(std::vector<uint64_t>& w, uint64_t f) {
+    u128 carry = 0;
+    for (auto& x : w) {
+        u128 t = (u128)x * f + carry;
+        x = (uint64_t)t;
+        carry = t >> 64;
+    }
+    if (carry) w.push_back((uint64_t)carry);
+}
+
+void csd_decode_scaled_launch(csd_ctx* ctx, const csd_view* src, const int64_t* primes, const csd_modulus* mods, size_t r,
+                              size_t log2_t, const csd_view* dst) {
+    check_view(src, "decode_scaled source");
+    check_view(dst, "decode_scaled output");
+    if (r < 1 || r > DEC_MAX_R) throw CsdError("decode_scaled: between 1 and 4 primes");
+    if (log2_t < 1 || log2_t > 126) throw CsdError("decode_scaled: t = 2^e with 1 <= e <= 126");
+    if (src->shape[1] != (int64_t)r || src->strides[3] != 1) throw CsdError("decode_scaled: source must be [count, r, n2, n3]");
+    if (dst->shape[0] != src->shape[0] || dst->shape[1] != 1 || dst->shape[2] != src->shape[2] ||
+        dst->shape[3] != src->shape[3] || dst->strides[3] != 1 || dst->strides[2] != dst->shape[3] ||
+        dst->strides[0] != dst->shape[2] * dst->shape[3])
+        throw CsdError("decode_scaled: output must be a contiguous i128 [count, 1, n2, n3] tensor");
+    DecArgs a;
+    memset(&a, 0, sizeof(a));
+    a.count = src->shape[0];
+    a.n2 = src->shape[2];
+    a.n3 = src->shape[3];
+    a.s0 = src->strides[0];
+    a.s1 = src->strides[1];
+    a.s2 = src->strides[2];
+    a.src = (const int64_t*)src->ptr;
+    a.out = (i128*)dst->ptr;
+    a.r = (int)r;
+    a.e = (int)log2_t;
+    std::vector<uint64_t> q{1};
+    for (size_t j = 0; j < r; j++) words_mul_small(q, (uint64_t)primes[j]);
+    if (q.size() > 4) throw CsdError("decode_scaled: modulus too large");
+    q.resize(4, 0);
+    for (size_t j = 0; j < r; j++) {
+        const int64_t p = primes[j];
+        std::vector<uint64_t> mj{1};
+        int64_t mj_mod_p = 1;
+        for (size_t i = 0; i < r; i++)
+            if (i != j) {
+                words_mul_small(mj, (uint64_t)primes[i]);
+                mj_mod_p = csd_host_mulmod(mj_mod_p, primes[i] % p, p);
+            }
+        mj.resize(4, 0);
+        for (int w = 0; w < 4; w++) a.M[j][w] = mj[w];
+        const int64_t kinv = csd_host_powmod(mj_mod_p, (uint64_t)(p - 2), p);
+        a.c[j] = csd_host_mulmod(kinv, csd_host_powmod(2, log2_t, p), p);
+        u128 inv = (u128)(uint64_t)p;  // p^-1 mod 2^128 by Newton's iteration
+        for (int it = 0; it < 7; it++) inv *= (u128)2 - (u128)(uint64_t)p * inv;
+        a.pinv[j] = inv;
+        a.m[j] = CsdMod{mods[j].N, mods[j].neg_inv_N, mods[j].r2, mods[j].r3};
+    }
+    for (size_t k = 0; k < r; k++) {
+        std::vector<uint64_t> t = q;
+        words_mul_small(t, 2 * k + 1);
+        if (t.size() > 4 && t[4]) throw CsdError("decode_scaled: modulus too large");
+        for (int w = 0; w < 4; w++) a.T[k][w] = t[w];
+    }
+    const int64_t total = a.count * a.n2 * a.n3;
+    if (total == 0) return;
+    if ((total + 255) / 256 > (int64_t)INT32_MAX) throw CsdError("decode_scaled: too many coefficients");
+    TimedScope ts(ctx, "decode_scaled", (double)total * (8.0 * r + 16.0));
+    hipLaunchKernelGGL(k_decode_scaled, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, a);
+    HIPCHECK(hipGetLastError());
+}
+
+int64_t csd_encode_digits_launch(csd_ctx* ctx, const csd_view* v, size_t log2_b, size_t max_digits, const csd_view* dst,
+                                 const csd_modulus* mods, size_t R) {
+    check_view(v, "encode_digits values");
+    check_view(dst, "encode_digits output");
+    const int64_t count = v->shape[0], n2 = v->shape[2], n3 = v->shape[3];
+    if (v->shape[1] != 1 || v->strides[3] != 1 || v->strides[2] != n3 || v->strides[0] != n2 * n3)
+        throw CsdError("encode_digits: values must be a contiguous i128 [count, 1, n2, n3] tensor");
+    const int64_t D = log2_b == 0 ? 1 : (int64_t)max_digits;
+    if (log2_b > 62 || D < 1) throw CsdError("encode_digits: digit base 2^1 .. 2^62 and at least one digit");
+    if (R < 1 || R > CSD_MAX_MODULI) throw CsdError("encode_digits: target base size");
+    if (dst->shape[0] != D * count || dst->shape[1] != (int64_t)R || dst->shape[2] != n2 || dst->shape[3] != n3 ||
+        dst->strides[3] != 1 || dst->strides[2] != n3 || dst->strides[1] != n2 * n3 || dst->strides[0] != (int64_t)R * n2 * n3)
+        throw CsdError("encode_digits: output must be a contiguous [planes * count, R, n2, n3] tensor");
+    EncArgs a;
+    memset(&a, 0, sizeof(a));
+    a.count = count;
+    a.n2 = n2;
+    a.n3 = n3;
+    a.R = (int64_t)R;
+    a.D = D;
+    a.v = (const i128*)v->ptr;
+    a.dst = (int64_t*)dst->ptr;
+    a.log2_b = (int)log2_b;
+    const int64_t total = count * n2 * n3;
+    if (total == 0) return 0;
+    if ((total + 255) / 256 > (int64_t)INT32_MAX) throw CsdError("encode_digits: too many coefficients");
+    int* flag = (int*)ctx->alloc_bytes(8);
+    int host = 0;
+    try {
+        csd_zero_fill(ctx, flag, 8);
+        a.ndig = flag;
+        ModTable t = make_mod_table(mods, R, (int64_t)R);
+        {
+            TimedScope ts(ctx, "encode_digits", (double)total * (16.0 + 8.0 * (double)(D * R)));
+            hipLaunchKernelGGL(k_encode_digits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, a, t);
+            HIPCHECK(hipGetLastError());
+        }
+        HIPCHECK(hipMemcpyAsync(&host, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        ctx->free_bytes(flag);
+        throw;
+    }
+    ctx->free_bytes(flag);
+    return log2_b == 0 ? 1 : host;
+}

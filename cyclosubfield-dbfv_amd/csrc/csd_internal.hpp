@@ -90,6 +90,15 @@ struct GksRows {
     int64_t grp, c0_hi, c0_lo, o_hi, o_lo, o_s;
 };
 
+// decode_pb_scaled on the device (csd_dbfv.hip): src [count, r, n2, n3] period basis over the
+// primes (r <= 4, their moduli), dst i128 [count, 1, n2, n3] = round(2^log2_t x / Q) balanced
+void csd_decode_scaled_launch(csd_ctx* ctx, const csd_view* src, const int64_t* primes, const csd_modulus* mods, size_t r,
+                              size_t log2_t, const csd_view* dst);
+// encode_pb_unscaled of i128 values (log2_b = 0) or of their balanced base-2^log2_b digits
+// (max_digits planes) over R moduli; returns the number of digit planes balanced_digits_vec keeps
+int64_t csd_encode_digits_launch(csd_ctx* ctx, const csd_view* v, size_t log2_b, size_t max_digits, const csd_view* dst,
+                                 const csd_modulus* mods, size_t R);
+
 // key: row 2i + s of key digit i at key + (2i + s) krow (a restricted key keeps its row stride)
 void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t r, int64_t n2, int64_t n3, const int64_t* parts,
                                  const int64_t* c0, const int64_t* key, int64_t krow, const uint32_t* perm2,

@@ -1510,6 +1510,28 @@ extern "C" int csd_ring_lprime_transform(csd_ring* ring, const csd_view* data, c
     CSD_CATCH
 }
 
+extern "C" int csd_ring_decode_scaled(csd_ring* ring, const csd_view* src, const size_t* base, size_t r, size_t log2_t,
+                                      const csd_view* dst) {
+    CSD_TRY auto b = to_vec(base, r);
+    std::vector<int64_t> ps;
+    for (size_t i : b) {
+        if (i >= ring->primes.size()) throw CsdError("decode_scaled: base index out of range");
+        ps.push_back(ring->primes[i]);
+    }
+    auto m = ring->mods_for(b);
+    csd_decode_scaled_launch(ring->ctx, src, ps.data(), m.data(), r, log2_t, dst);
+    CSD_CATCH
+}
+
+extern "C" int csd_ring_encode_digits(csd_ring* ring, const csd_view* values, size_t log2_b, size_t max_digits,
+                                      const csd_view* dst, const size_t* base, size_t r, int64_t* planes) {
+    CSD_TRY auto b = to_vec(base, r);
+    auto m = ring->mods_for(b);
+    int64_t n = csd_encode_digits_launch(ring->ctx, values, log2_b, max_digits, dst, m.data(), r);
+    if (planes) *planes = n;
+    CSD_CATCH
+}
+
 extern "C" int csd_ring_crt_to_pb(csd_ring* ring, const csd_view* data, const size_t* base, size_t r) {
     CSD_TRY check_view(data, "crt_to_pb");
     ring->transform(*data, to_vec(base, r), true);

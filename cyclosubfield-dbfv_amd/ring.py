@@ -240,6 +240,30 @@ class RingContext:
         self._consts[key] = t
         return t
 
+    def decode_scaled(self, src, base, log2_t, dst=None):
+        """decode_pb_scaled (bfv.rs:262-274) of every element of the period-basis tensor `src` over
+        `base` (at most 4 primes) on the device: an i128 [count, 1, ., .] tensor of round(t x / Q)
+        balanced modulo t = 2^log2_t (csd_ring_decode_scaled)"""
+        if dst is None:
+            dst = self.backend.allocate_i128_uninit([src.shape[0], 1, src.shape[2], src.shape[3]])
+        b, n = _sizes(base)
+        check(lib().csd_ring_decode_scaled(self.handle, ctypes.byref(src.view()), b, n, ctypes.c_size_t(log2_t),
+                                           ctypes.byref(dst.view())))
+        return dst
+
+    def encode_digits(self, values, base, log2_b=0, max_digits=1):
+        """encode_pb_unscaled (bfv.rs:226-235) over `base` of i128 values (log2_b = 0) or of their
+        balanced base-2^log2_b digit vectors (dbfv.rs:97-112, max_digits planes, plane i of element c
+        at row i count + c): (period-basis tensor, planes kept) (csd_ring_encode_digits)"""
+        count = values.shape[0]
+        planes = 1 if log2_b == 0 else max_digits
+        dst = self.allocate_uninit(planes * count, base)
+        b, n = _sizes(base)
+        kept = ctypes.c_int64()
+        check(lib().csd_ring_encode_digits(self.handle, ctypes.byref(values.view()), ctypes.c_size_t(log2_b),
+                                           ctypes.c_size_t(max_digits), ctypes.byref(dst.view()), b, n, ctypes.byref(kept)))
+        return dst, kept.value
+
     def fill_uniform(self, dst, base, seed):
         """Synthetic residues uniform in [0, p) per limb (splitmix64 of the flat index)."""
         mods = [self.backend.hal_create_modulus(self.primes[i]) for i in base]

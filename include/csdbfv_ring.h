@@ -156,6 +156,22 @@ int csd_ring_galois_keyswitch_rows(csd_ring* ring, const size_t* base, size_t r,
                                    size_t digits, const csd_view* parts, const csd_view* c0, int64_t c0_hi, int64_t c0_lo,
                                    const uint64_t* gs, size_t ngal, const csd_view* keys, const csd_view* outs, int64_t o_hi,
                                    int64_t o_lo, int64_t o_s);
+/* decode_pb_scaled (bfv.rs:262-274) on the device, exact: for the period-basis [count, r, n2, n3]
+ * tensor src over `base` (r <= 4 primes, montgomery form, bound <= 2) writes round(2^log2_t x / Q)
+ * balanced into (-t/2, t/2] to the contiguous i128 [count, 1, n2, n3] tensor dst, x the lift of
+ * each coefficient (element_lifts, ring.rs:1465-1480); 1 <= log2_t <= 126. The host decoding of
+ * bootstrap_bfv_dbfv (bootstrap/mod.rs:378-380), without the host round trip. */
+int csd_ring_decode_scaled(csd_ring* ring, const csd_view* src, const size_t* base, size_t r, size_t log2_t,
+                           const csd_view* dst);
+/* encode_pb_unscaled (bfv.rs:226-235) over `base` of the i128 values ([count, 1, n2, n3],
+ * contiguous, |v| < 2^100) with log2_b = 0 (dst [count, r, n2, n3]), or of their balanced
+ * base-2^log2_b digit vectors (balanced_digits_vec, dbfv.rs:97-112: max_digits planes, dst
+ * [max_digits * count, r, n2, n3], plane i of element c at row i * count + c) as dbfv_acc_fma_plain
+ * encodes them (dbfv.rs:262-289); *planes receives the number of planes balanced_digits_vec keeps
+ * (it stops once every coefficient's remaining value is zero). Representatives are those of
+ * element_from_pb_coeffs (ring.rs:1449-1461). */
+int csd_ring_encode_digits(csd_ring* ring, const csd_view* values, size_t log2_b, size_t max_digits, const csd_view* dst,
+                           const size_t* base, size_t r, int64_t* planes);
 /* The L' half of pb_to_crt_tensor along dim 2 (ring.rs:1118-1141: ntt_mul with the L' CRT
  * kernel, then intt) on a [count, r, deg_L', w] tensor; with w = 1 and montgomery-form
  * coefficients it is lprime_element_tensor (ring.rs:1227-1250). */

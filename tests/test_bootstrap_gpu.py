@@ -176,3 +176,128 @@ def test_bootstrap_field_32(gpu):
     """bootstrap/mod.rs:429-455: two packed slot values survive the bootstrap"""
     got, want = _bootstrap_case(gpu, "field_32", 30, 2, slot_values=[2025, -777])
     assert got == want
+
+
+def test_bootstrap_batch_and_a_wrong_key(gpu):
+    """three ciphertexts bootstrapped as one batch decrypt to their own values; with the
+    encapsulation of a different main key the decryption is wrong (the check is sensitive to the
+    homomorphic decryption step)"""
+    pkg, ring, f, rng, base, la, sk = _setup(gpu, "field_18", (800, 820), 20, 40)
+    K, D, H, B = pkg.keys, pkg.dbfv, pkg.homcrt, pkg.bootstrap
+    t = 1 << 15
+    modulus = D.DBFVModulus.new(5, 3)
+    inter = D.DBFVModulus.new_incomplete(modulus.log2_b, 20)
+    tr = H.UnsplitCRTTransform(inter.t(), f)
+    encaps, _ = B.create_sparse_key_encaps(ring, f, inter, sk, 4, rng, la)
+    digits = pkg.bfv.select_digits(len(base), len(base))
+    gks = {g: K.gen_gk(ring, sk, g, digits, rng, la) for g in H.required_galois_elements("field_18")}
+    rk = K.gen_rk(ring, sk, digits, rng, la)
+    values = [2025, -1, 77]
+    n = len(values)
+    cts = [K.bfv_enc(ring, base, t, K.encode_scalar(ring, v), sk, rng, la) for v in values]
+    batch = ring.allocate_uninit(2 * n, base)
+    for c, x in enumerate(cts):
+        ring.backend.hal_assign(batch[c:c + 1], x[0:1])
+        ring.backend.hal_assign(batch[n + c:n + c + 1], x[1:2])
+    res = B.bootstrap_bfv_dbfv(ring, f, tr, modulus, D.DBFVBatch(batch, base, n, "crt", 1), rk, gks, encaps)
+    for c, v in enumerate(values):
+        assert K.decode_scalar(ring, t, K.dbfv_dec(ring, modulus, res, sk, c)) == D.cmod(v, t), c
+    other = K.gen_sk(ring, rng)
+    wrong, _ = B.create_sparse_key_encaps(ring, f, inter, other, 4, rng, la)
+    res2 = B.bootstrap_bfv_dbfv(ring, f, tr, modulus, D.DBFVBatch(batch, base, n, "crt", 1), rk, gks, wrong)
+    assert K.decode_scalar(ring, t, K.dbfv_dec(ring, modulus, res2, sk, 0)) != D.cmod(2025, t)
+    ring.close()
+
+
+@pytest.mark.parametrize("nprimes,log2_t", [(3, 76), (1, 20), (2, 64), (4, 126)])
+def test_device_decode_and_encode_match_the_host(gpu, nprimes, log2_t):
+    """csd_ring_decode_scaled equals decode_pb_scaled (bfv.rs:262-274, exact host big integers) on
+    uniform residues and on lifts within +-3 of a rounding boundary (t x / Q = k + 1/2 exactly is
+    impossible, these are the nearest ones); csd_ring_encode_digits gives element_from_pb_coeffs'
+    representatives mont_repr((c rem p) as i64, p) (ring.rs:109, 1449-1461) of the values and of
+    their balanced digit vectors (dbfv.rs:97-112), and the digit count balanced_digits_vec keeps"""
+    from oracle import ring as R
+
+    pkg, ring, f, rng, base, la, sk = _setup(gpu, "field_18", (300, 320), 16, 11)
+    K, D = pkg.keys, pkg.dbfv
+    be = ring.backend
+    sub = base[:nprimes]
+    primes = [ring.primes[i] for i in sub]
+    q = ring.modulus_of(sub)
+    t = 1 << log2_t
+    n = f.deg_l
+    gen = np.random.default_rng(nprimes)
+    lifts = [int(x) for x in gen.integers(0, 1 << 62, n)]
+    lifts = [(x * (q >> 40) + x) % q for x in lifts]
+    ks = list(range(0, n, 3))
+    for idx, k in enumerate(ks):  # near (k + 1/2) Q / t
+        center = ((2 * (k + 1)) * q) // (2 * t) + (q // (2 * t))
+        lifts[k] = (center + (idx % 7) - 3) % q
+    host = np.zeros((2, nprimes, f.deg_lprime, f.deg_k), dtype=np.int64)
+    for j, p in enumerate(primes):
+        host[0, j] = np.array([R.mont_repr(x % p, p) for x in lifts], dtype=np.int64).reshape(f.deg_lprime, f.deg_k)
+    host[1] = host[0]
+    src = ring.allocate_uninit(2, sub)
+    be.hal_import(src, host)
+    got = be.export_i128(ring.decode_scaled(src, sub, log2_t))
+    got = [(int(h) << 64) + (int(lo) & ((1 << 64) - 1)) for lo, h in zip(got["lo"], got["hi"])]
+    want = K.decode_pb_scaled(ring, t, src[0:1], sub)
+    exact = [D.cmod(D.rounded_div((x - q if x > q // 2 else x) * t, q), t) for x in lifts]
+    assert want == exact
+    assert got[:n] == want and got[n:] == want
+    # encoding over the whole base, of the values and of their balanced digits (base 2^8)
+    vals = ring.decode_scaled(src, sub, log2_t)
+    if log2_t <= 100:
+        enc, kept = ring.encode_digits(vals[0:1], base)
+        assert kept == 1
+        e = be.hal_export(enc).reshape(1, len(base), f.deg_lprime, f.deg_k)
+        for j, i in enumerate(base):
+            p = ring.primes[i]
+            assert [int(v) for v in e[0, j].reshape(-1)] == [R.mont_repr(R._trunc_mod(c, p), p) for c in want]
+    dmax = (log2_t + 7) // 8
+    planes, kept = ring.encode_digits(vals, base, 8, dmax)
+    digs = pkg.homcrt.balanced_digits_vec(want, 256, dmax)
+    assert kept == len(digs)
+    e = be.hal_export(planes).reshape(dmax, 2, len(base), f.deg_lprime, f.deg_k)
+    for j in range(dmax):
+        dv = digs[j] if j < len(digs) else [0] * n
+        for jj, i in enumerate(base[:3]):
+            p = ring.primes[i]
+            assert [int(v) for v in e[j, 1, jj].reshape(-1)] == [R.mont_repr(d, p) for d in dv], (j, jj)
+    ring.close()
+
+
+def test_bootstrap_field_36864(gpu):
+    """the reference's full-size test (bootstrap/mod.rs:461-509, ignored there for its CPU time):
+    field_36864, 17 primes, (8, 8) -> intermediate 2^76, sparse weight 32 with amplification at
+    most v - 2 = 10, 27 Galois keys of 17 digits, a relinearization key of 9 digits: every one of
+    the 3072 slots of the bootstrapped encryption of 2025 holds 2025"""
+    pkg = load()
+    K, D, H, B = pkg.keys, pkg.dbfv, pkg.homcrt, pkg.bootstrap
+    ring = pkg.ring.RingContext(gpu, "field_36864", (960, 980), 76)
+    f = pkg.field.FieldDescriptor("field_36864")
+    rng = K.Rng(31)
+    base = ring.initial_rns_base()
+    la = 960.0
+    modulus = D.DBFVModulus.new(8, 8)
+    inter = D.DBFVModulus.new_incomplete(8, 76)
+    sk = K.gen_sk(ring, rng)
+    tr = H.UnsplitCRTTransform(inter.t(), f)
+    encaps, _ = B.create_sparse_key_encaps(ring, f, inter, sk, 32, rng, la, max_log2_amplification=10.0)
+    gdig = pkg.bfv.select_digits(len(base), len(base))
+    gks = {g: K.gen_gk(ring, sk, g, gdig, rng, la) for g in H.required_galois_elements("field_36864")}
+    assert len(gks) == 27
+    rk = K.gen_rk(ring, sk, pkg.bfv.select_digits((len(base) - 1) // 2 + 1, len(base)), rng, la)
+    ct = D.DBFVBatch(K.bfv_enc(ring, base, modulus.t(), K.encode_scalar(ring, 2025), sk, rng, la), base, 1, "crt", 1)
+    res = B.bootstrap_bfv_dbfv(ring, f, tr, modulus, ct, rk, gks, encaps)
+    dec = K.dbfv_dec(ring, modulus, res, sk)
+    t = modulus.t()
+    el = K.encode_pb_unscaled(ring, base, t, dec)
+    ring.pb_to_crt_tensor(el, base)
+    for gen, cs in tr.reduce_t(t).forward():
+        el = H.apply_cyclic_transform_plain(ring, el, base, gen, cs)
+    ring.crt_to_pb_tensor(el, base)
+    s = f.scale_m_k_to_m_period_basis()
+    slots = [D.cmod(v * s, t) for v in K.decode_pb_unscaled(ring, t, el, base)[:f.deg_k]]
+    assert slots == [2025] * f.deg_k
+    ring.close()
