@@ -443,3 +443,172 @@ int64_t csd_encode_digits_launch(csd_ctx* ctx, const csd_view* v, size_t log2_b,
     ctx->free_bytes(flag);
     return log2_b == 0 ? 1 : host;
 }
+
+// ---------------------------------------------------------------------------------------------
+// One combination of dBFV ciphertexts with scalars (the pattern of every step of the digit
+// extraction circuit, digitex.rs:62-241): dbfv_allocate_accumulator, dbfv_acc_inner_product_scalar
+// (each scalar in balanced base-b digits, dbfv.rs:354-373), dbfv_acc_add of further ciphertexts,
+// dbfv_acc_add_constant and dbfv_finish_accumulator (dbfv.rs:143-406), as one pass. Every output
+// digit k of component s is sum_(t, i + j = k) x_t[i] sd_t[j] + sum_a a[k] mont(1) (+ the constant
+// times mont(1) on digit 0, component 0), an exact i128 sum held in registers; the reference
+// forms the same sum in the rows of a WideRingEls, so the reduction (red128, or double_red128 and
+// to_mont64 where the row's bound exceeds 2^63 min_p, decided on the host from the same bound
+// bookkeeping) gives the same residues. Operands are digit-major batches over one base.
+
+#define CB_MAX_TERMS 8
+#define CB_MAX_ADDS 4
+#define CB_MAX_DIGITS 12
+
+struct CombArgs {
+    int64_t count, r, plane;  // ciphertexts, limbs, n2 * n3
+    int nt, na, dout;
+    const int64_t* x[CB_MAX_TERMS];
+    int ld[CB_MAX_TERMS];
+    const int64_t* a[CB_MAX_ADDS];
+    int lda[CB_MAX_ADDS];
+    const int64_t* sd;     // [nt][CB_MAX_DIGITS][r]: mont(digit j of scalar t) per limb
+    int nsd[CB_MAX_TERMS];
+    const int64_t* one;    // [r] mont(1)
+    const int64_t* cst;    // [r] the constant's tensor, or null
+    uint32_t full;         // bit 2 k + s: row kind of output digit k, component s
+    int64_t* out;          // [2 dout count, r, plane]
+};
+
+__global__ __launch_bounds__(256) void k_dbfv_combine(CombArgs a, ModTable mt) {
+    const int64_t pos = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t c = blockIdx.y % a.count, j = blockIdx.y / a.count;
+    if (pos >= a.plane) return;
+    const int64_t limb = a.r * a.plane;  // one row of the batch
+    const int64_t off = j * a.plane + pos;
+    i128 acc[CB_MAX_DIGITS][2];
+#pragma unroll
+    for (int k = 0; k < CB_MAX_DIGITS; k++) acc[k][0] = acc[k][1] = 0;
+    for (int t = 0; t < a.nt; t++) {
+        const int64_t* xt = a.x[t];
+        const int64_t* sdt = a.sd + (int64_t)t * CB_MAX_DIGITS * a.r + j;
+        for (int i = 0; i < a.ld[t]; i++) {
+            const int64_t x0 = xt[((2 * i) * a.count + c) * limb + off];
+            const int64_t x1 = xt[((2 * i + 1) * a.count + c) * limb + off];
+#pragma unroll
+            for (int k = 0; k < CB_MAX_DIGITS; k++) {
+                const int jj = k - i;
+                if (k < a.dout && jj >= 0 && jj < a.nsd[t]) {
+                    const int64_t s = sdt[(int64_t)jj * a.r];
+                    acc[k][0] = (i128)((u128)acc[k][0] + (u128)((i128)x0 * (i128)s));
+                    acc[k][1] = (i128)((u128)acc[k][1] + (u128)((i128)x1 * (i128)s));
+                }
+            }
+        }
+    }
+    const int64_t one = a.one[j];
+    for (int q = 0; q < a.na; q++) {
+        const int64_t* aq = a.a[q];
+#pragma unroll
+        for (int k = 0; k < CB_MAX_DIGITS; k++) {
+            if (k < a.lda[q]) {
+                const int64_t y0 = aq[((2 * k) * a.count + c) * limb + off];
+                const int64_t y1 = aq[((2 * k + 1) * a.count + c) * limb + off];
+                acc[k][0] = (i128)((u128)acc[k][0] + (u128)((i128)y0 * (i128)one));
+                acc[k][1] = (i128)((u128)acc[k][1] + (u128)((i128)y1 * (i128)one));
+            }
+        }
+    }
+    if (a.cst) acc[0][0] = (i128)((u128)acc[0][0] + (u128)((i128)a.cst[j] * (i128)one));
+    const CsdMod m = mt.m[j];
+#pragma unroll
+    for (int k = 0; k < CB_MAX_DIGITS; k++) {
+        if (k < a.dout) {
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const i128 v = acc[k][s];
+                const int64_t y = ((a.full >> (2 * k + s)) & 1u) ? csd_to_mont64(m, csd_double_red128(m, v)) : csd_red128(m, v);
+                a.out[((2 * k + s) * a.count + c) * limb + off] = y;
+            }
+        }
+    }
+}
+
+void csd_dbfv_combine_launch(csd_ctx* ctx, const csd_view* out, int64_t count, int dout, const csd_view* terms,
+                             const int* term_digits, const int64_t* sd_dev, const int* term_sd, int nt, const csd_view* adds,
+                             const int* add_digits, int na, const int64_t* one_dev, const int64_t* cst_dev, uint32_t full,
+                             const csd_modulus* mods, size_t r) {
+    check_view(out, "dbfv_combine output");
+    if (nt < 0 || nt > CB_MAX_TERMS || na < 0 || na > CB_MAX_ADDS) throw CsdError("dbfv_combine: too many operands");
+    if (dout < 1 || dout > CB_MAX_DIGITS) throw CsdError("dbfv_combine: output digits out of range");
+    if (count < 1 || r < 1 || r > CSD_MAX_MODULI) throw CsdError("dbfv_combine: batch size or base");
+    const int64_t n2 = out->shape[2], n3 = out->shape[3], plane = n2 * n3;
+    auto contiguous = [&](const csd_view* v, int64_t rows) {
+        return v->shape[0] == rows && v->shape[1] == (int64_t)r && v->shape[2] == n2 && v->shape[3] == n3 &&
+               v->strides[3] == 1 && (n2 == 1 || v->strides[2] == n3) && v->strides[1] == plane &&
+               v->strides[0] == (int64_t)r * plane;
+    };
+    if (!contiguous(out, 2 * (int64_t)dout * count)) throw CsdError("dbfv_combine: output must be a contiguous [2 dout count, r, n2, n3] batch");
+    CombArgs a;
+    memset(&a, 0, sizeof(a));
+    a.count = count;
+    a.r = (int64_t)r;
+    a.plane = plane;
+    a.nt = nt;
+    a.na = na;
+    a.dout = dout;
+    for (int t = 0; t < nt; t++) {
+        if (term_digits[t] < 1 || term_digits[t] > CB_MAX_DIGITS || term_sd[t] < 1 || term_sd[t] > CB_MAX_DIGITS)
+            throw CsdError("dbfv_combine: term digits out of range");
+        if (!contiguous(&terms[t], 2 * (int64_t)term_digits[t] * count)) throw CsdError("dbfv_combine: terms must be contiguous batches over the base");
+        a.x[t] = (const int64_t*)terms[t].ptr;
+        a.ld[t] = term_digits[t];
+        a.nsd[t] = term_sd[t];
+    }
+    for (int q = 0; q < na; q++) {
+        if (add_digits[q] < 1 || add_digits[q] > dout) throw CsdError("dbfv_combine: added digits out of range");
+        if (!contiguous(&adds[q], 2 * (int64_t)add_digits[q] * count)) throw CsdError("dbfv_combine: added ciphertexts must be contiguous batches over the base");
+        a.a[q] = (const int64_t*)adds[q].ptr;
+        a.lda[q] = add_digits[q];
+    }
+    a.sd = sd_dev;
+    a.one = one_dev;
+    a.cst = cst_dev;
+    a.full = full;
+    a.out = (int64_t*)out->ptr;
+    ModTable t = make_mod_table(mods, r, (int64_t)r);
+    const int64_t ycount = count * (int64_t)r;
+    if (ycount > 65535) throw CsdError("dbfv_combine: too many (ciphertext, limb) pairs for one launch");
+    double reads = 0;
+    for (int q = 0; q < nt; q++) reads += 2.0 * term_digits[q];
+    for (int q = 0; q < na; q++) reads += 2.0 * add_digits[q];
+    TimedScope ts(ctx, "dbfv_combine", 8.0 * (double)ycount * (double)plane * (reads + 2.0 * dout));
+    hipLaunchKernelGGL(k_dbfv_combine, dim3((unsigned)((plane + 255) / 256), (unsigned)ycount), dim3(256), 0, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+}
+
+extern "C" int csd_dbfv_combine(csd_ctx* ctx, const csd_view* out, size_t count, size_t dout, const csd_view* terms,
+                                const int* term_digits, const csd_view* scalars, const int* term_sd, size_t nt,
+                                const csd_view* adds, const int* add_digits, size_t na, const csd_view* one,
+                                const csd_view* constant, uint32_t full, const csd_modulus* mods, size_t nmods) {
+    CSD_TRY if (nt > CB_MAX_TERMS) throw CsdError("dbfv_combine: too many terms");
+    if (!one) throw CsdError("dbfv_combine: mont(1) tensor missing");
+    check_view(one, "dbfv_combine one");
+    if (one->shape[0] * one->shape[1] * one->shape[2] * one->shape[3] != (int64_t)nmods || one->strides[1] != 1)
+        throw CsdError("dbfv_combine: mont(1) must be [1, r, 1, 1], contiguous");
+    const int64_t* cst = nullptr;
+    if (constant) {
+        check_view(constant, "dbfv_combine constant");
+        if (constant->shape[0] * constant->shape[1] * constant->shape[2] * constant->shape[3] != (int64_t)nmods ||
+            constant->strides[1] != 1)
+            throw CsdError("dbfv_combine: the constant must be [1, r, 1, 1], contiguous");
+        cst = (const int64_t*)constant->ptr;
+    }
+    const int64_t* sd = nullptr;
+    if (nt) {
+        if (!scalars) throw CsdError("dbfv_combine: scalar digit table missing");
+        check_view(scalars, "dbfv_combine scalars");
+        if (scalars->shape[0] != 1 || scalars->shape[1] != (int64_t)nt || scalars->shape[2] != CB_MAX_DIGITS ||
+            scalars->shape[3] != (int64_t)nmods || scalars->strides[3] != 1 || scalars->strides[2] != (int64_t)nmods ||
+            scalars->strides[1] != (int64_t)(CB_MAX_DIGITS * nmods))
+            throw CsdError("dbfv_combine: scalar digits must be a contiguous [1, terms, 12, r] table");
+        sd = (const int64_t*)scalars->ptr;
+    }
+    csd_dbfv_combine_launch(ctx, out, (int64_t)count, (int)dout, terms, term_digits, sd, term_sd, (int)nt, adds, add_digits,
+                            (int)na, (const int64_t*)one->ptr, cst, full, mods, nmods);
+    CSD_CATCH
+}

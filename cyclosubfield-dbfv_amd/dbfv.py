@@ -604,3 +604,106 @@ def dbfv_mod_switch_down(ring, ct, new_len):
     out = ring.allocate_uninit(ct.data.shape[0], ct.base[:new_len])
     ring.mod_switch_down(out, ct.data, ct.base, new_len)
     return DBFVBatch(out, ct.base[:new_len], ct.count, "pb", 1)
+
+
+def _mont_repr(x, p):
+    """mont_repr (ring.rs:109) of (x rem p): (y 2^64) rem p with Rust's remainder, y = x rem p"""
+    y = abs(x) % p
+    y = -y if x < 0 else y
+    v = (abs(y) << 64) % p
+    return -v if y < 0 else v
+
+
+def dbfv_combine(ring, modulus, like, terms, adds=(), constant=None, basis="pb"):
+    """One combination of dBFV batches, as one kernel (csd_dbfv_combine): the result of
+    dbfv_allocate_accumulator, dbfv_acc_inner_product_scalar(terms), dbfv_acc_add for each of
+    `adds`, dbfv_acc_add_constant(constant) and dbfv_finish_accumulator (dbfv.rs:143-406). The
+    reduction kind of every row is that of the accumulator's bound bookkeeping (ring.rs:354-366),
+    so the residues are those of the accumulator sequence. `like` gives the base and batch size."""
+    from .hal import check, lib, CsdView
+    import ctypes
+
+    base, n = like.base, like.count
+    cap, b = modulus.digits(), modulus.b()
+    max_p = max(ring.primes[i] for i in base)
+    min_p = min(ring.primes[i] for i in base)
+    step = max_p * max_p
+    tl = []
+    for lhs, s in terms:
+        ds = balanced_digits(s, b, cap)
+        ld = lhs.digits()
+        if ld == 0 or not ds:
+            continue
+        if lhs.count != n or lhs.base != base or lhs.basis != basis:
+            raise ValueError("combination operands differ in batch size, base or basis")
+        if ld > cap:
+            raise ValueError("ciphertext has more digits than the accumulator")
+        tl.append((lhs, ds, min(cap, ld + len(ds) - 1)))
+    al = []
+    for a in adds:
+        if a.digits() == 0:
+            continue
+        if a.count != n or a.base != base or a.basis != basis or a.digits() > cap:
+            raise ValueError("added ciphertext differs in batch size, base, basis or digits")
+        al.append(a)
+    dout = max([t[2] for t in tl] + [a.digits() for a in al] + [0])
+    if dout == 0:
+        return DBFVBatch(like.data[0:0], base, n, basis, 1)
+    if len(tl) > 8 or len(al) > 4 or dout > 12:
+        raise ValueError("combination beyond the fused kernel's operand limits")
+    # the rows' bounds as the accumulator would track them
+    bounds = [[0, 0] for _ in range(dout)]
+    for lhs, ds, new_t in tl:
+        for k in range(new_t):
+            for jj in range(len(ds)):
+                i = k - jj
+                if 0 <= i < lhs.digits():
+                    for s_ in range(2):
+                        bounds[k][s_] += lhs.bound * step
+    for a in al:
+        for k in range(a.digits()):
+            for s_ in range(2):
+                bounds[k][s_] += a.bound * step
+    cst = None
+    if constant is not None:
+        delta = rounded_div(ring.modulus_of(base), modulus.t())
+        c = constant * (scale_q_to_m_period_basis(ring) if basis == "pb" else 1)
+        cst = ring.scalar_tensor_big(base, c * delta)
+        bounds[0][0] += step
+    full = 0
+    for k in range(dout):
+        for s_ in range(2):
+            if bounds[k][s_] > (1 << 127) - 1:
+                raise OverflowError("wide ring element overflow: too many accumulation steps")
+            if bounds[k][s_] > (min_p << 63):
+                full |= 1 << (2 * k + s_)
+    be = ring.backend
+    r = len(base)
+    sd = None
+    if tl:
+        key = ("combine_sd", tuple(base), tuple(tuple(ds) for _, ds, _ in tl))
+        sd = ring._consts.get(key)
+        if sd is None:
+            vals = [0] * (len(tl) * 12 * r)
+            for t, (_, ds, _) in enumerate(tl):
+                for jj, d in enumerate(ds):
+                    for j, i in enumerate(base):
+                        vals[(t * 12 + jj) * r + j] = _mont_repr(d, ring.primes[i])
+            sd = be.hal_allocate_i64([1, len(tl), 12, r])
+            be.hal_import(sd, vals)
+            ring._consts[key] = sd
+    out = ring.allocate_uninit(2 * dout * n, base)
+    tv = (CsdView * max(1, len(tl)))(*[t[0].data.view() for t in tl])
+    av = (CsdView * max(1, len(al)))(*[a.data.view() for a in al])
+    tdig = (ctypes.c_int * max(1, len(tl)))(*[t[0].digits() for t in tl])
+    tsd = (ctypes.c_int * max(1, len(tl)))(*[len(t[1]) for t in tl])
+    adig = (ctypes.c_int * max(1, len(al)))(*[a.digits() for a in al])
+    mods = ring.mods_for(base)
+    marr = (type(mods[0]) * len(mods))(*mods)
+    one = ring.scalar_tensor(base, 1)
+    check(lib().csd_dbfv_combine(be.ctx, ctypes.byref(out.view()), ctypes.c_size_t(n), ctypes.c_size_t(dout), tv, tdig,
+                                 ctypes.byref(sd.view()) if sd is not None else None, tsd, ctypes.c_size_t(len(tl)),
+                                 av, adig, ctypes.c_size_t(len(al)), ctypes.byref(one.view()),
+                                 ctypes.byref(cst.view()) if cst is not None else None, ctypes.c_uint32(full), marr,
+                                 ctypes.c_size_t(len(mods))))
+    return DBFVBatch(out, base, n, basis, 1)

@@ -98,9 +98,12 @@ def _add_to(ring, target, value):
     D.dbfv_acc_add(ring, target, _switched_for(ring, target, value))
 
 
-def _combine(ring, modulus, like, terms, adds=(), constant=None):
+def _combine(ring, modulus, like, terms, adds=(), constant=None, fused=True):
     """allocate an accumulator, one scalar inner product, additions (and a constant), finish: the
-    pattern of every combination in digitex.rs:62-241"""
+    pattern of every combination in digitex.rs:62-241; `fused` runs it as one kernel
+    (dbfv.dbfv_combine, bit-identical), otherwise as the accumulator's HAL calls"""
+    if fused:
+        return D.dbfv_combine(ring, modulus, like, terms, adds, constant)
     acc = D.dbfv_allocate_accumulator(ring, modulus, like.base, "pb", like.count)
     D.dbfv_acc_inner_product_scalar(ring, acc, terms)
     for a in adds:
@@ -144,52 +147,52 @@ def hom_eval_digit_extract_circuit_dbfv(ring, modulus, ct, subtract_intermediate
         return
 
     x2 = ms(x2, 2)
-    val = _combine(ring, modulus, x2, [(x2, 94), (x4, 121)])
+    val = _combine(ring, modulus, x2, fused=products.fused, terms= [(x2, 94), (x4, 121)])
     pre_x8 = ms(products.square(modulus, val), 3)
     x2, x4 = ms(x2, 3), ms(x4, 3)
 
     if pending or mults <= 3:
-        x8 = _combine(ring, modulus, x2, [(x2, 112)], adds=[pre_x8])
+        x8 = _combine(ring, modulus, x2, fused=products.fused, terms= [(x2, 112)], adds=[pre_x8])
         for target in take(4):
             _sub_from(ring, target, x8)
         if mults <= 3:
             _sub_from(ring, subtract_final, x8)
             return
 
-    lhs = _combine(ring, modulus, x2, [(x2, 12656), (x4, 226), (pre_x8, 113)])
-    rhs = _combine(ring, modulus, x2, [(x2, 33720), (x4, 2), (pre_x8, 301)])
+    lhs = _combine(ring, modulus, x2, fused=products.fused, terms= [(x2, 12656), (x4, 226), (pre_x8, 113)])
+    rhs = _combine(ring, modulus, x2, fused=products.fused, terms= [(x2, 33720), (x4, 2), (pre_x8, 301)])
     pre_x16 = ms(products.mul(modulus, lhs, rhs), 4)
     x2, x4, pre_x8 = ms(x2, 4), ms(x4, 4), ms(pre_x8, 4)
 
     if pending or mults <= 4:
-        x16 = _combine(ring, modulus, x2, [(x2, 2535424), (x4, 528), (pre_x8, 22620)], adds=[pre_x16])
+        x16 = _combine(ring, modulus, x2, fused=products.fused, terms= [(x2, 2535424), (x4, 528), (pre_x8, 22620)], adds=[pre_x16])
         for target in take(8):
             _sub_from(ring, target, x16)
         if mults <= 4:
             _sub_from(ring, subtract_final, x16)
             return
 
-    lhs = _combine(ring, modulus, x2, [(x2, 163551), (x4, -64628), (pre_x8, 88914), (pre_x16, -168751)])
-    rhs = _combine(ring, modulus, x2, [(x2, 142065), (x4, 426380), (pre_x8, 456690), (pre_x16, -290191)])
+    lhs = _combine(ring, modulus, x2, fused=products.fused, terms= [(x2, 163551), (x4, -64628), (pre_x8, 88914), (pre_x16, -168751)])
+    rhs = _combine(ring, modulus, x2, fused=products.fused, terms= [(x2, 142065), (x4, 426380), (pre_x8, 456690), (pre_x16, -290191)])
     pre_x23 = ms(products.mul(modulus, lhs, rhs), 5)
 
     x2, x4, pre_x8, pre_x16 = ms(x2, 5), ms(x4, 5), ms(pre_x8, 5), ms(pre_x16, 5)
-    val = _combine(ring, modulus, x2, [(x4, 460241), (pre_x8, -520176), (pre_x16, 272152)], adds=[pre_x23])
+    val = _combine(ring, modulus, x2, fused=products.fused, terms= [(x4, 460241), (pre_x8, -520176), (pre_x16, 272152)], adds=[pre_x23])
     s1 = ms(products.square(modulus, val), 6)
 
     x2, x4, pre_x8, pre_x16, pre_x23 = ms(x2, 6), ms(x4, 6), ms(pre_x8, 6), ms(pre_x16, 6), ms(pre_x23, 6)
-    val = _combine(ring, modulus, x2,
+    val = _combine(ring, modulus, x2, fused=products.fused, terms=
                    [(x4, -32106558060291480110540), (pre_x8, 64784965186867951517504),
                     (pre_x16, -70466562361570490438944), (pre_x23, 403089028126664267137012)], adds=[s1])
     s2 = ms(products.mul(modulus, val, s1), 7)
 
     x2, x4, pre_x8, pre_x16 = ms(x2, 7), ms(x4, 7), ms(pre_x8, 7), ms(pre_x16, 7)
     pre_x23, s1 = ms(pre_x23, 7), ms(s1, 7)
-    lhs = _combine(ring, modulus, x2,
+    lhs = _combine(ring, modulus, x2, fused=products.fused, terms=
                    [(x4, -306594109877109866124476), (pre_x8, 55340232221151557156928),
                     (pre_x16, 66482065641637241715808), (pre_x23, -201964177491009025911804),
                     (s1, 982289121925033623516)], adds=[s2], constant=152453116397172589318468)
-    rhs = _combine(ring, modulus, x2,
+    rhs = _combine(ring, modulus, x2, fused=products.fused, terms=
                    [(x4, -541458055423539766375756), (pre_x8, -321268494787747983014080),
                     (pre_x16, 389373873907872951793376), (pre_x23, -168833825134626671122316),
                     (s1, 982289121925033623520)], adds=[s2])
@@ -197,7 +200,7 @@ def hom_eval_digit_extract_circuit_dbfv(ring, modulus, ct, subtract_intermediate
 
     x2, x4, pre_x8, pre_x16 = ms(x2, 8), ms(x4, 8), ms(pre_x8, 8), ms(pre_x16, 8)
     pre_x23, s1 = ms(pre_x23, 8), ms(s1, 8)
-    res = _combine(ring, modulus, x2,
+    res = _combine(ring, modulus, x2, fused=products.fused, terms=
                    [(x4, 138276793813981485937712), (pre_x8, -394317601587992658717952),
                     (pre_x16, 277439031009004544298112), (pre_x23, -62423781945432606733008),
                     (s1, 376442706312192718108336)], adds=[s3])

@@ -156,6 +156,19 @@ int csd_ring_galois_keyswitch_rows(csd_ring* ring, const size_t* base, size_t r,
                                    size_t digits, const csd_view* parts, const csd_view* c0, int64_t c0_hi, int64_t c0_lo,
                                    const uint64_t* gs, size_t ngal, const csd_view* keys, const csd_view* outs, int64_t o_hi,
                                    int64_t o_lo, int64_t o_s);
+/* One combination of digit-major dBFV batches over one base as one kernel (no reference
+ * counterpart: dbfv_allocate_accumulator, dbfv_acc_inner_product_scalar, dbfv_acc_add,
+ * dbfv_acc_add_constant and dbfv_finish_accumulator of dbfv.rs:143-406 in sequence, the pattern of
+ * every step of digitex.rs:62-241): out ([2 dout count, r, n2, n3]) digit k, component s =
+ * reduce(sum_(t, i + j = k) terms[t] digit i * scalars[0, t, j, :] + sum_q adds[q] digit k * one
+ * + [k = s = 0] constant * one). terms[t] has term_digits[t] digits, its scalar term_sd[t]
+ * balanced digits (scalars: [1, nt, 12, r], mont form); bit 2 k + s of `full` selects
+ * double_red128 + to_mont64 for that row (the accumulator's bound exceeds 2^63 min_p), else
+ * red128. At most 8 terms, 4 adds, 12 output digits. Bit-identical to the accumulator sequence. */
+int csd_dbfv_combine(csd_ctx* ctx, const csd_view* out, size_t count, size_t dout, const csd_view* terms,
+                     const int* term_digits, const csd_view* scalars, const int* term_sd, size_t nt, const csd_view* adds,
+                     const int* add_digits, size_t na, const csd_view* one, const csd_view* constant, uint32_t full,
+                     const csd_modulus* mods, size_t nmods);
 /* decode_pb_scaled (bfv.rs:262-274) on the device, exact: for the period-basis [count, r, n2, n3]
  * tensor src over `base` (r <= 4 primes, montgomery form, bound <= 2) writes round(2^log2_t x / Q)
  * balanced into (-t/2, t/2] to the contiguous i128 [count, 1, n2, n3] tensor dst, x the lift of
