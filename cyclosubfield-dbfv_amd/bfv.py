@@ -165,7 +165,7 @@ def bfv_hom_mul_batch(ring, t, lhs, rhs, base, keys, digits):
     return out
 
 
-def relin_batch(ring, prod, K, base, keys, digits, output_basis, bound=1, interleave=True):
+def relin_batch(ring, prod, K, base, keys, digits, output_basis, bound=1, interleave=True, add_c01=True):
     """relin (bfv.rs:699-741) of K products at once: `prod` is a [3K, r, ., .] period-basis
     tensor (destroyed) with component c of product k in row c K + k, of bound `bound`. Returns
     (ciphertexts [2K, r, ., .] with product k in rows 2k, 2k + 1, or with interleave=False in
@@ -209,6 +209,10 @@ def relin_batch(ring, prod, K, base, keys, digits, output_basis, bound=1, interl
         raise ValueError("output_basis must be 'crt' or 'pb'")
     be.keyswitch_reduce(out, (K, 1), parts, K, d, keys, None, None, wide_bound > (min_p << 63), ring.mods_for(base))
     ring.crt_to_pb_tensor(out, base)
+    if not add_c01:  # the caller adds c0, c1 itself (add_els, ring.rs:990-1023)
+        if interleave:
+            raise ValueError("add_c01=False returns the key-switch part in the component-major layout")
+        return out, 1
     be.hal_add_i64(out, prod[0:2 * K])  # add_els (ring.rs:990-1023)
     if not interleave:
         return out, 1 + bound

@@ -298,20 +298,38 @@ def _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, base_len, keys, di
     d1, d2 = lhs_l.shape[0] // (2 * n), rhs_l.shape[0] // (2 * n)
     K = min(modulus.digits(), d1 + d2 - 1)
 
-    def blk(t, row):  # rows row n .. row n + n
-        return t[row * n:(row + 1) * n]
-
     # the inner products of every output digit and component and reduce_wide, as one kernel
     prod = ring.convolve_reduce(lhs_l, (2 * n, n, 1), rhs_l, (2 * n, n, 1), n, d1, d2, K, lifted)  # CRT basis
     ring.crt_to_pb_tensor(prod, lifted)
     rescaled = rescale_to_base(ring, prod, lifted, base_len, modulus.t())  # bound 1
     out, bound = relin_batch(ring, rescaled, K * n, lifted[:base_len], keys, digits, output_basis, bound=1,
-                             interleave=False)  # product (k, pair) component s at row s K n + k n + pair
-    result = ring.allocate_uninit(2 * K * n, lifted[:base_len])
-    for k in range(K):
-        for s_ in (0, 1):
-            be.hal_assign(blk(result, 2 * k + s_), out[(s_ * K + k) * n:(s_ * K + k + 1) * n])
+                             interleave=False, add_c01=output_basis == "crt")
+    # product (k, pair) component s sits at row s K n + k n + pair; the result is digit-major, row
+    # (2 k + s) n + pair. Seen as [K, n r, ., .] tensors (the n rows of one k, limbs included, are
+    # consecutive planes), each component moves with one strided copy; in period basis c0, c1 of
+    # the rescaled product are added on the way (add_els, ring.rs:990-1023)
+    base = lifted[:base_len]
+    result = ring.allocate_uninit(2 * K * n, base)
+    for s_ in (0, 1):
+        dst = _digit_rows(result, s_ * n, K, n, 2 * n)
+        be.hal_assign(dst, _digit_rows(out, s_ * K * n, K, n, n))
+        if output_basis == "pb":
+            be.hal_add_i64(dst, _digit_rows(rescaled, s_ * K * n, K, n, n))
+    if output_basis == "pb":
+        bound = 2
     return result, bound
+
+
+def _digit_rows(t, first, K, n, step):
+    """rows first + k step + pair (k < K, pair < n) of a [rows, r, n2, n3] tensor as a [K, n r, n2,
+    n3] view: the n rows of one k and their limbs are consecutive planes (no copy)"""
+    from .hal import DeviceTensor
+
+    plane = t.shape[2] * t.shape[3]
+    if t.strides[1] != plane or t.strides[0] != t.shape[1] * plane:
+        raise ValueError("digit rows need a contiguous tensor")
+    return DeviceTensor(t.backend, t.ptr + first * t.strides[0] * t.elem, (K, n * t.shape[1], t.shape[2], t.shape[3]),
+                        (step * t.strides[0], plane, t.strides[2], t.strides[3]), t.elem, t._owner or t)
 
 
 def _reduced_operand(ring, modulus, ct, ct_bound, count, base, fused):
