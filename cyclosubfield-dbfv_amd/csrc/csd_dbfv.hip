@@ -47,26 +47,93 @@ __device__ __forceinline__ int row_nbd(int limbs, int k) { return 2 + limbs + k;
 __device__ __forceinline__ int row_one(int limbs) { return 2 + 2 * limbs; }           // mont(1)
 __device__ __forceinline__ int row_delta(int limbs, int k) { return 3 + 2 * limbs + k; }  // mont^2(Delta 2^64k)
 
-// One thread per coefficient position. Its r residues wait in its own LDS column (slot p at
-// p * blockDim.x + threadIdx.x) between the passes over the primes, so the loops keep only the
-// running sums in registers. The constant table is staged in LDS once per workgroup (broadcast
-// reads, faster here than per-prime scalar loads).
+// One digit's step for one coefficient position: the carry step (when the digit has a carry
+// in), split_overflow, x_lo written to X; (cl0, cl1) carry the overflow in and out. The r residues
+// wait in the thread's own LDS column (slot p at p * stride) between the passes over the primes, so
+// the loops keep only the running sums in registers.
+struct SoStep {
+    int r, limbs, shift, log2_r;
+    i128 f0, f1;
+};
+
+__device__ __forceinline__ void so_digit(const SoStep& a, const ModTable& mt, const int64_t* __restrict__ K, int64_t* col,
+                                         int stride, const int64_t* XI, int64_t sxi, bool zero_in, bool has_carry,
+                                         bool full_split, bool full_carry, int64_t* X, int64_t sx1, int64_t& cl0, int64_t& cl1) {
+    const int r = a.r, limbs = a.limbs;
+    // the digit's residues, eight loads in flight at a time, parked in the LDS column
+    if (!zero_in) {
+        for (int p0 = 0; p0 < r; p0 += 8) {
+            int64_t raw[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                if (p0 + t < r) raw[t] = XI[(p0 + t) * sxi];
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                if (p0 + t < r) col[(p0 + t) * stride] = raw[t];
+        }
+    }
+    // pass 1: the carry step (dbfv.rs:995-1003), then the lifts and the matmul rows
+    // (dbfv.rs:909-917)
+    u128 s0 = 0, s1 = 0, sg = 0;
+    for (int p = 0; p < r; p++) {
+        const CsdMod m = mt.m[p];
+        int64_t xv = zero_in ? 0 : col[p * stride];
+        if (has_carry) {
+            // wide fma of each carry limb with mont^2(Delta 2^64k) (mul for the first, fma
+            // after), add of the digit with mont(1), reduce_wide
+            u128 acc = (u128)((i128)cl0 * (i128)K[row_delta(limbs, 0) * r + p]);
+            if (limbs == 2) acc += (u128)((i128)cl1 * (i128)K[row_delta(limbs, 1) * r + p]);
+            acc += (u128)((i128)xv * (i128)K[row_one(limbs) * r + p]);
+            xv = full_carry ? csd_to_mont64(m, csd_double_red128(m, (i128)acc)) : csd_red128(m, (i128)acc);
+        }
+        col[p * stride] = xv;
+        const int64_t lift = csd_mont_mul(m, xv, K[row_lift() * r + p]);
+        s0 += (u128)((i128)K[row_value(0) * r + p] * (i128)lift);
+        if (limbs == 2) s1 += (u128)((i128)K[row_value(1) * r + p] * (i128)lift);
+        sg += (u128)((i128)K[row_corr(limbs) * r + p] * (i128)lift);
+    }
+    // the overflow (bconv_correct_pow2{,_wide}, dbfv.rs:919-938)
+    const u128 half = (u128)1 << (a.shift - 1);
+    const i128 quot = (i128)(sg + half) >> a.shift;
+    int64_t ov0, ov1 = 0;
+    if (limbs == 1) {
+        const i128 v = (i128)((s0 - (u128)quot * (u128)a.f0) * (u128)a.f1);
+        ov0 = (int64_t)csd_pow2_centre(v, a.log2_r);
+    } else {
+        const i128 v = (i128)(s0 + (s1 << 64));
+        const i128 cc = csd_pow2_centre((i128)(((u128)v - (u128)quot * (u128)a.f0) * (u128)a.f1), a.log2_r);
+        ov0 = (int64_t)cc;
+        ov1 = (int64_t)((i128)((u128)cc - (u128)(i128)ov0) >> 64);
+    }
+    cl0 = ov0;
+    cl1 = ov1;
+    // pass 2: x_lo = reduce(overflow mont^2(-b Delta) + x mont(1)) (dbfv.rs:940-947)
+    for (int p = 0; p < r; p++) {
+        const CsdMod m = mt.m[p];
+        const int64_t xv = col[p * stride];
+        u128 acc = (u128)((i128)ov0 * (i128)K[row_nbd(limbs, 0) * r + p]);
+        if (limbs == 2) acc += (u128)((i128)ov1 * (i128)K[row_nbd(limbs, 1) * r + p]);
+        acc += (u128)((i128)xv * (i128)K[row_one(limbs) * r + p]);
+        X[p * sx1] = full_split ? csd_to_mont64(m, csd_double_red128(m, (i128)acc)) : csd_red128(m, (i128)acc);
+    }
+}
+
+// the constant table staged in LDS once per workgroup, after the columns (broadcast reads,
+// faster here than per-prime scalar loads)
+__device__ __forceinline__ const int64_t* so_stage_table(int64_t* xs, const int64_t* k, int r, int limbs) {
+    int64_t* kl = xs + (int64_t)r * blockDim.x;
+    const int rows = 3 + 3 * limbs;
+    for (int q = threadIdx.x; q < rows * r; q += blockDim.x) kl[q] = k[q];
+    __syncthreads();
+    return kl;
+}
+
+// One thread per coefficient position: one digit's carry step and split.
 __global__ __launch_bounds__(256) void k_split_overflow(SoArgs a, ModTable mt) {
     extern __shared__ int64_t xs[];
     const int64_t per = a.n2 * a.n3;
     const int64_t total = a.count * per;
-    const int r = (int)a.r;
-    const int limbs = a.limbs;
-    const int64_t* K = a.k;
-    int64_t* col = xs + threadIdx.x;
-    const int stride = (int)blockDim.x;
-    {
-        int64_t* kl = xs + (int64_t)r * stride;  // the table, after the columns
-        const int rows = 3 + 3 * limbs;
-        for (int q = threadIdx.x; q < rows * r; q += stride) kl[q] = a.k[q];
-        __syncthreads();
-        K = kl;
-    }
+    const int64_t* K = so_stage_table(xs, a.k, (int)a.r, a.limbs);
     // one coefficient per thread and no grid-stride loop: a loop with a per-lane exit would make
     // the prime loops below divergent
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,64 +147,50 @@ __global__ __launch_bounds__(256) void k_split_overflow(SoArgs a, ModTable mt) {
         if (a.cin) {
             const int64_t* C = a.cin + c * a.sc[0] + j2 * a.sc[2] + j3;
             cl0 = C[0];
-            if (limbs == 2) cl1 = C[a.sc[1]];
+            if (a.limbs == 2) cl1 = C[a.sc[1]];
         }
-        // the digit's residues, eight loads in flight at a time, parked in the LDS column
-        if (!a.zero_in) {
-            for (int p0 = 0; p0 < r; p0 += 8) {
-                int64_t raw[8];
-#pragma unroll
-                for (int t = 0; t < 8; t++)
-                    if (p0 + t < r) raw[t] = XI[(p0 + t) * sxi];
-#pragma unroll
-                for (int t = 0; t < 8; t++)
-                    if (p0 + t < r) col[(p0 + t) * stride] = raw[t];
-            }
-        }
-        // pass 1: the carry step (dbfv.rs:995-1003), then the lifts and the matmul rows
-        // (dbfv.rs:909-917)
-        u128 s0 = 0, s1 = 0, sg = 0;
-        for (int p = 0; p < r; p++) {
-            const CsdMod m = mt.m[p];
-            int64_t xv = a.zero_in ? 0 : col[p * stride];
-            if (a.cin) {
-                // wide fma of each carry limb with mont^2(Delta 2^64k) (mul for the first, fma
-                // after), add of the digit with mont(1), reduce_wide
-                u128 acc = (u128)((i128)cl0 * (i128)K[row_delta(limbs, 0) * r + p]);
-                if (limbs == 2) acc += (u128)((i128)cl1 * (i128)K[row_delta(limbs, 1) * r + p]);
-                acc += (u128)((i128)xv * (i128)K[row_one(limbs) * r + p]);
-                xv = a.full_carry ? csd_to_mont64(m, csd_double_red128(m, (i128)acc)) : csd_red128(m, (i128)acc);
-            }
-            col[p * stride] = xv;
-            const int64_t lift = csd_mont_mul(m, xv, K[row_lift() * r + p]);
-            s0 += (u128)((i128)K[row_value(0) * r + p] * (i128)lift);
-            if (limbs == 2) s1 += (u128)((i128)K[row_value(1) * r + p] * (i128)lift);
-            sg += (u128)((i128)K[row_corr(limbs) * r + p] * (i128)lift);
-        }
-        // the overflow (bconv_correct_pow2{,_wide}, dbfv.rs:919-938)
-        const u128 half = (u128)1 << (a.shift - 1);
-        const i128 quot = (i128)(sg + half) >> a.shift;
-        int64_t ov0, ov1 = 0;
-        if (limbs == 1) {
-            const i128 v = (i128)((s0 - (u128)quot * (u128)a.f0) * (u128)a.f1);
-            ov0 = (int64_t)csd_pow2_centre(v, a.log2_r);
-        } else {
-            const i128 v = (i128)(s0 + (s1 << 64));
-            const i128 cc = csd_pow2_centre((i128)(((u128)v - (u128)quot * (u128)a.f0) * (u128)a.f1), a.log2_r);
-            ov0 = (int64_t)cc;
-            ov1 = (int64_t)((i128)((u128)cc - (u128)(i128)ov0) >> 64);
-        }
+        const SoStep st{(int)a.r, a.limbs, a.shift, a.log2_r, a.f0, a.f1};
+        so_digit(st, mt, K, xs + threadIdx.x, (int)blockDim.x, XI, sxi, a.zero_in != 0, a.cin != nullptr, a.full_split != 0,
+                 a.full_carry != 0, X, a.sx[1], cl0, cl1);
         int64_t* O = a.ov + c * a.so[0] + j2 * a.so[2] + j3;
-        O[0] = ov0;
-        if (limbs == 2) O[a.so[1]] = ov1;
-        // pass 2: x_lo = reduce(overflow mont^2(-b Delta) + x mont(1)) (dbfv.rs:940-947)
-        for (int p = 0; p < r; p++) {
-            const CsdMod m = mt.m[p];
-            const int64_t xv = col[p * stride];
-            u128 acc = (u128)((i128)ov0 * (i128)K[row_nbd(limbs, 0) * r + p]);
-            if (limbs == 2) acc += (u128)((i128)ov1 * (i128)K[row_nbd(limbs, 1) * r + p]);
-            acc += (u128)((i128)xv * (i128)K[row_one(limbs) * r + p]);
-            X[p * a.sx[1]] = a.full_split ? csd_to_mont64(m, csd_double_red128(m, (i128)acc)) : csd_red128(m, (i128)acc);
+        O[0] = cl0;
+        if (a.limbs == 2) O[a.so[1]] = cl1;
+    }
+}
+
+// The whole carry chain of dbfv_reduce (dbfv.rs:990-1023) in one launch: each thread takes one
+// coefficient position of one row of a digit through every digit in turn, the carry of digit i
+// held in registers for digit i + 1 (the chain is sequential along the digits only). Digit i of
+// the result is rows [i per, (i + 1) per) of x, read from the same rows of src for i < d and zero
+// beyond; the carry of the last digit is dropped, as the reference drops it.
+struct RcArgs {
+    int64_t per, n2, n3;
+    int64_t sx[3], si[3];
+    int64_t* x;
+    const int64_t* src;
+    const int64_t* k;
+    int digits, d;
+    uint32_t full_split, full_carry;  // bit i: digit i's reductions
+    SoStep st;
+};
+
+__global__ __launch_bounds__(256) void k_dbfv_reduce_chain(RcArgs a, ModTable mt) {
+    extern __shared__ int64_t xs[];
+    const int64_t npos = a.n2 * a.n3;
+    const int64_t total = a.per * npos;
+    const int64_t* K = so_stage_table(xs, a.k, a.st.r, a.st.limbs);
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < total) {
+        const int64_t c = e / npos, j = e - c * npos;
+        const int64_t j2 = j / a.n3, j3 = j - j2 * a.n3;
+        int64_t cl0 = 0, cl1 = 0;
+        for (int i = 0; i < a.digits; i++) {
+            const int64_t row = (int64_t)i * a.per + c;
+            int64_t* X = a.x + row * a.sx[0] + j2 * a.sx[2] + j3;
+            const bool zero_in = i >= a.d;
+            const int64_t* XI = zero_in ? X : a.src + row * a.si[0] + j2 * a.si[2] + j3;
+            so_digit(a.st, mt, K, xs + threadIdx.x, (int)blockDim.x, XI, a.si[1], zero_in, i > 0, (a.full_split >> i) & 1,
+                     (a.full_carry >> i) & 1, X, a.sx[1], cl0, cl1);
         }
     }
 }
@@ -210,6 +263,81 @@ extern "C" int csd_dbfv_split_overflow(csd_ctx* ctx, const csd_view* x, const cs
     const int64_t total = a.count * a.n2 * a.n3;
     if ((total + 255) / 256 > (int64_t)INT32_MAX) throw CsdError("dbfv_split_overflow: too many coefficients for one launch");
     hipLaunchKernelGGL(k_split_overflow, dim3((unsigned)((total + 255) / 256)), dim3(256), lds, ctx->stream, a, t);
+    HIPCHECK(hipGetLastError());
+    CSD_CATCH
+}
+
+// whether the memory spans (first to last element) of two int64 views with non-negative strides meet
+static bool rc_spans_meet(const csd_view& a, const csd_view& b) {
+    auto span = [](const csd_view& v, uintptr_t& lo, uintptr_t& hi) {
+        int64_t last = 0;
+        for (int q = 0; q < 4; q++) {
+            if (v.strides[q] < 0) throw CsdError("dbfv_reduce_chain: negative strides");
+            last += (v.shape[q] - 1) * v.strides[q];
+        }
+        lo = (uintptr_t)v.ptr;
+        hi = lo + (uintptr_t)(last + 1) * 8;
+    };
+    uintptr_t alo, ahi, blo, bhi;
+    span(a, alo, ahi);
+    span(b, blo, bhi);
+    return alo < bhi && blo < ahi;
+}
+
+extern "C" int csd_dbfv_reduce_chain(csd_ctx* ctx, const csd_view* x, const csd_view* src, size_t per, size_t digits,
+                                     const csd_view* consts, const csd_modulus* mods, size_t nmods, size_t limbs,
+                                     int64_t f0_lo, int64_t f0_hi, int64_t f1_lo, int64_t f1_hi, size_t shift, size_t log2_r,
+                                     uint32_t full_split, uint32_t full_carry) {
+    CSD_TRY check_view(x, "dbfv_reduce_chain x");
+    check_view(src, "dbfv_reduce_chain src");
+    check_view(consts, "dbfv_reduce_chain constants");
+    RcArgs a;
+    memset(&a, 0, sizeof(a));
+    if (per < 1) throw CsdError("dbfv_reduce_chain: at least one row per digit");
+    if (digits < 1 || digits > 32) throw CsdError("dbfv_reduce_chain: 1 to 32 digits");
+    if (limbs != 1 && limbs != 2) throw CsdError("dbfv_reduce_chain: the carries have 1 or 2 limbs");
+    if (log2_r < 1 || log2_r > (limbs == 1 ? 62u : 126u)) throw CsdError("dbfv_reduce_chain: log2_r out of range for the limbs");
+    if (shift < 1 || shift >= 127) throw CsdError("dbfv_reduce_chain: shift out of range");
+    const int64_t r = x->shape[1];
+    if (r < 1 || r > CSD_MAX_MODULI || (size_t)r != nmods) throw CsdError("dbfv_reduce_chain: one modulus per limb of x");
+    if (x->shape[0] != (int64_t)(per * digits)) throw CsdError("dbfv_reduce_chain: x must hold digits x per rows");
+    if (src->shape[0] % (int64_t)per != 0 || src->shape[0] > x->shape[0]) throw CsdError("dbfv_reduce_chain: src rows");
+    for (int d = 1; d < 4; d++)
+        if (src->shape[d] != x->shape[d]) throw CsdError("dbfv_reduce_chain: src shape differs from x");
+    if (x->strides[3] != 1 || src->strides[3] != 1) throw CsdError("dbfv_reduce_chain: dimension 3 must be contiguous");
+    const int64_t rows = 3 + 3 * (int64_t)limbs;
+    if (consts->shape[0] != 1 || consts->shape[1] != 1 || consts->shape[2] != rows || consts->shape[3] != r ||
+        consts->strides[2] != r || consts->strides[3] != 1)
+        throw CsdError("dbfv_reduce_chain: constants must be a contiguous [1, 1, 3 + 3 limbs, r] table");
+    // a thread reads its src rows and writes the same rows of x, so src may be x itself (same
+    // rows and strides); any other sharing of memory is refused
+    bool same = src->ptr == x->ptr;
+    for (int d = 0; d < 4; d++) same = same && src->strides[d] == x->strides[d];
+    if (numel(x) == 0) return 0;
+    if (!same && numel(src) > 0 && rc_spans_meet(*x, *src)) throw CsdError("dbfv_reduce_chain: src overlaps x");
+    a.per = (int64_t)per;
+    a.n2 = x->shape[2];
+    a.n3 = x->shape[3];
+    for (int d = 0; d < 3; d++) {
+        a.sx[d] = x->strides[d];
+        a.si[d] = src->strides[d];
+    }
+    a.x = (int64_t*)x->ptr;
+    a.src = (const int64_t*)src->ptr;
+    a.k = (const int64_t*)consts->ptr;
+    a.digits = (int)digits;
+    a.d = (int)(src->shape[0] / (int64_t)per);
+    a.full_split = full_split;
+    a.full_carry = full_carry;
+    a.st = SoStep{(int)r, (int)limbs, (int)shift, (int)log2_r, (i128)(((u128)(uint64_t)f0_hi << 64) | (uint64_t)f0_lo),
+                  (i128)(((u128)(uint64_t)f1_hi << 64) | (uint64_t)f1_lo)};
+    ModTable t = make_mod_table(mods, nmods, r);
+    TimedScope ts(ctx, "dbfv_reduce_chain", 8.0 * (double)a.per * a.n2 * a.n3 * r * (double)(digits + a.d));
+    const size_t lds = ((size_t)r * 256 + (size_t)rows * r) * sizeof(int64_t);
+    csd_allow_lds((const void*)k_dbfv_reduce_chain, 160 * 1024);
+    const int64_t total = a.per * a.n2 * a.n3;
+    if ((total + 255) / 256 > (int64_t)INT32_MAX) throw CsdError("dbfv_reduce_chain: too many coefficients for one launch");
+    hipLaunchKernelGGL(k_dbfv_reduce_chain, dim3((unsigned)((total + 255) / 256)), dim3(256), lds, ctx->stream, a, t);
     HIPCHECK(hipGetLastError());
     CSD_CATCH
 }

@@ -205,12 +205,42 @@ def split_overflow(ring, data, x, x_bound, base, fused=True):
     return overflow
 
 
+# whether the fused chain runs as one launch (csd_dbfv_reduce_chain) or one launch per digit
+# (csd_dbfv_split_overflow); both give identical residues
+_CHAIN_ONE_LAUNCH = [True]
+
+
+def _chain_one_launch(ring, modulus, data, result, src, d, per_digit, ct_bound, base):
+    """the fused carry chain as one kernel: every digit's carry step and split, with the checks
+    and reduction kinds _split_fused takes for it digit by digit"""
+    limbs = data.limbs()
+    unbounded = [data.max_p << 63] * limbs
+    full_split, full_carry = [], []
+    for i in range(modulus.digits()):
+        input_row_bound = ct_bound if i < d else 1
+        x_bound = input_row_bound
+        if i > 0:
+            _assert_mul_mont_safe(ring, input_row_bound, base)
+            full_carry.append(_needs_full_reduction(data, unbounded + [input_row_bound * data.max_p * data.max_p]))
+            x_bound = 1
+        else:
+            full_carry.append(False)
+        _assert_mul_mont_safe(ring, x_bound, base)
+        full_split.append(_needs_full_reduction(data, unbounded + [x_bound * data.max_p * data.max_p]))
+    ring.backend.dbfv_reduce_chain(result, src[0:d * per_digit], per_digit, modulus.digits(), data.consts,
+                                   ring.mods_for(base), limbs, data.q_mod_r, data.neg_q_inv_mod_r, data.log2_gamma,
+                                   data.log2_r, full_split, full_carry)
+
+
 def _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base, fused, src=None):
     """the carry chain of dbfv_reduce (dbfv.rs:990-1023) over `result`, whose digit i occupies
     rows [i per_digit, (i + 1) per_digit) and whose first d digits come from the input: already
     copied into `result`, or read from `src` (same layout) by the fused kernels, which then
     take the digits beyond the input as zero instead of reading zeroed rows"""
     data = split_overflow_data(ring, modulus, base)
+    if fused and src is not None and _CHAIN_ONE_LAUNCH[0] and modulus.digits() <= 32:
+        _chain_one_launch(ring, modulus, data, result, src, d, per_digit, ct_bound, base)
+        return
     delta_mont2 = data.delta_mont2
     carry = None
     rows = range(per_digit)

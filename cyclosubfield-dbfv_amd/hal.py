@@ -500,6 +500,27 @@ class HipBackend:
                                             ctypes.c_size_t(shift), ctypes.c_size_t(log2_r),
                                             ctypes.c_int(1 if full_split else 0), ctypes.c_int(1 if full_carry else 0)))
 
+    def dbfv_reduce_chain(self, x, src, per, digits, consts, mods, limbs, f0, f1, shift, log2_r, full_split, full_carry):
+        """csd_dbfv_reduce_chain: the carry chain of dbfv_reduce (dbfv.rs:990-1023) in one launch;
+        digit i of x is rows [i per, (i + 1) per), read from the same rows of src (zero beyond its
+        rows). full_split / full_carry are per-digit lists of the double-reduction choices."""
+        def words(v):
+            v &= (1 << 128) - 1
+            lo, hi = v & ((1 << 64) - 1), v >> 64
+            s = lambda w: ctypes.c_int64(w - (1 << 64) if w >= 1 << 63 else w)
+            return s(lo), s(hi)
+
+        def mask(flags):
+            return ctypes.c_uint32(sum(1 << i for i, f in enumerate(flags) if f))
+
+        arr, n = _mods(mods)
+        f0l, f0h = words(f0)
+        f1l, f1h = words(f1)
+        check(lib().csd_dbfv_reduce_chain(self.ctx, ctypes.byref(x.view()), ctypes.byref(src.view()), ctypes.c_size_t(per),
+                                          ctypes.c_size_t(digits), ctypes.byref(consts.view()), arr, n,
+                                          ctypes.c_size_t(limbs), f0l, f0h, f1l, f1h, ctypes.c_size_t(shift),
+                                          ctypes.c_size_t(log2_r), mask(full_split), mask(full_carry)))
+
     def host_alloc(self, shape):
         """pinned host memory for a host-resident int64 tensor (PinnedHostTensor)"""
         return PinnedHostTensor(self, shape)

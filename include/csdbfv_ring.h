@@ -207,6 +207,19 @@ int csd_dbfv_split_overflow(csd_ctx* ctx, const csd_view* x, const csd_view* x_i
                             const csd_view* carry_in, const csd_view* consts, const csd_modulus* mods, size_t nmods,
                             int64_t f0_lo, int64_t f0_hi, int64_t f1_lo, int64_t f1_hi, size_t shift, size_t log2_r,
                             int full_split, int full_carry);
+/* The carry chain of dbfv_reduce (dbfv.rs:990-1023) in one launch: digit i of the result x
+ * ([digits per, r, n2, n3], digit i in rows [i per, (i + 1) per)) is the split of the input digit
+ * (the same rows of src for i < src rows / per, zero beyond) after the carry step with the overflow
+ * of digit i - 1; the last digit's overflow is dropped. Each (row, position) runs the digits in
+ * turn with its carry in registers. consts, f0, f1, shift and log2_r as for
+ * csd_dbfv_split_overflow; limbs is the carries' limb count; bit i of full_split / full_carry
+ * selects digit i's double reductions. src may be x itself (same view), else must not share
+ * memory with it. Bit-identical to csd_dbfv_split_overflow digit by digit. No counterpart in the
+ * reference's HAL (a fused form of its dbfv.rs sequence). */
+int csd_dbfv_reduce_chain(csd_ctx* ctx, const csd_view* x, const csd_view* src, size_t per, size_t digits,
+                          const csd_view* consts, const csd_modulus* mods, size_t nmods, size_t limbs, int64_t f0_lo,
+                          int64_t f0_hi, int64_t f1_lo, int64_t f1_hi, size_t shift, size_t log2_r, uint32_t full_split,
+                          uint32_t full_carry);
 /* The same key switch with the keys in pinned host memory instead of HBM, the fallback for key
  * sets larger than the device (the CPU build streams them from its NVMe KeyStore,
  * keystore.rs:50-127): host_keys[t] points to the contiguous [2*digits, r, ., .] key of rotation t

@@ -2,8 +2,8 @@
 dbfv.rs:873-1025), bit-exact against the oracle's restatement (oracle/dbfv.py, itself pinned by
 the reference's property test and the carry-chain invariant) on the same seeded inputs: one-limb
 and two-limb overflows (R = t/b up to 2^100), partial digit counts, unreduced inputs, and the
-batched form against the per-ciphertext form; both the fused one-kernel-per-digit form and the
-reference's HAL call sequence."""
+batched form against the per-ciphertext form; the fused chain as one kernel, the fused form with
+one kernel per digit, and the reference's HAL call sequence."""
 import numpy as np
 import pytest
 
@@ -121,6 +121,55 @@ def test_dbfv_reduce_batch_matches_single(gpu, rings, field, log2_q, log2_b, log
     for cidx in range(count):
         single, _ = pkg.dbfv.dbfv_reduce(g, mg, to_dev(gpu, x[cidx::count]), 1, base, fused=False)
         assert same(out[cidx::count], from_dev(gpu, single))
+
+
+@pytest.mark.parametrize("field,log2_q,log2_b,log2_t,d,count,bound", [("field_192", (200, 220), 16, 64, 2, 3, 2),
+                                                                    ("field_192", (260, 280), 27, 108, 4, 2, 1),
+                                                                    ("field_36864", (960, 980), 8, 64, 3, 2, 2)])
+def test_reduce_chain_one_launch_matches_per_digit(gpu, rings, field, log2_q, log2_b, log2_t, d, count, bound):
+    """the whole carry chain in one launch (csd_dbfv_reduce_chain) writes what one fused launch
+    per digit writes, and leaves its input untouched"""
+    pkg = load()
+    g, _ = rings(field, log2_q, log2_t)
+    base = g.initial_rns_base()
+    mg, _ = moduli(log2_b, log2_t)
+    host = inputs(g, base, 2 * d * count, 53 + d, bound)
+    src = to_dev(gpu, host)
+    one, _ = pkg.dbfv.dbfv_reduce_batch(g, mg, src, count, bound, base)
+    pkg.dbfv._CHAIN_ONE_LAUNCH[0] = False
+    try:
+        per_digit, _ = pkg.dbfv.dbfv_reduce_batch(g, mg, src, count, bound, base)
+    finally:
+        pkg.dbfv._CHAIN_ONE_LAUNCH[0] = True
+    assert same(from_dev(gpu, one), from_dev(gpu, per_digit))
+    assert same(from_dev(gpu, src), host)
+
+
+def test_reduce_chain_errors(gpu, rings):
+    """csd_dbfv_reduce_chain refuses row counts, limbs and overlaps it cannot run"""
+    pkg = load()
+    g, _ = rings("field_192", (200, 220), 64)
+    base = g.initial_rns_base()
+    mg, _ = moduli(16, 64)
+    data = pkg.dbfv.split_overflow_data(g, mg, base)
+    mods = g.mods_for(base)
+    src = to_dev(gpu, inputs(g, base, 4, 11, 1))
+    out = gpu.allocate_i64_uninit([8] + list(src.shape[1:]))
+
+    def run(x, s, per=2, digits=4, limbs=1):
+        gpu.dbfv_reduce_chain(x, s, per, digits, data.consts, mods, limbs, data.q_mod_r, data.neg_q_inv_mod_r,
+                              data.log2_gamma, data.log2_r, [False] * digits, [False] * digits)
+
+    run(out, src)  # the valid call
+    for args in [dict(digits=3),  # x holds 4 digits
+                 dict(per=3),  # rows not a multiple of per
+                 dict(limbs=3),
+                 dict(digits=0)]:
+        with pytest.raises(pkg.HalError):
+            run(out, src, **args)
+    with pytest.raises(pkg.HalError):
+        run(out, out[2:6])  # src inside x, other rows
+    run(out, out[0:4])  # src as x's own first rows: read and written by the same thread
 
 
 def test_dbfv_reduce_edges(gpu, rings):
