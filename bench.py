@@ -482,9 +482,10 @@ def bench_bootstrap36864(pkg, be, reps=3, batch=8, seed=31, cpu=True):
 
     ms_mfma, ph_mfma, res_mfma = run(2, reps)
     ms_plain, ph_plain, res_plain = run(1, reps)
+    ms_all, _, res_all = run(3, reps)
     ring.set_bconv_form(2)
     same = all((be.hal_export(a) == be.hal_export(b)).all()
-               for a, b in ((res_mfma.data, res_plain.data),))
+               for a, b in ((res_mfma.data, res_plain.data), (res_all.data, res_plain.data)))
     if not same:
         raise SystemExit("bench: the bootstrap's matrix-core and VALU conversion forms differ")
     dec = K.dbfv_dec(ring, modulus, res_mfma, sk)
@@ -520,13 +521,16 @@ def bench_bootstrap36864(pkg, be, reps=3, batch=8, seed=31, cpu=True):
                     "17 primes, dBFV (8, 8) -> intermediate 2^76, v = 12, sparse weight 32, 27 Galois keys x 17 "
                     "digits, relinearization key of 9 digits; one ciphertext encrypting 2025",
         "value": round(ms_mfma, 1), "unit": "ms per bootstrap", "higher_is_better": False,
-        "form": "mfma: the BEHZ conversions' matrix products on int8 MFMA (bconv form 2), every other step as in "
-                "the plain form",
+        "form": "mfma: the BEHZ conversions' matrix products on int8 MFMA where that measured faster (bconv form 2: "
+                "conversions from 9 or more source primes, rescale steps from 17 or more; the VALU kernel below), "
+                "every other step as in the plain form",
         "plain": {"ms": round(ms_plain, 1), "form": "plain: the conversions as fused VALU kernels (bconv form 1)"},
         "mfma_over_plain": round(ms_plain / ms_mfma, 3),
+        "mfma_every_conversion": {"ms": round(ms_all, 1), "over_plain": round(ms_plain / ms_all, 3),
+                                  "form": "every conversion's matrix product on int8 MFMA (bconv form 3)"},
         "phases_ms": ph_mfma, "phases_ms_plain": ph_plain,
         "batch": {"ciphertexts": batch, "ms": round(batch_ms, 1), "ms_per_ciphertext": round(batch_ms / batch, 1)},
-        "checks": "both forms bit-identical; the result decrypts to 2025 in all 3072 slots (extract_slot_values, "
+        "checks": "the three forms bit-identical; the result decrypts to 2025 in all 3072 slots (extract_slot_values, "
                   "homcrt.rs:250-274); batch ciphertexts 0 and last decrypt to their plaintexts",
         "transform_cost": {"hoisted_automorphisms": h, "unhoisted_automorphisms": u},
         "setup_s": round(setup_s, 2),

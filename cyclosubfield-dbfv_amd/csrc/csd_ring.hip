@@ -535,7 +535,7 @@ struct csd_ring {
     // the ring's execution-form settings (every form computes identical residues): see
     // csd_ring_set_overlap, csd_ring_set_bconv_form and csd_ring_set_gadget_form
     std::atomic<bool> overlap_halves{false};
-    std::atomic<int> bconv_form{2};  // csd_ring_set_bconv_form: 0 HAL sequence, 1 fused, 2 fused on the matrix cores
+    std::atomic<int> bconv_form{2};  // csd_ring_set_bconv_form: 0 HAL sequence, 1 fused, 2 / 3 fused on the matrix cores
     std::atomic<bool> gadget_fused{true};
 
     ~csd_ring() {
@@ -814,6 +814,16 @@ struct csd_ring {
         return ref;
     }
 
+    // Whether form 2 takes a conversion from r source limbs to the matrix cores. One MFMA covers a
+    // 32-byte K slice, source limb u in byte u: with few limbs most of it is padding, while the
+    // lift, carry chain and reduction tails stay. Measured at field_36864 (bootstrap shapes, MI355X),
+    // the MFMA kernel against the fused VALU kernel: r <= 4 takes 1.9 x as long, r = 5-8 1.2-1.4 x,
+    // r = 9-12 0.96-1.1 x and r >= 13 0.75-0.97 x; in the rescale form, which adds two products per
+    // target row to both, r = 9-16 takes 1.01-1.09 x and r >= 17 1.0 x. Below these thresholds form 2
+    // runs the VALU kernel, whose residues are the same; form 3 takes every conversion whose tables
+    // fit to the matrix cores.
+    static bool mfma_pays(int64_t r, bool rescale) { return rescale ? r >= 17 : r >= 9; }
+
     // BConvCache::bconv (ring.rs:1663-1701). The fused form (default, when src and dst do not
     // overlap) is one kernel and leaves src as it is; the HAL sequence destroys src, like the
     // reference, unless preserve_src is set, when it runs on a copy. The form is decided once per
@@ -828,7 +838,8 @@ struct csd_ring {
             throw CsdError("bconv: destination shape");
         auto smods = mods_for(sbase), dmods = mods_for(dbase);
         const int form = bconv_form;
-        if (form == 2 && c.mfma_atab && c.acc >= (size_t)r && !views_overlap(dst, src_in, 8)) {
+        if ((form == 3 || (form == 2 && mfma_pays(r, false))) && c.mfma_atab && c.acc >= (size_t)r &&
+            !views_overlap(dst, src_in, 8)) {
             csd_bconv_mfma_launch(ctx, dst, src_in, (const int64_t*)c.q_over_p_inv->p, (const int8_t*)c.mfma_atab->p,
                                   (const int64_t*)c.mfma_rows->p, c.log2_qtilde, (const CsdMod*)c.src_mods->p, "bconv_mfma");
             return;
@@ -891,7 +902,7 @@ struct csd_ring {
         if (!scalar_ok(s_src, r) || !scalar_ok(s_ext, re) || !scalar_ok(inv, re)) throw CsdError("rescale: scalar tensor shape");
         auto smods = mods_for(sbase), emods = mods_for(ebase);
         const int form = bconv_form;
-        if (form == 2 && c.mfma_atab && c.acc >= (size_t)r && limb_disjoint(ext, src)) {
+        if ((form == 3 || (form == 2 && mfma_pays(r, true))) && c.mfma_atab && c.acc >= (size_t)r && limb_disjoint(ext, src)) {
             csd_bconv_mfma_launch(ctx, ext, src, (const int64_t*)c.q_over_p_inv->p, (const int8_t*)c.mfma_atab->p,
                                   (const int64_t*)c.mfma_rows->p, c.log2_qtilde, (const CsdMod*)c.src_mods->p, "rescale_step_mfma",
                                   (const int64_t*)s_src.ptr, (const int64_t*)s_ext.ptr, (const int64_t*)inv.ptr);
@@ -1611,7 +1622,9 @@ extern "C" int csd_ring_get_bconv_form(const csd_ring* ring) { return ring ? rin
 
 extern "C" int csd_ring_set_bconv_form(csd_ring* ring, int form) {
     CSD_TRY if (!ring) throw CsdError("null ring");
-    if (form < 0 || form > 2) throw CsdError("bconv form must be 0 (HAL sequence), 1 (fused) or 2 (fused, matrix cores)");
+    if (form < 0 || form > 3)
+        throw CsdError("bconv form must be 0 (HAL sequence), 1 (fused), 2 (fused, matrix cores where they pay) or 3 (fused, "
+                       "matrix cores wherever the tables fit)");
     ring->bconv_form = form;
     CSD_CATCH
 }

@@ -136,13 +136,14 @@ class RingContext:
     def set_bconv_form(self, form):
         """How this ring's bconv_tensors, rescale_step (and the conversions inside its sequences)
         run (csd_ring_set_bconv_form): 2 (default) one kernel with the matrix product on the int8
-        matrix cores, 1 one fused kernel on the vector ALUs, 0 the reference's HAL call sequence;
-        residues are identical."""
+        matrix cores where they are the faster (from 9 source primes, 17 for the rescale step), 3
+        the matrix-core kernel wherever its tables fit, 1 one fused kernel on the vector ALUs, 0 the
+        reference's HAL call sequence; residues are identical."""
         check(lib().csd_ring_set_bconv_form(self.handle, ctypes.c_int(int(form))))
 
     def bconv_form(self):
-        """the current csd_ring_set_bconv_form setting of this ring (2 matrix cores, 1 fused, 0 HAL
-        sequence)"""
+        """the current csd_ring_set_bconv_form setting of this ring (3 / 2 matrix cores, 1 fused, 0
+        HAL sequence)"""
         return lib().csd_ring_get_bconv_form(self.handle)
 
     def set_gadget_form(self, form):

@@ -90,12 +90,14 @@ int csd_ring_rescale_step(csd_ring* ring, const csd_view* ext, const size_t* ext
                           const csd_view* src, const size_t* src_base, size_t r_src, const csd_view* s_src,
                           const csd_view* s_ext, const csd_view* inv);
 /* 2 (default): csd_ring_bconv, csd_ring_rescale_step and the conversions inside the ring
- * sequences run as one kernel whose matrix product is on the int8 matrix cores (bases of at most
- * 32 source and 31 target primes; others take form 1); 1: one fused kernel on the vector ALUs;
- * 0: the reference's HAL call sequence (ring.rs:1663-1701). Residues are identical in every form
- * (no reference counterpart). */
+ * sequences run as one kernel whose matrix product is on the int8 matrix cores where that measured
+ * at least as fast as the vector ALUs (conversions from 9 or more source primes, rescale steps from
+ * 17 or more), else as form 1; 3: the matrix-core kernel for every conversion whose tables fit (at
+ * most 32 source and 31 target primes; others take form 1); 1: one fused kernel on the vector
+ * ALUs; 0: the reference's HAL call sequence (ring.rs:1663-1701). Residues are identical in every
+ * form (no reference counterpart). */
 int csd_ring_set_bconv_form(csd_ring* ring, int form);
-/* the current csd_ring_set_bconv_form setting (2 matrix cores, 1 fused, 0 HAL sequence) */
+/* the current csd_ring_set_bconv_form setting (3 / 2 matrix cores, 1 fused, 0 HAL sequence) */
 int csd_ring_get_bconv_form(const csd_ring* ring);
 /* bconv_gadget_tensors (ring.rs:1548-1557, GadgetBConvCache::bconv 1809-1851); digits are
  * half-open position ranges [digit_bounds[2i], digit_bounds[2i+1]). src is [B, r, ., .] and dst

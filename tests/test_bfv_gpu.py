@@ -176,7 +176,7 @@ def _oracle_rescale_step(c, x, lifted, base_len, t):
     return x[:, base_len:]
 
 
-@pytest.mark.parametrize("form", [2, 1, 0])
+@pytest.mark.parametrize("form", [3, 2, 1, 0])
 @pytest.mark.parametrize("layout", ["split", "separate", "ext_first"])
 @pytest.mark.parametrize("field,log2_q,t_bits,ndigits", CASES[1:])
 def test_rescale_step_parity(gpu, cpu, rings, field, log2_q, t_bits, ndigits, layout, form):

@@ -113,11 +113,11 @@ def test_field18_square_kat_on_device(gpu, rings):
             assert int(out[blk * 6 + j]) % p == R.montgomery_repr(exp[j], p)
 
 
-@pytest.fixture(params=[2, 1, 0], ids=["mfma", "fused", "hal_sequence"])
+@pytest.fixture(params=[3, 1, 0], ids=["mfma", "fused", "hal_sequence"])
 def bconv_form(request):
-    """runs a test with the matrix-core conversion kernel, the fused VALU kernel and the reference's
-    HAL sequence: the test calls the yielded function on each ring it uses, which is reset to the
-    default after"""
+    """runs a test with the matrix-core conversion kernel at every shape (form 3), the fused VALU
+    kernel and the reference's HAL sequence: the test calls the yielded function on each ring it
+    uses, which is reset to the default after"""
     used = []
 
     def apply(ring):
@@ -189,7 +189,8 @@ def test_bconv_parity(gpu, cpu, rings, bconv_form):
 def test_bconv_matrix_core_path_is_taken(gpu, rings):
     """The default conversion form runs the matrix-core kernel (its own timer name) for the lift of
     field_36864 and field_49152 to their extensions, the reverse conversions and the rescale step;
-    form 1 runs the vector-ALU kernel for the same calls."""
+    form 1 runs the vector-ALU kernel for the same calls. A conversion from 5 primes stays on the
+    vector ALUs in the default form and takes the matrix cores in form 3."""
     for field, log2_q in (("field_36864", (960, 980)), ("field_49152", (1280, 1300))):
         g, _ = rings(field, log2_q, 76)
         assert g.bconv_form() == 2
@@ -217,6 +218,21 @@ def test_bconv_matrix_core_path_is_taken(gpu, rings):
             rep = gpu.timing_report()
             assert names <= set(rep), (form, sorted(rep))
             assert rep.get("bconv_mfma" if form == 2 else "bconv", (0,))[0] == 2
+        # a conversion from 5 primes: the vector ALUs in form 2 (faster there), the matrix cores in form 3
+        src5 = g.allocate(1, base[:5])
+        g.fill_uniform(src5, base[:5], 4)
+        d4 = gpu.hal_allocate_i64((1, 4, n2, n3))
+        for form, name in ((2, "bconv"), (3, "bconv_mfma")):
+            g.set_bconv_form(form)
+            gpu.timing_report()
+            gpu.set_timing(True)
+            try:
+                g.bconv_tensors(d4, base[5:9], src5, base[:5])
+                gpu.sync()
+            finally:
+                gpu.set_timing(False)
+                g.set_bconv_form(2)
+            assert set(gpu.timing_report()) == {name}, form
 
 
 def test_bconv_parity_field49152(gpu, cpu, rings, bconv_form):
@@ -372,7 +388,7 @@ def test_transform_round_trip_full_batch(gpu, cpu, rings):
     assert (np.mod(back, primes) == data).all()
 
 
-@pytest.mark.parametrize("form", [2, 1, 0])
+@pytest.mark.parametrize("form", [3, 2, 1, 0])
 def test_bconv_preserve_src(gpu, cpu, rings, form):
     """csd_ring_bconv with preserve_src leaves the source unchanged in every conversion form, with
     the same residues as the destroying call"""
