@@ -158,11 +158,19 @@ __global__ __launch_bounds__(256) void k_split_overflow(SoArgs a, ModTable mt) {
     }
 }
 
-// The whole carry chain of dbfv_reduce (dbfv.rs:990-1023) in one launch: each thread takes one
-// coefficient position of one row of a digit through every digit in turn, the carry of digit i
-// held in registers for digit i + 1 (the chain is sequential along the digits only). Digit i of
-// the result is rows [i per, (i + 1) per) of x, read from the same rows of src for i < d and zero
-// beyond; the carry of the last digit is dropped, as the reference drops it.
+// The whole carry chain of dbfv_reduce (dbfv.rs:990-1023) in one launch: each coefficient
+// position of one row of a digit is taken through every digit in turn, the carry of digit i held in
+// registers for digit i + 1 (the chain is sequential along the digits only). Digit i of the result
+// is rows [i per, (i + 1) per) of x, read from the same rows of src for i < d and zero beyond; the
+// carry of the last digit is dropped, as the reference drops it.
+//
+// Two kernels: for large batches one lane per position runs so_digit digit after digit
+// (k_dbfv_reduce_chain1); for small ones four lanes share a position (k_dbfv_reduce_chain4), lane
+// quarter q owning primes q, q + 4, ...: a digit's residues stay in registers, the three i128 sums
+// of the matmul rows are added across the quarters (wrapping adds, so the order does not change
+// them) and every quarter forms the same overflow. With one ciphertext that gives four times the
+// waves, which the per-digit latency (loads, then two passes over the primes) needs; a wave holds
+// 16 positions. Both give the residues of so_digit.
 struct RcArgs {
     int64_t per, n2, n3;
     int64_t sx[3], si[3];
@@ -174,7 +182,26 @@ struct RcArgs {
     SoStep st;
 };
 
-__global__ __launch_bounds__(256) void k_dbfv_reduce_chain(RcArgs a, ModTable mt) {
+#define RC_Q 4
+#define RC_MAX_R 40  // primes of one chain (the registers of a digit: RC_MAXP per lane)
+#define RC_MAXP (RC_MAX_R / RC_Q)
+// positions (rows x coefficients) from which one lane per position has waves enough: 2^18 positions
+// are 4096 waves, four per SIMD of the 256 CUs
+#define RC_ONE_LANE_FROM ((int64_t)1 << 18)
+
+__device__ __forceinline__ u128 rc_sum_quarters(u128 v) {
+    uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
+#pragma unroll
+    for (int m = 16; m <= 32; m *= 2) {
+        const uint64_t olo = __shfl_xor(lo, m), ohi = __shfl_xor(hi, m);
+        const u128 t = (((u128)hi << 64) | lo) + (((u128)ohi << 64) | olo);
+        lo = (uint64_t)t;
+        hi = (uint64_t)(t >> 64);
+    }
+    return ((u128)hi << 64) | lo;
+}
+
+__global__ __launch_bounds__(256) void k_dbfv_reduce_chain1(RcArgs a, ModTable mt) {
     extern __shared__ int64_t xs[];
     const int64_t npos = a.n2 * a.n3;
     const int64_t total = a.per * npos;
@@ -191,6 +218,101 @@ __global__ __launch_bounds__(256) void k_dbfv_reduce_chain(RcArgs a, ModTable mt
             const int64_t* XI = zero_in ? X : a.src + row * a.si[0] + j2 * a.si[2] + j3;
             so_digit(a.st, mt, K, xs + threadIdx.x, (int)blockDim.x, XI, a.si[1], zero_in, i > 0, (a.full_split >> i) & 1,
                      (a.full_carry >> i) & 1, X, a.sx[1], cl0, cl1);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dbfv_reduce_chain4(RcArgs a, ModTable mt) {
+    extern __shared__ int64_t xs[];
+    const int r = a.st.r, limbs = a.st.limbs;
+    const int trows = 3 + 3 * limbs;
+    int64_t* K = xs;                          // the constant table [trows][r]
+    CsdMod* M = (CsdMod*)(xs + trows * r);    // the moduli [r]
+    for (int q = threadIdx.x; q < trows * r; q += blockDim.x) K[q] = a.k[q];
+    for (int p = threadIdx.x; p < r; p += blockDim.x) M[p] = mt.m[p];
+    __syncthreads();
+    const int64_t npos = a.n2 * a.n3;
+    const int64_t total = a.per * npos;
+    const int lane = threadIdx.x & 63;
+    const int qd = lane >> 4;
+    const int64_t e = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16 + (lane & 15);
+    // lanes past the end work on the last position (they take part in the cross-lane sums) and
+    // write nothing
+    const bool act = e < total;
+    const int64_t ec = act ? e : total - 1;
+    const int64_t c = ec / npos, j = ec - c * npos;
+    const int64_t j2 = j / a.n3, j3 = j - j2 * a.n3;
+    const u128 half = (u128)1 << (a.st.shift - 1);
+    int64_t cl0 = 0, cl1 = 0;
+    for (int i = 0; i < a.digits; i++) {
+        const int64_t row = (int64_t)i * a.per + c;
+        int64_t* X = a.x + row * a.sx[0] + j2 * a.sx[2] + j3;
+        const bool zero_in = i >= a.d;
+        const bool has_carry = i > 0;
+        const bool full_carry = (a.full_carry >> i) & 1, full_split = (a.full_split >> i) & 1;
+        int64_t xv[RC_MAXP];
+        if (!zero_in) {
+            const int64_t* XI = a.src + row * a.si[0] + j2 * a.si[2] + j3;
+#pragma unroll
+            for (int k = 0; k < RC_MAXP; k++) {
+                const int p = qd + RC_Q * k;
+                if (p < r) xv[k] = XI[p * a.si[1]];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < RC_MAXP; k++) xv[k] = 0;
+        }
+        // pass 1: the carry step (dbfv.rs:995-1003), then the lifts and the matmul rows
+        // (dbfv.rs:909-917), over this quarter's primes
+        u128 s0 = 0, s1 = 0, sg = 0;
+#pragma unroll
+        for (int k = 0; k < RC_MAXP; k++) {
+            const int p = qd + RC_Q * k;
+            if (p < r) {
+                const CsdMod m = M[p];
+                int64_t v = xv[k];
+                if (has_carry) {
+                    u128 acc = (u128)((i128)cl0 * (i128)K[row_delta(limbs, 0) * r + p]);
+                    if (limbs == 2) acc += (u128)((i128)cl1 * (i128)K[row_delta(limbs, 1) * r + p]);
+                    acc += (u128)((i128)v * (i128)K[row_one(limbs) * r + p]);
+                    v = full_carry ? csd_to_mont64(m, csd_double_red128(m, (i128)acc)) : csd_red128(m, (i128)acc);
+                    xv[k] = v;
+                }
+                const int64_t lift = csd_mont_mul(m, v, K[row_lift() * r + p]);
+                s0 += (u128)((i128)K[row_value(0) * r + p] * (i128)lift);
+                if (limbs == 2) s1 += (u128)((i128)K[row_value(1) * r + p] * (i128)lift);
+                sg += (u128)((i128)K[row_corr(limbs) * r + p] * (i128)lift);
+            }
+        }
+        s0 = rc_sum_quarters(s0);
+        if (limbs == 2) s1 = rc_sum_quarters(s1);
+        sg = rc_sum_quarters(sg);
+        // the overflow (bconv_correct_pow2{,_wide}, dbfv.rs:919-938), the same in every quarter
+        const i128 quot = (i128)(sg + half) >> a.st.shift;
+        int64_t ov0, ov1 = 0;
+        if (limbs == 1) {
+            const i128 v = (i128)((s0 - (u128)quot * (u128)a.st.f0) * (u128)a.st.f1);
+            ov0 = (int64_t)csd_pow2_centre(v, a.st.log2_r);
+        } else {
+            const i128 v = (i128)(s0 + (s1 << 64));
+            const i128 cc = csd_pow2_centre((i128)(((u128)v - (u128)quot * (u128)a.st.f0) * (u128)a.st.f1), a.st.log2_r);
+            ov0 = (int64_t)cc;
+            ov1 = (int64_t)((i128)((u128)cc - (u128)(i128)ov0) >> 64);
+        }
+        cl0 = ov0;
+        cl1 = ov1;
+        // pass 2: x_lo = reduce(overflow mont^2(-b Delta) + x mont(1)) (dbfv.rs:940-947)
+#pragma unroll
+        for (int k = 0; k < RC_MAXP; k++) {
+            const int p = qd + RC_Q * k;
+            if (p < r) {
+                const CsdMod m = M[p];
+                u128 acc = (u128)((i128)ov0 * (i128)K[row_nbd(limbs, 0) * r + p]);
+                if (limbs == 2) acc += (u128)((i128)ov1 * (i128)K[row_nbd(limbs, 1) * r + p]);
+                acc += (u128)((i128)xv[k] * (i128)K[row_one(limbs) * r + p]);
+                const int64_t out = full_split ? csd_to_mont64(m, csd_double_red128(m, (i128)acc)) : csd_red128(m, (i128)acc);
+                if (act) X[p * a.sx[1]] = out;
+            }
         }
     }
 }
@@ -299,7 +421,7 @@ extern "C" int csd_dbfv_reduce_chain(csd_ctx* ctx, const csd_view* x, const csd_
     if (log2_r < 1 || log2_r > (limbs == 1 ? 62u : 126u)) throw CsdError("dbfv_reduce_chain: log2_r out of range for the limbs");
     if (shift < 1 || shift >= 127) throw CsdError("dbfv_reduce_chain: shift out of range");
     const int64_t r = x->shape[1];
-    if (r < 1 || r > CSD_MAX_MODULI || (size_t)r != nmods) throw CsdError("dbfv_reduce_chain: one modulus per limb of x");
+    if (r < 1 || r > RC_MAX_R || (size_t)r != nmods) throw CsdError("dbfv_reduce_chain: one modulus per limb of x, at most 40");
     if (x->shape[0] != (int64_t)(per * digits)) throw CsdError("dbfv_reduce_chain: x must hold digits x per rows");
     if (src->shape[0] % (int64_t)per != 0 || src->shape[0] > x->shape[0]) throw CsdError("dbfv_reduce_chain: src rows");
     for (int d = 1; d < 4; d++)
@@ -333,11 +455,20 @@ extern "C" int csd_dbfv_reduce_chain(csd_ctx* ctx, const csd_view* x, const csd_
                   (i128)(((u128)(uint64_t)f1_hi << 64) | (uint64_t)f1_lo)};
     ModTable t = make_mod_table(mods, nmods, r);
     TimedScope ts(ctx, "dbfv_reduce_chain", 8.0 * (double)a.per * a.n2 * a.n3 * r * (double)(digits + a.d));
-    const size_t lds = ((size_t)r * 256 + (size_t)rows * r) * sizeof(int64_t);
-    csd_allow_lds((const void*)k_dbfv_reduce_chain, 160 * 1024);
     const int64_t total = a.per * a.n2 * a.n3;
-    if ((total + 255) / 256 > (int64_t)INT32_MAX) throw CsdError("dbfv_reduce_chain: too many coefficients for one launch");
-    hipLaunchKernelGGL(k_dbfv_reduce_chain, dim3((unsigned)((total + 255) / 256)), dim3(256), lds, ctx->stream, a, t);
+    if (total >= RC_ONE_LANE_FROM) {
+        // 256 threads, one position each, its residues parked in its LDS column
+        const size_t lds = ((size_t)r * 256 + (size_t)rows * r) * sizeof(int64_t);
+        csd_allow_lds((const void*)k_dbfv_reduce_chain1, 160 * 1024);
+        const int64_t blocks = (total + 255) / 256;
+        if (blocks > (int64_t)INT32_MAX) throw CsdError("dbfv_reduce_chain: too many coefficients for one launch");
+        hipLaunchKernelGGL(k_dbfv_reduce_chain1, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, a, t);
+    } else {
+        // 256 threads: 64 positions of four lanes each
+        const size_t lds = (size_t)rows * r * sizeof(int64_t) + (size_t)r * sizeof(CsdMod);
+        const int64_t blocks = (total + 63) / 64;
+        hipLaunchKernelGGL(k_dbfv_reduce_chain4, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, a, t);
+    }
     HIPCHECK(hipGetLastError());
     CSD_CATCH
 }

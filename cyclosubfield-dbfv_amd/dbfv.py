@@ -238,7 +238,7 @@ def _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base, fused, sr
     copied into `result`, or read from `src` (same layout) by the fused kernels, which then
     take the digits beyond the input as zero instead of reading zeroed rows"""
     data = split_overflow_data(ring, modulus, base)
-    if fused and src is not None and _CHAIN_ONE_LAUNCH[0] and modulus.digits() <= 32:
+    if fused and src is not None and _CHAIN_ONE_LAUNCH[0] and modulus.digits() <= 32 and len(base) <= 40:
         _chain_one_launch(ring, modulus, data, result, src, d, per_digit, ct_bound, base)
         return
     delta_mont2 = data.delta_mont2

@@ -215,8 +215,8 @@ int csd_dbfv_split_overflow(csd_ctx* ctx, const csd_view* x, const csd_view* x_i
  * of digit i - 1; the last digit's overflow is dropped. Each (row, position) runs the digits in
  * turn with its carry in registers. consts, f0, f1, shift and log2_r as for
  * csd_dbfv_split_overflow; limbs is the carries' limb count; bit i of full_split / full_carry
- * selects digit i's double reductions. src may be x itself (same view), else must not share
- * memory with it. Bit-identical to csd_dbfv_split_overflow digit by digit. No counterpart in the
+ * selects digit i's double reductions; at most 32 digits and 40 primes. src may be x itself (same
+ * view), else must not share memory with it. Bit-identical to csd_dbfv_split_overflow digit by digit. No counterpart in the
  * reference's HAL (a fused form of its dbfv.rs sequence). */
 int csd_dbfv_reduce_chain(csd_ctx* ctx, const csd_view* x, const csd_view* src, size_t per, size_t digits,
                           const csd_view* consts, const csd_modulus* mods, size_t nmods, size_t limbs, int64_t f0_lo,

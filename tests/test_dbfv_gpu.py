@@ -125,10 +125,12 @@ def test_dbfv_reduce_batch_matches_single(gpu, rings, field, log2_q, log2_b, log
 
 @pytest.mark.parametrize("field,log2_q,log2_b,log2_t,d,count,bound", [("field_192", (200, 220), 16, 64, 2, 3, 2),
                                                                     ("field_192", (260, 280), 27, 108, 4, 2, 1),
-                                                                    ("field_36864", (960, 980), 8, 64, 3, 2, 2)])
+                                                                    ("field_36864", (960, 980), 8, 64, 3, 2, 2),
+                                                                    ("field_36864", (960, 980), 27, 108, 2, 4, 2)])
 def test_reduce_chain_one_launch_matches_per_digit(gpu, rings, field, log2_q, log2_b, log2_t, d, count, bound):
     """the whole carry chain in one launch (csd_dbfv_reduce_chain) writes what one fused launch
-    per digit writes, and leaves its input untouched"""
+    per digit writes, and leaves its input untouched: four lanes per position for the small
+    batches, one lane per position from 2^18 positions (the last case, 8 rows of 36864)"""
     pkg = load()
     g, _ = rings(field, log2_q, log2_t)
     base = g.initial_rns_base()
