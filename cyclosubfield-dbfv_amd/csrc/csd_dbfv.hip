@@ -713,8 +713,13 @@ int64_t csd_encode_digits_launch(csd_ctx* ctx, const csd_view* v, size_t log2_b,
 // forms the same sum in the rows of a WideRingEls, so the reduction (red128, or double_red128 and
 // to_mont64 where the row's bound exceeds 2^63 min_p, decided on the host from the same bound
 // bookkeeping) gives the same residues. Operands are digit-major batches over one base.
+//
+// The factors may also be elements of L' (dbfv_acc_inner_product_element_lprime, dbfv.rs:378-396,
+// the baby-step sums of the homomorphic CRT transform, homcrt.rs:365-429): their digits are then
+// [1, r, n2, 1] CRT-basis tensors, constant along dimension 3, and the table holds one value per
+// (term, digit, limb, row of dimension 2).
 
-#define CB_MAX_TERMS 8
+#define CB_MAX_TERMS 16
 #define CB_MAX_ADDS 4
 #define CB_MAX_DIGITS 12
 
@@ -725,8 +730,9 @@ struct CombArgs {
     int ld[CB_MAX_TERMS];
     const int64_t* a[CB_MAX_ADDS];
     int lda[CB_MAX_ADDS];
-    const int64_t* sd;     // [nt][CB_MAX_DIGITS][r]: mont(digit j of scalar t) per limb
+    const int64_t* sd;     // [nt][CB_MAX_DIGITS][r][ns2]: digit j of factor t per limb (and row of dim 2)
     int nsd[CB_MAX_TERMS];
+    int64_t ns2, n3;       // ns2 = 1 (scalar factors) or n2 (L' factors)
     const int64_t* one;    // [r] mont(1)
     const int64_t* cst;    // [r] the constant's tensor, or null
     uint32_t full;         // bit 2 k + s: row kind of output digit k, component s
@@ -742,9 +748,11 @@ __global__ __launch_bounds__(256) void k_dbfv_combine(CombArgs a, ModTable mt) {
     i128 acc[CB_MAX_DIGITS][2];
 #pragma unroll
     for (int k = 0; k < CB_MAX_DIGITS; k++) acc[k][0] = acc[k][1] = 0;
+    const int64_t k2 = a.ns2 > 1 ? pos / a.n3 : 0;
+    const int64_t sdstep = a.r * a.ns2;  // from one digit of a factor to the next
     for (int t = 0; t < a.nt; t++) {
         const int64_t* xt = a.x[t];
-        const int64_t* sdt = a.sd + (int64_t)t * CB_MAX_DIGITS * a.r + j;
+        const int64_t* sdt = a.sd + ((int64_t)t * CB_MAX_DIGITS * a.r + j) * a.ns2 + k2;
         for (int i = 0; i < a.ld[t]; i++) {
             const int64_t x0 = xt[((2 * i) * a.count + c) * limb + off];
             const int64_t x1 = xt[((2 * i + 1) * a.count + c) * limb + off];
@@ -752,7 +760,7 @@ __global__ __launch_bounds__(256) void k_dbfv_combine(CombArgs a, ModTable mt) {
             for (int k = 0; k < CB_MAX_DIGITS; k++) {
                 const int jj = k - i;
                 if (k < a.dout && jj >= 0 && jj < a.nsd[t]) {
-                    const int64_t s = sdt[(int64_t)jj * a.r];
+                    const int64_t s = sdt[(int64_t)jj * sdstep];
                     acc[k][0] = (i128)((u128)acc[k][0] + (u128)((i128)x0 * (i128)s));
                     acc[k][1] = (i128)((u128)acc[k][1] + (u128)((i128)x1 * (i128)s));
                 }
@@ -788,9 +796,9 @@ __global__ __launch_bounds__(256) void k_dbfv_combine(CombArgs a, ModTable mt) {
 }
 
 void csd_dbfv_combine_launch(csd_ctx* ctx, const csd_view* out, int64_t count, int dout, const csd_view* terms,
-                             const int* term_digits, const int64_t* sd_dev, const int* term_sd, int nt, const csd_view* adds,
-                             const int* add_digits, int na, const int64_t* one_dev, const int64_t* cst_dev, uint32_t full,
-                             const csd_modulus* mods, size_t r) {
+                             const int* term_digits, const int64_t* sd_dev, int64_t ns2, const int* term_sd, int nt,
+                             const csd_view* adds, const int* add_digits, int na, const int64_t* one_dev,
+                             const int64_t* cst_dev, uint32_t full, const csd_modulus* mods, size_t r) {
     check_view(out, "dbfv_combine output");
     if (nt < 0 || nt > CB_MAX_TERMS || na < 0 || na > CB_MAX_ADDS) throw CsdError("dbfv_combine: too many operands");
     if (dout < 1 || dout > CB_MAX_DIGITS) throw CsdError("dbfv_combine: output digits out of range");
@@ -824,7 +832,10 @@ void csd_dbfv_combine_launch(csd_ctx* ctx, const csd_view* out, int64_t count, i
         a.a[q] = (const int64_t*)adds[q].ptr;
         a.lda[q] = add_digits[q];
     }
+    if (ns2 != 1 && ns2 != n2) throw CsdError("dbfv_combine: factor table rows");
     a.sd = sd_dev;
+    a.ns2 = ns2;
+    a.n3 = n3;
     a.one = one_dev;
     a.cst = cst_dev;
     a.full = full;
@@ -858,16 +869,25 @@ extern "C" int csd_dbfv_combine(csd_ctx* ctx, const csd_view* out, size_t count,
         cst = (const int64_t*)constant->ptr;
     }
     const int64_t* sd = nullptr;
+    int64_t ns2 = 1;
     if (nt) {
-        if (!scalars) throw CsdError("dbfv_combine: scalar digit table missing");
-        check_view(scalars, "dbfv_combine scalars");
-        if (scalars->shape[0] != 1 || scalars->shape[1] != (int64_t)nt || scalars->shape[2] != CB_MAX_DIGITS ||
-            scalars->shape[3] != (int64_t)nmods || scalars->strides[3] != 1 || scalars->strides[2] != (int64_t)nmods ||
-            scalars->strides[1] != (int64_t)(CB_MAX_DIGITS * nmods))
-            throw CsdError("dbfv_combine: scalar digits must be a contiguous [1, terms, 12, r] table");
-        sd = (const int64_t*)scalars->ptr;
+        if (!scalars) throw CsdError("dbfv_combine: factor digit table missing");
+        check_view(scalars, "dbfv_combine factors");
+        check_view(out, "dbfv_combine output");
+        const int64_t r = (int64_t)nmods, n2 = out->shape[2];
+        const csd_view& v = *scalars;
+        const bool scalar_form = v.shape[0] == 1 && v.shape[1] == (int64_t)nt && v.shape[2] == CB_MAX_DIGITS && v.shape[3] == r &&
+                                 v.strides[3] == 1 && v.strides[2] == r && v.strides[1] == CB_MAX_DIGITS * r;
+        const bool lprime_form = v.shape[0] == (int64_t)nt && v.shape[1] == CB_MAX_DIGITS && v.shape[2] == r && v.shape[3] == n2 &&
+                                 v.strides[3] == 1 && v.strides[2] == n2 && v.strides[1] == r * n2 &&
+                                 v.strides[0] == CB_MAX_DIGITS * r * n2;
+        if (!scalar_form && !lprime_form)
+            throw CsdError("dbfv_combine: factor digits must be a contiguous [1, terms, 12, r] (scalars) or [terms, 12, r, n2] "
+                           "(L' elements) table");
+        sd = (const int64_t*)v.ptr;
+        ns2 = scalar_form ? 1 : n2;
     }
-    csd_dbfv_combine_launch(ctx, out, (int64_t)count, (int)dout, terms, term_digits, sd, term_sd, (int)nt, adds, add_digits,
-                            (int)na, (const int64_t*)one->ptr, cst, full, mods, nmods);
+    csd_dbfv_combine_launch(ctx, out, (int64_t)count, (int)dout, terms, term_digits, sd, ns2, term_sd, (int)nt, adds,
+                            add_digits, (int)na, (const int64_t*)one->ptr, cst, full, mods, nmods);
     CSD_CATCH
 }

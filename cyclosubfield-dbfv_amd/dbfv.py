@@ -15,6 +15,7 @@ Above the reduction sits the dBFV multiplication (dbfv.rs:1026-1143): ``dbfv_hom
 ``dbfv_hom_square`` and ``dbfv_hom_mul_batch`` (many pairs, digit-major) reduce both sides, lift
 them to the extended base, form the digit convolution in one wide batch, rescale by t and
 relinearize every output digit in one batch (``bfv.relin_batch``)."""
+import numpy as np
 
 
 def _log2_ceil(x):
@@ -341,16 +342,16 @@ def _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, base_len, keys, di
     base = lifted[:base_len]
     result = ring.allocate_uninit(2 * K * n, base)
     for s_ in (0, 1):
-        dst = _digit_rows(result, s_ * n, K, n, 2 * n)
-        be.hal_assign(dst, _digit_rows(out, s_ * K * n, K, n, n))
+        dst = digit_rows_view(result, s_ * n, K, n, 2 * n)
+        be.hal_assign(dst, digit_rows_view(out, s_ * K * n, K, n, n))
         if output_basis == "pb":
-            be.hal_add_i64(dst, _digit_rows(rescaled, s_ * K * n, K, n, n))
+            be.hal_add_i64(dst, digit_rows_view(rescaled, s_ * K * n, K, n, n))
     if output_basis == "pb":
         bound = 2
     return result, bound
 
 
-def _digit_rows(t, first, K, n, step):
+def digit_rows_view(t, first, K, n, step):
     """rows first + k step + pair (k < K, pair < n) of a [rows, r, n2, n3] tensor as a [K, n r, n2,
     n3] view: the n rows of one k and their limbs are consecutive planes (no copy)"""
     from .hal import DeviceTensor
@@ -460,6 +461,19 @@ def balanced_digits(x, b, max_digits):
         rem = cmod(x, b)
         out.append(rem)
         x = (x - rem) // b
+    return out
+
+
+def balanced_digits_vec(xs, b, max_digits):
+    """balanced_digits_vec (dbfv.rs:97-112): balanced base-b digit vectors of the coefficients, at
+    most max_digits, stopping once every coefficient is zero"""
+    out, cur = [], [int(x) for x in xs]
+    for _ in range(max_digits):
+        if all(x == 0 for x in cur):
+            break
+        rems = [cmod(x, b) for x in cur]
+        cur = [(x - y) // b for x, y in zip(cur, rems)]
+        out.append(rems)
     return out
 
 
@@ -662,12 +676,19 @@ def _mont_repr(x, p):
     return -v if y < 0 else v
 
 
-def dbfv_combine(ring, modulus, like, terms, adds=(), constant=None, basis="pb"):
+COMBINE_MAX_TERMS = 16
+
+
+def dbfv_combine(ring, modulus, like, terms, adds=(), constant=None, basis="pb", lprime=False):
     """One combination of dBFV batches, as one kernel (csd_dbfv_combine): the result of
     dbfv_allocate_accumulator, dbfv_acc_inner_product_scalar(terms), dbfv_acc_add for each of
     `adds`, dbfv_acc_add_constant(constant) and dbfv_finish_accumulator (dbfv.rs:143-406). The
     reduction kind of every row is that of the accumulator's bound bookkeeping (ring.rs:354-366),
-    so the residues are those of the accumulator sequence. `like` gives the base and batch size."""
+    so the residues are those of the accumulator sequence. `like` gives the base and batch size.
+    With `lprime` the factors are elements of L', given by their period-basis coefficient vectors,
+    and the inner product is dbfv_acc_inner_product_element_lprime's (dbfv.rs:378-396): the
+    balanced digit vectors of each become CRT-basis tensors (lprime_element_tensor). At most
+    COMBINE_MAX_TERMS terms with a non-zero factor, 4 adds and 12 output digits."""
     from .hal import check, lib, CsdView
     import ctypes
 
@@ -678,7 +699,7 @@ def dbfv_combine(ring, modulus, like, terms, adds=(), constant=None, basis="pb")
     step = max_p * max_p
     tl = []
     for lhs, s in terms:
-        ds = balanced_digits(s, b, cap)
+        ds = balanced_digits_vec(s, b, cap) if lprime else balanced_digits(s, b, cap)
         ld = lhs.digits()
         if ld == 0 or not ds:
             continue
@@ -697,7 +718,7 @@ def dbfv_combine(ring, modulus, like, terms, adds=(), constant=None, basis="pb")
     dout = max([t[2] for t in tl] + [a.digits() for a in al] + [0])
     if dout == 0:
         return DBFVBatch(like.data[0:0], base, n, basis, 1)
-    if len(tl) > 8 or len(al) > 4 or dout > 12:
+    if len(tl) > COMBINE_MAX_TERMS or len(al) > 4 or dout > 12:
         raise ValueError("combination beyond the fused kernel's operand limits")
     # the rows' bounds as the accumulator would track them
     bounds = [[0, 0] for _ in range(dout)]
@@ -728,7 +749,7 @@ def dbfv_combine(ring, modulus, like, terms, adds=(), constant=None, basis="pb")
     be = ring.backend
     r = len(base)
     sd = None
-    if tl:
+    if tl and not lprime:
         key = ("combine_sd", tuple(base), tuple(tuple(ds) for _, ds, _ in tl))
         sd = ring._consts.get(key)
         if sd is None:
@@ -739,6 +760,19 @@ def dbfv_combine(ring, modulus, like, terms, adds=(), constant=None, basis="pb")
                         vals[(t * 12 + jj) * r + j] = _mont_repr(d, ring.primes[i])
             sd = be.hal_allocate_i64([1, len(tl), 12, r])
             be.hal_import(sd, vals)
+            ring._consts[key] = sd
+    elif tl:
+        # [terms, 12, r, n2]: digit jj of factor t, from the cached L' tensors ([1, r, n2, 1] each)
+        key = ("combine_lprime", tuple(base), tuple(tuple(tuple(v) for v in ds) for _, ds, _ in tl))
+        sd = ring._consts.get(key)
+        if sd is None:
+            n2 = ring.info["deg_lprime"]
+            vals = np.zeros((len(tl), 12, r, n2), dtype=np.int64)
+            for t, (_, ds, _) in enumerate(tl):
+                for jj, dv in enumerate(ds):
+                    vals[t, jj] = np.asarray(be.hal_export(ring.lprime_element_tensor(base, dv))).reshape(r, n2)
+            sd = be.hal_allocate_i64([len(tl), 12, r, n2])
+            be.hal_import(sd, vals.reshape(-1))
             ring._consts[key] = sd
     out = ring.allocate_uninit(2 * dout * n, base)
     tv = (CsdView * max(1, len(tl)))(*[t[0].data.view() for t in tl])

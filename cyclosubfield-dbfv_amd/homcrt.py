@@ -68,17 +68,7 @@ class GaloisKey:
         self.data, self.digits = data, list(digits)
 
 
-def balanced_digits_vec(xs, b, max_digits):
-    """balanced_digits_vec (dbfv.rs:97-112): balanced base-b digit vectors of the coefficients, at
-    most max_digits, stopping once every coefficient is zero"""
-    out, cur = [], [int(x) for x in xs]
-    for _ in range(max_digits):
-        if all(x == 0 for x in cur):
-            break
-        rems = [D.cmod(x, b) for x in cur]
-        cur = [(x - y) // b for x, y in zip(cur, rems)]
-        out.append(rems)
-    return out
+balanced_digits_vec = D.balanced_digits_vec
 
 
 def dbfv_empty(ring, like):
@@ -160,8 +150,12 @@ def bfv_hom_gal_batch(ring, ct, gs, gks):
     dg = len(digits)
     # the c1 rows of all digits (row i n + c), to period basis, then one gadget conversion
     c1 = ring.allocate_uninit(d * n, base)
-    for i in range(d):
-        be.hal_assign(c1[i * n:(i + 1) * n], ct.data[(2 * i + 1) * n:(2 * i + 2) * n])
+    if _dense(ct):
+        # rows (2 i + 1) n + c of ct to rows i n + c, as one strided copy
+        be.hal_assign(D.digit_rows_view(c1, 0, d, n, n), D.digit_rows_view(ct.data, n, d, n, 2 * n))
+    else:
+        for i in range(d):
+            be.hal_assign(c1[i * n:(i + 1) * n], ct.data[(2 * i + 1) * n:(2 * i + 2) * n])
     ring.crt_to_pb_tensor(c1, base)
     parts = ring.allocate_uninit(d * n * dg, base)  # element e = i n + c, gadget digit k at row e dg + k
     ring.bconv_gadget_tensors(parts, c1, base, digits)
@@ -173,9 +167,19 @@ def bfv_hom_gal_batch(ring, ct, gs, gks):
     return [D.DBFVBatch(o, base, n, "crt", 1) for o in outs]
 
 
-def hom_cyclic_transform_dbfv(ring, modulus, automorphism, coeffs, ct, gks):
+def _dense(batch):
+    """whether a batch's rows are one contiguous [rows, r, n2, n3] block (the combination kernel's
+    operand layout)"""
+    t = batch.data
+    plane = t.shape[2] * t.shape[3]
+    return t.strides[3] == 1 and t.strides[2] == t.shape[3] and t.strides[1] == plane and t.strides[0] == t.shape[1] * plane
+
+
+def hom_cyclic_transform_dbfv(ring, modulus, automorphism, coeffs, ct, gks, fused=True):
     """hom_cyclic_transform_dbfv (homcrt.rs:355-429): sum_i coeffs[i] sigma^i(ct) for a CRT-basis
-    batch `ct`, coeffs a list of L' coefficient vectors, automorphism the generator sigma"""
+    batch `ct`, coeffs a list of L' coefficient vectors, automorphism the generator sigma. With
+    `fused` each giant step's accumulator (the baby-step inner product, the carried sum and the
+    reduction) is one kernel (dbfv.dbfv_combine with L' factors); the residues are the same."""
     length = len(coeffs)
     if ct.basis != "crt":
         raise ValueError("the cyclic transform needs CRT-basis ciphertexts")
@@ -192,20 +196,23 @@ def hom_cyclic_transform_dbfv(ring, modulus, automorphism, coeffs, ct, gks):
     for gi in reversed(range(giant)):
         if gi != giant - 1 and out.digits() > 0:
             [out] = bfv_hom_gal_batch(ring, out, [giant_exp], gks)
-        acc = D.dbfv_allocate_accumulator(ring, modulus, ct.base, "crt", ct.count)
         terms = [(babys[j], coeffs[gi * baby + j]) for j in range(min(baby, length - gi * baby))]
+        if fused and len(terms) <= D.COMBINE_MAX_TERMS and all(_dense(x) for x, _ in terms) and _dense(out):
+            out = D.dbfv_combine(ring, modulus, ct, terms, adds=[out], basis="crt", lprime=True)
+            continue
+        acc = D.dbfv_allocate_accumulator(ring, modulus, ct.base, "crt", ct.count)
         dbfv_acc_inner_product_element_lprime(ring, acc, terms)
         D.dbfv_acc_add(ring, acc, out)
         out = D.dbfv_finish_accumulator(ring, acc)
     return out
 
 
-def slots_to_coeffs_dbfv(ring, factors, modulus, ct, gks):
+def slots_to_coeffs_dbfv(ring, factors, modulus, ct, gks, fused=True):
     """slots_to_coeffs_dbfv (homcrt.rs:434-449): the cyclic transform of every factor in turn;
     factors is a list of (generator, coefficient list), the u_p of the transform (or their
     inverses for coeffs_to_slots_dbfv, homcrt.rs:452-467)"""
     for gen, coeffs in factors:
-        ct = hom_cyclic_transform_dbfv(ring, modulus, gen, coeffs, ct, gks)
+        ct = hom_cyclic_transform_dbfv(ring, modulus, gen, coeffs, ct, gks, fused)
     return ct
 
 

@@ -164,9 +164,12 @@ int csd_ring_galois_keyswitch_rows(csd_ring* ring, const size_t* base, size_t r,
  * every step of digitex.rs:62-241): out ([2 dout count, r, n2, n3]) digit k, component s =
  * reduce(sum_(t, i + j = k) terms[t] digit i * scalars[0, t, j, :] + sum_q adds[q] digit k * one
  * + [k = s = 0] constant * one). terms[t] has term_digits[t] digits, its scalar term_sd[t]
- * balanced digits (scalars: [1, nt, 12, r], mont form); bit 2 k + s of `full` selects
- * double_red128 + to_mont64 for that row (the accumulator's bound exceeds 2^63 min_p), else
- * red128. At most 8 terms, 4 adds, 12 output digits. Bit-identical to the accumulator sequence. */
+ * balanced digits (scalars: [1, nt, 12, r], mont form). With an [nt, 12, r, n2] table the factors
+ * are elements of L' instead (dbfv_acc_inner_product_element_lprime, dbfv.rs:378-396): digit j of
+ * factor t is the CRT-basis tensor scalars[t, j, :, :] broadcast along dimension 3. Bit 2 k + s of
+ * `full` selects double_red128 + to_mont64 for that row (the accumulator's bound exceeds 2^63
+ * min_p), else red128. At most 16 terms, 4 adds, 12 output digits. Bit-identical to the
+ * accumulator sequence. */
 int csd_dbfv_combine(csd_ctx* ctx, const csd_view* out, size_t count, size_t dout, const csd_view* terms,
                      const int* term_digits, const csd_view* scalars, const int* term_sd, size_t nt, const csd_view* adds,
                      const int* add_digits, size_t na, const csd_view* one, const csd_view* constant, uint32_t full,

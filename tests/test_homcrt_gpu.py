@@ -112,7 +112,7 @@ def test_row_addressed_keyswitch_matches_the_batched_form(gpu, rings):
 
 
 def _transform_case(gpu, rings, field, log2_q, t_bits, modulus, lengths, n, seed, ndigits, nbase=None, key_extra=0,
-                    zero_key=False):
+                    zero_key=False, fused=True):
     pkg = load()
     g, c = rings(field, log2_q, t_bits)
     init = g.initial_rns_base()
@@ -136,7 +136,7 @@ def _transform_case(gpu, rings, field, log2_q, t_bits, modulus, lengths, n, seed
     for L in lengths:
         coeffs = [[int(v) for v in rng.integers(-(t // 2) + 1, t // 2 + 1, f.deg_lprime)] for _ in range(L)]
         want = [OH.hom_cyclic_transform_dbfv(c, omod, gen, coeffs, ct, ogks) for ct in cts]
-        got = pkg.homcrt.hom_cyclic_transform_dbfv(g, modulus, gen, coeffs, crt_batch_of(gpu, pkg, cts, base), dgks)
+        got = pkg.homcrt.hom_cyclic_transform_dbfv(g, modulus, gen, coeffs, crt_batch_of(gpu, pkg, cts, base), dgks, fused)
         assert got.basis == "crt" and got.digits() == want[0].digits() and got.count == n
         for k, (a, b) in enumerate(zip(unbatch(gpu, got), want)):
             assert same(a, b.data), (L, k)
@@ -146,15 +146,16 @@ def _transform_case(gpu, rings, field, log2_q, t_bits, modulus, lengths, n, seed
                 assert dec == _plain_transform(c, base, omod, gen, coeffs, x), (L, k)
 
 
-@pytest.mark.parametrize("zero_key", [False, True])
-def test_hom_cyclic_transform_field18(gpu, rings, zero_key):
+@pytest.mark.parametrize("zero_key,fused", [(False, True), (True, True), (False, False)])
+def test_hom_cyclic_transform_field18(gpu, rings, zero_key, fused):
     """lengths 1 (one fma), 4 (baby steps only), 6 (3 x 2) and 20 (7 x 3, two carried giant steps)
-    over 3 ciphertexts of 4 digits; keys over a base one prime longer than the ciphertexts'"""
+    over 3 ciphertexts of 4 digits; keys over a base one prime longer than the ciphertexts'; the
+    giant steps as one combination kernel each (fused) and as the accumulator sequence"""
     from _pkg import load as _l
 
     mod = _l().dbfv.DBFVModulus.new(4, 4)
     _transform_case(gpu, rings, "field_18", (300, 320), 16, mod, [1, 4, 6, 20], 3, 31, 3, nbase=5, key_extra=1,
-                    zero_key=zero_key)
+                    zero_key=zero_key, fused=fused)
 
 
 def test_hom_cyclic_transform_field36864_factors(gpu, rings):
