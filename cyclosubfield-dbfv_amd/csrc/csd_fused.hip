@@ -340,8 +340,17 @@ struct GbArgs {
     const int64_t* pre;
     const int64_t* post_s;
     const int64_t* post_v;
+    // mod-switch form (null otherwise): each target row j of dst becomes mont(keep[j] - v, post_v[j]),
+    // keep a [B, rd, n2, n3] view (the kept limbs of the element)
+    const int64_t* keep;
+    int64_t ks[4];
     int32_t lo[CSD_MAX_MODULI], hi[CSD_MAX_MODULI];
 };
+
+// the output forms of k_bconv_fused
+#define GB_PLAIN 0
+#define GB_RESCALE 1
+#define GB_MODSWITCH 2
 
 __device__ __forceinline__ i128 gb_mac(i128 acc, int64_t a, int64_t b) {
     uint64_t lo;
@@ -380,8 +389,10 @@ __device__ __forceinline__ i128 gb_split_sum(const GbSplit& s) {
 // parked in the thread's LDS column for the passes over GB_GROUP target limbs. SPLIT accumulates
 // in the four int64 partial sums (worth it for long digits; short ones pay more for the final
 // i128 assembly than they save). WV = 1 stages the digit's weights in LDS once per workgroup
-// (broadcast reads); WV = 0 reads them with scalar loads, for weight tables beyond the LDS.
-template <bool SPLIT, int WV, bool RS>
+// (broadcast reads); WV = 0 reads them with scalar loads, for weight tables beyond the LDS. FM is
+// the output form: GB_PLAIN the conversion, GB_RESCALE the rescale step, GB_MODSWITCH the
+// subtraction from the kept limbs and the multiply by the inverse of mod_switch_down.
+template <bool SPLIT, int WV, int FM>
 __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
     extern __shared__ int64_t gb_x[];
     int64_t* xs = gb_x + threadIdx.x;
@@ -417,7 +428,7 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
                 if (u0 + t < hi) {
                     const int u = u0 + t;
                     const CsdMod mu = a.smods ? a.smods[u] : mt.m[u];
-                    const int64_t xin = RS ? csd_mont_mul(mu, raw[t], a.pre[u]) : raw[t];
+                    const int64_t xin = FM == GB_RESCALE ? csd_mont_mul(mu, raw[t], a.pre[u]) : raw[t];
                     const int64_t xv = csd_mont_mul(mu, xin, a.dopi[u]);
                     xs[(u - lo) * 256] = xv;
                     corr = gb_mac(corr, corr_row[u], xv);
@@ -465,9 +476,13 @@ __global__ __launch_bounds__(256) void k_bconv_fused(GbArgs a, ModTable mt) {
                     v = (int64_t)((uint64_t)v + (uint64_t)c);
                     v = csd_mont_mul_fc(f, v, csd_fast_const(mj.r2, mj.ninv));  // to_mont64
                     int64_t* oj = o + i * a.sd[0] + j * a.sd[1];
-                    if (RS) {
+                    if (FM == GB_RESCALE) {
                         const int64_t e = csd_mont_mul(mj, *oj, a.post_s[j]);  // any i64
                         v = csd_mont_mul_fc(f, (int64_t)((uint64_t)e - (uint64_t)v), csd_fast_const(a.post_v[j], mj.ninv));
+                    } else if (FM == GB_MODSWITCH) {
+                        // sub_i64 and mul_mont64 of the HAL sequence; the kept limb is any i64
+                        const int64_t kv = a.keep[b * a.ks[0] + k * a.ks[2] + l * a.ks[3] + j * a.ks[1]];
+                        v = csd_mont_mul(mj, (int64_t)((uint64_t)kv - (uint64_t)v), a.post_v[j]);
                     }
                     *oj = v;
                 }
@@ -480,7 +495,7 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
                             const int64_t* mat, const int64_t* matT, int64_t rdp, const int64_t* dmp,
                             const std::vector<size_t>& bounds, size_t shift, const csd_modulus* dmods,
                             const CsdMod* smods_dev, const char* timer, const int64_t* pre, const int64_t* post_s,
-                            const int64_t* post_v) {
+                            const int64_t* post_v, const csd_view* keep) {
     GbArgs a;
     memset(&a, 0, sizeof(a));
     a.B = src.shape[0];
@@ -505,9 +520,17 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
     a.pre = pre;
     a.post_s = post_s;
     a.post_v = post_v;
-    // the rescale form takes pre, post_s and post_v together (the kernel reads all three)
-    if ((pre == nullptr) != (post_s == nullptr) || (post_s == nullptr) != (post_v == nullptr) || (post_s && d != 1))
+    // the rescale form takes pre, post_s and post_v together (the kernel reads all three); the
+    // mod-switch form (keep) post_v alone
+    if (!keep && ((pre == nullptr) != (post_s == nullptr) || (post_s == nullptr) != (post_v == nullptr) || (post_s && d != 1)))
         throw CsdError("fused bconv: the rescale form needs pre, post_s and post_v together, and one digit");
+    if (keep) {
+        if (pre || post_s || !post_v || d != 1) throw CsdError("fused bconv: the mod-switch form takes post_v only, and one digit");
+        for (int q = 0; q < 4; q++)
+            if (keep->shape[q] != dst.shape[q]) throw CsdError("fused bconv: kept limbs must have the destination's shape");
+        a.keep = (const int64_t*)keep->ptr;
+        for (int q = 0; q < 4; q++) a.ks[q] = keep->strides[q];
+    }
     if (!smods_dev && a.rd != a.r) throw CsdError("fused bconv: source moduli missing");
     if (rdp < a.rd || rdp % GB_GROUP) throw CsdError("fused bconv: transposed weights not padded");
     for (int64_t i = 0; i < d; i++) {
@@ -518,7 +541,7 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
     const int64_t blocks = a.B * ((a.P + 255) / 256);
     if (blocks > (int64_t)INT32_MAX) throw CsdError("fused bconv: grid too large");
     ModTable t = make_mod_table(dmods, (size_t)a.rd, a.rd);
-    TimedScope ts(ctx, timer, 8.0 * (double)a.B * a.P * ((double)a.r + (double)d * a.rd * (post_s ? 2 : 1)));
+    TimedScope ts(ctx, timer, 8.0 * (double)a.B * a.P * ((double)a.r + (double)d * a.rd * (post_s || keep ? 2 : 1)));
     int64_t max_len = 0;
     for (int64_t i = 0; i < d; i++) max_len = std::max<int64_t>(max_len, a.hi[i] - a.lo[i]);
     const size_t lds0 = (size_t)max_len * 256 * sizeof(int64_t);
@@ -530,14 +553,18 @@ void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& s
         hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), wv ? lds1 : lds0, ctx->stream, a, t);
     };
     if (post_s) {  // the rescale form: one digit of many limbs
-        if (wv) launch(k_bconv_fused<true, 1, true>);
-        else launch(k_bconv_fused<true, 0, true>);
+        if (wv) launch(k_bconv_fused<true, 1, GB_RESCALE>);
+        else launch(k_bconv_fused<true, 0, GB_RESCALE>);
+    } else if (keep) {  // the mod-switch form: few dropped limbs
+        if (!wv) launch(split ? k_bconv_fused<true, 0, GB_MODSWITCH> : k_bconv_fused<false, 0, GB_MODSWITCH>);
+        else if (split) launch(k_bconv_fused<true, 1, GB_MODSWITCH>);
+        else launch(k_bconv_fused<false, 1, GB_MODSWITCH>);
     } else if (wv) {
-        if (split) launch(k_bconv_fused<true, 1, false>);
-        else launch(k_bconv_fused<false, 1, false>);
+        if (split) launch(k_bconv_fused<true, 1, GB_PLAIN>);
+        else launch(k_bconv_fused<false, 1, GB_PLAIN>);
     } else {
-        if (split) launch(k_bconv_fused<true, 0, false>);
-        else launch(k_bconv_fused<false, 0, false>);
+        if (split) launch(k_bconv_fused<true, 0, GB_PLAIN>);
+        else launch(k_bconv_fused<false, 0, GB_PLAIN>);
     }
     HIPCHECK(hipGetLastError());
 }

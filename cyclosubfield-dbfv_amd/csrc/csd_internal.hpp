@@ -110,12 +110,14 @@ void csd_galois_keyswitch_launch(csd_ctx* ctx, int64_t batch, int64_t d, int64_t
 // or is null when they are the target moduli. matT is csd_bconv_transpose of mat. With pre, post_s
 // and post_v (device, one int64 per source / target limb; one digit only) it runs the rescale form:
 // source limbs are Montgomery-multiplied by pre first, and each dst row becomes
-// mont(mont(dst, post_s) - converted, post_v), dst read in place.
+// mont(mont(dst, post_s) - converted, post_v), dst read in place. With keep (a view of dst's shape)
+// and post_v alone it runs the mod-switch form: each dst row becomes mont(keep - converted, post_v).
 void csd_bconv_fused_launch(csd_ctx* ctx, const csd_view& dst, const csd_view& src, int64_t d, const int64_t* dopi,
                             const int64_t* mat, const int64_t* matT, int64_t rdp, const int64_t* dmp,
                             const std::vector<size_t>& bounds, size_t shift, const csd_modulus* dmods,
                             const CsdMod* smods_dev, const char* timer, const int64_t* pre = nullptr,
-                            const int64_t* post_s = nullptr, const int64_t* post_v = nullptr);
+                            const int64_t* post_s = nullptr, const int64_t* post_v = nullptr,
+                            const csd_view* keep = nullptr);
 std::vector<int64_t> csd_bconv_transpose(const std::vector<int64_t>& mat, int64_t d, int64_t rd, int64_t r, int64_t* rdp);
 // The BEHZ conversion (one digit over the whole source base) with its matrix product on the int8
 // matrix cores (csd_fused.hip); residues identical to csd_bconv_fused_launch's. Tables from

@@ -1064,9 +1064,28 @@ struct csd_ring {
             throw CsdError("mod_switch_down: destination shape");
         std::vector<size_t> kept(base.begin(), base.begin() + nl), dropped(base.begin() + nl, base.end());
         csd_view src_kept = sub(src, 1, 0, nl, 8);
-        ok(csd_hal_assign(ctx, &dst, &src_kept));
-        if (dropped.empty()) return;
+        if (dropped.empty()) {
+            ok(csd_hal_assign(ctx, &dst, &src_kept));
+            return;
+        }
         int64_t rdr = (int64_t)dropped.size();
+        csd_view src_dropped0 = sub(src, 1, nl, r, 8);
+        {
+            // the fused forms: the conversion of the dropped limbs, the subtraction from the kept
+            // ones and the multiply by the inverse as one kernel, the same integer steps as below
+            BConvCache& c = bconv_cache(dropped, kept);
+            if (bconv_form >= 1 && c.acc >= (size_t)rdr && !views_overlap(dst, src, 8)) {
+                DevBuf& inv = inverse_product_tensor(kept, dropped);
+                auto km = mods_for(kept);
+                const std::vector<size_t> whole = {0, (size_t)rdr};
+                csd_bconv_fused_launch(ctx, dst, src_dropped0, 1, (const int64_t*)c.q_over_p_inv->p, (const int64_t*)c.matrix->p,
+                                       (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.q_mod_p->p, whole, c.log2_qtilde,
+                                       km.data(), (const CsdMod*)c.src_mods->p, "mod_switch_down", nullptr, nullptr,
+                                       (const int64_t*)inv.p, &src_kept);
+                return;
+            }
+        }
+        ok(csd_hal_assign(ctx, &dst, &src_kept));
         DevBuf tmp(ctx, (size_t)cnt * nl * n2 * n3 * 8);
         csd_view vt = make_view(tmp.p, cnt, nl, n2, n3);
         csd_view src_dropped = sub(src, 1, nl, r, 8);

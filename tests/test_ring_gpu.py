@@ -305,13 +305,17 @@ def test_gadget_bconv_errors(gpu, rings):
         g.set_gadget_form(2)
 
 
-@pytest.mark.parametrize("r,new_len", [(17, 16), (17, 3), (9, 8), (3, 1)])
-def test_mod_switch_parity(gpu, cpu, rings, r, new_len):
-    """Dynamic scaling (bfv.rs:628-662) at uneven levels, bit-exact."""
+@pytest.mark.parametrize("r,new_len", [(17, 16), (17, 3), (9, 8), (3, 1), (17, 17)])
+def test_mod_switch_parity(gpu, cpu, rings, bconv_form, r, new_len):
+    """Dynamic scaling (bfv.rs:628-662) at uneven levels, bit-exact, with unreduced (-2p, 2p)
+    inputs in part: one kernel in the fused conversion forms, the HAL sequence in form 0."""
     g, c = rings("field_36864", (960, 980), 76)
+    bconv_form(g)
     base = list(range(r))
     rng = np.random.default_rng(100 + r)
-    src = uniform_residues(rng, [g.primes[i] for i in base], (2, r, 12, 3072))
+    primes = [g.primes[i] for i in base]
+    src = uniform_residues(rng, primes, (2, r, 12, 3072))
+    src[1, :, 3, :256] -= np.array(primes, dtype=np.int64).reshape(-1, 1) * rng.integers(0, 2, (r, 256))
     dc = R.mod_switch_down(c, src.copy(), base, new_len)
     dg = gpu.hal_allocate_i64((2, new_len, 12, 3072))
     sg = to_dev(gpu, src)
