@@ -551,6 +551,68 @@ extern "C" int csd_export_i128(csd_ctx* ctx, const csd_view* src, int64_t* dst, 
 }
 
 // ---------------------------------------------------------------------------------------------
+// the debug range check of kernel outputs: how many entries of an int64 view have |x| > k N for the
+// modulus N of their limb (dimension 1). One thread per entry, the count gathered with one atomic
+// add per workgroup into a zeroed device word.
+
+__global__ __launch_bounds__(256) void k_range_check(const int64_t* __restrict__ p, int64_t d1, int64_t d2, int64_t d3,
+                                                     int64_t s0, int64_t s1, int64_t s2, int64_t s3, int64_t total,
+                                                     const int64_t* __restrict__ lim, unsigned long long* count) {
+    __shared__ unsigned int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < total) {
+        const int64_t l = e % d3, t = e / d3, k = t % d2, t2 = t / d2, j = t2 % d1, i = t2 / d1;
+        const int64_t x = p[i * s0 + j * s1 + k * s2 + l * s3];
+        const uint64_t ax = x < 0 ? (uint64_t)0 - (uint64_t)x : (uint64_t)x;
+        if (ax > (uint64_t)lim[j]) atomicAdd(&bad, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && bad) atomicAdd(count, (unsigned long long)bad);
+}
+
+extern "C" int csd_hal_range_check(csd_ctx* ctx, const csd_view* v, const csd_modulus* mods, size_t nmods, int64_t multiple,
+                                   uint64_t* out_of_range) {
+    CSD_TRY check_view(v, "range_check");
+    if (!out_of_range) throw CsdError("range_check: null result");
+    if ((int64_t)nmods != v->shape[1]) throw CsdError("range_check: one modulus per limb");
+    if (multiple < 1) throw CsdError("range_check: the bound multiple must be positive");
+    std::vector<int64_t> lim(nmods);
+    for (size_t j = 0; j < nmods; j++) {
+        if (mods[j].N <= 1) throw CsdError("range_check: invalid modulus");
+        const u128 b = (u128)(uint64_t)mods[j].N * (u128)(uint64_t)multiple;
+        lim[j] = b >= ((u128)1 << 63) ? INT64_MAX : (int64_t)b;  // k N beyond 2^63: every i64 passes
+    }
+    *out_of_range = 0;
+    const int64_t total = numel(v);
+    if (total == 0) return 0;
+    const size_t bytes = nmods * 8 + 8;
+    char* buf = (char*)ctx->alloc_bytes(bytes);
+    try {
+        int64_t* dlim = (int64_t*)(buf + 8);
+        unsigned long long* dcount = (unsigned long long*)buf;
+        csd_zero_fill(ctx, dcount, 8);
+        HIPCHECK(hipMemcpyAsync(dlim, lim.data(), nmods * 8, hipMemcpyHostToDevice, ctx->stream));
+        const int64_t blocks = (total + 255) / 256;
+        if (blocks > (int64_t)INT32_MAX) throw CsdError("range_check: tensor too large for one launch");
+        hipLaunchKernelGGL(k_range_check, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (const int64_t*)v->ptr,
+                           v->shape[1], v->shape[2], v->shape[3], v->strides[0], v->strides[1], v->strides[2], v->strides[3],
+                           total, dlim, dcount);
+        HIPCHECK(hipGetLastError());
+        unsigned long long host = 0;
+        HIPCHECK(hipMemcpyAsync(&host, dcount, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        *out_of_range = host;
+    } catch (...) {
+        ctx->free_bytes(buf);
+        throw;
+    }
+    ctx->free_bytes(buf);
+    CSD_CATCH
+}
+
+// ---------------------------------------------------------------------------------------------
 // element-wise entry points (hal/cpu.rs:591-704, 823-916)
 
 #define BIN_ENTRY(name, OP)                                                                  \

@@ -18,6 +18,14 @@ relinearize every output digit in one batch (``bfv.relin_batch``)."""
 import numpy as np
 
 
+def _checked(ring, t, base, bound, what):
+    """the debug range check (RingContext.set_range_check) of a produced tensor against its
+    tracked bound; returns t"""
+    if ring.range_checks and t.shape[0]:
+        ring.check_range(t, base, bound, what)
+    return t
+
+
 def _log2_ceil(x):
     """feanor-math abs_log2_ceil for x >= 1"""
     return (abs(x) - 1).bit_length()
@@ -278,7 +286,7 @@ def _reduce_into_new(ring, modulus, src, d, per_digit, ct_bound, base, fused):
         result = ring.allocate(rows, base)
         ring.backend.hal_assign(result[0:d * per_digit], src)
         _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base, False)
-    return result, 1
+    return _checked(ring, result, base, 1, "dbfv_reduce"), 1
 
 
 def dbfv_reduce(ring, modulus, ct, ct_bound, base, fused=True):
@@ -348,7 +356,7 @@ def _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, base_len, keys, di
             be.hal_add_i64(dst, digit_rows_view(rescaled, s_ * K * n, K, n, n))
     if output_basis == "pb":
         bound = 2
-    return result, bound
+    return _checked(ring, result, base, bound, "dbfv_hom_mul"), bound
 
 
 def digit_rows_view(t, first, K, n, step):
@@ -620,7 +628,8 @@ def dbfv_finish_accumulator(ring, acc):
     out = ring.allocate_uninit(rows, acc.rns_base())
     if rows:
         ring.reduce_wide_rows_into(out, acc.acc, rows)
-    return DBFVBatch(out, acc.rns_base(), acc.count, acc.basis, 1)
+    return DBFVBatch(_checked(ring, out, acc.rns_base(), 1, "dbfv_finish_accumulator"), acc.rns_base(), acc.count,
+                     acc.basis, 1)
 
 
 def dbfv_exact_div_pow2(ring, initial_modulus, val, log2_div):
@@ -650,7 +659,7 @@ def dbfv_exact_div_pow2(ring, initial_modulus, val, log2_div):
     acc = ring.allocate_wide(2 * n, base, zero=False)
     acc.inner_product(range(0, 2 * n), terms)
     ring.reduce_wide_into(result[0:2 * n], acc)
-    return DBFVBatch(result, base, n, val.basis, 1)
+    return DBFVBatch(_checked(ring, result, base, 1, "dbfv_exact_div_pow2"), base, n, val.basis, 1)
 
 
 def dbfv_mod_switch_down(ring, ct, new_len):
@@ -665,7 +674,7 @@ def dbfv_mod_switch_down(ring, ct, new_len):
     _assert_mul_mont_safe(ring, ct.bound + 1, ct.base[:new_len])
     out = ring.allocate_uninit(ct.data.shape[0], ct.base[:new_len])
     ring.mod_switch_down(out, ct.data, ct.base, new_len)
-    return DBFVBatch(out, ct.base[:new_len], ct.count, "pb", 1)
+    return DBFVBatch(_checked(ring, out, ct.base[:new_len], 1, "mod_switch_down"), ct.base[:new_len], ct.count, "pb", 1)
 
 
 def _mont_repr(x, p):
@@ -788,4 +797,4 @@ def dbfv_combine(ring, modulus, like, terms, adds=(), constant=None, basis="pb",
                                  av, adig, ctypes.c_size_t(len(al)), ctypes.byref(one.view()),
                                  ctypes.byref(cst.view()) if cst is not None else None, ctypes.c_uint32(full), marr,
                                  ctypes.c_size_t(len(mods))))
-    return DBFVBatch(out, base, n, basis, 1)
+    return DBFVBatch(_checked(ring, out, base, 1, "dbfv_combine"), base, n, basis, 1)

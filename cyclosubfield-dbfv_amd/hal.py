@@ -521,6 +521,15 @@ class HipBackend:
                                           ctypes.c_size_t(limbs), f0l, f0h, f1l, f1h, ctypes.c_size_t(shift),
                                           ctypes.c_size_t(log2_r), mask(full_split), mask(full_carry)))
 
+    def range_check(self, t, mods, multiple):
+        """csd_hal_range_check: the number of entries of the int64 tensor `t` with |x| > multiple N,
+        N the modulus of the entry's limb (a debug check of kernel outputs; synchronizes)"""
+        arr, n = _mods(mods)
+        out = ctypes.c_uint64()
+        check(lib().csd_hal_range_check(self.ctx, ctypes.byref(t.view()), arr, n, ctypes.c_int64(int(multiple)),
+                                        ctypes.byref(out)))
+        return out.value
+
     def host_alloc(self, shape):
         """pinned host memory for a host-resident int64 tensor (PinnedHostTensor)"""
         return PinnedHostTensor(self, shape)

@@ -78,6 +78,20 @@ class RingContext:
         self.primes = [out[i] for i in range(n.value)]
         self.initial_rns_base_len = il.value
         self._consts = {}
+        self.range_checks = False
+
+    def set_range_check(self, on):
+        """Debug mode: every dBFV batch the dbfv layer produces (reductions, products,
+        accumulators, combinations, mod switches) is checked against its tracked bound, |x| <=
+        bound p for the prime p of each limb (the bookkeeping of RingEls.bound, ring.rs:33-47),
+        and a violation raises. Each check synchronizes the device; off by default."""
+        self.range_checks = bool(on)
+
+    def check_range(self, t, base, bound, what):
+        """raises when an entry of the [count, r, ., .] tensor `t` over `base` is beyond bound p"""
+        bad = self.backend.range_check(t, self.mods_for(base), bound)
+        if bad:
+            raise ArithmeticError(f"{what}: {bad} entries beyond {bound} p (range check)")
 
     def close(self):
         self._consts = {}

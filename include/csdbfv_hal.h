@@ -87,6 +87,13 @@ int csd_hal_export(csd_ctx* ctx, const csd_view* src, int64_t* dst, size_t len);
  * hal/cpu.rs:1264-1454): two int64 words (low, high) per entry. */
 int csd_import_i128(csd_ctx* ctx, const csd_view* dst, const int64_t* src_words, size_t len);
 int csd_export_i128(csd_ctx* ctx, const csd_view* src, int64_t* dst_words, size_t len);
+/* Debug range check of kernel outputs: *out_of_range receives the number of entries of the int64
+ * view with |x| > multiple * N, N the modulus of the entry's limb (dimension 1; one modulus per
+ * limb). Synchronizes the context's stream. The host-side counterpart of the reference's bound
+ * bookkeeping (RingEls.bound, ring.rs:33-47; assert_mul_mont_safe, ring.rs:955-960), which it
+ * checks against the data: no counterpart in the reference's HAL. */
+int csd_hal_range_check(csd_ctx* ctx, const csd_view* v, const csd_modulus* mods, size_t nmods, int64_t multiple,
+                        uint64_t* out_of_range);
 
 /* ---- copies and wrapping integer arithmetic ------------------------------------------------ */
 int csd_hal_assign(csd_ctx* ctx, const csd_view* dst, const csd_view* src);        /* mod.rs:129 */
