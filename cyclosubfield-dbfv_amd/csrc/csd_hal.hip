@@ -336,6 +336,7 @@ enum ElemOp {
     OP_RED,
     OP_I128_TO_MONT,
     OP_I64_TO_MONT,
+    OP_SUM_I64,
 };
 
 struct ElemArgs {
@@ -374,6 +375,7 @@ __device__ __forceinline__ void elem_apply(const ElemArgs& a, const CsdMod& m, i
     if constexpr (OP == OP_RED) d64[od] = csd_red128(m, a128[oa]);
     if constexpr (OP == OP_I128_TO_MONT) d64[od] = csd_to_mont128(m, a128[oa]);
     if constexpr (OP == OP_I64_TO_MONT) d64[od] = csd_to_mont64(m, d64[od]);
+    if constexpr (OP == OP_SUM_I64) d64[od] = (int64_t)((uint64_t)a64[oa] + (uint64_t)b64[ob]);
 }
 
 template <int OP>
@@ -634,6 +636,13 @@ extern "C" int csd_hal_fma_i64(csd_ctx* ctx, const csd_view* dst, const csd_view
     check_view(lhs, "fma_i64");
     check_view(rhs, "fma_i64");
     launch_elem<OP_FMA_I64>(ctx, dst, lhs, rhs, nullptr);
+    CSD_CATCH
+}
+extern "C" int csd_hal_sum_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs) {
+    CSD_TRY check_view(dst, "sum_i64");
+    check_view(lhs, "sum_i64");
+    check_view(rhs, "sum_i64");
+    launch_elem<OP_SUM_I64>(ctx, dst, lhs, rhs, nullptr);
     CSD_CATCH
 }
 extern "C" int csd_hal_mul_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs) {

@@ -825,7 +825,7 @@ struct csd_ring {
     static bool mfma_pays(int64_t r, bool rescale) { return rescale ? r >= 17 : r >= 9; }
 
     // BConvCache::bconv (ring.rs:1663-1701). The fused form (default, when src and dst do not
-    // overlap) is one kernel and leaves src as it is; the HAL sequence destroys src, like the
+    // overlap or are disjoint limb ranges of one tensor) is one kernel and leaves src as it is; the HAL sequence destroys src, like the
     // reference, unless preserve_src is set, when it runs on a copy. The form is decided once per
     // call, so what happens to src never depends on a later change of the setting.
     void bconv(const csd_view& dst, const std::vector<size_t>& dbase, const csd_view& src_in, const std::vector<size_t>& sbase,
@@ -838,13 +838,15 @@ struct csd_ring {
             throw CsdError("bconv: destination shape");
         auto smods = mods_for(sbase), dmods = mods_for(dbase);
         const int form = bconv_form;
-        if ((form == 3 || (form == 2 && mfma_pays(r, false))) && c.mfma_atab && c.acc >= (size_t)r &&
-            !views_overlap(dst, src_in, 8)) {
+        // the fused kernels read a position's source limbs and write its target limbs in one
+        // thread (or wave), so src and dst may be disjoint limb ranges of one tensor
+        const bool disjoint = limb_disjoint(dst, src_in);
+        if ((form == 3 || (form == 2 && mfma_pays(r, false))) && c.mfma_atab && c.acc >= (size_t)r && disjoint) {
             csd_bconv_mfma_launch(ctx, dst, src_in, (const int64_t*)c.q_over_p_inv->p, (const int8_t*)c.mfma_atab->p,
                                   (const int64_t*)c.mfma_rows->p, c.log2_qtilde, (const CsdMod*)c.src_mods->p, "bconv_mfma");
             return;
         }
-        if (form >= 1 && c.acc >= (size_t)r && !views_overlap(dst, src_in, 8)) {
+        if (form >= 1 && c.acc >= (size_t)r && disjoint) {
             const std::vector<size_t> whole = {0, (size_t)r};
             csd_bconv_fused_launch(ctx, dst, src_in, 1, (const int64_t*)c.q_over_p_inv->p, (const int64_t*)c.matrix->p,
                                    (const int64_t*)c.matrix_t->p, c.rdp, (const int64_t*)c.q_mod_p->p, whole,

@@ -274,13 +274,18 @@ def _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base, fused, sr
         carry = split_overflow(ring, data, digit, row_bound, base, fused=False)
 
 
-def _reduce_into_new(ring, modulus, src, d, per_digit, ct_bound, base, fused):
+def _reduce_into_new(ring, modulus, src, d, per_digit, ct_bound, base, fused, out=None):
     """the result batch of dbfv_reduce (dbfv.rs:985-988, 1023-1024). The call sequence allocates
     it zeroed and copies the input digits in; the fused kernels read the input digits in place
-    and write every row, so the result needs neither the fill nor the copy."""
+    and write every row, so the result needs neither the fill nor the copy. With `out` (fused
+    only: a [rows, r, ., .] view, any row and limb strides) the fused kernels write the result
+    there."""
     rows = modulus.digits() * per_digit
+    if out is not None and (not fused or out.shape[0] != rows or out.shape[1] != len(base)):
+        raise ValueError("dbfv_reduce: an output view needs the fused form and the result's shape")
     if fused:
-        result = ring.backend.allocate_i64_uninit([rows, len(base), ring.info["deg_lprime"], ring.info["deg_k"]])
+        result = out if out is not None else ring.backend.allocate_i64_uninit(
+            [rows, len(base), ring.info["deg_lprime"], ring.info["deg_k"]])
         _reduce_chain(ring, modulus, result, d, per_digit, ct_bound, base, True, src=src)
     else:
         result = ring.allocate(rows, base)
@@ -319,6 +324,35 @@ def dbfv_reduce_batch(ring, modulus, cts, count, ct_bound, base, fused=True):
     return _reduce_into_new(ring, modulus, cts, d, 2 * count, ct_bound, base, fused)
 
 
+def _reduced_lifted(ring, modulus, ct, ct_bound, count, base, fused):
+    """dbfv_reduce_batch, then lift_to_extended (bfv.rs:564-595) of the reduced digits: returns
+    (reduced, bound, lifted tensor, lifted base), or (reduced, bound, None, None) when the reduced
+    batch has no digits. With the fused chain the reduced digits are written straight into the
+    base limbs of the lifted tensor, so the lift's copy of them is left out (reduced is then
+    None); the residues are the same."""
+    from .bfv import lift_to_extended
+
+    if count < 1 or ct.shape[0] % (2 * count):
+        raise ValueError("dBFV ciphertexts need two rows per digit and pair")
+    d = ct.shape[0] // (2 * count)
+    if d > modulus.digits():
+        raise ValueError("ciphertext has more digits than the modulus")
+    if not fused or modulus.digits() == 1 or d == 0:
+        red, bound = _reduced_operand(ring, modulus, ct, ct_bound, count, base, fused)
+        if red.shape[0] == 0:
+            return red, bound, None, None
+        lifted_t, lifted = lift_to_extended(ring, red, base)
+        return red, bound, lifted_t, lifted
+    ext = ring.extension_rns_base_for(base)
+    lifted = list(base) + ext
+    bl = len(base)
+    out = ring.allocate_uninit(modulus.digits() * 2 * count, lifted)
+    _reduce_into_new(ring, modulus, ct, d, 2 * count, ct_bound, base, True, out=out[:, :bl])  # bound 1
+    ring.bconv_tensors(out[:, bl:], ext, out[:, :bl], base, preserve_src=True)
+    ring.pb_to_crt_tensor(out, lifted)
+    return None, 1, out, lifted
+
+
 def _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, base_len, keys, digits, output_basis):
     """dbfv_mul_convolve (dbfv.rs:1078-1143) of `count` pairs of lifted CRT-basis dBFV
     ciphertexts (digit-major: row (2 i + s) count + c is row s of digit i of pair c): the digit
@@ -351,9 +385,10 @@ def _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, base_len, keys, di
     result = ring.allocate_uninit(2 * K * n, base)
     for s_ in (0, 1):
         dst = digit_rows_view(result, s_ * n, K, n, 2 * n)
-        be.hal_assign(dst, digit_rows_view(out, s_ * K * n, K, n, n))
         if output_basis == "pb":
-            be.hal_add_i64(dst, digit_rows_view(rescaled, s_ * K * n, K, n, n))
+            be.sum_i64(dst, digit_rows_view(out, s_ * K * n, K, n, n), digit_rows_view(rescaled, s_ * K * n, K, n, n))
+        else:
+            be.hal_assign(dst, digit_rows_view(out, s_ * K * n, K, n, n))
     if output_basis == "pb":
         bound = 2
     return _checked(ring, result, base, bound, "dbfv_hom_mul"), bound
@@ -392,18 +427,14 @@ def dbfv_hom_mul_batch(ring, modulus, lhs, lhs_bound, rhs, rhs_bound, count, bas
     output_basis "pb" (bound 2, the reference's choice before a reduction) or "crt" (bound 1).
     Every HAL call covers all pairs, and each pair gets the residues of its own dbfv_hom_mul. A
     side without digits is returned as it is."""
-    from .bfv import lift_to_extended
-
     if output_basis not in ("pb", "crt"):
         raise ValueError("output_basis must be 'crt' or 'pb'")
-    lhs_red, lb = _reduced_operand(ring, modulus, lhs, lhs_bound, count, base, fused)
-    rhs_red, rb = _reduced_operand(ring, modulus, rhs, rhs_bound, count, base, fused)
-    if lhs_red.shape[0] == 0:
+    lhs_red, lb, lhs_l, lifted = _reduced_lifted(ring, modulus, lhs, lhs_bound, count, base, fused)
+    if lhs_l is None:
         return lhs_red, lb
-    if rhs_red.shape[0] == 0:
+    rhs_red, rb, rhs_l, _ = _reduced_lifted(ring, modulus, rhs, rhs_bound, count, base, fused)
+    if rhs_l is None:
         return rhs_red, rb
-    lhs_l, lifted = lift_to_extended(ring, lhs_red, base)
-    rhs_l, _ = lift_to_extended(ring, rhs_red, base)
     return _mul_convolve(ring, modulus, lhs_l, rhs_l, count, lifted, len(base), keys, digits, output_basis)
 
 
@@ -416,14 +447,11 @@ def dbfv_hom_mul(ring, modulus, lhs, lhs_bound, rhs, rhs_bound, base, keys, digi
 def dbfv_hom_square(ring, modulus, ct, ct_bound, base, keys, digits, output_basis="pb", fused=True):
     """dbfv_hom_square (dbfv.rs:1058-1076): dbfv_hom_mul of a ciphertext with itself, reduced
     and lifted once."""
-    from .bfv import lift_to_extended
-
     if output_basis not in ("pb", "crt"):
         raise ValueError("output_basis must be 'crt' or 'pb'")
-    red, bound = _reduced_operand(ring, modulus, ct, ct_bound, 1, base, fused)
-    if red.shape[0] == 0:
+    red, bound, lifted_ct, lifted = _reduced_lifted(ring, modulus, ct, ct_bound, 1, base, fused)
+    if lifted_ct is None:
         return red, bound
-    lifted_ct, lifted = lift_to_extended(ring, red, base)
     return _mul_convolve(ring, modulus, lifted_ct, lifted_ct, 1, lifted, len(base), keys, digits, output_basis)
 
 
@@ -432,14 +460,11 @@ def dbfv_hom_square_batch(ring, modulus, ct, ct_bound, count, base, keys, digits
     (row (2 i + s) count + c is row s of digit i of ciphertext c): one reduction and one lift for
     all of them, then the digit convolution of each with itself and one batched
     relinearization. Each ciphertext gets the residues of its own dbfv_hom_square."""
-    from .bfv import lift_to_extended
-
     if output_basis not in ("pb", "crt"):
         raise ValueError("output_basis must be 'crt' or 'pb'")
-    red, bound = _reduced_operand(ring, modulus, ct, ct_bound, count, base, fused)
-    if red.shape[0] == 0:
+    red, bound, lifted_ct, lifted = _reduced_lifted(ring, modulus, ct, ct_bound, count, base, fused)
+    if lifted_ct is None:
         return red, bound
-    lifted_ct, lifted = lift_to_extended(ring, red, base)
     return _mul_convolve(ring, modulus, lifted_ct, lifted_ct, count, lifted, len(base), keys, digits, output_basis)
 
 

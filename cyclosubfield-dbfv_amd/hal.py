@@ -357,6 +357,10 @@ class HipBackend:
     def hal_fma_i64(self, dst, lhs, rhs):
         check(lib().csd_hal_fma_i64(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()), ctypes.byref(rhs.view())))
 
+    def sum_i64(self, dst, lhs, rhs):
+        """csd_hal_sum_i64: dst = lhs + rhs (wrapping), hal_assign and hal_add_i64 in one pass"""
+        check(lib().csd_hal_sum_i64(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()), ctypes.byref(rhs.view())))
+
     def hal_mul_i64(self, dst, lhs, rhs):
         check(lib().csd_hal_mul_i64(self.ctx, ctypes.byref(dst.view()), ctypes.byref(lhs.view()), ctypes.byref(rhs.view())))
 

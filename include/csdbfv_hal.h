@@ -103,6 +103,9 @@ int csd_hal_add_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* src);    
 int csd_hal_sub_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* src);       /* mod.rs:140 */
 int csd_hal_add_i128(csd_ctx* ctx, const csd_view* dst, const csd_view* src);      /* mod.rs:141 */
 int csd_hal_fma_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs); /* mod.rs:144 */
+/* dst = lhs + rhs, wrapping int64 (hal_assign then hal_add_i64, mod.rs:129, 139, in one pass; no
+ * trait method of its own) */
+int csd_hal_sum_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs);
 int csd_hal_mul_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs); /* mod.rs:149 */
 /* hal_inner_product_i64 (mod.rs:151-163): dst = sum_t lhs[t] * rhs[t], overwriting. */
 int csd_hal_inner_product_i64(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const csd_view* rhs, size_t terms);

@@ -73,7 +73,8 @@ int csd_ring_crt_to_pb(csd_ring* ring, const csd_view* data, const size_t* base,
  * slower (no reference counterpart). */
 int csd_ring_set_overlap(csd_ring* ring, int on);
 /* bconv_tensors (ring.rs:1526-1535, BConvCache::bconv 1663-1701). The default form is one kernel
- * that leaves src unchanged (used when src and dst do not overlap); the HAL-sequence form destroys
+ * that leaves src unchanged (used when src and dst do not overlap, or are disjoint limb ranges of
+ * one tensor); the HAL-sequence form destroys
  * src, like the reference, when preserve_src is 0, and works on a copy when it is 1. With
  * preserve_src = 1, src is unchanged whatever form the ring's setting selects. */
 int csd_ring_bconv(csd_ring* ring, const csd_view* dst, const size_t* dst_base, size_t r_dst, const csd_view* src,

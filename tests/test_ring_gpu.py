@@ -184,6 +184,13 @@ def test_bconv_parity(gpu, cpu, rings, bconv_form):
     out = gpu.hal_allocate_i64((2, 2 + len(db), 12, 3072))
     g.bconv_tensors(out[:, 2:], db, bg[:, 1:], sb)
     assert same(from_dev(gpu, out)[:, 2:], dc)
+    # source and destination as disjoint limb ranges of one tensor (the lift's layout)
+    one = np.zeros((2, len(sb) + len(db), 12, 3072), dtype=np.int64)
+    one[:, :len(sb)] = big[:, 1:]
+    og = to_dev(gpu, one)
+    g.bconv_tensors(og[:, len(sb):], db, og[:, :len(sb)], sb, preserve_src=True)
+    got = from_dev(gpu, og)
+    assert same(got[:, len(sb):], dc) and same(got[:, :len(sb)], big[:, 1:])
 
 
 def test_bconv_matrix_core_path_is_taken(gpu, rings):
