@@ -947,6 +947,61 @@ __global__ __launch_bounds__(256) void k_convolve_reduce(CvArgs a, ModTable mt) 
     }
 }
 
+// The same with every operand digit of the position loaded once into registers (d1, d2 <= CV_MAXD):
+// each value is read from memory once instead of once per output digit it enters, and all loads
+// are in flight together. The sums are formed in the same order, so the rows are identical.
+#define CV_MAXD 12
+__global__ __launch_bounds__(256) void k_convolve_reduce_regs(CvArgs a, ModTable mt) {
+    const int64_t chunks = (a.P + 255) / 256;
+    int64_t blk = blockIdx.x;
+    const int64_t c0 = blk % chunks;
+    blk /= chunks;
+    const int64_t jl = blk % a.r, b = blk / a.r;  // limb, pair
+    const int64_t pos = c0 * 256 + threadIdx.x;
+    if (pos >= a.P) return;
+    const int64_t k2 = pos / a.n3, l3 = pos % a.n3;
+    const CsdMod m = mt.m[jl];
+    const int64_t* L = a.lhs + jl * a.sl[1] + k2 * a.sl[2] + l3 * a.sl[3];
+    const int64_t* R = a.rhs + jl * a.sr[1] + k2 * a.sr[2] + l3 * a.sr[3];
+    int64_t* D = a.dst + jl * a.sd[1] + k2 * a.sd[2] + l3 * a.sd[3];
+    int64_t l0[CV_MAXD], l1[CV_MAXD], r0[CV_MAXD], r1[CV_MAXD];
+#pragma unroll
+    for (int j = 0; j < CV_MAXD; j++) {
+        if (j < a.d1) {
+            const int64_t lr = j * a.lrow[0] + b * a.lrow[2];
+            l0[j] = L[lr * a.sl[0]];
+            l1[j] = L[(lr + a.lrow[1]) * a.sl[0]];
+        }
+        if (j < a.d2) {
+            const int64_t rr = j * a.rrow[0] + b * a.rrow[2];
+            r0[j] = R[rr * a.sr[0]];
+            r1[j] = R[(rr + a.rrow[1]) * a.sr[0]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * CV_MAXD - 1; k++) {
+        if (k < a.K) {
+            i128 s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+            for (int j = 0; j < CV_MAXD; j++) {
+                // j from max(0, k - d2 + 1) to min(k, d1 - 1), as the general kernel runs it
+                if (j <= k && k - j < CV_MAXD && j < a.d1 && k - j < a.d2) {
+                    s0 = gb_mac(s0, l0[j], r0[k - j]);
+                    s1 = gb_mac(gb_mac(s1, l0[j], r1[k - j]), l1[j], r0[k - j]);
+                    s2 = gb_mac(s2, l1[j], r1[k - j]);
+                }
+            }
+            const i128 sums[3] = {s0, s1, s2};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int row = c * a.K + k;
+                const int64_t v = a.full[row] ? csd_to_mont64(m, csd_double_red128(m, sums[c])) : csd_red128(m, sums[c]);
+                D[((int64_t)row * a.n + b) * a.sd[0]] = v;
+            }
+        }
+    }
+}
+
 extern "C" int csd_convolve_reduce(csd_ctx* ctx, const csd_view* dst, const csd_view* lhs, const int64_t* lhs_rows,
                                    const csd_view* rhs, const int64_t* rhs_rows, size_t count, size_t d1, size_t d2,
                                    size_t K, const uint8_t* full, const csd_modulus* mods, size_t nmods) {
@@ -989,7 +1044,10 @@ extern "C" int csd_convolve_reduce(csd_ctx* ctx, const csd_view* dst, const csd_
     const int64_t blocks = a.n * a.r * ((a.P + 255) / 256);
     if (blocks > (int64_t)INT32_MAX) throw CsdError("convolve_reduce: grid too large");
     TimedScope ts(ctx, "convolve_reduce", 8.0 * (double)a.n * a.r * a.P * (2.0 * (d1 + d2) + 3.0 * K));
-    hipLaunchKernelGGL(k_convolve_reduce, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
+    if (d1 <= CV_MAXD && d2 <= CV_MAXD)
+        hipLaunchKernelGGL(k_convolve_reduce_regs, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
+    else
+        hipLaunchKernelGGL(k_convolve_reduce, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
     HIPCHECK(hipGetLastError());
     CSD_CATCH
 }
