@@ -244,24 +244,32 @@ __global__ __launch_bounds__(256) void k_dbfv_reduce_chain4(RcArgs a, ModTable m
     const int64_t j2 = j / a.n3, j3 = j - j2 * a.n3;
     const u128 half = (u128)1 << (a.st.shift - 1);
     int64_t cl0 = 0, cl1 = 0;
-    for (int i = 0; i < a.digits; i++) {
-        const int64_t row = (int64_t)i * a.per + c;
-        int64_t* X = a.x + row * a.sx[0] + j2 * a.sx[2] + j3;
-        const bool zero_in = i >= a.d;
-        const bool has_carry = i > 0;
-        const bool full_carry = (a.full_carry >> i) & 1, full_split = (a.full_split >> i) & 1;
-        int64_t xv[RC_MAXP];
-        if (!zero_in) {
-            const int64_t* XI = a.src + row * a.si[0] + j2 * a.si[2] + j3;
+    // digit i's residues of this quarter (zero beyond the input's digits); digit i + 1's are
+    // loaded while digit i is reduced (in place, src rows of a digit are only written by its own step)
+    auto load_digit = [&](int i, int64_t (&v)[RC_MAXP]) {
+        if (i < a.d) {
+            const int64_t* XI = a.src + ((int64_t)i * a.per + c) * a.si[0] + j2 * a.si[2] + j3;
 #pragma unroll
             for (int k = 0; k < RC_MAXP; k++) {
                 const int p = qd + RC_Q * k;
-                if (p < r) xv[k] = XI[p * a.si[1]];
+                v[k] = p < r ? XI[p * a.si[1]] : 0;
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < RC_MAXP; k++) xv[k] = 0;
+            for (int k = 0; k < RC_MAXP; k++) v[k] = 0;
         }
+    };
+    int64_t xn[RC_MAXP];
+    load_digit(0, xn);
+    for (int i = 0; i < a.digits; i++) {
+        const int64_t row = (int64_t)i * a.per + c;
+        int64_t* X = a.x + row * a.sx[0] + j2 * a.sx[2] + j3;
+        const bool has_carry = i > 0;
+        const bool full_carry = (a.full_carry >> i) & 1, full_split = (a.full_split >> i) & 1;
+        int64_t xv[RC_MAXP];
+#pragma unroll
+        for (int k = 0; k < RC_MAXP; k++) xv[k] = xn[k];
+        if (i + 1 < a.digits) load_digit(i + 1, xn);
         // pass 1: the carry step (dbfv.rs:995-1003), then the lifts and the matmul rows
         // (dbfv.rs:909-917), over this quarter's primes
         u128 s0 = 0, s1 = 0, sg = 0;
