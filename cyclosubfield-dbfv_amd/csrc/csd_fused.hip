@@ -951,6 +951,7 @@ __global__ __launch_bounds__(256) void k_convolve_reduce(CvArgs a, ModTable mt) 
 // each value is read from memory once instead of once per output digit it enters, and all loads
 // are in flight together. The sums are formed in the same order, so the rows are identical.
 #define CV_MAXD 12
+#define CV_MIN_REGS 6
 __global__ __launch_bounds__(256) void k_convolve_reduce_regs(CvArgs a, ModTable mt) {
     const int64_t chunks = (a.P + 255) / 256;
     int64_t blk = blockIdx.x;
@@ -1044,7 +1045,11 @@ extern "C" int csd_convolve_reduce(csd_ctx* ctx, const csd_view* dst, const csd_
     const int64_t blocks = a.n * a.r * ((a.P + 255) / 256);
     if (blocks > (int64_t)INT32_MAX) throw CsdError("convolve_reduce: grid too large");
     TimedScope ts(ctx, "convolve_reduce", 8.0 * (double)a.n * a.r * a.P * (2.0 * (d1 + d2) + 3.0 * K));
-    if (d1 <= CV_MAXD && d2 <= CV_MAXD)
+    // the register form pays where operands enter many output digits: measured faster with 8-10
+    // digits a side (a bootstrap's products: 22.4 -> 17.4 ms), slower with 4 (16 pairs: 1.26 ->
+    // 1.39 ms) and 1 (the tensor product of 64 pairs: 0.99 -> 1.86 ms), its 147 VGPRs halving the
+    // waves per SIMD
+    if (d1 >= CV_MIN_REGS && d2 >= CV_MIN_REGS && d1 <= CV_MAXD && d2 <= CV_MAXD)
         hipLaunchKernelGGL(k_convolve_reduce_regs, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);
     else
         hipLaunchKernelGGL(k_convolve_reduce, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a, t);

@@ -240,11 +240,11 @@ def test_fused_split_out_of_place(gpu, rings):
         pkg.dbfv._split_fused(g, data, out, 1, base, x_in=to_dev(gpu, inputs(g, base, 3, 42, 1)))
 
 
-# the last field_192 case has 13 digits on one side: the convolution kernel without the register
-# cache of the operands (which takes up to 12 digits a side)
+# the convolution kernel caches the operand digits in registers with 6 to 12 digits a side (the
+# 8 x 7 case); the others, among them 13 digits on one side, take the general kernel
 HOM_MUL_CASES = [("field_192", (200, 220), 16, 64, 4, 4, 2, "pb"), ("field_192", (200, 220), 16, 64, 2, 3, 4, "crt"),
                  ("field_192", (260, 280), 27, 108, 4, 1, 3, "pb"), ("field_36864", (960, 980), 27, 108, 4, 4, 9, "pb"),
-                 ("field_192", (260, 280), 8, 112, 13, 2, 2, "pb")]
+                 ("field_192", (260, 280), 8, 112, 13, 2, 2, "pb"), ("field_192", (260, 280), 8, 64, 8, 7, 2, "pb")]
 
 
 @pytest.mark.parametrize("field,log2_q,log2_b,log2_t,d1,d2,ndigits,basis", HOM_MUL_CASES)
